@@ -1,0 +1,192 @@
+/*
+ * lbib.h -- C ABI of the MI355X (gfx950) per-timestep lattice path of an
+ * LBIBCell-compatible tissue simulator.
+ *
+ * The library owns all device memory of a context (one HIP stream per
+ * context).  Every entry point takes and returns plain host pointers and
+ * sizes, never throws, and reports failure as a negative status with a message
+ * in lbib_last_error().  Population and species buffers exchanged with the
+ * host are in natural direction order, direction-major: f[q * n + y * nx + x],
+ * q = T,E,N,W,S,NE,NW,SW,SE (LbmLib/include/Direction.hpp:34-44), holding
+ * post-collision values (the reference keeps the same values in the "locally
+ * swapped" slot order, FluidSolver.cpp:259-264).
+ *
+ * Citations are relative to the reference repository root; "LbmLib/" means
+ * libs/LbmLib/.
+ */
+#ifndef LBIB_H
+#define LBIB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBIB_MAX_SPECIES 4
+
+typedef struct lbib_ctx lbib_ctx;
+
+/* Storage precision of populations. */
+enum lbib_precision {
+    LBIB_FP64 = 0,        /* FP64 storage and arithmetic (parity path) */
+    LBIB_FP32 = 1,        /* FP32 storage and arithmetic */
+    LBIB_FP16_DEV = 2,    /* FP16 storage of f_i - w_i*rho_ref, FP32 arithmetic */
+    LBIB_BF16_DEV = 3     /* BF16 storage of f_i - w_i*rho_ref, FP32 arithmetic */
+};
+
+/* Fluid force model: Luo 1993 is what the reference compiles
+ * (LbmLib/include/solver/FluidSolver/FluidSolver.hpp:41). */
+enum lbib_force_model {
+    LBIB_FORCE_LUO = 0,
+    LBIB_FORCE_GUO_AS_WRITTEN = 1,  /* GuoZhengShi2002ForceModel.cpp:29-78, incl. its NE y-term */
+    LBIB_FORCE_GUO = 2              /* Guo et al. 2002 with (9ux + 6uy + 3) in the NE y-term */
+};
+
+/* Reaction specs served by the species registry (CDESolverFactory names). */
+enum lbib_species_kind {
+    LBIB_SPECIES_PLAIN = 0,   /* CDESolverD2Q5 (CDESolverD2Q5.cpp:37-228) */
+    LBIB_SPECIES_SIGNAL = 1,  /* tutorial_01_CDESolverD2Q5_SIGNAL (..._SIGNAL.cpp:39-250) */
+    LBIB_SPECIES_LINEAR = 2   /* R_s = sum_j K[s][j] C_j + source inside cells */
+};
+
+typedef struct lbib_species_spec {
+    int32_t kind;             /* lbib_species_kind */
+    double tau;
+    /* SIGNAL: production rate in domain prod_domain (-1: any cell) while
+     * it < switchoff, otherwise decay*C sink; init_inside = C inside cells. */
+    double production, decay, init_inside;
+    uint32_t switchoff;
+    int32_t prod_domain;
+    /* LINEAR: source inside cells and coupling row K[s][0..nspecies-1]. */
+    double source_inside;
+    double krow[LBIB_MAX_SPECIES];
+} lbib_species_spec;
+
+typedef struct lbib_config {
+    int32_t nx, ny;           /* Parameters SizeX, SizeY */
+    double tau_fluid;         /* Parameters tauFluid */
+    int32_t force_model;      /* lbib_force_model */
+    int32_t precision;        /* lbib_precision */
+    int32_t nspecies;         /* 0..LBIB_MAX_SPECIES */
+    lbib_species_spec species[LBIB_MAX_SPECIES];
+    int32_t box_outlet;       /* MassSolverBoxOutlet registered */
+    int32_t max_markers;      /* capacity hints (0 = defaults) */
+    int32_t max_connections;
+} lbib_config;
+
+/* Status word bits returned by lbib_read_status(). */
+#define LBIB_ERR_VELOCITY      0x00000001u  /* LagrangianPoint.cpp:96-102 */
+#define LBIB_ERR_EDGE          0x00000002u  /* GeometryHandler.cpp:618-636 */
+#define LBIB_ERR_SCANLINE      0x00000004u  /* GeometryHandler.cpp:74-76 */
+#define LBIB_ERR_SLOPE         0x00000008u  /* Connection.cpp:315-323 */
+#define LBIB_ERR_CAPACITY      0x00000010u  /* a device work list overflowed */
+#define LBIB_ERR_MASK          0x000000ffu
+#define LBIB_HALT_PERTURB_NODE 0x00000100u  /* a moved marker needs GeometryNode.cpp:153-214 */
+#define LBIB_HALT_PERTURB_CONN 0x00000200u  /* a connection needs Connection.cpp:332-387 */
+#define LBIB_HALT_MASK         0x00000f00u
+#define LBIB_FLAG_REMESH       0x00001000u  /* a segment is longer than MAXLENGTH */
+#define LBIB_FLAG_COARSEN      0x00002000u  /* a marker's neighbours are closer than MAXLENGTH */
+
+typedef struct lbib_status {
+    uint32_t bits;
+    int32_t first_bad_marker;   /* dense marker index of the first error, or -1 */
+    int32_t n_touched_sites;    /* sites that received IB force this step */
+    int32_t n_boundary_nodes;   /* cut links kept after closest-wins */
+    int32_t n_reinit;           /* sites whose domain changed (CDE reinit) */
+    int32_t n_candidates;       /* cut-link candidates generated */
+} lbib_status;
+
+/* ---- context ---------------------------------------------------------- */
+int lbib_create(const lbib_config* cfg, lbib_ctx** out);
+/* Replaces: SimulationRunner ctor (SimulationRunner.cpp:50-64) + the per-site
+ * solver objects of GeometryHandler::generatePhysicalGrid (GeometryHandler.cpp:463-474). */
+void lbib_destroy(lbib_ctx* ctx);
+const char* lbib_last_error(const lbib_ctx* ctx);
+int lbib_synchronize(lbib_ctx* ctx);
+/* Device bytes held by the context. */
+size_t lbib_device_bytes(const lbib_ctx* ctx);
+
+/* ---- lattice state ----------------------------------------------------- */
+/* FluidSolver::initSolver / initWithVelocity (FluidSolver.cpp:50-101) and the
+ * species' initSolver (CDESolverD2Q5.cpp:37-40, ..._SIGNAL.cpp:46-57) on every
+ * site; u0 (2*n, interleaved ux,uy) may be NULL.  Species init reads the
+ * current domain identifiers. */
+int lbib_init_solvers(lbib_ctx* ctx, const double* u0);
+/* Whole-field copies (natural order, see top).  FP16/BF16 contexts convert. */
+int lbib_set_populations(lbib_ctx* ctx, const double* f);
+int lbib_get_populations(lbib_ctx* ctx, double* f);
+int lbib_set_species(lbib_ctx* ctx, int s, const double* g);
+int lbib_get_species(lbib_ctx* ctx, int s, double* g);
+/* FluidSolver::getVelocity for every site (FluidSolver.cpp:242-246). */
+int lbib_get_velocity(lbib_ctx* ctx, double* ux, double* uy);
+int lbib_get_domains(lbib_ctx* ctx, uint32_t* dom);
+int lbib_set_domains(lbib_ctx* ctx, const uint32_t* dom);
+/* Prescribed body force field (lattice-only use; the IB spread overwrites it). */
+int lbib_set_force_field(lbib_ctx* ctx, const double* fx, const double* fy);
+int lbib_get_force_field(lbib_ctx* ctx, double* fx, double* fy);
+/* FluidSolver::addMass on every site (FluidSolver.cpp:253-257); consumed and
+ * cleared by the next collide. */
+int lbib_set_mass_field(lbib_ctx* ctx, const double* m);
+
+/* ---- membrane (GeometryNode) and connections ------------------------------
+ * Markers are dense indices 0..nm-1 in increasing GeometryNode id order (the
+ * std::map order of Geometry::geometryNodes_).  xy, vxy, oxy are interleaved. */
+int lbib_set_markers(lbib_ctx* ctx, int nm, const double* xy, const double* vxy, const double* oxy);
+int lbib_get_markers(lbib_ctx* ctx, double* xy, double* vxy, double* fxy, double* oxy);
+/* Connections in Geometry::connections_ vector order: first/second marker
+ * index, domain id, boundary-condition descriptor index (-1 = none). */
+int lbib_set_connections(lbib_ctx* ctx, int nc, const int32_t* a, const int32_t* b,
+                         const uint32_t* dom, const int32_t* bc);
+/* Boundary-condition descriptors: descriptor d lists the species indices that
+ * BoundarySolverNoFluxD2Q5 is connected to (BoundaryNode.cpp:43-46). */
+int lbib_set_bc_descriptors(lbib_ctx* ctx, int nd, const int32_t* off, const int32_t* species);
+/* Per-marker ring neighbours (previous marker of the incoming connection and
+ * next marker of the outgoing one) and per-domain rings in the order of
+ * GeometryHandler::computeAreas (GeometryHandler.cpp:101-134). */
+int lbib_set_rings(lbib_ctx* ctx, const int32_t* prev, const int32_t* next,
+                   int ndom_rings, const uint32_t* ring_dom, const int32_t* ring_off,
+                   const int32_t* ring_marker);
+/* Cell type per domain id (GeometryHandler::cellTypeTrackerMap_). */
+int lbib_set_celltypes(lbib_ctx* ctx, int ndom, const uint32_t* celltype);
+/* Force structs of ForceSolver::forcemap_ (ForceSolver.cpp:42-85) in CSR by
+ * dense marker index, in map/push order.  prm holds 4 doubles per struct:
+ * type 0/7 (k,l0), 5/6 (f), 1 (x,y,k,l0), 2 (fx,fy), 3 (y,k,l0), 4 (x,k,l0). */
+int lbib_set_forces(lbib_ctx* ctx, int nm, const int32_t* off, const int32_t* type,
+                    const int32_t* partner, const double* prm);
+
+/* ---- the device-resident step ----------------------------------------------
+ * Stage A (SimulationRunner.cpp:73-79): calculateForce, distributeForce,
+ * pre/postAdvect + advect + collide (+ the MassSolverBoxOutlet rescale), and
+ * collectVelocity + the marker move of Geometry::moveGeometryNodes.
+ * it = Parameters.currentIteration at this step. */
+int lbib_step_lattice(lbib_ctx* ctx, uint32_t it);
+/* Stage B (GeometryHandler::moveLattice after the move, :361-382):
+ * connection checks, boundary-node generation with closest-wins, domain
+ * update + CDE reinit, domain scanline; plus the remesh/coarsen checks.
+ * check_coarsen != 0 when it % 10 == 0. */
+int lbib_step_geometry(lbib_ctx* ctx, uint32_t it, int check_coarsen);
+/* Lattice-only step with the current force field (no IB, no geometry). */
+int lbib_step_lattice_only(lbib_ctx* ctx, uint32_t it);
+/* Synchronises and returns the status of the last stages; clears HALT bits. */
+int lbib_read_status(lbib_ctx* ctx, lbib_status* st);
+int lbib_clear_status(lbib_ctx* ctx);
+
+/* ---- device BioSolver helpers ------------------------------------------- */
+/* Areas per ring (shoelace, FP64), in ring order of lbib_set_rings. */
+int lbib_compute_areas(lbib_ctx* ctx, double* area);
+/* tutorial_01_BioSolverAreaRegulator::applyBioProcess
+ * (tutorial_01_BioSolverAreaRegulator.cpp:43-100) as one device op:
+ * areas -> clamped proportional source per domain -> addMass on every site of
+ * a cell (MAXSOURCE for cell type 1).  target_factor > 0 sets the target area
+ * of a new domain to factor * its first area instead of target_area. */
+int lbib_area_regulator(lbib_ctx* ctx, double target_area, double kp, double maxsource,
+                        double target_factor);
+/* FluidSolver::addMass for every site of each domain: m_site = table[dom]. */
+int lbib_add_mass_by_domain(lbib_ctx* ctx, int ndom, const double* table);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBIB_H */

@@ -1,0 +1,190 @@
+// lbib_device_math.h -- per-site arithmetic shared by the FP64 kernels.
+//
+// Every expression keeps the operand order of the reference so that, compiled
+// with -ffp-contract=off, the FP64 kernels round exactly like the reference's
+// CPU code built without FMA contraction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "lbib_internal.h"
+
+namespace lbib {
+
+__device__ constexpr double kW0 = 4.0 / 9.0;    // FluidSolver.cpp:39-47
+__device__ constexpr double kW1 = 1.0 / 9.0;
+__device__ constexpr double kW2 = 1.0 / 36.0;
+__device__ constexpr double kHalfPi = M_PI / 2.0;     // GeometryNode.cpp:47-48
+__device__ constexpr double kDoublePi = 2.0 * M_PI;
+__device__ constexpr double kEPS = 10E-10;            // Constants.hpp:34
+__device__ constexpr double kMAXLENGTH = 0.5;         // Constants.hpp:49
+
+// GeometryNode.cpp:55-81 (parabolic sine approximation, not cos)
+__device__ __forceinline__ double fast_cos(const double x) {
+    double c;
+    double t = x + kHalfPi;
+    if (t > M_PI) t -= kDoublePi;
+    if (t < .00) {
+        c = 1.27323954 * t + 0.405284735 * t * t;
+    } else {
+        c = 1.27323954 * t - 0.405284735 * t * t;
+    }
+    if (c < 0.0) c = 0.225 * (c * -c - c) + c;
+    else         c = 0.225 * (c * c - c) + c;
+    return c;
+}
+// GeometryNode.cpp:89-94; dx = x_site - X_marker
+__device__ __forceinline__ double ib_delta(const double dx, const double dy) {
+    return ((1.0 + fast_cos(kHalfPi * dx)) * (1.0 + fast_cos(kHalfPi * dy))) / 16.0;
+}
+
+// FluidSolver::collide (FluidSolver.cpp:103-183) on natural-order pulled
+// populations d[9]; Luo1993ForceModel (Luo1993ForceModel.cpp:29-70) or
+// GuoZhengShi2002ForceModel (GuoZhengShi2002ForceModel.cpp:29-78).
+// gx,gy: stored velocity of the previous collide (Guo only).  Writes the
+// post-collision velocity to ux,uy.
+template <int FM>
+__device__ __forceinline__ void fluid_collide(double d[9], const double m, const double Fx,
+                                              const double Fy, const double tauI, const double tau,
+                                              const double gx, const double gy, double& uxo,
+                                              double& uyo) {
+    double rho0 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) rho0 += d[q];
+    double rho = rho0;
+    if (isfinite(m)) rho += m;
+    double ux, uy;
+    {
+        const double rhoI = 1.0 / rho;
+        if (FM == 0) {
+            ux = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW])) * rhoI;
+            uy = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE])) * rhoI;
+        } else {
+            ux = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW]) + 0.5 * Fx) * rhoI;
+            uy = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE]) + 0.5 * Fy) * rhoI;
+        }
+    }
+    const double ux2 = ux * ux, uy2 = uy * uy;
+    const double lastTerm = 3.0 / 2.0 * (ux2 + uy2);
+    const double W3 = 9.0 / 2.0;
+    const double W1rho = kW1 * rho, W2rho = kW2 * rho;
+    double fc[9];
+    if (FM == 0) {
+        const double W1rho3 = rho0 * 3.0 / 9.0, W2rho3 = rho0 * 3.0 / 36.0;
+        fc[DT] = 0.0;
+        fc[DE] = W1rho3 * Fx;
+        fc[DN] = W1rho3 * Fy;
+        fc[DW] = -W1rho3 * Fx;
+        fc[DS] = -W1rho3 * Fy;
+        fc[DNE] = W2rho3 * (Fx + Fy);
+        fc[DNW] = W2rho3 * (-Fx + Fy);
+        fc[DSW] = W2rho3 * (-Fx - Fy);
+        fc[DSE] = W2rho3 * (Fx - Fy);
+    } else {
+        const double G0 = (1.0 - 0.5 / tau) * 4.0 / 9.0;
+        const double G1 = (1.0 - 0.5 / tau) / 9.0;
+        const double G2 = (1.0 - 0.5 / tau) / 36.0;
+        fc[DT] = G0 * (-3.0 * gx * Fx - 3.0 * gy * Fy);
+        fc[DE] = G1 * ((6.0 * gx + 3.0) * Fx - 3.0 * gy * Fy);
+        fc[DN] = G1 * (-3.0 * gx * Fx + (6.0 * gy + 3.0) * Fy);
+        fc[DW] = G1 * ((6.0 * gx - 3.0) * Fx - 3.0 * gy * Fy);
+        fc[DS] = G1 * (-3.0 * gx * Fx + (6.0 * gy - 3.0) * Fy);
+        if (FM == 1)
+            fc[DNE] = G2 * ((6.0 * gx + 9.0 * gy + 3.0) * Fx + (9.0 * gx - 3.0 * gy + 3.0) * Fy);
+        else
+            fc[DNE] = G2 * ((6.0 * gx + 9.0 * gy + 3.0) * Fx + (9.0 * gx + 6.0 * gy + 3.0) * Fy);
+        fc[DNW] = G2 * ((6.0 * gx - 9.0 * gy - 3.0) * Fx + (-9.0 * gx + 6.0 * gy + 3.0) * Fy);
+        fc[DSW] = G2 * ((6.0 * gx + 9.0 * gy - 3.0) * Fx + (9.0 * gx + 6.0 * gy - 3.0) * Fy);
+        fc[DSE] = G2 * ((6.0 * gx - 9.0 * gy + 3.0) * Fx + (-9.0 * gx + 6.0 * gy - 3.0) * Fy);
+    }
+    double tmp[9];
+    tmp[DT] = kW0 * rho * (1.0 - lastTerm) * tauI + fc[DT];
+    tmp[DE] = W1rho * (1.0 + 3.0 * ux + W3 * ux2 - lastTerm) * tauI + fc[DE];
+    tmp[DNE] = W2rho * (1.0 + 3.0 * (ux + uy) + W3 * (ux + uy) * (ux + uy) - lastTerm) * tauI + fc[DNE];
+    tmp[DN] = W1rho * (1.0 + 3.0 * uy + W3 * uy2 - lastTerm) * tauI + fc[DN];
+    tmp[DNW] = W2rho * (1.0 + 3.0 * (-ux + uy) + W3 * (-ux + uy) * (-ux + uy) - lastTerm) * tauI + fc[DNW];
+    tmp[DW] = W1rho * (1.0 + 3.0 * (-ux) + W3 * ux2 - lastTerm) * tauI + fc[DW];
+    tmp[DSW] = W2rho * (1.0 + 3.0 * (-ux - uy) + W3 * (-ux - uy) * (-ux - uy) - lastTerm) * tauI + fc[DSW];
+    tmp[DS] = W1rho * (1.0 + 3.0 * (-uy) + W3 * uy2 - lastTerm) * tauI + fc[DS];
+    tmp[DSE] = W2rho * (1.0 + 3.0 * (ux - uy) + W3 * (ux - uy) * (ux - uy) - lastTerm) * tauI + fc[DSE];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const double td = d[q];
+        d[q] = td - td * tauI + tmp[q];
+    }
+    const double rhoI = 1.0 / rho;
+    if (FM == 0) {
+        uxo = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW])) * rhoI;
+        uyo = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE])) * rhoI;
+    } else {
+        uxo = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW]) + 0.5 * Fx) * rhoI;
+        uyo = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE]) + 0.5 * Fy) * rhoI;
+    }
+}
+
+// Reaction term (…_SIGNAL.cpp:92-100 for SIGNAL; the linear spec is
+// evaluated on the pre-collision concentrations cj[] of every species).
+template <int NS>
+__device__ __forceinline__ double species_reaction(const SpeciesDev& p, const uint32_t dom,
+                                                   const uint32_t it, const double C,
+                                                   const double* cj) {
+    if (p.kind == LBIB_SPECIES_SIGNAL) {
+        const bool prod = (p.prod_domain < 0) ? (dom != 0u) : (dom == (uint32_t)p.prod_domain);
+        if (prod && (it < p.switchoff)) return p.production;
+        return -p.decay * C;
+    }
+    if (p.kind == LBIB_SPECIES_LINEAR) {
+        double r = 0.0;
+#pragma unroll
+        for (int j = 0; j < NS; j++) r += p.krow[j] * cj[j];
+        if (dom != 0u) r += p.source_inside;
+        return r;
+    }
+    return 0.0;
+}
+
+// CDESolverD2Q5::collide (CDESolverD2Q5.cpp:75-107) / SIGNAL (…_SIGNAL.cpp:102-135)
+// on natural-order d[5]; C must be d[0]+…+d[4] in index order.
+template <int NS>
+__device__ __forceinline__ void species_collide(double d[5], const double C, const double ux,
+                                                const double uy, const SpeciesDev& p,
+                                                const uint32_t dom, const uint32_t it,
+                                                const double* cj) {
+    const double w0 = C / 3.0, w1 = C / 6.0;
+    const double tauI = p.tauI;
+    double tmp[5];
+    tmp[DT] = w0;
+    tmp[DE] = w1 * (1.0 + ux * 3.0);
+    tmp[DN] = w1 * (1.0 + uy * 3.0);
+    tmp[DW] = w1 * (1.0 + (-ux) * 3.0);
+    tmp[DS] = w1 * (1.0 + (-uy) * 3.0);
+    if (p.kind == LBIB_SPECIES_PLAIN) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            const double td = d[q];
+            d[q] = td - td * tauI + tmp[q] * tauI;
+        }
+    } else {
+        const double R = species_reaction<NS>(p, dom, it, C, cj);
+        const double R0 = 1.0 * R / 3.0, R1 = 1.0 * R / 6.0;
+        d[DT] = d[DT] - d[DT] * tauI + tmp[DT] * tauI + R0;
+#pragma unroll
+        for (int q = 1; q < 5; q++) d[q] = d[q] - d[q] * tauI + tmp[q] * tauI + R1;
+    }
+}
+
+// Sum of the 9 stored values in the reference's swapped slot order
+// (FluidSolver::getRho on the locally swapped state): T,W,S,E,N,SW,SE,NE,NW.
+__device__ __forceinline__ double rho_swapped(const double d[9]) {
+    double r = 0.0;
+    r += d[DT]; r += d[DW]; r += d[DS]; r += d[DE]; r += d[DN];
+    r += d[DSW]; r += d[DSE]; r += d[DNE]; r += d[DNW];
+    return r;
+}
+// CDESolverD2Q5::getC on the swapped state: T,W,S,E,N.
+__device__ __forceinline__ double c_swapped(const double d0, const double dE, const double dN,
+                                            const double dW, const double dS) {
+    double r = 0.0;
+    r += d0; r += dW; r += dS; r += dE; r += dN;
+    return r;
+}
+
+}  // namespace lbib

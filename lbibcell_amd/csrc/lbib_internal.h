@@ -1,0 +1,158 @@
+// lbib_internal.h -- device-side data layout and shared helpers of liblbib_hip.
+//
+// HBM layout of one context (n = nx*ny sites, row-major, x fastest):
+//   f[2][9][n]        populations, natural direction order, post-collision
+//                     (A/B ping-pong; pull streaming reads cur, writes 1-cur)
+//   g[2][ns][5][n]    species populations, same convention
+//   ux[n], uy[n]      FluidSolver::velocity_ (post-collision)
+//   Fx[n], Fy[n]      FluidSolver::force_ (IB spread target, zero off-membrane)
+//   tag[n]            bits 0-23 domain id, 24-27 boundary node present in slot
+//                     E,N,W,S, 28-31 that boundary node carries a no-flux BC
+//   ftile[ny][ntx]    1 if any site of the 256-wide row tile holds a force
+// Mixed-precision contexts replace f/g with 16-bit deviations from the rest
+// state (see lbib_lattice.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include "../../include/lbib.h"
+
+namespace lbib {
+
+enum { DT = 0, DE = 1, DN = 2, DW = 3, DS = 4, DNE = 5, DNW = 6, DSW = 7, DSE = 8 };
+
+constexpr uint32_t DOM_MASK = 0x00ffffffu;
+constexpr int BN_SHIFT = 24;        // bit 24 + (slot-1), slot in E,N,W,S
+constexpr int BC_SHIFT = 28;
+constexpr int TILE_W = 256;         // lattice kernel block width = force tile width
+constexpr int MAX_BC_SPECIES = LBIB_MAX_SPECIES;
+constexpr int REINIT_CAP = 8192;    // sites per step whose domain may change
+constexpr int SEQ_PER_CONN = 256;   // candidates per connection (segment < 127 lu)
+
+struct SpeciesDev {
+    int kind;
+    double tau, tauI;
+    double production, decay, init_inside;
+    uint32_t switchoff;
+    int prod_domain;
+    double source_inside;
+    double krow[LBIB_MAX_SPECIES];
+};
+
+struct DevStatus {
+    uint32_t bits;
+    int32_t first_bad;
+    uint32_t ntouched;      // spread: sites touched this step
+    uint32_t pool_used;     // spread: pool entries allocated
+    uint32_t ncand;         // cut-link candidates of the current step
+    uint32_t ncand_prev;    // ... of the previous step (reset list)
+    uint32_t nbn;           // winners
+    uint32_t nchanged;      // sites whose local domain changed
+};
+
+struct SpreadEntry {
+    int32_t marker;
+    int32_t pad;
+    double cx, cy;
+};
+
+struct Cand {
+    int32_t site;
+    int32_t slot;       // boundary slot of the site: E,N,W,S (1..4)
+    uint32_t seq;       // global insertion order: conn * SEQ_PER_CONN + k
+    uint32_t domain;
+    double dist2;
+    double bx, by;      // boundary-node position
+    int32_t conn;
+    int32_t win;
+};
+
+}  // namespace lbib
+
+struct lbib_ctx {
+    lbib_config cfg;
+    int nx = 0, ny = 0;
+    long n = 0;
+    int ns = 0;
+    int prec = LBIB_FP64;
+    int ntx = 0;                      // force tiles per row
+    hipStream_t stream = nullptr;
+    std::string err;
+    size_t bytes = 0;
+    // lattice
+    void* f[2] = {nullptr, nullptr};
+    void* g[2] = {nullptr, nullptr};
+    int cur = 0;
+    double *ux = nullptr, *uy = nullptr, *Fx = nullptr, *Fy = nullptr, *msite = nullptr;
+    bool msite_live = false;
+    uint32_t* tag = nullptr;
+    uint8_t* ftile = nullptr;
+    lbib::SpeciesDev sp[LBIB_MAX_SPECIES];
+    // spread
+    uint32_t *scnt = nullptr, *sbase = nullptr, *touched = nullptr, *srank = nullptr;
+    lbib::SpreadEntry* pool = nullptr;
+    // markers
+    int mcap = 0, nm = 0;
+    double *mx = nullptr, *my = nullptr, *mvx = nullptr, *mvy = nullptr;
+    double *mfx = nullptr, *mfy = nullptr, *mox = nullptr, *moy = nullptr;
+    int32_t *mprev = nullptr, *mnext = nullptr;
+    // forces (CSR by marker)
+    int fcap = 0;
+    int32_t *fs_off = nullptr, *fs_type = nullptr, *fs_partner = nullptr;
+    double* fs_prm = nullptr;
+    // connections
+    int ccap = 0, nc = 0;
+    int32_t *ca = nullptr, *cb = nullptr, *cbc = nullptr;
+    uint32_t* cdom = nullptr;
+    int bcd_cap = 0;
+    int32_t *bcd_off = nullptr, *bcd_sp = nullptr;
+    int nbcd = 0;
+    bool any_bc = false;
+    // cut links
+    int candcap = 0;
+    lbib::Cand* cand = nullptr;
+    unsigned long long* best_dist = nullptr;   // [4][n]
+    uint32_t* best_seq = nullptr;              // [4][n]
+    uint32_t* bn_dom = nullptr;                // [4][n]
+    double* bc_v = nullptr;                    // [4][n][2]
+    int32_t* bc_desc = nullptr;                // [4][n]
+    uint32_t* changed = nullptr;               // [REINIT_CAP][2]
+    // rings / domains
+    int rcap = 0, nring = 0, rmcap = 0;
+    uint32_t* ring_dom = nullptr;
+    int32_t *ring_off = nullptr, *ring_marker = nullptr;
+    double* area = nullptr;
+    int dcap = 0, ndom = 0;
+    uint32_t* celltype = nullptr;
+    double* target = nullptr;
+    double* mtab = nullptr;
+    bool mtab_live = false;
+    // status
+    lbib::DevStatus* st = nullptr;
+    lbib::DevStatus* st_host = nullptr;   // pinned
+};
+
+namespace lbib {
+// launchers implemented per translation unit
+int launch_init(lbib_ctx* c, const double* u0_dev);
+int launch_stream_collide(lbib_ctx* c, uint32_t it, bool use_ftile);
+int launch_force_and_spread(lbib_ctx* c);
+int launch_clear_force(lbib_ctx* c);
+int launch_interp_move(lbib_ctx* c);
+int launch_geometry(lbib_ctx* c, uint32_t it, int check_coarsen);
+int launch_areas(lbib_ctx* c);
+int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor);
+int launch_pack_f(lbib_ctx* c, const double* src_dev, int which);     // host layout -> storage
+int launch_unpack_f(lbib_ctx* c, double* dst_dev, int which);         // storage -> host layout
+int launch_pack_g(lbib_ctx* c, int s, const double* src_dev);
+int launch_unpack_g(lbib_ctx* c, int s, double* dst_dev);
+}  // namespace lbib
+
+#define LBIB_HIP(call)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            c->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
+            return -1;                                                         \
+        }                                                                      \
+    } while (0)
