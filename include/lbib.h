@@ -167,6 +167,12 @@ int lbib_step_lattice(lbib_ctx* ctx, uint32_t it);
  * update + CDE reinit, domain scanline; plus the remesh/coarsen checks.
  * check_coarsen != 0 when it % 10 == 0. */
 int lbib_step_geometry(lbib_ctx* ctx, uint32_t it, int check_coarsen);
+/* GeometryHandler::cureLattice (GeometryHandler.cpp:149-170) after the host
+ * has perturbed connections: boundary nodes + scanline, no CDE reinit. */
+int lbib_cure_lattice(lbib_ctx* ctx);
+/* Registers or removes MassSolverBoxOutlet (SimulationRunner::addMassSolver,
+ * SimulationRunner.cpp:401-403). */
+int lbib_set_box_outlet(lbib_ctx* ctx, int on);
 /* Lattice-only step with the current force field (no IB, no geometry). */
 int lbib_step_lattice_only(lbib_ctx* ctx, uint32_t it);
 /* Synchronises and returns the status of the last stages; clears HALT bits. */

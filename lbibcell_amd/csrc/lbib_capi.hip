@@ -1,0 +1,558 @@
+// lbib_capi.hip -- context management and the C ABI declared in include/lbib.h.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+#include "lbib_internal.h"
+
+using namespace lbib;
+
+namespace {
+
+template <typename T>
+int dev_alloc(lbib_ctx* c, T** p, size_t count, int fill = 0) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        c->err = std::string("hipMalloc: ") + hipGetErrorString(e);
+        return -1;
+    }
+    c->bytes += count * sizeof(T);
+    e = hipMemsetAsync(*p, fill, count * sizeof(T), c->stream);
+    if (e != hipSuccess) {
+        c->err = std::string("hipMemsetAsync: ") + hipGetErrorString(e);
+        return -1;
+    }
+    return 0;
+}
+template <typename T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+size_t storage_bytes(int prec) {
+    switch (prec) {
+        case LBIB_FP64: return 8;
+        case LBIB_FP32: return 4;
+        default: return 2;
+    }
+}
+
+int ensure_markers(lbib_ctx* c, int nm) {
+    if (nm <= c->mcap) return 0;
+    int cap = std::max(nm, c->mcap * 2);
+    cap = std::max(cap, 1024);
+    double** arrs[8] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy};
+    for (auto a : arrs) dev_free(*a);
+    dev_free(c->mprev); dev_free(c->mnext); dev_free(c->srank); dev_free(c->touched); dev_free(c->pool);
+    dev_free(c->fs_off);
+    for (auto a : arrs) if (dev_alloc(c, a, cap)) return -1;
+    if (dev_alloc(c, &c->mprev, cap, 0xff) || dev_alloc(c, &c->mnext, cap, 0xff)) return -1;
+    if (dev_alloc(c, &c->srank, (size_t)cap * 16) || dev_alloc(c, &c->touched, (size_t)cap * 16)) return -1;
+    if (dev_alloc(c, &c->pool, (size_t)cap * 16)) return -1;
+    if (dev_alloc(c, &c->fs_off, (size_t)cap + 1)) return -1;
+    c->mcap = cap;
+    return 0;
+}
+
+int ensure_connections(lbib_ctx* c, int nc) {
+    if (nc <= c->ccap && c->cand) return 0;
+    int cap = std::max(nc, c->ccap * 2);
+    cap = std::max(cap, 1024);
+    dev_free(c->ca); dev_free(c->cb); dev_free(c->cbc); dev_free(c->cdom); dev_free(c->cand);
+    if (dev_alloc(c, &c->ca, cap) || dev_alloc(c, &c->cb, cap) || dev_alloc(c, &c->cbc, cap, 0xff) ||
+        dev_alloc(c, &c->cdom, cap))
+        return -1;
+    // a connection of length L crosses at most L+1 lines of each family; keep
+    // room for 8 candidates per connection plus slack for long first segments.
+    c->candcap = cap * 8 + 65536;
+    if (dev_alloc(c, &c->cand, c->candcap)) return -1;
+    c->ccap = cap;
+    // the candidate list of the previous cure is gone: forget it on the device
+    return 0;
+}
+
+int ensure_domains(lbib_ctx* c, int ndom) {
+    if (ndom <= c->dcap) return 0;
+    int cap = std::max(ndom, c->dcap * 2);
+    cap = std::max(cap, 256);
+    std::vector<double> tgt;
+    if (c->target && c->dcap > 0) {
+        tgt.resize(c->dcap);
+        if (hipMemcpy(tgt.data(), c->target, c->dcap * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+            c->err = "copy of target areas failed";
+            return -1;
+        }
+    }
+    dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab);
+    if (dev_alloc(c, &c->celltype, cap) || dev_alloc(c, &c->target, cap, 0xff) || dev_alloc(c, &c->mtab, cap))
+        return -1;
+    if (!tgt.empty() && hipMemcpy(c->target, tgt.data(), tgt.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        c->err = "copy of target areas failed";
+        return -1;
+    }
+    c->dcap = cap;
+    return 0;
+}
+
+int check_ctx(lbib_ctx* c) { return c ? 0 : -1; }
+
+int upload(lbib_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return 0;
+    LBIB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+int download(lbib_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return 0;
+    LBIB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Stage a host FP64 field into device scratch (the force arrays are reused as
+// staging only when no step is in flight; a dedicated buffer keeps it simple).
+struct Staging {
+    double* p = nullptr;
+    ~Staging() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
+    if (!cfg || !out) return -1;
+    *out = nullptr;
+    lbib_ctx* c = new (std::nothrow) lbib_ctx();
+    if (!c) return -1;
+    c->cfg = *cfg;
+    if (cfg->nx < 4 || cfg->ny < 4 || cfg->nspecies < 0 || cfg->nspecies > LBIB_MAX_SPECIES ||
+        !(cfg->tau_fluid > 0.5)) {
+        c->err = "invalid configuration (nx, ny >= 4, 0 <= nspecies <= 4, tau_fluid > 0.5)";
+        *out = c;
+        return -1;
+    }
+    if (cfg->precision != LBIB_FP64 && cfg->precision != LBIB_FP32 && cfg->precision != LBIB_FP16_DEV &&
+        cfg->precision != LBIB_BF16_DEV) {
+        c->err = "unknown precision";
+        *out = c;
+        return -1;
+    }
+    *out = c;
+    c->nx = cfg->nx;
+    c->ny = cfg->ny;
+    c->n = (long)cfg->nx * cfg->ny;
+    c->ns = cfg->nspecies;
+    c->prec = cfg->precision;
+    c->ntx = (c->nx + TILE_W - 1) / TILE_W;
+    for (int s = 0; s < LBIB_MAX_SPECIES; s++) {
+        SpeciesDev& d = c->sp[s];
+        memset(&d, 0, sizeof(d));
+        if (s < c->ns) {
+            const lbib_species_spec& p = cfg->species[s];
+            d.kind = p.kind;
+            d.tau = p.tau;
+            d.tauI = 1.0 / p.tau;
+            d.production = p.production;
+            d.decay = p.decay;
+            d.init_inside = p.init_inside;
+            d.switchoff = p.switchoff;
+            d.prod_domain = p.prod_domain;
+            d.source_inside = p.source_inside;
+            for (int j = 0; j < LBIB_MAX_SPECIES; j++) d.krow[j] = p.krow[j];
+        }
+    }
+    LBIB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    const size_t sb = storage_bytes(c->prec);
+    const long n = c->n;
+    for (int b = 0; b < 2; b++) {
+        if (dev_alloc(c, (uint8_t**)&c->f[b], 9 * n * sb)) return -1;
+        if (dev_alloc(c, (uint8_t**)&c->g[b], (size_t)std::max(c->ns, 1) * 5 * n * sb)) return -1;
+    }
+    if (dev_alloc(c, &c->ux, n) || dev_alloc(c, &c->uy, n) || dev_alloc(c, &c->Fx, n) || dev_alloc(c, &c->Fy, n) ||
+        dev_alloc(c, &c->msite, n))
+        return -1;
+    if (dev_alloc(c, &c->tag, n) || dev_alloc(c, &c->ftile, (size_t)c->ny * c->ntx)) return -1;
+    if (dev_alloc(c, &c->scnt, n) || dev_alloc(c, &c->sbase, n)) return -1;
+    if (dev_alloc(c, &c->best_dist, 4 * n, 0xff) || dev_alloc(c, &c->best_seq, 4 * n, 0xff) ||
+        dev_alloc(c, &c->bn_dom, 4 * n))
+        return -1;
+    if (dev_alloc(c, &c->changed, 2 * REINIT_CAP)) return -1;
+    if (dev_alloc(c, &c->st, 1)) return -1;
+    LBIB_HIP(hipHostMalloc((void**)&c->st_host, sizeof(DevStatus), hipHostMallocDefault));
+    if (ensure_markers(c, cfg->max_markers > 0 ? cfg->max_markers : 1024)) return -1;
+    if (ensure_connections(c, cfg->max_connections > 0 ? cfg->max_connections : 1024)) return -1;
+    if (ensure_domains(c, 256)) return -1;
+    // empty force CSR
+    if (dev_alloc(c, &c->fs_type, 1) || dev_alloc(c, &c->fs_partner, 1) || dev_alloc(c, &c->fs_prm, 4)) return -1;
+    c->fcap = 1;
+    {
+        int32_t one[2] = {0, 0};
+        if (dev_alloc(c, &c->bcd_off, 2) || dev_alloc(c, &c->bcd_sp, 1)) return -1;
+        if (upload(c, c->bcd_off, one, sizeof(one))) return -1;
+    }
+    {
+        DevStatus s0;
+        memset(&s0, 0, sizeof(s0));
+        s0.first_bad = INT_MAX;
+        if (upload(c, c->st, &s0, sizeof(s0))) return -1;
+    }
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void lbib_destroy(lbib_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
+    dev_free(c->ux); dev_free(c->uy); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
+    dev_free(c->tag); dev_free(c->ftile); dev_free(c->scnt); dev_free(c->sbase); dev_free(c->touched);
+    dev_free(c->srank); dev_free(c->pool);
+    dev_free(c->mx); dev_free(c->my); dev_free(c->mvx); dev_free(c->mvy); dev_free(c->mfx); dev_free(c->mfy);
+    dev_free(c->mox); dev_free(c->moy); dev_free(c->mprev); dev_free(c->mnext);
+    dev_free(c->fs_off); dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
+    dev_free(c->ca); dev_free(c->cb); dev_free(c->cbc); dev_free(c->cdom); dev_free(c->bcd_off); dev_free(c->bcd_sp);
+    dev_free(c->cand); dev_free(c->best_dist); dev_free(c->best_seq); dev_free(c->bn_dom); dev_free(c->bc_v);
+    dev_free(c->bc_desc); dev_free(c->changed);
+    dev_free(c->ring_dom); dev_free(c->ring_off); dev_free(c->ring_marker); dev_free(c->area);
+    dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
+    if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* lbib_last_error(const lbib_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int lbib_synchronize(lbib_ctx* c) {
+    if (check_ctx(c)) return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+size_t lbib_device_bytes(const lbib_ctx* c) { return c ? c->bytes : 0; }
+
+// ---------------------------------------------------------------- lattice --
+int lbib_init_solvers(lbib_ctx* c, const double* u0) {
+    if (check_ctx(c)) return -1;
+    Staging st;
+    if (u0) {
+        LBIB_HIP(hipMalloc((void**)&st.p, 2 * c->n * sizeof(double)));
+        if (upload(c, st.p, u0, 2 * c->n * sizeof(double))) return -1;
+    }
+    int r = (c->prec == LBIB_FP64) ? launch_init_f64(c, st.p) : launch_init_mixed(c, st.p);
+    if (r) return r;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static int field_io(lbib_ctx* c, double* host, int field, int nq, bool to_device) {
+    // field: 0 = fluid, 1 + s = species s
+    void* base = field == 0 ? c->f[c->cur] : c->g[c->cur];
+    const size_t off = field == 0 ? 0 : (size_t)(field - 1) * 5 * c->n;
+    const size_t count = (size_t)nq * c->n;
+    if (c->prec == LBIB_FP64) {
+        double* d = (double*)base + off;
+        if (to_device) return upload(c, d, host, count * sizeof(double)) ? -1 : (hipStreamSynchronize(c->stream) == hipSuccess ? 0 : -1);
+        return download(c, host, d, count * sizeof(double));
+    }
+    Staging st;
+    LBIB_HIP(hipMalloc((void**)&st.p, count * sizeof(double)));
+    if (to_device) {
+        if (upload(c, st.p, host, count * sizeof(double))) return -1;
+        if (launch_pack(c, st.p, field, nq)) return -1;
+        LBIB_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    if (launch_unpack(c, st.p, field, nq)) return -1;
+    return download(c, host, st.p, count * sizeof(double));
+}
+
+int lbib_set_populations(lbib_ctx* c, const double* f) {
+    if (check_ctx(c) || !f) return -1;
+    return field_io(c, const_cast<double*>(f), 0, 9, true);
+}
+int lbib_get_populations(lbib_ctx* c, double* f) {
+    if (check_ctx(c) || !f) return -1;
+    return field_io(c, f, 0, 9, false);
+}
+int lbib_set_species(lbib_ctx* c, int s, const double* g) {
+    if (check_ctx(c) || !g) return -1;
+    if (s < 0 || s >= c->ns) { c->err = "species index out of range"; return -1; }
+    return field_io(c, const_cast<double*>(g), 1 + s, 5, true);
+}
+int lbib_get_species(lbib_ctx* c, int s, double* g) {
+    if (check_ctx(c) || !g) return -1;
+    if (s < 0 || s >= c->ns) { c->err = "species index out of range"; return -1; }
+    return field_io(c, g, 1 + s, 5, false);
+}
+int lbib_get_velocity(lbib_ctx* c, double* ux, double* uy) {
+    if (check_ctx(c) || !ux || !uy) return -1;
+    if (download(c, ux, c->ux, c->n * sizeof(double))) return -1;
+    return download(c, uy, c->uy, c->n * sizeof(double));
+}
+int lbib_get_domains(lbib_ctx* c, uint32_t* dom) {
+    if (check_ctx(c) || !dom) return -1;
+    if (download(c, dom, c->tag, c->n * sizeof(uint32_t))) return -1;
+    for (long i = 0; i < c->n; i++) dom[i] &= DOM_MASK;
+    return 0;
+}
+int lbib_set_domains(lbib_ctx* c, const uint32_t* dom) {
+    if (check_ctx(c) || !dom) return -1;
+    std::vector<uint32_t> t(c->n);
+    if (download(c, t.data(), c->tag, c->n * sizeof(uint32_t))) return -1;
+    for (long i = 0; i < c->n; i++) {
+        if (dom[i] > DOM_MASK) { c->err = "domain id exceeds 24 bits"; return -1; }
+        t[i] = (t[i] & ~DOM_MASK) | dom[i];
+    }
+    if (upload(c, c->tag, t.data(), c->n * sizeof(uint32_t))) return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_set_force_field(lbib_ctx* c, const double* fx, const double* fy) {
+    if (check_ctx(c) || !fx || !fy) return -1;
+    if (upload(c, c->Fx, fx, c->n * sizeof(double)) || upload(c, c->Fy, fy, c->n * sizeof(double))) return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_get_force_field(lbib_ctx* c, double* fx, double* fy) {
+    if (check_ctx(c) || !fx || !fy) return -1;
+    if (download(c, fx, c->Fx, c->n * sizeof(double))) return -1;
+    return download(c, fy, c->Fy, c->n * sizeof(double));
+}
+int lbib_set_mass_field(lbib_ctx* c, const double* m) {
+    if (check_ctx(c) || !m) return -1;
+    if (upload(c, c->msite, m, c->n * sizeof(double))) return -1;
+    c->msite_live = true;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- markers --
+int lbib_set_markers(lbib_ctx* c, int nm, const double* xy, const double* vxy, const double* oxy) {
+    if (check_ctx(c) || nm < 0 || (nm > 0 && !xy)) return -1;
+    if (ensure_markers(c, nm)) return -1;
+    std::vector<double> a(nm), b(nm);
+    auto split = [&](const double* src, double* dx, double* dy) -> int {
+        for (int k = 0; k < nm; k++) { a[k] = src[2 * k]; b[k] = src[2 * k + 1]; }
+        if (upload(c, dx, a.data(), nm * sizeof(double)) || upload(c, dy, b.data(), nm * sizeof(double))) return -1;
+        LBIB_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    };
+    if (nm > 0) {
+        if (split(xy, c->mx, c->my)) return -1;
+        if (vxy && split(vxy, c->mvx, c->mvy)) return -1;
+        if (oxy && split(oxy, c->mox, c->moy)) return -1;
+    }
+    c->nm = nm;
+    return 0;
+}
+int lbib_get_markers(lbib_ctx* c, double* xy, double* vxy, double* fxy, double* oxy) {
+    if (check_ctx(c)) return -1;
+    const int nm = c->nm;
+    std::vector<double> a(nm), b(nm);
+    auto join = [&](double* dst, const double* dx, const double* dy) -> int {
+        if (download(c, a.data(), dx, nm * sizeof(double)) || download(c, b.data(), dy, nm * sizeof(double))) return -1;
+        for (int k = 0; k < nm; k++) { dst[2 * k] = a[k]; dst[2 * k + 1] = b[k]; }
+        return 0;
+    };
+    if (nm == 0) return 0;
+    if (xy && join(xy, c->mx, c->my)) return -1;
+    if (vxy && join(vxy, c->mvx, c->mvy)) return -1;
+    if (fxy && join(fxy, c->mfx, c->mfy)) return -1;
+    if (oxy && join(oxy, c->mox, c->moy)) return -1;
+    return 0;
+}
+int lbib_set_connections(lbib_ctx* c, int nc, const int32_t* a, const int32_t* b, const uint32_t* dom,
+                         const int32_t* bc) {
+    if (check_ctx(c) || nc < 0) return -1;
+    if (nc > 0 && (!a || !b || !dom)) return -1;
+    for (int k = 0; k < nc; k++) {
+        if (a[k] < 0 || a[k] >= c->nm || b[k] < 0 || b[k] >= c->nm) {
+            c->err = "connection refers to a marker index out of range";
+            return -1;
+        }
+        if (dom[k] > DOM_MASK) { c->err = "domain id exceeds 24 bits"; return -1; }
+        if (bc && bc[k] >= c->nbcd) { c->err = "connection refers to an unknown BC descriptor"; return -1; }
+    }
+    if (nc > c->ccap) {
+        // keep the previous cure's candidate list valid across the reallocation
+        std::vector<Cand> keep;
+        DevStatus s;
+        if (download(c, &s, c->st, sizeof(s))) return -1;
+        keep.resize(s.ncand);
+        if (s.ncand && download(c, keep.data(), c->cand, s.ncand * sizeof(Cand))) return -1;
+        if (ensure_connections(c, nc)) return -1;
+        if ((int)keep.size() > c->candcap) { c->err = "candidate list too long"; return -1; }
+        if (!keep.empty() && upload(c, c->cand, keep.data(), keep.size() * sizeof(Cand))) return -1;
+    }
+    if (upload(c, c->ca, a, nc * sizeof(int32_t)) || upload(c, c->cb, b, nc * sizeof(int32_t)) ||
+        upload(c, c->cdom, dom, nc * sizeof(uint32_t)))
+        return -1;
+    std::vector<int32_t> none;
+    const int32_t* bcp = bc;
+    if (!bcp) { none.assign(std::max(nc, 1), -1); bcp = none.data(); }
+    if (upload(c, c->cbc, bcp, nc * sizeof(int32_t))) return -1;
+    c->nc = nc;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_set_bc_descriptors(lbib_ctx* c, int nd, const int32_t* off, const int32_t* species) {
+    if (check_ctx(c) || nd < 0 || !off) return -1;
+    const int total = off[nd];
+    for (int k = 0; k < total; k++)
+        if (species[k] < 0 || species[k] >= c->ns) { c->err = "BC descriptor species out of range"; return -1; }
+    dev_free(c->bcd_off);
+    dev_free(c->bcd_sp);
+    if (dev_alloc(c, &c->bcd_off, nd + 1) || dev_alloc(c, &c->bcd_sp, std::max(total, 1))) return -1;
+    if (upload(c, c->bcd_off, off, (nd + 1) * sizeof(int32_t)) || upload(c, c->bcd_sp, species, total * sizeof(int32_t)))
+        return -1;
+    c->nbcd = nd;
+    if (nd > 0 && !c->bc_v) {
+        if (dev_alloc(c, &c->bc_v, 8 * c->n) || dev_alloc(c, &c->bc_desc, 4 * c->n)) return -1;
+    }
+    c->any_bc = nd > 0;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_set_rings(lbib_ctx* c, const int32_t* prev, const int32_t* next, int nring, const uint32_t* ring_dom,
+                   const int32_t* ring_off, const int32_t* ring_marker) {
+    if (check_ctx(c) || nring < 0) return -1;
+    if (prev && next && c->nm > 0) {
+        if (upload(c, c->mprev, prev, c->nm * sizeof(int32_t)) || upload(c, c->mnext, next, c->nm * sizeof(int32_t)))
+            return -1;
+    }
+    if (nring > 0) {
+        const int tot = ring_off[nring];
+        if (nring + 1 > c->rcap) {
+            dev_free(c->ring_dom); dev_free(c->ring_off); dev_free(c->area);
+            c->rcap = std::max(nring + 1, 2 * c->rcap);
+            if (dev_alloc(c, &c->ring_dom, c->rcap) || dev_alloc(c, &c->ring_off, c->rcap) ||
+                dev_alloc(c, &c->area, c->rcap))
+                return -1;
+        }
+        if (tot > c->rmcap) {
+            dev_free(c->ring_marker);
+            c->rmcap = std::max(tot, 2 * c->rmcap);
+            if (dev_alloc(c, &c->ring_marker, c->rmcap)) return -1;
+        }
+        if (upload(c, c->ring_dom, ring_dom, nring * sizeof(uint32_t)) ||
+            upload(c, c->ring_off, ring_off, (nring + 1) * sizeof(int32_t)) ||
+            upload(c, c->ring_marker, ring_marker, tot * sizeof(int32_t)))
+            return -1;
+    }
+    c->nring = nring;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_set_celltypes(lbib_ctx* c, int ndom, const uint32_t* celltype) {
+    if (check_ctx(c) || ndom < 0 || (ndom > 0 && !celltype)) return -1;
+    if (ensure_domains(c, ndom)) return -1;
+    if (upload(c, c->celltype, celltype, ndom * sizeof(uint32_t))) return -1;
+    c->ndom = ndom;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_set_forces(lbib_ctx* c, int nm, const int32_t* off, const int32_t* type, const int32_t* partner,
+                    const double* prm) {
+    if (check_ctx(c) || nm != c->nm || !off) { if (c) c->err = "force CSR must cover every marker"; return -1; }
+    const int nf = off[nm];
+    for (int j = 0; j < nf; j++) {
+        if (type[j] < 0 || type[j] > 7) { c->err = "Invalid force solver type found."; return -1; }
+        const bool pair = type[j] == 0 || type[j] == 5 || type[j] == 6 || type[j] == 7;
+        if (pair && (partner[j] < 0 || partner[j] >= nm)) { c->err = "Could not compute and set the forces."; return -1; }
+    }
+    if (nf > c->fcap) {
+        dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
+        c->fcap = std::max(nf, 2 * c->fcap);
+        if (dev_alloc(c, &c->fs_type, c->fcap) || dev_alloc(c, &c->fs_partner, c->fcap) ||
+            dev_alloc(c, &c->fs_prm, 4 * (size_t)c->fcap))
+            return -1;
+    }
+    if (upload(c, c->fs_off, off, (nm + 1) * sizeof(int32_t)) || upload(c, c->fs_type, type, nf * sizeof(int32_t)) ||
+        upload(c, c->fs_partner, partner, nf * sizeof(int32_t)) || upload(c, c->fs_prm, prm, 4 * nf * sizeof(double)))
+        return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------------- step --
+static int stream_collide(lbib_ctx* c, uint32_t it, bool ftile) {
+    return c->prec == LBIB_FP64 ? launch_stream_collide_f64(c, it, ftile) : launch_stream_collide_mixed(c, it, ftile);
+}
+
+int lbib_step_lattice(lbib_ctx* c, uint32_t it) {
+    if (check_ctx(c)) return -1;
+    if (launch_force_and_spread(c)) return -1;
+    if (stream_collide(c, it, c->nm > 0)) return -1;
+    if (launch_clear_force(c)) return -1;
+    if (launch_interp_move(c)) return -1;
+    return 0;
+}
+int lbib_step_lattice_only(lbib_ctx* c, uint32_t it) {
+    if (check_ctx(c)) return -1;
+    return stream_collide(c, it, false);
+}
+int lbib_step_geometry(lbib_ctx* c, uint32_t it, int check_coarsen) {
+    if (check_ctx(c)) return -1;
+    return launch_geometry(c, it, check_coarsen ? 2 : 0);
+}
+int lbib_cure_lattice(lbib_ctx* c) {
+    if (check_ctx(c)) return -1;
+    return launch_geometry(c, 0, 1);
+}
+int lbib_set_box_outlet(lbib_ctx* c, int on) {
+    if (check_ctx(c)) return -1;
+    c->cfg.box_outlet = on ? 1 : 0;
+    return 0;
+}
+int lbib_read_status(lbib_ctx* c, lbib_status* out) {
+    if (check_ctx(c) || !out) return -1;
+    LBIB_HIP(hipMemcpyAsync(c->st_host, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    const DevStatus& s = *c->st_host;
+    out->bits = s.bits;
+    out->first_bad_marker = s.first_bad == INT_MAX ? -1 : s.first_bad;
+    out->n_touched_sites = (int32_t)s.ntouched;
+    out->n_boundary_nodes = (int32_t)s.nbn;
+    out->n_reinit = (int32_t)s.nchanged;
+    out->n_candidates = (int32_t)s.ncand;
+    return 0;
+}
+int lbib_clear_status(lbib_ctx* c) {
+    if (check_ctx(c)) return -1;
+    // bits and first_bad are the first two words of DevStatus
+    struct { uint32_t bits; int32_t first; } z = {0u, INT_MAX};
+    LBIB_HIP(hipMemcpyAsync(c->st, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// -------------------------------------------------------------- bio helpers --
+int lbib_compute_areas(lbib_ctx* c, double* area) {
+    if (check_ctx(c)) return -1;
+    if (launch_areas(c)) return -1;
+    if (area && c->nring > 0) return download(c, area, c->area, c->nring * sizeof(double));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_area_regulator(lbib_ctx* c, double target_area, double kp, double maxsource, double target_factor) {
+    if (check_ctx(c)) return -1;
+    return launch_area_regulator(c, target_area, kp, maxsource, target_factor);
+}
+int lbib_add_mass_by_domain(lbib_ctx* c, int ndom, const double* table) {
+    if (check_ctx(c) || ndom < 0 || !table) return -1;
+    if (ensure_domains(c, ndom)) return -1;
+    if (ndom > c->ndom) c->ndom = ndom;
+    std::vector<double> t(c->ndom, 0.0);
+    for (int d = 0; d < ndom; d++) t[d] = table[d];
+    if (upload(c, c->mtab, t.data(), c->ndom * sizeof(double))) return -1;
+    c->mtab_live = true;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,470 @@
+// lbib_geometry.hip -- per-step lattice "cure" on the device: connection
+// checks, boundary-node (cut-link) generation with closest-wins, the local
+// domain update with CDE re-initialisation, the per-row domain scanline, the
+// remesh/coarsen triggers and per-cell areas.
+//
+// Cut links.  Connection::generateBoundaryNodes (Connection.cpp:224-324)
+// visits, for each connection in vector order, every vertical lattice line it
+// crosses (south node, then north node) and then every horizontal one (west,
+// then east), and Connection::insertBoundaryNode (Connection.cpp:181-222)
+// keeps per (site, slot) the closest node, the first inserted on ties.  Here
+// each connection writes its candidates with a global insertion number
+// seq = conn * 256 + k; three passes pick min distance, then min seq among
+// equal distances, which is exactly the sequential outcome.
+#include "lbib_device_math.h"
+
+namespace lbib {
+
+__device__ __forceinline__ bool halted(const DevStatus* st) {
+    return (st->bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) != 0u;
+}
+
+// Connection::perturbConnection trigger (Connection.cpp:339-340, 363-364).
+__global__ void k_conn_check(int nc, const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
+                             const double* __restrict__ mx, const double* __restrict__ my, DevStatus* st) {
+    if (halted(st)) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ca[c], b = cb[c];
+    if (fabs(mx[a] - mx[b]) < kEPS || fabs(my[a] - my[b]) < kEPS)
+        atomicOr(&st->bits, LBIB_HALT_PERTURB_CONN);
+}
+
+// Delete the boundary nodes of the previous cure (GeometryHandler.cpp:363-369).
+__global__ void k_bn_reset(const Cand* __restrict__ cand, uint32_t* tag, DevStatus* st) {
+    if (halted(st)) return;
+    const uint32_t nprev = st->ncand;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nprev; j += gridDim.x * blockDim.x)
+        atomicAnd(&tag[cand[j].site], DOM_MASK);
+}
+
+__global__ void k_geo_counters(DevStatus* st) {
+    if (halted(st)) return;
+    st->ncand_prev = st->ncand;
+    st->ncand = 0;
+    st->nbn = 0;
+    st->nchanged = 0;
+}
+
+__device__ __forceinline__ double sqd(double x1, double y1, double x2, double y2) {
+    return (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);   // EulerianPoint.hpp:88-96
+}
+
+__global__ void k_cut_gen(int nc, int nx, int ny, long n, const int32_t* __restrict__ ca,
+                          const int32_t* __restrict__ cb, const uint32_t* __restrict__ cdom,
+                          const double* __restrict__ mx, const double* __restrict__ my, Cand* cand,
+                          int candcap, unsigned long long* best_dist, DevStatus* st) {
+    if (halted(st)) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double x1 = mx[ca[c]], y1 = my[ca[c]], x2 = mx[cb[c]], y2 = my[cb[c]];
+    const double maxX = fmax(x1, x2), maxY = fmax(y1, y2);
+    const double minX = fmin(x1, x2), minY = fmin(y1, y2);
+    const double m = (y2 - y1) / (x2 - x1);
+    const double nn = y1 - (y2 - y1) / (x2 - x1) * x1;
+    if (!isfinite(m)) {
+        atomicOr(&st->bits, LBIB_ERR_SLOPE);
+        return;
+    }
+    const int iv0 = (int)ceil(minX), iv1 = (int)floor(maxX);
+    const int ih0 = (int)ceil(minY), ih1 = (int)floor(maxY);
+    const int nv = iv1 >= iv0 ? iv1 - iv0 + 1 : 0;
+    const int nh = ih1 >= ih0 ? ih1 - ih0 + 1 : 0;
+    const int cnt = 2 * (nv + nh);
+    if (cnt == 0) return;
+    if (cnt > SEQ_PER_CONN) {
+        atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+        return;
+    }
+    const uint32_t base = atomicAdd(&st->ncand, (uint32_t)cnt);
+    if ((long)base + cnt > candcap) {
+        atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+        return;
+    }
+    const uint32_t dom = cdom[c];
+    int k = 0;
+    auto put = [&](int sx, int sy, int slot, uint32_t d, double bx, double by) {
+        Cand e;
+        if (sx < 0 || sx >= nx || sy < 0 || sy >= ny) {
+            atomicOr(&st->bits, LBIB_ERR_EDGE);
+            sx = 0; sy = 0;
+        }
+        e.site = sy * nx + sx;
+        e.slot = slot;
+        e.seq = (uint32_t)c * SEQ_PER_CONN + k;
+        e.domain = d;
+        e.dist2 = sqd((double)sx, (double)sy, bx, by);
+        e.bx = bx;
+        e.by = by;
+        e.conn = c;
+        e.win = 0;
+        cand[base + k] = e;
+        atomicMin(&best_dist[(long)(slot - 1) * n + e.site], (unsigned long long)__double_as_longlong(e.dist2));
+        k++;
+    };
+    // vertical lattice lines: node S at (i, floor y) [its slot N], node N at (i, ceil y) [slot S]
+    for (int i = iv0; i <= iv1; i++) {
+        const double y = m * i + nn;
+        const uint32_t dN = (x1 < x2) ? dom : 0u, dS = (x1 < x2) ? 0u : dom;
+        put(i, (int)floor(y), DN, dS, (double)i, y);
+        put(i, (int)ceil(y), DS, dN, (double)i, y);
+    }
+    // horizontal lattice lines: node W at (floor x, i) [slot E], node E at (ceil x, i) [slot W]
+    for (int i = ih0; i <= ih1; i++) {
+        const double x = (i - nn) / m;
+        const uint32_t dW = (y1 > y2) ? 0u : dom, dE = (y1 > y2) ? dom : 0u;
+        put((int)floor(x), i, DE, dW, x, (double)i);
+        put((int)ceil(x), i, DW, dE, x, (double)i);
+    }
+}
+
+__global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned long long* __restrict__ best_dist,
+                          uint32_t* best_seq, DevStatus* st) {
+    if (halted(st)) return;
+    const uint32_t nc = st->ncand;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        const Cand e = cand[j];
+        const long slotix = (long)(e.slot - 1) * n + e.site;
+        if ((unsigned long long)__double_as_longlong(e.dist2) == best_dist[slotix])
+            atomicMin(&best_seq[slotix], e.seq);
+    }
+}
+
+// Winners: mark the slot, record the domain and (for BC-carrying connections)
+// the node velocity of Connection::calculateVelocity (Connection.cpp:146-168).
+__global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_seq, const int32_t* __restrict__ ca,
+                          const int32_t* __restrict__ cb, const int32_t* __restrict__ cbc,
+                          const double* __restrict__ mx, const double* __restrict__ my,
+                          const double* __restrict__ mvx, const double* __restrict__ mvy, uint32_t* tag,
+                          uint32_t* bn_dom, double* bc_v, int32_t* bc_desc, DevStatus* st) {
+    if (halted(st)) return;
+    const uint32_t nc = st->ncand;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        Cand& e = cand[j];
+        const long slotix = (long)(e.slot - 1) * n + e.site;
+        if (best_seq[slotix] != e.seq) continue;
+        e.win = 1;
+        atomicAdd(&st->nbn, 1u);
+        bn_dom[slotix] = e.domain;
+        uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
+        const int desc = cbc[e.conn];
+        if (desc >= 0 && bc_v != nullptr) {
+            bits |= 1u << (BC_SHIFT + e.slot - 1);
+            const int a = ca[e.conn], b = cb[e.conn];
+            const double l = sqrt(sqd(mx[a], my[a], e.bx, e.by));
+            const double L = sqrt(sqd(mx[b], my[b], mx[a], my[a]));
+            bc_v[2 * slotix] = (mvx[b] - mvx[a]) * l / L + mvx[a];
+            bc_v[2 * slotix + 1] = (mvy[b] - mvy[a]) * l / L + mvy[a];
+            bc_desc[slotix] = desc;
+        }
+        atomicOr(&tag[e.site], bits);
+    }
+}
+
+// PhysicalNode::updateDomainIdentifier (PhysicalNode.cpp:109-124): a site takes
+// the domain of its last non-null boundary neighbour in slot order E,N,W,S.
+// Only sites with boundary nodes can change; the winner in the highest
+// occupied slot reports the site.  Also resets the selection buffers.
+__global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_t* __restrict__ tag,
+                            unsigned long long* best_dist, uint32_t* best_seq, uint32_t* changed,
+                            DevStatus* st) {
+    if (halted(st)) return;
+    const uint32_t nc = st->ncand;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        const Cand e = cand[j];
+        const long slotix = (long)(e.slot - 1) * n + e.site;
+        best_dist[slotix] = ~0ull;
+        best_seq[slotix] = ~0u;
+        if (!e.win) continue;
+        const uint32_t tg = tag[e.site];
+        const uint32_t mask = (tg >> BN_SHIFT) & 0xfu;
+        const int hi = 31 - __clz((int)mask);   // highest occupied slot - 1
+        if (hi + 1 != e.slot) continue;
+        if ((tg & DOM_MASK) != e.domain) {
+            const uint32_t p = atomicAdd(&st->nchanged, 1u);
+            if (p < (uint32_t)REINIT_CAP) {
+                changed[2 * p] = (uint32_t)e.site;
+                changed[2 * p + 1] = e.domain;
+            } else {
+                atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+            }
+        }
+    }
+}
+
+struct ReinitArgs {
+    int nx, ny;
+    long n;
+    int ns;
+    double* g;           // current species buffer [ns][5][n]
+    const double* ux;
+    const double* uy;
+    uint32_t* tag;
+    const uint32_t* changed;
+    uint32_t it;
+    SpeciesDev sp[LBIB_MAX_SPECIES];
+};
+
+template <int NS>
+__device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd) {
+    const int x = (int)(i % a.nx), y = (int)(i / a.nx);
+    const int EXd[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    const int EYd[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    auto nb = [&](int d) -> long {
+        const int xx = (x + EXd[d] + a.nx) % a.nx, yy = (y + EYd[d] + a.ny) % a.ny;
+        return (long)yy * a.nx + xx;
+    };
+    const long sn = 5 * a.n;
+    auto cswap = [&](int k, long j) -> double {
+        const double* gk = a.g + k * sn;
+        return c_swapped(gk[j], gk[a.n + j], gk[2 * a.n + j], gk[3 * a.n + j], gk[4 * a.n + j]);
+    };
+    a.tag[i] = (a.tag[i] & ~DOM_MASK) | nd;
+    const uint32_t tg = a.tag[i];
+    for (int k = 0; k < NS; k++) {
+        double sumC = 0.0;
+        int counter = 0;
+        const int dir1[4] = {DE, DN, DW, DS};
+        for (int q = 0; q < 4; q++) {
+            const int d = dir1[q];
+            const long j = nb(d);
+            const bool hasbn = (tg >> (BN_SHIFT + d - 1)) & 1u;
+            if (!hasbn && nd == (a.tag[j] & DOM_MASK)) { sumC += cswap(k, j); counter++; }
+        }
+        if (counter == 0) {
+            const int dir2[4] = {DNE, DNW, DSW, DSE};
+            for (int q = 0; q < 4; q++) {
+                const long j = nb(dir2[q]);
+                if (nd == (a.tag[j] & DOM_MASK)) { sumC += cswap(k, j); counter++; }
+            }
+        }
+        if (counter == 0 && a.sp[k].kind != LBIB_SPECIES_PLAIN) {
+            for (int q = 0; q < 4; q++) { sumC += cswap(k, nb(dir1[q])); counter++; }
+        }
+        sumC /= static_cast<double>(counter);
+        double d[5];
+        for (int q = 0; q < 5; q++) d[q] = sumC / 5.0;
+        double C = 0.0;
+        for (int q = 0; q < 5; q++) C += d[q];
+        double cj[NS > 0 ? NS : 1];
+        for (int j = 0; j < NS; j++) cj[j] = (j == k) ? C : cswap(j, i);
+        species_collide<NS>(d, C, a.ux[i], a.uy[i], a.sp[k], nd, a.it, cj);
+        double* gk = a.g + k * sn;
+        for (int q = 0; q < 5; q++) gk[q * a.n + i] = d[q];
+    }
+}
+
+// GeometryHandler::updateDomainIdentifier (GeometryHandler.cpp:58-66) runs the
+// per-site update in (y, x) order and each re-initialisation
+// (CDESolverD2Q5.cpp:156-228, ..._SIGNAL.cpp:184-250) reads neighbours updated
+// earlier in that order: sort the changed sites, then apply them in order.
+template <int NS>
+__global__ __launch_bounds__(256) void k_reinit(const ReinitArgs a, DevStatus* st) {
+    if (halted(st)) return;
+    __shared__ unsigned long long key[REINIT_CAP];
+    const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
+    if (cnt == 0) return;
+    uint32_t P = 1;
+    while (P < cnt) P <<= 1;
+    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x)
+        key[t] = (t < cnt) ? (((unsigned long long)a.changed[2 * t] << 32) | a.changed[2 * t + 1]) : ~0ull;
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= P; kk <<= 1) {
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                const uint32_t l = t ^ j;
+                if (l > t) {
+                    const bool up = (t & kk) == 0;
+                    const unsigned long long u = key[t], v = key[l];
+                    if ((u > v) == up) { key[t] = v; key[l] = u; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        for (uint32_t t = 0; t < cnt; t++) {
+            const long i = (long)(key[t] >> 32);
+            const uint32_t nd = (uint32_t)(key[t] & 0xffffffffu);
+            reinit_site<NS>(a, i, nd);
+        }
+    }
+}
+
+// GeometryHandler::updateAllDomainIdentifiers (GeometryHandler.cpp:68-85): per
+// row, the domain of the last west boundary node at or left of x (x >= 1), or
+// the domain of site (0, y), which must be 0.
+__global__ __launch_bounds__(256) void k_scanline(int nx, long n, uint32_t* tag, const uint32_t* __restrict__ bn_dom,
+                                                  DevStatus* st) {
+    if (halted(st)) return;
+    __shared__ int spos[256];
+    __shared__ uint32_t sdom[256];
+    __shared__ uint32_t carry;
+    const int y = blockIdx.x;
+    const long r0 = (long)y * nx;
+    if (threadIdx.x == 0) {
+        carry = tag[r0] & DOM_MASK;
+        if (carry != 0u) atomicOr(&st->bits, LBIB_ERR_SCANLINE);
+    }
+    __syncthreads();
+    const int wbit = BN_SHIFT + DW - 1;
+    for (int x0 = 0; x0 < nx; x0 += 256) {
+        const int x = x0 + threadIdx.x;
+        uint32_t tg = 0;
+        int pos = -1;
+        if (x < nx) {
+            tg = tag[r0 + x];
+            if (x >= 1 && ((tg >> wbit) & 1u)) {
+                pos = threadIdx.x;
+                sdom[threadIdx.x] = bn_dom[(long)(DW - 1) * n + r0 + x];
+            }
+        }
+        spos[threadIdx.x] = pos;
+        __syncthreads();
+        for (int off = 1; off < 256; off <<= 1) {
+            const int v = (threadIdx.x >= off) ? spos[threadIdx.x - off] : -1;
+            __syncthreads();
+            if (v > spos[threadIdx.x]) spos[threadIdx.x] = v;
+            __syncthreads();
+        }
+        const int last = spos[threadIdx.x];
+        const uint32_t val = (last >= 0) ? sdom[last] : carry;
+        if (x < nx && x >= 1) tag[r0 + x] = (tg & ~DOM_MASK) | val;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int lastall = spos[255];
+            if (lastall >= 0) carry = sdom[lastall];
+        }
+        __syncthreads();
+    }
+}
+
+// remeshBoundary trigger (GeometryHandler.cpp:393): a segment longer than MAXLENGTH.
+__global__ void k_remesh_check(int nc, const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
+                               const double* __restrict__ mx, const double* __restrict__ my, DevStatus* st) {
+    if (halted(st)) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int a = ca[c], b = cb[c];
+    const double len = sqrt((mx[a] - mx[b]) * (mx[a] - mx[b]) + (my[a] - my[b]) * (my[a] - my[b]));
+    if (len > kMAXLENGTH) atomicOr(&st->bits, LBIB_FLAG_REMESH);
+}
+
+// coarsenBoundary trigger (GeometryHandler.cpp:444-449): the first removal of the
+// sequential sweep happens iff some marker's neighbours are closer than MAXLENGTH.
+__global__ void k_coarsen_check(int nm, const int32_t* __restrict__ prev, const int32_t* __restrict__ next,
+                                const double* __restrict__ mx, const double* __restrict__ my, DevStatus* st) {
+    if (halted(st)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const int p = prev[k], q = next[k];
+    if (p < 0 || q < 0) return;
+    const double len = sqrt((mx[q] - mx[p]) * (mx[q] - mx[p]) + (my[q] - my[p]) * (my[q] - my[p]));
+    if (len < kMAXLENGTH) atomicOr(&st->bits, LBIB_FLAG_COARSEN);
+}
+
+// Shoelace area per ring, FP64, in ring order (GeometryHandler.cpp:101-134).
+__global__ void k_areas(int nring, const int32_t* __restrict__ off, const int32_t* __restrict__ rm,
+                        const double* __restrict__ mx, const double* __restrict__ my, double* area) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nring) return;
+    const int b = off[r], e = off[r + 1];
+    double sum = 0.0;
+    for (int k = b; k < e; k++) {
+        const int k1 = (k + 1 == e) ? b : k + 1;
+        sum += mx[rm[k]] * my[rm[k1]] - mx[rm[k1]] * my[rm[k]];
+    }
+    area[r] = sum * 0.5;
+}
+
+// tutorial_01_BioSolverAreaRegulator.cpp:66-99, per domain.
+__global__ void k_mass_default(int ndom, const uint32_t* __restrict__ celltype, double maxs, double* mtab) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ndom) return;
+    mtab[d] = (d == 0) ? 0.0 : ((celltype[d] == 1u) ? maxs : 0.0);
+}
+__global__ void k_mass_regulated(int nring, const uint32_t* __restrict__ ring_dom, const double* __restrict__ area,
+                                 const uint32_t* __restrict__ celltype, double* target, double target_area,
+                                 double kp, double maxs, double factor, int ndom, double* mtab) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nring) return;
+    const uint32_t d = ring_dom[r];
+    if ((int)d >= ndom) return;
+    const double A = area[r];
+    if (isnan(target[d])) target[d] = (factor > 0.0) ? factor * A : target_area;
+    double src = kp * (target[d] - A);
+    if (src > maxs) src = maxs;
+    else if (src < -maxs) src = -maxs;
+    if (d != 0u && celltype[d] != 1u) mtab[d] = src;
+}
+
+template <int NS>
+static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
+    k_reinit<NS><<<1, 256, 0, c->stream>>>(a, c->st);
+}
+
+int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
+    // mode: 0 = moveLattice (local update + reinit), 1 = cureLattice (scanline only),
+    // bit 2 = check coarsening
+    const bool cure = (mode & 1) != 0;
+    const bool coarsen = (mode & 2) != 0;
+    const int bs = 256;
+    const int gs = 1024;
+    if (c->nc > 0)
+        k_conn_check<<<(c->nc + bs - 1) / bs, bs, 0, c->stream>>>(c->nc, c->ca, c->cb, c->mx, c->my, c->st);
+    k_bn_reset<<<gs, bs, 0, c->stream>>>(c->cand, c->tag, c->st);
+    k_geo_counters<<<1, 1, 0, c->stream>>>(c->st);
+    if (c->nc > 0)
+        k_cut_gen<<<(c->nc + bs - 1) / bs, bs, 0, c->stream>>>(c->nc, c->nx, c->ny, c->n, c->ca, c->cb, c->cdom,
+                                                               c->mx, c->my, c->cand, c->candcap, c->best_dist,
+                                                               c->st);
+    k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
+    k_cut_win<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my, c->mvx,
+                                        c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->st);
+    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, c->st);
+    if (!cure) {
+        ReinitArgs a;
+        a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;
+        a.g = (double*)c->g[c->cur];
+        a.ux = c->ux; a.uy = c->uy; a.tag = c->tag; a.changed = c->changed; a.it = it;
+        for (int s = 0; s < LBIB_MAX_SPECIES; s++) a.sp[s] = c->sp[s];
+        switch (c->ns) {
+            case 0: launch_reinit_t<0>(c, a); break;
+            case 1: launch_reinit_t<1>(c, a); break;
+            case 2: launch_reinit_t<2>(c, a); break;
+            case 3: launch_reinit_t<3>(c, a); break;
+            default: launch_reinit_t<4>(c, a); break;
+        }
+    }
+    k_scanline<<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, c->tag, c->bn_dom, c->st);
+    if (c->nc > 0)
+        k_remesh_check<<<(c->nc + bs - 1) / bs, bs, 0, c->stream>>>(c->nc, c->ca, c->cb, c->mx, c->my, c->st);
+    if (coarsen && c->nm > 0)
+        k_coarsen_check<<<(c->nm + bs - 1) / bs, bs, 0, c->stream>>>(c->nm, c->mprev, c->mnext, c->mx, c->my,
+                                                                     c->st);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_areas(lbib_ctx* c) {
+    if (c->nring == 0) return 0;
+    k_areas<<<(c->nring + 255) / 256, 256, 0, c->stream>>>(c->nring, c->ring_off, c->ring_marker, c->mx, c->my,
+                                                          c->area);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor) {
+    if (launch_areas(c)) return -1;
+    if (c->ndom == 0) return 0;
+    k_mass_default<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->celltype, maxs, c->mtab);
+    if (c->nring > 0)
+        k_mass_regulated<<<(c->nring + 255) / 256, 256, 0, c->stream>>>(c->nring, c->ring_dom, c->area, c->celltype,
+                                                                       c->target, target, kp, maxs, factor,
+                                                                       c->ndom, c->mtab);
+    LBIB_HIP(hipGetLastError());
+    c->mtab_live = true;
+    return 0;
+}
+
+}  // namespace lbib

@@ -26,7 +26,7 @@ constexpr int BN_SHIFT = 24;        // bit 24 + (slot-1), slot in E,N,W,S
 constexpr int BC_SHIFT = 28;
 constexpr int TILE_W = 256;         // lattice kernel block width = force tile width
 constexpr int MAX_BC_SPECIES = LBIB_MAX_SPECIES;
-constexpr int REINIT_CAP = 8192;    // sites per step whose domain may change
+constexpr int REINIT_CAP = 4096;    // sites per step whose domain may change
 constexpr int SEQ_PER_CONN = 256;   // candidates per connection (segment < 127 lu)
 
 struct SpeciesDev {
@@ -134,18 +134,19 @@ struct lbib_ctx {
 
 namespace lbib {
 // launchers implemented per translation unit
-int launch_init(lbib_ctx* c, const double* u0_dev);
-int launch_stream_collide(lbib_ctx* c, uint32_t it, bool use_ftile);
+int launch_init_f64(lbib_ctx* c, const double* u0_dev);
+int launch_stream_collide_f64(lbib_ctx* c, uint32_t it, bool use_ftile);
+int launch_init_mixed(lbib_ctx* c, const double* u0_dev);
+int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile);
 int launch_force_and_spread(lbib_ctx* c);
 int launch_clear_force(lbib_ctx* c);
 int launch_interp_move(lbib_ctx* c);
-int launch_geometry(lbib_ctx* c, uint32_t it, int check_coarsen);
+int launch_geometry(lbib_ctx* c, uint32_t it, int mode);
 int launch_areas(lbib_ctx* c);
 int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor);
-int launch_pack_f(lbib_ctx* c, const double* src_dev, int which);     // host layout -> storage
-int launch_unpack_f(lbib_ctx* c, double* dst_dev, int which);         // storage -> host layout
-int launch_pack_g(lbib_ctx* c, int s, const double* src_dev);
-int launch_unpack_g(lbib_ctx* c, int s, double* dst_dev);
+// FP64 <-> storage conversion of the current buffers (mixed precision only)
+int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq);
+int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq);
 }  // namespace lbib
 
 #define LBIB_HIP(call)                                                         \
