@@ -1,0 +1,717 @@
+// lbib_host.cpp -- host runtime (see lbib_host.hpp).
+#include "lbib_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <unordered_map>
+
+namespace LbmLib {
+
+namespace {
+constexpr double kEPSILON = 10E-10;   // Constants.hpp:34
+constexpr double kMAXLENGTH = 0.5;    // Constants.hpp:49
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+}  // namespace
+
+// ------------------------------------------------------------- parameters --
+void GlobalSimulationParameters::loadGlobalSimulationParameters(const std::string& file) {
+    std::ifstream in(file);
+    if (!in.is_open()) LBIB_FAIL("Cannot find the parameter file.");
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::stringstream ls(line);
+        std::string key;
+        ls >> key;
+        if (key == "CurrentIterations:") ls >> currentIteration;
+        else if (key == "Iterations:") ls >> iterations;
+        else if (key == "ReportStep:") ls >> reportSteps;
+        else if (key == "SizeX:") ls >> sizeX;
+        else if (key == "SizeY:") ls >> sizeY;
+        else if (key == "tauFluid:") ls >> tauFluid;
+        else if (key == "CDESolvers:") {
+            std::string name;
+            double tau;
+            while (ls >> name >> tau) cdeSolvers[name] = tau;
+        } else {
+            throw Exception("An unknown parameter was passed in the global Parameter file");
+        }
+    }
+}
+void GlobalSimulationParameters::writeGlobalSimulationParameters(const std::string& file) const {
+    std::ofstream out(file);
+    if (!out.is_open()) LBIB_FAIL("Cannot open the parameter output file.");
+    out << "CurrentIterations:" << '\t' << currentIteration << "\n";
+    out << "Iterations:" << '\t' << iterations << "\n";
+    out << "ReportStep:" << '\t' << reportSteps << "\n";
+    out << "SizeX:" << '\t' << sizeX << "\n";
+    out << "SizeY:" << '\t' << sizeY << "\n";
+    out << "tauFluid:" << '\t' << tauFluid << "\n";
+    out << "CDESolvers:";
+    for (auto& kv : cdeSolvers) out << '\t' << kv.first << '\t' << kv.second;
+    out << "\n";
+}
+
+namespace geometry {
+
+// ------------------------------------------------------------- geometry ----
+Geometry::Geometry(const std::string& file) {
+    if (file.size() >= 4 && file.compare(file.size() - 4, 4, ".txt") == 0) loadGeometryTXT(file);
+    else LBIB_FAIL("only .txt geometries are supported by this loader");
+}
+
+void Geometry::perturbatePosition(GeometryNode& n) {
+    if (std::abs(static_cast<int>(n.x) - n.x) <= kEPSILON) {
+        double r = 0.0;
+        while (std::abs(r) < 5.0 * kEPSILON) r = disNode_(genNode_);
+        n.x = n.x + r;
+        nodePerturbations++;
+    }
+    if (std::abs(static_cast<int>(n.y) - n.y) <= kEPSILON) {
+        double r = 0.0;
+        while (std::abs(r) < 5.0 * kEPSILON) r = disNode_(genNode_);
+        n.y = n.y + r;
+        nodePerturbations++;
+    }
+    // The reference passes two draws as setPos(x + draw, y + draw); GCC
+    // evaluates those call arguments right to left, so y takes the first draw.
+    for (int pass = 0; pass < 2; pass++) {
+        const double lhs = (pass == 0) ? (n.y - std::floor(n.y)) : (std::ceil(n.y) - n.y);
+        if (std::abs(lhs - (n.x - std::floor(n.x))) <= kEPSILON) {
+            const double ry = disNode_(genNode_);
+            const double rx = disNode_(genNode_);
+            n.x = n.x + rx;
+            n.y = n.y + ry;
+            nodePerturbations++;
+        }
+    }
+}
+
+void Geometry::setPos(GeometryNode& n, double x, double y) {
+    n.x = x;
+    n.y = y;
+    perturbatePosition(n);
+}
+
+void Geometry::perturbConnection(Connection& c) {
+    if (std::abs(c.first->x - c.second->x) < kEPSILON) {
+        double r = 0.0;
+        while (std::abs(r) < 5.0 * kEPSILON) r = disConn_(genConn_);
+        setPos(*c.first, c.first->x + r, c.first->y);
+        connectionPerturbations++;
+    }
+    if (std::abs(c.first->y - c.second->y) < kEPSILON) {
+        double r = 0.0;
+        while (std::abs(r) < 5.0 * kEPSILON) r = disConn_(genConn_);
+        setPos(*c.first, c.first->x, c.first->y + r);
+        connectionPerturbations++;
+    }
+}
+
+void Geometry::perturbConnections() {
+    std::map<unsigned, Connection*> start;
+    for (auto& c : connections_)
+        if (!start.count(c->domain)) start[c->domain] = c.get();
+    for (auto& kv : start) {
+        Connection* c = kv.second;
+        do {
+            perturbConnection(*c);
+            c = c->second->out;
+            if (!c) LBIB_FAIL("open membrane ring");
+        } while (c != kv.second);
+    }
+}
+
+unsigned Geometry::addGeometryNode(unsigned id, double x, double y) {
+    if (nodes_.count(id)) LBIB_FAIL("GeometryNode id already exists");
+    auto n = std::unique_ptr<GeometryNode>(new GeometryNode());
+    n->id = id;
+    n->x = x;
+    n->y = y;
+    n->ox = x - 5.0;   // GeometryNode.cpp:102 forces the first stencil link
+    n->oy = y - 5.0;
+    perturbatePosition(*n);
+    nodes_[id] = std::move(n);
+    return id;
+}
+unsigned Geometry::addGeometryNode(double x, double y) {
+    const unsigned id = nodes_.empty() ? 0u : nodes_.rbegin()->first + 1;
+    return addGeometryNode(id, x, y);
+}
+
+void Geometry::addConnection(unsigned id1, unsigned id2, unsigned domain,
+                             const std::map<std::string, std::vector<std::string>>& descriptor) {
+    auto a = nodes_.find(id1), b = nodes_.find(id2);
+    if (a == nodes_.end() || b == nodes_.end()) LBIB_FAIL("connection refers to an unknown GeometryNode");
+    auto c = std::make_shared<Connection>();
+    c->first = a->second.get();
+    c->second = b->second.get();
+    c->domain = domain;
+    c->descriptor = descriptor;
+    connections_.push_back(c);
+    c->first->out = c.get();
+    c->second->in = c.get();
+    if (!cellTypes_.count(domain)) cellTypes_[domain] = 1;   // Geometry.cpp:140
+}
+
+void Geometry::eraseConnection(Connection* c) {
+    if (c->first->out == c) c->first->out = nullptr;
+    if (c->second->in == c) c->second->in = nullptr;
+    connections_.erase(std::remove_if(connections_.begin(), connections_.end(),
+                                      [c](const std::shared_ptr<Connection>& p) { return p.get() == c; }),
+                       connections_.end());
+}
+
+void Geometry::removeGeometryNode(unsigned id) {
+    auto it = nodes_.find(id);
+    if (it == nodes_.end() || !it->second->in || !it->second->out)
+        LBIB_FAIL("Geometry::removeGeometryNode: Could not remove the GeometryNode");
+    Connection* c1 = it->second->in;
+    Connection* c2 = it->second->out;
+    GeometryNode* n1 = c1->first;
+    GeometryNode* n2 = c2->second;
+    const auto descriptor = c1->descriptor;
+    const unsigned dom = c1->domain;
+    eraseConnection(c1);
+    eraseConnection(c2);
+    nodes_.erase(it);
+    addConnection(n1->id, n2->id, dom, descriptor);
+}
+
+bool Geometry::checkGeometryIntegrity() const {
+    for (auto& kv : nodes_) {
+        const GeometryNode* n = kv.second.get();
+        if (!n->in || !n->out || n->in->second != n || n->out->first != n) return false;
+    }
+    return connections_.size() == nodes_.size();
+}
+
+void Geometry::loadGeometryTXT(const std::string& file) {
+    std::ifstream in(file);
+    if (!in.is_open()) LBIB_FAIL("Cannot find the geometry.txt file.");
+    if (!nodes_.empty() || !connections_.empty()) LBIB_FAIL("Only load the geometry once.");
+    std::string line;
+    bool readingNodes = true;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        if (line.find("#Connection") != std::string::npos) readingNodes = false;
+        if (line[0] == '#') continue;
+        std::stringstream ls(line);
+        if (readingNodes) {
+            unsigned id;
+            double x, y;
+            ls >> id >> x >> y;
+            addGeometryNode(id, x, y);
+        } else {
+            unsigned id1, id2, dom;
+            ls >> id1 >> id2 >> dom;
+            std::map<std::string, std::vector<std::string>> desc;
+            std::string b, s;
+            while (ls >> b >> s) desc[b].push_back(s);
+            addConnection(id1, id2, dom, desc);
+            cellTypes_[dom] = 1;   // all cell types are 1 for txt geometries
+        }
+    }
+}
+
+void Geometry::writeGeometry(const std::string& file) const {
+    std::ofstream out(file);
+    if (!out.is_open()) LBIB_FAIL("Cannot open the output file for the geometry.");
+    out << "#Nodes (id\txPos\tyPos)\n";
+    for (auto& kv : nodes_) out << kv.first << '\t' << kv.second->x << '\t' << kv.second->y << '\n';
+    out << "#Connection (nodeId1\tnodeId2\tdomainId\tbsolver\tcdesolver\t...)\n";
+    for (auto& c : connections_) {
+        out << c->first->id << '\t' << c->second->id << '\t' << c->domain;
+        for (auto& kv : c->descriptor)
+            for (auto& s : kv.second) out << '\t' << kv.first << '\t' << s;
+        out << '\n';
+    }
+}
+
+}  // namespace geometry
+
+namespace solver {
+
+void ForceSolver::addForce(std::stringstream* d) {
+    std::stringstream copy(d->str());
+    unsigned type, id;
+    copy >> type >> id;
+    ForceStruct f{};
+    f.type = (int)type;
+    f.id1 = id;
+    std::stringstream s(d->str());
+    unsigned t;
+    s >> t;
+    switch (type) {
+        case 0: case 7: s >> f.id1 >> f.id2 >> f.p[0] >> f.p[1]; break;
+        case 5: case 6: s >> f.id1 >> f.id2 >> f.p[0]; break;
+        case 1: s >> f.id1 >> f.p[0] >> f.p[1] >> f.p[2] >> f.p[3]; break;
+        case 2: s >> f.id1 >> f.p[0] >> f.p[1]; break;
+        case 3: case 4: s >> f.id1 >> f.p[0] >> f.p[1] >> f.p[2]; break;
+        default: LBIB_FAIL("Invalid force solver type found.");
+    }
+    addForce(f);
+}
+
+void ForceSolver::loadForceFile(const std::string& file) {
+    std::ifstream in(file);
+    if (!in.is_open()) LBIB_FAIL("Cannot find the force solver file.");
+    std::string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::stringstream ls(line);
+        addForce(&ls);
+    }
+}
+
+void ForceSolver::writeForceSolver(const std::string& file) const {
+    std::ofstream out(file);
+    if (!out.is_open()) throw Exception("Problem to open the output file for the force solver");
+    for (auto& kv : forcemap_)
+        for (auto& f : kv.second) {
+            out << f.type << '\t' << f.id1;
+            switch (f.type) {
+                case 0: case 7: out << '\t' << f.id2 << '\t' << f.p[0] << '\t' << f.p[1]; break;
+                case 5: case 6: out << '\t' << f.id2 << '\t' << f.p[0]; break;
+                case 1: out << '\t' << f.p[0] << '\t' << f.p[1] << '\t' << f.p[2] << '\t' << f.p[3]; break;
+                case 2: out << '\t' << f.p[0] << '\t' << f.p[1]; break;
+                default: out << '\t' << f.p[0] << '\t' << f.p[1] << '\t' << f.p[2]; break;
+            }
+            out << '\n';
+        }
+}
+
+void ForceSolver::deleteForceType(unsigned type) {
+    for (auto& kv : forcemap_)
+        kv.second.erase(std::remove_if(kv.second.begin(), kv.second.end(),
+                                       [type](const ForceStruct& f) { return (unsigned)f.type == type; }),
+                        kv.second.end());
+    dirty_ = true;
+}
+
+}  // namespace solver
+
+// ------------------------------------------------------------ bio solvers --
+namespace {
+
+// tutorial_01_BioSolverMembraneTension (tutorial_01_BioSolverMembraneTension.cpp:42-77):
+// every FREQUENCY iterations, type-6 tension of FORCE along both directions of
+// every connection.  FORCE goes through a text descriptor (6 significant digits).
+struct MembraneTension : BioSolver {
+    unsigned freq = 10;
+    double force = 0.01;
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        auto& fs = r.getForceSolver();
+        fs.deleteForceType(6);
+        for (auto& c : r.getGeometry().getConnections()) {
+            std::stringstream d;
+            d << "6\t" << c->first->id << "\t" << c->second->id << "\t" << force;
+            fs.addForce(&d);
+            std::stringstream e;
+            e << "6\t" << c->second->id << "\t" << c->first->id << "\t" << force;
+            fs.addForce(&e);
+        }
+    }
+};
+
+// tutorial_01_BioSolverAreaRegulator (tutorial_01_BioSolverAreaRegulator.cpp:43-100)
+// served by one device op (areas, clamped source, addMass on every cell site).
+struct AreaRegulator : BioSolver {
+    unsigned freq = 1;
+    double target = 200.0, kp = 0.00001, maxsource = 0.004, factor = 0.0;
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        if (lbib_area_regulator(r.context(), target, kp, maxsource, factor) != 0)
+            throw Exception(lbib_last_error(r.context()));
+    }
+};
+
+}  // namespace
+
+// --------------------------------------------------------- the runner ------
+SimulationRunner::SimulationRunner(const GlobalSimulationParameters& params, geometry::Geometry& geo, int precision,
+                                   int forceModel)
+    : params_(params), geo_(geo), precision_(precision), forceModel_(forceModel) {
+    for (auto& kv : params_.cdeSolvers) {
+        if (!knownCDESolver(kv.first)) throw std::bad_typeid();   // Factory.hpp:45-47
+        speciesNames_.push_back(kv.first);
+        lbib_species_spec s;
+        std::memset(&s, 0, sizeof(s));
+        s.tau = kv.second;
+        if (kv.first == "CDESolverD2Q5") {
+            s.kind = LBIB_SPECIES_PLAIN;
+        } else {
+            // tutorial_01_CDESolverD2Q5_SIGNAL constants (…_SIGNAL.cpp:39-43)
+            s.kind = LBIB_SPECIES_SIGNAL;
+            s.production = 0.0001;
+            s.decay = 0.00001;
+            s.switchoff = 5000;
+            s.init_inside = 1.0;
+            s.prod_domain = 1;
+        }
+        specs_[kv.first] = s;
+    }
+}
+
+SimulationRunner::~SimulationRunner() {
+    if (ctx_) lbib_destroy(ctx_);
+}
+
+bool SimulationRunner::knownCDESolver(const std::string& name) {
+    return name == "CDESolverD2Q5" || name == "tutorial_01_CDESolverD2Q5_SIGNAL" ||
+           name.rfind("SIGNAL", 0) == 0 || name.rfind("LINEAR", 0) == 0;
+}
+
+void SimulationRunner::setSpeciesSpec(const std::string& name, const lbib_species_spec& spec) {
+    if (!specs_.count(name)) LBIB_FAIL("The CDE Solver with the name " + name + " does not exists.");
+    const double tau = specs_[name].tau;
+    specs_[name] = spec;
+    specs_[name].tau = tau;
+}
+
+int SimulationRunner::speciesIndex(const std::string& name) const {
+    for (size_t k = 0; k < speciesNames_.size(); k++)
+        if (speciesNames_[k] == name) return (int)k;
+    return -1;
+}
+
+void SimulationRunner::check(int rc, const char* what) {
+    if (rc != 0) throw Exception(std::string(what) + ": " + lbib_last_error(ctx_));
+}
+
+void SimulationRunner::setup() {
+    lbib_config cfg;
+    std::memset(&cfg, 0, sizeof(cfg));
+    cfg.nx = (int)params_.sizeX;
+    cfg.ny = (int)params_.sizeY;
+    cfg.tau_fluid = params_.tauFluid;
+    cfg.force_model = forceModel_;
+    cfg.precision = precision_;
+    cfg.nspecies = (int)speciesNames_.size();
+    if (cfg.nspecies > LBIB_MAX_SPECIES) LBIB_FAIL("at most 4 CDE species per context");
+    for (int k = 0; k < cfg.nspecies; k++) cfg.species[k] = specs_[speciesNames_[k]];
+    cfg.max_markers = (int)geo_.getGeometryNodes().size() * 8 + 1024;
+    cfg.max_connections = cfg.max_markers;
+    if (lbib_create(&cfg, &ctx_) != 0) {
+        std::string m = ctx_ ? lbib_last_error(ctx_) : "lbib_create failed";
+        if (ctx_) lbib_destroy(ctx_);
+        ctx_ = nullptr;
+        throw Exception(m);
+    }
+    // GeometryHandler::cureLattice: perturb connections, link stencils (implicit
+    // on the device), boundary nodes, scanline.
+    geo_.perturbConnections();
+    topologyDirty_ = true;
+    uploadTopology();
+    check(lbib_cure_lattice(ctx_), "cureLattice");
+    check(lbib_read_status(ctx_, &lastStatus), "status");
+    if (lastStatus.bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) throw Exception("cureLattice: " + describe(lastStatus));
+    check(lbib_clear_status(ctx_), "status");
+}
+
+void SimulationRunner::initSolvers(const double* u0) { check(lbib_init_solvers(ctx_, u0), "initSolvers"); }
+
+void SimulationRunner::initForceSolver(const std::string& file) { forceSolver_.loadForceFile(file); }
+
+void SimulationRunner::addMassSolver(const std::string& name) {
+    if (name != "MassSolverBoxOutlet") throw std::bad_typeid();
+    massSolvers_.push_back(name);
+    check(lbib_set_box_outlet(ctx_, 1), "addMassSolver");
+}
+
+void SimulationRunner::addBioSolver(const std::string& name) {
+    std::unique_ptr<BioSolver> b;
+    if (name == "tutorial_01_BioSolverMembraneTension") b.reset(new MembraneTension());
+    else if (name == "tutorial_01_BioSolverAreaRegulator") b.reset(new AreaRegulator());
+    else throw std::bad_typeid();
+    b->name = name;
+    bio_.push_back(std::move(b));
+}
+
+void SimulationRunner::addBioSolver(std::unique_ptr<BioSolver> b) { bio_.push_back(std::move(b)); }
+
+void SimulationRunner::syncMarkersFromDevice() {
+    if (hostMarkersFresh_) return;
+    const size_t nm = ids_.size();
+    std::vector<double> xy(2 * nm), v(2 * nm), o(2 * nm);
+    check(lbib_get_markers(ctx_, xy.data(), v.data(), nullptr, o.data()), "get_markers");
+    size_t k = 0;
+    for (auto& kv : geo_.nodes()) {
+        auto& n = *kv.second;
+        n.x = xy[2 * k]; n.y = xy[2 * k + 1];
+        n.vx = v[2 * k]; n.vy = v[2 * k + 1];
+        n.ox = o[2 * k]; n.oy = o[2 * k + 1];
+        k++;
+    }
+    hostMarkersFresh_ = true;
+}
+
+void SimulationRunner::uploadTopology() {
+    const auto& nodes = geo_.getGeometryNodes();
+    const int nm = (int)nodes.size();
+    ids_.clear();
+    std::unordered_map<const geometry::GeometryNode*, int> index;
+    std::vector<double> xy(2 * nm), v(2 * nm), o(2 * nm);
+    int k = 0;
+    for (auto& kv : nodes) {
+        const auto& n = *kv.second;
+        ids_.push_back(kv.first);
+        index[&n] = k;
+        xy[2 * k] = n.x; xy[2 * k + 1] = n.y;
+        v[2 * k] = n.vx; v[2 * k + 1] = n.vy;
+        o[2 * k] = n.ox; o[2 * k + 1] = n.oy;
+        k++;
+    }
+    check(lbib_set_markers(ctx_, nm, xy.data(), v.data(), o.data()), "set_markers");
+    // boundary-condition descriptors (BoundaryNode.cpp:43-46)
+    std::map<std::vector<int>, int> descIndex;
+    bcDescriptors_.clear();
+    const auto& conns = geo_.getConnections();
+    const int nc = (int)conns.size();
+    std::vector<int32_t> a(nc), b(nc), bc(nc, -1);
+    std::vector<uint32_t> dom(nc);
+    uint32_t maxdom = 0;
+    for (int c = 0; c < nc; c++) {
+        a[c] = index.at(conns[c]->first);
+        b[c] = index.at(conns[c]->second);
+        dom[c] = conns[c]->domain;
+        maxdom = std::max(maxdom, dom[c]);
+        for (auto& kv : conns[c]->descriptor) {
+            if (kv.first != "BoundarySolverNoFluxD2Q5") throw std::bad_typeid();
+            std::vector<int> sp;
+            for (auto& s : kv.second) {
+                const int si = speciesIndex(s);
+                if (si < 0) throw Exception("The CDE Solver with the name " + s + " does not exists.");
+                sp.push_back(si);
+            }
+            auto it = descIndex.find(sp);
+            if (it == descIndex.end()) {
+                it = descIndex.emplace(sp, (int)bcDescriptors_.size()).first;
+                bcDescriptors_.push_back(sp);
+            }
+            bc[c] = it->second;
+        }
+    }
+    if (!bcDescriptors_.empty()) {
+        std::vector<int32_t> off{0}, sp;
+        for (auto& d : bcDescriptors_) {
+            for (int s : d) sp.push_back(s);
+            off.push_back((int32_t)sp.size());
+        }
+        check(lbib_set_bc_descriptors(ctx_, (int)bcDescriptors_.size(), off.data(), sp.data()), "set_bc");
+    }
+    check(lbib_set_connections(ctx_, nc, a.data(), b.data(), dom.data(), bc.data()), "set_connections");
+    // ring neighbours and rings (GeometryHandler.cpp:101-134)
+    std::vector<int32_t> prev(nm, -1), next(nm, -1);
+    k = 0;
+    for (auto& kv : nodes) {
+        const auto& n = *kv.second;
+        if (n.in) prev[k] = index.at(n.in->first);
+        if (n.out) next[k] = index.at(n.out->second);
+        k++;
+    }
+    std::map<unsigned, const geometry::Connection*> start;
+    for (auto& c : conns)
+        if (!start.count(c->domain)) start[c->domain] = c.get();
+    std::vector<uint32_t> rdom;
+    std::vector<int32_t> roff{0}, rm;
+    for (auto& kv : start) {
+        const geometry::Connection* c = kv.second;
+        size_t guard = 0;
+        do {
+            rm.push_back(index.at(c->second));
+            c = c->second->out;
+            if (!c || ++guard > conns.size()) LBIB_FAIL("open membrane ring");
+        } while (c != kv.second);
+        rdom.push_back(kv.first);
+        roff.push_back((int32_t)rm.size());
+    }
+    check(lbib_set_rings(ctx_, prev.data(), next.data(), (int)rdom.size(), rdom.data(), roff.data(), rm.data()),
+          "set_rings");
+    // cell types per domain id (missing domains read as 1, GeometryHandler.cpp:278-284)
+    auto& ctm = geo_.cellTypeTrackerMap();
+    for (auto& kv : ctm) maxdom = std::max(maxdom, kv.first);
+    std::vector<uint32_t> ct(maxdom + 1, 1u);
+    for (auto& kv : ctm) ct[kv.first] = kv.second;
+    check(lbib_set_celltypes(ctx_, (int)ct.size(), ct.data()), "set_celltypes");
+    uploadForces();
+    topologyDirty_ = false;
+    hostMarkersFresh_ = true;
+}
+
+void SimulationRunner::uploadForces() {
+    const int nm = (int)ids_.size();
+    std::vector<int32_t> off(nm + 1, 0), type, partner;
+    std::vector<double> prm;
+    auto dense = [&](unsigned id) -> int {
+        auto it = std::lower_bound(ids_.begin(), ids_.end(), id);
+        if (it == ids_.end() || *it != id) throw Exception("Could not compute and set the forces.");
+        return (int)(it - ids_.begin());
+    };
+    const auto& fm = forceSolver_.forcemap();
+    for (auto& kv : fm)
+        if (!kv.second.empty()) dense(kv.first);
+    for (int k = 0; k < nm; k++) {
+        auto it = fm.find(ids_[k]);
+        if (it != fm.end()) {
+            for (auto& f : it->second) {
+                type.push_back(f.type);
+                const bool pair = f.type == 0 || f.type == 5 || f.type == 6 || f.type == 7;
+                partner.push_back(pair ? dense(f.id2) : -1);
+                for (int j = 0; j < 4; j++) prm.push_back(f.p[j]);
+            }
+        }
+        off[k + 1] = (int32_t)type.size();
+    }
+    check(lbib_set_forces(ctx_, nm, off.data(), type.data(), partner.data(), prm.data()), "set_forces");
+    forceSolver_.dirty_ = false;
+}
+
+unsigned SimulationRunner::remeshBoundary() {
+    std::vector<std::shared_ptr<geometry::Connection>> copy(geo_.getConnections());
+    unsigned counter = 0;
+    for (auto& c : copy) {
+        const double dx = c->first->x - c->second->x, dy = c->first->y - c->second->y;
+        if (std::sqrt(dx * dx + dy * dy) > kMAXLENGTH) {
+            const double x1 = c->first->x, y1 = c->first->y, x2 = c->second->x, y2 = c->second->y;
+            const unsigned nid = geo_.addGeometryNode((x1 + x2) / 2, (y1 + y2) / 2);
+            const unsigned a = c->first->id, b = c->second->id, dom = c->domain;
+            const auto desc = c->descriptor;
+            geo_.eraseConnection(c.get());
+            geo_.addConnection(a, nid, dom, desc);
+            geo_.addConnection(nid, b, dom, desc);
+            counter++;
+        }
+    }
+    return counter;
+}
+
+void SimulationRunner::coarsenBoundary() {
+    auto& nodes = geo_.nodes();
+    for (auto it = nodes.begin(); it != nodes.end();) {
+        const auto& n = *it->second;
+        const auto* p1 = n.in->first;
+        const auto* p2 = n.out->second;
+        const double len = std::sqrt((p2->x - p1->x) * (p2->x - p1->x) + (p2->y - p1->y) * (p2->y - p1->y));
+        const unsigned id = it->first;
+        ++it;
+        if (len < kMAXLENGTH) geo_.removeGeometryNode(id);
+    }
+}
+
+std::string SimulationRunner::describe(const lbib_status& s) {
+    std::string m;
+    if (s.bits & LBIB_ERR_VELOCITY) m += "The magnitude of velocity is too high; ";
+    if (s.bits & LBIB_ERR_EDGE) m += "geometry point moved too close to the boundary; ";
+    if (s.bits & LBIB_ERR_SCANLINE) m += "domainID at the left boundary must be 0.; ";
+    if (s.bits & LBIB_ERR_SLOPE) m += "This can only happen if something went wrong with the perturbation of the connection.; ";
+    if (s.bits & LBIB_ERR_CAPACITY) m += "device work list capacity exceeded; ";
+    if (s.first_bad_marker >= 0 && s.first_bad_marker < (int)ids_.size())
+        m += "GeometryNode ID=" + std::to_string(ids_[s.first_bad_marker]);
+    return m;
+}
+
+void SimulationRunner::handleGeometryStatus() {
+    for (int guard = 0; guard < 16; guard++) {
+        check(lbib_read_status(ctx_, &lastStatus), "status");
+        const uint32_t bits = lastStatus.bits;
+        if (bits & LBIB_ERR_MASK) throw Exception(describe(lastStatus));
+        if (!(bits & LBIB_HALT_MASK)) return;
+        auto t0 = std::chrono::steady_clock::now();
+        hostMarkersFresh_ = false;
+        syncMarkersFromDevice();
+        if (bits & LBIB_HALT_PERTURB_NODE) {
+            // Geometry::moveGeometryNodes calls setPos -> perturbatePosition on
+            // every node in id order; the device moved them already.
+            for (auto& kv : geo_.nodes()) geo_.perturbatePosition(*kv.second);
+        } else {
+            geo_.perturbConnections();
+        }
+        std::vector<double> xy(2 * ids_.size());
+        size_t k = 0;
+        for (auto& kv : geo_.getGeometryNodes()) { xy[2 * k] = kv.second->x; xy[2 * k + 1] = kv.second->y; k++; }
+        // keep velocities and old positions as they are on the device
+        std::vector<double> v(2 * ids_.size()), o(2 * ids_.size());
+        k = 0;
+        for (auto& kv : geo_.getGeometryNodes()) {
+            v[2 * k] = kv.second->vx; v[2 * k + 1] = kv.second->vy;
+            o[2 * k] = kv.second->ox; o[2 * k + 1] = kv.second->oy;
+            k++;
+        }
+        check(lbib_set_markers(ctx_, (int)ids_.size(), xy.data(), v.data(), o.data()), "set_markers");
+        lbib_status keep = lastStatus;
+        check(lbib_clear_status(ctx_), "status");
+        (void)keep;
+        times.host += seconds_since(t0);
+        check(lbib_step_geometry(ctx_, params_.currentIteration, params_.currentIteration % 10 == 0),
+              "moveLattice");
+    }
+    throw Exception("perturbation did not converge");
+}
+
+void SimulationRunner::step() {
+    const unsigned it = params_.currentIteration;
+    auto t0 = std::chrono::steady_clock::now();
+    if (forceSolver_.dirty_) uploadForces();
+    check(lbib_step_lattice(ctx_, it), "lattice step");
+    auto t1 = std::chrono::steady_clock::now();
+    check(lbib_step_geometry(ctx_, it, it % 10 == 0), "moveLattice");
+    hostMarkersFresh_ = false;
+    handleGeometryStatus();
+    auto t2 = std::chrono::steady_clock::now();
+    times.lattice += std::chrono::duration<double>(t1 - t0).count();
+    times.geometry += std::chrono::duration<double>(t2 - t1).count();
+    // remeshBoundary / coarsenBoundary on the host topology (SimulationRunner.cpp:83-87)
+    bool edited = false;
+    if (lastStatus.bits & LBIB_FLAG_REMESH) {
+        syncMarkersFromDevice();
+        while (remeshBoundary()) {}
+        edited = true;
+    }
+    if (it % 10 == 0) {
+        forceSolver_.deleteAllForces();
+        bool need = (lastStatus.bits & LBIB_FLAG_COARSEN) != 0;
+        if (edited) need = true;   // the device check ran before the remesh
+        if (need) {
+            syncMarkersFromDevice();
+            const size_t before = geo_.getGeometryNodes().size();
+            coarsenBoundary();
+            if (geo_.getGeometryNodes().size() != before) edited = true;
+        }
+    }
+    check(lbib_clear_status(ctx_), "status");
+    if (edited) uploadTopology();
+    auto t3 = std::chrono::steady_clock::now();
+    times.host += std::chrono::duration<double>(t3 - t2).count();
+    // mass solvers: MassSolverBoxOutlet is applied inside the collide kernel.
+    for (auto& b : bio_) b->applyBioProcess(*this);
+    times.bio += seconds_since(t3);
+    params_.currentIteration++;
+    times.steps++;
+}
+
+void SimulationRunner::run(unsigned steps) {
+    for (unsigned t = 0; t < steps; t++) step();
+}
+
+void SimulationRunner::runSimulation() { run(params_.iterations); }
+
+std::map<unsigned, double> SimulationRunner::computeAreas() {
+    std::map<unsigned, const geometry::Connection*> start;
+    for (auto& c : geo_.getConnections())
+        if (!start.count(c->domain)) start[c->domain] = c.get();
+    std::vector<double> a(start.size());
+    check(lbib_compute_areas(ctx_, a.data()), "computeAreas");
+    std::map<unsigned, double> out;
+    size_t k = 0;
+    for (auto& kv : start) out[kv.first] = a[k++];
+    return out;
+}
+
+}  // namespace LbmLib

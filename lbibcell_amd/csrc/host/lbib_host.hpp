@@ -1,0 +1,194 @@
+// lbib_host.hpp -- host runtime of the device-resident lattice path, keeping
+// the reference's plugin API: SimulationRunner, Geometry / GeometryHandler,
+// ForceSolver, the CDE / Bio / Mass solver registries by name, and the global
+// Parameters.  Membrane topology (node ids, connection order, remeshing, the
+// seeded perturbation streams) is owned here; every per-site and per-marker
+// computation runs on the GPU through the C ABI of include/lbib.h.
+//
+// Citations are relative to the reference repository root; "LbmLib/" means
+// libs/LbmLib/.
+#pragma once
+#include <array>
+#include <chrono>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <random>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/lbib.h"
+
+namespace LbmLib {
+
+// UtilLib::Exception / lbm_fail (UtilLib/include/Exception.hpp:90): message
+// "file:line: msg".
+struct Exception : std::runtime_error {
+    explicit Exception(const std::string& m) : std::runtime_error(m) {}
+};
+#define LBIB_FAIL(msg) throw ::LbmLib::Exception(std::string(__FILE__) + ":" + std::to_string(__LINE__) + ": " + (msg))
+
+// GlobalSimulationParameters (GlobalSimulationParameters.hpp:35-190).
+struct GlobalSimulationParameters {
+    unsigned currentIteration = 0;
+    unsigned iterations = 0;
+    unsigned reportSteps = 1;
+    unsigned sizeX = 0, sizeY = 0;
+    double tauFluid = 1.0;
+    std::map<std::string, double> cdeSolvers;   // name -> tau (std::map: name order)
+    void loadGlobalSimulationParameters(const std::string& file);   // :34-72
+    void writeGlobalSimulationParameters(const std::string& file) const;
+};
+
+namespace geometry {
+
+struct Connection;
+// GeometryNode (LbmLib/include/nodes/GeometryNode.hpp)
+struct GeometryNode {
+    unsigned id = 0;
+    double x = 0, y = 0, vx = 0, vy = 0, ox = 0, oy = 0;
+    Connection* in = nullptr;    // getConnection<0>
+    Connection* out = nullptr;   // getConnection<1>
+};
+
+// Connection (LbmLib/include/geometry/Connection.hpp:47-155)
+struct Connection {
+    GeometryNode* first = nullptr;
+    GeometryNode* second = nullptr;
+    unsigned domain = 0;
+    std::map<std::string, std::vector<std::string>> descriptor;   // boundary solver -> CDE names
+};
+
+// Geometry (LbmLib/include/geometry/Geometry.hpp:68-240): the membrane store.
+class Geometry {
+ public:
+    Geometry() = default;
+    explicit Geometry(const std::string& txtFile);     // Geometry.cpp:56-72 (.txt only)
+    void loadGeometryTXT(const std::string& file);      // Geometry.cpp:99-160
+    void writeGeometry(const std::string& file) const;  // Geometry.cpp:77-95
+    // GeometryNode ctor + Geometry::addGeometryNode (Geometry.cpp:381-399)
+    unsigned addGeometryNode(unsigned id, double x, double y);
+    unsigned addGeometryNode(double x, double y);
+    void addConnection(unsigned id1, unsigned id2, unsigned domain,
+                       const std::map<std::string, std::vector<std::string>>& descriptor = {});
+    void eraseConnection(Connection* c);                 // Geometry.cpp:471-491
+    void removeGeometryNode(unsigned id);                // Geometry.cpp:401-428
+    bool checkGeometryIntegrity() const;                 // Geometry.cpp:493-524
+    const std::map<unsigned, std::unique_ptr<GeometryNode>>& getGeometryNodes() const { return nodes_; }
+    std::map<unsigned, std::unique_ptr<GeometryNode>>& nodes() { return nodes_; }
+    const std::vector<std::shared_ptr<Connection>>& getConnections() const { return connections_; }
+    std::map<unsigned, unsigned>& cellTypeTrackerMap() { return cellTypes_; }
+    // seeded perturbation streams (GeometryNode.cpp:153-158, Connection.cpp:332-337)
+    void perturbatePosition(GeometryNode& n);
+    void setPos(GeometryNode& n, double x, double y);
+    void perturbConnection(Connection& c);
+    void perturbConnections();                           // GeometryHandler.cpp:326-356
+    long nodePerturbations = 0, connectionPerturbations = 0;
+
+ private:
+    std::map<unsigned, std::unique_ptr<GeometryNode>> nodes_;
+    std::vector<std::shared_ptr<Connection>> connections_;
+    std::map<unsigned, unsigned> cellTypes_{{0u, 0u}};
+    std::mt19937 genNode_{1}, genConn_{1};
+    std::uniform_real_distribution<> disNode_{-10E-7, 10E-7};
+    std::uniform_real_distribution<> disConn_{-10E-7, 10E-7};
+};
+
+}  // namespace geometry
+
+namespace solver {
+// ForceSolver (LbmLib/include/solver/ForceSolver.hpp:43-117)
+struct ForceStruct {
+    int type;
+    unsigned id1, id2;
+    double p[4];
+};
+class ForceSolver {
+ public:
+    void addForce(std::stringstream* descriptor);        // ForceSolver.cpp:107-139
+    void addForce(const ForceStruct& f) { forcemap_[f.id1].push_back(f); dirty_ = true; }
+    void loadForceFile(const std::string& file);         // ForceSolver.cpp:89-105
+    void writeForceSolver(const std::string& file) const;
+    void deleteAllForces() { forcemap_.clear(); dirty_ = true; }
+    void deleteForceType(unsigned type);                 // ForceSolver.cpp:168-204
+    const std::map<unsigned, std::vector<ForceStruct>>& forcemap() const { return forcemap_; }
+    bool dirty_ = true;
+
+ private:
+    std::map<unsigned, std::vector<ForceStruct>> forcemap_;
+};
+}  // namespace solver
+
+class SimulationRunner;
+
+// BioAbstractSolver::applyBioProcess (BioAbstractSolver.hpp:44-73)
+struct BioSolver {
+    virtual ~BioSolver() = default;
+    virtual void applyBioProcess(SimulationRunner& runner) = 0;
+    std::string name;
+};
+
+struct StageTimes {
+    double lattice = 0, geometry = 0, host = 0, bio = 0;
+    long steps = 0;
+};
+
+// SimulationRunner (LbmLib/include/SimulationRunner.hpp:43-231) fused with the
+// GeometryHandler's device-facing duties.
+class SimulationRunner {
+ public:
+    SimulationRunner(const GlobalSimulationParameters& params, geometry::Geometry& geo, int precision = LBIB_FP64,
+                     int forceModel = LBIB_FORCE_LUO);
+    ~SimulationRunner();
+    // CDE solver registry (CDESolverFactory): name -> reaction spec; called
+    // before the context is created, in Parameters::cdeSolvers order.
+    static bool knownCDESolver(const std::string& name);
+    void setSpeciesSpec(const std::string& name, const lbib_species_spec& spec);
+    void setup();                                         // GeometryHandler ctor: cureLattice (:149-170)
+    void initSolvers(const double* u0 = nullptr);         // SimulationRunner.cpp:290-320
+    void initForceSolver(const std::string& file);        // :392-394
+    void addMassSolver(const std::string& name);          // :401-403
+    void addBioSolver(const std::string& name);           // :405-408
+    void addBioSolver(std::unique_ptr<BioSolver> b);
+    void runSimulation();                                 // :69-110 (Parameters.iterations steps)
+    void run(unsigned steps);
+    void step();
+    solver::ForceSolver& getForceSolver() { return forceSolver_; }
+    geometry::Geometry& getGeometry() { return geo_; }
+    lbib_ctx* context() { return ctx_; }
+    GlobalSimulationParameters& parameters() { return params_; }
+    int speciesIndex(const std::string& name) const;
+    // host views (lazy device->host sync)
+    void syncMarkersFromDevice();
+    void uploadTopology();
+    std::map<unsigned, double> computeAreas();            // GeometryHandler.cpp:101-134 (device shoelace)
+    StageTimes times;
+    lbib_status lastStatus{};
+    bool syncEveryStep = true;
+
+ private:
+    void check(int rc, const char* what);
+    void uploadForces();
+    unsigned remeshBoundary();                            // GeometryHandler.cpp:385-431
+    void coarsenBoundary();                               // GeometryHandler.cpp:433-461
+    void handleGeometryStatus();
+    std::string describe(const lbib_status& s);
+
+    GlobalSimulationParameters params_;
+    geometry::Geometry& geo_;
+    int precision_, forceModel_;
+    lbib_ctx* ctx_ = nullptr;
+    std::vector<std::string> speciesNames_;
+    std::map<std::string, lbib_species_spec> specs_;
+    solver::ForceSolver forceSolver_;
+    std::vector<std::unique_ptr<BioSolver>> bio_;
+    std::vector<std::string> massSolvers_;
+    std::vector<unsigned> ids_;                 // dense index -> node id (ascending)
+    std::vector<std::vector<int>> bcDescriptors_;
+    bool topologyDirty_ = true;
+    bool hostMarkersFresh_ = true;
+};
+
+}  // namespace LbmLib

@@ -1,0 +1,260 @@
+// lbib_host_capi.cpp -- flat C entry points over the host runtime, used by the
+// Python tests and bench (ctypes) and by any non-C++ caller.  They follow the
+// reference's main() sequence (apps/tutorial_01.cpp:59-145): parameters,
+// geometry, GeometryHandler (setup), SimulationRunner, initSolvers,
+// initForceSolver, addMassSolver / addBioSolver, runSimulation.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <typeinfo>
+
+#include "lbib_host.hpp"
+
+using namespace LbmLib;
+
+namespace {
+struct Handle {
+    GlobalSimulationParameters params;
+    geometry::Geometry geo;
+    std::unique_ptr<SimulationRunner> runner;
+    std::map<std::string, lbib_species_spec> specs;
+    int precision = LBIB_FP64, forceModel = LBIB_FORCE_LUO;
+    std::string err;
+};
+
+template <typename F>
+int guarded(Handle* h, F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::bad_typeid&) {
+        h->err = "std::bad_typeid: unknown plugin name";
+    } catch (const std::exception& e) {
+        h->err = e.what();
+    }
+    return -1;
+}
+}  // namespace
+
+extern "C" {
+
+void* lbh_create(int nx, int ny, double tauf, int force_model, int precision) {
+    auto* h = new Handle();
+    h->params.sizeX = (unsigned)nx;
+    h->params.sizeY = (unsigned)ny;
+    h->params.tauFluid = tauf;
+    h->forceModel = force_model;
+    h->precision = precision;
+    return h;
+}
+void lbh_destroy(void* p) { delete static_cast<Handle*>(p); }
+const char* lbh_last_error(void* p) { return static_cast<Handle*>(p)->err.c_str(); }
+
+// Parameters "CDESolvers: name tau" plus an optional reaction spec for the name.
+int lbh_add_species(void* p, const char* name, const lbib_species_spec* spec) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (h->runner) LBIB_FAIL("species must be added before setup");
+        h->params.cdeSolvers[name] = spec->tau;
+        h->specs[name] = *spec;
+    });
+}
+int lbh_load_parameters(void* p, const char* file) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->params.loadGlobalSimulationParameters(file); });
+}
+int lbh_load_geometry_txt(void* p, const char* file) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->geo.loadGeometryTXT(file); });
+}
+int lbh_add_node(void* p, unsigned id, double x, double y) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->geo.addGeometryNode(id, x, y); });
+}
+// bc: comma-separated CDE names for a BoundarySolverNoFluxD2Q5 descriptor ("" = none)
+int lbh_add_connection(void* p, unsigned id1, unsigned id2, unsigned dom, const char* bc) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        std::map<std::string, std::vector<std::string>> d;
+        std::string s = bc ? bc : "";
+        size_t pos = 0;
+        while (!s.empty() && pos <= s.size()) {
+            size_t q = s.find(',', pos);
+            if (q == std::string::npos) q = s.size();
+            if (q > pos) d["BoundarySolverNoFluxD2Q5"].push_back(s.substr(pos, q - pos));
+            pos = q + 1;
+        }
+        h->geo.addConnection(id1, id2, dom, d);
+    });
+}
+int lbh_set_celltype(void* p, unsigned dom, unsigned type) {
+    auto* h = static_cast<Handle*>(p);
+    h->geo.cellTypeTrackerMap()[dom] = type;
+    return 0;
+}
+int lbh_setup(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        h->runner.reset(new SimulationRunner(h->params, h->geo, h->precision, h->forceModel));
+        for (auto& kv : h->specs) h->runner->setSpeciesSpec(kv.first, kv.second);
+        h->runner->setup();
+    });
+}
+int lbh_init_solvers(void* p, const double* u0) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->runner->initSolvers(u0); });
+}
+int lbh_add_force(void* p, int type, unsigned id1, unsigned id2, double a, double b, double c, double d) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        solver::ForceStruct f{type, id1, id2, {a, b, c, d}};
+        h->runner->getForceSolver().addForce(f);
+    });
+}
+int lbh_load_force_file(void* p, const char* file) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->runner->initForceSolver(file); });
+}
+int lbh_add_mass_solver(void* p, const char* name) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->runner->addMassSolver(name); });
+}
+int lbh_add_bio_solver(void* p, const char* name) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->runner->addBioSolver(name); });
+}
+// Parameterised tutorial BioSolvers: kind 0 tension (prm[0] = force), kind 1
+// area regulator (target, kp, maxsource, target factor).
+int lbh_add_bio_solver_params(void* p, int kind, const double* prm, unsigned freq);
+int lbh_run(void* p, unsigned steps) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->runner->run(steps); });
+}
+int lbh_sync(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (lbib_synchronize(h->runner->context()) != 0) throw Exception(lbib_last_error(h->runner->context()));
+    });
+}
+void* lbh_context(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    return h->runner ? h->runner->context() : nullptr;
+}
+unsigned lbh_iteration(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    return h->runner ? h->runner->parameters().currentIteration : h->params.currentIteration;
+}
+int lbh_set_iteration(void* p, unsigned it) {
+    auto* h = static_cast<Handle*>(p);
+    if (h->runner) h->runner->parameters().currentIteration = it;
+    else h->params.currentIteration = it;
+    return 0;
+}
+int lbh_num_markers(void* p) { return (int)static_cast<Handle*>(p)->geo.getGeometryNodes().size(); }
+int lbh_num_connections(void* p) { return (int)static_cast<Handle*>(p)->geo.getConnections().size(); }
+int lbh_get_markers(void* p, unsigned* ids, double* xy, double* v) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (h->runner) h->runner->syncMarkersFromDevice();
+        size_t k = 0;
+        for (auto& kv : h->geo.getGeometryNodes()) {
+            if (ids) ids[k] = kv.first;
+            if (xy) { xy[2 * k] = kv.second->x; xy[2 * k + 1] = kv.second->y; }
+            if (v) { v[2 * k] = kv.second->vx; v[2 * k + 1] = kv.second->vy; }
+            k++;
+        }
+    });
+}
+int lbh_get_connections(void* p, unsigned* a, unsigned* b, unsigned* dom) {
+    auto* h = static_cast<Handle*>(p);
+    size_t k = 0;
+    for (auto& c : h->geo.getConnections()) {
+        a[k] = c->first->id;
+        b[k] = c->second->id;
+        dom[k] = c->domain;
+        k++;
+    }
+    return 0;
+}
+long lbh_perturb_counts(void* p, int which) {
+    auto* h = static_cast<Handle*>(p);
+    return which == 0 ? h->geo.nodePerturbations : h->geo.connectionPerturbations;
+}
+// lattice, geometry, host, bio seconds and step count
+void lbh_times(void* p, double* out) {
+    auto* h = static_cast<Handle*>(p);
+    const auto& t = h->runner->times;
+    out[0] = t.lattice; out[1] = t.geometry; out[2] = t.host; out[3] = t.bio; out[4] = (double)t.steps;
+}
+void lbh_reset_times(void* p) { static_cast<Handle*>(p)->runner->times = StageTimes{}; }
+int lbh_last_status(void* p, lbib_status* st) {
+    *st = static_cast<Handle*>(p)->runner->lastStatus;
+    return 0;
+}
+int lbh_num_areas(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    std::map<unsigned, int> d;
+    for (auto& c : h->geo.getConnections()) d[c->domain] = 1;
+    return (int)d.size();
+}
+int lbh_get_areas(void* p, unsigned* dom, double* area) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        auto a = h->runner->computeAreas();
+        size_t k = 0;
+        for (auto& kv : a) { dom[k] = kv.first; area[k] = kv.second; k++; }
+    });
+}
+
+}  // extern "C"
+
+namespace {
+struct TensionP : BioSolver {
+    unsigned freq;
+    double force;
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        auto& fs = r.getForceSolver();
+        fs.deleteForceType(6);
+        for (auto& c : r.getGeometry().getConnections()) {
+            std::stringstream d;
+            d << "6\t" << c->first->id << "\t" << c->second->id << "\t" << force;
+            fs.addForce(&d);
+            std::stringstream e;
+            e << "6\t" << c->second->id << "\t" << c->first->id << "\t" << force;
+            fs.addForce(&e);
+        }
+    }
+};
+struct AreaP : BioSolver {
+    unsigned freq;
+    double prm[4];
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        if (lbib_area_regulator(r.context(), prm[0], prm[1], prm[2], prm[3]) != 0)
+            throw Exception(lbib_last_error(r.context()));
+    }
+};
+}  // namespace
+
+extern "C" int lbh_add_bio_solver_params(void* p, int kind, const double* prm, unsigned freq) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (freq == 0) LBIB_FAIL("frequency must be positive");
+        if (kind == 0) {
+            std::unique_ptr<TensionP> b(new TensionP());
+            b->freq = freq;
+            b->force = prm[0];
+            b->name = "tutorial_01_BioSolverMembraneTension";
+            h->runner->addBioSolver(std::move(b));
+        } else if (kind == 1) {
+            std::unique_ptr<AreaP> b(new AreaP());
+            b->freq = freq;
+            for (int k = 0; k < 4; k++) b->prm[k] = prm[k];
+            b->name = "tutorial_01_BioSolverAreaRegulator";
+            h->runner->addBioSolver(std::move(b));
+        } else {
+            throw std::bad_typeid();
+        }
+    });
+}

@@ -1,0 +1,200 @@
+"""ctypes binding of the C++ host runtime (liblbib_host.so): the reference's
+SimulationRunner sequence (apps/tutorial_01.cpp:59-145) driven from Python."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import capi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+HOST_LIB = os.path.join(_HERE, "liblbib_host.so")
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_U32 = C.POINTER(C.c_uint32)
+_SIG = {
+    "lbh_create": (_P, [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int]),
+    "lbh_destroy": (None, [_P]),
+    "lbh_last_error": (C.c_char_p, [_P]),
+    "lbh_add_species": (C.c_int, [_P, C.c_char_p, C.POINTER(capi.SpeciesSpec)]),
+    "lbh_load_parameters": (C.c_int, [_P, C.c_char_p]),
+    "lbh_load_geometry_txt": (C.c_int, [_P, C.c_char_p]),
+    "lbh_add_node": (C.c_int, [_P, C.c_uint, C.c_double, C.c_double]),
+    "lbh_add_connection": (C.c_int, [_P, C.c_uint, C.c_uint, C.c_uint, C.c_char_p]),
+    "lbh_set_celltype": (C.c_int, [_P, C.c_uint, C.c_uint]),
+    "lbh_setup": (C.c_int, [_P]),
+    "lbh_init_solvers": (C.c_int, [_P, _D]),
+    "lbh_add_force": (C.c_int, [_P, C.c_int, C.c_uint, C.c_uint, C.c_double, C.c_double, C.c_double, C.c_double]),
+    "lbh_load_force_file": (C.c_int, [_P, C.c_char_p]),
+    "lbh_add_mass_solver": (C.c_int, [_P, C.c_char_p]),
+    "lbh_add_bio_solver": (C.c_int, [_P, C.c_char_p]),
+    "lbh_add_bio_solver_params": (C.c_int, [_P, C.c_int, _D, C.c_uint]),
+    "lbh_run": (C.c_int, [_P, C.c_uint]),
+    "lbh_sync": (C.c_int, [_P]),
+    "lbh_context": (_P, [_P]),
+    "lbh_iteration": (C.c_uint, [_P]),
+    "lbh_set_iteration": (C.c_int, [_P, C.c_uint]),
+    "lbh_num_markers": (C.c_int, [_P]),
+    "lbh_num_connections": (C.c_int, [_P]),
+    "lbh_get_markers": (C.c_int, [_P, _U32, _D, _D]),
+    "lbh_get_connections": (C.c_int, [_P, _U32, _U32, _U32]),
+    "lbh_perturb_counts": (C.c_long, [_P, C.c_int]),
+    "lbh_times": (None, [_P, _D]),
+    "lbh_reset_times": (None, [_P]),
+    "lbh_last_status": (C.c_int, [_P, C.POINTER(capi.Status)]),
+    "lbh_num_areas": (C.c_int, [_P]),
+    "lbh_get_areas": (C.c_int, [_P, _U32, _D]),
+}
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        capi.load()
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(HOST_LIB)
+        for k, (r, a) in _SIG.items():
+            f = getattr(L, k)
+            f.restype = r
+            f.argtypes = a
+        _lib = L
+    return _lib
+
+
+class Simulation:
+    """One coupled simulation: geometry + lattice + plugins, stepped on the GPU."""
+
+    def __init__(self, nx, ny, tau_fluid, force_model=capi.FORCE_LUO, precision=capi.FP64):
+        self.L = load()
+        self.h = self.L.lbh_create(nx, ny, tau_fluid, force_model, precision)
+        self.nx, self.ny = nx, ny
+        self._ctx = None
+
+    def close(self):
+        if self.h:
+            self.L.lbh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise capi.LbibError(self.L.lbh_last_error(self.h).decode())
+
+    def add_species(self, name, spec):
+        self._chk(self.L.lbh_add_species(self.h, name.encode(), C.byref(spec)))
+
+    def add_node(self, nid, x, y):
+        self._chk(self.L.lbh_add_node(self.h, nid, x, y))
+
+    def add_connection(self, a, b, dom, bc=""):
+        self._chk(self.L.lbh_add_connection(self.h, a, b, dom, bc.encode()))
+
+    def add_geometry(self, nodes, conns, bc=""):
+        for nid, x, y in nodes:
+            self.add_node(nid, x, y)
+        for a, b, d in conns:
+            self.add_connection(a, b, d, bc)
+
+    def load_geometry_txt(self, path):
+        self._chk(self.L.lbh_load_geometry_txt(self.h, path.encode()))
+
+    def set_celltype(self, dom, t):
+        self._chk(self.L.lbh_set_celltype(self.h, dom, t))
+
+    def setup(self):
+        self._chk(self.L.lbh_setup(self.h))
+
+    def init_solvers(self, u0=None):
+        self._chk(self.L.lbh_init_solvers(self.h, capi.dptr(u0) if u0 is not None else None))
+
+    def add_force(self, t, id1, id2=0, a=0.0, b=0.0, c=0.0, d=0.0):
+        self._chk(self.L.lbh_add_force(self.h, t, id1, id2, a, b, c, d))
+
+    def add_mass_solver(self, name):
+        self._chk(self.L.lbh_add_mass_solver(self.h, name.encode()))
+
+    def add_bio_solver(self, name):
+        self._chk(self.L.lbh_add_bio_solver(self.h, name.encode()))
+
+    def add_tension(self, force=0.01, freq=10):
+        p = np.array([force, 0, 0, 0], dtype=np.float64)
+        self._chk(self.L.lbh_add_bio_solver_params(self.h, 0, capi.dptr(p), freq))
+
+    def add_area_regulator(self, target=200.0, kp=1e-5, maxsource=0.004, factor=0.0, freq=1):
+        p = np.array([target, kp, maxsource, factor], dtype=np.float64)
+        self._chk(self.L.lbh_add_bio_solver_params(self.h, 1, capi.dptr(p), freq))
+
+    def run(self, steps):
+        self._chk(self.L.lbh_run(self.h, steps))
+
+    def sync(self):
+        self._chk(self.L.lbh_sync(self.h))
+
+    @property
+    def iteration(self):
+        return self.L.lbh_iteration(self.h)
+
+    def set_iteration(self, it):
+        self.L.lbh_set_iteration(self.h, it)
+
+    def context(self):
+        """A capi.Context view of the device state (not owning)."""
+        if self._ctx is None:
+            ptr = self.L.lbh_context(self.h)
+            ctx = capi.Context.__new__(capi.Context)
+            ctx.lib = capi.load()
+            ctx.h = C.c_void_p(ptr)
+            ctx.nx, ctx.ny, ctx.n = self.nx, self.ny, self.nx * self.ny
+            ctx.ns = 0
+            ctx.close = lambda: None
+            self._ctx = ctx
+        return self._ctx
+
+    def markers(self):
+        m = self.L.lbh_num_markers(self.h)
+        ids = np.empty(m, dtype=np.uint32)
+        xy = np.empty((m, 2))
+        v = np.empty((m, 2))
+        self._chk(self.L.lbh_get_markers(self.h, capi.u32ptr(ids), capi.dptr(xy), capi.dptr(v)))
+        return ids, xy, v
+
+    def connections(self):
+        c = self.L.lbh_num_connections(self.h)
+        a = np.empty(c, dtype=np.uint32)
+        b = np.empty(c, dtype=np.uint32)
+        d = np.empty(c, dtype=np.uint32)
+        self.L.lbh_get_connections(self.h, capi.u32ptr(a), capi.u32ptr(b), capi.u32ptr(d))
+        return a, b, d
+
+    def perturb_counts(self):
+        return self.L.lbh_perturb_counts(self.h, 0), self.L.lbh_perturb_counts(self.h, 1)
+
+    def times(self):
+        out = np.zeros(5)
+        self.L.lbh_times(self.h, capi.dptr(out))
+        return dict(lattice=out[0], geometry=out[1], host=out[2], bio=out[3], steps=int(out[4]))
+
+    def reset_times(self):
+        self.L.lbh_reset_times(self.h)
+
+    def last_status(self):
+        s = capi.Status()
+        self.L.lbh_last_status(self.h, C.byref(s))
+        return s
+
+    def areas(self):
+        k = self.L.lbh_num_areas(self.h)
+        d = np.empty(k, dtype=np.uint32)
+        a = np.empty(k)
+        self._chk(self.L.lbh_get_areas(self.h, capi.u32ptr(d), capi.dptr(a)))
+        return dict(zip(d.tolist(), a.tolist()))

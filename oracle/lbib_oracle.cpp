@@ -127,7 +127,7 @@ struct BNode {                        // BoundaryNode.cpp:35-69
     double vx = 0, vy = 0;
     long site = -1;                   // physicalNeighbour_
     int dir = 0;                      // directionToPhysicalNeighbour_
-    const Conn* conn = nullptr;       // source connection (descriptor)
+    std::vector<int> bc_species;      // boundary solvers built from the descriptor (BoundaryNode.cpp:43-46)
     double dist = 0;                  // BC scratch (BoundarySolverNoFluxD2Q5 distribution_)
     bool alive = true;
 };
@@ -298,7 +298,7 @@ static void insertBN(Sim& s, const Conn& c, BNode nb, long site, int d) {
     const double px = (double)(site % s.nx), py = (double)(site / s.nx);
     if (old < 0 || sqd(px, py, nb.x, nb.y) < sqd(px, py, s.bnodes[old].x, s.bnodes[old].y)) {
         bnVelocity(c, nb);
-        nb.site = site; nb.dir = d; nb.conn = &c;
+        nb.site = site; nb.dir = d; nb.bc_species = c.bc_species;
         if (old >= 0) s.bnodes[old].alive = false;
         s.bnodes.push_back(nb);
         s.bn[site][slot] = (long)s.bnodes.size() - 1;   // setPhysicalNeighbours
@@ -578,8 +578,8 @@ static void advect(Sim& s) {
 // only the first species of the descriptor gets a value (the loop returns).
 static void preAdvect(Sim& s) {
     for (auto& b : s.bnodes) {
-        if (!b.alive || b.conn->bc_species.empty()) continue;
-        const int k = b.conn->bc_species[0];
+        if (!b.alive || b.bc_species.empty()) continue;
+        const int k = b.bc_species[0];
         const auto& d = s.g[k][b.site];
         const double C = cOfSlots(d);
         const double Cf = C / 6.0;
@@ -601,8 +601,8 @@ static void preAdvect(Sim& s) {
 // BoundarySolverNoFluxD2Q5::postAdvect (:37-47): swaps into every listed species.
 static void postAdvect(Sim& s) {
     for (auto& b : s.bnodes) {
-        if (!b.alive || b.conn->bc_species.empty()) continue;
-        for (int k : b.conn->bc_species) std::swap(b.dist, s.g[k][b.site][b.dir]);
+        if (!b.alive || b.bc_species.empty()) continue;
+        for (int k : b.bc_species) std::swap(b.dist, s.g[k][b.site][b.dir]);
     }
 }
 
