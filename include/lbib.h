@@ -192,6 +192,15 @@ int lbib_area_regulator(lbib_ctx* ctx, double target_area, double kp, double max
 /* FluidSolver::addMass for every site of each domain: m_site = table[dom]. */
 int lbib_add_mass_by_domain(lbib_ctx* ctx, int ndom, const double* table);
 
+/* ---- measurement ------------------------------------------------------------
+ * No counterpart in the reference (it has no profiling): HIP events recorded
+ * on the context's stream around every launch of the fused lattice kernel.
+ * lbib_profile_read synchronises, returns the summed kernel time, the launch
+ * count and the algorithmic bytes of one launch (population read + write at
+ * storage precision: (2*9 + 2*5*nspecies) * bytes * nx * ny), and resets. */
+int lbib_profile_enable(lbib_ctx* ctx, int on);
+int lbib_profile_read(lbib_ctx* ctx, double* kernel_ms, int64_t* launches, double* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif

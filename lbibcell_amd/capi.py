@@ -112,6 +112,8 @@ SIGNATURES = {
     "lbib_compute_areas": (C.c_int, [_P, _D]),
     "lbib_area_regulator": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double]),
     "lbib_add_mass_by_domain": (C.c_int, [_P, C.c_int, _D]),
+    "lbib_profile_enable": (C.c_int, [_P, C.c_int]),
+    "lbib_profile_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _D]),
 }
 
 
@@ -269,6 +271,17 @@ class Context:
         s = Status()
         self._chk(self.lib.lbib_read_status(self.h, C.byref(s)))
         return s
+
+    def profile_enable(self, on=True):
+        self._chk(self.lib.lbib_profile_enable(self.h, 1 if on else 0))
+
+    def profile_read(self):
+        """(summed lattice-kernel ms, launches, algorithmic bytes per launch)"""
+        ms = C.c_double()
+        n = C.c_int64()
+        b = C.c_double()
+        self._chk(self.lib.lbib_profile_read(self.h, C.byref(ms), C.byref(n), C.byref(b)))
+        return ms.value, n.value, b.value
 
     def device_bytes(self):
         return int(self.lib.lbib_device_bytes(self.h))

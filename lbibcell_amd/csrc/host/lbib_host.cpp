@@ -7,6 +7,7 @@
 #include <fstream>
 #include <iostream>
 #include <unordered_map>
+#include <unordered_set>
 
 namespace LbmLib {
 
@@ -181,6 +182,87 @@ void Geometry::removeGeometryNode(unsigned id) {
     eraseConnection(c2);
     nodes_.erase(it);
     addConnection(n1->id, n2->id, dom, descriptor);
+}
+
+unsigned Geometry::remeshBoundary(double maxLength) {
+    // Each long connection is erased (order of the others kept) and its two
+    // halves appended in visiting order; a survivor list does it in one pass.
+    std::vector<std::shared_ptr<Connection>> copy(connections_), keep, added;
+    keep.reserve(copy.size());
+    unsigned counter = 0;
+    for (auto& c : copy) {
+        const double dx = c->first->x - c->second->x, dy = c->first->y - c->second->y;
+        if (std::sqrt(dx * dx + dy * dy) > maxLength) {
+            const double x1 = c->first->x, y1 = c->first->y, x2 = c->second->x, y2 = c->second->y;
+            const unsigned nid = addGeometryNode((x1 + x2) / 2, (y1 + y2) / 2);
+            GeometryNode* m = nodes_.at(nid).get();
+            GeometryNode* a = c->first;
+            GeometryNode* b = c->second;
+            if (a->out == c.get()) a->out = nullptr;
+            if (b->in == c.get()) b->in = nullptr;
+            for (int h = 0; h < 2; h++) {
+                auto n = std::make_shared<Connection>();
+                n->first = h == 0 ? a : m;
+                n->second = h == 0 ? m : b;
+                n->domain = c->domain;
+                n->descriptor = c->descriptor;
+                n->first->out = n.get();
+                n->second->in = n.get();
+                added.push_back(n);
+            }
+            counter++;
+        } else {
+            keep.push_back(c);
+        }
+    }
+    if (counter) {
+        keep.insert(keep.end(), added.begin(), added.end());
+        connections_.swap(keep);
+    }
+    return counter;
+}
+
+unsigned Geometry::coarsenBoundary(double maxLength) {
+    std::vector<std::shared_ptr<Connection>> all(connections_);
+    std::unordered_map<const Connection*, bool> dead;
+    unsigned removed = 0;
+    for (auto it = nodes_.begin(); it != nodes_.end();) {
+        GeometryNode& n = *it->second;
+        const GeometryNode* p1 = n.in->first;
+        const GeometryNode* p2 = n.out->second;
+        const double len = std::sqrt((p2->x - p1->x) * (p2->x - p1->x) + (p2->y - p1->y) * (p2->y - p1->y));
+        if (len < maxLength) {
+            Connection* c1 = n.in;
+            Connection* c2 = n.out;
+            GeometryNode* n1 = c1->first;
+            GeometryNode* n2 = c2->second;
+            auto nc = std::make_shared<Connection>();
+            nc->first = n1;
+            nc->second = n2;
+            nc->domain = c1->domain;
+            nc->descriptor = c1->descriptor;
+            for (Connection* c : {c1, c2}) {
+                if (c->first->out == c) c->first->out = nullptr;
+                if (c->second->in == c) c->second->in = nullptr;
+                dead[c] = true;
+            }
+            it = nodes_.erase(it);
+            n1->out = nc.get();
+            n2->in = nc.get();
+            all.push_back(nc);
+            removed++;
+        } else {
+            ++it;
+        }
+    }
+    if (removed) {
+        std::vector<std::shared_ptr<Connection>> out;
+        out.reserve(all.size());
+        for (auto& c : all)
+            if (!dead.count(c.get())) out.push_back(c);
+        connections_.swap(out);
+    }
+    return removed;
 }
 
 bool Geometry::checkGeometryIntegrity() const {
@@ -573,37 +655,9 @@ void SimulationRunner::uploadForces() {
     forceSolver_.dirty_ = false;
 }
 
-unsigned SimulationRunner::remeshBoundary() {
-    std::vector<std::shared_ptr<geometry::Connection>> copy(geo_.getConnections());
-    unsigned counter = 0;
-    for (auto& c : copy) {
-        const double dx = c->first->x - c->second->x, dy = c->first->y - c->second->y;
-        if (std::sqrt(dx * dx + dy * dy) > kMAXLENGTH) {
-            const double x1 = c->first->x, y1 = c->first->y, x2 = c->second->x, y2 = c->second->y;
-            const unsigned nid = geo_.addGeometryNode((x1 + x2) / 2, (y1 + y2) / 2);
-            const unsigned a = c->first->id, b = c->second->id, dom = c->domain;
-            const auto desc = c->descriptor;
-            geo_.eraseConnection(c.get());
-            geo_.addConnection(a, nid, dom, desc);
-            geo_.addConnection(nid, b, dom, desc);
-            counter++;
-        }
-    }
-    return counter;
-}
+unsigned SimulationRunner::remeshBoundary() { return geo_.remeshBoundary(kMAXLENGTH); }
 
-void SimulationRunner::coarsenBoundary() {
-    auto& nodes = geo_.nodes();
-    for (auto it = nodes.begin(); it != nodes.end();) {
-        const auto& n = *it->second;
-        const auto* p1 = n.in->first;
-        const auto* p2 = n.out->second;
-        const double len = std::sqrt((p2->x - p1->x) * (p2->x - p1->x) + (p2->y - p1->y) * (p2->y - p1->y));
-        const unsigned id = it->first;
-        ++it;
-        if (len < kMAXLENGTH) geo_.removeGeometryNode(id);
-    }
-}
+void SimulationRunner::coarsenBoundary() { geo_.coarsenBoundary(kMAXLENGTH); }
 
 std::string SimulationRunner::describe(const lbib_status& s) {
     std::string m;

@@ -76,6 +76,10 @@ class Geometry {
     void eraseConnection(Connection* c);                 // Geometry.cpp:471-491
     void removeGeometryNode(unsigned id);                // Geometry.cpp:401-428
     bool checkGeometryIntegrity() const;                 // Geometry.cpp:493-524
+    // GeometryHandler::remeshBoundary / coarsenBoundary (GeometryHandler.cpp:385-461),
+    // one pass each, same resulting node ids and connection order.
+    unsigned remeshBoundary(double maxLength);
+    unsigned coarsenBoundary(double maxLength);
     const std::map<unsigned, std::unique_ptr<GeometryNode>>& getGeometryNodes() const { return nodes_; }
     std::map<unsigned, std::unique_ptr<GeometryNode>>& nodes() { return nodes_; }
     const std::vector<std::shared_ptr<Connection>>& getConnections() const { return connections_; }

@@ -221,6 +221,7 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->ring_dom); dev_free(c->ring_off); dev_free(c->ring_marker); dev_free(c->area);
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -480,8 +481,26 @@ int lbib_set_forces(lbib_ctx* c, int nm, const int32_t* off, const int32_t* type
 }
 
 // ------------------------------------------------------------------- step --
+static int next_event(lbib_ctx* c, hipEvent_t* ev) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        LBIB_HIP(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    *ev = c->ev_pool[c->ev_used++];
+    return 0;
+}
+
 static int stream_collide(lbib_ctx* c, uint32_t it, bool ftile) {
-    return c->prec == LBIB_FP64 ? launch_stream_collide_f64(c, it, ftile) : launch_stream_collide_mixed(c, it, ftile);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->prof_on) {
+        if (next_event(c, &e0) || next_event(c, &e1)) return -1;
+        LBIB_HIP(hipEventRecord(e0, c->stream));
+    }
+    const int r = c->prec == LBIB_FP64 ? launch_stream_collide_f64(c, it, ftile) : launch_stream_collide_mixed(c, it, ftile);
+    if (r) return r;
+    if (c->prof_on) LBIB_HIP(hipEventRecord(e1, c->stream));
+    return 0;
 }
 
 int lbib_step_lattice(lbib_ctx* c, uint32_t it) {
@@ -503,6 +522,26 @@ int lbib_step_geometry(lbib_ctx* c, uint32_t it, int check_coarsen) {
 int lbib_cure_lattice(lbib_ctx* c) {
     if (check_ctx(c)) return -1;
     return launch_geometry(c, 0, 1);
+}
+int lbib_profile_enable(lbib_ctx* c, int on) {
+    if (check_ctx(c)) return -1;
+    c->prof_on = on != 0;
+    return 0;
+}
+int lbib_profile_read(lbib_ctx* c, double* kernel_ms, int64_t* launches, double* bytes_per_launch) {
+    if (check_ctx(c)) return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    double ms = 0.0;
+    for (size_t k = 0; k + 1 < c->ev_used; k += 2) {
+        float t = 0.0f;
+        LBIB_HIP(hipEventElapsedTime(&t, c->ev_pool[k], c->ev_pool[k + 1]));
+        ms += t;
+    }
+    if (kernel_ms) *kernel_ms = ms;
+    if (launches) *launches = (int64_t)(c->ev_used / 2);
+    if (bytes_per_launch) *bytes_per_launch = (double)(2 * 9 + 2 * 5 * c->ns) * (double)storage_bytes(c->prec) * (double)c->n;
+    c->ev_used = 0;
+    return 0;
 }
 int lbib_set_box_outlet(lbib_ctx* c, int on) {
     if (check_ctx(c)) return -1;

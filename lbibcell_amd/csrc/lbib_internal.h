@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
+#include <vector>
 #include "../../include/lbib.h"
 
 namespace lbib {
@@ -127,6 +128,10 @@ struct lbib_ctx {
     double* target = nullptr;
     double* mtab = nullptr;
     bool mtab_live = false;
+    // measurement: event pairs around lattice-kernel launches
+    bool prof_on = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
     // status
     lbib::DevStatus* st = nullptr;
     lbib::DevStatus* st_host = nullptr;   // pinned

@@ -764,16 +764,14 @@ static void addConnection(Sim& s, Marker* a, Marker* b, unsigned dom, const std:
     a->c1 = c.get();
     b->c0 = c.get();
 }
-static void eraseConnection(Sim& s, Conn* c) {        // Geometry.cpp:471-491
-    if (c->a->c1 == c) c->a->c1 = nullptr;
-    if (c->b->c0 == c) c->b->c0 = nullptr;
-    s.conns.erase(std::remove_if(s.conns.begin(), s.conns.end(),
-                                 [c](const std::shared_ptr<Conn>& p) { return p.get() == c; }),
-                  s.conns.end());
-}
-// GeometryHandler::remeshBoundary (:385-431)
+// GeometryHandler::remeshBoundary (:385-431).  Each long connection is erased
+// from connections_ (order of the others kept) and two halves are appended, in
+// the order the copy is visited; one pass with a survivor list gives the same
+// vector without the reference's per-erase scans.
 static unsigned remeshBoundary(Sim& s) {
     std::vector<std::shared_ptr<Conn>> copy(s.conns);
+    std::vector<std::shared_ptr<Conn>> keep, added;
+    keep.reserve(copy.size());
     unsigned counter = 0;
     for (auto& c : copy) {
         const double len = std::sqrt((c->a->x - c->b->x) * (c->a->x - c->b->x) +
@@ -783,19 +781,34 @@ static unsigned remeshBoundary(Sim& s) {
             const unsigned nid = s.markers.rbegin()->first + 1;
             Marker* nm = addNode(s, nid, (x1 + x2) / 2, (y1 + y2) / 2);
             Marker* a = c->a; Marker* b = c->b;
-            const unsigned dom = c->domain;
-            const std::vector<int> bc = c->bc_species;
-            eraseConnection(s, c.get());
-            addConnection(s, a, nm, dom, bc);
-            addConnection(s, nm, b, dom, bc);
+            if (a->c1 == c.get()) a->c1 = nullptr;   // Geometry::eraseConnection
+            if (b->c0 == c.get()) b->c0 = nullptr;
+            for (int h = 0; h < 2; h++) {
+                auto n = std::make_shared<Conn>();   // Geometry::addConnection
+                n->a = h == 0 ? a : nm; n->b = h == 0 ? nm : b;
+                n->domain = c->domain; n->bc_species = c->bc_species;
+                n->a->c1 = n.get(); n->b->c0 = n.get();
+                added.push_back(n);
+            }
             counter++;
+        } else {
+            keep.push_back(c);
         }
+    }
+    if (counter) {
+        keep.insert(keep.end(), added.begin(), added.end());
+        s.conns.swap(keep);
     }
     connectAll(s);
     return counter;
 }
-// GeometryHandler::coarsenBoundary (:433-461) with Geometry::removeGeometryNode (Geometry.cpp:401-428)
+// GeometryHandler::coarsenBoundary (:433-461) with Geometry::removeGeometryNode
+// (Geometry.cpp:401-428): erased connections are flagged and compacted at the
+// end, appended ones keep their creation order.
 static void coarsenBoundary(Sim& s) {
+    std::vector<std::shared_ptr<Conn>> all(s.conns);
+    std::map<const Conn*, bool> dead;
+    bool any = false;
     for (auto it = s.markers.begin(); it != s.markers.end();) {
         Marker& m = *it->second;
         const Marker& P1 = *m.c0->a;
@@ -806,13 +819,27 @@ static void coarsenBoundary(Sim& s) {
             Marker* N1 = C1->a; Marker* N2 = C2->b;
             const std::vector<int> bc = C1->bc_species;
             const unsigned dom = C1->domain;     // getDomainIdOfAdjacentConnections
-            eraseConnection(s, C1);
-            eraseConnection(s, C2);
+            for (Conn* c : {C1, C2}) {
+                if (c->a->c1 == c) c->a->c1 = nullptr;
+                if (c->b->c0 == c) c->b->c0 = nullptr;
+                dead[c] = true;
+            }
             it = s.markers.erase(it);
-            addConnection(s, N1, N2, dom, bc);
+            auto n = std::make_shared<Conn>();
+            n->a = N1; n->b = N2; n->domain = dom; n->bc_species = bc;
+            N1->c1 = n.get(); N2->c0 = n.get();
+            all.push_back(n);
+            any = true;
         } else {
             ++it;
         }
+    }
+    if (any) {
+        std::vector<std::shared_ptr<Conn>> out;
+        out.reserve(all.size());
+        for (auto& c : all)
+            if (!dead.count(c.get())) out.push_back(c);
+        s.conns.swap(out);
     }
 }
 
