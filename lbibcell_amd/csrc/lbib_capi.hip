@@ -181,7 +181,7 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     if (dev_alloc(c, &c->best_dist, 4 * n, 0xff) || dev_alloc(c, &c->best_seq, 4 * n, 0xff) ||
         dev_alloc(c, &c->bn_dom, 4 * n))
         return -1;
-    if (dev_alloc(c, &c->changed, 2 * REINIT_CAP)) return -1;
+    if (dev_alloc(c, &c->changed, 2 * (size_t)REINIT_CAP) || dev_alloc(c, &c->rkeys, (size_t)REINIT_CAP)) return -1;
     if (dev_alloc(c, &c->st, 1)) return -1;
     LBIB_HIP(hipHostMalloc((void**)&c->st_host, sizeof(DevStatus), hipHostMallocDefault));
     if (ensure_markers(c, cfg->max_markers > 0 ? cfg->max_markers : 1024)) return -1;
@@ -217,7 +217,7 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->fs_off); dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
     dev_free(c->ca); dev_free(c->cb); dev_free(c->cbc); dev_free(c->cdom); dev_free(c->bcd_off); dev_free(c->bcd_sp);
     dev_free(c->cand); dev_free(c->best_dist); dev_free(c->best_seq); dev_free(c->bn_dom); dev_free(c->bc_v);
-    dev_free(c->bc_desc); dev_free(c->changed);
+    dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rkeys);
     dev_free(c->ring_dom); dev_free(c->ring_off); dev_free(c->ring_marker); dev_free(c->area);
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);

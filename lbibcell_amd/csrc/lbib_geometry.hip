@@ -167,7 +167,7 @@ __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_
 // occupied slot reports the site.  Also resets the selection buffers.
 __global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_t* __restrict__ tag,
                             unsigned long long* best_dist, uint32_t* best_seq, uint32_t* changed,
-                            DevStatus* st) {
+                            int local_update, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->ncand;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
@@ -175,7 +175,7 @@ __global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_
         const long slotix = (long)(e.slot - 1) * n + e.site;
         best_dist[slotix] = ~0ull;
         best_seq[slotix] = ~0u;
-        if (!e.win) continue;
+        if (!e.win || !local_update) continue;
         const uint32_t tg = tag[e.site];
         const uint32_t mask = (tg >> BN_SHIFT) & 0xfu;
         const int hi = 31 - __clz((int)mask);   // highest occupied slot - 1
@@ -259,13 +259,14 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
 // (CDESolverD2Q5.cpp:156-228, ..._SIGNAL.cpp:184-250) reads neighbours updated
 // earlier in that order: sort the changed sites, then apply them in order.
 template <int NS>
-__global__ __launch_bounds__(256) void k_reinit(const ReinitArgs a, DevStatus* st) {
+__global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, unsigned long long* gkeys, DevStatus* st) {
     if (halted(st)) return;
-    __shared__ unsigned long long key[REINIT_CAP];
+    __shared__ unsigned long long lkey[REINIT_LDS];
     const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
     if (cnt == 0) return;
     uint32_t P = 1;
     while (P < cnt) P <<= 1;
+    unsigned long long* key = (P <= (uint32_t)REINIT_LDS) ? lkey : gkeys;
     for (uint32_t t = threadIdx.x; t < P; t += blockDim.x)
         key[t] = (t < cnt) ? (((unsigned long long)a.changed[2 * t] << 32) | a.changed[2 * t + 1]) : ~0ull;
     __syncthreads();
@@ -400,7 +401,7 @@ __global__ void k_mass_regulated(int nring, const uint32_t* __restrict__ ring_do
 
 template <int NS>
 static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
-    k_reinit<NS><<<1, 256, 0, c->stream>>>(a, c->st);
+    k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rkeys, c->st);
 }
 
 int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
@@ -421,7 +422,8 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
     k_cut_win<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my, c->mvx,
                                         c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->st);
-    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, c->st);
+    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, cure ? 0 : 1,
+                                          c->st);
     if (!cure) {
         ReinitArgs a;
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;

@@ -27,7 +27,8 @@ constexpr int BN_SHIFT = 24;        // bit 24 + (slot-1), slot in E,N,W,S
 constexpr int BC_SHIFT = 28;
 constexpr int TILE_W = 256;         // lattice kernel block width = force tile width
 constexpr int MAX_BC_SPECIES = LBIB_MAX_SPECIES;
-constexpr int REINIT_CAP = 4096;    // sites per step whose domain may change
+constexpr int REINIT_LDS = 4096;          // changed sites sorted in LDS
+constexpr int REINIT_CAP = 1 << 20;       // changed sites per step (global-memory sort beyond REINIT_LDS)
 constexpr int SEQ_PER_CONN = 256;   // candidates per connection (segment < 127 lu)
 
 struct SpeciesDev {
@@ -118,6 +119,7 @@ struct lbib_ctx {
     double* bc_v = nullptr;                    // [4][n][2]
     int32_t* bc_desc = nullptr;                // [4][n]
     uint32_t* changed = nullptr;               // [REINIT_CAP][2]
+    unsigned long long* rkeys = nullptr;       // [REINIT_CAP] sort scratch
     // rings / domains
     int rcap = 0, nring = 0, rmcap = 0;
     uint32_t* ring_dom = nullptr;
