@@ -82,9 +82,12 @@ typedef struct lbib_config {
 #define LBIB_ERR_SCANLINE      0x00000004u  /* GeometryHandler.cpp:74-76 */
 #define LBIB_ERR_SLOPE         0x00000008u  /* Connection.cpp:315-323 */
 #define LBIB_ERR_CAPACITY      0x00000010u  /* a device work list overflowed */
+#define LBIB_ERR_TOPOLOGY      0x00000020u  /* a domain is not one closed ring (or ring > 4096 markers) */
 #define LBIB_ERR_MASK          0x000000ffu
 #define LBIB_HALT_PERTURB_NODE 0x00000100u  /* a moved marker needs GeometryNode.cpp:153-214 */
 #define LBIB_HALT_PERTURB_CONN 0x00000200u  /* a connection needs Connection.cpp:332-387 */
+#define LBIB_HALT_PERTURB_REMESH 0x00000400u /* a remesh midpoint needs GeometryNode.cpp:153-214 */
+#define LBIB_HALT_GROW         0x00000800u  /* marker/connection capacity too small for remeshing */
 #define LBIB_HALT_MASK         0x00000f00u
 #define LBIB_FLAG_REMESH       0x00001000u  /* a segment is longer than MAXLENGTH */
 #define LBIB_FLAG_COARSEN      0x00002000u  /* a marker's neighbours are closer than MAXLENGTH */
@@ -96,6 +99,10 @@ typedef struct lbib_status {
     int32_t n_boundary_nodes;   /* cut links kept after closest-wins */
     int32_t n_reinit;           /* sites whose domain changed (CDE reinit) */
     int32_t n_candidates;       /* cut-link candidates generated */
+    int32_t n_markers;          /* GeometryNodes after this step's remesh/coarsen */
+    int32_t n_connections;      /* Connections after this step's remesh/coarsen */
+    int32_t n_remeshed;         /* connections split by remeshing this step */
+    int32_t n_coarsened;        /* markers removed by coarsening this step */
 } lbib_status;
 
 /* ---- context ---------------------------------------------------------- */
@@ -142,12 +149,13 @@ int lbib_set_connections(lbib_ctx* ctx, int nc, const int32_t* a, const int32_t*
 /* Boundary-condition descriptors: descriptor d lists the species indices that
  * BoundarySolverNoFluxD2Q5 is connected to (BoundaryNode.cpp:43-46). */
 int lbib_set_bc_descriptors(lbib_ctx* ctx, int nd, const int32_t* off, const int32_t* species);
-/* Per-marker ring neighbours (previous marker of the incoming connection and
- * next marker of the outgoing one) and per-domain rings in the order of
- * GeometryHandler::computeAreas (GeometryHandler.cpp:101-134). */
-int lbib_set_rings(lbib_ctx* ctx, const int32_t* prev, const int32_t* next,
-                   int ndom_rings, const uint32_t* ring_dom, const int32_t* ring_off,
-                   const int32_t* ring_marker);
+/* GeometryNode ids per dense marker index (needed once remeshing has added
+ * markers on the device) and the current connection list. */
+int lbib_get_marker_ids(lbib_ctx* ctx, uint32_t* ids);
+int lbib_set_marker_ids(lbib_ctx* ctx, const uint32_t* ids);
+int lbib_get_connections(lbib_ctx* ctx, int32_t* a, int32_t* b, uint32_t* dom, int32_t* bc);
+/* The force table in the CSR layout of lbib_set_forces (off has nm+1 entries). */
+int lbib_get_forces(lbib_ctx* ctx, int32_t* off, int32_t* type, int32_t* partner, double* prm);
 /* Cell type per domain id (GeometryHandler::cellTypeTrackerMap_). */
 int lbib_set_celltypes(lbib_ctx* ctx, int ndom, const uint32_t* celltype);
 /* Force structs of ForceSolver::forcemap_ (ForceSolver.cpp:42-85) in CSR by
@@ -164,9 +172,12 @@ int lbib_set_forces(lbib_ctx* ctx, int nm, const int32_t* off, const int32_t* ty
 int lbib_step_lattice(lbib_ctx* ctx, uint32_t it);
 /* Stage B (GeometryHandler::moveLattice after the move, :361-382):
  * connection checks, boundary-node generation with closest-wins, domain
- * update + CDE reinit, domain scanline; plus the remesh/coarsen checks.
- * check_coarsen != 0 when it % 10 == 0. */
-int lbib_step_geometry(lbib_ctx* ctx, uint32_t it, int check_coarsen);
+ * update + CDE reinit, domain scanline; then remeshBoundary
+ * (GeometryHandler.cpp:385-431) and, when it % 10 == 0, deleteAllForces and
+ * coarsenBoundary (:433-461) on the device-resident topology. */
+int lbib_step_geometry(lbib_ctx* ctx, uint32_t it, int coarsen);
+/* Only the remesh/coarsen part of stage B (after a host-handled halt). */
+int lbib_step_topology(lbib_ctx* ctx, int coarsen);
 /* GeometryHandler::cureLattice (GeometryHandler.cpp:149-170) after the host
  * has perturbed connections: boundary nodes + scanline, no CDE reinit. */
 int lbib_cure_lattice(lbib_ctx* ctx);
@@ -180,7 +191,8 @@ int lbib_read_status(lbib_ctx* ctx, lbib_status* st);
 int lbib_clear_status(lbib_ctx* ctx);
 
 /* ---- device BioSolver helpers ------------------------------------------- */
-/* Areas per ring (shoelace, FP64), in ring order of lbib_set_rings. */
+/* Areas (shoelace, FP64, in ring order) per domain id 0..ndom-1; domains
+ * without connections read 0.  ndom = value given to lbib_set_celltypes. */
 int lbib_compute_areas(lbib_ctx* ctx, double* area);
 /* tutorial_01_BioSolverAreaRegulator::applyBioProcess
  * (tutorial_01_BioSolverAreaRegulator.cpp:43-100) as one device op:
@@ -189,6 +201,12 @@ int lbib_compute_areas(lbib_ctx* ctx, double* area);
  * of a new domain to factor * its first area instead of target_area. */
 int lbib_area_regulator(lbib_ctx* ctx, double target_area, double kp, double maxsource,
                         double target_factor);
+/* tutorial_01_BioSolverMembraneTension (…MembraneTension.cpp:42-77) as one
+ * device op: deleteForceType(6) + two type-6 structs of magnitude f per
+ * connection.  Only valid while the force table holds no other types. */
+int lbib_forces_tension(lbib_ctx* ctx, double f);
+/* ForceSolver::deleteAllForces (ForceSolver.cpp:158-166). */
+int lbib_forces_clear(lbib_ctx* ctx);
 /* FluidSolver::addMass for every site of each domain: m_site = table[dom]. */
 int lbib_add_mass_by_domain(lbib_ctx* ctx, int ndom, const double* table);
 

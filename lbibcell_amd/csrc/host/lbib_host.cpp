@@ -265,6 +265,35 @@ unsigned Geometry::coarsenBoundary(double maxLength) {
     return removed;
 }
 
+void Geometry::rebuild(const std::vector<unsigned>& ids, const std::vector<double>& xy, const std::vector<double>& vxy,
+                       const std::vector<double>& oxy, const std::vector<int32_t>& a, const std::vector<int32_t>& b,
+                       const std::vector<uint32_t>& dom,
+                       const std::vector<std::map<std::string, std::vector<std::string>>>& descriptors) {
+    nodes_.clear();
+    connections_.clear();
+    std::vector<GeometryNode*> byIndex(ids.size());
+    for (size_t k = 0; k < ids.size(); k++) {
+        auto n = std::unique_ptr<GeometryNode>(new GeometryNode());
+        n->id = ids[k];
+        n->x = xy[2 * k]; n->y = xy[2 * k + 1];
+        n->vx = vxy[2 * k]; n->vy = vxy[2 * k + 1];
+        n->ox = oxy[2 * k]; n->oy = oxy[2 * k + 1];
+        byIndex[k] = n.get();
+        nodes_[ids[k]] = std::move(n);
+    }
+    connections_.reserve(a.size());
+    for (size_t j = 0; j < a.size(); j++) {
+        auto c = std::make_shared<Connection>();
+        c->first = byIndex[a[j]];
+        c->second = byIndex[b[j]];
+        c->domain = dom[j];
+        if (j < descriptors.size()) c->descriptor = descriptors[j];
+        c->first->out = c.get();
+        c->second->in = c.get();
+        connections_.push_back(c);
+    }
+}
+
 bool Geometry::checkGeometryIntegrity() const {
     for (auto& kv : nodes_) {
         const GeometryNode* n = kv.second.get();
@@ -368,6 +397,13 @@ void ForceSolver::writeForceSolver(const std::string& file) const {
         }
 }
 
+bool ForceSolver::onlyType(int t) const {
+    for (auto& kv : forcemap_)
+        for (auto& f : kv.second)
+            if (f.type != t) return false;
+    return true;
+}
+
 void ForceSolver::deleteForceType(unsigned type) {
     for (auto& kv : forcemap_)
         kv.second.erase(std::remove_if(kv.second.begin(), kv.second.end(),
@@ -389,16 +425,7 @@ struct MembraneTension : BioSolver {
     double force = 0.01;
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
-        auto& fs = r.getForceSolver();
-        fs.deleteForceType(6);
-        for (auto& c : r.getGeometry().getConnections()) {
-            std::stringstream d;
-            d << "6\t" << c->first->id << "\t" << c->second->id << "\t" << force;
-            fs.addForce(&d);
-            std::stringstream e;
-            e << "6\t" << c->second->id << "\t" << c->first->id << "\t" << force;
-            fs.addForce(&e);
-        }
+        r.renewTension(force);
     }
 };
 
@@ -489,7 +516,6 @@ void SimulationRunner::setup() {
     // GeometryHandler::cureLattice: perturb connections, link stencils (implicit
     // on the device), boundary nodes, scanline.
     geo_.perturbConnections();
-    topologyDirty_ = true;
     uploadTopology();
     check(lbib_cure_lattice(ctx_), "cureLattice");
     check(lbib_read_status(ctx_, &lastStatus), "status");
@@ -518,20 +544,58 @@ void SimulationRunner::addBioSolver(const std::string& name) {
 
 void SimulationRunner::addBioSolver(std::unique_ptr<BioSolver> b) { bio_.push_back(std::move(b)); }
 
-void SimulationRunner::syncMarkersFromDevice() {
-    if (hostMarkersFresh_) return;
-    const size_t nm = ids_.size();
+void SimulationRunner::syncMirror() {
+    if (mirrorValid_) return;
+    const size_t nm = nm_, nc = nc_;
+    std::vector<unsigned> ids(nm);
     std::vector<double> xy(2 * nm), v(2 * nm), o(2 * nm);
+    check(lbib_get_marker_ids(ctx_, ids.data()), "get_marker_ids");
     check(lbib_get_markers(ctx_, xy.data(), v.data(), nullptr, o.data()), "get_markers");
-    size_t k = 0;
-    for (auto& kv : geo_.nodes()) {
-        auto& n = *kv.second;
-        n.x = xy[2 * k]; n.y = xy[2 * k + 1];
-        n.vx = v[2 * k]; n.vy = v[2 * k + 1];
-        n.ox = o[2 * k]; n.oy = o[2 * k + 1];
-        k++;
+    std::vector<int32_t> a(nc), b(nc), bc(nc);
+    std::vector<uint32_t> dom(nc);
+    check(lbib_get_connections(ctx_, a.data(), b.data(), dom.data(), bc.data()), "get_connections");
+    std::vector<std::map<std::string, std::vector<std::string>>> desc(nc);
+    for (size_t j = 0; j < nc; j++)
+        if (bc[j] >= 0) desc[j] = bcDescriptorText_.at(bc[j]);
+    geo_.rebuild(ids, xy, v, o, a, b, dom, desc);
+    ids_ = ids;
+    // force table back into ForceSolver::forcemap_ (ids instead of indices)
+    std::vector<int32_t> off(nm + 1);
+    check(lbib_get_forces(ctx_, off.data(), nullptr, nullptr, nullptr), "get_forces");
+    const int nf = off[nm];
+    std::vector<int32_t> type(nf), partner(nf);
+    std::vector<double> prm(4 * (size_t)nf);
+    check(lbib_get_forces(ctx_, off.data(), type.data(), partner.data(), prm.data()), "get_forces");
+    std::map<unsigned, std::vector<solver::ForceStruct>> fm;
+    for (size_t k = 0; k < nm; k++)
+        for (int j = off[k]; j < off[k + 1]; j++) {
+            solver::ForceStruct f{type[j], ids[k], partner[j] >= 0 ? ids[partner[j]] : 0u,
+                                  {prm[4 * j], prm[4 * j + 1], prm[4 * j + 2], prm[4 * j + 3]}};
+            fm[ids[k]].push_back(f);
+        }
+    forceSolver_.replace(std::move(fm));
+    forceSolver_.dirty_ = false;
+    mirrorValid_ = true;
+}
+
+void SimulationRunner::renewTension(double force) {
+    // FORCE travels through a text descriptor in the reference: 6 significant digits
+    std::stringstream ss;
+    ss << force;
+    double f;
+    ss >> f;
+    if (forceSolver_.dirty_ || !forceSolver_.onlyType(6)) {
+        syncMirror();
+        forceSolver_.deleteForceType(6);
+        for (auto& c : geo_.getConnections()) {
+            forceSolver_.addForce(solver::ForceStruct{6, c->first->id, c->second->id, {f, 0, 0, 0}});
+            forceSolver_.addForce(solver::ForceStruct{6, c->second->id, c->first->id, {f, 0, 0, 0}});
+        }
+        uploadForces();
+        return;
     }
-    hostMarkersFresh_ = true;
+    check(lbib_forces_tension(ctx_, f), "tension");
+    mirrorValid_ = false;   // the host forcemap no longer matches the device table
 }
 
 void SimulationRunner::uploadTopology() {
@@ -551,9 +615,11 @@ void SimulationRunner::uploadTopology() {
         k++;
     }
     check(lbib_set_markers(ctx_, nm, xy.data(), v.data(), o.data()), "set_markers");
+    check(lbib_set_marker_ids(ctx_, ids_.data()), "set_marker_ids");
     // boundary-condition descriptors (BoundaryNode.cpp:43-46)
     std::map<std::vector<int>, int> descIndex;
     bcDescriptors_.clear();
+    bcDescriptorText_.clear();
     const auto& conns = geo_.getConnections();
     const int nc = (int)conns.size();
     std::vector<int32_t> a(nc), b(nc), bc(nc, -1);
@@ -576,6 +642,7 @@ void SimulationRunner::uploadTopology() {
             if (it == descIndex.end()) {
                 it = descIndex.emplace(sp, (int)bcDescriptors_.size()).first;
                 bcDescriptors_.push_back(sp);
+                bcDescriptorText_.push_back(conns[c]->descriptor);
             }
             bc[c] = it->second;
         }
@@ -589,42 +656,16 @@ void SimulationRunner::uploadTopology() {
         check(lbib_set_bc_descriptors(ctx_, (int)bcDescriptors_.size(), off.data(), sp.data()), "set_bc");
     }
     check(lbib_set_connections(ctx_, nc, a.data(), b.data(), dom.data(), bc.data()), "set_connections");
-    // ring neighbours and rings (GeometryHandler.cpp:101-134)
-    std::vector<int32_t> prev(nm, -1), next(nm, -1);
-    k = 0;
-    for (auto& kv : nodes) {
-        const auto& n = *kv.second;
-        if (n.in) prev[k] = index.at(n.in->first);
-        if (n.out) next[k] = index.at(n.out->second);
-        k++;
-    }
-    std::map<unsigned, const geometry::Connection*> start;
-    for (auto& c : conns)
-        if (!start.count(c->domain)) start[c->domain] = c.get();
-    std::vector<uint32_t> rdom;
-    std::vector<int32_t> roff{0}, rm;
-    for (auto& kv : start) {
-        const geometry::Connection* c = kv.second;
-        size_t guard = 0;
-        do {
-            rm.push_back(index.at(c->second));
-            c = c->second->out;
-            if (!c || ++guard > conns.size()) LBIB_FAIL("open membrane ring");
-        } while (c != kv.second);
-        rdom.push_back(kv.first);
-        roff.push_back((int32_t)rm.size());
-    }
-    check(lbib_set_rings(ctx_, prev.data(), next.data(), (int)rdom.size(), rdom.data(), roff.data(), rm.data()),
-          "set_rings");
     // cell types per domain id (missing domains read as 1, GeometryHandler.cpp:278-284)
     auto& ctm = geo_.cellTypeTrackerMap();
     for (auto& kv : ctm) maxdom = std::max(maxdom, kv.first);
     std::vector<uint32_t> ct(maxdom + 1, 1u);
     for (auto& kv : ctm) ct[kv.first] = kv.second;
     check(lbib_set_celltypes(ctx_, (int)ct.size(), ct.data()), "set_celltypes");
+    nm_ = (size_t)nm;
+    nc_ = (size_t)nc;
     uploadForces();
-    topologyDirty_ = false;
-    hostMarkersFresh_ = true;
+    mirrorValid_ = true;
 }
 
 void SimulationRunner::uploadForces() {
@@ -655,10 +696,6 @@ void SimulationRunner::uploadForces() {
     forceSolver_.dirty_ = false;
 }
 
-unsigned SimulationRunner::remeshBoundary() { return geo_.remeshBoundary(kMAXLENGTH); }
-
-void SimulationRunner::coarsenBoundary() { geo_.coarsenBoundary(kMAXLENGTH); }
-
 std::string SimulationRunner::describe(const lbib_status& s) {
     std::string m;
     if (s.bits & LBIB_ERR_VELOCITY) m += "The magnitude of velocity is too high; ";
@@ -671,81 +708,96 @@ std::string SimulationRunner::describe(const lbib_status& s) {
     return m;
 }
 
-void SimulationRunner::handleGeometryStatus() {
-    for (int guard = 0; guard < 16; guard++) {
+// Remeshing/coarsening on the host mirror, used when the device halts on a
+// perturbation draw or runs out of capacity (both rare).
+void SimulationRunner::hostTopologyFallback(unsigned it) {
+    auto t0 = std::chrono::steady_clock::now();
+    check(lbib_clear_status(ctx_), "status");
+    mirrorValid_ = false;
+    syncMirror();
+    while (geo_.remeshBoundary(kMAXLENGTH)) {}
+    if (it % 10 == 0) {
+        forceSolver_.deleteAllForces();
+        geo_.coarsenBoundary(kMAXLENGTH);
+    }
+    uploadTopology();
+    times.host += seconds_since(t0);
+}
+
+void SimulationRunner::handleGeometryStatus(unsigned it) {
+    bool coarsened = false;
+    for (int guard = 0; guard < 64; guard++) {
         check(lbib_read_status(ctx_, &lastStatus), "status");
+        nm_ = (size_t)lastStatus.n_markers;
+        nc_ = (size_t)lastStatus.n_connections;
         const uint32_t bits = lastStatus.bits;
         if (bits & LBIB_ERR_MASK) throw Exception(describe(lastStatus));
-        if (!(bits & LBIB_HALT_MASK)) return;
-        auto t0 = std::chrono::steady_clock::now();
-        hostMarkersFresh_ = false;
-        syncMarkersFromDevice();
-        if (bits & LBIB_HALT_PERTURB_NODE) {
-            // Geometry::moveGeometryNodes calls setPos -> perturbatePosition on
-            // every node in id order; the device moved them already.
-            for (auto& kv : geo_.nodes()) geo_.perturbatePosition(*kv.second);
-        } else {
-            geo_.perturbConnections();
+        if (bits & (LBIB_HALT_PERTURB_NODE | LBIB_HALT_PERTURB_CONN)) {
+            // the move ran, nothing after it: perturb on the host, redo stage B
+            auto t0 = std::chrono::steady_clock::now();
+            mirrorValid_ = false;
+            syncMirror();
+            if (bits & LBIB_HALT_PERTURB_NODE) {
+                // Geometry::moveGeometryNodes -> setPos -> perturbatePosition, id order
+                for (auto& kv : geo_.nodes()) geo_.perturbatePosition(*kv.second);
+            } else {
+                geo_.perturbConnections();
+            }
+            std::vector<double> xy(2 * nm_), v(2 * nm_), o(2 * nm_);
+            size_t k = 0;
+            for (auto& kv : geo_.getGeometryNodes()) {
+                xy[2 * k] = kv.second->x; xy[2 * k + 1] = kv.second->y;
+                v[2 * k] = kv.second->vx; v[2 * k + 1] = kv.second->vy;
+                o[2 * k] = kv.second->ox; o[2 * k + 1] = kv.second->oy;
+                k++;
+            }
+            check(lbib_set_markers(ctx_, (int)nm_, xy.data(), v.data(), o.data()), "set_markers");
+            check(lbib_clear_status(ctx_), "status");
+            times.host += seconds_since(t0);
+            check(lbib_step_geometry(ctx_, it, it % 10 == 0), "moveLattice");
+            continue;
         }
-        std::vector<double> xy(2 * ids_.size());
-        size_t k = 0;
-        for (auto& kv : geo_.getGeometryNodes()) { xy[2 * k] = kv.second->x; xy[2 * k + 1] = kv.second->y; k++; }
-        // keep velocities and old positions as they are on the device
-        std::vector<double> v(2 * ids_.size()), o(2 * ids_.size());
-        k = 0;
-        for (auto& kv : geo_.getGeometryNodes()) {
-            v[2 * k] = kv.second->vx; v[2 * k + 1] = kv.second->vy;
-            o[2 * k] = kv.second->ox; o[2 * k + 1] = kv.second->oy;
-            k++;
+        if (bits & (LBIB_HALT_PERTURB_REMESH | LBIB_HALT_GROW)) {
+            hostTopologyFallback(it);
+            coarsened = true;
+            break;
         }
-        check(lbib_set_markers(ctx_, (int)ids_.size(), xy.data(), v.data(), o.data()), "set_markers");
-        lbib_status keep = lastStatus;
-        check(lbib_clear_status(ctx_), "status");
-        (void)keep;
-        times.host += seconds_since(t0);
-        check(lbib_step_geometry(ctx_, params_.currentIteration, params_.currentIteration % 10 == 0),
-              "moveLattice");
+        if (bits & LBIB_FLAG_REMESH) {
+            // more remesh rounds than one stage-B pass runs (then coarsen)
+            check(lbib_clear_status(ctx_), "status");
+            check(lbib_step_topology(ctx_, it % 10 == 0), "remesh");
+            continue;
+        }
+        break;
     }
-    throw Exception("perturbation did not converge");
+    (void)coarsened;
+    if (it % 10 == 0) {
+        // ForceSolver::deleteAllForces before the BioSolvers renew them
+        forceSolver_.deleteAllForces();
+        forceSolver_.dirty_ = false;
+        check(lbib_forces_clear(ctx_), "deleteAllForces");
+    }
+    check(lbib_clear_status(ctx_), "status");
 }
 
 void SimulationRunner::step() {
     const unsigned it = params_.currentIteration;
     auto t0 = std::chrono::steady_clock::now();
-    if (forceSolver_.dirty_) uploadForces();
+    if (forceSolver_.dirty_) {
+        syncMirror();
+        uploadForces();
+    }
     check(lbib_step_lattice(ctx_, it), "lattice step");
     auto t1 = std::chrono::steady_clock::now();
     check(lbib_step_geometry(ctx_, it, it % 10 == 0), "moveLattice");
-    hostMarkersFresh_ = false;
-    handleGeometryStatus();
+    mirrorValid_ = false;
+    handleGeometryStatus(it);
     auto t2 = std::chrono::steady_clock::now();
     times.lattice += std::chrono::duration<double>(t1 - t0).count();
     times.geometry += std::chrono::duration<double>(t2 - t1).count();
-    // remeshBoundary / coarsenBoundary on the host topology (SimulationRunner.cpp:83-87)
-    bool edited = false;
-    if (lastStatus.bits & LBIB_FLAG_REMESH) {
-        syncMarkersFromDevice();
-        while (remeshBoundary()) {}
-        edited = true;
-    }
-    if (it % 10 == 0) {
-        forceSolver_.deleteAllForces();
-        bool need = (lastStatus.bits & LBIB_FLAG_COARSEN) != 0;
-        if (edited) need = true;   // the device check ran before the remesh
-        if (need) {
-            syncMarkersFromDevice();
-            const size_t before = geo_.getGeometryNodes().size();
-            coarsenBoundary();
-            if (geo_.getGeometryNodes().size() != before) edited = true;
-        }
-    }
-    check(lbib_clear_status(ctx_), "status");
-    if (edited) uploadTopology();
-    auto t3 = std::chrono::steady_clock::now();
-    times.host += std::chrono::duration<double>(t3 - t2).count();
     // mass solvers: MassSolverBoxOutlet is applied inside the collide kernel.
     for (auto& b : bio_) b->applyBioProcess(*this);
-    times.bio += seconds_since(t3);
+    times.bio += seconds_since(t2);
     params_.currentIteration++;
     times.steps++;
 }
@@ -757,14 +809,16 @@ void SimulationRunner::run(unsigned steps) {
 void SimulationRunner::runSimulation() { run(params_.iterations); }
 
 std::map<unsigned, double> SimulationRunner::computeAreas() {
-    std::map<unsigned, const geometry::Connection*> start;
-    for (auto& c : geo_.getConnections())
-        if (!start.count(c->domain)) start[c->domain] = c.get();
-    std::vector<double> a(start.size());
+    syncMirror();
+    std::map<unsigned, int> present;
+    unsigned maxdom = 0;
+    for (auto& c : geo_.getConnections()) { present[c->domain] = 1; maxdom = std::max(maxdom, c->domain); }
+    auto& ctm = geo_.cellTypeTrackerMap();
+    for (auto& kv : ctm) maxdom = std::max(maxdom, kv.first);
+    std::vector<double> a(maxdom + 1);
     check(lbib_compute_areas(ctx_, a.data()), "computeAreas");
     std::map<unsigned, double> out;
-    size_t k = 0;
-    for (auto& kv : start) out[kv.first] = a[k++];
+    for (auto& kv : present) out[kv.first] = a[kv.first];
     return out;
 }
 

@@ -80,6 +80,12 @@ class Geometry {
     // one pass each, same resulting node ids and connection order.
     unsigned remeshBoundary(double maxLength);
     unsigned coarsenBoundary(double maxLength);
+    // replace nodes and connections (a device-side topology copied back); the
+    // perturbation streams and cell types are kept
+    void rebuild(const std::vector<unsigned>& ids, const std::vector<double>& xy, const std::vector<double>& vxy,
+                 const std::vector<double>& oxy, const std::vector<int32_t>& a, const std::vector<int32_t>& b,
+                 const std::vector<uint32_t>& dom,
+                 const std::vector<std::map<std::string, std::vector<std::string>>>& descriptors);
     const std::map<unsigned, std::unique_ptr<GeometryNode>>& getGeometryNodes() const { return nodes_; }
     std::map<unsigned, std::unique_ptr<GeometryNode>>& nodes() { return nodes_; }
     const std::vector<std::shared_ptr<Connection>>& getConnections() const { return connections_; }
@@ -116,6 +122,8 @@ class ForceSolver {
     void loadForceFile(const std::string& file);         // ForceSolver.cpp:89-105
     void writeForceSolver(const std::string& file) const;
     void deleteAllForces() { forcemap_.clear(); dirty_ = true; }
+    void replace(std::map<unsigned, std::vector<ForceStruct>>&& m) { forcemap_ = std::move(m); }
+    bool onlyType(int t) const;
     void deleteForceType(unsigned type);                 // ForceSolver.cpp:168-204
     const std::map<unsigned, std::vector<ForceStruct>>& forcemap() const { return forcemap_; }
     bool dirty_ = true;
@@ -164,9 +172,15 @@ class SimulationRunner {
     lbib_ctx* context() { return ctx_; }
     GlobalSimulationParameters& parameters() { return params_; }
     int speciesIndex(const std::string& name) const;
-    // host views (lazy device->host sync)
-    void syncMarkersFromDevice();
+    // host views: the device owns topology and forces between host edits;
+    // syncMirror() copies them back, uploadTopology() pushes host edits.
+    void syncMirror();
+    void syncMarkersFromDevice() { syncMirror(); }
     void uploadTopology();
+    void invalidateMirror() { mirrorValid_ = false; }
+    // device-resident tension renewal (falls back to the host table when it
+    // holds other force types)
+    void renewTension(double force);
     std::map<unsigned, double> computeAreas();            // GeometryHandler.cpp:101-134 (device shoelace)
     StageTimes times;
     lbib_status lastStatus{};
@@ -175,9 +189,8 @@ class SimulationRunner {
  private:
     void check(int rc, const char* what);
     void uploadForces();
-    unsigned remeshBoundary();                            // GeometryHandler.cpp:385-431
-    void coarsenBoundary();                               // GeometryHandler.cpp:433-461
-    void handleGeometryStatus();
+    void handleGeometryStatus(unsigned it);
+    void hostTopologyFallback(unsigned it);
     std::string describe(const lbib_status& s);
 
     GlobalSimulationParameters params_;
@@ -191,8 +204,9 @@ class SimulationRunner {
     std::vector<std::string> massSolvers_;
     std::vector<unsigned> ids_;                 // dense index -> node id (ascending)
     std::vector<std::vector<int>> bcDescriptors_;
-    bool topologyDirty_ = true;
-    bool hostMarkersFresh_ = true;
+    std::vector<std::map<std::string, std::vector<std::string>>> bcDescriptorText_;
+    bool mirrorValid_ = true;
+    size_t nm_ = 0, nc_ = 0;
 };
 
 }  // namespace LbmLib

@@ -150,12 +150,23 @@ int lbh_set_iteration(void* p, unsigned it) {
     else h->params.currentIteration = it;
     return 0;
 }
-int lbh_num_markers(void* p) { return (int)static_cast<Handle*>(p)->geo.getGeometryNodes().size(); }
-int lbh_num_connections(void* p) { return (int)static_cast<Handle*>(p)->geo.getConnections().size(); }
+static void mirror(Handle* h) {
+    if (h->runner) h->runner->syncMirror();
+}
+int lbh_num_markers(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    if (guarded(h, [&] { mirror(h); })) return -1;
+    return (int)h->geo.getGeometryNodes().size();
+}
+int lbh_num_connections(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    if (guarded(h, [&] { mirror(h); })) return -1;
+    return (int)h->geo.getConnections().size();
+}
 int lbh_get_markers(void* p, unsigned* ids, double* xy, double* v) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
-        if (h->runner) h->runner->syncMarkersFromDevice();
+        mirror(h);
         size_t k = 0;
         for (auto& kv : h->geo.getGeometryNodes()) {
             if (ids) ids[k] = kv.first;
@@ -167,6 +178,7 @@ int lbh_get_markers(void* p, unsigned* ids, double* xy, double* v) {
 }
 int lbh_get_connections(void* p, unsigned* a, unsigned* b, unsigned* dom) {
     auto* h = static_cast<Handle*>(p);
+    if (guarded(h, [&] { mirror(h); })) return -1;
     size_t k = 0;
     for (auto& c : h->geo.getConnections()) {
         a[k] = c->first->id;
@@ -193,6 +205,7 @@ int lbh_last_status(void* p, lbib_status* st) {
 }
 int lbh_num_areas(void* p) {
     auto* h = static_cast<Handle*>(p);
+    if (guarded(h, [&] { mirror(h); })) return -1;
     std::map<unsigned, int> d;
     for (auto& c : h->geo.getConnections()) d[c->domain] = 1;
     return (int)d.size();
@@ -214,16 +227,7 @@ struct TensionP : BioSolver {
     double force;
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
-        auto& fs = r.getForceSolver();
-        fs.deleteForceType(6);
-        for (auto& c : r.getGeometry().getConnections()) {
-            std::stringstream d;
-            d << "6\t" << c->first->id << "\t" << c->second->id << "\t" << force;
-            fs.addForce(&d);
-            std::stringstream e;
-            e << "6\t" << c->second->id << "\t" << c->first->id << "\t" << force;
-            fs.addForce(&e);
-        }
+        r.renewTension(force);
     }
 };
 struct AreaP : BioSolver {

@@ -1,6 +1,7 @@
 // lbib_capi.hip -- context management and the C ABI declared in include/lbib.h.
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -42,38 +43,57 @@ size_t storage_bytes(int prec) {
     }
 }
 
+int ensure_scan(lbib_ctx* c) {
+    const uint32_t need = (uint32_t)((std::max({c->mcap, c->ccap, c->dcap, 1}) + 1023) / 1024 + 1);
+    if (need <= c->scan_cap) return 0;
+    dev_free(c->scan_tmp);
+    if (dev_alloc(c, &c->scan_tmp, need)) return -1;
+    c->scan_cap = need;
+    return 0;
+}
+
+// Marker-indexed arrays (positions, links, spread work lists, coarsening
+// scratch, force CSR offsets).  Reallocation drops the contents: callers
+// re-upload the topology afterwards.
 int ensure_markers(lbib_ctx* c, int nm) {
-    if (nm <= c->mcap) return 0;
-    int cap = std::max(nm, c->mcap * 2);
+    if (nm + 1 <= c->mcap) return 0;
+    int cap = std::max(nm + 1, c->mcap * 2);
     cap = std::max(cap, 1024);
-    double** arrs[8] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy};
-    for (auto a : arrs) dev_free(*a);
-    dev_free(c->mprev); dev_free(c->mnext); dev_free(c->srank); dev_free(c->touched); dev_free(c->pool);
-    dev_free(c->fs_off);
-    for (auto a : arrs) if (dev_alloc(c, a, cap)) return -1;
-    if (dev_alloc(c, &c->mprev, cap, 0xff) || dev_alloc(c, &c->mnext, cap, 0xff)) return -1;
+    double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
+                         &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
+    uint32_t** uarr[9] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
+                          &c->pos_r, &c->mdom};
+    int32_t** iarr[6] = {&c->min_c, &c->mout_c, &c->rec_a, &c->rec_b, &c->rec_bc, &c->fs_off};
+    for (auto a : darr) dev_free(*a);
+    for (auto a : uarr) dev_free(*a);
+    for (auto a : iarr) dev_free(*a);
+    dev_free(c->srank); dev_free(c->touched); dev_free(c->pool);
+    for (auto a : darr) if (dev_alloc(c, a, cap)) return -1;
+    for (auto a : uarr) if (dev_alloc(c, a, cap)) return -1;
+    for (auto a : iarr) if (dev_alloc(c, a, cap + 1)) return -1;
     if (dev_alloc(c, &c->srank, (size_t)cap * 16) || dev_alloc(c, &c->touched, (size_t)cap * 16)) return -1;
     if (dev_alloc(c, &c->pool, (size_t)cap * 16)) return -1;
-    if (dev_alloc(c, &c->fs_off, (size_t)cap + 1)) return -1;
     c->mcap = cap;
-    return 0;
+    return ensure_scan(c);
 }
 
 int ensure_connections(lbib_ctx* c, int nc) {
     if (nc <= c->ccap && c->cand) return 0;
     int cap = std::max(nc, c->ccap * 2);
     cap = std::max(cap, 1024);
-    dev_free(c->ca); dev_free(c->cb); dev_free(c->cbc); dev_free(c->cdom); dev_free(c->cand);
-    if (dev_alloc(c, &c->ca, cap) || dev_alloc(c, &c->cb, cap) || dev_alloc(c, &c->cbc, cap, 0xff) ||
-        dev_alloc(c, &c->cdom, cap))
-        return -1;
+    int32_t** iarr[8] = {&c->ca, &c->cb, &c->cbc, &c->ca2, &c->cb2, &c->cbc2, &c->ring_bucket, &c->ring_marker};
+    uint32_t** uarr[8] = {&c->cdom, &c->cdom2, &c->flag_c, &c->rank_c, &c->cdead, &c->keep_c, &c->pos_c, &c->ring_slot};
+    for (auto a : iarr) dev_free(*a);
+    for (auto a : uarr) dev_free(*a);
+    dev_free(c->cand);
+    for (auto a : iarr) if (dev_alloc(c, a, cap)) return -1;
+    for (auto a : uarr) if (dev_alloc(c, a, cap)) return -1;
     // a connection of length L crosses at most L+1 lines of each family; keep
     // room for 8 candidates per connection plus slack for long first segments.
     c->candcap = cap * 8 + 65536;
     if (dev_alloc(c, &c->cand, c->candcap)) return -1;
     c->ccap = cap;
-    // the candidate list of the previous cure is gone: forget it on the device
-    return 0;
+    return ensure_scan(c);
 }
 
 int ensure_domains(lbib_ctx* c, int ndom) {
@@ -88,18 +108,40 @@ int ensure_domains(lbib_ctx* c, int ndom) {
             return -1;
         }
     }
-    dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab);
-    if (dev_alloc(c, &c->celltype, cap) || dev_alloc(c, &c->target, cap, 0xff) || dev_alloc(c, &c->mtab, cap))
+    dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->ring_start); dev_free(c->ring_cnt);
+    dev_free(c->ring_off_u); dev_free(c->area);
+    if (dev_alloc(c, &c->celltype, cap) || dev_alloc(c, &c->target, cap, 0xff) || dev_alloc(c, &c->mtab, cap) ||
+        dev_alloc(c, &c->ring_start, cap) || dev_alloc(c, &c->ring_cnt, cap) || dev_alloc(c, &c->ring_off_u, cap) ||
+        dev_alloc(c, &c->area, cap))
         return -1;
     if (!tgt.empty() && hipMemcpy(c->target, tgt.data(), tgt.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
         c->err = "copy of target areas failed";
         return -1;
     }
     c->dcap = cap;
-    return 0;
+    return ensure_scan(c);
 }
 
 int check_ctx(lbib_ctx* c) { return c ? 0 : -1; }
+
+int ensure_force_cap(lbib_ctx* c, int nf) {
+    if (nf <= c->fcap) return 0;
+    dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
+    c->fcap = std::max(nf, 2 * c->fcap);
+    if (dev_alloc(c, &c->fs_type, c->fcap) || dev_alloc(c, &c->fs_partner, c->fcap) ||
+        dev_alloc(c, &c->fs_prm, 4 * (size_t)c->fcap))
+        return -1;
+    return 0;
+}
+
+// write the host's marker/connection counts into the device status
+int put_counts(lbib_ctx* c) {
+    uint32_t v[2] = {(uint32_t)c->nm, (uint32_t)c->nc};
+    static_assert(offsetof(DevStatus, nc) == offsetof(DevStatus, nm) + sizeof(uint32_t), "layout");
+    LBIB_HIP(hipMemcpyAsync((char*)c->st + offsetof(DevStatus, nm), v, sizeof(v), hipMemcpyHostToDevice, c->stream));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
 
 int upload(lbib_ctx* c, void* dst, const void* src, size_t bytes) {
     if (bytes == 0) return 0;
@@ -190,6 +232,7 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     // empty force CSR
     if (dev_alloc(c, &c->fs_type, 1) || dev_alloc(c, &c->fs_partner, 1) || dev_alloc(c, &c->fs_prm, 4)) return -1;
     c->fcap = 1;
+    if (put_counts(c)) return -1;
     {
         int32_t one[2] = {0, 0};
         if (dev_alloc(c, &c->bcd_off, 2) || dev_alloc(c, &c->bcd_sp, 1)) return -1;
@@ -212,13 +255,21 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->ux); dev_free(c->uy); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
     dev_free(c->tag); dev_free(c->ftile); dev_free(c->scnt); dev_free(c->sbase); dev_free(c->touched);
     dev_free(c->srank); dev_free(c->pool);
-    dev_free(c->mx); dev_free(c->my); dev_free(c->mvx); dev_free(c->mvy); dev_free(c->mfx); dev_free(c->mfy);
-    dev_free(c->mox); dev_free(c->moy); dev_free(c->mprev); dev_free(c->mnext);
-    dev_free(c->fs_off); dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
-    dev_free(c->ca); dev_free(c->cb); dev_free(c->cbc); dev_free(c->cdom); dev_free(c->bcd_off); dev_free(c->bcd_sp);
+    double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
+                         &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
+    uint32_t** uarr[18] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
+                           &c->pos_r, &c->mdom, &c->cdom, &c->cdom2, &c->flag_c, &c->rank_c, &c->cdead, &c->keep_c,
+                           &c->pos_c, &c->ring_slot, &c->scan_tmp};
+    int32_t** iarr[16] = {&c->min_c, &c->mout_c, &c->rec_a, &c->rec_b, &c->rec_bc, &c->fs_off, &c->ca, &c->cb,
+                          &c->cbc, &c->ca2, &c->cb2, &c->cbc2, &c->ring_bucket, &c->ring_marker, &c->fs_type,
+                          &c->fs_partner};
+    for (auto a : darr) dev_free(*a);
+    for (auto a : uarr) dev_free(*a);
+    for (auto a : iarr) dev_free(*a);
+    dev_free(c->fs_prm); dev_free(c->bcd_off); dev_free(c->bcd_sp);
     dev_free(c->cand); dev_free(c->best_dist); dev_free(c->best_seq); dev_free(c->bn_dom); dev_free(c->bc_v);
     dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rkeys);
-    dev_free(c->ring_dom); dev_free(c->ring_off); dev_free(c->ring_marker); dev_free(c->area);
+    dev_free(c->ring_start); dev_free(c->ring_cnt); dev_free(c->ring_off_u); dev_free(c->area);
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
@@ -349,6 +400,42 @@ int lbib_set_markers(lbib_ctx* c, int nm, const double* xy, const double* vxy, c
         if (oxy && split(oxy, c->mox, c->moy)) return -1;
     }
     c->nm = nm;
+    return put_counts(c);
+}
+int lbib_set_marker_ids(lbib_ctx* c, const uint32_t* ids) {
+    if (check_ctx(c) || (c->nm > 0 && !ids)) return -1;
+    uint32_t mx = 0;
+    for (int k = 0; k < c->nm; k++) {
+        if (k > 0 && ids[k] <= ids[k - 1]) { c->err = "marker ids must increase with the dense index"; return -1; }
+        mx = std::max(mx, ids[k]);
+    }
+    if (upload(c, c->mid, ids, c->nm * sizeof(uint32_t))) return -1;
+    LBIB_HIP(hipMemcpyAsync((char*)c->st + offsetof(DevStatus, maxid), &mx, sizeof(uint32_t), hipMemcpyHostToDevice,
+                            c->stream));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+int lbib_get_marker_ids(lbib_ctx* c, uint32_t* ids) {
+    if (check_ctx(c) || (c->nm > 0 && !ids)) return -1;
+    return download(c, ids, c->mid, c->nm * sizeof(uint32_t));
+}
+int lbib_get_connections(lbib_ctx* c, int32_t* a, int32_t* b, uint32_t* dom, int32_t* bc) {
+    if (check_ctx(c)) return -1;
+    const size_t n = (size_t)c->nc;
+    if ((a && download(c, a, c->ca, n * sizeof(int32_t))) || (b && download(c, b, c->cb, n * sizeof(int32_t))) ||
+        (dom && download(c, dom, c->cdom, n * sizeof(uint32_t))) || (bc && download(c, bc, c->cbc, n * sizeof(int32_t))))
+        return -1;
+    return 0;
+}
+int lbib_get_forces(lbib_ctx* c, int32_t* off, int32_t* type, int32_t* partner, double* prm) {
+    if (check_ctx(c) || !off) return -1;
+    if (download(c, off, c->fs_off, (c->nm + 1) * sizeof(int32_t))) return -1;
+    const int nf = off[c->nm];
+    if (nf > c->fcap) { c->err = "force table larger than its capacity"; return -1; }
+    if ((type && download(c, type, c->fs_type, nf * sizeof(int32_t))) ||
+        (partner && download(c, partner, c->fs_partner, nf * sizeof(int32_t))) ||
+        (prm && download(c, prm, c->fs_prm, 4 * (size_t)nf * sizeof(double))))
+        return -1;
     return 0;
 }
 int lbib_get_markers(lbib_ctx* c, double* xy, double* vxy, double* fxy, double* oxy) {
@@ -398,6 +485,15 @@ int lbib_set_connections(lbib_ctx* c, int nc, const int32_t* a, const int32_t* b
     if (!bcp) { none.assign(std::max(nc, 1), -1); bcp = none.data(); }
     if (upload(c, c->cbc, bcp, nc * sizeof(int32_t))) return -1;
     c->nc = nc;
+    uint32_t maxdom = 0;
+    for (int k = 0; k < nc; k++) maxdom = std::max(maxdom, dom[k]);
+    if ((int)maxdom + 1 > c->ndom) {
+        if (ensure_domains(c, (int)maxdom + 1)) return -1;
+        c->ndom = (int)maxdom + 1;
+    }
+    if (put_counts(c)) return -1;
+    // marker in/out links and ring order of the new topology
+    if (launch_inout(c) || launch_rings(c)) return -1;
     LBIB_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -419,36 +515,6 @@ int lbib_set_bc_descriptors(lbib_ctx* c, int nd, const int32_t* off, const int32
     LBIB_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
-int lbib_set_rings(lbib_ctx* c, const int32_t* prev, const int32_t* next, int nring, const uint32_t* ring_dom,
-                   const int32_t* ring_off, const int32_t* ring_marker) {
-    if (check_ctx(c) || nring < 0) return -1;
-    if (prev && next && c->nm > 0) {
-        if (upload(c, c->mprev, prev, c->nm * sizeof(int32_t)) || upload(c, c->mnext, next, c->nm * sizeof(int32_t)))
-            return -1;
-    }
-    if (nring > 0) {
-        const int tot = ring_off[nring];
-        if (nring + 1 > c->rcap) {
-            dev_free(c->ring_dom); dev_free(c->ring_off); dev_free(c->area);
-            c->rcap = std::max(nring + 1, 2 * c->rcap);
-            if (dev_alloc(c, &c->ring_dom, c->rcap) || dev_alloc(c, &c->ring_off, c->rcap) ||
-                dev_alloc(c, &c->area, c->rcap))
-                return -1;
-        }
-        if (tot > c->rmcap) {
-            dev_free(c->ring_marker);
-            c->rmcap = std::max(tot, 2 * c->rmcap);
-            if (dev_alloc(c, &c->ring_marker, c->rmcap)) return -1;
-        }
-        if (upload(c, c->ring_dom, ring_dom, nring * sizeof(uint32_t)) ||
-            upload(c, c->ring_off, ring_off, (nring + 1) * sizeof(int32_t)) ||
-            upload(c, c->ring_marker, ring_marker, tot * sizeof(int32_t)))
-            return -1;
-    }
-    c->nring = nring;
-    LBIB_HIP(hipStreamSynchronize(c->stream));
-    return 0;
-}
 int lbib_set_celltypes(lbib_ctx* c, int ndom, const uint32_t* celltype) {
     if (check_ctx(c) || ndom < 0 || (ndom > 0 && !celltype)) return -1;
     if (ensure_domains(c, ndom)) return -1;
@@ -466,13 +532,7 @@ int lbib_set_forces(lbib_ctx* c, int nm, const int32_t* off, const int32_t* type
         const bool pair = type[j] == 0 || type[j] == 5 || type[j] == 6 || type[j] == 7;
         if (pair && (partner[j] < 0 || partner[j] >= nm)) { c->err = "Could not compute and set the forces."; return -1; }
     }
-    if (nf > c->fcap) {
-        dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
-        c->fcap = std::max(nf, 2 * c->fcap);
-        if (dev_alloc(c, &c->fs_type, c->fcap) || dev_alloc(c, &c->fs_partner, c->fcap) ||
-            dev_alloc(c, &c->fs_prm, 4 * (size_t)c->fcap))
-            return -1;
-    }
+    if (ensure_force_cap(c, nf)) return -1;
     if (upload(c, c->fs_off, off, (nm + 1) * sizeof(int32_t)) || upload(c, c->fs_type, type, nf * sizeof(int32_t)) ||
         upload(c, c->fs_partner, partner, nf * sizeof(int32_t)) || upload(c, c->fs_prm, prm, 4 * nf * sizeof(double)))
         return -1;
@@ -515,9 +575,22 @@ int lbib_step_lattice_only(lbib_ctx* c, uint32_t it) {
     if (check_ctx(c)) return -1;
     return stream_collide(c, it, false);
 }
-int lbib_step_geometry(lbib_ctx* c, uint32_t it, int check_coarsen) {
+int lbib_step_geometry(lbib_ctx* c, uint32_t it, int coarsen) {
     if (check_ctx(c)) return -1;
-    return launch_geometry(c, it, check_coarsen ? 2 : 0);
+    return launch_geometry(c, it, coarsen ? 2 : 0);
+}
+int lbib_step_topology(lbib_ctx* c, int coarsen) {
+    if (check_ctx(c)) return -1;
+    return launch_topology(c, coarsen != 0);
+}
+int lbib_forces_tension(lbib_ctx* c, double f) {
+    if (check_ctx(c)) return -1;
+    if (ensure_force_cap(c, 2 * c->mcap)) return -1;
+    return launch_forces_tension(c, f);
+}
+int lbib_forces_clear(lbib_ctx* c) {
+    if (check_ctx(c)) return -1;
+    return launch_forces_clear(c);
 }
 int lbib_cure_lattice(lbib_ctx* c) {
     if (check_ctx(c)) return -1;
@@ -559,6 +632,12 @@ int lbib_read_status(lbib_ctx* c, lbib_status* out) {
     out->n_boundary_nodes = (int32_t)s.nbn;
     out->n_reinit = (int32_t)s.nchanged;
     out->n_candidates = (int32_t)s.ncand;
+    out->n_markers = (int32_t)s.nm;
+    out->n_connections = (int32_t)s.nc;
+    out->n_remeshed = (int32_t)s.rm_total;
+    out->n_coarsened = (int32_t)s.co_removed;
+    c->nm = (int)s.nm;
+    c->nc = (int)s.nc;
     return 0;
 }
 int lbib_clear_status(lbib_ctx* c) {
@@ -573,8 +652,14 @@ int lbib_clear_status(lbib_ctx* c) {
 // -------------------------------------------------------------- bio helpers --
 int lbib_compute_areas(lbib_ctx* c, double* area) {
     if (check_ctx(c)) return -1;
-    if (launch_areas(c)) return -1;
-    if (area && c->nring > 0) return download(c, area, c->area, c->nring * sizeof(double));
+    if (launch_ring_areas(c)) return -1;
+    if (area && c->ndom > 0) {
+        std::vector<uint32_t> cnt(c->ndom);
+        if (download(c, cnt.data(), c->ring_cnt, c->ndom * sizeof(uint32_t))) return -1;
+        if (download(c, area, c->area, c->ndom * sizeof(double))) return -1;
+        for (int d = 0; d < c->ndom; d++)
+            if (!cnt[d]) area[d] = 0.0;
+    }
     LBIB_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }

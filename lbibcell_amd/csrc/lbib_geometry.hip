@@ -340,58 +340,18 @@ __global__ __launch_bounds__(256) void k_scanline(int nx, long n, uint32_t* tag,
     }
 }
 
-// remeshBoundary trigger (GeometryHandler.cpp:393): a segment longer than MAXLENGTH.
-__global__ void k_remesh_check(int nc, const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
-                               const double* __restrict__ mx, const double* __restrict__ my, DevStatus* st) {
-    if (halted(st)) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const int a = ca[c], b = cb[c];
-    const double len = sqrt((mx[a] - mx[b]) * (mx[a] - mx[b]) + (my[a] - my[b]) * (my[a] - my[b]));
-    if (len > kMAXLENGTH) atomicOr(&st->bits, LBIB_FLAG_REMESH);
-}
-
-// coarsenBoundary trigger (GeometryHandler.cpp:444-449): the first removal of the
-// sequential sweep happens iff some marker's neighbours are closer than MAXLENGTH.
-__global__ void k_coarsen_check(int nm, const int32_t* __restrict__ prev, const int32_t* __restrict__ next,
-                                const double* __restrict__ mx, const double* __restrict__ my, DevStatus* st) {
-    if (halted(st)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nm) return;
-    const int p = prev[k], q = next[k];
-    if (p < 0 || q < 0) return;
-    const double len = sqrt((mx[q] - mx[p]) * (mx[q] - mx[p]) + (my[q] - my[p]) * (my[q] - my[p]));
-    if (len < kMAXLENGTH) atomicOr(&st->bits, LBIB_FLAG_COARSEN);
-}
-
-// Shoelace area per ring, FP64, in ring order (GeometryHandler.cpp:101-134).
-__global__ void k_areas(int nring, const int32_t* __restrict__ off, const int32_t* __restrict__ rm,
-                        const double* __restrict__ mx, const double* __restrict__ my, double* area) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nring) return;
-    const int b = off[r], e = off[r + 1];
-    double sum = 0.0;
-    for (int k = b; k < e; k++) {
-        const int k1 = (k + 1 == e) ? b : k + 1;
-        sum += mx[rm[k]] * my[rm[k1]] - mx[rm[k1]] * my[rm[k]];
-    }
-    area[r] = sum * 0.5;
-}
-
 // tutorial_01_BioSolverAreaRegulator.cpp:66-99, per domain.
 __global__ void k_mass_default(int ndom, const uint32_t* __restrict__ celltype, double maxs, double* mtab) {
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndom) return;
     mtab[d] = (d == 0) ? 0.0 : ((celltype[d] == 1u) ? maxs : 0.0);
 }
-__global__ void k_mass_regulated(int nring, const uint32_t* __restrict__ ring_dom, const double* __restrict__ area,
+__global__ void k_mass_regulated(const uint32_t* __restrict__ ring_cnt, const double* __restrict__ area,
                                  const uint32_t* __restrict__ celltype, double* target, double target_area,
                                  double kp, double maxs, double factor, int ndom, double* mtab) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nring) return;
-    const uint32_t d = ring_dom[r];
-    if ((int)d >= ndom) return;
-    const double A = area[r];
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ndom || ring_cnt[d] == 0) return;
+    const double A = area[d];
     if (isnan(target[d])) target[d] = (factor > 0.0) ? factor * A : target_area;
     double src = kp * (target[d] - A);
     if (src > maxs) src = maxs;
@@ -403,6 +363,8 @@ template <int NS>
 static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
     k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rkeys, c->st);
 }
+
+int launch_topology(lbib_ctx* c, bool coarsen);
 
 int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     // mode: 0 = moveLattice (local update + reinit), 1 = cureLattice (scanline only),
@@ -439,31 +401,30 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
         }
     }
     k_scanline<<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, c->tag, c->bn_dom, c->st);
-    if (c->nc > 0)
-        k_remesh_check<<<(c->nc + bs - 1) / bs, bs, 0, c->stream>>>(c->nc, c->ca, c->cb, c->mx, c->my, c->st);
-    if (coarsen && c->nm > 0)
-        k_coarsen_check<<<(c->nm + bs - 1) / bs, bs, 0, c->stream>>>(c->nm, c->mprev, c->mnext, c->mx, c->my,
-                                                                     c->st);
     LBIB_HIP(hipGetLastError());
-    return 0;
+    if (cure) return 0;
+    return launch_topology(c, coarsen);
 }
 
-int launch_areas(lbib_ctx* c) {
-    if (c->nring == 0) return 0;
-    k_areas<<<(c->nring + 255) / 256, 256, 0, c->stream>>>(c->nring, c->ring_off, c->ring_marker, c->mx, c->my,
-                                                          c->area);
-    LBIB_HIP(hipGetLastError());
+// remeshBoundary, then at it % 10 == 0 deleteAllForces + coarsenBoundary
+// (SimulationRunner.cpp:83-87); rings are rebuilt for the area helpers.
+int launch_topology(lbib_ctx* c, bool coarsen) {
+    if (c->nc == 0) return 0;
+    if (launch_remesh(c, 2)) return -1;
+    if (launch_rings(c)) return -1;
+    if (coarsen) {
+        if (launch_coarsen(c)) return -1;
+        if (launch_rings(c)) return -1;
+    }
     return 0;
 }
 
 int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor) {
-    if (launch_areas(c)) return -1;
     if (c->ndom == 0) return 0;
+    if (launch_ring_areas(c)) return -1;
     k_mass_default<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->celltype, maxs, c->mtab);
-    if (c->nring > 0)
-        k_mass_regulated<<<(c->nring + 255) / 256, 256, 0, c->stream>>>(c->nring, c->ring_dom, c->area, c->celltype,
-                                                                       c->target, target, kp, maxs, factor,
-                                                                       c->ndom, c->mtab);
+    k_mass_regulated<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ring_cnt, c->area, c->celltype, c->target,
+                                                                   target, kp, maxs, factor, c->ndom, c->mtab);
     LBIB_HIP(hipGetLastError());
     c->mtab_live = true;
     return 0;

@@ -50,6 +50,11 @@ struct DevStatus {
     uint32_t ncand_prev;    // ... of the previous step (reset list)
     uint32_t nbn;           // winners
     uint32_t nchanged;      // sites whose local domain changed
+    uint32_t nm, nc, maxid; // topology counts and the largest GeometryNode id
+    uint32_t rm_L;          // long connections in the current remesh round
+    uint32_t rm_total;      // connections split this step
+    uint32_t co_any, co_removed;
+    uint32_t co_nkeepc, co_nkeepm, co_nkeepr;
 };
 
 struct SpreadEntry {
@@ -97,7 +102,13 @@ struct lbib_ctx {
     int mcap = 0, nm = 0;
     double *mx = nullptr, *my = nullptr, *mvx = nullptr, *mvy = nullptr;
     double *mfx = nullptr, *mfy = nullptr, *mox = nullptr, *moy = nullptr;
-    int32_t *mprev = nullptr, *mnext = nullptr;
+    uint32_t* mid = nullptr;                   // GeometryNode id per dense index
+    int32_t *min_c = nullptr, *mout_c = nullptr;   // incoming / outgoing connection
+    double *sx_m = nullptr, *sy_m = nullptr, *svx_m = nullptr, *svy_m = nullptr, *sox_m = nullptr, *soy_m = nullptr;
+    uint32_t* sid_m = nullptr;
+    uint32_t *mdead = nullptr, *rec_alive = nullptr, *keep_m = nullptr, *pos_m = nullptr, *keep_r = nullptr,
+             *pos_r = nullptr, *mdom = nullptr;
+    int32_t *rec_a = nullptr, *rec_b = nullptr, *rec_bc = nullptr;
     // forces (CSR by marker)
     int fcap = 0;
     int32_t *fs_off = nullptr, *fs_type = nullptr, *fs_partner = nullptr;
@@ -106,6 +117,14 @@ struct lbib_ctx {
     int ccap = 0, nc = 0;
     int32_t *ca = nullptr, *cb = nullptr, *cbc = nullptr;
     uint32_t* cdom = nullptr;
+    int32_t *ca2 = nullptr, *cb2 = nullptr, *cbc2 = nullptr;   // compaction scratch
+    uint32_t* cdom2 = nullptr;
+    uint32_t *flag_c = nullptr, *rank_c = nullptr, *cdead = nullptr, *keep_c = nullptr, *pos_c = nullptr;
+    uint32_t* ring_slot = nullptr;
+    int32_t* ring_bucket = nullptr;
+    int32_t* ring_marker = nullptr;
+    uint32_t* scan_tmp = nullptr;
+    uint32_t scan_cap = 0;
     int bcd_cap = 0;
     int32_t *bcd_off = nullptr, *bcd_sp = nullptr;
     int nbcd = 0;
@@ -120,12 +139,11 @@ struct lbib_ctx {
     int32_t* bc_desc = nullptr;                // [4][n]
     uint32_t* changed = nullptr;               // [REINIT_CAP][2]
     unsigned long long* rkeys = nullptr;       // [REINIT_CAP] sort scratch
-    // rings / domains
-    int rcap = 0, nring = 0, rmcap = 0;
-    uint32_t* ring_dom = nullptr;
-    int32_t *ring_off = nullptr, *ring_marker = nullptr;
-    double* area = nullptr;
+    // rings / domains (indexed by domain id)
     int dcap = 0, ndom = 0;
+    int32_t* ring_start = nullptr;
+    uint32_t *ring_cnt = nullptr, *ring_off_u = nullptr;
+    double* area = nullptr;
     uint32_t* celltype = nullptr;
     double* target = nullptr;
     double* mtab = nullptr;
@@ -149,8 +167,17 @@ int launch_force_and_spread(lbib_ctx* c);
 int launch_clear_force(lbib_ctx* c);
 int launch_interp_move(lbib_ctx* c);
 int launch_geometry(lbib_ctx* c, uint32_t it, int mode);
-int launch_areas(lbib_ctx* c);
 int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor);
+int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
+                   uint32_t* total);
+int launch_inout(lbib_ctx* c);
+int launch_rings(lbib_ctx* c);
+int launch_ring_areas(lbib_ctx* c);
+int launch_remesh(lbib_ctx* c, int rounds);
+int launch_coarsen(lbib_ctx* c);
+int launch_forces_clear(lbib_ctx* c);
+int launch_forces_tension(lbib_ctx* c, double f);
+int launch_topology(lbib_ctx* c, bool coarsen);
 // FP64 <-> storage conversion of the current buffers (mixed precision only)
 int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq);
 int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq);

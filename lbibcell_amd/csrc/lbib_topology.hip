@@ -1,0 +1,683 @@
+// lbib_topology.hip -- membrane topology kept on the device: remeshing,
+// coarsening, ring order for the cell areas, marker in/out links and the
+// membrane-tension force table.
+//
+// Remesh (GeometryHandler::remeshBoundary, GeometryHandler.cpp:385-431): every
+// connection longer than MAXLENGTH is erased from connections_ and its two
+// halves are appended in visiting order, with new GeometryNode ids max+1,
+// max+2, ... .  One round = flag, exclusive scan, scatter: survivors keep
+// their relative order, the halves of the r-th long connection land at
+// (nc - L) + 2r and (nc - L) + 2r + 1.
+//
+// Coarsen (GeometryHandler::coarsenBoundary, :433-461): markers are visited in
+// id order; removing one splices its neighbours with a new connection appended
+// to the vector.  Rings do not share markers, so each ring is walked by one
+// lane in id order inside LDS; the appended connections are then ordered by
+// the id of the marker whose removal created them (the global visiting order),
+// and those erased again by a later removal are dropped.
+//
+// Ring order (GeometryHandler::computeAreas, :101-134): the polygon of a domain
+// starts at its first connection in vector order and follows the outgoing
+// links; positions come from pointer jumping in LDS.
+#include <algorithm>
+#include <climits>
+#include "lbib_device_math.h"
+
+namespace lbib {
+
+__device__ __forceinline__ bool topo_halted(const DevStatus* st) {
+    return (st->bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) != 0u;
+}
+
+// ------------------------------------------------------------------ scan ----
+// Exclusive scan of n (device-resident count) uint32 values, blocks of 1024.
+constexpr int SCAN_B = 1024;
+
+__global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* __restrict__ in, const uint32_t* nptr, uint32_t nfix,
+                                                     uint32_t* __restrict__ bsum) {
+    const uint32_t n = nptr ? *nptr : nfix;
+    const uint32_t b0 = blockIdx.x * SCAN_B;
+    __shared__ uint32_t red[256];
+    uint32_t s = 0;
+    if (b0 < n)
+        for (uint32_t k = threadIdx.x; k < SCAN_B; k += 256) {
+            const uint32_t i = b0 + k;
+            if (i < n) s += in[i];
+        }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+}
+
+// single block: exclusive scan of the block sums; total -> *total
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* bsum, uint32_t nb, uint32_t* total) {
+    __shared__ uint32_t s[1024];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = (i < nb) ? bsum[i] : 0u;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint32_t t = (threadIdx.x >= (uint32_t)o) ? s[threadIdx.x - o] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
+        __syncthreads();
+        carry += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, const uint32_t* nptr, uint32_t nfix,
+                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ out) {
+    const uint32_t n = nptr ? *nptr : nfix;
+    const uint32_t b0 = blockIdx.x * SCAN_B;
+    if (b0 >= n) return;
+    __shared__ uint32_t s[256];
+    // each thread owns 4 consecutive elements
+    uint32_t v[4];
+    uint32_t loc = 0;
+    for (int q = 0; q < 4; q++) {
+        const uint32_t i = b0 + threadIdx.x * 4 + q;
+        v[q] = (i < n) ? in[i] : 0u;
+        loc += v[q];
+    }
+    s[threadIdx.x] = loc;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+        const uint32_t t = (threadIdx.x >= (uint32_t)o) ? s[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = bsum[blockIdx.x] + s[threadIdx.x] - loc;
+    for (int q = 0; q < 4; q++) {
+        const uint32_t i = b0 + threadIdx.x * 4 + q;
+        if (i < n) out[i] = run;
+        run += v[q];
+    }
+}
+
+int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
+                   uint32_t* total) {
+    const uint32_t nb = (cap + SCAN_B - 1) / SCAN_B;
+    if (nb == 0) return 0;
+    if (nb > c->scan_cap) {
+        c->err = "scan scratch too small";
+        return -1;
+    }
+    k_scan_blocks<<<nb, 256, 0, c->stream>>>(in, nptr, nfix, c->scan_tmp);
+    k_scan_top<<<1, 1024, 0, c->stream>>>(c->scan_tmp, nb, total);
+    k_scan_apply<<<nb, 256, 0, c->stream>>>(in, nptr, nfix, c->scan_tmp, out);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------- in/out ------
+__global__ void k_inout(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb, int32_t* min_c,
+                        int32_t* mout_c, const DevStatus* st) {
+    const uint32_t nc = st->nc;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        mout_c[ca[j]] = (int32_t)j;
+        min_c[cb[j]] = (int32_t)j;
+    }
+}
+
+// ------------------------------------------------------------- remesh ------
+// Perturbation trigger of the GeometryNode ctor (GeometryNode.cpp:153-214)
+__device__ __forceinline__ bool needs_perturbation(const double X, const double Y) {
+    const bool p1 = fabs((double)(int)X - X) <= kEPS;
+    const bool p2 = fabs((double)(int)Y - Y) <= kEPS;
+    const bool p3 = fabs(Y - floor(Y) - (X - floor(X))) <= kEPS;
+    const bool p4 = fabs(ceil(Y) - Y - (X - floor(X))) <= kEPS;
+    return p1 || p2 || p3 || p4;
+}
+
+__global__ void k_rm_flag(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb, const double* __restrict__ mx,
+                          const double* __restrict__ my, uint32_t* flag, DevStatus* st) {
+    if (topo_halted(st)) return;
+    const uint32_t nc = st->nc;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const int a = ca[c], b = cb[c];
+        const double len = sqrt((mx[a] - mx[b]) * (mx[a] - mx[b]) + (my[a] - my[b]) * (my[a] - my[b]));
+        const bool lg = len > kMAXLENGTH;
+        flag[c] = lg ? 1u : 0u;
+        if (lg && needs_perturbation((mx[a] + mx[b]) / 2, (my[a] + my[b]) / 2))
+            atomicOr(&st->bits, LBIB_HALT_PERTURB_REMESH);
+    }
+}
+
+struct TopoArrays {
+    double *mx, *my, *mvx, *mvy, *mox, *moy;
+    uint32_t* mid;
+    int32_t *ca, *cb, *cbc;
+    uint32_t* cdom;
+    int32_t *ca2, *cb2, *cbc2;
+    uint32_t* cdom2;
+    int32_t* fs_off;
+};
+
+// Scatter one remesh round into the scratch connection arrays and append the
+// midpoint markers.  L = number of long connections (st->rm_L).
+__global__ void k_rm_apply(TopoArrays t, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank, int nx,
+                           int ny, DevStatus* st) {
+    if (topo_halted(st)) return;
+    const uint32_t L = st->rm_L;
+    if (L == 0) return;
+    const uint32_t nc = st->nc, nm = st->nm, maxid = st->maxid;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const uint32_t r = rank[c];
+        if (!flag[c]) {
+            const uint32_t p = c - r;
+            t.ca2[p] = t.ca[c]; t.cb2[p] = t.cb[c]; t.cdom2[p] = t.cdom[c]; t.cbc2[p] = t.cbc[c];
+            continue;
+        }
+        const int a = t.ca[c], b = t.cb[c];
+        const uint32_t k = nm + r;
+        const double x = (t.mx[a] + t.mx[b]) / 2, y = (t.my[a] + t.my[b]) / 2;
+        t.mx[k] = x; t.my[k] = y;
+        t.mvx[k] = 0.0; t.mvy[k] = 0.0;
+        t.mox[k] = x - 5.0; t.moy[k] = y - 5.0;   // GeometryNode.cpp:102
+        t.mid[k] = maxid + 1 + r;
+        // the new node is linked at once (update needed): boundary checks of
+        // GeometryHandler.cpp:613-636
+        if (x - 2.0 < 0 || y - 2.0 < 0 || x + 2.0 > nx || y + 2.0 > ny) {
+            atomicOr(&st->bits, LBIB_ERR_EDGE);
+            atomicMin(&st->first_bad, (int)k);
+        }
+        const uint32_t p = (nc - L) + 2 * r;
+        t.ca2[p] = a; t.cb2[p] = (int32_t)k; t.cdom2[p] = t.cdom[c]; t.cbc2[p] = t.cbc[c];
+        t.ca2[p + 1] = (int32_t)k; t.cb2[p + 1] = b; t.cdom2[p + 1] = t.cdom[c]; t.cbc2[p + 1] = t.cbc[c];
+    }
+}
+
+__global__ void k_rm_commit(TopoArrays t, DevStatus* st) {
+    if (topo_halted(st)) return;
+    const uint32_t L = st->rm_L;
+    if (L == 0) return;
+    const uint32_t nn = st->nc + L;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nn; j += gridDim.x * blockDim.x) {
+        t.ca[j] = t.ca2[j]; t.cb[j] = t.cb2[j]; t.cdom[j] = t.cdom2[j]; t.cbc[j] = t.cbc2[j];
+    }
+    // new markers start with no force structs
+    const uint32_t nm = st->nm;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < L; r += gridDim.x * blockDim.x)
+        t.fs_off[nm + 1 + r] = t.fs_off[nm];
+}
+
+__global__ void k_rm_counts(DevStatus* st, uint32_t mcap, uint32_t ccap) {
+    if (topo_halted(st)) return;
+    const uint32_t L = st->rm_L;
+    st->rm_total += L;
+    st->nm += L;
+    st->nc += L;
+    st->maxid += L;
+    if (st->nm + 1 > mcap || st->nc > ccap) atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+}
+
+// capacity guard before a round: the round may add up to nc markers
+__global__ void k_rm_guard(DevStatus* st, uint32_t mcap, uint32_t ccap) {
+    if (st->nm + st->nc + 1 > mcap || 2 * st->nc > ccap) atomicOr(&st->bits, LBIB_HALT_GROW);
+}
+
+__global__ void k_rm_pending(const uint32_t* __restrict__ flag, DevStatus* st) {
+    if (topo_halted(st)) return;
+    const uint32_t nc = st->nc;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        if (flag[c]) { atomicOr(&st->bits, LBIB_FLAG_REMESH); return; }
+}
+
+__global__ void k_rm_reset(DevStatus* st) {
+    st->rm_total = 0;
+    st->co_removed = 0;
+}
+
+// ------------------------------------------------------------- rings -------
+__global__ void k_ring_reset(int ndom, int32_t* start, uint32_t* cnt) {
+    const int d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= ndom) return;
+    start[d] = INT_MAX;
+    cnt[d] = 0;
+}
+__global__ void k_ring_count(const uint32_t* __restrict__ cdom, int32_t* start, uint32_t* cnt, uint32_t* slot,
+                             int ndom, DevStatus* st) {
+    const uint32_t nc = st->nc;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const uint32_t d = cdom[c];
+        if ((int)d >= ndom) { atomicOr(&st->bits, LBIB_ERR_TOPOLOGY); continue; }
+        atomicMin(&start[d], (int32_t)c);
+        slot[c] = atomicAdd(&cnt[d], 1u);
+    }
+}
+__global__ void k_ring_bucket(const uint32_t* __restrict__ cdom, const uint32_t* __restrict__ slot,
+                              const uint32_t* __restrict__ roff, int32_t* bucket, DevStatus* st) {
+    const uint32_t nc = st->nc;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        bucket[roff[cdom[c]] + slot[c]] = (int32_t)c;
+}
+
+constexpr int RING_LDS = 4096;
+
+// one workgroup per domain: pointer jumping on the ring broken at its start
+__global__ __launch_bounds__(1024) void k_ring_order(const int32_t* __restrict__ start, const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ roff, const int32_t* __restrict__ bucket,
+                                                     const uint32_t* __restrict__ slot, const int32_t* __restrict__ cb,
+                                                     const int32_t* __restrict__ mout_c, int32_t* ring_marker,
+                                                     DevStatus* st) {
+    const int d = blockIdx.x;
+    const uint32_t n = cnt[d];
+    if (n == 0) return;
+    if (n > (uint32_t)RING_LDS) {
+        if (threadIdx.x == 0) atomicOr(&st->bits, LBIB_ERR_TOPOLOGY);
+        return;
+    }
+    __shared__ int32_t nxt[2][RING_LDS];
+    __shared__ int32_t dist[2][RING_LDS];
+    __shared__ int any_live;
+    const uint32_t base = roff[d];
+    const int32_t s0 = start[d];
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int32_t c = bucket[base + i];
+        const int32_t nc_ = mout_c[cb[c]];
+        nxt[0][i] = (nc_ == s0) ? -1 : (int32_t)slot[nc_];
+        dist[0][i] = (nc_ == s0) ? 0 : 1;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int round = 0; round < 14; round++) {
+        if (threadIdx.x == 0) any_live = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const int32_t j = nxt[cur][i];
+            if (j >= 0) {
+                dist[1 - cur][i] = dist[cur][i] + dist[cur][j];
+                nxt[1 - cur][i] = nxt[cur][j];
+                if (nxt[cur][j] >= 0) any_live = 1;
+            } else {
+                dist[1 - cur][i] = dist[cur][i];
+                nxt[1 - cur][i] = -1;
+            }
+        }
+        __syncthreads();
+        cur = 1 - cur;
+        if (!any_live) break;
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        if (nxt[cur][i] >= 0 || dist[cur][i] >= (int32_t)n) {
+            atomicOr(&st->bits, LBIB_ERR_TOPOLOGY);   // open or multiple rings in one domain
+            continue;
+        }
+        const uint32_t pos = n - 1 - (uint32_t)dist[cur][i];
+        ring_marker[base + pos] = cb[bucket[base + i]];
+    }
+}
+
+// Shoelace area in ring order (FP64, sequential sum as the reference's
+// boost::geometry::area call site walks the ring), vertices staged in LDS.
+__global__ __launch_bounds__(256) void k_ring_area(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ roff,
+                                                   const int32_t* __restrict__ ring_marker, const double* __restrict__ mx,
+                                                   const double* __restrict__ my, double* area) {
+    const int d = blockIdx.x;
+    const uint32_t n = cnt[d];
+    if (n == 0) return;
+    constexpr int CH = 1024;
+    __shared__ double sx[CH + 1], sy[CH + 1];
+    const uint32_t base = roff[d];
+    double sum = 0.0;
+    const double x0 = mx[ring_marker[base]], y0 = my[ring_marker[base]];
+    for (uint32_t c0 = 0; c0 < n; c0 += CH) {
+        const uint32_t m = min((uint32_t)CH, n - c0);
+        for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
+            const int32_t q = ring_marker[base + c0 + k];
+            sx[k] = mx[q];
+            sy[k] = my[q];
+        }
+        if (threadIdx.x == 0) {
+            const bool last = (c0 + m == n);
+            sx[m] = last ? x0 : mx[ring_marker[base + c0 + m]];
+            sy[m] = last ? y0 : my[ring_marker[base + c0 + m]];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (uint32_t k = 0; k < m; k++) sum += sx[k] * sy[k + 1] - sx[k + 1] * sy[k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) area[d] = sum * 0.5;
+}
+
+// ------------------------------------------------------------- coarsen -----
+__global__ void k_co_reset(DevStatus* st) {
+    st->co_any = 0;
+    st->co_removed = 0;
+}
+
+__global__ void k_co_flag(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb, const int32_t* __restrict__ min_c,
+                          const int32_t* __restrict__ mout_c, const double* __restrict__ mx,
+                          const double* __restrict__ my, DevStatus* st) {
+    // coarsening runs only on a fully remeshed topology
+    if (topo_halted(st) || (st->bits & LBIB_FLAG_REMESH)) return;
+    const uint32_t nm = st->nm;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        const int p = ca[min_c[k]], q = cb[mout_c[k]];
+        const double len = sqrt((mx[q] - mx[p]) * (mx[q] - mx[p]) + (my[q] - my[p]) * (my[q] - my[p]));
+        if (len < kMAXLENGTH) st->co_any = 1u;
+    }
+}
+
+struct CoarsenArgs {
+    const uint32_t* cnt;
+    const uint32_t* roff;
+    const int32_t* ring_marker;   // ring members (any order is fine: sorted below)
+    const int32_t *ca, *cb, *cbc, *min_c, *mout_c;
+    const double *mx, *my;
+    uint32_t* cdead;     // [ccap]
+    uint32_t* mdead;     // [mcap]
+    int32_t *rec_a, *rec_b, *rec_bc;   // [mcap] indexed by the removed marker
+    uint32_t* rec_alive;               // [mcap]
+};
+
+// one workgroup per domain; lane 0 walks the ring's markers in id order
+__global__ __launch_bounds__(256) void k_co_ring(const CoarsenArgs a, DevStatus* st) {
+    if (topo_halted(st) || !st->co_any) return;
+    const int d = blockIdx.x;
+    const uint32_t n = a.cnt[d];
+    if (n == 0) return;
+    if (n > (uint32_t)RING_LDS) {
+        if (threadIdx.x == 0) atomicOr(&st->bits, LBIB_ERR_TOPOLOGY);
+        return;
+    }
+    __shared__ int32_t gid[RING_LDS];      // sorted global marker indices
+    __shared__ int16_t lprev[RING_LDS], lnext[RING_LDS];
+    __shared__ int32_t inc[RING_LDS], outc[RING_LDS];   // >=0 original conn, <0: -(creator local idx)-1
+    __shared__ int32_t inbc[RING_LDS];
+    uint32_t P = 1;
+    while (P < n) P <<= 1;
+    const uint32_t base = a.roff[d];
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) gid[i] = (i < n) ? a.ring_marker[base + i] : INT_MAX;
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= P; kk <<= 1)
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
+                const uint32_t l = t ^ j;
+                if (l > t) {
+                    const bool up = (t & kk) == 0;
+                    const int32_t u = gid[t], v = gid[l];
+                    if ((u > v) == up) { gid[t] = v; gid[l] = u; }
+                }
+            }
+            __syncthreads();
+        }
+    auto local = [&](int32_t g) -> int {
+        int lo = 0, hi = (int)n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (gid[mid] < g) lo = mid + 1; else hi = mid;
+        }
+        return lo;
+    };
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const int32_t g = gid[i];
+        inc[i] = a.min_c[g];
+        outc[i] = a.mout_c[g];
+        inbc[i] = a.cbc[a.min_c[g]];
+        lprev[i] = (int16_t)local(a.ca[a.min_c[g]]);
+        lnext[i] = (int16_t)local(a.cb[a.mout_c[g]]);
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    uint32_t removed = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const int p = lprev[i], q = lnext[i];
+        const double px = a.mx[gid[p]], py = a.my[gid[p]], qx = a.mx[gid[q]], qy = a.my[gid[q]];
+        const double len = sqrt((qx - px) * (qx - px) + (qy - py) * (qy - py));
+        if (!(len < kMAXLENGTH)) continue;
+        // Geometry::removeGeometryNode (Geometry.cpp:401-428)
+        const int32_t g = gid[i];
+        for (int s = 0; s < 2; s++) {
+            const int32_t cc = s == 0 ? inc[i] : outc[i];
+            if (cc >= 0) a.cdead[cc] = 1u;
+            else a.rec_alive[gid[-cc - 1]] = 0u;
+        }
+        a.mdead[g] = 1u;
+        a.rec_a[g] = gid[p];
+        a.rec_b[g] = gid[q];
+        a.rec_bc[g] = inbc[i];
+        a.rec_alive[g] = 1u;
+        lnext[p] = (int16_t)q;
+        lprev[q] = (int16_t)p;
+        outc[p] = -(int32_t)i - 1;
+        inc[q] = -(int32_t)i - 1;
+        inbc[q] = inbc[i];
+        removed++;
+    }
+    if (removed) atomicAdd(&st->co_removed, removed);
+}
+
+// keep flags for the compaction scans
+__global__ void k_co_keep(const uint32_t* __restrict__ cdead, const uint32_t* __restrict__ mdead,
+                          const uint32_t* __restrict__ rec_alive, uint32_t* keepc, uint32_t* keepm, uint32_t* keepr,
+                          DevStatus* st) {
+    if (topo_halted(st) || !st->co_removed) return;
+    const uint32_t nc = st->nc, nm = st->nm;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x)
+        keepc[j] = cdead[j] ? 0u : 1u;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        keepm[k] = mdead[k] ? 0u : 1u;
+        keepr[k] = (mdead[k] && rec_alive[k]) ? 1u : 0u;
+    }
+}
+
+// build the compacted connection list into the scratch arrays:
+// surviving originals in order, then alive records in removed-marker order
+__global__ void k_co_conns(TopoArrays t, const uint32_t* __restrict__ keepc, const uint32_t* __restrict__ posc,
+                           const uint32_t* __restrict__ keepr, const uint32_t* __restrict__ posr,
+                           const uint32_t* __restrict__ newm, const int32_t* __restrict__ rec_a,
+                           const int32_t* __restrict__ rec_b, const int32_t* __restrict__ rec_bc,
+                           const uint32_t* __restrict__ mdom, const uint32_t* nkeepc, DevStatus* st) {
+    if (topo_halted(st) || !st->co_removed) return;
+    const uint32_t nc = st->nc, nm = st->nm, ns = *nkeepc;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        if (!keepc[j]) continue;
+        const uint32_t p = posc[j];
+        t.ca2[p] = (int32_t)newm[t.ca[j]];
+        t.cb2[p] = (int32_t)newm[t.cb[j]];
+        t.cdom2[p] = t.cdom[j];
+        t.cbc2[p] = t.cbc[j];
+    }
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        if (!keepr[k]) continue;
+        const uint32_t p = ns + posr[k];
+        t.ca2[p] = (int32_t)newm[rec_a[k]];
+        t.cb2[p] = (int32_t)newm[rec_b[k]];
+        t.cdom2[p] = mdom[k];
+        t.cbc2[p] = rec_bc[k];
+    }
+}
+
+// domain of each marker = domain of its incoming connection (before compaction)
+__global__ void k_marker_dom(const int32_t* __restrict__ min_c, const uint32_t* __restrict__ cdom, uint32_t* mdom,
+                             DevStatus* st) {
+    if (topo_halted(st) || !st->co_removed) return;
+    const uint32_t nm = st->nm;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) mdom[k] = cdom[min_c[k]];
+}
+
+struct MarkerScratch {
+    double *x, *y, *vx, *vy, *ox, *oy;
+    uint32_t* id;
+};
+
+__global__ void k_co_markers(TopoArrays t, MarkerScratch s, const uint32_t* __restrict__ keepm,
+                             const uint32_t* __restrict__ newm, DevStatus* st) {
+    if (topo_halted(st) || !st->co_removed) return;
+    const uint32_t nm = st->nm;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        if (!keepm[k]) continue;
+        const uint32_t p = newm[k];
+        s.x[p] = t.mx[k]; s.y[p] = t.my[k]; s.vx[p] = t.mvx[k]; s.vy[p] = t.mvy[k];
+        s.ox[p] = t.mox[k]; s.oy[p] = t.moy[k]; s.id[p] = t.mid[k];
+    }
+}
+
+__global__ void k_co_commit(TopoArrays t, MarkerScratch s, const uint32_t* nkeepm, const uint32_t* nkeepc,
+                            const uint32_t* nkeepr, DevStatus* st) {
+    if (topo_halted(st) || !st->co_removed) return;
+    const uint32_t nm = *nkeepm, nc = *nkeepc + *nkeepr;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        t.mx[k] = s.x[k]; t.my[k] = s.y[k]; t.mvx[k] = s.vx[k]; t.mvy[k] = s.vy[k];
+        t.mox[k] = s.ox[k]; t.moy[k] = s.oy[k]; t.mid[k] = s.id[k];
+    }
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
+        t.ca[j] = t.ca2[j]; t.cb[j] = t.cb2[j]; t.cdom[j] = t.cdom2[j]; t.cbc[j] = t.cbc2[j];
+    }
+}
+
+__global__ void k_co_counts(const uint32_t* nkeepm, const uint32_t* nkeepc, const uint32_t* nkeepr, DevStatus* st) {
+    if (topo_halted(st) || !st->co_removed) return;
+    st->nm = *nkeepm;
+    st->nc = *nkeepc + *nkeepr;
+}
+
+__global__ void k_clear_u32(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// ForceSolver::deleteAllForces (ForceSolver.cpp:158-166): empty CSR rows
+__global__ void k_forces_clear(int32_t* fs_off, DevStatus* st) {
+    const uint32_t nm = st->nm;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= nm; k += gridDim.x * blockDim.x) fs_off[k] = 0;
+}
+
+// tutorial_01_BioSolverMembraneTension (…MembraneTension.cpp:53-76): after
+// deleteForceType(6) every marker holds exactly its two type-6 structs, one per
+// adjacent connection; with two terms the summation order does not matter.
+__global__ void k_forces_tension(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
+                                 const int32_t* __restrict__ min_c, const int32_t* __restrict__ mout_c, double f,
+                                 int32_t* fs_off, int32_t* fs_type, int32_t* fs_partner, double* fs_prm,
+                                 DevStatus* st) {
+    const uint32_t nm = st->nm;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        fs_off[k] = (int32_t)(2 * k);
+        if (k + 1 == nm) fs_off[nm] = (int32_t)(2 * nm);
+        const uint32_t e = 2 * k;
+        fs_type[e] = 6; fs_partner[e] = cb[mout_c[k]];
+        fs_type[e + 1] = 6; fs_partner[e + 1] = ca[min_c[k]];
+        for (int q = 0; q < 4; q++) { fs_prm[4 * e + q] = 0.0; fs_prm[4 * e + 4 + q] = 0.0; }
+        fs_prm[4 * e] = f;
+        fs_prm[4 * e + 4] = f;
+    }
+}
+
+// --------------------------------------------------------------- drivers ---
+static inline int grid_for(long n) { return (int)std::min<long>(std::max<long>((n + 255) / 256, 1), 4096); }
+
+static TopoArrays topo_arrays(lbib_ctx* c) {
+    TopoArrays t;
+    t.mx = c->mx; t.my = c->my; t.mvx = c->mvx; t.mvy = c->mvy; t.mox = c->mox; t.moy = c->moy; t.mid = c->mid;
+    t.ca = c->ca; t.cb = c->cb; t.cbc = c->cbc; t.cdom = c->cdom;
+    t.ca2 = c->ca2; t.cb2 = c->cb2; t.cbc2 = c->cbc2; t.cdom2 = c->cdom2;
+    t.fs_off = c->fs_off;
+    return t;
+}
+
+int launch_inout(lbib_ctx* c) {
+    k_inout<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->min_c, c->mout_c, c->st);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_rings(lbib_ctx* c) {
+    if (c->ndom == 0) return 0;
+    k_ring_reset<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->ring_start, c->ring_cnt);
+    k_ring_count<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->cdom, c->ring_start, c->ring_cnt, c->ring_slot, c->ndom,
+                                                          c->st);
+    if (scan_exclusive(c, c->ring_cnt, c->ring_off_u, nullptr, (uint32_t)c->ndom, (uint32_t)c->ndom, nullptr)) return -1;
+    k_ring_bucket<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->cdom, c->ring_slot, c->ring_off_u, c->ring_bucket,
+                                                           c->st);
+    k_ring_order<<<c->ndom, 1024, 0, c->stream>>>(c->ring_start, c->ring_cnt, c->ring_off_u, c->ring_bucket,
+                                                  c->ring_slot, c->cb, c->mout_c, c->ring_marker, c->st);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_ring_areas(lbib_ctx* c) {
+    if (c->ndom == 0) return 0;
+    k_ring_area<<<c->ndom, 256, 0, c->stream>>>(c->ring_cnt, c->ring_off_u, c->ring_marker, c->mx, c->my, c->area);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+// rounds of remeshing; the host repeats while LBIB_FLAG_REMESH comes back
+int launch_remesh(lbib_ctx* c, int rounds) {
+    TopoArrays t = topo_arrays(c);
+    k_rm_reset<<<1, 1, 0, c->stream>>>(c->st);
+    for (int r = 0; r < rounds; r++) {
+        k_rm_guard<<<1, 1, 0, c->stream>>>(c->st, (uint32_t)c->mcap, (uint32_t)c->ccap);
+        k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, c->st);
+        if (scan_exclusive(c, c->flag_c, c->rank_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->rm_L)) return -1;
+        k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->nx, c->ny, c->st);
+        k_rm_commit<<<grid_for(2 * (long)c->ccap), 256, 0, c->stream>>>(t, c->st);
+        k_rm_counts<<<1, 1, 0, c->stream>>>(c->st, (uint32_t)c->mcap, (uint32_t)c->ccap);
+        if (launch_inout(c)) return -1;
+    }
+    // anything still long: the host runs more rounds
+    k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, c->st);
+    k_rm_pending<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->flag_c, c->st);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_coarsen(lbib_ctx* c) {
+    TopoArrays t = topo_arrays(c);
+    k_co_reset<<<1, 1, 0, c->stream>>>(c->st);
+    k_co_flag<<<grid_for(c->mcap), 256, 0, c->stream>>>(c->ca, c->cb, c->min_c, c->mout_c, c->mx, c->my, c->st);
+    k_clear_u32<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->cdead, (uint32_t)c->ccap);
+    k_clear_u32<<<grid_for(c->mcap), 256, 0, c->stream>>>(c->mdead, (uint32_t)c->mcap);
+    CoarsenArgs a;
+    a.cnt = c->ring_cnt; a.roff = c->ring_off_u; a.ring_marker = c->ring_marker;
+    a.ca = c->ca; a.cb = c->cb; a.cbc = c->cbc; a.min_c = c->min_c; a.mout_c = c->mout_c;
+    a.mx = c->mx; a.my = c->my; a.cdead = c->cdead; a.mdead = c->mdead;
+    a.rec_a = c->rec_a; a.rec_b = c->rec_b; a.rec_bc = c->rec_bc; a.rec_alive = c->rec_alive;
+    if (c->ndom > 0) k_co_ring<<<c->ndom, 256, 0, c->stream>>>(a, c->st);
+    k_marker_dom<<<grid_for(c->mcap), 256, 0, c->stream>>>(c->min_c, c->cdom, c->mdom, c->st);
+    k_co_keep<<<grid_for(std::max(c->mcap, c->ccap)), 256, 0, c->stream>>>(c->cdead, c->mdead, c->rec_alive, c->keep_c,
+                                                                          c->keep_m, c->keep_r, c->st);
+    if (scan_exclusive(c, c->keep_c, c->pos_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->co_nkeepc)) return -1;
+    if (scan_exclusive(c, c->keep_m, c->pos_m, &c->st->nm, 0, (uint32_t)c->mcap, &c->st->co_nkeepm)) return -1;
+    if (scan_exclusive(c, c->keep_r, c->pos_r, &c->st->nm, 0, (uint32_t)c->mcap, &c->st->co_nkeepr)) return -1;
+    k_co_conns<<<grid_for(std::max(c->mcap, c->ccap)), 256, 0, c->stream>>>(
+        t, c->keep_c, c->pos_c, c->keep_r, c->pos_r, c->pos_m, c->rec_a, c->rec_b, c->rec_bc, c->mdom,
+        &c->st->co_nkeepc, c->st);
+    MarkerScratch s;
+    s.x = c->sx_m; s.y = c->sy_m; s.vx = c->svx_m; s.vy = c->svy_m; s.ox = c->sox_m; s.oy = c->soy_m; s.id = c->sid_m;
+    k_co_markers<<<grid_for(c->mcap), 256, 0, c->stream>>>(t, s, c->keep_m, c->pos_m, c->st);
+    k_co_commit<<<grid_for(std::max(c->mcap, c->ccap)), 256, 0, c->stream>>>(t, s, &c->st->co_nkeepm,
+                                                                            &c->st->co_nkeepc, &c->st->co_nkeepr, c->st);
+    k_co_counts<<<1, 1, 0, c->stream>>>(&c->st->co_nkeepm, &c->st->co_nkeepc, &c->st->co_nkeepr, c->st);
+    LBIB_HIP(hipGetLastError());
+    if (launch_inout(c)) return -1;
+    return 0;
+}
+
+int launch_forces_clear(lbib_ctx* c) {
+    k_forces_clear<<<grid_for(c->mcap + 1), 256, 0, c->stream>>>(c->fs_off, c->st);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_forces_tension(lbib_ctx* c, double f) {
+    k_forces_tension<<<grid_for(c->mcap), 256, 0, c->stream>>>(c->ca, c->cb, c->min_c, c->mout_c, f, c->fs_off,
+                                                               c->fs_type, c->fs_partner, c->fs_prm, c->st);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+}  // namespace lbib
