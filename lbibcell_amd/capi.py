@@ -23,9 +23,12 @@ ERR_EDGE = 0x2
 ERR_SCANLINE = 0x4
 ERR_SLOPE = 0x8
 ERR_CAPACITY = 0x10
+ERR_TOPOLOGY = 0x20
 ERR_MASK = 0xFF
 HALT_PERTURB_NODE = 0x100
 HALT_PERTURB_CONN = 0x200
+HALT_PERTURB_REMESH = 0x400
+HALT_GROW = 0x800
 HALT_MASK = 0xF00
 FLAG_REMESH = 0x1000
 FLAG_COARSEN = 0x2000
@@ -68,6 +71,10 @@ class Status(C.Structure):
         ("n_boundary_nodes", C.c_int32),
         ("n_reinit", C.c_int32),
         ("n_candidates", C.c_int32),
+        ("n_markers", C.c_int32),
+        ("n_connections", C.c_int32),
+        ("n_remeshed", C.c_int32),
+        ("n_coarsened", C.c_int32),
     ]
 
 
@@ -99,11 +106,17 @@ SIGNATURES = {
     "lbib_get_markers": (C.c_int, [_P, _D, _D, _D, _D]),
     "lbib_set_connections": (C.c_int, [_P, C.c_int, _I32, _I32, _U32, _I32]),
     "lbib_set_bc_descriptors": (C.c_int, [_P, C.c_int, _I32, _I32]),
-    "lbib_set_rings": (C.c_int, [_P, _I32, _I32, C.c_int, _U32, _I32, _I32]),
+    "lbib_get_marker_ids": (C.c_int, [_P, _U32]),
+    "lbib_set_marker_ids": (C.c_int, [_P, _U32]),
+    "lbib_get_connections": (C.c_int, [_P, _I32, _I32, _U32, _I32]),
+    "lbib_get_forces": (C.c_int, [_P, _I32, _I32, _I32, _D]),
     "lbib_set_celltypes": (C.c_int, [_P, C.c_int, _U32]),
     "lbib_set_forces": (C.c_int, [_P, C.c_int, _I32, _I32, _I32, _D]),
     "lbib_step_lattice": (C.c_int, [_P, C.c_uint32]),
     "lbib_step_geometry": (C.c_int, [_P, C.c_uint32, C.c_int]),
+    "lbib_step_topology": (C.c_int, [_P, C.c_int]),
+    "lbib_forces_tension": (C.c_int, [_P, C.c_double]),
+    "lbib_forces_clear": (C.c_int, [_P]),
     "lbib_cure_lattice": (C.c_int, [_P]),
     "lbib_set_box_outlet": (C.c_int, [_P, C.c_int]),
     "lbib_step_lattice_only": (C.c_int, [_P, C.c_uint32]),

@@ -34,8 +34,9 @@ def test_host_runtime_loads():
 def test_status_bits_match_header():
     from lbibcell_amd import capi
     text = open(os.path.join(ROOT, "include", "lbib.h")).read()
-    for name in ["ERR_VELOCITY", "ERR_EDGE", "ERR_SCANLINE", "ERR_SLOPE", "ERR_CAPACITY",
-                 "HALT_PERTURB_NODE", "HALT_PERTURB_CONN", "FLAG_REMESH", "FLAG_COARSEN"]:
+    for name in ["ERR_VELOCITY", "ERR_EDGE", "ERR_SCANLINE", "ERR_SLOPE", "ERR_CAPACITY", "ERR_TOPOLOGY",
+                 "HALT_PERTURB_NODE", "HALT_PERTURB_CONN", "HALT_PERTURB_REMESH", "HALT_GROW",
+                 "FLAG_REMESH", "FLAG_COARSEN"]:
         m = re.search(r"#define LBIB_%s\s+(0x[0-9a-fA-F]+)u" % name, text)
         assert m and int(m.group(1), 16) == getattr(capi, name), name
 
