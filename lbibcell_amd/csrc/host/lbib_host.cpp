@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iostream>
@@ -703,6 +704,11 @@ std::string SimulationRunner::describe(const lbib_status& s) {
     if (s.bits & LBIB_ERR_SCANLINE) m += "domainID at the left boundary must be 0.; ";
     if (s.bits & LBIB_ERR_SLOPE) m += "This can only happen if something went wrong with the perturbation of the connection.; ";
     if (s.bits & LBIB_ERR_CAPACITY) m += "device work list capacity exceeded; ";
+    if (s.bits & LBIB_ERR_TOPOLOGY) m += "a domain is not one closed membrane ring of at most 4096 markers; ";
+    char buf[160];
+    snprintf(buf, sizeof(buf), "[status bits 0x%x, markers %d, connections %d, iteration %u] ", s.bits, s.n_markers,
+             s.n_connections, params_.currentIteration);
+    m += buf;
     if (s.first_bad_marker >= 0 && s.first_bad_marker < (int)ids_.size())
         m += "GeometryNode ID=" + std::to_string(ids_[s.first_bad_marker]);
     return m;

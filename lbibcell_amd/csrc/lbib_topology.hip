@@ -308,7 +308,9 @@ __global__ __launch_bounds__(1024) void k_ring_order(const int32_t* __restrict__
         }
         __syncthreads();
         cur = 1 - cur;
-        if (!any_live) break;
+        const int live = any_live;
+        __syncthreads();   // every lane has read the flag before lane 0 clears it
+        if (!live) break;
     }
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         if (nxt[cur][i] >= 0 || dist[cur][i] >= (int32_t)n) {
