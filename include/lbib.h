@@ -149,6 +149,9 @@ int lbib_set_connections(lbib_ctx* ctx, int nc, const int32_t* a, const int32_t*
 /* Boundary-condition descriptors: descriptor d lists the species indices that
  * BoundarySolverNoFluxD2Q5 is connected to (BoundaryNode.cpp:43-46). */
 int lbib_set_bc_descriptors(lbib_ctx* ctx, int nd, const int32_t* off, const int32_t* species);
+/* Capacity for markers and connections (device topology grows by remeshing).
+ * Reallocation discards the device topology: upload it again afterwards. */
+int lbib_reserve(lbib_ctx* ctx, int max_markers, int max_connections);
 /* GeometryNode ids per dense marker index (needed once remeshing has added
  * markers on the device) and the current connection list. */
 int lbib_get_marker_ids(lbib_ctx* ctx, uint32_t* ids);

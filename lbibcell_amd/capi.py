@@ -106,6 +106,7 @@ SIGNATURES = {
     "lbib_get_markers": (C.c_int, [_P, _D, _D, _D, _D]),
     "lbib_set_connections": (C.c_int, [_P, C.c_int, _I32, _I32, _U32, _I32]),
     "lbib_set_bc_descriptors": (C.c_int, [_P, C.c_int, _I32, _I32]),
+    "lbib_reserve": (C.c_int, [_P, C.c_int, C.c_int]),
     "lbib_get_marker_ids": (C.c_int, [_P, _U32]),
     "lbib_set_marker_ids": (C.c_int, [_P, _U32]),
     "lbib_get_connections": (C.c_int, [_P, _I32, _I32, _U32, _I32]),

@@ -716,7 +716,7 @@ std::string SimulationRunner::describe(const lbib_status& s) {
 
 // Remeshing/coarsening on the host mirror, used when the device halts on a
 // perturbation draw or runs out of capacity (both rare).
-void SimulationRunner::hostTopologyFallback(unsigned it) {
+void SimulationRunner::hostTopologyFallback(unsigned it, bool grow) {
     auto t0 = std::chrono::steady_clock::now();
     check(lbib_clear_status(ctx_), "status");
     mirrorValid_ = false;
@@ -725,6 +725,10 @@ void SimulationRunner::hostTopologyFallback(unsigned it) {
     if (it % 10 == 0) {
         forceSolver_.deleteAllForces();
         geo_.coarsenBoundary(kMAXLENGTH);
+    }
+    if (grow) {
+        const int nm = (int)geo_.getGeometryNodes().size(), nc = (int)geo_.getConnections().size();
+        check(lbib_reserve(ctx_, 2 * nm + 1024, 2 * nc + 1024), "reserve");
     }
     uploadTopology();
     times.host += seconds_since(t0);
@@ -764,7 +768,7 @@ void SimulationRunner::handleGeometryStatus(unsigned it) {
             continue;
         }
         if (bits & (LBIB_HALT_PERTURB_REMESH | LBIB_HALT_GROW)) {
-            hostTopologyFallback(it);
+            hostTopologyFallback(it, (bits & LBIB_HALT_GROW) != 0);
             coarsened = true;
             break;
         }

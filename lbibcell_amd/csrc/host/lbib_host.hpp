@@ -190,7 +190,7 @@ class SimulationRunner {
     void check(int rc, const char* what);
     void uploadForces();
     void handleGeometryStatus(unsigned it);
-    void hostTopologyFallback(unsigned it);
+    void hostTopologyFallback(unsigned it, bool grow);
     std::string describe(const lbib_status& s);
 
     GlobalSimulationParameters params_;

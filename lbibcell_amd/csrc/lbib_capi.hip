@@ -123,6 +123,23 @@ int ensure_domains(lbib_ctx* c, int ndom) {
 }
 
 int check_ctx(lbib_ctx* c) { return c ? 0 : -1; }
+int upload(lbib_ctx* c, void* dst, const void* src, size_t bytes);
+int download(lbib_ctx* c, void* dst, const void* src, size_t bytes);
+
+// ensure_connections that keeps the previous cure's candidate list (the next
+// stage B clears those boundary-node bits) valid across the reallocation
+int grow_connections(lbib_ctx* c, int nc) {
+    std::vector<Cand> keep;
+    DevStatus s;
+    if (download(c, &s, c->st, sizeof(s))) return -1;
+    keep.resize(s.ncand);
+    if (s.ncand && download(c, keep.data(), c->cand, s.ncand * sizeof(Cand))) return -1;
+    if (ensure_connections(c, nc)) return -1;
+    if ((int)keep.size() > c->candcap) { c->err = "candidate list too long"; return -1; }
+    if (!keep.empty() && upload(c, c->cand, keep.data(), keep.size() * sizeof(Cand))) return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
 
 int ensure_force_cap(lbib_ctx* c, int nf) {
     if (nf <= c->fcap) return 0;
@@ -466,17 +483,7 @@ int lbib_set_connections(lbib_ctx* c, int nc, const int32_t* a, const int32_t* b
         if (dom[k] > DOM_MASK) { c->err = "domain id exceeds 24 bits"; return -1; }
         if (bc && bc[k] >= c->nbcd) { c->err = "connection refers to an unknown BC descriptor"; return -1; }
     }
-    if (nc > c->ccap) {
-        // keep the previous cure's candidate list valid across the reallocation
-        std::vector<Cand> keep;
-        DevStatus s;
-        if (download(c, &s, c->st, sizeof(s))) return -1;
-        keep.resize(s.ncand);
-        if (s.ncand && download(c, keep.data(), c->cand, s.ncand * sizeof(Cand))) return -1;
-        if (ensure_connections(c, nc)) return -1;
-        if ((int)keep.size() > c->candcap) { c->err = "candidate list too long"; return -1; }
-        if (!keep.empty() && upload(c, c->cand, keep.data(), keep.size() * sizeof(Cand))) return -1;
-    }
+    if (nc > c->ccap && grow_connections(c, nc)) return -1;
     if (upload(c, c->ca, a, nc * sizeof(int32_t)) || upload(c, c->cb, b, nc * sizeof(int32_t)) ||
         upload(c, c->cdom, dom, nc * sizeof(uint32_t)))
         return -1;
@@ -614,6 +621,16 @@ int lbib_profile_read(lbib_ctx* c, double* kernel_ms, int64_t* launches, double*
     if (launches) *launches = (int64_t)(c->ev_used / 2);
     if (bytes_per_launch) *bytes_per_launch = (double)(2 * 9 + 2 * 5 * c->ns) * (double)storage_bytes(c->prec) * (double)c->n;
     c->ev_used = 0;
+    return 0;
+}
+int lbib_reserve(lbib_ctx* c, int max_markers, int max_connections) {
+    if (check_ctx(c) || max_markers < c->nm || max_connections < c->nc) {
+        if (c) c->err = "reserve below the current counts";
+        return -1;
+    }
+    // reallocation drops the topology: the caller uploads it again
+    if (ensure_markers(c, max_markers)) return -1;
+    if (max_connections > c->ccap && grow_connections(c, max_connections)) return -1;
     return 0;
 }
 int lbib_set_box_outlet(lbib_ctx* c, int on) {

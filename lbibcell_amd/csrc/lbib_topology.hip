@@ -223,9 +223,11 @@ __global__ void k_rm_counts(DevStatus* st, uint32_t mcap, uint32_t ccap) {
     if (st->nm + 1 > mcap || st->nc > ccap) atomicOr(&st->bits, LBIB_ERR_CAPACITY);
 }
 
-// capacity guard before a round: the round may add up to nc markers
+// capacity guard once the round's size L is known (before anything is written)
 __global__ void k_rm_guard(DevStatus* st, uint32_t mcap, uint32_t ccap) {
-    if (st->nm + st->nc + 1 > mcap || 2 * st->nc > ccap) atomicOr(&st->bits, LBIB_HALT_GROW);
+    if (st->bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) return;
+    const uint32_t L = st->rm_L;
+    if (st->nm + L + 1 > mcap || st->nc + L > ccap) atomicOr(&st->bits, LBIB_HALT_GROW);
 }
 
 __global__ void k_rm_pending(const uint32_t* __restrict__ flag, DevStatus* st) {
@@ -622,9 +624,9 @@ int launch_remesh(lbib_ctx* c, int rounds) {
     TopoArrays t = topo_arrays(c);
     k_rm_reset<<<1, 1, 0, c->stream>>>(c->st);
     for (int r = 0; r < rounds; r++) {
-        k_rm_guard<<<1, 1, 0, c->stream>>>(c->st, (uint32_t)c->mcap, (uint32_t)c->ccap);
         k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, c->st);
         if (scan_exclusive(c, c->flag_c, c->rank_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->rm_L)) return -1;
+        k_rm_guard<<<1, 1, 0, c->stream>>>(c->st, (uint32_t)c->mcap, (uint32_t)c->ccap);
         k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->nx, c->ny, c->st);
         k_rm_commit<<<grid_for(2 * (long)c->ccap), 256, 0, c->stream>>>(t, c->st);
         k_rm_counts<<<1, 1, 0, c->stream>>>(c->st, (uint32_t)c->mcap, (uint32_t)c->ccap);
