@@ -44,7 +44,8 @@ size_t storage_bytes(int prec) {
 }
 
 int ensure_scan(lbib_ctx* c) {
-    const uint32_t need = (uint32_t)((std::max({c->mcap, c->ccap, c->dcap, 1}) + 1023) / 1024 + 1);
+    const uint32_t need =
+        (uint32_t)((std::max({c->mcap, c->ccap, c->dcap, c->stx * c->sty, 1}) + 1023) / 1024 + 1);
     if (need <= c->scan_cap) return 0;
     dev_free(c->scan_tmp);
     if (dev_alloc(c, &c->scan_tmp, need)) return -1;
@@ -67,12 +68,11 @@ int ensure_markers(lbib_ctx* c, int nm) {
     for (auto a : darr) dev_free(*a);
     for (auto a : uarr) dev_free(*a);
     for (auto a : iarr) dev_free(*a);
-    dev_free(c->srank); dev_free(c->touched); dev_free(c->pool);
+    dev_free(c->tslot); dev_free(c->tlist);
     for (auto a : darr) if (dev_alloc(c, a, cap)) return -1;
     for (auto a : uarr) if (dev_alloc(c, a, cap)) return -1;
     for (auto a : iarr) if (dev_alloc(c, a, cap + 1)) return -1;
-    if (dev_alloc(c, &c->srank, (size_t)cap * 16) || dev_alloc(c, &c->touched, (size_t)cap * 16)) return -1;
-    if (dev_alloc(c, &c->pool, (size_t)cap * 16)) return -1;
+    if (dev_alloc(c, &c->tslot, cap) || dev_alloc(c, &c->tlist, cap)) return -1;
     c->mcap = cap;
     return ensure_scan(c);
 }
@@ -235,12 +235,21 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     if (dev_alloc(c, &c->ux, n) || dev_alloc(c, &c->uy, n) || dev_alloc(c, &c->Fx, n) || dev_alloc(c, &c->Fy, n) ||
         dev_alloc(c, &c->msite, n))
         return -1;
-    if (dev_alloc(c, &c->tag, n) || dev_alloc(c, &c->ftile, (size_t)c->ny * c->ntx)) return -1;
-    if (dev_alloc(c, &c->scnt, n) || dev_alloc(c, &c->sbase, n)) return -1;
+    c->stx = (c->nx + 15) / 16;
+    c->sty = (c->ny + 15) / 16;
+    {
+        const size_t nt = (size_t)c->stx * c->sty;
+        if (dev_alloc(c, &c->tag, n) || dev_alloc(c, &c->tact, nt) || dev_alloc(c, &c->tcnt, nt) ||
+            dev_alloc(c, &c->toff, nt) || dev_alloc(c, &c->tflag, nt) || dev_alloc(c, &c->tpos, nt) ||
+            dev_alloc(c, &c->alist, nt))
+            return -1;
+    }
     if (dev_alloc(c, &c->best_dist, 4 * n, 0xff) || dev_alloc(c, &c->best_seq, 4 * n, 0xff) ||
         dev_alloc(c, &c->bn_dom, 4 * n))
         return -1;
-    if (dev_alloc(c, &c->changed, 2 * (size_t)REINIT_CAP) || dev_alloc(c, &c->rkeys, (size_t)REINIT_CAP)) return -1;
+    if (dev_alloc(c, &c->changed, 2 * (size_t)REINIT_CAP) || dev_alloc(c, &c->rkeys, (size_t)REINIT_CAP) ||
+        dev_alloc(c, &c->rstate, (size_t)REINIT_CAP))
+        return -1;
     if (dev_alloc(c, &c->st, 1)) return -1;
     LBIB_HIP(hipHostMalloc((void**)&c->st_host, sizeof(DevStatus), hipHostMallocDefault));
     if (ensure_markers(c, cfg->max_markers > 0 ? cfg->max_markers : 1024)) return -1;
@@ -270,8 +279,8 @@ void lbib_destroy(lbib_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->ux); dev_free(c->uy); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
-    dev_free(c->tag); dev_free(c->ftile); dev_free(c->scnt); dev_free(c->sbase); dev_free(c->touched);
-    dev_free(c->srank); dev_free(c->pool);
+    dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff); dev_free(c->tflag);
+    dev_free(c->tpos); dev_free(c->alist); dev_free(c->tslot); dev_free(c->tlist);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
                          &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
     uint32_t** uarr[18] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
@@ -285,7 +294,7 @@ void lbib_destroy(lbib_ctx* c) {
     for (auto a : iarr) dev_free(*a);
     dev_free(c->fs_prm); dev_free(c->bcd_off); dev_free(c->bcd_sp);
     dev_free(c->cand); dev_free(c->best_dist); dev_free(c->best_seq); dev_free(c->bn_dom); dev_free(c->bc_v);
-    dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rkeys);
+    dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rkeys); dev_free(c->rstate);
     dev_free(c->ring_start); dev_free(c->ring_cnt); dev_free(c->ring_off_u); dev_free(c->area);
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);

@@ -258,17 +258,29 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
 // per-site update in (y, x) order and each re-initialisation
 // (CDESolverD2Q5.cpp:156-228, ..._SIGNAL.cpp:184-250) reads neighbours updated
 // earlier in that order: sort the changed sites, then apply them in order.
+// The sequential pass of the reference is replayed level by level: a changed
+// site is ready once every changed neighbour that precedes it in (y, x) order
+// has been processed; neighbours that follow it depend on it, so they still
+// hold their old values when it runs.  Ready sites are never adjacent, which
+// makes each level safe to run in parallel and the result identical.
 template <int NS>
-__global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, unsigned long long* gkeys, DevStatus* st) {
+__global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, unsigned long long* gkeys, uint8_t* gstate,
+                                                 DevStatus* st) {
     if (halted(st)) return;
     __shared__ unsigned long long lkey[REINIT_LDS];
+    __shared__ uint8_t lstate[REINIT_LDS];
+    __shared__ int pending;
     const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
     if (cnt == 0) return;
     uint32_t P = 1;
     while (P < cnt) P <<= 1;
-    unsigned long long* key = (P <= (uint32_t)REINIT_LDS) ? lkey : gkeys;
-    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x)
+    const bool in_lds = P <= (uint32_t)REINIT_LDS;
+    unsigned long long* key = in_lds ? lkey : gkeys;
+    uint8_t* state = in_lds ? lstate : gstate;
+    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
         key[t] = (t < cnt) ? (((unsigned long long)a.changed[2 * t] << 32) | a.changed[2 * t + 1]) : ~0ull;
+        if (t < cnt) state[t] = 0;
+    }
     __syncthreads();
     for (uint32_t kk = 2; kk <= P; kk <<= 1) {
         for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
@@ -283,12 +295,48 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, unsigned lo
             __syncthreads();
         }
     }
-    if (threadIdx.x == 0) {
-        for (uint32_t t = 0; t < cnt; t++) {
-            const long i = (long)(key[t] >> 32);
-            const uint32_t nd = (uint32_t)(key[t] & 0xffffffffu);
-            reinit_site<NS>(a, i, nd);
+    const int EXd[8] = {1, 0, -1, 0, 1, -1, -1, 1};
+    const int EYd[8] = {0, 1, 0, -1, 1, 1, -1, -1};
+    auto find = [&](uint32_t site) -> int {
+        int lo = 0, hi = (int)cnt - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)(key[mid] >> 32) < site) lo = mid + 1; else hi = mid;
         }
+        return ((uint32_t)(key[lo] >> 32) == site) ? lo : -1;
+    };
+    for (int level = 0; level < (1 << 20); level++) {
+        if (threadIdx.x == 0) pending = 0;
+        __syncthreads();
+        // phase 1: mark the ready sites (state 1)
+        for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+            if (state[t] != 0) continue;
+            const uint32_t i = (uint32_t)(key[t] >> 32);
+            const int x = (int)(i % a.nx), y = (int)(i / a.nx);
+            bool ready = true;
+            for (int d = 0; d < 8 && ready; d++) {
+                const int xx = (x + EXd[d] + a.nx) % a.nx, yy = (y + EYd[d] + a.ny) % a.ny;
+                const uint32_t j = (uint32_t)yy * a.nx + xx;
+                if (j < i) {
+                    const int u = find(j);
+                    if (u >= 0 && state[u] != 2) ready = false;
+                }
+            }
+            if (ready) state[t] = 1;
+            else pending = 1;
+        }
+        __syncthreads();
+        // phase 2: run them
+        for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+            if (state[t] != 1) continue;
+            reinit_site<NS>(a, (long)(key[t] >> 32), (uint32_t)(key[t] & 0xffffffffu));
+        }
+        __syncthreads();
+        for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
+            if (state[t] == 1) state[t] = 2;
+        const int more = pending;
+        __syncthreads();
+        if (!more) break;
     }
 }
 
@@ -361,7 +409,7 @@ __global__ void k_mass_regulated(const uint32_t* __restrict__ ring_cnt, const do
 
 template <int NS>
 static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
-    k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rkeys, c->st);
+    k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rkeys, c->rstate, c->st);
 }
 
 int launch_topology(lbib_ctx* c, bool coarsen);

@@ -15,6 +15,8 @@
 // toucher lists the site), allocate per-site pool ranges, scatter
 // (marker, contribution) records, then per site sort by marker and add in
 // that order.  Atomics only count; no floating-point atomics.
+#include <algorithm>
+#include <climits>
 #include "lbib_device_math.h"
 
 namespace lbib {
@@ -69,104 +71,124 @@ __global__ void k_marker_force(int nm, const double* __restrict__ mx, const doub
     mfy[k] = fy;
 }
 
-__device__ __forceinline__ void stencil_site(const double X, const double Y, const int a, const int b,
-                                             int& sx, int& sy) {
-    sx = (int)floor(X) - 1 + a;
-    sy = (int)floor(Y) - 1 + b;
-}
+// ---- ordered spread over 16x16 site tiles --------------------------------
+// Markers are binned by the tile of floor(X), floor(Y).  A tile's sites can be
+// reached only by markers binned in the 3x3 tiles around it (the stencil
+// reaches 2 sites).  One workgroup per tile gathers those markers, sorts them
+// by index (= GeometryNode id order) and every lane adds delta*F for its site
+// in that order, starting from 0.0 like FluidSolver::resetForce; tiles with no
+// marker nearby are flagged inactive and the lattice kernel reads no force.
+constexpr int ST = 16;            // spread tile edge
+constexpr int SP_IDX = 4096;      // markers gathered per tile
+constexpr int SP_CH = 1024;       // marker data staged per pass
 
-// pass 1: count contributions per site; first toucher appends the site and
-// raises its tile flag.
-__global__ void k_spread_count(int nm, int nx, int ntx, const double* __restrict__ mx,
-                               const double* __restrict__ my, uint32_t* scnt, uint32_t* srank,
-                               uint32_t* touched, uint8_t* ftile, DevStatus* st) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nm * 16) return;
-    const int k = (int)(t >> 4), s = (int)(t & 15);
-    int sx, sy;
-    stencil_site(mx[k], my[k], s >> 2, s & 3, sx, sy);
-    const long site = (long)sy * nx + sx;
-    const uint32_t r = atomicAdd(&scnt[site], 1u);
-    srank[t] = r;
-    if (r == 0) {
-        const uint32_t p = atomicAdd(&st->ntouched, 1u);
-        touched[p] = (uint32_t)site;
-        ftile[(long)sy * ntx + sx / TILE_W] = 1;
+__global__ void k_tile_clear(int ntiles, uint32_t* tcnt, uint8_t* tact) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        tcnt[t] = 0;
+        tact[t] = 0;
     }
 }
 
-// pass 2: allocate a pool range per touched site.
-__global__ void k_spread_alloc(const uint32_t* __restrict__ touched, const uint32_t* __restrict__ scnt,
-                               uint32_t* sbase, DevStatus* st) {
-    const uint32_t nt = st->ntouched;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-        const uint32_t site = touched[t];
-        sbase[site] = atomicAdd(&st->pool_used, scnt[site]);
-    }
+__global__ void k_tile_count(int nm, int ntx, int nty, const double* __restrict__ mx, const double* __restrict__ my,
+                             uint32_t* tcnt, uint32_t* tslot, uint8_t* tact) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
+    const int home = (fy / ST) * ntx + fx / ST;
+    tslot[k] = atomicAdd(&tcnt[home], 1u);
+    for (int ty = (fy - 1) / ST; ty <= (fy + 2) / ST; ty++)
+        for (int tx = (fx - 1) / ST; tx <= (fx + 2) / ST; tx++)
+            if (tx < ntx && ty < nty) tact[ty * ntx + tx] = 1;
 }
 
-// pass 3: scatter (marker, delta*F) records.  Same arithmetic as
-// GeometryNode::distributeForce (GeometryNode.cpp:228-235).
-__global__ void k_spread_scatter(int nm, int nx, const double* __restrict__ mx,
-                                 const double* __restrict__ my, const double* __restrict__ mfx,
-                                 const double* __restrict__ mfy, const uint32_t* __restrict__ srank,
-                                 const uint32_t* __restrict__ sbase, SpreadEntry* pool) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nm * 16) return;
-    const int k = (int)(t >> 4), s = (int)(t & 15);
-    const double X = mx[k], Y = my[k];
-    int sx, sy;
-    stencil_site(X, Y, s >> 2, s & 3, sx, sy);
-    const long site = (long)sy * nx + sx;
-    const double dd = ib_delta((double)sx - X, (double)sy - Y);
-    SpreadEntry e;
-    e.marker = k;
-    e.pad = 0;
-    e.cx = dd * mfx[k];
-    e.cy = dd * mfy[k];
-    pool[sbase[site] + srank[t]] = e;
+__global__ void k_tile_scatter(int nm, int ntx, const double* __restrict__ mx, const double* __restrict__ my,
+                               const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tslot, int32_t* tlist) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
+    tlist[toff[(fy / ST) * ntx + fx / ST] + tslot[k]] = k;
 }
 
-// pass 4: per touched site, sum in marker order (FluidSolver::addForce,
-// FluidSolver.cpp:196-198, starting from resetForce's 0.0).
-__global__ void k_spread_reduce(const uint32_t* __restrict__ touched, uint32_t* scnt,
-                                const uint32_t* __restrict__ sbase, SpreadEntry* pool, double* Fx,
-                                double* Fy, DevStatus* st) {
-    const uint32_t nt = st->ntouched;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-        const uint32_t site = touched[t];
-        const uint32_t cnt = scnt[site];
-        SpreadEntry* e = pool + sbase[site];
-        // insertion sort by marker index (counts are small: markers within 2 lu)
-        for (uint32_t a = 1; a < cnt; a++) {
-            SpreadEntry v = e[a];
-            int b = (int)a - 1;
-            while (b >= 0 && e[b].marker > v.marker) { e[b + 1] = e[b]; b--; }
-            e[b + 1] = v;
+__global__ void k_tile_active(int ntiles, const uint8_t* __restrict__ tact, uint32_t* tflag) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) tflag[t] = tact[t];
+}
+__global__ void k_tile_compact(int ntiles, const uint32_t* __restrict__ tflag, const uint32_t* __restrict__ tpos,
+                               int32_t* alist) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
+        if (tflag[t]) alist[tpos[t]] = t;
+}
+
+__global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const int32_t* __restrict__ alist,
+                                                     const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
+                                                     const int32_t* __restrict__ tlist, const double* __restrict__ mx,
+                                                     const double* __restrict__ my, const double* __restrict__ mfx,
+                                                     const double* __restrict__ mfy, double* Fx, double* Fy,
+                                                     DevStatus* st) {
+    if ((uint32_t)blockIdx.x >= st->ntouched) return;
+    __shared__ int32_t idx[SP_IDX];
+    __shared__ double sx[SP_CH], sy[SP_CH], sfx[SP_CH], sfy[SP_CH];
+    __shared__ uint32_t n;
+    const int t = alist[blockIdx.x];
+    const int tx = t % ntx, ty = t / ntx;
+    if (threadIdx.x == 0) n = 0;
+    __syncthreads();
+    for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+            const int hx = tx + dx, hy = ty + dy;
+            if (hx < 0 || hy < 0 || hx >= ntx || hy >= nty) continue;
+            const int h = hy * ntx + hx;
+            const uint32_t c = tcnt[h], o = toff[h];
+            for (uint32_t j = threadIdx.x; j < c; j += blockDim.x) {
+                const uint32_t p = atomicAdd(&n, 1u);
+                if (p < (uint32_t)SP_IDX) idx[p] = tlist[o + j];
+            }
         }
-        double sx = 0.0, sy = 0.0;
-        for (uint32_t a = 0; a < cnt; a++) { sx += e[a].cx; sy += e[a].cy; }
-        Fx[site] = sx;
-        Fy[site] = sy;
-        scnt[site] = 0;
+    __syncthreads();
+    const uint32_t m = n;
+    if (m > (uint32_t)SP_IDX) {
+        if (threadIdx.x == 0) atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+        return;
     }
-}
-
-// after the collide: zero the force of touched sites and their tile flags.
-__global__ void k_clear_force(const uint32_t* __restrict__ touched, double* Fx, double* Fy, int nx,
-                              int ntx, uint8_t* ftile, DevStatus* st) {
-    const uint32_t nt = st->ntouched;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
-        const uint32_t site = touched[t];
-        Fx[site] = 0.0;
-        Fy[site] = 0.0;
-        ftile[(site / nx) * (long)ntx + (site % nx) / TILE_W] = 0;
+    uint32_t P = 1;
+    while (P < m) P <<= 1;
+    for (uint32_t j = m + threadIdx.x; j < P; j += blockDim.x) idx[j] = INT_MAX;
+    __syncthreads();
+    for (uint32_t kk = 2; kk <= P; kk <<= 1)
+        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+            for (uint32_t q = threadIdx.x; q < P; q += blockDim.x) {
+                const uint32_t l = q ^ j;
+                if (l > q) {
+                    const bool up = (q & kk) == 0;
+                    const int32_t u = idx[q], v = idx[l];
+                    if ((u > v) == up) { idx[q] = v; idx[l] = u; }
+                }
+            }
+            __syncthreads();
+        }
+    const int px = tx * ST + (int)(threadIdx.x % ST), py = ty * ST + (int)(threadIdx.x / ST);
+    double ax = 0.0, ay = 0.0;
+    for (uint32_t c0 = 0; c0 < m; c0 += SP_CH) {
+        const uint32_t mc = min((uint32_t)SP_CH, m - c0);
+        for (uint32_t j = threadIdx.x; j < mc; j += blockDim.x) {
+            const int k = idx[c0 + j];
+            sx[j] = mx[k]; sy[j] = my[k]; sfx[j] = mfx[k]; sfy[j] = mfy[k];
+        }
+        __syncthreads();
+        for (uint32_t j = 0; j < mc; j++) {
+            const double X = sx[j], Y = sy[j];
+            const int fx = (int)floor(X), fy = (int)floor(Y);
+            if (px < fx - 1 || px > fx + 2 || py < fy - 1 || py > fy + 2) continue;
+            const double dd = ib_delta((double)px - X, (double)py - Y);
+            ax += dd * sfx[j];
+            ay += dd * sfy[j];
+        }
+        __syncthreads();
     }
-}
-
-__global__ void k_reset_spread_counters(DevStatus* st) {
-    st->ntouched = 0;
-    st->pool_used = 0;
+    if (px < nx && py < ny) {
+        const long site = (long)py * nx + px;
+        Fx[site] = ax;
+        Fy[site] = ay;
+    }
 }
 
 // GeometryNode::collectVelocity (GeometryNode.cpp:216-226) in stencil-slot
@@ -220,32 +242,36 @@ __global__ void k_interp_move(int nm, int nx, int ny, double* mx, double* my, do
     }
 }
 
+__global__ void k_reset_spread_counters(DevStatus* st) { st->ntouched = 0; }
+
 static inline int nblocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 int launch_force_and_spread(lbib_ctx* c) {
-    if (c->nm == 0) return 0;
     const int bs = 256;
+    const int ntiles = c->stx * c->sty;
+    k_tile_clear<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tcnt, c->tact);
+    if (c->nm == 0) {
+        k_reset_spread_counters<<<1, 1, 0, c->stream>>>(c->st);
+        LBIB_HIP(hipGetLastError());
+        return 0;
+    }
     k_marker_force<<<nblocks(c->nm, bs), bs, 0, c->stream>>>(c->nm, c->mx, c->my, c->fs_off, c->fs_type,
                                                              c->fs_partner, c->fs_prm, c->mfx, c->mfy);
-    k_reset_spread_counters<<<1, 1, 0, c->stream>>>(c->st);
-    const long nc = (long)c->nm * 16;
-    k_spread_count<<<nblocks(nc, bs), bs, 0, c->stream>>>(c->nm, c->nx, c->ntx, c->mx, c->my, c->scnt,
-                                                          c->srank, c->touched, c->ftile, c->st);
-    const int gs = 1024;
-    k_spread_alloc<<<gs, bs, 0, c->stream>>>(c->touched, c->scnt, c->sbase, c->st);
-    k_spread_scatter<<<nblocks(nc, bs), bs, 0, c->stream>>>(c->nm, c->nx, c->mx, c->my, c->mfx, c->mfy,
-                                                            c->srank, c->sbase, c->pool);
-    k_spread_reduce<<<gs, bs, 0, c->stream>>>(c->touched, c->scnt, c->sbase, c->pool, c->Fx, c->Fy, c->st);
+    k_tile_count<<<nblocks(c->nm, bs), bs, 0, c->stream>>>(c->nm, c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
+                                                           c->tact);
+    if (scan_exclusive(c, c->tcnt, c->toff, nullptr, (uint32_t)ntiles, (uint32_t)ntiles, nullptr)) return -1;
+    k_tile_scatter<<<nblocks(c->nm, bs), bs, 0, c->stream>>>(c->nm, c->stx, c->mx, c->my, c->toff, c->tslot,
+                                                             c->tlist);
+    k_tile_active<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tact, c->tflag);
+    if (scan_exclusive(c, c->tflag, c->tpos, nullptr, (uint32_t)ntiles, (uint32_t)ntiles, &c->st->ntouched)) return -1;
+    k_tile_compact<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tflag, c->tpos, c->alist);
+    k_tile_spread<<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->alist, c->toff, c->tcnt, c->tlist,
+                                                 c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
 
-int launch_clear_force(lbib_ctx* c) {
-    if (c->nm == 0) return 0;
-    k_clear_force<<<1024, 256, 0, c->stream>>>(c->touched, c->Fx, c->Fy, c->nx, c->ntx, c->ftile, c->st);
-    LBIB_HIP(hipGetLastError());
-    return 0;
-}
+int launch_clear_force(lbib_ctx*) { return 0; }
 
 int launch_interp_move(lbib_ctx* c) {
     if (c->nm == 0) return 0;

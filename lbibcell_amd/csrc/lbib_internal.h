@@ -8,7 +8,7 @@
 //   Fx[n], Fy[n]      FluidSolver::force_ (IB spread target, zero off-membrane)
 //   tag[n]            bits 0-23 domain id, 24-27 boundary node present in slot
 //                     E,N,W,S, 28-31 that boundary node carries a no-flux BC
-//   ftile[ny][ntx]    1 if any site of the 256-wide row tile holds a force
+//   tact[sty][stx]    1 if the 16x16 site tile was written by this step's spread
 // Mixed-precision contexts replace f/g with 16-bit deviations from the rest
 // state (see lbib_lattice.hip).
 #pragma once
@@ -44,8 +44,8 @@ struct SpeciesDev {
 struct DevStatus {
     uint32_t bits;
     int32_t first_bad;
-    uint32_t ntouched;      // spread: sites touched this step
-    uint32_t pool_used;     // spread: pool entries allocated
+    uint32_t ntouched;      // spread: 16x16 tiles written this step
+    uint32_t pool_used;     // unused
     uint32_t ncand;         // cut-link candidates of the current step
     uint32_t ncand_prev;    // ... of the previous step (reset list)
     uint32_t nbn;           // winners
@@ -93,11 +93,12 @@ struct lbib_ctx {
     double *ux = nullptr, *uy = nullptr, *Fx = nullptr, *Fy = nullptr, *msite = nullptr;
     bool msite_live = false;
     uint32_t* tag = nullptr;
-    uint8_t* ftile = nullptr;
     lbib::SpeciesDev sp[LBIB_MAX_SPECIES];
-    // spread
-    uint32_t *scnt = nullptr, *sbase = nullptr, *touched = nullptr, *srank = nullptr;
-    lbib::SpreadEntry* pool = nullptr;
+    // spread over 16x16 tiles
+    int stx = 0, sty = 0;
+    uint32_t *tcnt = nullptr, *toff = nullptr, *tflag = nullptr, *tpos = nullptr, *tslot = nullptr;
+    uint8_t* tact = nullptr;
+    int32_t *tlist = nullptr, *alist = nullptr;
     // markers
     int mcap = 0, nm = 0;
     double *mx = nullptr, *my = nullptr, *mvx = nullptr, *mvy = nullptr;
@@ -139,6 +140,7 @@ struct lbib_ctx {
     int32_t* bc_desc = nullptr;                // [4][n]
     uint32_t* changed = nullptr;               // [REINIT_CAP][2]
     unsigned long long* rkeys = nullptr;       // [REINIT_CAP] sort scratch
+    uint8_t* rstate = nullptr;                 // [REINIT_CAP] level state
     // rings / domains (indexed by domain id)
     int dcap = 0, ndom = 0;
     int32_t* ring_start = nullptr;

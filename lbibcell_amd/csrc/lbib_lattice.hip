@@ -69,7 +69,7 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
     const uint32_t tg = a.tag[i];
     const uint32_t dom = tg & DOM_MASK;
     double Fx = 0.0, Fy = 0.0;
-    if (a.ftile == nullptr || a.ftile[(long)y * a.ntx + blockIdx.x]) {
+    if (a.ftile == nullptr || a.ftile[(long)(y >> 4) * a.ntx + (x >> 4)]) {
         Fx = a.Fx[i];
         Fy = a.Fy[i];
     }
@@ -229,13 +229,13 @@ static LatticeArgs make_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
     a.gd = (double*)c->g[1 - c->cur];
     a.ux = c->ux; a.uy = c->uy;
     a.Fx = c->Fx; a.Fy = c->Fy;
-    a.ftile = use_ftile ? c->ftile : nullptr;
+    a.ftile = use_ftile ? c->tact : nullptr;
     a.tag = c->tag;
     a.mtab = c->mtab_live ? c->mtab : nullptr;
     a.ntab = c->ndom;
     a.msite = c->msite_live ? c->msite : nullptr;
     a.bc_v = c->bc_v; a.bc_desc = c->bc_desc; a.bcd_off = c->bcd_off; a.bcd_sp = c->bcd_sp;
-    a.ntx = c->ntx;
+    a.ntx = c->stx;
     a.tauI = 1.0 / c->cfg.tau_fluid;
     a.tau = c->cfg.tau_fluid;
     a.it = it;
