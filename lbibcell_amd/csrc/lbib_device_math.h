@@ -187,4 +187,21 @@ __device__ __forceinline__ double c_swapped(const double d0, const double dE, co
     return r;
 }
 
+// Wave-aggregated counter allocation: every lane of the wavefront must call
+// it (v = 0 for idle lanes); returns the lane's base in *ctr.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr, uint32_t v) {
+    const int lane = threadIdx.x & 63;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(ctr, total);
+    base = __shfl(base, 63, 64);
+    return base + incl - v;
+}
+
 }  // namespace lbib

@@ -56,27 +56,29 @@ __global__ void k_cut_gen(int nc, int nx, int ny, long n, const int32_t* __restr
                           int candcap, unsigned long long* best_dist, DevStatus* st) {
     if (halted(st)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const double x1 = mx[ca[c]], y1 = my[ca[c]], x2 = mx[cb[c]], y2 = my[cb[c]];
+    const bool live = c < nc;
+    double x1 = 0, y1 = 0, x2 = 1, y2 = 1;
+    if (live) { x1 = mx[ca[c]]; y1 = my[ca[c]]; x2 = mx[cb[c]]; y2 = my[cb[c]]; }
     const double maxX = fmax(x1, x2), maxY = fmax(y1, y2);
     const double minX = fmin(x1, x2), minY = fmin(y1, y2);
     const double m = (y2 - y1) / (x2 - x1);
     const double nn = y1 - (y2 - y1) / (x2 - x1) * x1;
-    if (!isfinite(m)) {
+    bool ok = live;
+    if (live && !isfinite(m)) {
         atomicOr(&st->bits, LBIB_ERR_SLOPE);
-        return;
+        ok = false;
     }
     const int iv0 = (int)ceil(minX), iv1 = (int)floor(maxX);
     const int ih0 = (int)ceil(minY), ih1 = (int)floor(maxY);
-    const int nv = iv1 >= iv0 ? iv1 - iv0 + 1 : 0;
-    const int nh = ih1 >= ih0 ? ih1 - ih0 + 1 : 0;
-    const int cnt = 2 * (nv + nh);
-    if (cnt == 0) return;
+    const int nv = (ok && iv1 >= iv0) ? iv1 - iv0 + 1 : 0;
+    const int nh = (ok && ih1 >= ih0) ? ih1 - ih0 + 1 : 0;
+    int cnt = 2 * (nv + nh);
     if (cnt > SEQ_PER_CONN) {
         atomicOr(&st->bits, LBIB_ERR_CAPACITY);
-        return;
+        cnt = 0;
     }
-    const uint32_t base = atomicAdd(&st->ncand, (uint32_t)cnt);
+    const uint32_t base = wave_alloc(&st->ncand, (uint32_t)cnt);
+    if (cnt == 0) return;
     if ((long)base + cnt > candcap) {
         atomicOr(&st->bits, LBIB_ERR_CAPACITY);
         return;
@@ -103,14 +105,14 @@ __global__ void k_cut_gen(int nc, int nx, int ny, long n, const int32_t* __restr
         k++;
     };
     // vertical lattice lines: node S at (i, floor y) [its slot N], node N at (i, ceil y) [slot S]
-    for (int i = iv0; i <= iv1; i++) {
+    for (int i = iv0; i < iv0 + nv; i++) {
         const double y = m * i + nn;
         const uint32_t dN = (x1 < x2) ? dom : 0u, dS = (x1 < x2) ? 0u : dom;
         put(i, (int)floor(y), DN, dS, (double)i, y);
         put(i, (int)ceil(y), DS, dN, (double)i, y);
     }
     // horizontal lattice lines: node W at (floor x, i) [slot E], node E at (ceil x, i) [slot W]
-    for (int i = ih0; i <= ih1; i++) {
+    for (int i = ih0; i < ih0 + nh; i++) {
         const double x = (i - nn) / m;
         const uint32_t dW = (y1 > y2) ? 0u : dom, dE = (y1 > y2) ? dom : 0u;
         put((int)floor(x), i, DE, dW, x, (double)i);
@@ -130,6 +132,22 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
     }
 }
 
+// status counter of kept boundary nodes (one atomic per wavefront)
+__global__ void k_count_bn(const Cand* __restrict__ cand, long n, const uint32_t* __restrict__ best_seq, DevStatus* st) {
+    if (halted(st)) return;
+    const uint32_t nc = st->ncand;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
+        const uint32_t j = j0 + threadIdx.x;
+        uint32_t w = 0;
+        if (j < nc) {
+            const Cand e = cand[j];
+            w = best_seq[(long)(e.slot - 1) * n + e.site] == e.seq ? 1u : 0u;
+        }
+        (void)wave_alloc(&st->nbn, w);
+    }
+}
+
 // Winners: mark the slot, record the domain and (for BC-carrying connections)
 // the node velocity of Connection::calculateVelocity (Connection.cpp:146-168).
 __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_seq, const int32_t* __restrict__ ca,
@@ -144,7 +162,6 @@ __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_
         const long slotix = (long)(e.slot - 1) * n + e.site;
         if (best_seq[slotix] != e.seq) continue;
         e.win = 1;
-        atomicAdd(&st->nbn, 1u);
         bn_dom[slotix] = e.domain;
         uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
         const int desc = cbc[e.conn];
@@ -430,6 +447,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
                                                                c->mx, c->my, c->cand, c->candcap, c->best_dist,
                                                                c->st);
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
+    k_count_bn<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->st);
     k_cut_win<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my, c->mvx,
                                         c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->st);
     k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, cure ? 0 : 1,

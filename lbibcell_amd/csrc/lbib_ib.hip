@@ -139,8 +139,13 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
             const int h = hy * ntx + hx;
             const uint32_t c = tcnt[h], o = toff[h];
             for (uint32_t j = threadIdx.x; j < c; j += blockDim.x) {
+                const int k = tlist[o + j];
+                // keep only markers whose 4x4 stencil overlaps this tile
+                const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
+                if (fx + 2 < tx * ST || fx - 1 > tx * ST + ST - 1 || fy + 2 < ty * ST || fy - 1 > ty * ST + ST - 1)
+                    continue;
                 const uint32_t p = atomicAdd(&n, 1u);
-                if (p < (uint32_t)SP_IDX) idx[p] = tlist[o + j];
+                if (p < (uint32_t)SP_IDX) idx[p] = k;
             }
         }
     __syncthreads();

@@ -249,14 +249,33 @@ __global__ void k_ring_reset(int ndom, int32_t* start, uint32_t* cnt) {
     start[d] = INT_MAX;
     cnt[d] = 0;
 }
+// Connections of one ring mostly sit in contiguous runs of the vector: the
+// head lane of each run inside a wavefront does one atomic for the whole run.
 __global__ void k_ring_count(const uint32_t* __restrict__ cdom, int32_t* start, uint32_t* cnt, uint32_t* slot,
                              int ndom, DevStatus* st) {
     const uint32_t nc = st->nc;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const uint32_t d = cdom[c];
-        if ((int)d >= ndom) { atomicOr(&st->bits, LBIB_ERR_TOPOLOGY); continue; }
-        atomicMin(&start[d], (int32_t)c);
-        slot[c] = atomicAdd(&cnt[d], 1u);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const int lane = threadIdx.x & 63;
+    for (uint32_t c0 = blockIdx.x * blockDim.x; c0 < nc; c0 += stride) {
+        const uint32_t c = c0 + threadIdx.x;
+        const bool live = c < nc;
+        uint32_t d = live ? cdom[c] : 0xffffffffu;
+        if (live && (int)d >= ndom) { atomicOr(&st->bits, LBIB_ERR_TOPOLOGY); d = 0xfffffffeu; }
+        const uint32_t dprev = __shfl_up(d, 1, 64);
+        const bool head = (lane == 0) || (dprev != d);
+        const unsigned long long heads = __ballot(head);
+        const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+        const int hl = 63 - __clzll((long long)(heads & le));           // my run's head lane
+        const unsigned long long gt = heads & ~le;                        // heads after me
+        const int nextl = gt ? __ffsll((long long)gt) - 1 : 64;
+        const uint32_t len = (uint32_t)(nextl - hl);
+        uint32_t base = 0;
+        if (head && live && d < (uint32_t)ndom) {
+            base = atomicAdd(&cnt[d], len);
+            atomicMin(&start[d], (int32_t)c);
+        }
+        base = __shfl(base, hl, 64);
+        if (live && d < (uint32_t)ndom) slot[c] = base + (uint32_t)(lane - hl);
     }
 }
 __global__ void k_ring_bucket(const uint32_t* __restrict__ cdom, const uint32_t* __restrict__ slot,
