@@ -204,4 +204,30 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr, uint32_t v) {
     return base + incl - v;
 }
 
+// Block-aggregated counter allocation (blockDim.x == 256): one atomic per
+// workgroup.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_alloc(uint32_t* ctr, uint32_t v) {
+    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t bbase;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t incl = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        bbase = total ? atomicAdd(ctr, total) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = bbase;
+    for (int k = 0; k < w; k++) off += wsum[k];
+    const uint32_t r = off + incl - v;
+    __syncthreads();
+    return r;
+}
+
 }  // namespace lbib

@@ -77,7 +77,7 @@ __global__ void k_cut_gen(int nc, int nx, int ny, long n, const int32_t* __restr
         atomicOr(&st->bits, LBIB_ERR_CAPACITY);
         cnt = 0;
     }
-    const uint32_t base = wave_alloc(&st->ncand, (uint32_t)cnt);
+    const uint32_t base = block_alloc(&st->ncand, (uint32_t)cnt);
     if (cnt == 0) return;
     if ((long)base + cnt > candcap) {
         atomicOr(&st->bits, LBIB_ERR_CAPACITY);
@@ -132,22 +132,6 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
     }
 }
 
-// status counter of kept boundary nodes (one atomic per wavefront)
-__global__ void k_count_bn(const Cand* __restrict__ cand, long n, const uint32_t* __restrict__ best_seq, DevStatus* st) {
-    if (halted(st)) return;
-    const uint32_t nc = st->ncand;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
-        const uint32_t j = j0 + threadIdx.x;
-        uint32_t w = 0;
-        if (j < nc) {
-            const Cand e = cand[j];
-            w = best_seq[(long)(e.slot - 1) * n + e.site] == e.seq ? 1u : 0u;
-        }
-        (void)wave_alloc(&st->nbn, w);
-    }
-}
-
 // Winners: mark the slot, record the domain and (for BC-carrying connections)
 // the node velocity of Connection::calculateVelocity (Connection.cpp:146-168).
 __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_seq, const int32_t* __restrict__ ca,
@@ -157,24 +141,32 @@ __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_
                           uint32_t* bn_dom, double* bc_v, int32_t* bc_desc, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->ncand;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
-        Cand& e = cand[j];
-        const long slotix = (long)(e.slot - 1) * n + e.site;
-        if (best_seq[slotix] != e.seq) continue;
-        e.win = 1;
-        bn_dom[slotix] = e.domain;
-        uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
-        const int desc = cbc[e.conn];
-        if (desc >= 0 && bc_v != nullptr) {
-            bits |= 1u << (BC_SHIFT + e.slot - 1);
-            const int a = ca[e.conn], b = cb[e.conn];
-            const double l = sqrt(sqd(mx[a], my[a], e.bx, e.by));
-            const double L = sqrt(sqd(mx[b], my[b], mx[a], my[a]));
-            bc_v[2 * slotix] = (mvx[b] - mvx[a]) * l / L + mvx[a];
-            bc_v[2 * slotix + 1] = (mvy[b] - mvy[a]) * l / L + mvy[a];
-            bc_desc[slotix] = desc;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
+        const uint32_t j = j0 + threadIdx.x;
+        uint32_t won = 0;
+        if (j < nc) {
+            Cand& e = cand[j];
+            const long slotix = (long)(e.slot - 1) * n + e.site;
+            if (best_seq[slotix] == e.seq) {
+                won = 1;
+                e.win = 1;
+                bn_dom[slotix] = e.domain;
+                uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
+                const int desc = cbc[e.conn];
+                if (desc >= 0 && bc_v != nullptr) {
+                    bits |= 1u << (BC_SHIFT + e.slot - 1);
+                    const int a = ca[e.conn], b = cb[e.conn];
+                    const double l = sqrt(sqd(mx[a], my[a], e.bx, e.by));
+                    const double L = sqrt(sqd(mx[b], my[b], mx[a], my[a]));
+                    bc_v[2 * slotix] = (mvx[b] - mvx[a]) * l / L + mvx[a];
+                    bc_v[2 * slotix + 1] = (mvy[b] - mvy[a]) * l / L + mvy[a];
+                    bc_desc[slotix] = desc;
+                }
+                atomicOr(&tag[e.site], bits);
+            }
         }
-        atomicOr(&tag[e.site], bits);
+        (void)block_alloc(&st->nbn, won);
     }
 }
 
@@ -447,7 +439,6 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
                                                                c->mx, c->my, c->cand, c->candcap, c->best_dist,
                                                                c->st);
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
-    k_count_bn<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->st);
     k_cut_win<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my, c->mvx,
                                         c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->st);
     k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, cure ? 0 : 1,

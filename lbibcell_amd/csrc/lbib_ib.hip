@@ -79,8 +79,8 @@ __global__ void k_marker_force(int nm, const double* __restrict__ mx, const doub
 // in that order, starting from 0.0 like FluidSolver::resetForce; tiles with no
 // marker nearby are flagged inactive and the lattice kernel reads no force.
 constexpr int ST = 16;            // spread tile edge
-constexpr int SP_IDX = 4096;      // markers gathered per tile
-constexpr int SP_CH = 1024;       // marker data staged per pass
+constexpr int SP_IDX = 1024;      // markers gathered per tile
+constexpr int SP_CH = 256;        // marker data staged per pass
 
 __global__ void k_tile_clear(int ntiles, uint32_t* tcnt, uint8_t* tact) {
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
