@@ -76,7 +76,13 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
     double m = 0.0;
     if (a.msite) { m = a.msite[i]; a.msite[i] = 0.0; }
     else if (a.mtab) m = (dom < (uint32_t)a.ntab) ? a.mtab[dom] : 0.0;
-    const double pu = a.ux[i], pv = a.uy[i];   // stored velocity_ of the previous collide
+    // stored velocity_ of the previous collide: read only where it is used
+    // (Guo forcing everywhere, the no-flux BC at boundary-node sites)
+    double pu = 0.0, pv = 0.0;
+    if (FM != 0 || (NS > 0 && (tg >> BC_SHIFT))) {
+        pu = a.ux[i];
+        pv = a.uy[i];
+    }
 
     double g[NS > 0 ? NS : 1][5];
     const long sn = 5 * n;
@@ -156,12 +162,13 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
         }
     }
 
+    // streaming stores: the next pass reads these from HBM, not from cache
 #pragma unroll
-    for (int q = 0; q < 9; q++) a.fd[(long)q * n + i] = f[q];
+    for (int q = 0; q < 9; q++) __builtin_nontemporal_store(f[q], &a.fd[(long)q * n + i]);
 #pragma unroll
     for (int s = 0; s < NS; s++)
 #pragma unroll
-        for (int q = 0; q < 5; q++) a.gd[s * sn + (long)q * n + i] = g[s][q];
+        for (int q = 0; q < 5; q++) __builtin_nontemporal_store(g[s][q], &a.gd[s * sn + (long)q * n + i]);
     a.ux[i] = uxn;
     a.uy[i] = uyn;
 }
