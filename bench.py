@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64"])
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--run-ahead", type=int, default=16,
+                    help="iterations enqueued between device status reads")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice path runs on a single MI355X per process; "
@@ -104,6 +106,7 @@ def main():
     nodes, conns = tissue_case(nx, ny, cols, rows)
     log(f"workload {nx}x{ny}, {len(set(c[2] for c in conns))} cells, {len(nodes)} initial markers")
     sim = build_device(nx, ny, nodes, conns, prec)
+    sim.set_run_ahead(args.run_ahead)
     log("device setup done; warming up")
     sim.run(args.warmup)
     sim.sync()
@@ -124,6 +127,9 @@ def main():
     mlups = n * args.steps / dt / 1e6
     kern_s = ms / 1e3 / max(launches, 1)
     achieved = bpl / kern_s / 1e9
+    copy_bytes = 1 << 30
+    copy_ms = ctx.profile_copy(copy_bytes, 10)
+    copy_gbs = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
     nmark = len(sim.markers()[0])
     times = sim.times()
     traffic = None
@@ -145,12 +151,15 @@ def main():
         "data": "synthetic (seeded hex tissue geometry)",
         "config": {"workload": f"tissue {nx}x{ny}: D2Q9+D2Q5, {len(set(c[2] for c in conns))} cells r=40+-4, "
                                f"150 markers/cell (remeshed), tension 0.01, area regulator 1.2xA0, box outlet",
-                   "precision": args.precision, "markers_after_remesh": nmark},
+                   "precision": args.precision, "markers_after_remesh": nmark, "run_ahead": args.run_ahead},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_stream_collide_f64<1,0>", "kernel_ms_avg": kern_s * 1e3,
-                     "algorithmic_bytes_per_launch": bpl},
-        "stage_seconds_per_step": {k: v / max(times["steps"], 1) for k, v in times.items() if k != "steps"},
+                     "algorithmic_bytes_per_launch": bpl,
+                     "copy_probe_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
+        "host_seconds_per_step": {k: v / max(times["steps"], 1) for k, v in times.items()
+                                  if k not in ("steps", "stops")},
+        "resolved_stops": times["stops"],
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")

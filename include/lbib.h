@@ -91,6 +91,7 @@ typedef struct lbib_config {
 #define LBIB_HALT_MASK         0x00000f00u
 #define LBIB_FLAG_REMESH       0x00001000u  /* a segment is longer than MAXLENGTH */
 #define LBIB_FLAG_COARSEN      0x00002000u  /* a marker's neighbours are closer than MAXLENGTH */
+#define LBIB_SKIPPED           0x00004000u  /* a later step began while stopped: its kernels did nothing */
 
 typedef struct lbib_status {
     uint32_t bits;
@@ -103,6 +104,8 @@ typedef struct lbib_status {
     int32_t n_connections;      /* Connections after this step's remesh/coarsen */
     int32_t n_remeshed;         /* connections split by remeshing this step */
     int32_t n_coarsened;        /* markers removed by coarsening this step */
+    uint32_t step_iteration;    /* iteration of the last lbib_step_begin that found the status clear */
+    int32_t n_reinit_levels;    /* sequential levels of the CDE re-initialisation (dependency depth) */
 } lbib_status;
 
 /* ---- context ---------------------------------------------------------- */
@@ -193,6 +196,27 @@ int lbib_step_lattice_only(lbib_ctx* ctx, uint32_t it);
 int lbib_read_status(lbib_ctx* ctx, lbib_status* st);
 int lbib_clear_status(lbib_ctx* ctx);
 
+/* ---- run-ahead ---------------------------------------------------------------
+ * No counterpart in the reference, whose loop (SimulationRunner.cpp:72-100)
+ * inspects its geometry after every iteration.  Here a caller may enqueue
+ * several iterations and read the status once.  The status is "stopped" when
+ * it holds an ERR or HALT bit, LBIB_FLAG_REMESH or LBIB_SKIPPED: then every
+ * lattice, IB, force and area-regulator kernel returns without work, and the
+ * geometry kernels do so on all of these but LBIB_FLAG_REMESH (the rest of
+ * that iteration's topology stage still runs, as it does without run-ahead).
+ *
+ * lbib_step_begin(ctx, it) opens iteration it.  On the device it records it
+ * as lbib_status.step_iteration if the status is clear, and otherwise sets
+ * LBIB_SKIPPED, so nothing after the stop changes device state.  On the host
+ * it snapshots the launch state (ping-pong buffer, pending mass table,
+ * profiling events).  After a stop in iteration t, lbib_rewind(ctx, t + 1)
+ * restores the snapshot taken when iteration t + 1 began (and is a no-op if
+ * it never began); the caller then resolves iteration t exactly as without
+ * run-ahead (host halts, remesh rounds, forces, BioSolvers) and continues
+ * with t + 1.  Snapshots of the last 1024 iterations are kept. */
+int lbib_step_begin(lbib_ctx* ctx, uint32_t it);
+int lbib_rewind(lbib_ctx* ctx, uint32_t it);
+
 /* ---- device BioSolver helpers ------------------------------------------- */
 /* Areas (shoelace, FP64, in ring order) per domain id 0..ndom-1; domains
  * without connections read 0.  ndom = value given to lbib_set_celltypes. */
@@ -221,6 +245,11 @@ int lbib_add_mass_by_domain(lbib_ctx* ctx, int ndom, const double* table);
  * storage precision: (2*9 + 2*5*nspecies) * bytes * nx * ny), and resets. */
 int lbib_profile_enable(lbib_ctx* ctx, int on);
 int lbib_profile_read(lbib_ctx* ctx, double* kernel_ms, int64_t* launches, double* bytes_per_launch);
+/* Attainable-bandwidth yardstick: a plain 16-byte-vector copy of nbytes
+ * (read nbytes + write nbytes) on the context's stream, reps timed launches
+ * after one warm-up; returns the average launch time.  Allocates and frees two
+ * nbytes buffers, so it is a set-up/measurement call, never a per-step one. */
+int lbib_profile_copy(lbib_ctx* ctx, int64_t nbytes, int reps, double* kernel_ms);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,7 @@ HALT_GROW = 0x800
 HALT_MASK = 0xF00
 FLAG_REMESH = 0x1000
 FLAG_COARSEN = 0x2000
+SKIPPED = 0x4000
 
 
 class SpeciesSpec(C.Structure):
@@ -75,6 +76,8 @@ class Status(C.Structure):
         ("n_connections", C.c_int32),
         ("n_remeshed", C.c_int32),
         ("n_coarsened", C.c_int32),
+        ("step_iteration", C.c_uint32),
+        ("n_reinit_levels", C.c_int32),
     ]
 
 
@@ -123,11 +126,14 @@ SIGNATURES = {
     "lbib_step_lattice_only": (C.c_int, [_P, C.c_uint32]),
     "lbib_read_status": (C.c_int, [_P, C.POINTER(Status)]),
     "lbib_clear_status": (C.c_int, [_P]),
+    "lbib_step_begin": (C.c_int, [_P, C.c_uint32]),
+    "lbib_rewind": (C.c_int, [_P, C.c_uint32]),
     "lbib_compute_areas": (C.c_int, [_P, _D]),
     "lbib_area_regulator": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double]),
     "lbib_add_mass_by_domain": (C.c_int, [_P, C.c_int, _D]),
     "lbib_profile_enable": (C.c_int, [_P, C.c_int]),
     "lbib_profile_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _D]),
+    "lbib_profile_copy": (C.c_int, [_P, C.c_int64, C.c_int, _D]),
 }
 
 
@@ -286,6 +292,12 @@ class Context:
         self._chk(self.lib.lbib_read_status(self.h, C.byref(s)))
         return s
 
+    def step_begin(self, it):
+        self._chk(self.lib.lbib_step_begin(self.h, it))
+
+    def rewind(self, it):
+        self._chk(self.lib.lbib_rewind(self.h, it))
+
     def profile_enable(self, on=True):
         self._chk(self.lib.lbib_profile_enable(self.h, 1 if on else 0))
 
@@ -296,6 +308,12 @@ class Context:
         b = C.c_double()
         self._chk(self.lib.lbib_profile_read(self.h, C.byref(ms), C.byref(n), C.byref(b)))
         return ms.value, n.value, b.value
+
+    def profile_copy(self, nbytes=1 << 30, reps=10):
+        """Average ms of one plain copy of nbytes (read + write) on the context stream."""
+        ms = C.c_double()
+        self._chk(self.lib.lbib_profile_copy(self.h, int(nbytes), int(reps), C.byref(ms)))
+        return ms.value
 
     def device_bytes(self):
         return int(self.lib.lbib_device_bytes(self.h))

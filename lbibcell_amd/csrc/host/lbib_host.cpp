@@ -547,6 +547,7 @@ void SimulationRunner::addBioSolver(std::unique_ptr<BioSolver> b) { bio_.push_ba
 
 void SimulationRunner::syncMirror() {
     if (mirrorValid_) return;
+    settle();
     const size_t nm = nm_, nc = nc_;
     std::vector<unsigned> ids(nm);
     std::vector<double> xy(2 * nm), v(2 * nm), o(2 * nm);
@@ -600,6 +601,7 @@ void SimulationRunner::renewTension(double force) {
 }
 
 void SimulationRunner::uploadTopology() {
+    settle();
     const auto& nodes = geo_.getGeometryNodes();
     const int nm = (int)nodes.size();
     ids_.clear();
@@ -670,6 +672,7 @@ void SimulationRunner::uploadTopology() {
 }
 
 void SimulationRunner::uploadForces() {
+    settle();
     const int nm = (int)ids_.size();
     std::vector<int32_t> off(nm + 1, 0), type, partner;
     std::vector<double> prm;
@@ -790,9 +793,22 @@ void SimulationRunner::handleGeometryStatus(unsigned it) {
     check(lbib_clear_status(ctx_), "status");
 }
 
-void SimulationRunner::step() {
-    const unsigned it = params_.currentIteration;
+namespace {
+// Raised by settle() when enqueued iterations hit a stop; run() resolves it.
+struct RunAheadStop {
+    unsigned it;
+};
+constexpr uint32_t kStopBits = LBIB_ERR_MASK | LBIB_HALT_MASK | LBIB_FLAG_REMESH | LBIB_SKIPPED;
+}  // namespace
+
+// One SimulationRunner iteration (SimulationRunner.cpp:72-100) enqueued
+// without reading the status: lattice stage, moveLattice + remesh/coarsen,
+// deleteAllForces every 10th iteration, BioSolvers.
+void SimulationRunner::enqueueStep(unsigned it) {
+    params_.currentIteration = it;
     auto t0 = std::chrono::steady_clock::now();
+    check(lbib_step_begin(ctx_, it), "step begin");
+    pending_++;
     if (forceSolver_.dirty_) {
         syncMirror();
         uploadForces();
@@ -801,19 +817,67 @@ void SimulationRunner::step() {
     auto t1 = std::chrono::steady_clock::now();
     check(lbib_step_geometry(ctx_, it, it % 10 == 0), "moveLattice");
     mirrorValid_ = false;
-    handleGeometryStatus(it);
+    if (it % 10 == 0) {
+        // ForceSolver::deleteAllForces before the BioSolvers renew them
+        forceSolver_.deleteAllForces();
+        forceSolver_.dirty_ = false;
+        check(lbib_forces_clear(ctx_), "deleteAllForces");
+    }
     auto t2 = std::chrono::steady_clock::now();
     times.lattice += std::chrono::duration<double>(t1 - t0).count();
     times.geometry += std::chrono::duration<double>(t2 - t1).count();
     // mass solvers: MassSolverBoxOutlet is applied inside the collide kernel.
     for (auto& b : bio_) b->applyBioProcess(*this);
     times.bio += seconds_since(t2);
-    params_.currentIteration++;
+    params_.currentIteration = it + 1;
     times.steps++;
 }
 
+// Reads the status of everything enqueued; throws RunAheadStop when an
+// enqueued iteration stopped.  Host reads and uploads of the topology call it
+// first, so they never act on (or overwrite) state past a stop.
+void SimulationRunner::settle() {
+    if (pending_ == 0) return;
+    auto t0 = std::chrono::steady_clock::now();
+    check(lbib_read_status(ctx_, &lastStatus), "status");
+    times.wait += seconds_since(t0);
+    nm_ = (size_t)lastStatus.n_markers;
+    nc_ = (size_t)lastStatus.n_connections;
+    if (lastStatus.bits & kStopBits) throw RunAheadStop{lastStatus.step_iteration};
+    pending_ = 0;
+}
+
+// Iteration it stopped and nothing after it changed device state: restore the
+// launch state of iteration it + 1, finish it the way the synchronous loop
+// does, and drop host-side edits made while enqueueing the skipped iterations.
+void SimulationRunner::resolveStop(unsigned it) {
+    check(lbib_rewind(ctx_, it + 1), "rewind");
+    pending_ = 0;
+    times.stops++;
+    params_.currentIteration = it;
+    mirrorValid_ = false;
+    forceSolver_.dirty_ = false;
+    handleGeometryStatus(it);
+    auto t2 = std::chrono::steady_clock::now();
+    for (auto& b : bio_) b->applyBioProcess(*this);
+    times.bio += seconds_since(t2);
+    params_.currentIteration = it + 1;
+}
+
+void SimulationRunner::step() { run(1); }
+
 void SimulationRunner::run(unsigned steps) {
-    for (unsigned t = 0; t < steps; t++) step();
+    const unsigned end = params_.currentIteration + steps;
+    const unsigned window = std::max(1u, runAhead);
+    while (params_.currentIteration < end) {
+        try {
+            const unsigned last = std::min(end, params_.currentIteration + window);
+            while (params_.currentIteration < last) enqueueStep(params_.currentIteration);
+            settle();
+        } catch (const RunAheadStop& s) {
+            resolveStop(s.it);
+        }
+    }
 }
 
 void SimulationRunner::runSimulation() { run(params_.iterations); }

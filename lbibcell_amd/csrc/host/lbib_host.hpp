@@ -142,9 +142,12 @@ struct BioSolver {
     std::string name;
 };
 
+// Host seconds spent enqueueing each stage, waiting for the device at status
+// reads, and in host fallbacks.
 struct StageTimes {
-    double lattice = 0, geometry = 0, host = 0, bio = 0;
+    double lattice = 0, geometry = 0, host = 0, bio = 0, wait = 0;
     long steps = 0;
+    long stops = 0;   // enqueued iterations that stopped and were resolved
 };
 
 // SimulationRunner (LbmLib/include/SimulationRunner.hpp:43-231) fused with the
@@ -184,12 +187,19 @@ class SimulationRunner {
     std::map<unsigned, double> computeAreas();            // GeometryHandler.cpp:101-134 (device shoelace)
     StageTimes times;
     lbib_status lastStatus{};
-    bool syncEveryStep = true;
+    // Iterations enqueued before the device status is read (1 reads it after
+    // every iteration, as the reference loop does).  A stop (perturbation,
+    // pending remesh, capacity) is resolved from the iteration that raised it,
+    // so results do not depend on this value.
+    unsigned runAhead = 16;
 
  private:
     void check(int rc, const char* what);
     void uploadForces();
     void handleGeometryStatus(unsigned it);
+    void enqueueStep(unsigned it);
+    void settle();
+    void resolveStop(unsigned it);
     void hostTopologyFallback(unsigned it, bool grow);
     std::string describe(const lbib_status& s);
 
@@ -207,6 +217,7 @@ class SimulationRunner {
     std::vector<std::map<std::string, std::vector<std::string>>> bcDescriptorText_;
     bool mirrorValid_ = true;
     size_t nm_ = 0, nc_ = 0;
+    unsigned pending_ = 0;   // iterations enqueued since the last clear status
 };
 
 }  // namespace LbmLib

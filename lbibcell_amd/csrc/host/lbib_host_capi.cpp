@@ -192,11 +192,22 @@ long lbh_perturb_counts(void* p, int which) {
     auto* h = static_cast<Handle*>(p);
     return which == 0 ? h->geo.nodePerturbations : h->geo.connectionPerturbations;
 }
-// lattice, geometry, host, bio seconds and step count
+// host seconds (lattice, geometry, host fallbacks, bio enqueue; status
+// waits), the step count and the number of resolved stops
 void lbh_times(void* p, double* out) {
     auto* h = static_cast<Handle*>(p);
     const auto& t = h->runner->times;
     out[0] = t.lattice; out[1] = t.geometry; out[2] = t.host; out[3] = t.bio; out[4] = (double)t.steps;
+    out[5] = t.wait;
+    out[6] = (double)t.stops;
+}
+// iterations enqueued between status reads (1 = after every iteration)
+int lbh_set_run_ahead(void* p, unsigned n) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (!h->runner) LBIB_FAIL("run-ahead is set after setup");
+        h->runner->runAhead = n < 1 ? 1 : n;
+    });
 }
 void lbh_reset_times(void* p) { static_cast<Handle*>(p)->runner->times = StageTimes{}; }
 int lbh_last_status(void* p, lbib_status* st) {

@@ -10,6 +10,16 @@
 
 using namespace lbib;
 
+namespace lbib {
+// Opens an iteration: records it while the status is clear, otherwise marks
+// the rest of the enqueued work as skipped (a single thread; one stream).
+__global__ void k_step_begin(DevStatus* st, uint32_t it) {
+    const uint32_t b = st->bits;
+    if (b & STOP_BITS_STEP) st->bits = b | LBIB_SKIPPED;
+    else st->step_it = it;
+}
+}  // namespace lbib
+
 namespace {
 
 template <typename T>
@@ -240,14 +250,13 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     {
         const size_t nt = (size_t)c->stx * c->sty;
         if (dev_alloc(c, &c->tag, n) || dev_alloc(c, &c->tact, nt) || dev_alloc(c, &c->tcnt, nt) ||
-            dev_alloc(c, &c->toff, nt) || dev_alloc(c, &c->tflag, nt) || dev_alloc(c, &c->tpos, nt) ||
-            dev_alloc(c, &c->alist, nt))
+            dev_alloc(c, &c->toff, nt))
             return -1;
     }
     if (dev_alloc(c, &c->best_dist, 4 * n, 0xff) || dev_alloc(c, &c->best_seq, 4 * n, 0xff) ||
         dev_alloc(c, &c->bn_dom, 4 * n))
         return -1;
-    if (dev_alloc(c, &c->changed, 2 * (size_t)REINIT_CAP) || dev_alloc(c, &c->rkeys, (size_t)REINIT_CAP) ||
+    if (dev_alloc(c, &c->changed, 2 * (size_t)REINIT_CAP) || dev_alloc(c, &c->rmark, (size_t)c->n) ||
         dev_alloc(c, &c->rstate, (size_t)REINIT_CAP))
         return -1;
     if (dev_alloc(c, &c->st, 1)) return -1;
@@ -279,8 +288,8 @@ void lbib_destroy(lbib_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->ux); dev_free(c->uy); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
-    dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff); dev_free(c->tflag);
-    dev_free(c->tpos); dev_free(c->alist); dev_free(c->tslot); dev_free(c->tlist);
+    dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
+    dev_free(c->tslot); dev_free(c->tlist);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
                          &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
     uint32_t** uarr[18] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
@@ -294,7 +303,7 @@ void lbib_destroy(lbib_ctx* c) {
     for (auto a : iarr) dev_free(*a);
     dev_free(c->fs_prm); dev_free(c->bcd_off); dev_free(c->bcd_sp);
     dev_free(c->cand); dev_free(c->best_dist); dev_free(c->best_seq); dev_free(c->bn_dom); dev_free(c->bc_v);
-    dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rkeys); dev_free(c->rstate);
+    dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rmark); dev_free(c->rstate);
     dev_free(c->ring_start); dev_free(c->ring_cnt); dev_free(c->ring_off_u); dev_free(c->area);
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
@@ -632,6 +641,43 @@ int lbib_profile_read(lbib_ctx* c, double* kernel_ms, int64_t* launches, double*
     c->ev_used = 0;
     return 0;
 }
+int lbib_profile_copy(lbib_ctx* c, int64_t nbytes, int reps, double* kernel_ms) {
+    if (check_ctx(c)) return -1;
+    if (nbytes < 4096 || reps < 1 || !kernel_ms) {
+        c->err = "lbib_profile_copy: nbytes >= 4096, reps >= 1 and an output pointer are required";
+        return -1;
+    }
+    nbytes &= ~(int64_t)15;
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t e[2] = {nullptr, nullptr};
+    int rc = -1;
+    float t = 0.0f;
+    if (hipMalloc(&buf[0], (size_t)nbytes) != hipSuccess || hipMalloc(&buf[1], (size_t)nbytes) != hipSuccess ||
+        hipEventCreate(&e[0]) != hipSuccess || hipEventCreate(&e[1]) != hipSuccess) {
+        c->err = "lbib_profile_copy: allocation failed";
+        goto done;
+    }
+    if (hipMemsetAsync(buf[0], 0, (size_t)nbytes, c->stream) != hipSuccess) goto hip_fail;
+    if (launch_copy_probe(c, buf[0], buf[1], (long)nbytes)) goto done;  // warm-up
+    if (hipEventRecord(e[0], c->stream) != hipSuccess) goto hip_fail;
+    for (int r = 0; r < reps; r++)
+        if (launch_copy_probe(c, buf[r & 1], buf[1 - (r & 1)], (long)nbytes)) goto done;
+    if (hipEventRecord(e[1], c->stream) != hipSuccess || hipEventSynchronize(e[1]) != hipSuccess ||
+        hipEventElapsedTime(&t, e[0], e[1]) != hipSuccess)
+        goto hip_fail;
+    *kernel_ms = (double)t / reps;
+    rc = 0;
+    goto done;
+hip_fail:
+    c->err = std::string("lbib_profile_copy: ") + hipGetErrorString(hipGetLastError());
+done:
+    (void)hipStreamSynchronize(c->stream);
+    for (auto* b : buf)
+        if (b) (void)hipFree(b);
+    for (auto ev : e)
+        if (ev) (void)hipEventDestroy(ev);
+    return rc;
+}
 int lbib_reserve(lbib_ctx* c, int max_markers, int max_connections) {
     if (check_ctx(c) || max_markers < c->nm || max_connections < c->nc) {
         if (c) c->err = "reserve below the current counts";
@@ -662,8 +708,36 @@ int lbib_read_status(lbib_ctx* c, lbib_status* out) {
     out->n_connections = (int32_t)s.nc;
     out->n_remeshed = (int32_t)s.rm_total;
     out->n_coarsened = (int32_t)s.co_removed;
+    out->step_iteration = s.step_it;
+    out->n_reinit_levels = (int32_t)s.reinit_levels;
     c->nm = (int)s.nm;
     c->nc = (int)s.nc;
+    return 0;
+}
+int lbib_step_begin(lbib_ctx* c, uint32_t it) {
+    if (check_ctx(c)) return -1;
+    lbib::k_step_begin<<<1, 1, 0, c->stream>>>(c->st, it);
+    LBIB_HIP(hipGetLastError());
+    if (!c->snaps.empty() && c->snaps.back().it >= it) c->snaps.clear();   // a new run
+    c->snaps.push_back({it, c->cur, c->mtab_live, c->msite_live, c->ev_used});
+    if (c->snaps.size() > 1024) c->snaps.pop_front();
+    return 0;
+}
+int lbib_rewind(lbib_ctx* c, uint32_t it) {
+    if (check_ctx(c)) return -1;
+    if (c->snaps.empty() || it > c->snaps.back().it) return 0;   // iteration it never began
+    while (!c->snaps.empty() && c->snaps.back().it > it) c->snaps.pop_back();
+    if (c->snaps.empty() || c->snaps.back().it != it) {
+        c->snaps.clear();
+        c->err = "lbib_rewind: no snapshot for iteration " + std::to_string(it);
+        return -1;
+    }
+    const lbib_ctx::Snapshot sn = c->snaps.back();
+    c->snaps.pop_back();
+    c->cur = sn.cur;
+    c->mtab_live = sn.mtab_live;
+    c->msite_live = sn.msite_live;
+    c->ev_used = sn.ev_used;
     return 0;
 }
 int lbib_clear_status(lbib_ctx* c) {
@@ -678,7 +752,7 @@ int lbib_clear_status(lbib_ctx* c) {
 // -------------------------------------------------------------- bio helpers --
 int lbib_compute_areas(lbib_ctx* c, double* area) {
     if (check_ctx(c)) return -1;
-    if (launch_ring_areas(c)) return -1;
+    if (launch_ring_areas(c, nullptr)) return -1;
     if (area && c->ndom > 0) {
         std::vector<uint32_t> cnt(c->ndom);
         if (download(c, cnt.data(), c->ring_cnt, c->ndom * sizeof(uint32_t))) return -1;

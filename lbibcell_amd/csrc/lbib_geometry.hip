@@ -15,48 +15,47 @@
 
 namespace lbib {
 
-__device__ __forceinline__ bool halted(const DevStatus* st) {
-    return (st->bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) != 0u;
-}
+__device__ __forceinline__ bool halted(const DevStatus* st) { return (st->bits & STOP_BITS) != 0u; }
 
-// Connection::perturbConnection trigger (Connection.cpp:339-340, 363-364).
-__global__ void k_conn_check(int nc, const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
-                             const double* __restrict__ mx, const double* __restrict__ my, DevStatus* st) {
+// Start of the geometry stage, one launch: the Connection::perturbConnection
+// trigger (Connection.cpp:339-340, 363-364), deletion of the boundary nodes of
+// the last completed cure (GeometryHandler.cpp:363-369; their candidate count
+// is ncand_prev, fixed by k_cut_tie) and the stage counters.  A halt raised
+// meanwhile leaves both parts to be rerun, which repeats them unchanged.
+__global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
+                                                   const double* __restrict__ mx, const double* __restrict__ my,
+                                                   const Cand* __restrict__ cand, uint32_t* tag, DevStatus* st) {
     if (halted(st)) return;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nc) return;
-    const int a = ca[c], b = cb[c];
-    if (fabs(mx[a] - mx[b]) < kEPS || fabs(my[a] - my[b]) < kEPS)
-        atomicOr(&st->bits, LBIB_HALT_PERTURB_CONN);
-}
-
-// Delete the boundary nodes of the previous cure (GeometryHandler.cpp:363-369).
-__global__ void k_bn_reset(const Cand* __restrict__ cand, uint32_t* tag, DevStatus* st) {
-    if (halted(st)) return;
-    const uint32_t nprev = st->ncand;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nprev; j += gridDim.x * blockDim.x)
+    const uint32_t nc = st->nc, nprev = st->ncand_prev;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += stride) {
+        const int a = ca[c], b = cb[c];
+        if (fabs(mx[a] - mx[b]) < kEPS || fabs(my[a] - my[b]) < kEPS)
+            atomicOr(&st->bits, LBIB_HALT_PERTURB_CONN);
+    }
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nprev; j += stride)
         atomicAnd(&tag[cand[j].site], DOM_MASK);
-}
-
-__global__ void k_geo_counters(DevStatus* st) {
-    if (halted(st)) return;
-    st->ncand_prev = st->ncand;
-    st->ncand = 0;
-    st->nbn = 0;
-    st->nchanged = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->ncand = 0;
+        st->nbn = 0;
+        st->nchanged = 0;
+        st->rm_total = 0;
+        st->co_removed = 0;
+        st->reinit_levels = 0;
+    }
 }
 
 __device__ __forceinline__ double sqd(double x1, double y1, double x2, double y2) {
     return (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);   // EulerianPoint.hpp:88-96
 }
 
-__global__ void k_cut_gen(int nc, int nx, int ny, long n, const int32_t* __restrict__ ca,
+__global__ void k_cut_gen(int nx, int ny, long n, const int32_t* __restrict__ ca,
                           const int32_t* __restrict__ cb, const uint32_t* __restrict__ cdom,
                           const double* __restrict__ mx, const double* __restrict__ my, Cand* cand,
                           int candcap, unsigned long long* best_dist, DevStatus* st) {
     if (halted(st)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = c < nc;
+    const bool live = c < (int)st->nc;
     double x1 = 0, y1 = 0, x2 = 1, y2 = 1;
     if (live) { x1 = mx[ca[c]]; y1 = my[ca[c]]; x2 = mx[cb[c]]; y2 = my[cb[c]]; }
     const double maxX = fmax(x1, x2), maxY = fmax(y1, y2);
@@ -124,6 +123,7 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
                           uint32_t* best_seq, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->ncand;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->ncand_prev = nc;   // this cure's reset list
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
         const Cand e = cand[j];
         const long slotix = (long)(e.slot - 1) * n + e.site;
@@ -176,7 +176,7 @@ __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_
 // occupied slot reports the site.  Also resets the selection buffers.
 __global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_t* __restrict__ tag,
                             unsigned long long* best_dist, uint32_t* best_seq, uint32_t* changed,
-                            int local_update, DevStatus* st) {
+                            uint32_t* rmark, int local_update, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->ncand;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
@@ -194,6 +194,7 @@ __global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_
             if (p < (uint32_t)REINIT_CAP) {
                 changed[2 * p] = (uint32_t)e.site;
                 changed[2 * p + 1] = e.domain;
+                rmark[e.site] = p + 1;
             } else {
                 atomicOr(&st->bits, LBIB_ERR_CAPACITY);
             }
@@ -210,6 +211,7 @@ struct ReinitArgs {
     const double* uy;
     uint32_t* tag;
     const uint32_t* changed;
+    const uint32_t* rmark;
     uint32_t it;
     SpeciesDev sp[LBIB_MAX_SPECIES];
 };
@@ -266,53 +268,29 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
 // GeometryHandler::updateDomainIdentifier (GeometryHandler.cpp:58-66) runs the
 // per-site update in (y, x) order and each re-initialisation
 // (CDESolverD2Q5.cpp:156-228, ..._SIGNAL.cpp:184-250) reads neighbours updated
-// earlier in that order: sort the changed sites, then apply them in order.
-// The sequential pass of the reference is replayed level by level: a changed
-// site is ready once every changed neighbour that precedes it in (y, x) order
-// has been processed; neighbours that follow it depend on it, so they still
-// hold their old values when it runs.  Ready sites are never adjacent, which
-// makes each level safe to run in parallel and the result identical.
+// earlier in that order.  The sequential pass is replayed level by level: a
+// changed site is ready once every changed neighbour that precedes it in
+// (y, x) order (a smaller site index) has been processed; neighbours that
+// follow it still hold their old values when it runs, as in the reference.
+// Ready sites are never adjacent, so each level runs in parallel and the
+// result is identical.  A changed neighbour is found through rmark[site]
+// (1 + its position in the list, checked against the list, so entries left by
+// earlier steps never match).
 template <int NS>
-__global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, unsigned long long* gkeys, uint8_t* gstate,
-                                                 DevStatus* st) {
+__global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gstate, DevStatus* st) {
     if (halted(st)) return;
-    __shared__ unsigned long long lkey[REINIT_LDS];
     __shared__ uint8_t lstate[REINIT_LDS];
     __shared__ int pending;
     const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
     if (cnt == 0) return;
-    uint32_t P = 1;
-    while (P < cnt) P <<= 1;
-    const bool in_lds = P <= (uint32_t)REINIT_LDS;
-    unsigned long long* key = in_lds ? lkey : gkeys;
-    uint8_t* state = in_lds ? lstate : gstate;
-    for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-        key[t] = (t < cnt) ? (((unsigned long long)a.changed[2 * t] << 32) | a.changed[2 * t + 1]) : ~0ull;
-        if (t < cnt) state[t] = 0;
-    }
+    uint8_t* state = (cnt <= (uint32_t)REINIT_LDS) ? lstate : gstate;
+    for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) state[t] = 0;
     __syncthreads();
-    for (uint32_t kk = 2; kk <= P; kk <<= 1) {
-        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-            for (uint32_t t = threadIdx.x; t < P; t += blockDim.x) {
-                const uint32_t l = t ^ j;
-                if (l > t) {
-                    const bool up = (t & kk) == 0;
-                    const unsigned long long u = key[t], v = key[l];
-                    if ((u > v) == up) { key[t] = v; key[l] = u; }
-                }
-            }
-            __syncthreads();
-        }
-    }
     const int EXd[8] = {1, 0, -1, 0, 1, -1, -1, 1};
     const int EYd[8] = {0, 1, 0, -1, 1, 1, -1, -1};
     auto find = [&](uint32_t site) -> int {
-        int lo = 0, hi = (int)cnt - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if ((uint32_t)(key[mid] >> 32) < site) lo = mid + 1; else hi = mid;
-        }
-        return ((uint32_t)(key[lo] >> 32) == site) ? lo : -1;
+        const uint32_t m = a.rmark[site];
+        return (m != 0u && m <= cnt && a.changed[2 * (m - 1)] == site) ? (int)(m - 1) : -1;
     };
     for (int level = 0; level < (1 << 20); level++) {
         if (threadIdx.x == 0) pending = 0;
@@ -320,7 +298,7 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, unsigned lo
         // phase 1: mark the ready sites (state 1)
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
             if (state[t] != 0) continue;
-            const uint32_t i = (uint32_t)(key[t] >> 32);
+            const uint32_t i = a.changed[2 * t];
             const int x = (int)(i % a.nx), y = (int)(i / a.nx);
             bool ready = true;
             for (int d = 0; d < 8 && ready; d++) {
@@ -338,14 +316,17 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, unsigned lo
         // phase 2: run them
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
             if (state[t] != 1) continue;
-            reinit_site<NS>(a, (long)(key[t] >> 32), (uint32_t)(key[t] & 0xffffffffu));
+            reinit_site<NS>(a, (long)a.changed[2 * t], a.changed[2 * t + 1]);
         }
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
             if (state[t] == 1) state[t] = 2;
         const int more = pending;
         __syncthreads();
-        if (!more) break;
+        if (!more) {
+            if (threadIdx.x == 0) st->reinit_levels = (uint32_t)level + 1;
+            break;
+        }
     }
 }
 
@@ -397,28 +378,9 @@ __global__ __launch_bounds__(256) void k_scanline(int nx, long n, uint32_t* tag,
     }
 }
 
-// tutorial_01_BioSolverAreaRegulator.cpp:66-99, per domain.
-__global__ void k_mass_default(int ndom, const uint32_t* __restrict__ celltype, double maxs, double* mtab) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= ndom) return;
-    mtab[d] = (d == 0) ? 0.0 : ((celltype[d] == 1u) ? maxs : 0.0);
-}
-__global__ void k_mass_regulated(const uint32_t* __restrict__ ring_cnt, const double* __restrict__ area,
-                                 const uint32_t* __restrict__ celltype, double* target, double target_area,
-                                 double kp, double maxs, double factor, int ndom, double* mtab) {
-    const int d = blockIdx.x * blockDim.x + threadIdx.x;
-    if (d >= ndom || ring_cnt[d] == 0) return;
-    const double A = area[d];
-    if (isnan(target[d])) target[d] = (factor > 0.0) ? factor * A : target_area;
-    double src = kp * (target[d] - A);
-    if (src > maxs) src = maxs;
-    else if (src < -maxs) src = -maxs;
-    if (d != 0u && celltype[d] != 1u) mtab[d] = src;
-}
-
 template <int NS>
 static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
-    k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rkeys, c->rstate, c->st);
+    k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rstate, c->st);
 }
 
 int launch_topology(lbib_ctx* c, bool coarsen);
@@ -430,24 +392,23 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     const bool coarsen = (mode & 2) != 0;
     const int bs = 256;
     const int gs = 1024;
-    if (c->nc > 0)
-        k_conn_check<<<(c->nc + bs - 1) / bs, bs, 0, c->stream>>>(c->nc, c->ca, c->cb, c->mx, c->my, c->st);
-    k_bn_reset<<<gs, bs, 0, c->stream>>>(c->cand, c->tag, c->st);
-    k_geo_counters<<<1, 1, 0, c->stream>>>(c->st);
-    if (c->nc > 0)
-        k_cut_gen<<<(c->nc + bs - 1) / bs, bs, 0, c->stream>>>(c->nc, c->nx, c->ny, c->n, c->ca, c->cb, c->cdom,
+    // connection counts are device-resident (remeshing changes them): grids
+    // cover the capacity and the kernels read st->nc
+    k_geo_begin<<<gs, bs, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->cand, c->tag, c->st);
+    if (c->ccap > 0)
+        k_cut_gen<<<(c->ccap + bs - 1) / bs, bs, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom,
                                                                c->mx, c->my, c->cand, c->candcap, c->best_dist,
                                                                c->st);
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
     k_cut_win<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my, c->mvx,
                                         c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->st);
-    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, cure ? 0 : 1,
+    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, c->rmark, cure ? 0 : 1,
                                           c->st);
     if (!cure) {
         ReinitArgs a;
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;
         a.g = (double*)c->g[c->cur];
-        a.ux = c->ux; a.uy = c->uy; a.tag = c->tag; a.changed = c->changed; a.it = it;
+        a.ux = c->ux; a.uy = c->uy; a.tag = c->tag; a.changed = c->changed; a.rmark = c->rmark; a.it = it;
         for (int s = 0; s < LBIB_MAX_SPECIES; s++) a.sp[s] = c->sp[s];
         switch (c->ns) {
             case 0: launch_reinit_t<0>(c, a); break;
@@ -467,7 +428,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
 // (SimulationRunner.cpp:83-87); rings are rebuilt for the area helpers.
 int launch_topology(lbib_ctx* c, bool coarsen) {
     if (c->nc == 0) return 0;
-    if (launch_remesh(c, 2)) return -1;
+    if (launch_remesh(c, 1)) return -1;
     if (launch_rings(c)) return -1;
     if (coarsen) {
         if (launch_coarsen(c)) return -1;
@@ -478,11 +439,8 @@ int launch_topology(lbib_ctx* c, bool coarsen) {
 
 int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor) {
     if (c->ndom == 0) return 0;
-    if (launch_ring_areas(c)) return -1;
-    k_mass_default<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->celltype, maxs, c->mtab);
-    k_mass_regulated<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ring_cnt, c->area, c->celltype, c->target,
-                                                                   target, kp, maxs, factor, c->ndom, c->mtab);
-    LBIB_HIP(hipGetLastError());
+    const AreaMass am{c->celltype, c->target, c->mtab, target, kp, maxs, factor};
+    if (launch_ring_areas(c, &am)) return -1;
     c->mtab_live = true;
     return 0;
 }

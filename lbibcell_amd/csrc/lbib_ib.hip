@@ -23,12 +23,13 @@ namespace lbib {
 
 // ForceSolver::calculateForce (ForceSolver.cpp:42-85) with the force structs of
 // ForceStructs.hpp:49-588; per marker, structs in map/push order.
-__global__ void k_marker_force(int nm, const double* __restrict__ mx, const double* __restrict__ my,
+__global__ void k_marker_force(const double* __restrict__ mx, const double* __restrict__ my,
                                const int32_t* __restrict__ off, const int32_t* __restrict__ type,
                                const int32_t* __restrict__ partner, const double* __restrict__ prm,
-                               double* __restrict__ mfx, double* __restrict__ mfy) {
+                               double* __restrict__ mfx, double* __restrict__ mfy, const DevStatus* st) {
+    if (step_stopped(st)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nm) return;
+    if (k >= (int)st->nm) return;
     const double x1 = mx[k], y1 = my[k];
     double fx = 0.0, fy = 0.0;
     for (int j = off[k]; j < off[k + 1]; j++) {
@@ -82,17 +83,20 @@ constexpr int ST = 16;            // spread tile edge
 constexpr int SP_IDX = 1024;      // markers gathered per tile
 constexpr int SP_CH = 256;        // marker data staged per pass
 
-__global__ void k_tile_clear(int ntiles, uint32_t* tcnt, uint8_t* tact) {
+__global__ void k_tile_clear(int ntiles, uint32_t* tcnt, uint8_t* tact, DevStatus* st) {
+    if (step_stopped(st)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->ntouched = 0;
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
         tcnt[t] = 0;
         tact[t] = 0;
     }
 }
 
-__global__ void k_tile_count(int nm, int ntx, int nty, const double* __restrict__ mx, const double* __restrict__ my,
-                             uint32_t* tcnt, uint32_t* tslot, uint8_t* tact) {
+__global__ void k_tile_count(int ntx, int nty, const double* __restrict__ mx, const double* __restrict__ my,
+                             uint32_t* tcnt, uint32_t* tslot, uint8_t* tact, const DevStatus* st) {
+    if (step_stopped(st)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nm) return;
+    if (k >= (int)st->nm) return;
     const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
     const int home = (fy / ST) * ntx + fx / ST;
     tslot[k] = atomicAdd(&tcnt[home], 1u);
@@ -101,34 +105,29 @@ __global__ void k_tile_count(int nm, int ntx, int nty, const double* __restrict_
             if (tx < ntx && ty < nty) tact[ty * ntx + tx] = 1;
 }
 
-__global__ void k_tile_scatter(int nm, int ntx, const double* __restrict__ mx, const double* __restrict__ my,
-                               const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tslot, int32_t* tlist) {
+__global__ void k_tile_scatter(int ntx, const double* __restrict__ mx, const double* __restrict__ my,
+                               const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tslot, int32_t* tlist,
+                               const DevStatus* st) {
+    if (step_stopped(st)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nm) return;
+    if (k >= (int)st->nm) return;
     const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
     tlist[toff[(fy / ST) * ntx + fx / ST] + tslot[k]] = k;
 }
 
-__global__ void k_tile_active(int ntiles, const uint8_t* __restrict__ tact, uint32_t* tflag) {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) tflag[t] = tact[t];
-}
-__global__ void k_tile_compact(int ntiles, const uint32_t* __restrict__ tflag, const uint32_t* __restrict__ tpos,
-                               int32_t* alist) {
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x)
-        if (tflag[t]) alist[tpos[t]] = t;
-}
-
-__global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const int32_t* __restrict__ alist,
+// one workgroup per tile; tiles no marker stencil reaches return at once
+__global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
                                                      const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
                                                      const int32_t* __restrict__ tlist, const double* __restrict__ mx,
                                                      const double* __restrict__ my, const double* __restrict__ mfx,
                                                      const double* __restrict__ mfy, double* Fx, double* Fy,
                                                      DevStatus* st) {
-    if ((uint32_t)blockIdx.x >= st->ntouched) return;
+    const int t = blockIdx.x;
+    if (!tact[t] || step_stopped(st)) return;
     __shared__ int32_t idx[SP_IDX];
     __shared__ double sx[SP_CH], sy[SP_CH], sfx[SP_CH], sfy[SP_CH];
     __shared__ uint32_t n;
-    const int t = alist[blockIdx.x];
+    if (threadIdx.x == 0) atomicAdd(&st->ntouched, 1u);
     const int tx = t % ntx, ty = t / ntx;
     if (threadIdx.x == 0) n = 0;
     __syncthreads();
@@ -202,11 +201,12 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
 // and detection of the perturbation cases of GeometryNode::perturbatePosition
 // (GeometryNode.cpp:153-214) and of the boundary checks of
 // GeometryHandler.cpp:613-636.
-__global__ void k_interp_move(int nm, int nx, int ny, double* mx, double* my, double* mvx,
+__global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mvx,
                               double* mvy, double* mox, double* moy, const double* __restrict__ ux,
                               const double* __restrict__ uy, DevStatus* st) {
+    if (step_stopped(st)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nm) return;
+    if (k >= (int)st->nm) return;
     const double X = mx[k], Y = my[k];
     const int fx = (int)floor(X), fy = (int)floor(Y);
     const int base = 4 * (fx - 1) + (fy - 1);
@@ -247,30 +247,31 @@ __global__ void k_interp_move(int nm, int nx, int ny, double* mx, double* my, do
     }
 }
 
-__global__ void k_reset_spread_counters(DevStatus* st) { st->ntouched = 0; }
+__global__ void k_reset_spread_counters(DevStatus* st) {
+    if (!step_stopped(st)) st->ntouched = 0;
+}
 
 static inline int nblocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 int launch_force_and_spread(lbib_ctx* c) {
     const int bs = 256;
     const int ntiles = c->stx * c->sty;
-    k_tile_clear<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tcnt, c->tact);
+    k_tile_clear<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tcnt, c->tact, c->st);
     if (c->nm == 0) {
         k_reset_spread_counters<<<1, 1, 0, c->stream>>>(c->st);
         LBIB_HIP(hipGetLastError());
         return 0;
     }
-    k_marker_force<<<nblocks(c->nm, bs), bs, 0, c->stream>>>(c->nm, c->mx, c->my, c->fs_off, c->fs_type,
-                                                             c->fs_partner, c->fs_prm, c->mfx, c->mfy);
-    k_tile_count<<<nblocks(c->nm, bs), bs, 0, c->stream>>>(c->nm, c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
-                                                           c->tact);
+    // marker counts are device-resident (remeshing changes them): grids cover
+    // the capacity and the kernels read st->nm
+    k_marker_force<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->mx, c->my, c->fs_off, c->fs_type, c->fs_partner,
+                                                               c->fs_prm, c->mfx, c->mfy, c->st);
+    k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
+                                                             c->tact, c->st);
     if (scan_exclusive(c, c->tcnt, c->toff, nullptr, (uint32_t)ntiles, (uint32_t)ntiles, nullptr)) return -1;
-    k_tile_scatter<<<nblocks(c->nm, bs), bs, 0, c->stream>>>(c->nm, c->stx, c->mx, c->my, c->toff, c->tslot,
-                                                             c->tlist);
-    k_tile_active<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tact, c->tflag);
-    if (scan_exclusive(c, c->tflag, c->tpos, nullptr, (uint32_t)ntiles, (uint32_t)ntiles, &c->st->ntouched)) return -1;
-    k_tile_compact<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tflag, c->tpos, c->alist);
-    k_tile_spread<<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->alist, c->toff, c->tcnt, c->tlist,
+    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist,
+                                                               c->st);
+    k_tile_spread<<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt, c->tlist,
                                                  c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
@@ -280,8 +281,8 @@ int launch_clear_force(lbib_ctx*) { return 0; }
 
 int launch_interp_move(lbib_ctx* c) {
     if (c->nm == 0) return 0;
-    k_interp_move<<<nblocks(c->nm, 256), 256, 0, c->stream>>>(c->nm, c->nx, c->ny, c->mx, c->my, c->mvx,
-                                                             c->mvy, c->mox, c->moy, c->ux, c->uy, c->st);
+    k_interp_move<<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy,
+                                                               c->mox, c->moy, c->ux, c->uy, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

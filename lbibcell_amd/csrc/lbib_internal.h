@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <deque>
 #include <string>
 #include <vector>
 #include "../../include/lbib.h"
@@ -27,8 +28,8 @@ constexpr int BN_SHIFT = 24;        // bit 24 + (slot-1), slot in E,N,W,S
 constexpr int BC_SHIFT = 28;
 constexpr int TILE_W = 256;         // lattice kernel block width = force tile width
 constexpr int MAX_BC_SPECIES = LBIB_MAX_SPECIES;
-constexpr int REINIT_LDS = 4096;          // changed sites sorted in LDS
-constexpr int REINIT_CAP = 1 << 20;       // changed sites per step (global-memory sort beyond REINIT_LDS)
+constexpr int REINIT_LDS = 8192;          // level state of the changed sites kept in LDS
+constexpr int REINIT_CAP = 1 << 20;       // changed sites per step (global-memory state beyond REINIT_LDS)
 constexpr int SEQ_PER_CONN = 256;   // candidates per connection (segment < 127 lu)
 
 struct SpeciesDev {
@@ -47,7 +48,7 @@ struct DevStatus {
     uint32_t ntouched;      // spread: 16x16 tiles written this step
     uint32_t pool_used;     // unused
     uint32_t ncand;         // cut-link candidates of the current step
-    uint32_t ncand_prev;    // ... of the previous step (reset list)
+    uint32_t ncand_prev;    // candidates of the last completed cure (boundary-node reset list)
     uint32_t nbn;           // winners
     uint32_t nchanged;      // sites whose local domain changed
     uint32_t nm, nc, maxid; // topology counts and the largest GeometryNode id
@@ -55,7 +56,17 @@ struct DevStatus {
     uint32_t rm_total;      // connections split this step
     uint32_t co_any, co_removed;
     uint32_t co_nkeepc, co_nkeepm, co_nkeepr;
+    uint32_t step_it;       // iteration of the last step begun with a clear status
+    uint32_t rm_nc0, rm_nm0;   // counts before the current remesh round
+    uint32_t reinit_levels;    // dependency levels of this step's CDE re-initialisation
 };
+
+// A stopped status makes later kernels return without work.  Geometry and
+// topology kernels stop on errors, halts and skipped steps; the lattice, IB,
+// force and area-regulator kernels also while a remesh round is pending.
+constexpr uint32_t STOP_BITS = LBIB_ERR_MASK | LBIB_HALT_MASK | LBIB_SKIPPED;
+constexpr uint32_t STOP_BITS_STEP = STOP_BITS | LBIB_FLAG_REMESH;
+__device__ __forceinline__ bool step_stopped(const DevStatus* st) { return (st->bits & STOP_BITS_STEP) != 0u; }
 
 struct SpreadEntry {
     int32_t marker;
@@ -96,9 +107,9 @@ struct lbib_ctx {
     lbib::SpeciesDev sp[LBIB_MAX_SPECIES];
     // spread over 16x16 tiles
     int stx = 0, sty = 0;
-    uint32_t *tcnt = nullptr, *toff = nullptr, *tflag = nullptr, *tpos = nullptr, *tslot = nullptr;
+    uint32_t *tcnt = nullptr, *toff = nullptr, *tslot = nullptr;
     uint8_t* tact = nullptr;
-    int32_t *tlist = nullptr, *alist = nullptr;
+    int32_t* tlist = nullptr;
     // markers
     int mcap = 0, nm = 0;
     double *mx = nullptr, *my = nullptr, *mvx = nullptr, *mvy = nullptr;
@@ -139,7 +150,7 @@ struct lbib_ctx {
     double* bc_v = nullptr;                    // [4][n][2]
     int32_t* bc_desc = nullptr;                // [4][n]
     uint32_t* changed = nullptr;               // [REINIT_CAP][2]
-    unsigned long long* rkeys = nullptr;       // [REINIT_CAP] sort scratch
+    uint32_t* rmark = nullptr;                 // [n] 1 + position in changed (validated on read)
     uint8_t* rstate = nullptr;                 // [REINIT_CAP] level state
     // rings / domains (indexed by domain id)
     int dcap = 0, ndom = 0;
@@ -154,6 +165,14 @@ struct lbib_ctx {
     bool prof_on = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
+    // run-ahead: host launch state when each iteration began
+    struct Snapshot {
+        uint32_t it;
+        int cur;
+        bool mtab_live, msite_live;
+        size_t ev_used;
+    };
+    std::deque<Snapshot> snaps;
     // status
     lbib::DevStatus* st = nullptr;
     lbib::DevStatus* st_host = nullptr;   // pinned
@@ -163,6 +182,7 @@ namespace lbib {
 // launchers implemented per translation unit
 int launch_init_f64(lbib_ctx* c, const double* u0_dev);
 int launch_stream_collide_f64(lbib_ctx* c, uint32_t it, bool use_ftile);
+int launch_copy_probe(lbib_ctx* c, const void* src, void* dst, long nbytes);
 int launch_init_mixed(lbib_ctx* c, const double* u0_dev);
 int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile);
 int launch_force_and_spread(lbib_ctx* c);
@@ -174,7 +194,16 @@ int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_
                    uint32_t* total);
 int launch_inout(lbib_ctx* c);
 int launch_rings(lbib_ctx* c);
-int launch_ring_areas(lbib_ctx* c);
+// per-domain inputs and outputs of the area regulator's mass source
+struct AreaMass {
+    const uint32_t* celltype;
+    double* target;
+    double* mtab;
+    double target_area, kp, maxs, factor;
+};
+// mass != nullptr: also the area regulator's mass table, skipped while the
+// status is stopped (an iteration's BioSolver stage)
+int launch_ring_areas(lbib_ctx* c, const AreaMass* mass);
 int launch_remesh(lbib_ctx* c, int rounds);
 int launch_coarsen(lbib_ctx* c);
 int launch_forces_clear(lbib_ctx* c);

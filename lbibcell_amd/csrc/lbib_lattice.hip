@@ -39,11 +39,13 @@ struct LatticeArgs {
     double tauI, tau;
     uint32_t it;
     int box_outlet;
+    const DevStatus* st;   // the step is skipped while the status is stopped
     SpeciesDev sp[LBIB_MAX_SPECIES];
 };
 
 template <int NS, int FM>
 __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs a) {
+    if (step_stopped(a.st)) return;
     const int x = blockIdx.x * TILE_W + threadIdx.x;
     const int y = blockIdx.y;
     if (x >= a.nx) return;
@@ -247,6 +249,7 @@ static LatticeArgs make_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
     a.tau = c->cfg.tau_fluid;
     a.it = it;
     a.box_outlet = c->cfg.box_outlet;
+    a.st = c->st;
     for (int s = 0; s < LBIB_MAX_SPECIES; s++) a.sp[s] = c->sp[s];
     return a;
 }
@@ -296,6 +299,28 @@ int launch_init_f64(lbib_ctx* c, const double* u0_dev) {
     if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_init<0>(c, a, u0_dev);
     else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_init<1>(c, a, u0_dev);
     else dispatch_init<2>(c, a, u0_dev);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+// Plain streaming copy on the context stream: the attainable-bandwidth
+// yardstick for the lattice kernel (same nontemporal store path) and the
+// scale for hardware byte counters.
+typedef double probe_vec __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void k_copy_probe(const probe_vec* __restrict__ src, probe_vec* __restrict__ dst,
+                                                    long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) __builtin_nontemporal_store(src[i], &dst[i]);
+}
+
+int launch_copy_probe(lbib_ctx* c, const void* src, void* dst, long nbytes) {
+    const long n = nbytes / (long)sizeof(probe_vec);
+    const long nb = (n + 255) / 256;
+    if (n <= 0 || nb > 0x7fffffffL) {
+        c->err = "copy probe size out of range";
+        return -1;
+    }
+    k_copy_probe<<<(unsigned)nb, 256, 0, c->stream>>>((const probe_vec*)src, (probe_vec*)dst, n);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

@@ -25,9 +25,7 @@
 
 namespace lbib {
 
-__device__ __forceinline__ bool topo_halted(const DevStatus* st) {
-    return (st->bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) != 0u;
-}
+__device__ __forceinline__ bool topo_halted(const DevStatus* st) { return (st->bits & STOP_BITS) != 0u; }
 
 // ------------------------------------------------------------------ scan ----
 // Exclusive scan of n (device-resident count) uint32 values, blocks of 1024.
@@ -106,10 +104,50 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
     }
 }
 
+// Short arrays (tiles, domains): one workgroup, 4096 values per pass.
+constexpr uint32_t SCAN_SINGLE_MAX = 1u << 16;
+__global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict__ in, const uint32_t* nptr,
+                                                      uint32_t nfix, uint32_t* __restrict__ out, uint32_t* total) {
+    const uint32_t n = nptr ? *nptr : nfix;
+    __shared__ uint32_t s[1024];
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < n; b0 += 4096) {
+        uint32_t v[4];
+        uint32_t loc = 0;
+        for (int q = 0; q < 4; q++) {
+            const uint32_t i = b0 + threadIdx.x * 4 + q;
+            v[q] = (i < n) ? in[i] : 0u;
+            loc += v[q];
+        }
+        s[threadIdx.x] = loc;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const uint32_t t = (threadIdx.x >= (uint32_t)o) ? s[threadIdx.x - o] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += t;
+            __syncthreads();
+        }
+        uint32_t run = carry + s[threadIdx.x] - loc;
+        for (int q = 0; q < 4; q++) {
+            const uint32_t i = b0 + threadIdx.x * 4 + q;
+            if (i < n) out[i] = run;
+            run += v[q];
+        }
+        carry += s[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
 int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
                    uint32_t* total) {
     const uint32_t nb = (cap + SCAN_B - 1) / SCAN_B;
     if (nb == 0) return 0;
+    if (cap <= SCAN_SINGLE_MAX) {
+        k_scan_single<<<1, 1024, 0, c->stream>>>(in, nptr, nfix, out, total);
+        LBIB_HIP(hipGetLastError());
+        return 0;
+    }
     if (nb > c->scan_cap) {
         c->err = "scan scratch too small";
         return -1;
@@ -141,18 +179,25 @@ __device__ __forceinline__ bool needs_perturbation(const double X, const double 
     return p1 || p2 || p3 || p4;
 }
 
+// flag != nullptr: mark this round's long connections (and the perturbation
+// halt of their midpoints); flag == nullptr: only raise LBIB_FLAG_REMESH if a
+// connection is still long after the rounds of this pass.
 __global__ void k_rm_flag(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb, const double* __restrict__ mx,
                           const double* __restrict__ my, uint32_t* flag, DevStatus* st) {
     if (topo_halted(st)) return;
     const uint32_t nc = st->nc;
+    bool any = false;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         const int a = ca[c], b = cb[c];
         const double len = sqrt((mx[a] - mx[b]) * (mx[a] - mx[b]) + (my[a] - my[b]) * (my[a] - my[b]));
         const bool lg = len > kMAXLENGTH;
+        any |= lg;
+        if (!flag) continue;
         flag[c] = lg ? 1u : 0u;
         if (lg && needs_perturbation((mx[a] + mx[b]) / 2, (my[a] + my[b]) / 2))
             atomicOr(&st->bits, LBIB_HALT_PERTURB_REMESH);
     }
+    if (!flag && __any(any) && (threadIdx.x & 63) == 0) atomicOr(&st->bits, LBIB_FLAG_REMESH);
 }
 
 struct TopoArrays {
@@ -166,13 +211,23 @@ struct TopoArrays {
 };
 
 // Scatter one remesh round into the scratch connection arrays and append the
-// midpoint markers.  L = number of long connections (st->rm_L).
+// midpoint markers.  L = number of long connections (st->rm_L).  Capacity is
+// checked first, before anything is written (every block evaluates the same
+// condition; one raises the halt).
 __global__ void k_rm_apply(TopoArrays t, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank, int nx,
-                           int ny, DevStatus* st) {
+                           int ny, uint32_t mcap, uint32_t ccap, DevStatus* st) {
     if (topo_halted(st)) return;
     const uint32_t L = st->rm_L;
     if (L == 0) return;
     const uint32_t nc = st->nc, nm = st->nm, maxid = st->maxid;
+    if (nm + L + 1 > mcap || nc + L > ccap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&st->bits, LBIB_HALT_GROW);
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rm_nc0 = nc;
+        st->rm_nm0 = nm;
+    }
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
         const uint32_t r = rank[c];
         if (!flag[c]) {
@@ -199,47 +254,29 @@ __global__ void k_rm_apply(TopoArrays t, const uint32_t* __restrict__ flag, cons
     }
 }
 
-__global__ void k_rm_commit(TopoArrays t, DevStatus* st) {
+// Copy the round's connections back, derive each marker's incoming/outgoing
+// connection from them, give the new markers empty force rows and advance the
+// counts (the workgroups read the counts the round started from).
+__global__ void k_rm_commit(TopoArrays t, int32_t* min_c, int32_t* mout_c, DevStatus* st) {
     if (topo_halted(st)) return;
     const uint32_t L = st->rm_L;
     if (L == 0) return;
-    const uint32_t nn = st->nc + L;
+    const uint32_t nn = st->rm_nc0 + L, nm = st->rm_nm0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nn; j += gridDim.x * blockDim.x) {
-        t.ca[j] = t.ca2[j]; t.cb[j] = t.cb2[j]; t.cdom[j] = t.cdom2[j]; t.cbc[j] = t.cbc2[j];
+        const int32_t a = t.ca2[j], b = t.cb2[j];
+        t.ca[j] = a; t.cb[j] = b; t.cdom[j] = t.cdom2[j]; t.cbc[j] = t.cbc2[j];
+        mout_c[a] = (int32_t)j;
+        min_c[b] = (int32_t)j;
     }
     // new markers start with no force structs
-    const uint32_t nm = st->nm;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < L; r += gridDim.x * blockDim.x)
         t.fs_off[nm + 1 + r] = t.fs_off[nm];
-}
-
-__global__ void k_rm_counts(DevStatus* st, uint32_t mcap, uint32_t ccap) {
-    if (topo_halted(st)) return;
-    const uint32_t L = st->rm_L;
-    st->rm_total += L;
-    st->nm += L;
-    st->nc += L;
-    st->maxid += L;
-    if (st->nm + 1 > mcap || st->nc > ccap) atomicOr(&st->bits, LBIB_ERR_CAPACITY);
-}
-
-// capacity guard once the round's size L is known (before anything is written)
-__global__ void k_rm_guard(DevStatus* st, uint32_t mcap, uint32_t ccap) {
-    if (st->bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) return;
-    const uint32_t L = st->rm_L;
-    if (st->nm + L + 1 > mcap || st->nc + L > ccap) atomicOr(&st->bits, LBIB_HALT_GROW);
-}
-
-__global__ void k_rm_pending(const uint32_t* __restrict__ flag, DevStatus* st) {
-    if (topo_halted(st)) return;
-    const uint32_t nc = st->nc;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
-        if (flag[c]) { atomicOr(&st->bits, LBIB_FLAG_REMESH); return; }
-}
-
-__global__ void k_rm_reset(DevStatus* st) {
-    st->rm_total = 0;
-    st->co_removed = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rm_total += L;
+        st->nm = nm + L;
+        st->nc = nn;
+        st->maxid += L;
+    }
 }
 
 // ------------------------------------------------------------- rings -------
@@ -345,12 +382,32 @@ __global__ __launch_bounds__(1024) void k_ring_order(const int32_t* __restrict__
 
 // Shoelace area in ring order (FP64, sequential sum as the reference's
 // boost::geometry::area call site walks the ring), vertices staged in LDS.
+// With mass: tutorial_01_BioSolverAreaRegulator.cpp:66-99 for the domain as
+// well: default source (maxsource for cell type 1, else 0; none in domain 0),
+// then for a present ring the clamped proportional source, the target being
+// fixed on first use.
+__device__ __forceinline__ void area_regulator_mass(const AreaMass& am, int d, bool present, double A) {
+    const uint32_t ct = am.celltype[d];
+    am.mtab[d] = (d == 0) ? 0.0 : ((ct == 1u) ? am.maxs : 0.0);
+    if (!present) return;
+    if (isnan(am.target[d])) am.target[d] = (am.factor > 0.0) ? am.factor * A : am.target_area;
+    double src = am.kp * (am.target[d] - A);
+    if (src > am.maxs) src = am.maxs;
+    else if (src < -am.maxs) src = -am.maxs;
+    if (d != 0 && ct != 1u) am.mtab[d] = src;
+}
+
 __global__ __launch_bounds__(256) void k_ring_area(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ roff,
                                                    const int32_t* __restrict__ ring_marker, const double* __restrict__ mx,
-                                                   const double* __restrict__ my, double* area) {
+                                                   const double* __restrict__ my, double* area,
+                                                   const DevStatus* gate, const AreaMass am, int with_mass) {
+    if (gate && step_stopped(gate)) return;
     const int d = blockIdx.x;
     const uint32_t n = cnt[d];
-    if (n == 0) return;
+    if (n == 0) {
+        if (with_mass && threadIdx.x == 0) area_regulator_mass(am, d, false, 0.0);
+        return;
+    }
     constexpr int CH = 1024;
     __shared__ double sx[CH + 1], sy[CH + 1];
     const uint32_t base = roff[d];
@@ -373,11 +430,15 @@ __global__ __launch_bounds__(256) void k_ring_area(const uint32_t* __restrict__ 
             for (uint32_t k = 0; k < m; k++) sum += sx[k] * sy[k + 1] - sx[k + 1] * sy[k];
         __syncthreads();
     }
-    if (threadIdx.x == 0) area[d] = sum * 0.5;
+    if (threadIdx.x == 0) {
+        area[d] = sum * 0.5;
+        if (with_mass) area_regulator_mass(am, d, true, sum * 0.5);
+    }
 }
 
 // ------------------------------------------------------------- coarsen -----
 __global__ void k_co_reset(DevStatus* st) {
+    if (topo_halted(st)) return;
     st->co_any = 0;
     st->co_removed = 0;
 }
@@ -575,6 +636,7 @@ __global__ void k_clear_u32(uint32_t* p, uint32_t n) {
 
 // ForceSolver::deleteAllForces (ForceSolver.cpp:158-166): empty CSR rows
 __global__ void k_forces_clear(int32_t* fs_off, DevStatus* st) {
+    if (step_stopped(st)) return;
     const uint32_t nm = st->nm;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= nm; k += gridDim.x * blockDim.x) fs_off[k] = 0;
 }
@@ -586,6 +648,7 @@ __global__ void k_forces_tension(const int32_t* __restrict__ ca, const int32_t* 
                                  const int32_t* __restrict__ min_c, const int32_t* __restrict__ mout_c, double f,
                                  int32_t* fs_off, int32_t* fs_type, int32_t* fs_partner, double* fs_prm,
                                  DevStatus* st) {
+    if (step_stopped(st)) return;
     const uint32_t nm = st->nm;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
         fs_off[k] = (int32_t)(2 * k);
@@ -631,29 +694,28 @@ int launch_rings(lbib_ctx* c) {
     return 0;
 }
 
-int launch_ring_areas(lbib_ctx* c) {
+int launch_ring_areas(lbib_ctx* c, const AreaMass* mass) {
     if (c->ndom == 0) return 0;
-    k_ring_area<<<c->ndom, 256, 0, c->stream>>>(c->ring_cnt, c->ring_off_u, c->ring_marker, c->mx, c->my, c->area);
+    const AreaMass none{};
+    k_ring_area<<<c->ndom, 256, 0, c->stream>>>(c->ring_cnt, c->ring_off_u, c->ring_marker, c->mx, c->my, c->area,
+                                                mass ? c->st : nullptr, mass ? *mass : none, mass ? 1 : 0);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
 
-// rounds of remeshing; the host repeats while LBIB_FLAG_REMESH comes back
+// Rounds of remeshing; LBIB_FLAG_REMESH asks for another pass when a
+// connection is still long (a marker moves far less than MAXLENGTH per
+// iteration, so one round is the steady-state case).
 int launch_remesh(lbib_ctx* c, int rounds) {
     TopoArrays t = topo_arrays(c);
-    k_rm_reset<<<1, 1, 0, c->stream>>>(c->st);
     for (int r = 0; r < rounds; r++) {
         k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, c->st);
         if (scan_exclusive(c, c->flag_c, c->rank_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->rm_L)) return -1;
-        k_rm_guard<<<1, 1, 0, c->stream>>>(c->st, (uint32_t)c->mcap, (uint32_t)c->ccap);
-        k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->nx, c->ny, c->st);
-        k_rm_commit<<<grid_for(2 * (long)c->ccap), 256, 0, c->stream>>>(t, c->st);
-        k_rm_counts<<<1, 1, 0, c->stream>>>(c->st, (uint32_t)c->mcap, (uint32_t)c->ccap);
-        if (launch_inout(c)) return -1;
+        k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->nx, c->ny,
+                                                             (uint32_t)c->mcap, (uint32_t)c->ccap, c->st);
+        k_rm_commit<<<grid_for(2 * (long)c->ccap), 256, 0, c->stream>>>(t, c->min_c, c->mout_c, c->st);
     }
-    // anything still long: the host runs more rounds
-    k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, c->st);
-    k_rm_pending<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->flag_c, c->st);
+    k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, nullptr, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

@@ -43,6 +43,7 @@ _SIG = {
     "lbh_get_connections": (C.c_int, [_P, _U32, _U32, _U32]),
     "lbh_perturb_counts": (C.c_long, [_P, C.c_int]),
     "lbh_times": (None, [_P, _D]),
+    "lbh_set_run_ahead": (C.c_int, [_P, C.c_uint]),
     "lbh_reset_times": (None, [_P]),
     "lbh_last_status": (C.c_int, [_P, C.POINTER(capi.Status)]),
     "lbh_num_areas": (C.c_int, [_P]),
@@ -137,6 +138,10 @@ class Simulation:
     def run(self, steps):
         self._chk(self.L.lbh_run(self.h, steps))
 
+    def set_run_ahead(self, n):
+        """Iterations enqueued between device status reads (1 = every iteration)."""
+        self._chk(self.L.lbh_set_run_ahead(self.h, n))
+
     def sync(self):
         self._chk(self.L.lbh_sync(self.h))
 
@@ -180,9 +185,10 @@ class Simulation:
         return self.L.lbh_perturb_counts(self.h, 0), self.L.lbh_perturb_counts(self.h, 1)
 
     def times(self):
-        out = np.zeros(5)
+        out = np.zeros(7)
         self.L.lbh_times(self.h, capi.dptr(out))
-        return dict(lattice=out[0], geometry=out[1], host=out[2], bio=out[3], steps=int(out[4]))
+        return dict(lattice=out[0], geometry=out[1], host=out[2], bio=out[3], wait=out[5], steps=int(out[4]),
+                    stops=int(out[6]))
 
     def reset_times(self):
         self.L.lbh_reset_times(self.h)

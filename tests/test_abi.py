@@ -36,7 +36,7 @@ def test_status_bits_match_header():
     text = open(os.path.join(ROOT, "include", "lbib.h")).read()
     for name in ["ERR_VELOCITY", "ERR_EDGE", "ERR_SCANLINE", "ERR_SLOPE", "ERR_CAPACITY", "ERR_TOPOLOGY",
                  "HALT_PERTURB_NODE", "HALT_PERTURB_CONN", "HALT_PERTURB_REMESH", "HALT_GROW",
-                 "FLAG_REMESH", "FLAG_COARSEN"]:
+                 "FLAG_REMESH", "FLAG_COARSEN", "SKIPPED"]:
         m = re.search(r"#define LBIB_%s\s+(0x[0-9a-fA-F]+)u" % name, text)
         assert m and int(m.group(1), 16) == getattr(capi, name), name
 

@@ -118,3 +118,28 @@ def test_noflux_boundary_two_species_match_oracle(orc):
     o.run(60)
     _compare(sim, o, 2, "no-flux BC")
     sim.close()
+
+
+@pytest.mark.parametrize("window", [1, 5, 16])
+def test_run_ahead_resolves_stops_like_the_synchronous_loop(orc, window):
+    """Iterations enqueued ahead of the status read: a stop (here a remesh that
+    needs more rounds than one geometry pass runs, then perturbations and
+    coarsening as they come) skips everything enqueued after it and is
+    resolved from the iteration that raised it, so every window gives the
+    oracle's result."""
+    nx = ny = 64
+    # 12 markers on r = 8: 4 lu segments need four remesh rounds in iteration 0
+    nodes, conns = workloads.circle_cell(32.0, 32.0, 8.0, 12, np.random.default_rng(4))
+    print(f"case run-ahead window={window}", flush=True)
+    sim, o = _build(orc, nx, ny, 1.0, nodes, conns,
+                    [("tutorial_01_CDESolverD2Q5_SIGNAL", capi.SPECIES_SIGNAL, 0.8, SIGNAL)],
+                    [(1, [200.0, 1e-5, 0.004, 1.2], 1), (0, [0.01], 10)], celltypes={1: 2})
+    sim.set_run_ahead(window)
+    sim.run(45)
+    o.run(45)
+    assert sim.iteration == o.iteration() == 45
+    _compare(sim, o, 1, f"window={window}")
+    t = sim.times()
+    print("resolved stops", t["stops"], "perturbations", sim.perturb_counts(), flush=True)
+    assert t["stops"] >= 1
+    sim.close()

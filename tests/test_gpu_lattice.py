@@ -111,3 +111,22 @@ def test_init_with_velocity_matches_oracle(orc):
     np.testing.assert_array_equal(ctx.get_populations(), o.get_f())
     np.testing.assert_array_equal(ctx.get_species(0), o.get_g(0))
     ctx.close()
+
+
+def test_profile_hooks_count_launches_and_time_a_copy():
+    ctx = capi.Context(64, 48, 1.0, species=[capi.species_spec(capi.SPECIES_PLAIN, 0.8)])
+    try:
+        ctx.init_solvers()
+        ctx.profile_enable(True)
+        ctx.profile_read()
+        for it in range(3):
+            ctx.step_lattice_only(it)
+        ms, n, b = ctx.profile_read()
+        assert n == 3 and ms > 0.0
+        assert b == (2 * 9 + 2 * 5) * 8 * 64 * 48
+        ctx.profile_enable(False)
+        assert ctx.profile_copy(1 << 20, 2) > 0.0
+        with pytest.raises(capi.LbibError, match="nbytes"):
+            ctx.profile_copy(16, 1)
+    finally:
+        ctx.close()
