@@ -96,7 +96,7 @@ typedef struct lbib_config {
 typedef struct lbib_status {
     uint32_t bits;
     int32_t first_bad_marker;   /* dense marker index of the first error, or -1 */
-    int32_t n_touched_sites;    /* sites that received IB force this step */
+    int32_t n_touched_sites;    /* 16x16 site tiles that received IB force this step */
     int32_t n_boundary_nodes;   /* cut links kept after closest-wins */
     int32_t n_reinit;           /* sites whose domain changed (CDE reinit) */
     int32_t n_candidates;       /* cut-link candidates generated */

@@ -35,6 +35,13 @@ __device__ __forceinline__ double fast_cos(const double x) {
 __device__ __forceinline__ double ib_delta(const double dx, const double dy) {
     return ((1.0 + fast_cos(kHalfPi * dx)) * (1.0 + fast_cos(kHalfPi * dy))) / 16.0;
 }
+// The two factors of ib_delta for the four stencil offsets along one axis:
+// f[q] = 1 + fastCos(pi/2 * ((s0 + q) - X)), so that
+// ib_delta((s0x + qx) - X, (s0y + qy) - Y) == (fx[qx] * fy[qy]) / 16 bit for bit.
+__device__ __forceinline__ void ib_factors(const int s0, const double X, double f[4]) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) f[q] = 1.0 + fast_cos(kHalfPi * ((double)(s0 + q) - X));
+}
 
 // FluidSolver::collide (FluidSolver.cpp:103-183) on natural-order pulled
 // populations d[9]; Luo1993ForceModel (Luo1993ForceModel.cpp:29-70) or

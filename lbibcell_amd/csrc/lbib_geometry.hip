@@ -176,7 +176,7 @@ __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_
 // occupied slot reports the site.  Also resets the selection buffers.
 __global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_t* __restrict__ tag,
                             unsigned long long* best_dist, uint32_t* best_seq, uint32_t* changed,
-                            uint32_t* rmark, int local_update, DevStatus* st) {
+                            uint32_t* rmark, uint8_t* rstate, int local_update, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->ncand;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
@@ -195,6 +195,7 @@ __global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_
                 changed[2 * p] = (uint32_t)e.site;
                 changed[2 * p + 1] = e.domain;
                 rmark[e.site] = p + 1;
+                rstate[p] = 0;
             } else {
                 atomicOr(&st->bits, LBIB_ERR_CAPACITY);
             }
@@ -276,6 +277,52 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
 // result is identical.  A changed neighbour is found through rmark[site]
 // (1 + its position in the list, checked against the list, so entries left by
 // earlier steps never match).
+//
+// The first REINIT_GRID_LEVELS levels run as one grid-wide launch each (most
+// sites are ready at once); state 2 + L marks a site done by level launch L,
+// so a launch only trusts neighbours finished by earlier launches.  The rest
+// runs in one workgroup, level after level with barriers (state 1 = ready,
+// >= 2 = done).
+constexpr int REINIT_GRID_LEVELS = 2;
+
+__device__ __forceinline__ int reinit_find(const ReinitArgs& a, uint32_t cnt, uint32_t site) {
+    const uint32_t m = a.rmark[site];
+    return (m != 0u && m <= cnt && a.changed[2 * (m - 1)] == site) ? (int)(m - 1) : -1;
+}
+
+// are all changed neighbours preceding site i finished (state in [lo, hi))?
+__device__ __forceinline__ bool reinit_ready(const ReinitArgs& a, uint32_t cnt, uint32_t i, const uint8_t* state,
+                                             int lo, int hi) {
+    const int EXd[8] = {1, 0, -1, 0, 1, -1, -1, 1};
+    const int EYd[8] = {0, 1, 0, -1, 1, 1, -1, -1};
+    const int x = (int)(i % a.nx), y = (int)(i / a.nx);
+    for (int d = 0; d < 8; d++) {
+        const int xx = (x + EXd[d] + a.nx) % a.nx, yy = (y + EYd[d] + a.ny) % a.ny;
+        const uint32_t j = (uint32_t)yy * a.nx + xx;
+        if (j < i) {
+            const int u = reinit_find(a, cnt, j);
+            if (u >= 0 && (state[u] < lo || state[u] >= hi)) return false;
+        }
+    }
+    return true;
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_reinit_level(const ReinitArgs a, uint8_t* state, int level, DevStatus* st) {
+    if (halted(st)) return;
+    const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
+    bool any = false;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+        if (state[t] != 0) continue;
+        const uint32_t i = a.changed[2 * t];
+        if (!reinit_ready(a, cnt, i, state, 2, 2 + level)) continue;
+        reinit_site<NS>(a, (long)i, a.changed[2 * t + 1]);
+        state[t] = (uint8_t)(2 + level);
+        any = true;
+    }
+    if (__any(any) && (threadIdx.x & 63) == 0) atomicMax(&st->reinit_levels, (uint32_t)level + 1);
+}
+
 template <int NS>
 __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gstate, DevStatus* st) {
     if (halted(st)) return;
@@ -283,36 +330,27 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gs
     __shared__ int pending;
     const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
     if (cnt == 0) return;
-    uint8_t* state = (cnt <= (uint32_t)REINIT_LDS) ? lstate : gstate;
-    for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) state[t] = 0;
+    const bool in_lds = cnt <= (uint32_t)REINIT_LDS;
+    uint8_t* state = in_lds ? lstate : gstate;
+    if (in_lds)
+        for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) lstate[t] = gstate[t];
     __syncthreads();
-    const int EXd[8] = {1, 0, -1, 0, 1, -1, -1, 1};
-    const int EYd[8] = {0, 1, 0, -1, 1, 1, -1, -1};
-    auto find = [&](uint32_t site) -> int {
-        const uint32_t m = a.rmark[site];
-        return (m != 0u && m <= cnt && a.changed[2 * (m - 1)] == site) ? (int)(m - 1) : -1;
-    };
     for (int level = 0; level < (1 << 20); level++) {
         if (threadIdx.x == 0) pending = 0;
         __syncthreads();
         // phase 1: mark the ready sites (state 1)
+        bool left = false;
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
             if (state[t] != 0) continue;
-            const uint32_t i = a.changed[2 * t];
-            const int x = (int)(i % a.nx), y = (int)(i / a.nx);
-            bool ready = true;
-            for (int d = 0; d < 8 && ready; d++) {
-                const int xx = (x + EXd[d] + a.nx) % a.nx, yy = (y + EYd[d] + a.ny) % a.ny;
-                const uint32_t j = (uint32_t)yy * a.nx + xx;
-                if (j < i) {
-                    const int u = find(j);
-                    if (u >= 0 && state[u] != 2) ready = false;
-                }
-            }
-            if (ready) state[t] = 1;
-            else pending = 1;
+            left = true;
+            if (reinit_ready(a, cnt, a.changed[2 * t], state, 2, 256)) state[t] = 1;
         }
+        if (left) pending = 1;
         __syncthreads();
+        if (!pending) {
+            if (threadIdx.x == 0 && level > 0) st->reinit_levels = REINIT_GRID_LEVELS + (uint32_t)level;
+            break;
+        }
         // phase 2: run them
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
             if (state[t] != 1) continue;
@@ -321,12 +359,7 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gs
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
             if (state[t] == 1) state[t] = 2;
-        const int more = pending;
         __syncthreads();
-        if (!more) {
-            if (threadIdx.x == 0) st->reinit_levels = (uint32_t)level + 1;
-            break;
-        }
     }
 }
 
@@ -380,6 +413,8 @@ __global__ __launch_bounds__(256) void k_scanline(int nx, long n, uint32_t* tag,
 
 template <int NS>
 static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
+    for (int level = 0; level < REINIT_GRID_LEVELS; level++)
+        k_reinit_level<NS><<<64, 256, 0, c->stream>>>(a, c->rstate, level, c->st);
     k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rstate, c->st);
 }
 
@@ -402,7 +437,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
     k_cut_win<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my, c->mvx,
                                         c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->st);
-    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, c->rmark, cure ? 0 : 1,
+    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, c->rmark, c->rstate, cure ? 0 : 1,
                                           c->st);
     if (!cure) {
         ReinitArgs a;
@@ -429,7 +464,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
 int launch_topology(lbib_ctx* c, bool coarsen) {
     if (c->nc == 0) return 0;
     if (launch_remesh(c, 1)) return -1;
-    if (launch_rings(c)) return -1;
+    if (launch_rings(c, true)) return -1;
     if (coarsen) {
         if (launch_coarsen(c)) return -1;
         if (launch_rings(c)) return -1;

@@ -75,13 +75,15 @@ __global__ void k_marker_force(const double* __restrict__ mx, const double* __re
 // ---- ordered spread over 16x16 site tiles --------------------------------
 // Markers are binned by the tile of floor(X), floor(Y).  A tile's sites can be
 // reached only by markers binned in the 3x3 tiles around it (the stencil
-// reaches 2 sites).  One workgroup per tile gathers those markers, sorts them
-// by index (= GeometryNode id order) and every lane adds delta*F for its site
-// in that order, starting from 0.0 like FluidSolver::resetForce; tiles with no
-// marker nearby are flagged inactive and the lattice kernel reads no force.
+// reaches 2 sites).  One workgroup per tile gathers the markers whose stencil
+// overlaps it, ranks them by index (= GeometryNode id order; indices are
+// distinct, so a marker's rank is the number of smaller ones), evaluates each
+// marker's eight delta factors once, and every lane adds delta*F for its site
+// in rank order, starting from 0.0 like FluidSolver::resetForce.  Tiles with
+// no marker nearby are flagged inactive and the lattice kernel reads no force.
 constexpr int ST = 16;            // spread tile edge
 constexpr int SP_IDX = 1024;      // markers gathered per tile
-constexpr int SP_CH = 256;        // marker data staged per pass
+constexpr int SP_CH = 128;        // marker data staged per pass
 
 __global__ void k_tile_clear(int ntiles, uint32_t* tcnt, uint8_t* tact, DevStatus* st) {
     if (step_stopped(st)) return;
@@ -125,64 +127,71 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
     const int t = blockIdx.x;
     if (!tact[t] || step_stopped(st)) return;
     __shared__ int32_t idx[SP_IDX];
-    __shared__ double sx[SP_CH], sy[SP_CH], sfx[SP_CH], sfy[SP_CH];
+    __shared__ int32_t sorted[SP_IDX];
+    __shared__ double fxa[SP_CH][4], fya[SP_CH][4], sfx[SP_CH], sfy[SP_CH];
+    __shared__ int32_t s0x[SP_CH], s0y[SP_CH];
+    __shared__ uint32_t hcnt[9], hoff[9], hpre[10];
     __shared__ uint32_t n;
-    if (threadIdx.x == 0) atomicAdd(&st->ntouched, 1u);
     const int tx = t % ntx, ty = t / ntx;
+    // the 3x3 home tiles' lists, loaded at once
+    if (threadIdx.x < 9) {
+        const int hx = tx + (int)(threadIdx.x % 3) - 1, hy = ty + (int)(threadIdx.x / 3) - 1;
+        const bool in = hx >= 0 && hy >= 0 && hx < ntx && hy < nty;
+        const int h = hy * ntx + hx;
+        hcnt[threadIdx.x] = in ? tcnt[h] : 0u;
+        hoff[threadIdx.x] = in ? toff[h] : 0u;
+    }
     if (threadIdx.x == 0) n = 0;
     __syncthreads();
-    for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-            const int hx = tx + dx, hy = ty + dy;
-            if (hx < 0 || hy < 0 || hx >= ntx || hy >= nty) continue;
-            const int h = hy * ntx + hx;
-            const uint32_t c = tcnt[h], o = toff[h];
-            for (uint32_t j = threadIdx.x; j < c; j += blockDim.x) {
-                const int k = tlist[o + j];
-                // keep only markers whose 4x4 stencil overlaps this tile
-                const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
-                if (fx + 2 < tx * ST || fx - 1 > tx * ST + ST - 1 || fy + 2 < ty * ST || fy - 1 > ty * ST + ST - 1)
-                    continue;
-                const uint32_t p = atomicAdd(&n, 1u);
-                if (p < (uint32_t)SP_IDX) idx[p] = k;
-            }
-        }
+    if (threadIdx.x == 0) {
+        uint32_t a = 0;
+        for (int h = 0; h < 9; h++) { hpre[h] = a; a += hcnt[h]; }
+        hpre[9] = a;
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < hpre[9]; j += blockDim.x) {
+        int h = 0;
+        while (j >= hpre[h + 1]) h++;
+        const int k = tlist[hoff[h] + (j - hpre[h])];
+        // keep only markers whose 4x4 stencil overlaps this tile
+        const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
+        if (fx + 2 < tx * ST || fx - 1 > tx * ST + ST - 1 || fy + 2 < ty * ST || fy - 1 > ty * ST + ST - 1) continue;
+        const uint32_t p = atomicAdd(&n, 1u);
+        if (p < (uint32_t)SP_IDX) idx[p] = k;
+    }
     __syncthreads();
     const uint32_t m = n;
     if (m > (uint32_t)SP_IDX) {
         if (threadIdx.x == 0) atomicOr(&st->bits, LBIB_ERR_CAPACITY);
         return;
     }
-    uint32_t P = 1;
-    while (P < m) P <<= 1;
-    for (uint32_t j = m + threadIdx.x; j < P; j += blockDim.x) idx[j] = INT_MAX;
+    for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) {
+        const int32_t v = idx[j];
+        uint32_t r = 0;
+        for (uint32_t i = 0; i < m; i++) r += (idx[i] < v) ? 1u : 0u;
+        sorted[r] = v;
+    }
     __syncthreads();
-    for (uint32_t kk = 2; kk <= P; kk <<= 1)
-        for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
-            for (uint32_t q = threadIdx.x; q < P; q += blockDim.x) {
-                const uint32_t l = q ^ j;
-                if (l > q) {
-                    const bool up = (q & kk) == 0;
-                    const int32_t u = idx[q], v = idx[l];
-                    if ((u > v) == up) { idx[q] = v; idx[l] = u; }
-                }
-            }
-            __syncthreads();
-        }
     const int px = tx * ST + (int)(threadIdx.x % ST), py = ty * ST + (int)(threadIdx.x / ST);
     double ax = 0.0, ay = 0.0;
     for (uint32_t c0 = 0; c0 < m; c0 += SP_CH) {
         const uint32_t mc = min((uint32_t)SP_CH, m - c0);
         for (uint32_t j = threadIdx.x; j < mc; j += blockDim.x) {
-            const int k = idx[c0 + j];
-            sx[j] = mx[k]; sy[j] = my[k]; sfx[j] = mfx[k]; sfy[j] = mfy[k];
+            const int k = sorted[c0 + j];
+            const double X = mx[k], Y = my[k];
+            const int x0 = (int)floor(X) - 1, y0 = (int)floor(Y) - 1;
+            s0x[j] = x0;
+            s0y[j] = y0;
+            ib_factors(x0, X, fxa[j]);
+            ib_factors(y0, Y, fya[j]);
+            sfx[j] = mfx[k];
+            sfy[j] = mfy[k];
         }
         __syncthreads();
         for (uint32_t j = 0; j < mc; j++) {
-            const double X = sx[j], Y = sy[j];
-            const int fx = (int)floor(X), fy = (int)floor(Y);
-            if (px < fx - 1 || px > fx + 2 || py < fy - 1 || py > fy + 2) continue;
-            const double dd = ib_delta((double)px - X, (double)py - Y);
+            const int qx = px - s0x[j], qy = py - s0y[j];
+            if ((unsigned)qx > 3u || (unsigned)qy > 3u) continue;
+            const double dd = (fxa[j][qx] * fya[j][qy]) / 16.0;
             ax += dd * sfx[j];
             ay += dd * sfy[j];
         }
@@ -210,13 +219,16 @@ __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mv
     const double X = mx[k], Y = my[k];
     const int fx = (int)floor(X), fy = (int)floor(Y);
     const int base = 4 * (fx - 1) + (fy - 1);
+    double fxa[4], fya[4];
+    ib_factors(fx - 1, X, fxa);
+    ib_factors(fy - 1, Y, fya);
     double vx = 0.0, vy = 0.0;
 #pragma unroll
     for (int s = 0; s < 16; s++) {
         const int t = (((s - base) % 16) + 16) % 16;
         const int sx = fx - 1 + (t >> 2), sy = fy - 1 + (t & 3);
         const long site = (long)sy * nx + sx;
-        const double dd = ib_delta((double)sx - X, (double)sy - Y);
+        const double dd = (fxa[t >> 2] * fya[t & 3]) / 16.0;
         vx += dd * ux[site];
         vy += dd * uy[site];
     }
@@ -268,7 +280,7 @@ int launch_force_and_spread(lbib_ctx* c) {
                                                                c->fs_prm, c->mfx, c->mfy, c->st);
     k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
                                                              c->tact, c->st);
-    if (scan_exclusive(c, c->tcnt, c->toff, nullptr, (uint32_t)ntiles, (uint32_t)ntiles, nullptr)) return -1;
+    if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;
     k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist,
                                                                c->st);
     k_tile_spread<<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt, c->tlist,

@@ -192,8 +192,11 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode);
 int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor);
 int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
                    uint32_t* total);
+// exclusive scan of the per-tile marker counts (tcnt -> toff); for short tile
+// arrays also the number of nonzero active flags into *nactive
+int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive);
 int launch_inout(lbib_ctx* c);
-int launch_rings(lbib_ctx* c);
+int launch_rings(lbib_ctx* c, bool counters_cleared = false);
 // per-domain inputs and outputs of the area regulator's mass source
 struct AreaMass {
     const uint32_t* celltype;
@@ -204,7 +207,7 @@ struct AreaMass {
 // mass != nullptr: also the area regulator's mass table, skipped while the
 // status is stopped (an iteration's BioSolver stage)
 int launch_ring_areas(lbib_ctx* c, const AreaMass* mass);
-int launch_remesh(lbib_ctx* c, int rounds);
+int launch_remesh(lbib_ctx* c, int rounds);   // also clears the ring counters
 int launch_coarsen(lbib_ctx* c);
 int launch_forces_clear(lbib_ctx* c);
 int launch_forces_tension(lbib_ctx* c, double f);

@@ -106,10 +106,20 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
 
 // Short arrays (tiles, domains): one workgroup, 4096 values per pass.
 constexpr uint32_t SCAN_SINGLE_MAX = 1u << 16;
+// nz != nullptr: also count the nonzero bytes of nz[0..n) into *nz_total.
 __global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict__ in, const uint32_t* nptr,
-                                                      uint32_t nfix, uint32_t* __restrict__ out, uint32_t* total) {
+                                                      uint32_t nfix, uint32_t* __restrict__ out, uint32_t* total,
+                                                      const uint8_t* __restrict__ nz, uint32_t* nz_total) {
     const uint32_t n = nptr ? *nptr : nfix;
     __shared__ uint32_t s[1024];
+    __shared__ uint32_t nzc;
+    if (threadIdx.x == 0) nzc = 0;
+    if (nz) {
+        __syncthreads();
+        uint32_t c = 0;
+        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c += nz[i] ? 1u : 0u;
+        atomicAdd(&nzc, c);
+    }
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < n; b0 += 4096) {
         uint32_t v[4];
@@ -136,7 +146,17 @@ __global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict
         carry += s[1023];
         __syncthreads();
     }
+    __syncthreads();
     if (threadIdx.x == 0 && total) *total = carry;
+    if (threadIdx.x == 0 && nz) *nz_total = nzc;
+}
+
+int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive) {
+    const uint32_t nt = (uint32_t)c->stx * (uint32_t)c->sty;
+    if (nt > SCAN_SINGLE_MAX) return scan_exclusive(c, c->tcnt, c->toff, nullptr, nt, nt, nullptr);
+    k_scan_single<<<1, 1024, 0, c->stream>>>(c->tcnt, nullptr, nt, c->toff, nullptr, active, nactive);
+    LBIB_HIP(hipGetLastError());
+    return 0;
 }
 
 int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
@@ -144,7 +164,7 @@ int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_
     const uint32_t nb = (cap + SCAN_B - 1) / SCAN_B;
     if (nb == 0) return 0;
     if (cap <= SCAN_SINGLE_MAX) {
-        k_scan_single<<<1, 1024, 0, c->stream>>>(in, nptr, nfix, out, total);
+        k_scan_single<<<1, 1024, 0, c->stream>>>(in, nptr, nfix, out, total, nullptr, nullptr);
         LBIB_HIP(hipGetLastError());
         return 0;
     }
@@ -181,9 +201,17 @@ __device__ __forceinline__ bool needs_perturbation(const double X, const double 
 
 // flag != nullptr: mark this round's long connections (and the perturbation
 // halt of their midpoints); flag == nullptr: only raise LBIB_FLAG_REMESH if a
-// connection is still long after the rounds of this pass.
+// connection is still long after the rounds of this pass.  ring_start != nullptr:
+// also clear the per-domain ring counters for the ring rebuild that follows
+// (always, like k_ring_reset).
 __global__ void k_rm_flag(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb, const double* __restrict__ mx,
-                          const double* __restrict__ my, uint32_t* flag, DevStatus* st) {
+                          const double* __restrict__ my, uint32_t* flag, int ndom, int32_t* ring_start,
+                          uint32_t* ring_cnt, DevStatus* st) {
+    if (ring_start)
+        for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < ndom; d += gridDim.x * blockDim.x) {
+            ring_start[d] = INT_MAX;
+            ring_cnt[d] = 0;
+        }
     if (topo_halted(st)) return;
     const uint32_t nc = st->nc;
     bool any = false;
@@ -680,9 +708,10 @@ int launch_inout(lbib_ctx* c) {
     return 0;
 }
 
-int launch_rings(lbib_ctx* c) {
+int launch_rings(lbib_ctx* c, bool counters_cleared) {
     if (c->ndom == 0) return 0;
-    k_ring_reset<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->ring_start, c->ring_cnt);
+    if (!counters_cleared)
+        k_ring_reset<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->ring_start, c->ring_cnt);
     k_ring_count<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->cdom, c->ring_start, c->ring_cnt, c->ring_slot, c->ndom,
                                                           c->st);
     if (scan_exclusive(c, c->ring_cnt, c->ring_off_u, nullptr, (uint32_t)c->ndom, (uint32_t)c->ndom, nullptr)) return -1;
@@ -709,13 +738,16 @@ int launch_ring_areas(lbib_ctx* c, const AreaMass* mass) {
 int launch_remesh(lbib_ctx* c, int rounds) {
     TopoArrays t = topo_arrays(c);
     for (int r = 0; r < rounds; r++) {
-        k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, c->st);
+        k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, 0, nullptr,
+                                                            nullptr, c->st);
         if (scan_exclusive(c, c->flag_c, c->rank_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->rm_L)) return -1;
         k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->nx, c->ny,
                                                              (uint32_t)c->mcap, (uint32_t)c->ccap, c->st);
         k_rm_commit<<<grid_for(2 * (long)c->ccap), 256, 0, c->stream>>>(t, c->min_c, c->mout_c, c->st);
     }
-    k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, nullptr, c->st);
+    // with the clearing of the ring counters for the rebuild that follows
+    k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, nullptr, c->ndom, c->ring_start,
+                                                        c->ring_cnt, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
