@@ -11,13 +11,7 @@
 using namespace lbib;
 
 namespace lbib {
-// Opens an iteration: records it while the status is clear, otherwise marks
-// the rest of the enqueued work as skipped (a single thread; one stream).
-__global__ void k_step_begin(DevStatus* st, uint32_t it) {
-    const uint32_t b = st->bits;
-    if (b & STOP_BITS_STEP) st->bits = b | LBIB_SKIPPED;
-    else st->step_it = it;
-}
+__global__ void k_step_begin(DevStatus* st, uint32_t it) { step_begin_dev(st, it); }
 }  // namespace lbib
 
 namespace {
@@ -132,7 +126,17 @@ int ensure_domains(lbib_ctx* c, int ndom) {
     return ensure_scan(c);
 }
 
-int check_ctx(lbib_ctx* c) { return c ? 0 : -1; }
+// a pending lbib_step_begin that no lattice stage consumed gets a kernel of
+// its own before anything else is enqueued
+int flush_step_begin(lbib_ctx* c) {
+    if (!c->begin_pending) return 0;
+    c->begin_pending = false;
+    k_step_begin<<<1, 1, 0, c->stream>>>(c->st, c->begin_it);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+int check_ctx(lbib_ctx* c) { return c ? flush_step_begin(c) : -1; }
+int check_ctx_keep_begin(lbib_ctx* c) { return c ? 0 : -1; }
 int upload(lbib_ctx* c, void* dst, const void* src, size_t bytes);
 int download(lbib_ctx* c, void* dst, const void* src, size_t bytes);
 
@@ -589,7 +593,7 @@ static int stream_collide(lbib_ctx* c, uint32_t it, bool ftile) {
 }
 
 int lbib_step_lattice(lbib_ctx* c, uint32_t it) {
-    if (check_ctx(c)) return -1;
+    if (check_ctx_keep_begin(c)) return -1;   // the first lattice kernel opens the iteration
     if (launch_force_and_spread(c)) return -1;
     if (stream_collide(c, it, c->nm > 0)) return -1;
     if (launch_clear_force(c)) return -1;
@@ -716,15 +720,17 @@ int lbib_read_status(lbib_ctx* c, lbib_status* out) {
 }
 int lbib_step_begin(lbib_ctx* c, uint32_t it) {
     if (check_ctx(c)) return -1;
-    lbib::k_step_begin<<<1, 1, 0, c->stream>>>(c->st, it);
-    LBIB_HIP(hipGetLastError());
+    c->begin_pending = true;
+    c->begin_it = it;
     if (!c->snaps.empty() && c->snaps.back().it >= it) c->snaps.clear();   // a new run
     c->snaps.push_back({it, c->cur, c->mtab_live, c->msite_live, c->ev_used});
     if (c->snaps.size() > 1024) c->snaps.pop_front();
     return 0;
 }
 int lbib_rewind(lbib_ctx* c, uint32_t it) {
-    if (check_ctx(c)) return -1;
+    if (check_ctx_keep_begin(c)) return -1;
+    if (c->begin_pending && c->begin_it >= it) c->begin_pending = false;   // never reached the device
+    if (flush_step_begin(c)) return -1;
     if (c->snaps.empty() || it > c->snaps.back().it) return 0;   // iteration it never began
     while (!c->snaps.empty() && c->snaps.back().it > it) c->snaps.pop_back();
     if (c->snaps.empty() || c->snaps.back().it != it) {

@@ -196,6 +196,48 @@ __device__ __forceinline__ double c_swapped(const double d0, const double dE, co
 
 // Wave-aggregated counter allocation: every lane of the wavefront must call
 // it (v = 0 for idle lanes); returns the lane's base in *ctr.
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Renumbering them so that every XCD works on a contiguous range keeps
+// neighbouring data (halo rows, stencil sites of adjacent markers) in one L2.
+// A bijection of [0, total) for any total.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t lid, uint32_t total) {
+    constexpr uint32_t NXCD = 8;
+    const uint32_t q = total / NXCD, r = total % NXCD;
+    const uint32_t x = lid % NXCD, local = lid / NXCD;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + local;
+}
+
+// Inclusive prefix sum across a 64-lane wavefront.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive prefix sum across the workgroup (blockDim.x a multiple of 64, at
+// most 1024); wsum: 16 words of LDS.  Returns the workgroup total.  Every
+// thread must call it.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* excl, uint32_t* wsum) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        uint32_t t = (lane < nw) ? wsum[lane] : 0u;
+        t = wave_incl_scan(t);
+        if (lane < nw) wsum[lane] = t;
+    }
+    __syncthreads();
+    *excl = (w ? wsum[w - 1] : 0u) + inc - v;
+    const uint32_t total = wsum[nw - 1];
+    __syncthreads();
+    return total;
+}
+
 __device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr, uint32_t v) {
     const int lane = threadIdx.x & 63;
     uint32_t incl = v;

@@ -179,26 +179,32 @@ __global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_
                             uint32_t* rmark, uint8_t* rstate, int local_update, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->ncand;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
-        const Cand e = cand[j];
-        const long slotix = (long)(e.slot - 1) * n + e.site;
-        best_dist[slotix] = ~0ull;
-        best_seq[slotix] = ~0u;
-        if (!e.win || !local_update) continue;
-        const uint32_t tg = tag[e.site];
-        const uint32_t mask = (tg >> BN_SHIFT) & 0xfu;
-        const int hi = 31 - __clz((int)mask);   // highest occupied slot - 1
-        if (hi + 1 != e.slot) continue;
-        if ((tg & DOM_MASK) != e.domain) {
-            const uint32_t p = atomicAdd(&st->nchanged, 1u);
-            if (p < (uint32_t)REINIT_CAP) {
-                changed[2 * p] = (uint32_t)e.site;
-                changed[2 * p + 1] = e.domain;
-                rmark[e.site] = p + 1;
-                rstate[p] = 0;
-            } else {
-                atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
+        const uint32_t j = j0 + threadIdx.x;
+        bool add = false;
+        Cand e;
+        if (j < nc) {
+            e = cand[j];
+            const long slotix = (long)(e.slot - 1) * n + e.site;
+            best_dist[slotix] = ~0ull;
+            best_seq[slotix] = ~0u;
+            if (e.win && local_update) {
+                const uint32_t tg = tag[e.site];
+                const uint32_t mask = (tg >> BN_SHIFT) & 0xfu;
+                const int hi = 31 - __clz((int)mask);   // highest occupied slot - 1
+                add = (hi + 1 == e.slot) && (tg & DOM_MASK) != e.domain;
             }
+        }
+        const uint32_t p = wave_alloc(&st->nchanged, add ? 1u : 0u);
+        if (!add) continue;
+        if (p < (uint32_t)REINIT_CAP) {
+            changed[2 * p] = (uint32_t)e.site;
+            changed[2 * p + 1] = e.domain;
+            rmark[e.site] = p + 1;
+            rstate[p] = 0;
+        } else {
+            atomicOr(&st->bits, LBIB_ERR_CAPACITY);
         }
     }
 }
@@ -366,47 +372,76 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gs
 // GeometryHandler::updateAllDomainIdentifiers (GeometryHandler.cpp:68-85): per
 // row, the domain of the last west boundary node at or left of x (x >= 1), or
 // the domain of site (0, y), which must be 0.
+// Each thread owns 8 consecutive sites of a 2048-site chunk; the last west
+// boundary node before a thread's sites comes from a wavefront max-scan of
+// positions and the wavefronts before it.
 __global__ __launch_bounds__(256) void k_scanline(int nx, long n, uint32_t* tag, const uint32_t* __restrict__ bn_dom,
                                                   DevStatus* st) {
     if (halted(st)) return;
-    __shared__ int spos[256];
-    __shared__ uint32_t sdom[256];
-    __shared__ uint32_t carry;
+    constexpr int PER = 8;
+    __shared__ int wpos[4];
+    __shared__ uint32_t wdom[4];
     const int y = blockIdx.x;
     const long r0 = (long)y * nx;
-    if (threadIdx.x == 0) {
-        carry = tag[r0] & DOM_MASK;
-        if (carry != 0u) atomicOr(&st->bits, LBIB_ERR_SCANLINE);
-    }
-    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t carry = tag[r0] & DOM_MASK;
+    if (threadIdx.x == 0 && carry != 0u) atomicOr(&st->bits, LBIB_ERR_SCANLINE);
     const int wbit = BN_SHIFT + DW - 1;
-    for (int x0 = 0; x0 < nx; x0 += 256) {
-        const int x = x0 + threadIdx.x;
-        uint32_t tg = 0;
-        int pos = -1;
-        if (x < nx) {
-            tg = tag[r0 + x];
-            if (x >= 1 && ((tg >> wbit) & 1u)) {
-                pos = threadIdx.x;
-                sdom[threadIdx.x] = bn_dom[(long)(DW - 1) * n + r0 + x];
+    const uint32_t* wdom_row = bn_dom + (long)(DW - 1) * n + r0;
+    for (int x0 = 0; x0 < nx; x0 += 256 * PER) {
+        const int xb = x0 + (int)threadIdx.x * PER;
+        uint32_t tg[PER], dm[PER];
+        int lpos = -1;
+        uint32_t ldom = 0;
+        if (xb + PER <= nx && ((r0 + xb) & 3) == 0) {
+            const uint4* p4 = reinterpret_cast<const uint4*>(tag + r0 + xb);
+            const uint4 u0 = p4[0], u1 = p4[1];
+            tg[0] = u0.x; tg[1] = u0.y; tg[2] = u0.z; tg[3] = u0.w;
+            tg[4] = u1.x; tg[5] = u1.y; tg[6] = u1.z; tg[7] = u1.w;
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; q++) tg[q] = (xb + q < nx) ? tag[r0 + xb + q] : 0u;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int x = xb + q;
+            dm[q] = 0;
+            if (x >= 1 && x < nx && ((tg[q] >> wbit) & 1u)) {
+                dm[q] = wdom_row[x];
+                lpos = x;
+                ldom = dm[q];
             }
         }
-        spos[threadIdx.x] = pos;
-        __syncthreads();
-        for (int off = 1; off < 256; off <<= 1) {
-            const int v = (threadIdx.x >= off) ? spos[threadIdx.x - off] : -1;
-            __syncthreads();
-            if (v > spos[threadIdx.x]) spos[threadIdx.x] = v;
-            __syncthreads();
+        // inclusive max-scan of (position, domain) over the wavefront
+        int p = lpos;
+        uint32_t d = ldom;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int p2 = __shfl_up(p, o, 64);
+            const uint32_t d2 = __shfl_up(d, o, 64);
+            if (lane >= o && p2 > p) { p = p2; d = d2; }
         }
-        const int last = spos[threadIdx.x];
-        const uint32_t val = (last >= 0) ? sdom[last] : carry;
-        if (x < nx && x >= 1) tag[r0 + x] = (tg & ~DOM_MASK) | val;
+        int pe = __shfl_up(p, 1, 64);
+        uint32_t de = __shfl_up(d, 1, 64);
+        if (lane == 0) pe = -1;
+        if (lane == 63) { wpos[w] = p; wdom[w] = d; }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const int lastall = spos[255];
-            if (lastall >= 0) carry = sdom[lastall];
+        int bp = -1;
+        uint32_t cur = carry;
+        for (int k = 0; k < w; k++)
+            if (wpos[k] > bp) { bp = wpos[k]; cur = wdom[k]; }
+        if (pe > bp) cur = de;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int x = xb + q;
+            if (x >= 1 && x < nx) {
+                if ((tg[q] >> wbit) & 1u) cur = dm[q];
+                const uint32_t nt = (tg[q] & ~DOM_MASK) | cur;
+                if (nt != tg[q]) tag[r0 + x] = nt;
+            }
         }
+        for (int k = 0; k < 4; k++)
+            if (wpos[k] >= 0) carry = wdom[k];
         __syncthreads();
     }
 }

@@ -23,11 +23,20 @@ namespace lbib {
 
 // ForceSolver::calculateForce (ForceSolver.cpp:42-85) with the force structs of
 // ForceStructs.hpp:49-588; per marker, structs in map/push order.
+// Also the first kernel of the lattice stage: opens a pending iteration
+// (lbib_step_begin) and clears the spread tiles.
 __global__ void k_marker_force(const double* __restrict__ mx, const double* __restrict__ my,
                                const int32_t* __restrict__ off, const int32_t* __restrict__ type,
                                const int32_t* __restrict__ partner, const double* __restrict__ prm,
-                               double* __restrict__ mfx, double* __restrict__ mfy, const DevStatus* st) {
+                               double* __restrict__ mfx, double* __restrict__ mfy, int ntiles, uint32_t* tcnt,
+                               uint8_t* tact, int begin, uint32_t begin_it, DevStatus* st) {
+    if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     if (step_stopped(st)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->ntouched = 0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        tcnt[t] = 0;
+        tact[t] = 0;
+    }
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (int)st->nm) return;
     const double x1 = mx[k], y1 = my[k];
@@ -85,23 +94,38 @@ constexpr int ST = 16;            // spread tile edge
 constexpr int SP_IDX = 1024;      // markers gathered per tile
 constexpr int SP_CH = 128;        // marker data staged per pass
 
-__global__ void k_tile_clear(int ntiles, uint32_t* tcnt, uint8_t* tact, DevStatus* st) {
-    if (step_stopped(st)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->ntouched = 0;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
-        tcnt[t] = 0;
-        tact[t] = 0;
-    }
-}
-
 __global__ void k_tile_count(int ntx, int nty, const double* __restrict__ mx, const double* __restrict__ my,
                              uint32_t* tcnt, uint32_t* tslot, uint8_t* tact, const DevStatus* st) {
     if (step_stopped(st)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (int)st->nm) return;
-    const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
-    const int home = (fy / ST) * ntx + fx / ST;
-    tslot[k] = atomicAdd(&tcnt[home], 1u);
+    const bool live = k < (int)st->nm;
+    int fx = 0, fy = 0, home = -1;
+    if (live) {
+        fx = (int)floor(mx[k]);
+        fy = (int)floor(my[k]);
+        home = (fy / ST) * ntx + fx / ST;
+    }
+    // consecutive markers (one membrane) mostly share a home tile: the head
+    // lane of each run of equal tiles in the wavefront takes one atomic
+    const int lane = threadIdx.x & 63;
+    const int hprev = __shfl_up(home, 1, 64);
+    const bool head = live && (lane == 0 || hprev != home);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int hl = 63 - __clzll((long long)(heads & le));
+    const unsigned long long gt = heads & ~le;
+    const int nextl = gt ? __ffsll((long long)gt) - 1 : 64;
+    uint32_t base = 0;
+    if (head) {
+        // lanes past the last live marker are not live: the run ends there
+        int end = nextl;
+        const int nlive = (int)st->nm - (k - lane);
+        if (end > nlive) end = nlive;
+        base = atomicAdd(&tcnt[home], (uint32_t)(end - lane));
+    }
+    base = __shfl(base, hl < 0 ? 0 : hl, 64);
+    if (!live) return;
+    tslot[k] = base + (uint32_t)(lane - hl);
     for (int ty = (fy - 1) / ST; ty <= (fy + 2) / ST; ty++)
         for (int tx = (fx - 1) / ST; tx <= (fx + 2) / ST; tx++)
             if (tx < ntx && ty < nty) tact[ty * ntx + tx] = 1;
@@ -124,7 +148,7 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
                                                      const double* __restrict__ my, const double* __restrict__ mfx,
                                                      const double* __restrict__ mfy, double* Fx, double* Fy,
                                                      DevStatus* st) {
-    const int t = blockIdx.x;
+    const int t = (int)xcd_contiguous(blockIdx.x, gridDim.x);
     if (!tact[t] || step_stopped(st)) return;
     __shared__ int32_t idx[SP_IDX];
     __shared__ int32_t sorted[SP_IDX];
@@ -214,8 +238,11 @@ __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mv
                               double* mvy, double* mox, double* moy, const double* __restrict__ ux,
                               const double* __restrict__ uy, DevStatus* st) {
     if (step_stopped(st)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (int)st->nm) return;
+    // the grid covers the capacity: renumber over the workgroups with markers
+    const uint32_t nm = st->nm, nact = (nm + blockDim.x - 1) / blockDim.x;
+    if (blockIdx.x >= nact) return;
+    const int k = (int)(xcd_contiguous(blockIdx.x, nact) * blockDim.x + threadIdx.x);
+    if (k >= (int)nm) return;
     const double X = mx[k], Y = my[k];
     const int fx = (int)floor(X), fy = (int)floor(Y);
     const int base = 4 * (fx - 1) + (fy - 1);
@@ -259,25 +286,22 @@ __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mv
     }
 }
 
-__global__ void k_reset_spread_counters(DevStatus* st) {
-    if (!step_stopped(st)) st->ntouched = 0;
-}
-
 static inline int nblocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
 int launch_force_and_spread(lbib_ctx* c) {
     const int bs = 256;
     const int ntiles = c->stx * c->sty;
-    k_tile_clear<<<std::min((ntiles + bs - 1) / bs, 4096), bs, 0, c->stream>>>(ntiles, c->tcnt, c->tact, c->st);
+    // marker counts are device-resident (remeshing changes them): grids cover
+    // the capacity and the kernels read st->nm
+    const int begin = c->begin_pending ? 1 : 0;
+    c->begin_pending = false;
+    k_marker_force<<<std::max(nblocks(c->mcap, bs), 1), bs, 0, c->stream>>>(
+        c->mx, c->my, c->fs_off, c->fs_type, c->fs_partner, c->fs_prm, c->mfx, c->mfy, ntiles, c->tcnt, c->tact, begin,
+        c->begin_it, c->st);
     if (c->nm == 0) {
-        k_reset_spread_counters<<<1, 1, 0, c->stream>>>(c->st);
         LBIB_HIP(hipGetLastError());
         return 0;
     }
-    // marker counts are device-resident (remeshing changes them): grids cover
-    // the capacity and the kernels read st->nm
-    k_marker_force<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->mx, c->my, c->fs_off, c->fs_type, c->fs_partner,
-                                                               c->fs_prm, c->mfx, c->mfy, c->st);
     k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
                                                              c->tact, c->st);
     if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;

@@ -67,6 +67,13 @@ struct DevStatus {
 constexpr uint32_t STOP_BITS = LBIB_ERR_MASK | LBIB_HALT_MASK | LBIB_SKIPPED;
 constexpr uint32_t STOP_BITS_STEP = STOP_BITS | LBIB_FLAG_REMESH;
 __device__ __forceinline__ bool step_stopped(const DevStatus* st) { return (st->bits & STOP_BITS_STEP) != 0u; }
+// Opens an iteration: records it while the status is clear, otherwise marks
+// the rest of the enqueued work as skipped (one thread).
+__device__ __forceinline__ void step_begin_dev(DevStatus* st, uint32_t it) {
+    const uint32_t b = st->bits;
+    if (b & STOP_BITS_STEP) st->bits = b | LBIB_SKIPPED;
+    else st->step_it = it;
+}
 
 struct SpreadEntry {
     int32_t marker;
@@ -173,6 +180,10 @@ struct lbib_ctx {
         size_t ev_used;
     };
     std::deque<Snapshot> snaps;
+    // lbib_step_begin is carried out by the next kernel launched (the first
+    // kernel of the lattice stage, or a kernel of its own)
+    bool begin_pending = false;
+    uint32_t begin_it = 0;
     // status
     lbib::DevStatus* st = nullptr;
     lbib::DevStatus* st_host = nullptr;   // pinned

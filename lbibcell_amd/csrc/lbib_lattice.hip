@@ -46,8 +46,9 @@ struct LatticeArgs {
 template <int NS, int FM>
 __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs a) {
     if (step_stopped(a.st)) return;
-    const int x = blockIdx.x * TILE_W + threadIdx.x;
-    const int y = blockIdx.y;
+    const uint32_t wg = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int x = (int)(wg % gridDim.x) * TILE_W + threadIdx.x;
+    const int y = (int)(wg / gridDim.x);
     if (x >= a.nx) return;
     const long n = a.n;
     const int xm = (x == 0) ? a.nx - 1 : x - 1;

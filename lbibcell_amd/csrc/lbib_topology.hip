@@ -35,41 +35,29 @@ __global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ bsum) {
     const uint32_t n = nptr ? *nptr : nfix;
     const uint32_t b0 = blockIdx.x * SCAN_B;
-    __shared__ uint32_t red[256];
+    __shared__ uint32_t wsum[16];
     uint32_t s = 0;
     if (b0 < n)
         for (uint32_t k = threadIdx.x; k < SCAN_B; k += 256) {
             const uint32_t i = b0 + k;
             if (i < n) s += in[i];
         }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) bsum[blockIdx.x] = red[0];
+    uint32_t ex;
+    const uint32_t total = block_excl_scan(s, &ex, wsum);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
 // single block: exclusive scan of the block sums; total -> *total
 __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* bsum, uint32_t nb, uint32_t* total) {
-    __shared__ uint32_t s[1024];
+    __shared__ uint32_t wsum[16];
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nb; base += 1024) {
         const uint32_t i = base + threadIdx.x;
         const uint32_t v = (i < nb) ? bsum[i] : 0u;
-        s[threadIdx.x] = v;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const uint32_t t = (threadIdx.x >= (uint32_t)o) ? s[threadIdx.x - o] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += t;
-            __syncthreads();
-        }
-        if (i < nb) bsum[i] = carry + s[threadIdx.x] - v;
-        __syncthreads();
-        carry += s[1023];
-        __syncthreads();
+        uint32_t ex;
+        const uint32_t t = block_excl_scan(v, &ex, wsum);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += t;
     }
     if (threadIdx.x == 0 && total) *total = carry;
 }
@@ -79,7 +67,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
     const uint32_t n = nptr ? *nptr : nfix;
     const uint32_t b0 = blockIdx.x * SCAN_B;
     if (b0 >= n) return;
-    __shared__ uint32_t s[256];
+    __shared__ uint32_t wsum[16];
     // each thread owns 4 consecutive elements
     uint32_t v[4];
     uint32_t loc = 0;
@@ -88,15 +76,9 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
         v[q] = (i < n) ? in[i] : 0u;
         loc += v[q];
     }
-    s[threadIdx.x] = loc;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-        const uint32_t t = (threadIdx.x >= (uint32_t)o) ? s[threadIdx.x - o] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = bsum[blockIdx.x] + s[threadIdx.x] - loc;
+    uint32_t ex;
+    (void)block_excl_scan(loc, &ex, wsum);
+    uint32_t run = bsum[blockIdx.x] + ex;
     for (int q = 0; q < 4; q++) {
         const uint32_t i = b0 + threadIdx.x * 4 + q;
         if (i < n) out[i] = run;
@@ -107,48 +89,56 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
 // Short arrays (tiles, domains): one workgroup, 4096 values per pass.
 constexpr uint32_t SCAN_SINGLE_MAX = 1u << 16;
 // nz != nullptr: also count the nonzero bytes of nz[0..n) into *nz_total.
+// A pass covers 16384 values: coalesced loads into LDS, each thread then scans
+// 16 consecutive ones (row stride 17 words avoids bank conflicts), coalesced
+// stores from LDS.
 __global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict__ in, const uint32_t* nptr,
                                                       uint32_t nfix, uint32_t* __restrict__ out, uint32_t* total,
                                                       const uint8_t* __restrict__ nz, uint32_t* nz_total) {
+    constexpr int PER = 16, PASS = 1024 * PER, ROW = PER + 1;
     const uint32_t n = nptr ? *nptr : nfix;
-    __shared__ uint32_t s[1024];
-    __shared__ uint32_t nzc;
-    if (threadIdx.x == 0) nzc = 0;
-    if (nz) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t buf[1024 * ROW];
+    uint32_t carry = 0, nzc = 0;
+    for (uint32_t b0 = 0; b0 < n; b0 += PASS) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const uint32_t e = (uint32_t)q * 1024u + threadIdx.x;   // element within the pass
+            const uint32_t i = b0 + e;
+            const uint32_t v = (i < n) ? in[i] : 0u;
+            if (nz && i < n && nz[i]) nzc++;
+            buf[(e / PER) * ROW + e % PER] = v;
+        }
         __syncthreads();
-        uint32_t c = 0;
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) c += nz[i] ? 1u : 0u;
-        atomicAdd(&nzc, c);
-    }
-    uint32_t carry = 0;
-    for (uint32_t b0 = 0; b0 < n; b0 += 4096) {
-        uint32_t v[4];
+        uint32_t* row = buf + threadIdx.x * ROW;
         uint32_t loc = 0;
-        for (int q = 0; q < 4; q++) {
-            const uint32_t i = b0 + threadIdx.x * 4 + q;
-            v[q] = (i < n) ? in[i] : 0u;
-            loc += v[q];
+#pragma unroll
+        for (int q = 0; q < PER; q++) loc += row[q];
+        uint32_t ex;
+        const uint32_t t = block_excl_scan(loc, &ex, wsum);
+        uint32_t run = carry + ex;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const uint32_t v = row[q];
+            row[q] = run;
+            run += v;
         }
-        s[threadIdx.x] = loc;
         __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {
-            const uint32_t t = (threadIdx.x >= (uint32_t)o) ? s[threadIdx.x - o] : 0u;
-            __syncthreads();
-            s[threadIdx.x] += t;
-            __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const uint32_t e = (uint32_t)q * 1024u + threadIdx.x;
+            const uint32_t i = b0 + e;
+            if (i < n) out[i] = buf[(e / PER) * ROW + e % PER];
         }
-        uint32_t run = carry + s[threadIdx.x] - loc;
-        for (int q = 0; q < 4; q++) {
-            const uint32_t i = b0 + threadIdx.x * 4 + q;
-            if (i < n) out[i] = run;
-            run += v[q];
-        }
-        carry += s[1023];
+        carry += t;
         __syncthreads();
     }
-    __syncthreads();
+    if (nz) {
+        uint32_t ex;
+        const uint32_t t = block_excl_scan(nzc, &ex, wsum);
+        if (threadIdx.x == 0) *nz_total = t;
+    }
     if (threadIdx.x == 0 && total) *total = carry;
-    if (threadIdx.x == 0 && nz) *nz_total = nzc;
 }
 
 int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive) {
@@ -436,26 +426,22 @@ __global__ __launch_bounds__(256) void k_ring_area(const uint32_t* __restrict__ 
         if (with_mass && threadIdx.x == 0) area_regulator_mass(am, d, false, 0.0);
         return;
     }
-    constexpr int CH = 1024;
-    __shared__ double sx[CH + 1], sy[CH + 1];
+    // the shoelace terms in parallel, their sum in ring order by one lane
+    constexpr int CH = 2048;
+    __shared__ double term[CH];
     const uint32_t base = roff[d];
     double sum = 0.0;
-    const double x0 = mx[ring_marker[base]], y0 = my[ring_marker[base]];
     for (uint32_t c0 = 0; c0 < n; c0 += CH) {
         const uint32_t m = min((uint32_t)CH, n - c0);
         for (uint32_t k = threadIdx.x; k < m; k += blockDim.x) {
-            const int32_t q = ring_marker[base + c0 + k];
-            sx[k] = mx[q];
-            sy[k] = my[q];
-        }
-        if (threadIdx.x == 0) {
-            const bool last = (c0 + m == n);
-            sx[m] = last ? x0 : mx[ring_marker[base + c0 + m]];
-            sy[m] = last ? y0 : my[ring_marker[base + c0 + m]];
+            const uint32_t j = c0 + k, jn = (j + 1 == n) ? 0u : j + 1;
+            const int32_t q = ring_marker[base + j], r = ring_marker[base + jn];
+            const double xk = mx[q], yk = my[q], xn = mx[r], yn = my[r];
+            term[k] = xk * yn - xn * yk;
         }
         __syncthreads();
         if (threadIdx.x == 0)
-            for (uint32_t k = 0; k < m; k++) sum += sx[k] * sy[k + 1] - sx[k + 1] * sy[k];
+            for (uint32_t k = 0; k < m; k++) sum += term[k];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
@@ -496,7 +482,14 @@ struct CoarsenArgs {
     uint32_t* rec_alive;               // [mcap]
 };
 
-// one workgroup per domain; lane 0 walks the ring's markers in id order
+// One workgroup per domain.  GeometryHandler::coarsenBoundary
+// (GeometryHandler.cpp:433-461) visits the ring's markers in id order and
+// removes a marker when its neighbours are closer than MAXLENGTH, with the
+// links as left by earlier removals.  The spans of all markers are evaluated
+// once in parallel with the initial links; the walk (one wavefront, in id
+// order) re-evaluates only markers whose links a removal changed and
+// otherwise jumps with a ballot to the next marker that may be removed.
+// Positions are staged in LDS.
 __global__ __launch_bounds__(256) void k_co_ring(const CoarsenArgs a, DevStatus* st) {
     if (topo_halted(st) || !st->co_any) return;
     const int d = blockIdx.x;
@@ -510,6 +503,8 @@ __global__ __launch_bounds__(256) void k_co_ring(const CoarsenArgs a, DevStatus*
     __shared__ int16_t lprev[RING_LDS], lnext[RING_LDS];
     __shared__ int32_t inc[RING_LDS], outc[RING_LDS];   // >=0 original conn, <0: -(creator local idx)-1
     __shared__ int32_t inbc[RING_LDS];
+    __shared__ double sx[RING_LDS], sy[RING_LDS];
+    __shared__ uint8_t flag[RING_LDS];     // bit 0: span < MAXLENGTH (initial links), bit 1: links changed
     uint32_t P = 1;
     while (P < n) P <<= 1;
     const uint32_t base = a.roff[d];
@@ -542,35 +537,62 @@ __global__ __launch_bounds__(256) void k_co_ring(const CoarsenArgs a, DevStatus*
         inbc[i] = a.cbc[a.min_c[g]];
         lprev[i] = (int16_t)local(a.ca[a.min_c[g]]);
         lnext[i] = (int16_t)local(a.cb[a.mout_c[g]]);
+        sx[i] = a.mx[g];
+        sy[i] = a.my[g];
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    uint32_t removed = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const int p = lprev[i], q = lnext[i];
-        const double px = a.mx[gid[p]], py = a.my[gid[p]], qx = a.mx[gid[q]], qy = a.my[gid[q]];
+    auto span_short = [&](int p, int q) -> bool {
+        const double px = sx[p], py = sy[p], qx = sx[q], qy = sy[q];
         const double len = sqrt((qx - px) * (qx - px) + (qy - py) * (qy - py));
-        if (!(len < kMAXLENGTH)) continue;
-        // Geometry::removeGeometryNode (Geometry.cpp:401-428)
-        const int32_t g = gid[i];
-        for (int s = 0; s < 2; s++) {
-            const int32_t cc = s == 0 ? inc[i] : outc[i];
-            if (cc >= 0) a.cdead[cc] = 1u;
-            else a.rec_alive[gid[-cc - 1]] = 0u;
+        return len < kMAXLENGTH;
+    };
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) flag[i] = span_short(lprev[i], lnext[i]) ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x >= 64) return;
+    // one wavefront; every lane takes the same decisions and performs the
+    // same LDS updates, lane 0 the global writes
+    const int lane = threadIdx.x;
+    uint32_t removed = 0;
+    for (uint32_t c0 = 0; c0 < n; c0 += 64) {
+        int from = 0;
+        while (true) {
+            const uint32_t i = c0 + lane;
+            const bool want = i < n && lane >= from && flag[i] != 0;
+            const unsigned long long m = __ballot(want);
+            if (!m) break;
+            const int l = __ffsll((long long)m) - 1;
+            from = l + 1;
+            const uint32_t ii = c0 + (uint32_t)l;
+            const int p = lprev[ii], q = lnext[ii];
+            const uint8_t f = flag[ii];
+            const bool rm = (f & 2) ? span_short(p, q) : (f & 1) != 0;
+            if (!rm) continue;
+            // Geometry::removeGeometryNode (Geometry.cpp:401-428)
+            const int32_t g = gid[ii];
+            if (lane == 0) {
+                for (int s2 = 0; s2 < 2; s2++) {
+                    const int32_t cc = s2 == 0 ? inc[ii] : outc[ii];
+                    if (cc >= 0) a.cdead[cc] = 1u;
+                    else a.rec_alive[gid[-cc - 1]] = 0u;
+                }
+                a.mdead[g] = 1u;
+                a.rec_a[g] = gid[p];
+                a.rec_b[g] = gid[q];
+                a.rec_bc[g] = inbc[ii];
+                a.rec_alive[g] = 1u;
+            }
+            const int32_t bcv = inbc[ii];
+            lnext[p] = (int16_t)q;
+            lprev[q] = (int16_t)p;
+            outc[p] = -(int32_t)ii - 1;
+            inc[q] = -(int32_t)ii - 1;
+            inbc[q] = bcv;
+            flag[p] |= 2;
+            flag[q] |= 2;
+            removed++;
         }
-        a.mdead[g] = 1u;
-        a.rec_a[g] = gid[p];
-        a.rec_b[g] = gid[q];
-        a.rec_bc[g] = inbc[i];
-        a.rec_alive[g] = 1u;
-        lnext[p] = (int16_t)q;
-        lprev[q] = (int16_t)p;
-        outc[p] = -(int32_t)i - 1;
-        inc[q] = -(int32_t)i - 1;
-        inbc[q] = inbc[i];
-        removed++;
     }
-    if (removed) atomicAdd(&st->co_removed, removed);
+    if (lane == 0 && removed) atomicAdd(&st->co_removed, removed);
 }
 
 // keep flags for the compaction scans
