@@ -253,22 +253,20 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t* ctr, uint32_t v) {
     return base + incl - v;
 }
 
-// Block-aggregated counter allocation (blockDim.x == 256): one atomic per
-// workgroup.  Every thread of the block must call it.
+// Block-aggregated counter allocation (blockDim.x a multiple of 64, at most
+// 1024): one atomic per workgroup.  Atomics on one address serialise (about
+// 10 ns each on MI355X), so counters shared by the whole grid are advanced at
+// most once per workgroup.  Every thread of the block must call it.
 __device__ __forceinline__ uint32_t block_alloc(uint32_t* ctr, uint32_t v) {
-    __shared__ uint32_t wsum[4];
+    __shared__ uint32_t wsum[16];
     __shared__ uint32_t bbase;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    uint32_t incl = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint32_t incl = wave_incl_scan(v);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        uint32_t total = 0;
+        for (int k = 0; k < nw; k++) total += wsum[k];
         bbase = total ? atomicAdd(ctr, total) : 0u;
     }
     __syncthreads();

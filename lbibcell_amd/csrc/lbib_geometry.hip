@@ -17,14 +17,20 @@ namespace lbib {
 
 __device__ __forceinline__ bool halted(const DevStatus* st) { return (st->bits & STOP_BITS) != 0u; }
 
+// closest-wins selection entries, site-major: the four slots of a site share
+// one 32-byte (distance) / 16-byte (sequence) span
+__device__ __forceinline__ long sel_index(int site, int slot) { return (long)site * 4 + (slot - 1); }
+
 // Start of the geometry stage, one launch: the Connection::perturbConnection
 // trigger (Connection.cpp:339-340, 363-364), deletion of the boundary nodes of
 // the last completed cure (GeometryHandler.cpp:363-369; their candidate count
-// is ncand_prev, fixed by k_cut_tie) and the stage counters.  A halt raised
-// meanwhile leaves both parts to be rerun, which repeats them unchanged.
+// is ncand_prev, fixed by k_cut_tie) together with the clearing of that cure's
+// selection entries, and the stage counters.  A halt raised meanwhile leaves
+// these to be rerun, which repeats them unchanged.
 __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
                                                    const double* __restrict__ mx, const double* __restrict__ my,
-                                                   const Cand* __restrict__ cand, uint32_t* tag, DevStatus* st) {
+                                                   const Cand* __restrict__ cand, uint32_t* tag,
+                                                   unsigned long long* best_dist, uint32_t* best_seq, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->nc, nprev = st->ncand_prev;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -33,8 +39,13 @@ __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ c
         if (fabs(mx[a] - mx[b]) < kEPS || fabs(my[a] - my[b]) < kEPS)
             atomicOr(&st->bits, LBIB_HALT_PERTURB_CONN);
     }
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nprev; j += stride)
-        atomicAnd(&tag[cand[j].site], DOM_MASK);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nprev; j += stride) {
+        const int site = cand[j].site, slot = cand[j].slot;
+        atomicAnd(&tag[site], DOM_MASK);
+        const long b = sel_index(site, slot);
+        best_dist[b] = ~0ull;
+        best_seq[b] = ~0u;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->ncand = 0;
         st->nbn = 0;
@@ -49,7 +60,7 @@ __device__ __forceinline__ double sqd(double x1, double y1, double x2, double y2
     return (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);   // EulerianPoint.hpp:88-96
 }
 
-__global__ void k_cut_gen(int nx, int ny, long n, const int32_t* __restrict__ ca,
+__global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const int32_t* __restrict__ ca,
                           const int32_t* __restrict__ cb, const uint32_t* __restrict__ cdom,
                           const double* __restrict__ mx, const double* __restrict__ my, Cand* cand,
                           int candcap, unsigned long long* best_dist, DevStatus* st) {
@@ -100,7 +111,7 @@ __global__ void k_cut_gen(int nx, int ny, long n, const int32_t* __restrict__ ca
         e.conn = c;
         e.win = 0;
         cand[base + k] = e;
-        atomicMin(&best_dist[(long)(slot - 1) * n + e.site], (unsigned long long)__double_as_longlong(e.dist2));
+        atomicMin(&best_dist[sel_index(e.site, slot)], (unsigned long long)__double_as_longlong(e.dist2));
         k++;
     };
     // vertical lattice lines: node S at (i, floor y) [its slot N], node N at (i, ceil y) [slot S]
@@ -126,31 +137,43 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
     if (blockIdx.x == 0 && threadIdx.x == 0) st->ncand_prev = nc;   // this cure's reset list
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
         const Cand e = cand[j];
-        const long slotix = (long)(e.slot - 1) * n + e.site;
-        if ((unsigned long long)__double_as_longlong(e.dist2) == best_dist[slotix])
-            atomicMin(&best_seq[slotix], e.seq);
+        const long b = sel_index(e.site, e.slot);
+        if ((unsigned long long)__double_as_longlong(e.dist2) == best_dist[b]) atomicMin(&best_seq[b], e.seq);
     }
 }
 
 // Winners: mark the slot, record the domain and (for BC-carrying connections)
 // the node velocity of Connection::calculateVelocity (Connection.cpp:146-168).
-__global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_seq, const int32_t* __restrict__ ca,
+// Then PhysicalNode::updateDomainIdentifier (PhysicalNode.cpp:109-124): a site
+// takes the domain of its last non-null boundary neighbour in slot order
+// E,N,W,S; every slot with a candidate has exactly one winner, so that is the
+// winner in the site's highest slot with a candidate (a set best distance),
+// which reports the site when its domain changes.
+__global__ __launch_bounds__(1024) void k_cut_win(const Cand* cand, long n, const unsigned long long* __restrict__ best_dist,
+                          const uint32_t* __restrict__ best_seq, const int32_t* __restrict__ ca,
                           const int32_t* __restrict__ cb, const int32_t* __restrict__ cbc,
                           const double* __restrict__ mx, const double* __restrict__ my,
                           const double* __restrict__ mvx, const double* __restrict__ mvy, uint32_t* tag,
-                          uint32_t* bn_dom, double* bc_v, int32_t* bc_desc, DevStatus* st) {
+                          uint32_t* bn_dom, double* bc_v, int32_t* bc_desc, uint32_t* changed, uint32_t* rmark,
+                          uint8_t* rstate, int local_update, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->ncand;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
         const uint32_t j = j0 + threadIdx.x;
         uint32_t won = 0;
+        bool add = false;
+        int site = 0;
+        uint32_t dom = 0;
         if (j < nc) {
-            Cand& e = cand[j];
+            const Cand& e = cand[j];
             const long slotix = (long)(e.slot - 1) * n + e.site;
-            if (best_seq[slotix] == e.seq) {
+            const long b = sel_index(e.site, e.slot);
+            const uint32_t dom_now = tag[e.site] & DOM_MASK;   // domain bits are not changed here
+            if (best_seq[b] == e.seq) {
                 won = 1;
-                e.win = 1;
+                site = e.site;
+                dom = e.domain;
                 bn_dom[slotix] = e.domain;
                 uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
                 const int desc = cbc[e.conn];
@@ -164,44 +187,21 @@ __global__ void k_cut_win(Cand* cand, long n, const uint32_t* __restrict__ best_
                     bc_desc[slotix] = desc;
                 }
                 atomicOr(&tag[e.site], bits);
+                if (local_update && dom_now != e.domain) {
+                    bool highest = true;
+                    for (int s2 = e.slot + 1; s2 <= 4; s2++)
+                        if (best_dist[sel_index(e.site, s2)] != ~0ull) highest = false;
+                    add = highest;
+                }
             }
         }
         (void)block_alloc(&st->nbn, won);
-    }
-}
-
-// PhysicalNode::updateDomainIdentifier (PhysicalNode.cpp:109-124): a site takes
-// the domain of its last non-null boundary neighbour in slot order E,N,W,S.
-// Only sites with boundary nodes can change; the winner in the highest
-// occupied slot reports the site.  Also resets the selection buffers.
-__global__ void k_cut_local(const Cand* __restrict__ cand, long n, const uint32_t* __restrict__ tag,
-                            unsigned long long* best_dist, uint32_t* best_seq, uint32_t* changed,
-                            uint32_t* rmark, uint8_t* rstate, int local_update, DevStatus* st) {
-    if (halted(st)) return;
-    const uint32_t nc = st->ncand;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
-        const uint32_t j = j0 + threadIdx.x;
-        bool add = false;
-        Cand e;
-        if (j < nc) {
-            e = cand[j];
-            const long slotix = (long)(e.slot - 1) * n + e.site;
-            best_dist[slotix] = ~0ull;
-            best_seq[slotix] = ~0u;
-            if (e.win && local_update) {
-                const uint32_t tg = tag[e.site];
-                const uint32_t mask = (tg >> BN_SHIFT) & 0xfu;
-                const int hi = 31 - __clz((int)mask);   // highest occupied slot - 1
-                add = (hi + 1 == e.slot) && (tg & DOM_MASK) != e.domain;
-            }
-        }
-        const uint32_t p = wave_alloc(&st->nchanged, add ? 1u : 0u);
+        const uint32_t p = block_alloc(&st->nchanged, add ? 1u : 0u);
         if (!add) continue;
         if (p < (uint32_t)REINIT_CAP) {
-            changed[2 * p] = (uint32_t)e.site;
-            changed[2 * p + 1] = e.domain;
-            rmark[e.site] = p + 1;
+            changed[2 * p] = (uint32_t)site;
+            changed[2 * p + 1] = dom;
+            rmark[site] = p + 1;
             rstate[p] = 0;
         } else {
             atomicOr(&st->bits, LBIB_ERR_CAPACITY);
@@ -326,7 +326,7 @@ __global__ __launch_bounds__(256) void k_reinit_level(const ReinitArgs a, uint8_
         state[t] = (uint8_t)(2 + level);
         any = true;
     }
-    if (__any(any) && (threadIdx.x & 63) == 0) atomicMax(&st->reinit_levels, (uint32_t)level + 1);
+    if (__syncthreads_or(any) && threadIdx.x == 0) atomicMax(&st->reinit_levels, (uint32_t)level + 1);
 }
 
 template <int NS>
@@ -464,16 +464,17 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     const int gs = 1024;
     // connection counts are device-resident (remeshing changes them): grids
     // cover the capacity and the kernels read st->nc
-    k_geo_begin<<<gs, bs, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->cand, c->tag, c->st);
+    k_geo_begin<<<gs, bs, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->cand, c->tag, c->best_dist, c->best_seq,
+                                          c->st);
+    // 1024-wide workgroups: one candidate-counter atomic per 1024 connections
     if (c->ccap > 0)
-        k_cut_gen<<<(c->ccap + bs - 1) / bs, bs, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom,
+        k_cut_gen<<<(c->ccap + 1023) / 1024, 1024, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom,
                                                                c->mx, c->my, c->cand, c->candcap, c->best_dist,
                                                                c->st);
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
-    k_cut_win<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my, c->mvx,
-                                        c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->st);
-    k_cut_local<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->tag, c->best_dist, c->best_seq, c->changed, c->rmark, c->rstate, cure ? 0 : 1,
-                                          c->st);
+    k_cut_win<<<256, 1024, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my,
+                                        c->mvx, c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->changed, c->rmark,
+                                        c->rstate, cure ? 0 : 1, c->st);
     if (!cure) {
         ReinitArgs a;
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;

@@ -151,8 +151,8 @@ struct lbib_ctx {
     // cut links
     int candcap = 0;
     lbib::Cand* cand = nullptr;
-    unsigned long long* best_dist = nullptr;   // [4][n]
-    uint32_t* best_seq = nullptr;              // [4][n]
+    unsigned long long* best_dist = nullptr;   // [n][4] (site-major)
+    uint32_t* best_seq = nullptr;              // [n][4]
     uint32_t* bn_dom = nullptr;                // [4][n]
     double* bc_v = nullptr;                    // [4][n][2]
     int32_t* bc_desc = nullptr;                // [4][n]
