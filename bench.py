@@ -84,12 +84,55 @@ def cpu_baseline(nx, ny, nodes, conns, budget_s=15.0, max_steps=50):
                       f"oracle with {threads} OpenMP threads on the lattice loops, serial IB/geometry"}
 
 
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+DTYPES = {"fp64": "f64", "fp32": "f32", "fp16": "f32 arithmetic, f16 storage (f - w)",
+          "bf16": "f32 arithmetic, bf16 storage (f - w)"}
+KERNELS = {"fp64": "k_stream_collide_f64<1,0>", "fp32": "k_stream_collide_mx<1,0,FP32>",
+           "fp16": "k_stream_collide_mx<1,0,FP16_DEV>", "bf16": "k_stream_collide_mx<1,0,BF16_DEV>"}
+
+
+def measure(nx, ny, nodes, conns, prec_name, steps, warmup, run_ahead):
+    """Build the workload at one storage precision, warm up, time `steps`
+    coupled steps.  Returns timings, the lattice-kernel roofline figures and
+    the final fields (for the error of mixed precision against FP64)."""
+    sim = build_device(nx, ny, nodes, conns, PRECISIONS[prec_name])
+    sim.set_run_ahead(run_ahead)
+    log(f"[{prec_name}] device setup done; warming up")
+    sim.run(warmup)
+    sim.sync()
+    ctx = sim.context()
+    ctx.profile_enable(True)
+    ctx.profile_read()
+    sim.reset_times()
+    log(f"[{prec_name}] timing {steps} steps")
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    ms, launches, bpl = ctx.profile_read()
+    ctx.profile_enable(False)
+    ctx.ns = 1
+    u, v = ctx.get_velocity()
+    fields = dict(speed=np.hypot(u, v), conc=ctx.get_species(0).sum(axis=0), area=sim.areas())
+    out = dict(dt=t1 - t0, kernel_s=ms / 1e3 / max(launches, 1), bytes_per_launch=bpl,
+               markers=len(sim.markers()[0]), times=sim.times(), fields=fields, ctx=ctx, sim=sim)
+    return out
+
+
+def l2_rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--precision", default="fp64", choices=["fp64"])
+    ap.add_argument("--precision", default="fp64", choices=list(PRECISIONS),
+                    help="storage precision of the headline line (fp64 is the parity path)")
+    ap.add_argument("--mixed", default="fp32,fp16,bf16",
+                    help="comma-separated mixed precisions also measured and compared with fp64 ('' = none)")
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--run-ahead", type=int, default=16,
@@ -99,72 +142,83 @@ def main():
         print("bench.py measures one GPU: the lattice path runs on a single MI355X per process; "
               "multi-GPU slab decomposition is a design note, not built", file=sys.stderr)
         sys.exit(2)
-    from lbibcell_amd import capi
-    prec = {"fp64": capi.FP64}[args.precision]
     nx = ny = args.size
     cols = rows = max(1, round(20 * args.size / 2048))
     nodes, conns = tissue_case(nx, ny, cols, rows)
-    log(f"workload {nx}x{ny}, {len(set(c[2] for c in conns))} cells, {len(nodes)} initial markers")
-    sim = build_device(nx, ny, nodes, conns, prec)
-    sim.set_run_ahead(args.run_ahead)
-    log("device setup done; warming up")
-    sim.run(args.warmup)
-    sim.sync()
-    ctx = sim.context()
-    ctx.profile_enable(True)
-    ctx.profile_read()
-    sim.reset_times()
-    log(f"timing {args.steps} steps")
-    sim.sync()
-    t0 = time.perf_counter()
-    sim.run(args.steps)
-    sim.sync()
-    t1 = time.perf_counter()
-    ms, launches, bpl = ctx.profile_read()
-    ctx.profile_enable(False)
-    dt = t1 - t0
+    ncell = len(set(c[2] for c in conns))
+    log(f"workload {nx}x{ny}, {ncell} cells, {len(nodes)} initial markers")
     n = nx * ny
-    mlups = n * args.steps / dt / 1e6
-    kern_s = ms / 1e3 / max(launches, 1)
-    achieved = bpl / kern_s / 1e9
-    copy_bytes = 1 << 30
-    copy_ms = ctx.profile_copy(copy_bytes, 10)
-    copy_gbs = 2 * copy_bytes / (copy_ms / 1e3) / 1e9
-    nmark = len(sim.markers()[0])
-    times = sim.times()
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_stream_collide_{args.precision}_{nx}.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("bytes_per_launch")
+    variants = [args.precision] + [p for p in args.mixed.split(",") if p and p != args.precision]
+    if "fp64" not in variants and variants[1:]:
+        variants.insert(1, "fp64")   # mixed errors are taken against the FP64 path
+    res = {}
+    copy_gbs = None
+    for pname in variants:
+        r = measure(nx, ny, nodes, conns, pname, args.steps, args.warmup, args.run_ahead)
+        if copy_gbs is None:
+            copy_bytes = 1 << 30
+            copy_gbs = 2 * copy_bytes / (r["ctx"].profile_copy(copy_bytes, 10) / 1e3) / 1e9
+        r["sim"].close()
+        del r["sim"], r["ctx"]
+        res[pname] = r
+
+    def roofline(pname):
+        r = res[pname]
+        achieved = r["bytes_per_launch"] / r["kernel_s"] / 1e9
+        traffic = None
+        tfile = os.path.join(ROOT, "profiles", f"traffic_stream_collide_{pname}_{nx}.json")
+        if os.path.exists(tfile):
+            traffic = json.load(open(tfile)).get("bytes_per_launch")
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": KERNELS[pname],
+                "kernel_ms_avg": r["kernel_s"] * 1e3, "algorithmic_bytes_per_launch": r["bytes_per_launch"],
+                "copy_probe_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs}
+
+    head = res[args.precision]
+    times = head["times"]
     out = {
         "metric": "MLUPS (coupled D2Q9+D2Q5+IB lattice step)",
-        "value": mlups,
+        "value": n * args.steps / head["dt"] / 1e6,
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
+        "ms_per_step": head["dt"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64"}[args.precision],
+        "dtype": DTYPES[args.precision],
         "data": "synthetic (seeded hex tissue geometry)",
-        "config": {"workload": f"tissue {nx}x{ny}: D2Q9+D2Q5, {len(set(c[2] for c in conns))} cells r=40+-4, "
+        "config": {"workload": f"tissue {nx}x{ny}: D2Q9+D2Q5, {ncell} cells r=40+-4, "
                                f"150 markers/cell (remeshed), tension 0.01, area regulator 1.2xA0, box outlet",
-                   "precision": args.precision, "markers_after_remesh": nmark, "run_ahead": args.run_ahead},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_stream_collide_f64<1,0>", "kernel_ms_avg": kern_s * 1e3,
-                     "algorithmic_bytes_per_launch": bpl,
-                     "copy_probe_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs},
+                   "precision": args.precision, "markers_after_remesh": head["markers"],
+                   "run_ahead": args.run_ahead},
+        "roofline": roofline(args.precision),
         "host_seconds_per_step": {k: v / max(times["steps"], 1) for k, v in times.items()
                                   if k not in ("steps", "stops")},
         "resolved_stops": times["stops"],
     }
+    mixed = {}
+    for pname in variants:
+        if pname == "fp64":
+            continue
+        r, ref = res[pname], res.get("fp64")
+        mixed[pname] = {
+            "value": n * args.steps / r["dt"] / 1e6, "unit": "MLUPS", "ms_per_step": r["dt"] / args.steps * 1e3,
+            "dtype": DTYPES[pname], "roofline": roofline(pname)}
+        if ref is not None:
+            common = sorted(set(r["fields"]["area"]) & set(ref["fields"]["area"]))
+            a_m = np.array([r["fields"]["area"][d] for d in common])
+            a_r = np.array([ref["fields"]["area"][d] for d in common])
+            mixed[pname]["l2_rel_error_vs_fp64"] = {
+                "speed": l2_rel(r["fields"]["speed"], ref["fields"]["speed"]),
+                "concentration": l2_rel(r["fields"]["conc"], ref["fields"]["conc"]),
+                "cell_area": l2_rel(a_m, a_r), "after_steps": args.warmup + args.steps}
+    if mixed:
+        out["mixed_precision"] = mixed
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(nx, ny, nodes, conns)
-    sim.close()
     print(json.dumps(out), flush=True)
 
 

@@ -246,7 +246,7 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
         if (dev_alloc(c, (uint8_t**)&c->f[b], 9 * n * sb)) return -1;
         if (dev_alloc(c, (uint8_t**)&c->g[b], (size_t)std::max(c->ns, 1) * 5 * n * sb)) return -1;
     }
-    if (dev_alloc(c, &c->ux, n) || dev_alloc(c, &c->uy, n) || dev_alloc(c, &c->Fx, n) || dev_alloc(c, &c->Fy, n) ||
+    if (dev_alloc(c, &c->u, 2 * n) || dev_alloc(c, &c->Fx, n) || dev_alloc(c, &c->Fy, n) ||
         dev_alloc(c, &c->msite, n))
         return -1;
     c->stx = (c->nx + 15) / 16;
@@ -291,7 +291,7 @@ void lbib_destroy(lbib_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
-    dev_free(c->ux); dev_free(c->uy); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
+    dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
     dev_free(c->tslot); dev_free(c->tlist);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
@@ -382,8 +382,23 @@ int lbib_get_species(lbib_ctx* c, int s, double* g) {
 }
 int lbib_get_velocity(lbib_ctx* c, double* ux, double* uy) {
     if (check_ctx(c) || !ux || !uy) return -1;
-    if (download(c, ux, c->ux, c->n * sizeof(double))) return -1;
-    return download(c, uy, c->uy, c->n * sizeof(double));
+    if (c->prec == LBIB_FP64) {
+        std::vector<double> u(2 * (size_t)c->n);
+        if (download(c, u.data(), c->u, u.size() * sizeof(double))) return -1;
+        for (long i = 0; i < c->n; i++) {
+            ux[i] = u[2 * i];
+            uy[i] = u[2 * i + 1];
+        }
+        return 0;
+    }
+    // mixed precision keeps FP32 pairs in the same allocation
+    std::vector<float> u(2 * (size_t)c->n);
+    if (download(c, u.data(), c->u, u.size() * sizeof(float))) return -1;
+    for (long i = 0; i < c->n; i++) {
+        ux[i] = (double)u[2 * i];
+        uy[i] = (double)u[2 * i + 1];
+    }
+    return 0;
 }
 int lbib_get_domains(lbib_ctx* c, uint32_t* dom) {
     if (check_ctx(c) || !dom) return -1;
@@ -403,16 +418,36 @@ int lbib_set_domains(lbib_ctx* c, const uint32_t* dom) {
     LBIB_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
+// Mixed-precision contexts keep the force as FP32 (Fx, Fy) pairs in Fx's
+// allocation.
 int lbib_set_force_field(lbib_ctx* c, const double* fx, const double* fy) {
     if (check_ctx(c) || !fx || !fy) return -1;
-    if (upload(c, c->Fx, fx, c->n * sizeof(double)) || upload(c, c->Fy, fy, c->n * sizeof(double))) return -1;
+    if (c->prec == LBIB_FP64) {
+        if (upload(c, c->Fx, fx, c->n * sizeof(double)) || upload(c, c->Fy, fy, c->n * sizeof(double))) return -1;
+    } else {
+        std::vector<float> p(2 * (size_t)c->n);
+        for (long i = 0; i < c->n; i++) {
+            p[2 * i] = (float)fx[i];
+            p[2 * i + 1] = (float)fy[i];
+        }
+        if (upload(c, c->Fx, p.data(), p.size() * sizeof(float))) return -1;
+    }
     LBIB_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
 int lbib_get_force_field(lbib_ctx* c, double* fx, double* fy) {
     if (check_ctx(c) || !fx || !fy) return -1;
-    if (download(c, fx, c->Fx, c->n * sizeof(double))) return -1;
-    return download(c, fy, c->Fy, c->n * sizeof(double));
+    if (c->prec == LBIB_FP64) {
+        if (download(c, fx, c->Fx, c->n * sizeof(double))) return -1;
+        return download(c, fy, c->Fy, c->n * sizeof(double));
+    }
+    std::vector<float> p(2 * (size_t)c->n);
+    if (download(c, p.data(), c->Fx, p.size() * sizeof(float))) return -1;
+    for (long i = 0; i < c->n; i++) {
+        fx[i] = (double)p[2 * i];
+        fy[i] = (double)p[2 * i + 1];
+    }
+    return 0;
 }
 int lbib_set_mass_field(lbib_ctx* c, const double* m) {
     if (check_ctx(c) || !m) return -1;

@@ -17,6 +17,45 @@ __device__ constexpr double kDoublePi = 2.0 * M_PI;
 __device__ constexpr double kEPS = 10E-10;            // Constants.hpp:34
 __device__ constexpr double kMAXLENGTH = 0.5;         // Constants.hpp:49
 
+// Population storage formats: FP64 (the parity path), FP32, and FP16 / BF16
+// (fluid populations as deviations f_i - w_i; species populations as they
+// are).  dec/enc convert one stored value to and from the arithmetic type.
+template <int P>
+struct Codec;
+template <>
+struct Codec<LBIB_FP64> {
+    using T = double;
+    static constexpr bool dev = false;
+    __device__ static double dec(T v) { return v; }
+    __device__ static T enc(double x) { return x; }
+};
+template <>
+struct Codec<LBIB_FP32> {
+    using T = float;
+    static constexpr bool dev = false;
+    __device__ static float dec(T v) { return v; }
+    __device__ static T enc(float x) { return x; }
+};
+template <>
+struct Codec<LBIB_FP16_DEV> {
+    using T = uint16_t;
+    static constexpr bool dev = true;
+    __device__ static float dec(T v) { return (float)__builtin_bit_cast(_Float16, v); }
+    __device__ static T enc(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }   // round to nearest even
+};
+template <>
+struct Codec<LBIB_BF16_DEV> {
+    using T = uint16_t;
+    static constexpr bool dev = true;
+    __device__ static float dec(T v) { return __uint_as_float((uint32_t)v << 16); }
+    __device__ static T enc(float x) {   // round to nearest even, NaN kept quiet
+        uint32_t u = __float_as_uint(x);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (T)((u >> 16) | 0x40u);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (T)(u >> 16);
+    }
+};
+
 // GeometryNode.cpp:55-81 (parabolic sine approximation, not cos)
 __device__ __forceinline__ double fast_cos(const double x) {
     double c;

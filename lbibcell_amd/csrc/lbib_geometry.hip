@@ -213,9 +213,8 @@ struct ReinitArgs {
     int nx, ny;
     long n;
     int ns;
-    double* g;           // current species buffer [ns][5][n]
-    const double* ux;
-    const double* uy;
+    void* g;             // current species buffer [ns][5][n] in the storage format
+    const void* u;       // velocity pairs (ux, uy): FP64, or FP32 for mixed precision
     uint32_t* tag;
     const uint32_t* changed;
     const uint32_t* rmark;
@@ -223,8 +222,12 @@ struct ReinitArgs {
     SpeciesDev sp[LBIB_MAX_SPECIES];
 };
 
-template <int NS>
+// P: storage format; the re-initialisation itself is FP64 arithmetic.
+template <int NS, int P>
 __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd) {
+    using CD = Codec<P>;
+    using T = typename CD::T;
+    T* const gb = static_cast<T*>(a.g);
     const int x = (int)(i % a.nx), y = (int)(i / a.nx);
     const int EXd[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
     const int EYd[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -234,9 +237,18 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
     };
     const long sn = 5 * a.n;
     auto cswap = [&](int k, long j) -> double {
-        const double* gk = a.g + k * sn;
-        return c_swapped(gk[j], gk[a.n + j], gk[2 * a.n + j], gk[3 * a.n + j], gk[4 * a.n + j]);
+        const T* gk = gb + k * sn;
+        return c_swapped((double)CD::dec(gk[j]), (double)CD::dec(gk[a.n + j]), (double)CD::dec(gk[2 * a.n + j]),
+                         (double)CD::dec(gk[3 * a.n + j]), (double)CD::dec(gk[4 * a.n + j]));
     };
+    double ux_i, uy_i;
+    if (P == LBIB_FP64) {
+        ux_i = static_cast<const double*>(a.u)[2 * i];
+        uy_i = static_cast<const double*>(a.u)[2 * i + 1];
+    } else {
+        ux_i = (double)static_cast<const float*>(a.u)[2 * i];
+        uy_i = (double)static_cast<const float*>(a.u)[2 * i + 1];
+    }
     a.tag[i] = (a.tag[i] & ~DOM_MASK) | nd;
     const uint32_t tg = a.tag[i];
     for (int k = 0; k < NS; k++) {
@@ -266,9 +278,9 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         for (int q = 0; q < 5; q++) C += d[q];
         double cj[NS > 0 ? NS : 1];
         for (int j = 0; j < NS; j++) cj[j] = (j == k) ? C : cswap(j, i);
-        species_collide<NS>(d, C, a.ux[i], a.uy[i], a.sp[k], nd, a.it, cj);
-        double* gk = a.g + k * sn;
-        for (int q = 0; q < 5; q++) gk[q * a.n + i] = d[q];
+        species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], nd, a.it, cj);
+        T* gk = gb + k * sn;
+        for (int q = 0; q < 5; q++) gk[q * a.n + i] = CD::enc(d[q]);
     }
 }
 
@@ -313,7 +325,7 @@ __device__ __forceinline__ bool reinit_ready(const ReinitArgs& a, uint32_t cnt, 
     return true;
 }
 
-template <int NS>
+template <int NS, int P>
 __global__ __launch_bounds__(256) void k_reinit_level(const ReinitArgs a, uint8_t* state, int level, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
@@ -322,14 +334,14 @@ __global__ __launch_bounds__(256) void k_reinit_level(const ReinitArgs a, uint8_
         if (state[t] != 0) continue;
         const uint32_t i = a.changed[2 * t];
         if (!reinit_ready(a, cnt, i, state, 2, 2 + level)) continue;
-        reinit_site<NS>(a, (long)i, a.changed[2 * t + 1]);
+        reinit_site<NS, P>(a, (long)i, a.changed[2 * t + 1]);
         state[t] = (uint8_t)(2 + level);
         any = true;
     }
     if (__syncthreads_or(any) && threadIdx.x == 0) atomicMax(&st->reinit_levels, (uint32_t)level + 1);
 }
 
-template <int NS>
+template <int NS, int P>
 __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gstate, DevStatus* st) {
     if (halted(st)) return;
     __shared__ uint8_t lstate[REINIT_LDS];
@@ -360,7 +372,7 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gs
         // phase 2: run them
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
             if (state[t] != 1) continue;
-            reinit_site<NS>(a, (long)a.changed[2 * t], a.changed[2 * t + 1]);
+            reinit_site<NS, P>(a, (long)a.changed[2 * t], a.changed[2 * t + 1]);
         }
         __syncthreads();
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x)
@@ -446,11 +458,20 @@ __global__ __launch_bounds__(256) void k_scanline(int nx, long n, uint32_t* tag,
     }
 }
 
+template <int NS, int P>
+static void launch_reinit_p(lbib_ctx* c, const ReinitArgs& a) {
+    for (int level = 0; level < REINIT_GRID_LEVELS; level++)
+        k_reinit_level<NS, P><<<64, 256, 0, c->stream>>>(a, c->rstate, level, c->st);
+    k_reinit<NS, P><<<1, 1024, 0, c->stream>>>(a, c->rstate, c->st);
+}
 template <int NS>
 static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
-    for (int level = 0; level < REINIT_GRID_LEVELS; level++)
-        k_reinit_level<NS><<<64, 256, 0, c->stream>>>(a, c->rstate, level, c->st);
-    k_reinit<NS><<<1, 1024, 0, c->stream>>>(a, c->rstate, c->st);
+    switch (c->prec) {
+        case LBIB_FP64: launch_reinit_p<NS, LBIB_FP64>(c, a); break;
+        case LBIB_FP32: launch_reinit_p<NS, LBIB_FP32>(c, a); break;
+        case LBIB_FP16_DEV: launch_reinit_p<NS, LBIB_FP16_DEV>(c, a); break;
+        default: launch_reinit_p<NS, LBIB_BF16_DEV>(c, a); break;
+    }
 }
 
 int launch_topology(lbib_ctx* c, bool coarsen);
@@ -478,8 +499,8 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     if (!cure) {
         ReinitArgs a;
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;
-        a.g = (double*)c->g[c->cur];
-        a.ux = c->ux; a.uy = c->uy; a.tag = c->tag; a.changed = c->changed; a.rmark = c->rmark; a.it = it;
+        a.g = c->g[c->cur];
+        a.u = c->u; a.tag = c->tag; a.changed = c->changed; a.rmark = c->rmark; a.it = it;
         for (int s = 0; s < LBIB_MAX_SPECIES; s++) a.sp[s] = c->sp[s];
         switch (c->ns) {
             case 0: launch_reinit_t<0>(c, a); break;

@@ -141,7 +141,10 @@ __global__ void k_tile_scatter(int ntx, const double* __restrict__ mx, const dou
     tlist[toff[(fy / ST) * ntx + fx / ST] + tslot[k]] = k;
 }
 
-// one workgroup per tile; tiles no marker stencil reaches return at once
+// one workgroup per tile; tiles no marker stencil reaches return at once.
+// F32OUT: mixed-precision lattices take the force as FP32 pairs in Fx's
+// allocation (the sums themselves are FP64 either way).
+template <bool F32OUT>
 __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
                                                      const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
                                                      const int32_t* __restrict__ tlist, const double* __restrict__ mx,
@@ -223,8 +226,12 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
     }
     if (px < nx && py < ny) {
         const long site = (long)py * nx + px;
-        Fx[site] = ax;
-        Fy[site] = ay;
+        if (F32OUT) {
+            reinterpret_cast<float2*>(Fx)[site] = make_float2((float)ax, (float)ay);
+        } else {
+            Fx[site] = ax;
+            Fy[site] = ay;
+        }
     }
 }
 
@@ -234,9 +241,12 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
 // and detection of the perturbation cases of GeometryNode::perturbatePosition
 // (GeometryNode.cpp:153-214) and of the boundary checks of
 // GeometryHandler.cpp:613-636.
+// UT: storage type of the velocity pairs (FP64, or FP32 for mixed-precision
+// lattices); the interpolation and the move are FP64 either way.
+template <typename UT>
 __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mvx,
-                              double* mvy, double* mox, double* moy, const double* __restrict__ ux,
-                              const double* __restrict__ uy, DevStatus* st) {
+                              double* mvy, double* mox, double* moy, const UT* __restrict__ u,
+                              DevStatus* st) {
     if (step_stopped(st)) return;
     // the grid covers the capacity: renumber over the workgroups with markers
     const uint32_t nm = st->nm, nact = (nm + blockDim.x - 1) / blockDim.x;
@@ -256,8 +266,18 @@ __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mv
         const int sx = fx - 1 + (t >> 2), sy = fy - 1 + (t & 3);
         const long site = (long)sy * nx + sx;
         const double dd = (fxa[t >> 2] * fya[t & 3]) / 16.0;
-        vx += dd * ux[site];
-        vy += dd * uy[site];
+        double ux_s, uy_s;
+        if constexpr (sizeof(UT) == 8) {
+            const double2 us = *reinterpret_cast<const double2*>(u + 2 * site);
+            ux_s = us.x;
+            uy_s = us.y;
+        } else {
+            const float2 us = *reinterpret_cast<const float2*>(u + 2 * site);
+            ux_s = (double)us.x;
+            uy_s = (double)us.y;
+        }
+        vx += dd * ux_s;
+        vy += dd * uy_s;
     }
     mvx[k] = vx;
     mvy[k] = vy;
@@ -307,8 +327,12 @@ int launch_force_and_spread(lbib_ctx* c) {
     if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;
     k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist,
                                                                c->st);
-    k_tile_spread<<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt, c->tlist,
-                                                 c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
+    if (c->prec == LBIB_FP64)
+        k_tile_spread<false><<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
+                                                            c->tlist, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
+    else
+        k_tile_spread<true><<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
+                                                           c->tlist, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
@@ -317,8 +341,12 @@ int launch_clear_force(lbib_ctx*) { return 0; }
 
 int launch_interp_move(lbib_ctx* c) {
     if (c->nm == 0) return 0;
-    k_interp_move<<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy,
-                                                               c->mox, c->moy, c->ux, c->uy, c->st);
+    if (c->prec == LBIB_FP64)
+        k_interp_move<double><<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(c->nx, c->ny, c->mx, c->my, c->mvx,
+                                                                           c->mvy, c->mox, c->moy, c->u, c->st);
+    else
+        k_interp_move<float><<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(
+            c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy, c->mox, c->moy, reinterpret_cast<const float*>(c->u), c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

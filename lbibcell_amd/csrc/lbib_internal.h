@@ -4,7 +4,8 @@
 //   f[2][9][n]        populations, natural direction order, post-collision
 //                     (A/B ping-pong; pull streaming reads cur, writes 1-cur)
 //   g[2][ns][5][n]    species populations, same convention
-//   ux[n], uy[n]      FluidSolver::velocity_ (post-collision)
+//   u[n][2]           FluidSolver::velocity_ (post-collision), (ux, uy) per site
+//                     so that one 16-byte access serves an interpolation stencil site
 //   Fx[n], Fy[n]      FluidSolver::force_ (IB spread target, zero off-membrane)
 //   tag[n]            bits 0-23 domain id, 24-27 boundary node present in slot
 //                     E,N,W,S, 28-31 that boundary node carries a no-flux BC
@@ -108,7 +109,7 @@ struct lbib_ctx {
     void* f[2] = {nullptr, nullptr};
     void* g[2] = {nullptr, nullptr};
     int cur = 0;
-    double *ux = nullptr, *uy = nullptr, *Fx = nullptr, *Fy = nullptr, *msite = nullptr;
+    double *u = nullptr, *Fx = nullptr, *Fy = nullptr, *msite = nullptr;
     bool msite_live = false;
     uint32_t* tag = nullptr;
     lbib::SpeciesDev sp[LBIB_MAX_SPECIES];
@@ -192,6 +193,9 @@ struct lbib_ctx {
 namespace lbib {
 // launchers implemented per translation unit
 int launch_init_f64(lbib_ctx* c, const double* u0_dev);
+// the FP64 initial state into the given natural-order buffers (f[9][n],
+// g[ns][5][n], velocity pairs u[n][2])
+int launch_init_f64_into(lbib_ctx* c, const double* u0_dev, double* f, double* g, double* u);
 int launch_stream_collide_f64(lbib_ctx* c, uint32_t it, bool use_ftile);
 int launch_copy_probe(lbib_ctx* c, const void* src, void* dst, long nbytes);
 int launch_init_mixed(lbib_ctx* c, const double* u0_dev);

@@ -22,8 +22,7 @@ struct LatticeArgs {
     double* __restrict__ fd;
     const double* __restrict__ gs;
     double* __restrict__ gd;
-    double* ux;
-    double* uy;
+    double* u;           // velocity pairs (ux, uy)
     const double* Fx;
     const double* Fy;
     const uint8_t* ftile;
@@ -83,8 +82,8 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
     // (Guo forcing everywhere, the no-flux BC at boundary-node sites)
     double pu = 0.0, pv = 0.0;
     if (FM != 0 || (NS > 0 && (tg >> BC_SHIFT))) {
-        pu = a.ux[i];
-        pv = a.uy[i];
+        pu = a.u[2 * i];
+        pv = a.u[2 * i + 1];
     }
 
     double g[NS > 0 ? NS : 1][5];
@@ -172,8 +171,7 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
     for (int s = 0; s < NS; s++)
 #pragma unroll
         for (int q = 0; q < 5; q++) __builtin_nontemporal_store(g[s][q], &a.gd[s * sn + (long)q * n + i]);
-    a.ux[i] = uxn;
-    a.uy[i] = uyn;
+    *reinterpret_cast<double2*>(a.u + 2 * i) = make_double2(uxn, uyn);
 }
 
 // FluidSolver::initSolver / initWithVelocity and the species initSolver.
@@ -210,8 +208,7 @@ __global__ void k_init_f64(const LatticeArgs a, const double* u0) {
     }
 #pragma unroll
     for (int q = 0; q < 9; q++) a.fd[(long)q * n + i] = f[q];
-    a.ux[i] = uxn;
-    a.uy[i] = uyn;
+    *reinterpret_cast<double2*>(a.u + 2 * i) = make_double2(uxn, uyn);
     const uint32_t dom = a.tag[i] & DOM_MASK;
     const long sn = 5 * n;
     for (int s = 0; s < NS; s++) {
@@ -237,7 +234,7 @@ static LatticeArgs make_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
     a.fd = (double*)c->f[1 - c->cur];
     a.gs = (const double*)c->g[c->cur];
     a.gd = (double*)c->g[1 - c->cur];
-    a.ux = c->ux; a.uy = c->uy;
+    a.u = c->u;
     a.Fx = c->Fx; a.Fy = c->Fy;
     a.ftile = use_ftile ? c->tact : nullptr;
     a.tag = c->tag;
@@ -293,10 +290,15 @@ static void dispatch_init(lbib_ctx* c, const LatticeArgs& a, const double* u0) {
 }
 
 int launch_init_f64(lbib_ctx* c, const double* u0_dev) {
-    LatticeArgs a = make_args(c, 0, false);
     // init writes the *current* buffers
-    a.fd = (double*)c->f[c->cur];
-    a.gd = (double*)c->g[c->cur];
+    return launch_init_f64_into(c, u0_dev, (double*)c->f[c->cur], (double*)c->g[c->cur], c->u);
+}
+
+int launch_init_f64_into(lbib_ctx* c, const double* u0_dev, double* f, double* g, double* u) {
+    LatticeArgs a = make_args(c, 0, false);
+    a.fd = f;
+    a.gd = g;
+    a.u = u;
     if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_init<0>(c, a, u0_dev);
     else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_init<1>(c, a, u0_dev);
     else dispatch_init<2>(c, a, u0_dev);

@@ -1,22 +1,663 @@
-// lbib_mixed.hip -- mixed-precision lattice kernels (FP32 arithmetic, 16-bit
-// deviation storage).  Placeholder launchers until the kernels land.
-#include "lbib_internal.h"
+// lbib_mixed.hip -- mixed-precision lattice kernels: FP32 arithmetic on
+// populations stored in FP32, or as FP16 / BF16 deviations from the rest
+// state (f_i - w_i for the fluid; species populations are stored as they are,
+// their rest state being C = 0).  Membrane markers, forces and areas stay
+// FP64; the velocity field is kept as FP32 pairs.
+//
+// The physics is that of the FP64 kernels (FluidSolver::collide,
+// FluidSolver.cpp:103-183 with Luo1993ForceModel.cpp:29-70 or
+// GuoZhengShi2002ForceModel.cpp:29-78; CDESolverD2Q5::collide,
+// CDESolverD2Q5.cpp:75-107 and the SIGNAL reaction; the no-flux boundary of
+// BoundarySolverNoFluxD2Q5.cpp:37-113; MassSolverBoxOutlet.cpp:42-72), written
+// in deviation form so that the rest state cancels exactly:
+//   rho0 = 1 + sum(d),  u = (momentum of d [+ F/2]) / rho,
+//   d_eq = w (rho - 1) + w rho (3 e.u + 9/2 (e.u)^2 - 3/2 u^2),
+//   d'   = d - (d - d_eq) / tau + force term.
+// Rounding differs from the reference by construction; the error against
+// the FP64 path is measured, not bounded here.
+//
+// Roofline: HBM bound.  Algorithmic bytes per site = (2*9 + 2*5*ns) * b with
+// b = 4 (FP32) or 2 (FP16/BF16).
+#include "lbib_device_math.h"
 
 namespace lbib {
-int launch_init_mixed(lbib_ctx* c, const double*) {
-    c->err = "mixed-precision storage is not available in this build";
-    return -1;
+
+// lattice weights and velocities in natural order T,E,N,W,S,NE,NW,SW,SE
+__device__ __forceinline__ float wq(int q) { return q == 0 ? 4.0f / 9.0f : (q < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
+__device__ __forceinline__ int exq(int q) {
+    const int e[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+    return e[q];
 }
-int launch_stream_collide_mixed(lbib_ctx* c, uint32_t, bool) {
-    c->err = "mixed-precision storage is not available in this build";
-    return -1;
+__device__ __forceinline__ int eyq(int q) {
+    const int e[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    return e[q];
 }
-int launch_pack(lbib_ctx* c, const double*, int, int) {
-    c->err = "mixed-precision storage is not available in this build";
-    return -1;
+
+struct SpeciesF {
+    int kind;
+    float tauI;
+    float production, decay;
+    uint32_t switchoff;
+    int prod_domain;
+    float source_inside;
+    float krow[LBIB_MAX_SPECIES];
+};
+
+struct MixedArgs {
+    int nx, ny;
+    long n;
+    const void* fs;
+    void* fd;
+    const void* gs;
+    void* gd;
+    float* u;            // velocity pairs
+    const float2* F;     // force pairs (FP32)
+    const uint8_t* ftile;
+    const uint32_t* tag;
+    const double* mtab;
+    int ntab;
+    double* msite;
+    const double* bc_v;
+    const int32_t* bc_desc;
+    const int32_t* bcd_off;
+    const int32_t* bcd_sp;
+    int ntx;
+    float tauI, tau;
+    uint32_t it;
+    int box_outlet;
+    const DevStatus* st;
+    SpeciesF sp[LBIB_MAX_SPECIES];
+};
+
+// Deviation-form BGK with Luo (FM 0) or Guo (FM 1 as written, FM 2 textbook)
+// forcing; d[] in, d[] out, post-collision velocity out.
+template <int FM>
+__device__ __forceinline__ void collide_dev_f32(float d[9], float m, float Fx, float Fy, float tauI, float tau,
+                                                float gx, float gy, float& uxo, float& uyo) {
+    float sd = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; q++) sd += d[q];
+    const float rho0 = 1.0f + sd;
+    float rho = rho0;
+    if (isfinite(m)) rho += m;
+    const float rhoI = 1.0f / rho;
+    float jx = d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW]);
+    float jy = d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE]);
+    if (FM != 0) { jx += 0.5f * Fx; jy += 0.5f * Fy; }
+    const float ux = jx * rhoI, uy = jy * rhoI;
+    const float u2 = 1.5f * (ux * ux + uy * uy);
+    const float G = (FM != 0) ? (1.0f - 0.5f / tau) : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const float w = wq(q);
+        const float ex = (float)exq(q), ey = (float)eyq(q);
+        const float eu = ex * ux + ey * uy;
+        const float deq = w * (rho - 1.0f) + w * rho * (3.0f * eu + 4.5f * eu * eu - u2);
+        float fc;
+        if (FM == 0) {
+            fc = 3.0f * w * rho0 * (ex * Fx + ey * Fy);
+        } else {
+            // w (1 - 1/(2 tau)) [3 (e - u) + 9 (e.u) e] . F with the stored velocity
+            const float eg = ex * gx + ey * gy;
+            float cx = 3.0f * (ex - gx) + 9.0f * eg * ex;
+            float cy = 3.0f * (ey - gy) + 9.0f * eg * ey;
+            if (FM == 1 && q == DNE) cy = 9.0f * gx - 3.0f * gy + 3.0f;   // GuoZhengShi2002ForceModel.cpp NE y-term
+            fc = G * w * (cx * Fx + cy * Fy);
+        }
+        d[q] = d[q] - (d[q] - deq) * tauI + fc;
+    }
+    float px = d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW]);
+    float py = d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE]);
+    if (FM != 0) { px += 0.5f * Fx; py += 0.5f * Fy; }
+    uxo = px * rhoI;
+    uyo = py * rhoI;
 }
-int launch_unpack(lbib_ctx* c, double*, int, int) {
-    c->err = "mixed-precision storage is not available in this build";
-    return -1;
+
+template <int NS>
+__device__ __forceinline__ float reaction_f32(const SpeciesF& p, uint32_t dom, uint32_t it, float C, const float* cj) {
+    if (p.kind == LBIB_SPECIES_SIGNAL) {
+        const bool prod = (p.prod_domain < 0) ? (dom != 0u) : (dom == (uint32_t)p.prod_domain);
+        if (prod && it < p.switchoff) return p.production;
+        return -p.decay * C;
+    }
+    if (p.kind == LBIB_SPECIES_LINEAR) {
+        float r = 0.0f;
+#pragma unroll
+        for (int j = 0; j < NS; j++) r += p.krow[j] * cj[j];
+        if (dom != 0u) r += p.source_inside;
+        return r;
+    }
+    return 0.0f;
 }
+
+template <int NS>
+__device__ __forceinline__ void species_collide_f32(float g[5], float C, float ux, float uy, const SpeciesF& p,
+                                                    uint32_t dom, uint32_t it, const float* cj) {
+    const float w0 = C / 3.0f, w1 = C / 6.0f;
+    float eq[5];
+    eq[DT] = w0;
+    eq[DE] = w1 * (1.0f + 3.0f * ux);
+    eq[DN] = w1 * (1.0f + 3.0f * uy);
+    eq[DW] = w1 * (1.0f - 3.0f * ux);
+    eq[DS] = w1 * (1.0f - 3.0f * uy);
+    float R0 = 0.0f, R1 = 0.0f;
+    if (p.kind != LBIB_SPECIES_PLAIN) {
+        const float R = reaction_f32<NS>(p, dom, it, C, cj);
+        R0 = R / 3.0f;
+        R1 = R / 6.0f;
+    }
+    g[DT] = g[DT] - (g[DT] - eq[DT]) * p.tauI + R0;
+#pragma unroll
+    for (int q = 1; q < 5; q++) g[q] = g[q] - (g[q] - eq[q]) * p.tauI + R1;
+}
+
+// SPT sites per thread, TILE_W apart so that every load stays coalesced; all
+// loads of a thread are issued before its arithmetic (16-bit values carry
+// few bytes per instruction, so the bytes in flight per wavefront matter).
+constexpr int SPT = 2;
+
+template <int NS, int FM, int P>
+__global__ __launch_bounds__(TILE_W) void k_stream_collide_mx(const MixedArgs a) {
+    using C = Codec<P>;
+    using T = typename C::T;
+    if (step_stopped(a.st)) return;
+    const uint32_t wg = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int xb = (int)(wg % gridDim.x) * (TILE_W * SPT) + threadIdx.x;
+    const int y = (int)(wg / gridDim.x);
+    if (xb >= a.nx) return;
+    const long n = a.n;
+    const long r0 = (long)y * a.nx;
+    const long rS = (long)((y == 0) ? a.ny - 1 : y - 1) * a.nx;
+    const long rN = (long)((y == a.ny - 1) ? 0 : y + 1) * a.nx;
+    const T* fs = static_cast<const T*>(a.fs);
+    const T* gsb = static_cast<const T*>(a.gs);
+    const long sn = 5 * n;
+
+    bool live[SPT];
+    int xs[SPT];
+    float d[SPT][9];
+    float g[SPT][NS > 0 ? NS : 1][5];
+    uint32_t tg[SPT];
+    float Fx[SPT], Fy[SPT], m[SPT], pu[SPT], pv[SPT];
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        const int x = xb + k * TILE_W;
+        live[k] = x < a.nx;
+        xs[k] = live[k] ? x : xb;
+        const int xx = xs[k];
+        const int xm = (xx == 0) ? a.nx - 1 : xx - 1;
+        const int xp = (xx == a.nx - 1) ? 0 : xx + 1;
+        const long i = r0 + xx;
+        // pulled deviations d_q = f_q - w_q
+        d[k][DT] = C::dec(fs[i]);
+        d[k][DE] = C::dec(fs[1 * n + r0 + xm]);
+        d[k][DN] = C::dec(fs[2 * n + rS + xx]);
+        d[k][DW] = C::dec(fs[3 * n + r0 + xp]);
+        d[k][DS] = C::dec(fs[4 * n + rN + xx]);
+        d[k][DNE] = C::dec(fs[5 * n + rS + xm]);
+        d[k][DNW] = C::dec(fs[6 * n + rS + xp]);
+        d[k][DSW] = C::dec(fs[7 * n + rN + xp]);
+        d[k][DSE] = C::dec(fs[8 * n + rN + xm]);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const T* gs = gsb + s * sn;
+            g[k][s][DT] = C::dec(gs[i]);
+            g[k][s][DE] = C::dec(gs[1 * n + r0 + xm]);
+            g[k][s][DN] = C::dec(gs[2 * n + rS + xx]);
+            g[k][s][DW] = C::dec(gs[3 * n + r0 + xp]);
+            g[k][s][DS] = C::dec(gs[4 * n + rN + xx]);
+        }
+        tg[k] = a.tag[i];
+        Fx[k] = 0.0f;
+        Fy[k] = 0.0f;
+        if (a.ftile == nullptr || a.ftile[(long)(y >> 4) * a.ntx + (xx >> 4)]) {
+            const float2 F = a.F[i];
+            Fx[k] = F.x;
+            Fy[k] = F.y;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        const long i = r0 + xs[k];
+        const uint32_t dom = tg[k] & DOM_MASK;
+        m[k] = 0.0f;
+        if (a.msite) { m[k] = (float)a.msite[i]; if (live[k]) a.msite[i] = 0.0; }
+        else if (a.mtab) m[k] = (dom < (uint32_t)a.ntab) ? (float)a.mtab[dom] : 0.0f;
+        pu[k] = 0.0f;
+        pv[k] = 0.0f;
+        if (FM != 0 || (NS > 0 && (tg[k] >> BC_SHIFT))) {
+            const float2 up = *reinterpret_cast<const float2*>(a.u + 2 * i);
+            pu[k] = up.x;
+            pv[k] = up.y;
+        }
+        if (!C::dev) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) d[k][q] -= wq(q);
+        }
+    }
+
+    T* fd = static_cast<T*>(a.fd);
+    T* gd = static_cast<T*>(a.gd);
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
+        if (!live[k]) continue;
+        const int x = xs[k];
+        const long i = r0 + x;
+        const uint32_t dom = tg[k] & DOM_MASK;
+        if (NS > 0 && (tg[k] >> BC_SHIFT)) {
+            // no-flux cut links: incoming population from the first listed
+            // species' pre-stream state, swapped into every listed species
+            const int inv[5] = {DT, DW, DS, DE, DN};
+            for (int o = 1; o <= 4; o++) {
+                if (!((tg[k] >> (BC_SHIFT + o - 1)) & 1u)) continue;
+                const int dd = inv[o];
+                const long e = (long)(o - 1) * n + i;
+                const int desc = a.bc_desc[e];
+                const int b0 = a.bcd_off[desc], b1 = a.bcd_off[desc + 1];
+                if (b1 <= b0) continue;
+                const T* gs = gsb + a.bcd_sp[b0] * sn;
+                const float Cp = C::dec(gs[i]) + C::dec(gs[1 * n + i]) + C::dec(gs[2 * n + i]) +
+                                 C::dec(gs[3 * n + i]) + C::dec(gs[4 * n + i]);
+                const float c6 = Cp / 6.0f;
+                const float bvx = (float)a.bc_v[2 * e], bvy = (float)a.bc_v[2 * e + 1];
+                float feq, eq;
+                if (dd == DE) { feq = c6 * (1.0f + 3.0f * bvx); eq = c6 * (1.0f + 3.0f * pu[k]); }
+                else if (dd == DN) { feq = c6 * (1.0f + 3.0f * bvy); eq = c6 * (1.0f + 3.0f * pv[k]); }
+                else if (dd == DW) { feq = c6 * (1.0f - 3.0f * bvx); eq = c6 * (1.0f - 3.0f * pu[k]); }
+                else { feq = c6 * (1.0f - 3.0f * bvy); eq = c6 * (1.0f - 3.0f * pv[k]); }
+                float carry = feq + (C::dec(gs[(long)dd * n + i]) - eq);
+                for (int kk = b0; kk < b1; kk++) {
+                    const int sk = a.bcd_sp[kk];
+#pragma unroll
+                    for (int s = 0; s < NS; s++)
+                        if (s == sk) { const float t = g[k][s][dd]; g[k][s][dd] = carry; carry = t; }
+                }
+            }
+        }
+
+        float uxn, uyn;
+        collide_dev_f32<FM>(d[k], m[k], Fx[k], Fy[k], a.tauI, a.tau, pu[k], pv[k], uxn, uyn);
+
+        if (NS > 0) {
+            float cj[NS > 0 ? NS : 1];
+#pragma unroll
+            for (int s = 0; s < NS; s++) cj[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+#pragma unroll
+            for (int s = 0; s < NS; s++) species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj);
+        }
+
+        if (a.box_outlet && (x == 0 || y == 0)) {
+            const int passes = (x == 0) + (y == 0);
+            for (int p = 0; p < passes; p++) {
+                float sd = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 9; q++) sd += d[k][q];
+                const float sf = 1.0f / (1.0f + sd);
+#pragma unroll
+                for (int q = 0; q < 9; q++) d[k][q] = (wq(q) + d[k][q]) * sf - wq(q);
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+#pragma unroll
+                    for (int q = 0; q < 5; q++) g[k][s][q] *= sf;
+            }
+        }
+
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            __builtin_nontemporal_store(C::enc(C::dev ? d[k][q] : d[k][q] + wq(q)), &fd[(long)q * n + i]);
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) __builtin_nontemporal_store(C::enc(g[k][s][q]), &gd[s * sn + (long)q * n + i]);
+        *reinterpret_cast<float2*>(a.u + 2 * i) = make_float2(uxn, uyn);
+    }
+}
+
+// 16-bit storage, even nx: each lane owns the adjacent sites (x0, x0+1), so
+// every load and store moves 32 bits per lane.  Pulls shifted by one site
+// along x combine the lane's aligned pair with its neighbour lane's (a
+// wavefront shuffle); the lanes at the wavefront's ends (and at the periodic
+// wrap) load the one value from outside the wavefront themselves.
+template <int P>
+struct PairLoad {
+    using T = uint16_t;
+    const T* base;
+    long rowo;          // row offset (population block + row start)
+    int x0, nx, lane;
+    bool lo_edge, hi_edge;
+    __device__ uint32_t aligned() const { return *reinterpret_cast<const uint32_t*>(base + rowo + x0); }
+    // values at x0-1, x0 (pull from the west)
+    __device__ void west(float& a0, float& a1) const {
+        const uint32_t own = aligned();
+        uint32_t prev = (uint32_t)__shfl_up((int)own, 1, 64);
+        const T lo = lo_edge ? base[rowo + (x0 == 0 ? nx - 1 : x0 - 1)] : (T)(prev >> 16);
+        a0 = Codec<P>::dec(lo);
+        a1 = Codec<P>::dec((T)(own & 0xffffu));
+    }
+    // values at x0+1, x0+2 (pull from the east)
+    __device__ void east(float& a0, float& a1) const {
+        const uint32_t own = aligned();
+        uint32_t next = (uint32_t)__shfl_down((int)own, 1, 64);
+        const T hi = hi_edge ? base[rowo + (x0 + 2 >= nx ? x0 + 2 - nx : x0 + 2)] : (T)(next & 0xffffu);
+        a0 = Codec<P>::dec((T)(own >> 16));
+        a1 = Codec<P>::dec(hi);
+    }
+    __device__ void here(float& a0, float& a1) const {
+        const uint32_t own = aligned();
+        a0 = Codec<P>::dec((T)(own & 0xffffu));
+        a1 = Codec<P>::dec((T)(own >> 16));
+    }
+};
+
+template <int NS, int FM, int P>
+__global__ __launch_bounds__(TILE_W) void k_stream_collide_h16(const MixedArgs a) {
+    using C = Codec<P>;
+    using T = uint16_t;
+    if (step_stopped(a.st)) return;
+    const uint32_t wg = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+    const int lane = threadIdx.x & 63;
+    const int x0 = (int)(wg % gridDim.x) * (2 * TILE_W) + 2 * (int)threadIdx.x;
+    const int y = (int)(wg / gridDim.x);
+    // lanes past the row end take part in the shuffles with a valid address
+    const bool live = x0 < a.nx;
+    const int xl = live ? x0 : a.nx - 2;
+    const long n = a.n;
+    const long r0 = (long)y * a.nx;
+    const long rS = (long)((y == 0) ? a.ny - 1 : y - 1) * a.nx;
+    const long rN = (long)((y == a.ny - 1) ? 0 : y + 1) * a.nx;
+    const T* fs = static_cast<const T*>(a.fs);
+    const T* gsb = static_cast<const T*>(a.gs);
+    const long sn = 5 * n;
+    const bool lo_edge = lane == 0 || xl == 0;
+    const bool hi_edge = lane == 63 || xl + 2 >= a.nx;
+    auto pl = [&](const T* base, long rowo) { return PairLoad<P>{base, rowo, xl, a.nx, lane, lo_edge, hi_edge}; };
+
+    float d[2][9];
+    pl(fs, 0 * n + r0).here(d[0][DT], d[1][DT]);
+    pl(fs, 1 * n + r0).west(d[0][DE], d[1][DE]);
+    pl(fs, 2 * n + rS).here(d[0][DN], d[1][DN]);
+    pl(fs, 3 * n + r0).east(d[0][DW], d[1][DW]);
+    pl(fs, 4 * n + rN).here(d[0][DS], d[1][DS]);
+    pl(fs, 5 * n + rS).west(d[0][DNE], d[1][DNE]);
+    pl(fs, 6 * n + rS).east(d[0][DNW], d[1][DNW]);
+    pl(fs, 7 * n + rN).east(d[0][DSW], d[1][DSW]);
+    pl(fs, 8 * n + rN).west(d[0][DSE], d[1][DSE]);
+    float g[2][NS > 0 ? NS : 1][5];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const T* gs = gsb + s * sn;
+        pl(gs, 0 * n + r0).here(g[0][s][DT], g[1][s][DT]);
+        pl(gs, 1 * n + r0).west(g[0][s][DE], g[1][s][DE]);
+        pl(gs, 2 * n + rS).here(g[0][s][DN], g[1][s][DN]);
+        pl(gs, 3 * n + r0).east(g[0][s][DW], g[1][s][DW]);
+        pl(gs, 4 * n + rN).here(g[0][s][DS], g[1][s][DS]);
+    }
+    if (!live) return;
+    const long i0 = r0 + x0;
+    const uint2 tg2 = *reinterpret_cast<const uint2*>(a.tag + i0);
+    const uint32_t tg[2] = {tg2.x, tg2.y};
+    float Fx[2] = {0.0f, 0.0f}, Fy[2] = {0.0f, 0.0f};
+    if (a.ftile == nullptr || a.ftile[(long)(y >> 4) * a.ntx + (x0 >> 4)]) {
+        const float4 F = *reinterpret_cast<const float4*>(a.F + i0);
+        Fx[0] = F.x; Fy[0] = F.y; Fx[1] = F.z; Fy[1] = F.w;
+    }
+    float pu[2] = {0.0f, 0.0f}, pv[2] = {0.0f, 0.0f};
+    if (FM != 0 || (NS > 0 && ((tg[0] | tg[1]) >> BC_SHIFT))) {
+        const float4 up = *reinterpret_cast<const float4*>(a.u + 2 * i0);
+        pu[0] = up.x; pv[0] = up.y; pu[1] = up.z; pv[1] = up.w;
+    }
+    float uo[4];
+    uint32_t outf[9] = {}, outg[NS > 0 ? NS : 1][5] = {};
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int x = x0 + k;
+        const long i = i0 + k;
+        const uint32_t dom = tg[k] & DOM_MASK;
+        float m = 0.0f;
+        if (a.msite) { m = (float)a.msite[i]; a.msite[i] = 0.0; }
+        else if (a.mtab) m = (dom < (uint32_t)a.ntab) ? (float)a.mtab[dom] : 0.0f;
+        if (NS > 0 && (tg[k] >> BC_SHIFT)) {
+            const int inv[5] = {DT, DW, DS, DE, DN};
+            for (int o = 1; o <= 4; o++) {
+                if (!((tg[k] >> (BC_SHIFT + o - 1)) & 1u)) continue;
+                const int dd = inv[o];
+                const long e = (long)(o - 1) * n + i;
+                const int desc = a.bc_desc[e];
+                const int b0 = a.bcd_off[desc], b1 = a.bcd_off[desc + 1];
+                if (b1 <= b0) continue;
+                const T* gs = gsb + a.bcd_sp[b0] * sn;
+                const float Cp = C::dec(gs[i]) + C::dec(gs[1 * n + i]) + C::dec(gs[2 * n + i]) +
+                                 C::dec(gs[3 * n + i]) + C::dec(gs[4 * n + i]);
+                const float c6 = Cp / 6.0f;
+                const float bvx = (float)a.bc_v[2 * e], bvy = (float)a.bc_v[2 * e + 1];
+                float feq, eq;
+                if (dd == DE) { feq = c6 * (1.0f + 3.0f * bvx); eq = c6 * (1.0f + 3.0f * pu[k]); }
+                else if (dd == DN) { feq = c6 * (1.0f + 3.0f * bvy); eq = c6 * (1.0f + 3.0f * pv[k]); }
+                else if (dd == DW) { feq = c6 * (1.0f - 3.0f * bvx); eq = c6 * (1.0f - 3.0f * pu[k]); }
+                else { feq = c6 * (1.0f - 3.0f * bvy); eq = c6 * (1.0f - 3.0f * pv[k]); }
+                float carry = feq + (C::dec(gs[(long)dd * n + i]) - eq);
+                for (int kk = b0; kk < b1; kk++) {
+                    const int sk = a.bcd_sp[kk];
+#pragma unroll
+                    for (int s = 0; s < NS; s++)
+                        if (s == sk) { const float t = g[k][s][dd]; g[k][s][dd] = carry; carry = t; }
+                }
+            }
+        }
+        float uxn, uyn;
+        collide_dev_f32<FM>(d[k], m, Fx[k], Fy[k], a.tauI, a.tau, pu[k], pv[k], uxn, uyn);
+        if (NS > 0) {
+            float cj[NS > 0 ? NS : 1];
+#pragma unroll
+            for (int s = 0; s < NS; s++) cj[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+#pragma unroll
+            for (int s = 0; s < NS; s++) species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj);
+        }
+        if (a.box_outlet && (x == 0 || y == 0)) {
+            const int passes = (x == 0) + (y == 0);
+            for (int p = 0; p < passes; p++) {
+                float sd = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 9; q++) sd += d[k][q];
+                const float sf = 1.0f / (1.0f + sd);
+#pragma unroll
+                for (int q = 0; q < 9; q++) d[k][q] = (wq(q) + d[k][q]) * sf - wq(q);
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+#pragma unroll
+                    for (int q = 0; q < 5; q++) g[k][s][q] *= sf;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) outf[q] |= k ? ((uint32_t)C::enc(d[k][q]) << 16) : (uint32_t)C::enc(d[k][q]);
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++)
+                outg[s][q] |= k ? ((uint32_t)C::enc(g[k][s][q]) << 16) : (uint32_t)C::enc(g[k][s][q]);
+        uo[2 * k] = uxn;
+        uo[2 * k + 1] = uyn;
+    }
+    uint32_t* fd = static_cast<uint32_t*>(a.fd);
+    uint32_t* gd = static_cast<uint32_t*>(a.gd);
+    const long h = i0 >> 1;   // pair index
+#pragma unroll
+    for (int q = 0; q < 9; q++) __builtin_nontemporal_store(outf[q], &fd[((long)q * n >> 1) + h]);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) __builtin_nontemporal_store(outg[s][q], &gd[((s * sn + (long)q * n) >> 1) + h]);
+    *reinterpret_cast<float4*>(a.u + 2 * i0) = make_float4(uo[0], uo[1], uo[2], uo[3]);
+}
+
+// FP64 natural-order fields <-> storage (current buffers); field 0 = fluid.
+template <int P>
+__global__ void k_pack(const double* __restrict__ src, void* dst, long count, int fluid, long n) {
+    using C = Codec<P>;
+    typename C::T* out = static_cast<typename C::T*>(dst);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (long)gridDim.x * blockDim.x) {
+        float v = (float)src[k];
+        if (fluid && C::dev) v = (float)(src[k] - (double)wq((int)(k / n)));
+        out[k] = C::enc(v);
+    }
+}
+template <int P>
+__global__ void k_unpack(const void* src, double* __restrict__ dst, long count, int fluid, long n) {
+    using C = Codec<P>;
+    const typename C::T* in = static_cast<const typename C::T*>(src);
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (long)gridDim.x * blockDim.x) {
+        double v = (double)C::dec(in[k]);
+        if (fluid && C::dev) v += (double)wq((int)(k / n));
+        dst[k] = v;
+    }
+}
+__global__ void k_velocity_to_f32(const double* __restrict__ src, float* dst, long count) {
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (long)gridDim.x * blockDim.x)
+        dst[k] = (float)src[k];
+}
+
+static MixedArgs make_mixed_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
+    MixedArgs a;
+    a.nx = c->nx; a.ny = c->ny; a.n = c->n;
+    a.fs = c->f[c->cur];
+    a.fd = c->f[1 - c->cur];
+    a.gs = c->g[c->cur];
+    a.gd = c->g[1 - c->cur];
+    a.u = reinterpret_cast<float*>(c->u);
+    a.F = reinterpret_cast<const float2*>(c->Fx);
+    a.ftile = use_ftile ? c->tact : nullptr;
+    a.tag = c->tag;
+    a.mtab = c->mtab_live ? c->mtab : nullptr;
+    a.ntab = c->ndom;
+    a.msite = c->msite_live ? c->msite : nullptr;
+    a.bc_v = c->bc_v; a.bc_desc = c->bc_desc; a.bcd_off = c->bcd_off; a.bcd_sp = c->bcd_sp;
+    a.ntx = c->stx;
+    a.tauI = (float)(1.0 / c->cfg.tau_fluid);
+    a.tau = (float)c->cfg.tau_fluid;
+    a.it = it;
+    a.box_outlet = c->cfg.box_outlet;
+    a.st = c->st;
+    for (int s = 0; s < LBIB_MAX_SPECIES; s++) {
+        const SpeciesDev& p = c->sp[s];
+        SpeciesF& q = a.sp[s];
+        q.kind = p.kind;
+        q.tauI = (float)p.tauI;
+        q.production = (float)p.production;
+        q.decay = (float)p.decay;
+        q.switchoff = p.switchoff;
+        q.prod_domain = p.prod_domain;
+        q.source_inside = (float)p.source_inside;
+        for (int j = 0; j < LBIB_MAX_SPECIES; j++) q.krow[j] = (float)p.krow[j];
+    }
+    return a;
+}
+
+template <int FM, int P>
+static void dispatch_mx(lbib_ctx* c, const MixedArgs& a) {
+    dim3 grid((c->nx + TILE_W * SPT - 1) / (TILE_W * SPT), c->ny);
+    switch (c->ns) {
+        case 0: k_stream_collide_mx<0, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 1: k_stream_collide_mx<1, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 2: k_stream_collide_mx<2, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 3: k_stream_collide_mx<3, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        default: k_stream_collide_mx<4, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+    }
+}
+
+template <int FM, int P>
+static void dispatch_h16(lbib_ctx* c, const MixedArgs& a) {
+    dim3 grid((c->nx + 2 * TILE_W - 1) / (2 * TILE_W), c->ny);
+    switch (c->ns) {
+        case 0: k_stream_collide_h16<0, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 1: k_stream_collide_h16<1, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 2: k_stream_collide_h16<2, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 3: k_stream_collide_h16<3, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        default: k_stream_collide_h16<4, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+    }
+}
+
+template <int P>
+static void dispatch_mx_fm(lbib_ctx* c, const MixedArgs& a) {
+    // 16-bit storage with an even row length: the paired-lane kernel
+    if (P != LBIB_FP32 && c->nx % 2 == 0) {
+        if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_h16<0, P>(c, a);
+        else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_h16<1, P>(c, a);
+        else dispatch_h16<2, P>(c, a);
+        return;
+    }
+    if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_mx<0, P>(c, a);
+    else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_mx<1, P>(c, a);
+    else dispatch_mx<2, P>(c, a);
+}
+
+int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile) {
+    const MixedArgs a = make_mixed_args(c, it, use_ftile);
+    if (c->prec == LBIB_FP32) dispatch_mx_fm<LBIB_FP32>(c, a);
+    else if (c->prec == LBIB_FP16_DEV) dispatch_mx_fm<LBIB_FP16_DEV>(c, a);
+    else dispatch_mx_fm<LBIB_BF16_DEV>(c, a);
+    LBIB_HIP(hipGetLastError());
+    c->cur = 1 - c->cur;
+    c->mtab_live = false;
+    c->msite_live = false;
+    return 0;
+}
+
+static int grid_of(long count) { return (int)std::min<long>((count + 255) / 256, 8192); }
+
+int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq) {
+    void* base = field == 0 ? c->f[c->cur] : c->g[c->cur];
+    const size_t sb = c->prec == LBIB_FP32 ? 4 : 2;
+    void* dst = static_cast<char*>(base) + (field == 0 ? 0 : (size_t)(field - 1) * 5 * c->n * sb);
+    const long count = (long)nq * c->n;
+    const int fl = field == 0 ? 1 : 0;
+    if (c->prec == LBIB_FP32) k_pack<LBIB_FP32><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n);
+    else if (c->prec == LBIB_FP16_DEV)
+        k_pack<LBIB_FP16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n);
+    else k_pack<LBIB_BF16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq) {
+    const void* base = field == 0 ? c->f[c->cur] : c->g[c->cur];
+    const size_t sb = c->prec == LBIB_FP32 ? 4 : 2;
+    const void* src = static_cast<const char*>(base) + (field == 0 ? 0 : (size_t)(field - 1) * 5 * c->n * sb);
+    const long count = (long)nq * c->n;
+    const int fl = field == 0 ? 1 : 0;
+    if (c->prec == LBIB_FP32) k_unpack<LBIB_FP32><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n);
+    else if (c->prec == LBIB_FP16_DEV)
+        k_unpack<LBIB_FP16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n);
+    else k_unpack<LBIB_BF16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+// Initialisation: the FP64 initial state (FluidSolver::initSolver /
+// initWithVelocity, species initSolver) into scratch, then packed.
+int launch_init_mixed(lbib_ctx* c, const double* u0_dev) {
+    double *f = nullptr, *g = nullptr, *u = nullptr;
+    const size_t nf = 9 * (size_t)c->n, ng = (size_t)std::max(c->ns, 1) * 5 * c->n;
+    if (hipMalloc((void**)&f, nf * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&g, ng * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&u, 2 * (size_t)c->n * sizeof(double)) != hipSuccess) {
+        if (f) (void)hipFree(f);
+        if (g) (void)hipFree(g);
+        if (u) (void)hipFree(u);
+        c->err = "mixed-precision init: scratch allocation failed";
+        return -1;
+    }
+    int r = launch_init_f64_into(c, u0_dev, f, g, u);
+    if (!r) r = launch_pack(c, f, 0, 9);
+    for (int s = 0; !r && s < c->ns; s++) r = launch_pack(c, g + (size_t)s * 5 * c->n, 1 + s, 5);
+    if (!r) {
+        k_velocity_to_f32<<<grid_of(2 * c->n), 256, 0, c->stream>>>(u, reinterpret_cast<float*>(c->u), 2 * c->n);
+        if (hipGetLastError() != hipSuccess) { c->err = "mixed-precision init: launch failed"; r = -1; }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(f);
+    (void)hipFree(g);
+    (void)hipFree(u);
+    return r;
+}
+
 }  // namespace lbib

@@ -1,0 +1,135 @@
+"""Mixed-precision lattice path (FP32 arithmetic; FP32, FP16 or BF16 storage,
+fluid populations stored as deviations f_i - w_i) against the FP64 oracle.
+
+Bars (absolute errors after 12 steps from a perturbed state with populations
+near w_i, species near 0.1; relative errors for the coupled run after 300
+steps).  They are set from the storage format's unit roundoff u (FP32 2^-24,
+FP16 2^-11, BF16 2^-8) times the magnitude stored: fluid deviations about
+0.05 w, species values about 0.1."""
+import numpy as np
+import pytest
+
+from lbibcell_amd import capi, runner, workloads
+
+pytestmark = pytest.mark.gpu
+
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+PRECS = {"fp32": capi.FP32, "fp16": capi.FP16_DEV, "bf16": capi.BF16_DEV}
+# per-step absolute tolerance on populations (fluid, species) and on velocity
+TOL = {"fp32": (1e-6, 1e-6, 1e-6), "fp16": (2e-5, 2e-4, 2e-5), "bf16": (2e-4, 2e-3, 2e-4)}
+
+
+def _max_abs(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+# (48, 40): partial wavefronts; (600, 9): several workgroups per row;
+# (37, 23): odd rows, served by the one-site-per-lane kernel
+@pytest.mark.parametrize("shape", [(48, 40), (600, 9), (37, 23)])
+@pytest.mark.parametrize("prec", list(PRECS))
+@pytest.mark.parametrize("force_model", [capi.FORCE_LUO, capi.FORCE_GUO])
+def test_mixed_lattice_steps_close_to_oracle(orc, prec, force_model, shape):
+    (nx, ny), tau = shape, 0.83
+    rng = np.random.default_rng(5 + force_model)
+    sig = dict(production=1e-3, decay=1e-5, switchoff=7, init_inside=0.5, prod_domain=1)
+    print(f"case mixed {prec} fm={force_model} shape={shape}", flush=True)
+    ctx = capi.Context(nx, ny, tau, species=[capi.species_spec(capi.SPECIES_SIGNAL, 0.8, **sig)],
+                       force_model=force_model, precision=PRECS[prec], box_outlet=True)
+    o = orc.Oracle(nx, ny, tau, force_model)
+    o.add_species(orc.SP_SIGNAL, 0.8, [sig["production"], sig["decay"], sig["switchoff"], sig["init_inside"],
+                                       sig["prod_domain"]])
+    o.box_outlet(True)
+    dom = np.zeros((ny, nx), dtype=np.uint32)
+    dom[ny // 4: 3 * ny // 4, nx // 3: 2 * nx // 3] = 1
+    ctx.set_domains(dom)
+    o.set_domains(dom)
+    ctx.init_solvers()
+    o.init_solvers()
+    tf, tg, tu = TOL[prec]
+    assert _max_abs(ctx.get_populations(), o.get_f()) <= tf
+    f = W[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny, nx)))
+    g = 0.1 * (1 + 0.05 * rng.standard_normal((5, ny, nx)))
+    ctx.set_populations(f)
+    o.set_f(f)
+    ctx.set_species(0, g)
+    o.set_g(0, g)
+    Fx = 1e-4 * rng.standard_normal((ny, nx))
+    Fy = 1e-4 * rng.standard_normal((ny, nx))
+    ctx.set_force_field(Fx, Fy)
+    o.set_force_field(Fx, Fy)
+    for it in range(12):
+        ctx.step_lattice_only(it)
+        o.set_iteration(it)
+        o.lattice_step(1)
+    ef = _max_abs(ctx.get_populations(), o.get_f())
+    eg = _max_abs(ctx.get_species(0), o.get_g(0))
+    ud, vd = ctx.get_velocity()
+    uo, vo = o.get_velocity()
+    eu = max(_max_abs(ud, uo), _max_abs(vd, vo))
+    print(f"  max abs error: populations {ef:.2e} species {eg:.2e} velocity {eu:.2e}", flush=True)
+    assert ef <= 12 * tf and eg <= 12 * tg and eu <= 12 * tu
+    ctx.close()
+
+
+@pytest.mark.parametrize("prec", list(PRECS))
+def test_mixed_storage_round_trip(prec):
+    nx, ny = 33, 21
+    rng = np.random.default_rng(2)
+    ctx = capi.Context(nx, ny, 1.0, species=[capi.species_spec(capi.SPECIES_PLAIN, 0.8)], precision=PRECS[prec])
+    f = W[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny, nx)))
+    g = 0.1 * (1 + 0.05 * rng.standard_normal((5, ny, nx)))
+    ctx.set_populations(f)
+    ctx.set_species(0, g)
+    uround = {"fp32": 2.0 ** -24, "fp16": 2.0 ** -11, "bf16": 2.0 ** -8}[prec]
+    # fluid: deviation (or value for FP32) rounded once (FP16 subnormals: half
+    # of the 2^-24 spacing); species: value rounded once
+    mag = np.abs(f - W[:, None, None]) if prec != "fp32" else np.abs(f)
+    floor = 2.0 ** -25 if prec == "fp16" else 0.0
+    assert np.all(np.abs(ctx.get_populations() - f) <= 1.001 * uround * mag + floor)
+    assert np.all(np.abs(ctx.get_species(0) - g) <= 1.001 * uround * np.abs(g))
+    ctx.close()
+
+
+def _coupled(prec, steps):
+    nx = ny = 96
+    nodes, conns = workloads.circle_cell(48.0, 48.0, 14.0, 80, np.random.default_rng(8))
+    sim = runner.Simulation(nx, ny, 1.0, precision=prec)
+    sim.add_species("tutorial_01_CDESolverD2Q5_SIGNAL",
+                    capi.species_spec(capi.SPECIES_SIGNAL, 0.8, production=1e-3, decay=1e-5, switchoff=5000,
+                                      init_inside=0.0, prod_domain=1))
+    sim.add_geometry(nodes, conns)
+    sim.set_celltype(1, 2)
+    sim.setup()
+    sim.init_solvers()
+    sim.add_mass_solver("MassSolverBoxOutlet")
+    sim.add_area_regulator(200.0, 1e-5, 0.004, 1.2, freq=1)
+    sim.add_tension(0.01, 10)
+    sim.run(steps)
+    ctx = sim.context()
+    ctx.ns = 1
+    u, v = ctx.get_velocity()
+    out = dict(u=np.hypot(u, v), c=ctx.get_species(0).sum(axis=0), dom=ctx.get_domains(),
+               xy=sim.markers()[1], area=sim.areas())
+    sim.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", list(PRECS))
+def test_mixed_coupled_run_error_against_fp64(prec):
+    """Relative L2 errors of speed, concentration, marker positions and cell
+    area after 300 coupled steps (tension, area regulator, remeshing,
+    coarsening), mixed against the FP64 path."""
+    steps = 300
+    print(f"case coupled {prec}", flush=True)
+    ref = _coupled(capi.FP64, steps)
+    mx = _coupled(PRECS[prec], steps)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    e_u, e_c = rel(mx["u"], ref["u"]), rel(mx["c"], ref["c"])
+    same_topology = mx["xy"].shape == ref["xy"].shape
+    e_x = rel(mx["xy"], ref["xy"]) if same_topology else float("nan")
+    e_a = abs(mx["area"][1] - ref["area"][1]) / ref["area"][1]
+    print(f"  L2 rel error: speed {e_u:.2e} concentration {e_c:.2e} markers {e_x:.2e} area {e_a:.2e}", flush=True)
+    # measured on MI355X: FP32 9e-5 / 9e-6 / 4e-7, FP16 3e-4 / 4e-4 / 4e-7,
+    # BF16 5e-3 / 2e-2 / 7e-6 (speed / concentration / area)
+    bound = {"fp32": 5e-4, "fp16": 3e-3, "bf16": 1e-1}[prec]
+    assert e_u <= bound and e_c <= bound and e_a <= bound
