@@ -526,6 +526,100 @@ void SimulationRunner::setup() {
 
 void SimulationRunner::initSolvers(const double* u0) { check(lbib_init_solvers(ctx_, u0), "initSolvers"); }
 
+namespace {
+// The stored state of the reference is post-collision with localSwap applied
+// (FluidSolver.cpp:259-264, CDESolverD2Q5.cpp:151-154): slot q of a site holds
+// the natural-order value of the opposite direction (T stays).
+constexpr int kSlotOfD2Q9[9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+constexpr int kSlotOfD2Q5[5] = {0, 3, 4, 1, 2};
+
+// std::ostream << double with the default precision and floatfield: %g
+inline void put_g(std::string& out, double v) {
+    char b[32];
+    const int k = std::snprintf(b, sizeof(b), "%g", v);
+    out.append(b, (size_t)k);
+}
+}  // namespace
+
+void SimulationRunner::writeSimulation(const std::string& forceFile, const std::string& geometryFile,
+                                       const std::string& parameterFile, const std::string& solverFile) {
+    syncMirror();
+    forceSolver_.writeForceSolver(forceFile);
+    geo_.writeGeometry(geometryFile);
+    params_.writeGlobalSimulationParameters(parameterFile);
+    std::ofstream out(solverFile);
+    if (!out.is_open()) LBIB_FAIL("Cannot open the output file for the force solver.");
+    const int nx = (int)params_.sizeX, ny = (int)params_.sizeY;
+    const size_t n = (size_t)nx * ny;
+    auto dump = [&](const std::string& name, const std::vector<double>& nat, int nq, const int* slot_of) {
+        out << "#BEGIN " << name << "\n";
+        std::string line;
+        for (int y = 0; y < ny; y++) {
+            for (int x = 0; x < nx; x++) {
+                line.clear();
+                line += std::to_string(x);
+                line += '\t';
+                line += std::to_string(y);
+                const size_t i = (size_t)y * nx + x;
+                for (int q = 0; q < nq; q++) {
+                    line += '\t';
+                    put_g(line, nat[(size_t)slot_of[q] * n + i]);
+                }
+                line += '\n';
+                out << line;
+            }
+        }
+        out << "#END " << name << "\n";
+    };
+    out << "#Solvers (xPos\tyPos\tDistributions...)\n";
+    std::vector<double> f(9 * n);
+    check(lbib_get_populations(ctx_, f.data()), "get_populations");
+    dump("FluidSolver", f, 9, kSlotOfD2Q9);
+    std::vector<double> g(5 * n);
+    for (size_t k = 0; k < speciesNames_.size(); k++) {
+        check(lbib_get_species(ctx_, (int)k, g.data()), "get_species");
+        dump(speciesNames_[k], g, 5, kSlotOfD2Q5);
+    }
+    if (!out.good()) LBIB_FAIL("Cannot open the output file for the force solver.");
+}
+
+void SimulationRunner::initSolvers(const std::string& solverFile) {
+    std::ifstream in(solverFile);
+    if (!in.is_open()) LBIB_FAIL("Cannot find the force file.");
+    const int nx = (int)params_.sizeX, ny = (int)params_.sizeY;
+    const size_t n = (size_t)nx * ny;
+    // start from the current state: sites the file omits keep their values
+    std::vector<double> f(9 * n);
+    check(lbib_get_populations(ctx_, f.data()), "get_populations");
+    std::vector<std::vector<double>> g(speciesNames_.size(), std::vector<double>(5 * n));
+    for (size_t k = 0; k < g.size(); k++) check(lbib_get_species(ctx_, (int)k, g[k].data()), "get_species");
+    std::string line, solverName;
+    while (std::getline(in, line)) {
+        if (line.empty()) continue;
+        std::stringstream ls(line);
+        if (line[0] != '#') {
+            int x, y;
+            ls >> x >> y;
+            if (x < 0 || y < 0 || x >= nx || y >= ny) LBIB_FAIL("The position does not match");
+            const size_t i = (size_t)y * nx + x;
+            if (solverName == "FluidSolver") {
+                for (int q = 0; q < 9; q++) ls >> f[(size_t)kSlotOfD2Q9[q] * n + i];
+            } else {
+                const int k = speciesIndex(solverName);
+                if (k < 0) LBIB_FAIL("The CDE Solver with the name " + solverName + " does not exists.");
+                for (int q = 0; q < 5; q++) ls >> g[k][(size_t)kSlotOfD2Q5[q] * n + i];
+            }
+        } else if (line.find("#BEGIN") != std::string::npos) {
+            std::string temp;
+            ls >> temp >> solverName;
+        } else if (line.find("#END") != std::string::npos) {
+            solverName.clear();
+        }
+    }
+    check(lbib_set_populations(ctx_, f.data()), "set_populations");
+    for (size_t k = 0; k < g.size(); k++) check(lbib_set_species(ctx_, (int)k, g[k].data()), "set_species");
+}
+
 void SimulationRunner::initForceSolver(const std::string& file) { forceSolver_.loadForceFile(file); }
 
 void SimulationRunner::addMassSolver(const std::string& name) {
@@ -880,7 +974,18 @@ void SimulationRunner::run(unsigned steps) {
     }
 }
 
-void SimulationRunner::runSimulation() { run(params_.iterations); }
+// SimulationRunner::runSimulation (SimulationRunner.cpp:69-110): on failure
+// the state is dumped to output/CRASH_*_<t>.txt and the message printed.
+void SimulationRunner::runSimulation() {
+    try {
+        run(params_.iterations);
+    } catch (const std::exception& exp) {
+        const std::string t = std::to_string(params_.currentIteration);
+        writeSimulation("output/CRASH_forceOut_" + t + ".txt", "output/CRASH_geoOut_" + t + ".txt",
+                        "output/CRASH_parameterOut_" + t + ".txt", "output/CRASH_solverOut_" + t + ".txt");
+        std::cout << exp.what() << std::endl;
+    }
+}
 
 std::map<unsigned, double> SimulationRunner::computeAreas() {
     syncMirror();

@@ -163,6 +163,13 @@ class SimulationRunner {
     void setSpeciesSpec(const std::string& name, const lbib_species_spec& spec);
     void setup();                                         // GeometryHandler ctor: cureLattice (:149-170)
     void initSolvers(const double* u0 = nullptr);         // SimulationRunner.cpp:290-320
+    // SimulationRunner::initSolvers(const std::string&) (SimulationRunner.cpp:148-180):
+    // the solver text dump written by writeSimulation
+    void initSolvers(const std::string& solverFile);
+    // SimulationRunner::writeSimulation (SimulationRunner.cpp:112-146): force,
+    // geometry, parameter and solver text files in the reference's formats
+    void writeSimulation(const std::string& forceFile, const std::string& geometryFile,
+                         const std::string& parameterFile, const std::string& solverFile);
     void initForceSolver(const std::string& file);        // :392-394
     void addMassSolver(const std::string& name);          // :401-403
     void addBioSolver(const std::string& name);           // :405-408

@@ -126,6 +126,15 @@ int lbh_add_bio_solver(void* p, const char* name) {
 // Parameterised tutorial BioSolvers: kind 0 tension (prm[0] = force), kind 1
 // area regulator (target, kp, maxsource, target factor).
 int lbh_add_bio_solver_params(void* p, int kind, const double* prm, unsigned freq);
+int lbh_write_simulation(void* p, const char* force, const char* geometry, const char* parameters,
+                         const char* solvers) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->runner->writeSimulation(force, geometry, parameters, solvers); });
+}
+int lbh_init_solvers_file(void* p, const char* solvers) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->runner->initSolvers(std::string(solvers)); });
+}
 int lbh_run(void* p, unsigned steps) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] { h->runner->run(steps); });

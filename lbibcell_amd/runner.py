@@ -33,6 +33,8 @@ _SIG = {
     "lbh_add_bio_solver": (C.c_int, [_P, C.c_char_p]),
     "lbh_add_bio_solver_params": (C.c_int, [_P, C.c_int, _D, C.c_uint]),
     "lbh_run": (C.c_int, [_P, C.c_uint]),
+    "lbh_write_simulation": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
+    "lbh_init_solvers_file": (C.c_int, [_P, C.c_char_p]),
     "lbh_sync": (C.c_int, [_P]),
     "lbh_context": (_P, [_P]),
     "lbh_iteration": (C.c_uint, [_P]),
@@ -137,6 +139,15 @@ class Simulation:
 
     def run(self, steps):
         self._chk(self.L.lbh_run(self.h, steps))
+
+    def write_simulation(self, force, geometry, parameters, solvers):
+        """SimulationRunner::writeSimulation: the reference's four text files."""
+        self._chk(self.L.lbh_write_simulation(self.h, str(force).encode(), str(geometry).encode(),
+                                              str(parameters).encode(), str(solvers).encode()))
+
+    def init_solvers_file(self, solvers):
+        """SimulationRunner::initSolvers(filename): load a solver dump."""
+        self._chk(self.L.lbh_init_solvers_file(self.h, str(solvers).encode()))
 
     def set_run_ahead(self, n):
         """Iterations enqueued between device status reads (1 = every iteration)."""

@@ -46,6 +46,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <random>
@@ -1077,6 +1078,46 @@ void orc_set_g(void* p, int k, const double* in) {
     const size_t n = s->f.size();
     for (size_t i = 0; i < n; i++) for (int q = 0; q < 5; q++) s->g[k][i][q] = in[INV[q] * n + i];
 }
+// SimulationRunner::writeSimulation's solver file (SimulationRunner.cpp:123-143)
+// from the stored slots: FluidSolver::writeSolver (FluidSolver.cpp:200-206) and
+// CDESolverD2Q5::writeSolver (CDESolverD2Q5.cpp:52-58) print "x\ty" and the
+// distributions_ array in index order with the stream's default formatting,
+// row by row over the grid[y][x] of physical nodes.  species_names: one name
+// per line, in the order the species were added.
+int orc_write_solver_dump(void* p, const char* file, const char* species_names) {
+    Sim* s = static_cast<Sim*>(p);
+    std::ofstream out(file);
+    if (!out.is_open()) return -1;
+    std::vector<std::string> names;
+    {
+        std::stringstream ss(species_names ? species_names : "");
+        std::string nm;
+        while (std::getline(ss, nm)) if (!nm.empty()) names.push_back(nm);
+    }
+    out << "#Solvers (xPos\tyPos\tDistributions...)\n";
+    out << "#BEGIN FluidSolver\n";
+    for (int y = 0; y < s->ny; y++)
+        for (int x = 0; x < s->nx; x++) {
+            const auto& d = s->f[(size_t)y * s->nx + x];
+            out << x << '\t' << y;
+            for (double v : d) out << '\t' << v;
+            out << '\n';
+        }
+    out << "#END FluidSolver\n";
+    for (size_t k = 0; k < s->g.size() && k < names.size(); k++) {
+        out << "#BEGIN " << names[k] << "\n";
+        for (int y = 0; y < s->ny; y++)
+            for (int x = 0; x < s->nx; x++) {
+                const auto& d = s->g[k][(size_t)y * s->nx + x];
+                out << x << '\t' << y;
+                for (double v : d) out << '\t' << v;
+                out << '\n';
+            }
+        out << "#END " << names[k] << "\n";
+    }
+    return out.good() ? 0 : -1;
+}
+
 void orc_get_velocity(void* p, double* ux, double* uy) {
     Sim* s = static_cast<Sim*>(p);
     for (size_t i = 0; i < s->f.size(); i++) { ux[i] = s->ux[i]; uy[i] = s->uy[i]; }

@@ -42,6 +42,7 @@ _SIG = {
     "orc_set_domains": (None, [_P, _U32]),
     "orc_lattice_step": (None, [_P, C.c_int]),
     "orc_get_f": (None, [_P, _D]),
+    "orc_write_solver_dump": (C.c_int, [_P, C.c_char_p, C.c_char_p]),
     "orc_set_f": (None, [_P, _D]),
     "orc_get_g": (None, [_P, C.c_int, _D]),
     "orc_set_g": (None, [_P, C.c_int, _D]),
@@ -175,6 +176,12 @@ class Oracle:
 
     def lattice_step(self, steps=1):
         self.L.orc_lattice_step(self.h, steps)
+
+    def write_solver_dump(self, path, species_names):
+        """The reference's solver text dump of the current state."""
+        rc = self.L.orc_write_solver_dump(self.h, str(path).encode(), "\n".join(species_names).encode())
+        if rc != 0:
+            raise RuntimeError(f"cannot write {path}")
 
     def get_f(self):
         f = np.empty((9, self.ny, self.nx))
