@@ -48,7 +48,17 @@ enum lbib_force_model {
 enum lbib_species_kind {
     LBIB_SPECIES_PLAIN = 0,   /* CDESolverD2Q5 (CDESolverD2Q5.cpp:37-228) */
     LBIB_SPECIES_SIGNAL = 1,  /* tutorial_01_CDESolverD2Q5_SIGNAL (..._SIGNAL.cpp:39-250) */
-    LBIB_SPECIES_LINEAR = 2   /* R_s = sum_j K[s][j] C_j + source inside cells */
+    LBIB_SPECIES_LINEAR = 2,  /* R_s = sum_j K[s][j] C_j + source inside cells */
+    /* tutorial_02_CDESolverD2Q5_R / _L (Schnakenberg pair, ..._R.cpp:38-42,
+     * 98-110; ..._L.cpp:38-42, 94-106): inside cells
+     *   R: gamma (a - R + R^2 L),  L: gamma (b - R^2 L),
+     * added as deltaT*react/3 (rest) and /6 (moving).  Spec fields:
+     * production = gamma, source_inside = a (R) or b (L), decay = deltaT,
+     * prod_domain = index of the partner species, init_inside = C inside
+     * cells at init.  The partner's C is read as the reference reads it:
+     * after its collide when it comes earlier in the species order. */
+    LBIB_SPECIES_SCHNAKENBERG_R = 3,
+    LBIB_SPECIES_SCHNAKENBERG_L = 4
 };
 
 typedef struct lbib_species_spec {

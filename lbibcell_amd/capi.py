@@ -17,6 +17,7 @@ MAX_SPECIES = 4
 FP64, FP32, FP16_DEV, BF16_DEV = 0, 1, 2, 3
 FORCE_LUO, FORCE_GUO_AS_WRITTEN, FORCE_GUO = 0, 1, 2
 SPECIES_PLAIN, SPECIES_SIGNAL, SPECIES_LINEAR = 0, 1, 2
+SPECIES_SCHNAKENBERG_R, SPECIES_SCHNAKENBERG_L = 3, 4
 
 ERR_VELOCITY = 0x1
 ERR_EDGE = 0x2

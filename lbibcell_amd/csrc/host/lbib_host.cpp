@@ -456,6 +456,16 @@ SimulationRunner::SimulationRunner(const GlobalSimulationParameters& params, geo
         s.tau = kv.second;
         if (kv.first == "CDESolverD2Q5") {
             s.kind = LBIB_SPECIES_PLAIN;
+        } else if (kv.first == "tutorial_02_CDESolverD2Q5_R" || kv.first == "tutorial_02_CDESolverD2Q5_L") {
+            // …_R.cpp:38-42 / …_L.cpp:38-42: gamma, a = 0.1 or b = 0.9, deltaT;
+            // the partner index is resolved below once every name is known
+            const bool isR = kv.first.back() == 'R';
+            s.kind = isR ? LBIB_SPECIES_SCHNAKENBERG_R : LBIB_SPECIES_SCHNAKENBERG_L;
+            s.production = 100.0;
+            s.source_inside = isR ? 0.1 : 0.9;
+            s.decay = 1.0e-4;
+            s.init_inside = 1.0;
+            s.prod_domain = -1;
         } else {
             // tutorial_01_CDESolverD2Q5_SIGNAL constants (…_SIGNAL.cpp:39-43)
             s.kind = LBIB_SPECIES_SIGNAL;
@@ -467,6 +477,16 @@ SimulationRunner::SimulationRunner(const GlobalSimulationParameters& params, geo
         }
         specs_[kv.first] = s;
     }
+    // getCDESolverSlow(partner) (…_R.cpp:97, …_L.cpp:96) throws when the
+    // partner is not registered; the lookup is done once here
+    for (auto& kv : specs_) {
+        if (kv.second.kind != LBIB_SPECIES_SCHNAKENBERG_R && kv.second.kind != LBIB_SPECIES_SCHNAKENBERG_L) continue;
+        const std::string partner = kv.second.kind == LBIB_SPECIES_SCHNAKENBERG_R ? "tutorial_02_CDESolverD2Q5_L"
+                                                                                  : "tutorial_02_CDESolverD2Q5_R";
+        const int j = speciesIndex(partner);
+        if (j < 0) LBIB_FAIL("The CDE Solver with the name " + partner + " does not exists.");
+        kv.second.prod_domain = j;
+    }
 }
 
 SimulationRunner::~SimulationRunner() {
@@ -475,6 +495,7 @@ SimulationRunner::~SimulationRunner() {
 
 bool SimulationRunner::knownCDESolver(const std::string& name) {
     return name == "CDESolverD2Q5" || name == "tutorial_01_CDESolverD2Q5_SIGNAL" ||
+           name == "tutorial_02_CDESolverD2Q5_R" || name == "tutorial_02_CDESolverD2Q5_L" ||
            name.rfind("SIGNAL", 0) == 0 || name.rfind("LINEAR", 0) == 0;
 }
 
@@ -524,7 +545,30 @@ void SimulationRunner::setup() {
     check(lbib_clear_status(ctx_), "status");
 }
 
-void SimulationRunner::initSolvers(const double* u0) { check(lbib_init_solvers(ctx_, u0), "initSolvers"); }
+void SimulationRunner::initSolvers(const double* u0) {
+    check(lbib_init_solvers(ctx_, u0), "initSolvers");
+    std::vector<uint32_t> dom;
+    std::vector<double> g;
+    const size_t n = (size_t)params_.sizeX * params_.sizeY;
+    std::uniform_real_distribution<> dis(-0.01, 0.01);
+    for (size_t k = 0; k < speciesNames_.size(); k++) {
+        const lbib_species_spec& s = specs_[speciesNames_[k]];
+        if (s.kind != LBIB_SPECIES_SCHNAKENBERG_R && s.kind != LBIB_SPECIES_SCHNAKENBERG_L) continue;
+        if (dom.empty()) {
+            dom.resize(n);
+            check(lbib_get_domains(ctx_, dom.data()), "initSolvers");
+        }
+        // …_R.cpp:44-58: 0 outside cells, init_inside * (1 + U(-0.01, 0.01)) / 5
+        // per distribution inside, sites in (y, x) order
+        g.resize(5 * n);
+        check(lbib_get_species(ctx_, (int)k, g.data()), "initSolvers");
+        for (size_t i = 0; i < n; i++) {
+            if (dom[i] == 0) continue;
+            for (int q = 0; q < 5; q++) g[q * n + i] = s.init_inside * (1.0 + dis(genInit_)) / 5.0;
+        }
+        check(lbib_set_species(ctx_, (int)k, g.data()), "initSolvers");
+    }
+}
 
 namespace {
 // The stored state of the reference is post-collision with localSwap applied

@@ -56,7 +56,8 @@ int lbh_add_species(void* p, const char* name, const lbib_species_spec* spec) {
     return guarded(h, [&] {
         if (h->runner) LBIB_FAIL("species must be added before setup");
         h->params.cdeSolvers[name] = spec->tau;
-        h->specs[name] = *spec;
+        // kind < 0: the registry's constants for that solver name
+        if (spec->kind >= 0) h->specs[name] = *spec;
     });
 }
 int lbh_load_parameters(void* p, const char* file) {

@@ -215,6 +215,20 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
         *out = c;
         return -1;
     }
+    for (int s = 0; s < cfg->nspecies; s++) {
+        const lbib_species_spec& p = cfg->species[s];
+        if (p.kind < LBIB_SPECIES_PLAIN || p.kind > LBIB_SPECIES_SCHNAKENBERG_L || !(p.tau > 0.0)) {
+            c->err = "species " + std::to_string(s) + ": unknown kind or tau <= 0";
+            *out = c;
+            return -1;
+        }
+        const bool sch = p.kind == LBIB_SPECIES_SCHNAKENBERG_R || p.kind == LBIB_SPECIES_SCHNAKENBERG_L;
+        if (sch && (p.prod_domain < 0 || p.prod_domain >= cfg->nspecies || p.prod_domain == s)) {
+            c->err = "species " + std::to_string(s) + ": the Schnakenberg partner index must name another species";
+            *out = c;
+            return -1;
+        }
+    }
     *out = c;
     c->nx = cfg->nx;
     c->ny = cfg->ny;

@@ -167,11 +167,21 @@ __device__ __forceinline__ void fluid_collide(double d[9], const double m, const
 }
 
 // Reaction term (…_SIGNAL.cpp:92-100 for SIGNAL; the linear spec is
-// evaluated on the pre-collision concentrations cj[] of every species).
+// evaluated on the pre-collision concentrations cj[] of every species; the
+// Schnakenberg pair reads its partner's concentration from cseq[], which holds
+// each species' C as the reference's getC() sees it at that moment: after the
+// partner's collide (slot-order sum) if it comes earlier in the species
+// order, before it otherwise).
 template <int NS>
 __device__ __forceinline__ double species_reaction(const SpeciesDev& p, const uint32_t dom,
                                                    const uint32_t it, const double C,
-                                                   const double* cj) {
+                                                   const double* cj, const double* cseq) {
+    if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R || p.kind == LBIB_SPECIES_SCHNAKENBERG_L) {
+        if (dom == 0u) return 0.0;
+        const double Cp = cseq[p.prod_domain];
+        if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R) return p.production * (p.source_inside - C + C * C * Cp);
+        return p.production * (p.source_inside - Cp * Cp * C);
+    }
     if (p.kind == LBIB_SPECIES_SIGNAL) {
         const bool prod = (p.prod_domain < 0) ? (dom != 0u) : (dom == (uint32_t)p.prod_domain);
         if (prod && (it < p.switchoff)) return p.production;
@@ -193,7 +203,7 @@ template <int NS>
 __device__ __forceinline__ void species_collide(double d[5], const double C, const double ux,
                                                 const double uy, const SpeciesDev& p,
                                                 const uint32_t dom, const uint32_t it,
-                                                const double* cj) {
+                                                const double* cj, const double* cseq) {
     const double w0 = C / 3.0, w1 = C / 6.0;
     const double tauI = p.tauI;
     double tmp[5];
@@ -209,8 +219,10 @@ __device__ __forceinline__ void species_collide(double d[5], const double C, con
             d[q] = td - td * tauI + tmp[q] * tauI;
         }
     } else {
-        const double R = species_reaction<NS>(p, dom, it, C, cj);
-        const double R0 = 1.0 * R / 3.0, R1 = 1.0 * R / 6.0;
+        const double R = species_reaction<NS>(p, dom, it, C, cj, cseq);
+        const bool sch = p.kind == LBIB_SPECIES_SCHNAKENBERG_R || p.kind == LBIB_SPECIES_SCHNAKENBERG_L;
+        const double R0 = sch ? p.decay * R / 3.0 : 1.0 * R / 3.0;
+        const double R1 = sch ? p.decay * R / 6.0 : 1.0 * R / 6.0;
         d[DT] = d[DT] - d[DT] * tauI + tmp[DT] * tauI + R0;
 #pragma unroll
         for (int q = 1; q < 5; q++) d[q] = d[q] - d[q] * tauI + tmp[q] * tauI + R1;

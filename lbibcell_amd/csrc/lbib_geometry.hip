@@ -278,7 +278,7 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         for (int q = 0; q < 5; q++) C += d[q];
         double cj[NS > 0 ? NS : 1];
         for (int j = 0; j < NS; j++) cj[j] = (j == k) ? C : cswap(j, i);
-        species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], nd, a.it, cj);
+        species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], nd, a.it, cj, cj);
         T* gk = gb + k * sn;
         for (int q = 0; q < 5; q++) gk[q * a.n + i] = CD::enc(d[q]);
     }

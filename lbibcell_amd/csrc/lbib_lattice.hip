@@ -138,16 +138,21 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
     fluid_collide<FM>(f, m, Fx, Fy, a.tauI, a.tau, pu, pv, uxn, uyn);
 
     if (NS > 0) {
-        double cj[NS > 0 ? NS : 1];
+        double cj[NS > 0 ? NS : 1], cseq[NS > 0 ? NS : 1];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             double c = 0.0;
 #pragma unroll
             for (int q = 0; q < 5; q++) c += g[s][q];
             cj[s] = c;
+            cseq[s] = c;
         }
 #pragma unroll
-        for (int s = 0; s < NS; s++) species_collide<NS>(g[s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj);
+        for (int s = 0; s < NS; s++) {
+            species_collide<NS>(g[s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj, cseq);
+            // later species read this one's C from its post-collision slots
+            cseq[s] = c_swapped(g[s][DT], g[s][DE], g[s][DN], g[s][DW], g[s][DS]);
+        }
     }
 
     if (a.box_outlet && (x == 0 || y == 0)) {
@@ -220,7 +225,7 @@ __global__ void k_init_f64(const LatticeArgs a, const double* u0) {
         } else {
             // CDESolverD2Q5.cpp:37-40
             g[0] = 1.0; g[1] = 0.0; g[2] = 0.0; g[3] = 0.0; g[4] = 0.0;
-            species_collide<NS>(g, 1.0, uxn, uyn, a.sp[s], dom, a.it, nullptr);
+            species_collide<NS>(g, 1.0, uxn, uyn, a.sp[s], dom, a.it, nullptr, nullptr);
         }
 #pragma unroll
         for (int q = 0; q < 5; q++) a.gd[s * sn + (long)q * n + i] = g[q];

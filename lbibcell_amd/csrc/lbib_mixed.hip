@@ -114,7 +114,14 @@ __device__ __forceinline__ void collide_dev_f32(float d[9], float m, float Fx, f
 }
 
 template <int NS>
-__device__ __forceinline__ float reaction_f32(const SpeciesF& p, uint32_t dom, uint32_t it, float C, const float* cj) {
+__device__ __forceinline__ float reaction_f32(const SpeciesF& p, uint32_t dom, uint32_t it, float C, const float* cj,
+                                              const float* cseq) {
+    if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R || p.kind == LBIB_SPECIES_SCHNAKENBERG_L) {
+        if (dom == 0u) return 0.0f;
+        const float Cp = cseq[p.prod_domain];
+        if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R) return p.production * (p.source_inside - C + C * C * Cp);
+        return p.production * (p.source_inside - Cp * Cp * C);
+    }
     if (p.kind == LBIB_SPECIES_SIGNAL) {
         const bool prod = (p.prod_domain < 0) ? (dom != 0u) : (dom == (uint32_t)p.prod_domain);
         if (prod && it < p.switchoff) return p.production;
@@ -132,7 +139,7 @@ __device__ __forceinline__ float reaction_f32(const SpeciesF& p, uint32_t dom, u
 
 template <int NS>
 __device__ __forceinline__ void species_collide_f32(float g[5], float C, float ux, float uy, const SpeciesF& p,
-                                                    uint32_t dom, uint32_t it, const float* cj) {
+                                                    uint32_t dom, uint32_t it, const float* cj, const float* cseq) {
     const float w0 = C / 3.0f, w1 = C / 6.0f;
     float eq[5];
     eq[DT] = w0;
@@ -142,7 +149,8 @@ __device__ __forceinline__ void species_collide_f32(float g[5], float C, float u
     eq[DS] = w1 * (1.0f - 3.0f * uy);
     float R0 = 0.0f, R1 = 0.0f;
     if (p.kind != LBIB_SPECIES_PLAIN) {
-        const float R = reaction_f32<NS>(p, dom, it, C, cj);
+        float R = reaction_f32<NS>(p, dom, it, C, cj, cseq);
+        if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R || p.kind == LBIB_SPECIES_SCHNAKENBERG_L) R *= p.decay;
         R0 = R / 3.0f;
         R1 = R / 6.0f;
     }
@@ -279,11 +287,18 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_mx(const MixedArgs a)
         collide_dev_f32<FM>(d[k], m[k], Fx[k], Fy[k], a.tauI, a.tau, pu[k], pv[k], uxn, uyn);
 
         if (NS > 0) {
-            float cj[NS > 0 ? NS : 1];
+            float cj[NS > 0 ? NS : 1], cseq[NS > 0 ? NS : 1];
 #pragma unroll
-            for (int s = 0; s < NS; s++) cj[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+            for (int s = 0; s < NS; s++) {
+                cj[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+                cseq[s] = cj[s];
+            }
 #pragma unroll
-            for (int s = 0; s < NS; s++) species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj);
+            for (int s = 0; s < NS; s++) {
+                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj, cseq);
+                // a later species sees this one's concentration after its collide
+                cseq[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+            }
         }
 
         if (a.box_outlet && (x == 0 || y == 0)) {
@@ -447,11 +462,18 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_h16(const MixedArgs a
         float uxn, uyn;
         collide_dev_f32<FM>(d[k], m, Fx[k], Fy[k], a.tauI, a.tau, pu[k], pv[k], uxn, uyn);
         if (NS > 0) {
-            float cj[NS > 0 ? NS : 1];
+            float cj[NS > 0 ? NS : 1], cseq[NS > 0 ? NS : 1];
 #pragma unroll
-            for (int s = 0; s < NS; s++) cj[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+            for (int s = 0; s < NS; s++) {
+                cj[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+                cseq[s] = cj[s];
+            }
 #pragma unroll
-            for (int s = 0; s < NS; s++) species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj);
+            for (int s = 0; s < NS; s++) {
+                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj, cseq);
+                // a later species sees this one's concentration after its collide
+                cseq[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
+            }
         }
         if (a.box_outlet && (x == 0 || y == 0)) {
             const int passes = (x == 0) + (y == 0);

@@ -93,7 +93,12 @@ class Simulation:
         if rc != 0:
             raise capi.LbibError(self.L.lbh_last_error(self.h).decode())
 
-    def add_species(self, name, spec):
+    def add_species(self, name, spec=None, tau=1.0):
+        """Register a CDE solver by name (CDESolverFactory names).  Without a
+        spec the registry's constants for that name are used (tau from `tau`)."""
+        if spec is None:
+            spec = capi.species_spec(tau=tau)
+            spec.kind = -1
         self._chk(self.L.lbh_add_species(self.h, name.encode(), C.byref(spec)))
 
     def add_node(self, nid, x, y):

@@ -85,6 +85,8 @@ enum SpeciesKind {
     SP_PLAIN = 0,    // CDESolverD2Q5 (CDESolverD2Q5.cpp:37-228)
     SP_SIGNAL = 1,   // tutorial_01_CDESolverD2Q5_SIGNAL (…_SIGNAL.cpp:39-250)
     SP_LINEAR = 2,   // linear reaction R_s = sum_j K[s][j] C_j (+source by domain)
+    SP_SCHNAKENBERG_R = 3,   // tutorial_02_CDESolverD2Q5_R (…_R.cpp:38-42, 94-140)
+    SP_SCHNAKENBERG_L = 4,   // tutorial_02_CDESolverD2Q5_L (…_L.cpp:38-42, 94-141)
 };
 
 struct Species {
@@ -99,6 +101,9 @@ struct Species {
     // LINEAR parameters: row of the coupling matrix and a source inside cells.
     std::vector<double> krow;  // length = number of species
     double source_inside = 0.0;
+    // Schnakenberg pair: gamma, a (R) or b (L), deltaT, index of the partner
+    double gamma = 100.0, ab = 0.1, deltaT = 1.0e-4;
+    int partner = 0;
     std::string name;
 };
 
@@ -475,6 +480,19 @@ static double reaction(const Sim& s, int k, long i, double C, const double* cj) 
         if (s.dom[i] != 0) r += p.source_inside;
         return r;
     }
+    if (p.kind == SP_SCHNAKENBERG_R || p.kind == SP_SCHNAKENBERG_L) {
+        // …_R.cpp:94-105 / …_L.cpp:94-106: the partner's getC() on its array as
+        // it is stored right now (already collided and swapped at this site
+        // when it comes first in the solver order)
+        const double Cp = cOfSlots(s.g[p.partner][i]);
+        if (s.dom[i] == 0) return 0.0;
+        if (p.kind == SP_SCHNAKENBERG_R) {
+            const double R = C, L = Cp;
+            return p.gamma * (p.ab - R + R * R * L);
+        }
+        const double R = Cp, L = C;
+        return p.gamma * (p.ab - R * R * L);
+    }
     return 0.0;
 }
 // CDESolverD2Q5::collide (CDESolverD2Q5.cpp:75-107) and the SIGNAL variant
@@ -499,7 +517,10 @@ static void speciesCollide(Sim& s, int k, long i, const double* cj) {
         }
     } else {
         const double R = reaction(s, k, i, C, cj);
-        const double R0 = 1.0 * R / 3.0, R1 = 1.0 * R / 6.0;
+        const bool sch = s.sp[k].kind == SP_SCHNAKENBERG_R || s.sp[k].kind == SP_SCHNAKENBERG_L;
+        // …_R.cpp:128-130: deltaT*reactionTerm / 3.0 and / 6.0
+        const double R0 = sch ? s.sp[k].deltaT * R / 3.0 : 1.0 * R / 3.0;
+        const double R1 = sch ? s.sp[k].deltaT * R / 6.0 : 1.0 * R / 6.0;
         d[T] = d[T] - d[T] * tauI + tmp[T] * tauI + R0;
         d[E] = d[E] - d[E] * tauI + tmp[E] * tauI + R1;
         d[N] = d[N] - d[N] * tauI + tmp[N] * tauI + R1;
@@ -510,7 +531,8 @@ static void speciesCollide(Sim& s, int k, long i, const double* cj) {
 }
 static void speciesInit(Sim& s, int k, long i) {
     const Species& p = s.sp[k];
-    if (p.kind != SP_PLAIN) {       // …_SIGNAL.cpp:46-57 (no collide)
+    if (p.kind != SP_PLAIN) {       // …_SIGNAL.cpp:46-57, …_R.cpp:44-58 (no collide;
+                                    // the R/L noise is the caller's)
         const double v = (s.dom[i] == 0) ? 0.0 : p.init_inside / 5.0;
         s.g[k][i] = {v, v, v, v, v};
     } else {                        // CDESolverD2Q5.cpp:37-40
@@ -987,6 +1009,11 @@ int orc_add_species(void* p, int kind, double tau, const double* prm, int nprm) 
         if (nprm >= 1) sp.source_inside = prm[0];
         if (nprm >= 2) sp.init_inside = prm[1];
         for (int j = 2; j < nprm; j++) sp.krow.push_back(prm[j]);
+    }
+    if ((kind == SP_SCHNAKENBERG_R || kind == SP_SCHNAKENBERG_L) && nprm >= 5) {
+        // prm = [gamma, a or b, deltaT, partner index, init_inside]
+        sp.gamma = prm[0]; sp.ab = prm[1]; sp.deltaT = prm[2]; sp.partner = (int)prm[3];
+        sp.init_inside = prm[4];
     }
     s->sp.push_back(sp);
     s->g.emplace_back(s->f.size(), std::array<double, 5>{});

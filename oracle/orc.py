@@ -14,7 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblbib_oracle.so")
 
-SP_PLAIN, SP_SIGNAL, SP_LINEAR = 0, 1, 2
+SP_PLAIN, SP_SIGNAL, SP_LINEAR, SP_SCHNAKENBERG_R, SP_SCHNAKENBERG_L = 0, 1, 2, 3, 4
 _D = C.POINTER(C.c_double)
 _U32 = C.POINTER(C.c_uint32)
 _I32 = C.POINTER(C.c_int32)

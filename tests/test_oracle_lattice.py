@@ -176,3 +176,59 @@ def test_box_outlet_sets_unit_density_on_edges(orc):
     np.testing.assert_allclose(rho[:, 0], 1.0, rtol=1e-14)
     np.testing.assert_allclose(rho[0, :], 1.0, rtol=1e-14)
     assert np.abs(rho[1:, 1:] - 1).max() > 1e-3
+
+
+def _schnakenberg_pair(orc, nx, ny, dom, tau=0.9, init=1.0):
+    # registry order is by name: L (index 0) before R (index 1)
+    o = orc.Oracle(nx, ny, 1.0)
+    o.add_species(orc.SP_SCHNAKENBERG_L, tau, [100.0, 0.9, 1e-4, 1, init])
+    o.add_species(orc.SP_SCHNAKENBERG_R, tau, [100.0, 0.1, 1e-4, 0, init])
+    o.set_domains(dom)
+    o.init_solvers()
+    return o
+
+
+def test_schnakenberg_uniform_cell_follows_the_sequential_recurrence(orc):
+    """tutorial_02_CDESolverD2Q5_{L,R}::reaction (…_L.cpp:94-106, …_R.cpp:94-105)
+    on a lattice that is one cell: the fluid stays at rest, streaming moves
+    nothing, and each step is the explicit map
+        L' = L + dt g (b - R^2 L),  R' = R + dt g (a - R + R^2 L')
+    where R reads L after L's collide (L is collided first at every site).
+    Reading L before its collide (L instead of L') fails this check."""
+    nx, ny = 6, 5
+    o = _schnakenberg_pair(orc, nx, ny, np.ones((ny, nx), dtype=np.uint32))
+    gam, a, b, dt = 100.0, 0.1, 0.9, 1e-4
+    L = R = 1.0
+    for _ in range(40):
+        L = L + dt * gam * (b - R * R * L)
+        R = R + dt * gam * (a - R + R * R * L)
+    o.lattice_step(40)
+    cL, cR = o.get_g(0).sum(axis=0), o.get_g(1).sum(axis=0)
+    np.testing.assert_allclose(cL, L, rtol=1e-13)
+    np.testing.assert_allclose(cR, R, rtol=1e-13)
+    # the Jacobi variant (both read the other's pre-collision value) differs
+    Lj = Rj = 1.0
+    for _ in range(40):
+        Lj, Rj = Lj + dt * gam * (b - Rj * Rj * Lj), Rj + dt * gam * (a - Rj + Rj * Rj * Lj)
+    assert abs(Rj - R) > 1e-6
+
+
+def test_schnakenberg_steady_state_and_no_reaction_outside_cells(orc):
+    # homogeneous steady state R* = a + b, L* = b / (a + b)^2 is a fixed point
+    nx, ny = 8, 6
+    o = _schnakenberg_pair(orc, nx, ny, np.ones((ny, nx), dtype=np.uint32))
+    Rs, Ls = 1.0, 0.9
+    o.set_g(0, np.full((5, ny, nx), Ls / 5.0))
+    o.set_g(1, np.full((5, ny, nx), Rs / 5.0))
+    o.lattice_step(10)
+    np.testing.assert_allclose(o.get_g(0).sum(axis=0), Ls, rtol=1e-14)
+    np.testing.assert_allclose(o.get_g(1).sum(axis=0), Rs, rtol=1e-14)
+    # outside cells alone: pure diffusion conserves each species exactly
+    o = _schnakenberg_pair(orc, nx, ny, np.zeros((ny, nx), dtype=np.uint32))
+    rng = np.random.default_rng(3)
+    gL, gR = rng.uniform(0.1, 0.3, (5, ny, nx)), rng.uniform(0.1, 0.3, (5, ny, nx))
+    o.set_g(0, gL)
+    o.set_g(1, gR)
+    o.lattice_step(6)
+    np.testing.assert_allclose(o.get_g(0).sum(), gL.sum(), rtol=1e-13)
+    np.testing.assert_allclose(o.get_g(1).sum(), gR.sum(), rtol=1e-13)
