@@ -247,6 +247,43 @@ int lbib_forces_clear(lbib_ctx* ctx);
 /* FluidSolver::addMass for every site of each domain: m_site = table[dom]. */
 int lbib_add_mass_by_domain(lbib_ctx* ctx, int ndom, const double* table);
 
+/* ---- batched ensemble -------------------------------------------------------
+ * No single counterpart in the reference: a parameter sweep there is one
+ * SimulationRunner (SimulationRunner.hpp:51-119) per domain, each a periodic
+ * nx x ny lattice (GeometryHandler.cpp:520-589).  Here many such domains share
+ * one context: the lattice becomes an atlas in which domain r owns the
+ * rectangle [x0, x0+w) x [y0, y0+h) and is periodic inside it, with its own
+ * fluid and species relaxation times.  Every per-step launch then serves all
+ * domains at once.  x0 and y0 must be multiples of 16 and no 16x16 tile may
+ * be shared by two rectangles; sites outside every rectangle are never
+ * updated.  Cells of domain r must stay at least 3 sites inside its
+ * rectangle (the delta stencil and the cut links never wrap), and their
+ * domain identifiers must differ from those of every other domain; the
+ * box outlet then acts on each rectangle's x = x0 column and y = y0 row.
+ * nrect = 0 returns to one periodic nx x ny domain.  Call after lbib_create
+ * and before lbib_init_solvers. */
+typedef struct lbib_rect {
+    int32_t x0, y0, w, h;
+    double tau_fluid;
+    double tau_species[LBIB_MAX_SPECIES];
+} lbib_rect;
+int lbib_set_batch_layout(lbib_ctx* ctx, int nrect, const lbib_rect* rects);
+
+/* Lattice launch shape.  variant 0 = one site per lane in a row tile (the
+ * FP64 kernel, and the mixed kernel with sites_per_lane 1, 2 or 4 at
+ * workgroup widths 64, 128 or 256); variant 1 = two adjacent sites per lane
+ * with 32-bit accesses (16-bit storage, even widths only).  lbib_autotune
+ * times every shape that applies on the current state (scratch velocity
+ * output, mass and status untouched: the simulation is not advanced), keeps
+ * the fastest and reports all timings; lbib_set_lattice_shape fixes one. */
+typedef struct lbib_lattice_shape {
+    int32_t variant, workgroup, sites_per_lane;
+    double ms;                  /* average launch time measured by the tuner */
+} lbib_lattice_shape;
+int lbib_autotune(lbib_ctx* ctx, int reps, lbib_lattice_shape* tried, int max_tried, int* n_tried,
+                  lbib_lattice_shape* chosen);
+int lbib_set_lattice_shape(lbib_ctx* ctx, const lbib_lattice_shape* shape);
+
 /* ---- measurement ------------------------------------------------------------
  * No counterpart in the reference (it has no profiling): HIP events recorded
  * on the context's stream around every launch of the fused lattice kernel.

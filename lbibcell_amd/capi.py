@@ -65,6 +65,25 @@ class Config(C.Structure):
     ]
 
 
+class Rect(C.Structure):
+    """lbib_rect: one periodic domain [x0, x0+w) x [y0, y0+h) of a batched atlas."""
+    _fields_ = [
+        ("x0", C.c_int32), ("y0", C.c_int32), ("w", C.c_int32), ("h", C.c_int32),
+        ("tau_fluid", C.c_double),
+        ("tau_species", C.c_double * MAX_SPECIES),
+    ]
+
+
+class LatticeShape(C.Structure):
+    _fields_ = [
+        ("variant", C.c_int32), ("workgroup", C.c_int32), ("sites_per_lane", C.c_int32),
+        ("ms", C.c_double),
+    ]
+
+    def as_tuple(self):
+        return (self.variant, self.workgroup, self.sites_per_lane)
+
+
 class Status(C.Structure):
     _fields_ = [
         ("bits", C.c_uint32),
@@ -135,6 +154,10 @@ SIGNATURES = {
     "lbib_profile_enable": (C.c_int, [_P, C.c_int]),
     "lbib_profile_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _D]),
     "lbib_profile_copy": (C.c_int, [_P, C.c_int64, C.c_int, _D]),
+    "lbib_set_batch_layout": (C.c_int, [_P, C.c_int, C.POINTER(Rect)]),
+    "lbib_autotune": (C.c_int, [_P, C.c_int, C.POINTER(LatticeShape), C.c_int, C.POINTER(C.c_int),
+                                C.POINTER(LatticeShape)]),
+    "lbib_set_lattice_shape": (C.c_int, [_P, C.POINTER(LatticeShape)]),
 }
 
 
@@ -172,6 +195,19 @@ def u32ptr(a: np.ndarray):
 
 class LbibError(RuntimeError):
     pass
+
+
+def make_rects(rects):
+    """(x0, y0, w, h, tau_fluid, taus) tuples -> ctypes array of lbib_rect."""
+    rects = list(rects)
+    arr = (Rect * max(len(rects), 1))()
+    for k, r in enumerate(rects):
+        x0, y0, w, h, tf = r[:5]
+        taus = list(r[5]) if len(r) > 5 else []
+        arr[k].x0, arr[k].y0, arr[k].w, arr[k].h, arr[k].tau_fluid = int(x0), int(y0), int(w), int(h), float(tf)
+        for j in range(MAX_SPECIES):
+            arr[k].tau_species[j] = float(taus[j]) if j < len(taus) else 1.0
+    return arr if rects else (Rect * 0)()
 
 
 def species_spec(kind=SPECIES_PLAIN, tau=1.0, production=1e-4, decay=1e-5, init_inside=1.0,
@@ -318,3 +354,22 @@ class Context:
 
     def device_bytes(self):
         return int(self.lib.lbib_device_bytes(self.h))
+
+    def set_batch_layout(self, rects):
+        """rects: iterable of (x0, y0, w, h, tau_fluid, [tau_species...])."""
+        arr = make_rects(rects)
+        self._chk(self.lib.lbib_set_batch_layout(self.h, len(arr), arr if len(arr) else None))
+
+    def autotune(self, reps=5):
+        """Times every applicable lattice launch shape on the current state and
+        keeps the fastest; returns (chosen, [(variant, wg, spt, ms), ...])."""
+        tried = (LatticeShape * 32)()
+        n = C.c_int()
+        best = LatticeShape()
+        self._chk(self.lib.lbib_autotune(self.h, int(reps), tried, 32, C.byref(n), C.byref(best)))
+        rows = [(t.variant, t.workgroup, t.sites_per_lane, t.ms) for t in tried[: n.value]]
+        return (best.variant, best.workgroup, best.sites_per_lane, best.ms), rows
+
+    def set_lattice_shape(self, variant, workgroup, sites_per_lane=1):
+        s = LatticeShape(variant, workgroup, sites_per_lane, 0.0)
+        self._chk(self.lib.lbib_set_lattice_shape(self.h, C.byref(s)))

@@ -67,16 +67,30 @@ Geometry::Geometry(const std::string& file) {
     else LBIB_FAIL("only .txt geometries are supported by this loader");
 }
 
+void Geometry::setBatchRects(const std::vector<lbib_rect>& rects) {
+    rects_ = rects;
+    rectStreams_.assign(rects.size(), Streams());
+}
+
+Geometry::Streams& Geometry::streamsAt(double x, double y) {
+    for (size_t r = 0; r < rects_.size(); r++) {
+        const lbib_rect& q = rects_[r];
+        if (x >= q.x0 && x < q.x0 + q.w && y >= q.y0 && y < q.y0 + q.h) return rectStreams_[r];
+    }
+    return global_;
+}
+
 void Geometry::perturbatePosition(GeometryNode& n) {
+    std::mt19937& gen = streamsAt(n.x, n.y).node;
     if (std::abs(static_cast<int>(n.x) - n.x) <= kEPSILON) {
         double r = 0.0;
-        while (std::abs(r) < 5.0 * kEPSILON) r = disNode_(genNode_);
+        while (std::abs(r) < 5.0 * kEPSILON) r = disNode_(gen);
         n.x = n.x + r;
         nodePerturbations++;
     }
     if (std::abs(static_cast<int>(n.y) - n.y) <= kEPSILON) {
         double r = 0.0;
-        while (std::abs(r) < 5.0 * kEPSILON) r = disNode_(genNode_);
+        while (std::abs(r) < 5.0 * kEPSILON) r = disNode_(gen);
         n.y = n.y + r;
         nodePerturbations++;
     }
@@ -85,8 +99,8 @@ void Geometry::perturbatePosition(GeometryNode& n) {
     for (int pass = 0; pass < 2; pass++) {
         const double lhs = (pass == 0) ? (n.y - std::floor(n.y)) : (std::ceil(n.y) - n.y);
         if (std::abs(lhs - (n.x - std::floor(n.x))) <= kEPSILON) {
-            const double ry = disNode_(genNode_);
-            const double rx = disNode_(genNode_);
+            const double ry = disNode_(gen);
+            const double rx = disNode_(gen);
             n.x = n.x + rx;
             n.y = n.y + ry;
             nodePerturbations++;
@@ -101,15 +115,16 @@ void Geometry::setPos(GeometryNode& n, double x, double y) {
 }
 
 void Geometry::perturbConnection(Connection& c) {
+    std::mt19937& genc = streamsAt(c.first->x, c.first->y).conn;
     if (std::abs(c.first->x - c.second->x) < kEPSILON) {
         double r = 0.0;
-        while (std::abs(r) < 5.0 * kEPSILON) r = disConn_(genConn_);
+        while (std::abs(r) < 5.0 * kEPSILON) r = disConn_(genc);
         setPos(*c.first, c.first->x + r, c.first->y);
         connectionPerturbations++;
     }
     if (std::abs(c.first->y - c.second->y) < kEPSILON) {
         double r = 0.0;
-        while (std::abs(r) < 5.0 * kEPSILON) r = disConn_(genConn_);
+        while (std::abs(r) < 5.0 * kEPSILON) r = disConn_(genc);
         setPos(*c.first, c.first->x, c.first->y + r);
         connectionPerturbations++;
     }
@@ -535,6 +550,7 @@ void SimulationRunner::setup() {
         ctx_ = nullptr;
         throw Exception(m);
     }
+    if (!rects_.empty()) check(lbib_set_batch_layout(ctx_, (int)rects_.size(), rects_.data()), "batch layout");
     // GeometryHandler::cureLattice: perturb connections, link stencils (implicit
     // on the device), boundary nodes, scanline.
     geo_.perturbConnections();

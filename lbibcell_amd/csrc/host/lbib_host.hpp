@@ -96,12 +96,22 @@ class Geometry {
     void perturbConnection(Connection& c);
     void perturbConnections();                           // GeometryHandler.cpp:326-356
     long nodePerturbations = 0, connectionPerturbations = 0;
+    // Batched ensemble: each rectangle is an independent run, with perturbation
+    // streams of its own seeded as a lone run's; a node draws from the
+    // rectangle that contains it.
+    void setBatchRects(const std::vector<lbib_rect>& rects);
 
  private:
+    struct Streams {
+        std::mt19937 node{1}, conn{1};
+    };
+    Streams& streamsAt(double x, double y);
+    std::vector<lbib_rect> rects_;
+    std::vector<Streams> rectStreams_;
     std::map<unsigned, std::unique_ptr<GeometryNode>> nodes_;
     std::vector<std::shared_ptr<Connection>> connections_;
     std::map<unsigned, unsigned> cellTypes_{{0u, 0u}};
-    std::mt19937 genNode_{1}, genConn_{1};
+    Streams global_;
     std::uniform_real_distribution<> disNode_{-10E-7, 10E-7};
     std::uniform_real_distribution<> disConn_{-10E-7, 10E-7};
 };
@@ -161,6 +171,12 @@ class SimulationRunner {
     // before the context is created, in Parameters::cdeSolvers order.
     static bool knownCDESolver(const std::string& name);
     void setSpeciesSpec(const std::string& name, const lbib_species_spec& spec);
+    // Batched ensemble: the lattice is an atlas of periodic rectangles, one
+    // per independent domain (lbib_set_batch_layout); applied by setup().
+    void setBatchLayout(const std::vector<lbib_rect>& rects) {
+        rects_ = rects;
+        geo_.setBatchRects(rects);
+    }
     void setup();                                         // GeometryHandler ctor: cureLattice (:149-170)
     void initSolvers(const double* u0 = nullptr);         // SimulationRunner.cpp:290-320
     // SimulationRunner::initSolvers(const std::string&) (SimulationRunner.cpp:148-180):
@@ -229,6 +245,7 @@ class SimulationRunner {
     // per distribution inside cells (…_R.cpp:44-58) from an unseeded device;
     // here the stream is seeded so that runs repeat
     std::mt19937 genInit_{1};
+    std::vector<lbib_rect> rects_;
 };
 
 }  // namespace LbmLib

@@ -18,6 +18,7 @@ struct Handle {
     geometry::Geometry geo;
     std::unique_ptr<SimulationRunner> runner;
     std::map<std::string, lbib_species_spec> specs;
+    std::vector<lbib_rect> rects;
     int precision = LBIB_FP64, forceModel = LBIB_FORCE_LUO;
     std::string err;
 };
@@ -60,6 +61,17 @@ int lbh_add_species(void* p, const char* name, const lbib_species_spec* spec) {
         if (spec->kind >= 0) h->specs[name] = *spec;
     });
 }
+int lbh_set_batch_layout(void* p, int n, const lbib_rect* rects) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (h->runner) LBIB_FAIL("the batch layout must be set before setup");
+        // nodes draw their perturbations when added: from their domain's streams
+        if (!h->geo.getGeometryNodes().empty()) LBIB_FAIL("the batch layout must be set before adding nodes");
+        if (n < 0 || (n > 0 && !rects)) LBIB_FAIL("bad batch layout");
+        h->rects.assign(rects, rects + n);
+        h->geo.setBatchRects(h->rects);
+    });
+}
 int lbh_load_parameters(void* p, const char* file) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] { h->params.loadGlobalSimulationParameters(file); });
@@ -98,6 +110,7 @@ int lbh_setup(void* p) {
     return guarded(h, [&] {
         h->runner.reset(new SimulationRunner(h->params, h->geo, h->precision, h->forceModel));
         for (auto& kv : h->specs) h->runner->setSpeciesSpec(kv.first, kv.second);
+        h->runner->setBatchLayout(h->rects);
         h->runner->setup();
     });
 }

@@ -265,6 +265,7 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
         return -1;
     c->stx = (c->nx + 15) / 16;
     c->sty = (c->ny + 15) / 16;
+    c->live_sites = c->n;
     {
         const size_t nt = (size_t)c->stx * c->sty;
         if (dev_alloc(c, &c->tag, n) || dev_alloc(c, &c->tact, nt) || dev_alloc(c, &c->tcnt, nt) ||
@@ -306,6 +307,7 @@ void lbib_destroy(lbib_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
+    dev_free(c->rtile); dev_free(c->rects);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
     dev_free(c->tslot); dev_free(c->tlist);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
@@ -690,7 +692,8 @@ int lbib_profile_read(lbib_ctx* c, double* kernel_ms, int64_t* launches, double*
     }
     if (kernel_ms) *kernel_ms = ms;
     if (launches) *launches = (int64_t)(c->ev_used / 2);
-    if (bytes_per_launch) *bytes_per_launch = (double)(2 * 9 + 2 * 5 * c->ns) * (double)storage_bytes(c->prec) * (double)c->n;
+    if (bytes_per_launch)
+        *bytes_per_launch = (double)(2 * 9 + 2 * 5 * c->ns) * (double)storage_bytes(c->prec) * (double)c->live_sites;
     c->ev_used = 0;
     return 0;
 }
@@ -831,6 +834,102 @@ int lbib_add_mass_by_domain(lbib_ctx* c, int ndom, const double* table) {
     if (upload(c, c->mtab, t.data(), c->ndom * sizeof(double))) return -1;
     c->mtab_live = true;
     LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ------------------------------------------------------------ batched atlas --
+int lbib_set_batch_layout(lbib_ctx* c, int nrect, const lbib_rect* rects) {
+    if (check_ctx(c)) return -1;
+    if (nrect < 0 || (nrect > 0 && !rects)) { c->err = "lbib_set_batch_layout: bad arguments"; return -1; }
+    const long nt = (long)c->stx * c->sty;
+    std::vector<int32_t> owner(nt, -1);
+    std::vector<RectDev> rd(nrect);
+    long live = 0;
+    bool pairs = true;
+    for (int r = 0; r < nrect; r++) {
+        const lbib_rect& q = rects[r];
+        const std::string tag = "lbib_set_batch_layout: rectangle " + std::to_string(r) + ": ";
+        if (q.x0 < 0 || q.y0 < 0 || q.x0 % 16 || q.y0 % 16 || q.w < 4 || q.h < 4 || q.x0 + q.w > c->nx ||
+            q.y0 + q.h > c->ny) {
+            c->err = tag + "origin must be a multiple of 16, sides >= 4, inside the lattice";
+            return -1;
+        }
+        if (!(q.tau_fluid > 0.5)) { c->err = tag + "tau_fluid must exceed 0.5"; return -1; }
+        for (int s = 0; s < c->ns; s++)
+            if (!(q.tau_species[s] > 0.0)) { c->err = tag + "species tau must be positive"; return -1; }
+        for (int ty = q.y0 / 16; ty < (q.y0 + q.h + 15) / 16; ty++)
+            for (int tx = q.x0 / 16; tx < (q.x0 + q.w + 15) / 16; tx++) {
+                int32_t& o = owner[(long)ty * c->stx + tx];
+                if (o >= 0) {
+                    c->err = tag + "shares a 16x16 tile with rectangle " + std::to_string(o);
+                    return -1;
+                }
+                o = r;
+            }
+        RectDev& d = rd[r];
+        std::memset(&d, 0, sizeof(d));
+        d.x0 = q.x0; d.y0 = q.y0; d.w = q.w; d.h = q.h;
+        d.tau = q.tau_fluid;
+        d.tauI = 1.0 / q.tau_fluid;
+        for (int s = 0; s < LBIB_MAX_SPECIES; s++) d.tauIs[s] = s < c->ns ? 1.0 / q.tau_species[s] : 1.0;
+        live += (long)q.w * q.h;
+        if (q.w % 2) pairs = false;
+    }
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    dev_free(c->rtile);
+    dev_free(c->rects);
+    c->nrect = 0;
+    c->live_sites = c->n;
+    c->rect_pairs = true;
+    if (nrect == 0) return 0;
+    if (dev_alloc(c, &c->rtile, nt) || dev_alloc(c, &c->rects, nrect)) return -1;
+    if (upload(c, c->rtile, owner.data(), nt * sizeof(int32_t)) ||
+        upload(c, c->rects, rd.data(), nrect * sizeof(RectDev)))
+        return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    c->nrect = nrect;
+    c->live_sites = live;
+    c->rect_pairs = pairs;
+    return 0;
+}
+
+// ---------------------------------------------------------- launch shapes --
+static bool shape_valid(const lbib_ctx* c, const LatticeShape& s) {
+    const bool wg_ok = s.wg == 64 || s.wg == 128 || s.wg == 256;
+    if (s.variant == 1) return wg_ok && pairs_apply(c);
+    if (s.variant != 0 || !wg_ok) return false;
+    return c->prec == LBIB_FP64 ? s.spt == 1 : (s.spt == 1 || s.spt == 2 || s.spt == 4);
+}
+int lbib_set_lattice_shape(lbib_ctx* c, const lbib_lattice_shape* shape) {
+    if (check_ctx(c) || !shape) return -1;
+    const LatticeShape s{shape->variant, shape->workgroup, shape->sites_per_lane};
+    if (!shape_valid(c, s)) {
+        c->err = "lbib_set_lattice_shape: shape does not apply to this context";
+        return -1;
+    }
+    c->lat_variant = s.variant; c->lat_wg = s.wg; c->lat_spt = s.spt;
+    return 0;
+}
+int lbib_autotune(lbib_ctx* c, int reps, lbib_lattice_shape* tried, int max_tried, int* n_tried,
+                  lbib_lattice_shape* chosen) {
+    if (check_ctx(c)) return -1;
+    if (reps < 1) reps = 1;
+    std::vector<TunedShape> out;
+    if ((c->prec == LBIB_FP64 ? autotune_f64(c, reps, out) : autotune_mixed(c, reps, out)) != 0) return -1;
+    if (out.empty()) { c->err = "lbib_autotune: no shape applies"; return -1; }
+    size_t best = 0;
+    for (size_t k = 1; k < out.size(); k++)
+        if (out[k].ms < out[best].ms) best = k;
+    c->lat_variant = out[best].shape.variant;
+    c->lat_wg = out[best].shape.wg;
+    c->lat_spt = out[best].shape.spt;
+    auto put = [](lbib_lattice_shape* d, const TunedShape& t) {
+        d->variant = t.shape.variant; d->workgroup = t.shape.wg; d->sites_per_lane = t.shape.spt; d->ms = t.ms;
+    };
+    if (tried)
+        for (int k = 0; k < (int)out.size() && k < max_tried; k++) put(&tried[k], out[k]);
+    if (n_tried) *n_tried = (int)out.size();
+    if (chosen) put(chosen, out[best]);
     return 0;
 }
 

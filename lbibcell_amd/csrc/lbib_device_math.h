@@ -198,14 +198,14 @@ __device__ __forceinline__ double species_reaction(const SpeciesDev& p, const ui
 }
 
 // CDESolverD2Q5::collide (CDESolverD2Q5.cpp:75-107) / SIGNAL (…_SIGNAL.cpp:102-135)
-// on natural-order d[5]; C must be d[0]+…+d[4] in index order.
+// on natural-order d[5]; C must be d[0]+…+d[4] in index order.  tauI is the
+// species' 1/tau in the site's domain.
 template <int NS>
 __device__ __forceinline__ void species_collide(double d[5], const double C, const double ux,
-                                                const double uy, const SpeciesDev& p,
+                                                const double uy, const SpeciesDev& p, const double tauI,
                                                 const uint32_t dom, const uint32_t it,
                                                 const double* cj, const double* cseq) {
     const double w0 = C / 3.0, w1 = C / 6.0;
-    const double tauI = p.tauI;
     double tmp[5];
     tmp[DT] = w0;
     tmp[DE] = w1 * (1.0 + ux * 3.0);

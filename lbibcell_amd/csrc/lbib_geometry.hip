@@ -219,6 +219,9 @@ struct ReinitArgs {
     const uint32_t* changed;
     const uint32_t* rmark;
     uint32_t it;
+    int stx;
+    const int32_t* rtile;   // batched atlas (null: one periodic domain)
+    const RectDev* rects;
     SpeciesDev sp[LBIB_MAX_SPECIES];
 };
 
@@ -231,8 +234,13 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
     const int x = (int)(i % a.nx), y = (int)(i / a.nx);
     const int EXd[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
     const int EYd[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+    Frame fr;
+    if (!site_frame(a.rtile, a.rects, a.stx, a.nx, a.ny, x, y, fr)) {
+        fr.x0 = 0; fr.y0 = 0; fr.w = a.nx; fr.h = a.ny; fr.r = -1;   // no domain owns it
+    }
     auto nb = [&](int d) -> long {
-        const int xx = (x + EXd[d] + a.nx) % a.nx, yy = (y + EYd[d] + a.ny) % a.ny;
+        const int xx = fr.x0 + (x - fr.x0 + EXd[d] + fr.w) % fr.w;
+        const int yy = fr.y0 + (y - fr.y0 + EYd[d] + fr.h) % fr.h;
         return (long)yy * a.nx + xx;
     };
     const long sn = 5 * a.n;
@@ -278,7 +286,8 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         for (int q = 0; q < 5; q++) C += d[q];
         double cj[NS > 0 ? NS : 1];
         for (int j = 0; j < NS; j++) cj[j] = (j == k) ? C : cswap(j, i);
-        species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], nd, a.it, cj, cj);
+        const double tI = fr.r < 0 ? a.sp[k].tauI : a.rects[fr.r].tauIs[k];
+        species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], tI, nd, a.it, cj, cj);
         T* gk = gb + k * sn;
         for (int q = 0; q < 5; q++) gk[q * a.n + i] = CD::enc(d[q]);
     }
@@ -501,6 +510,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;
         a.g = c->g[c->cur];
         a.u = c->u; a.tag = c->tag; a.changed = c->changed; a.rmark = c->rmark; a.it = it;
+        a.stx = c->stx; a.rtile = c->nrect ? c->rtile : nullptr; a.rects = c->rects;
         for (int s = 0; s < LBIB_MAX_SPECIES; s++) a.sp[s] = c->sp[s];
         switch (c->ns) {
             case 0: launch_reinit_t<0>(c, a); break;

@@ -76,6 +76,32 @@ __device__ __forceinline__ void step_begin_dev(DevStatus* st, uint32_t it) {
     else st->step_it = it;
 }
 
+// One periodic domain of a batched atlas (lbib_rect on the device).
+struct RectDev {
+    int32_t x0, y0, w, h;
+    double tau, tauI;                        // fluid
+    double tauIs[LBIB_MAX_SPECIES];          // species 1/tau
+};
+
+// The periodic frame of site (x, y): the whole lattice, or in a batched atlas
+// the rectangle owning the site's 16x16 tile.  false: the site belongs to no
+// domain and is never updated.
+struct Frame {
+    int x0, y0, w, h, r;
+};
+__device__ __forceinline__ bool site_frame(const int32_t* rtile, const RectDev* rects, int stx, int nx, int ny,
+                                           int x, int y, Frame& fr) {
+    if (rtile == nullptr) {
+        fr.x0 = 0; fr.y0 = 0; fr.w = nx; fr.h = ny; fr.r = -1;
+        return true;
+    }
+    const int r = rtile[(long)(y >> 4) * stx + (x >> 4)];
+    if (r < 0) return false;
+    const RectDev& R = rects[r];
+    fr.x0 = R.x0; fr.y0 = R.y0; fr.w = R.w; fr.h = R.h; fr.r = r;
+    return x - R.x0 < R.w && y - R.y0 < R.h;
+}
+
 struct SpreadEntry {
     int32_t marker;
     int32_t pad;
@@ -113,6 +139,14 @@ struct lbib_ctx {
     bool msite_live = false;
     uint32_t* tag = nullptr;
     lbib::SpeciesDev sp[LBIB_MAX_SPECIES];
+    // batched atlas (lbib_set_batch_layout): rectangle per 16x16 tile
+    int nrect = 0;
+    int32_t* rtile = nullptr;
+    lbib::RectDev* rects = nullptr;
+    long live_sites = 0;               // sites inside some rectangle (all sites without a layout)
+    bool rect_pairs = true;            // every rectangle width even: paired-lane kernel applies
+    // lattice launch shape (lbib_autotune / lbib_set_lattice_shape)
+    int lat_variant = -1, lat_wg = 256, lat_spt = 2;
     // spread over 16x16 tiles
     int stx = 0, sty = 0;
     uint32_t *tcnt = nullptr, *toff = nullptr, *tslot = nullptr;
@@ -192,6 +226,15 @@ struct lbib_ctx {
 
 namespace lbib {
 // launchers implemented per translation unit
+#define LBIB_HIP(call)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            c->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
+            return -1;                                                         \
+        }                                                                      \
+    } while (0)
+
 int launch_init_f64(lbib_ctx* c, const double* u0_dev);
 // the FP64 initial state into the given natural-order buffers (f[9][n],
 // g[ns][5][n], velocity pairs u[n][2])
@@ -200,6 +243,66 @@ int launch_stream_collide_f64(lbib_ctx* c, uint32_t it, bool use_ftile);
 int launch_copy_probe(lbib_ctx* c, const void* src, void* dst, long nbytes);
 int launch_init_mixed(lbib_ctx* c, const double* u0_dev);
 int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile);
+// Lattice launch shape: variant 0 = wg-wide row tiles with spt sites per lane
+// (FP64: spt 1), variant 1 = paired-lane 16-bit kernel.
+struct LatticeShape {
+    int variant, wg, spt;
+};
+// the paired-lane kernel needs 16-bit storage and even periodic widths
+inline bool pairs_apply(const lbib_ctx* c) {
+    return (c->prec == LBIB_FP16_DEV || c->prec == LBIB_BF16_DEV) && c->nx % 2 == 0 &&
+           (c->nrect == 0 || c->rect_pairs);
+}
+inline LatticeShape lattice_shape(const lbib_ctx* c) {
+    if (c->lat_variant >= 0 && (c->lat_variant == 0 || pairs_apply(c))) return {c->lat_variant, c->lat_wg, c->lat_spt};
+    if (c->prec == LBIB_FP64) return {0, 256, 1};
+    return pairs_apply(c) ? LatticeShape{1, 256, 1} : LatticeShape{0, 256, 2};
+}
+struct TunedShape {
+    LatticeShape shape;
+    double ms;
+};
+// Times every applicable shape of the lattice kernel on the current state
+// (reps launches each after one warm-up) into out.  The launches read the
+// current buffers and write the next ones, a scratch velocity array and no
+// mass or status, so the simulation is not advanced.
+// Lattice-kernel tuner: one scratch velocity array and a clear status stand
+// in for the live ones, everything else is the context's own state.
+struct TuneScratch {
+    void* u = nullptr;
+    DevStatus* st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~TuneScratch() {
+        if (u) (void)hipFree(u);
+        if (st) (void)hipFree(st);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+    int init(lbib_ctx* c, size_t ubytes) {
+        LBIB_HIP(hipMalloc(&u, ubytes));
+        LBIB_HIP(hipMemcpyAsync(u, c->u, ubytes, hipMemcpyDeviceToDevice, c->stream));
+        LBIB_HIP(hipMalloc((void**)&st, sizeof(DevStatus)));
+        LBIB_HIP(hipMemsetAsync(st, 0, sizeof(DevStatus), c->stream));
+        LBIB_HIP(hipEventCreate(&e0));
+        LBIB_HIP(hipEventCreate(&e1));
+        return 0;
+    }
+    template <typename Launch>
+    int time(lbib_ctx* c, int reps, Launch launch, double& ms) {
+        if (launch()) return -1;
+        LBIB_HIP(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; r++)
+            if (launch()) return -1;
+        LBIB_HIP(hipEventRecord(e1, c->stream));
+        LBIB_HIP(hipEventSynchronize(e1));
+        float t = 0.0f;
+        LBIB_HIP(hipEventElapsedTime(&t, e0, e1));
+        ms = (double)t / reps;
+        return 0;
+    }
+};
+int autotune_f64(lbib_ctx* c, int reps, std::vector<TunedShape>& out);
+int autotune_mixed(lbib_ctx* c, int reps, std::vector<TunedShape>& out);
 int launch_force_and_spread(lbib_ctx* c);
 int launch_clear_force(lbib_ctx* c);
 int launch_interp_move(lbib_ctx* c);
@@ -232,11 +335,3 @@ int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq);
 int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq);
 }  // namespace lbib
 
-#define LBIB_HIP(call)                                                         \
-    do {                                                                       \
-        hipError_t e_ = (call);                                                \
-        if (e_ != hipSuccess) {                                                \
-            c->err = std::string(#call) + ": " + hipGetErrorString(e_);        \
-            return -1;                                                         \
-        }                                                                      \
-    } while (0)

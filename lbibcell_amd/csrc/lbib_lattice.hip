@@ -39,23 +39,29 @@ struct LatticeArgs {
     uint32_t it;
     int box_outlet;
     const DevStatus* st;   // the step is skipped while the status is stopped
+    const int32_t* rtile;  // batched atlas: rectangle per 16x16 tile (null: one domain)
+    const RectDev* rects;
     SpeciesDev sp[LBIB_MAX_SPECIES];
 };
 
 template <int NS, int FM>
-__global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs a) {
+__global__ __launch_bounds__(256) void k_stream_collide_f64(const LatticeArgs a) {
     if (step_stopped(a.st)) return;
     const uint32_t wg = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int x = (int)(wg % gridDim.x) * TILE_W + threadIdx.x;
+    const int x = (int)(wg % gridDim.x) * (int)blockDim.x + threadIdx.x;
     const int y = (int)(wg / gridDim.x);
     if (x >= a.nx) return;
+    Frame fr;
+    if (!site_frame(a.rtile, a.rects, a.ntx, a.nx, a.ny, x, y, fr)) return;
     const long n = a.n;
-    const int xm = (x == 0) ? a.nx - 1 : x - 1;
-    const int xp = (x == a.nx - 1) ? 0 : x + 1;
+    const int xm = (x == fr.x0) ? fr.x0 + fr.w - 1 : x - 1;
+    const int xp = (x == fr.x0 + fr.w - 1) ? fr.x0 : x + 1;
     const long r0 = (long)y * a.nx;
-    const long rS = (long)((y == 0) ? a.ny - 1 : y - 1) * a.nx;
-    const long rN = (long)((y == a.ny - 1) ? 0 : y + 1) * a.nx;
+    const long rS = (long)((y == fr.y0) ? fr.y0 + fr.h - 1 : y - 1) * a.nx;
+    const long rN = (long)((y == fr.y0 + fr.h - 1) ? fr.y0 : y + 1) * a.nx;
     const long i = r0 + x;
+    const double tauI = fr.r < 0 ? a.tauI : a.rects[fr.r].tauI;
+    const double tau = fr.r < 0 ? a.tau : a.rects[fr.r].tau;
 
     double f[9];
     f[DT] = a.fs[i];
@@ -135,7 +141,7 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
     }
 
     double uxn, uyn;
-    fluid_collide<FM>(f, m, Fx, Fy, a.tauI, a.tau, pu, pv, uxn, uyn);
+    fluid_collide<FM>(f, m, Fx, Fy, tauI, tau, pu, pv, uxn, uyn);
 
     if (NS > 0) {
         double cj[NS > 0 ? NS : 1], cseq[NS > 0 ? NS : 1];
@@ -149,15 +155,16 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_f64(const LatticeArgs
         }
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            species_collide<NS>(g[s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj, cseq);
+            const double tI = fr.r < 0 ? a.sp[s].tauI : a.rects[fr.r].tauIs[s];
+            species_collide<NS>(g[s], cj[s], uxn, uyn, a.sp[s], tI, dom, a.it, cj, cseq);
             // later species read this one's C from its post-collision slots
             cseq[s] = c_swapped(g[s][DT], g[s][DE], g[s][DN], g[s][DW], g[s][DS]);
         }
     }
 
-    if (a.box_outlet && (x == 0 || y == 0)) {
+    if (a.box_outlet && (x == fr.x0 || y == fr.y0)) {
         // x = 0 column first, then the y = 0 row; (0,0) is rescaled twice.
-        int passes = (x == 0) + (y == 0);
+        int passes = (x == fr.x0) + (y == fr.y0);
         for (int p = 0; p < passes; p++) {
             const double sf = 1.0 / rho_swapped(f);
 #pragma unroll
@@ -185,6 +192,9 @@ __global__ void k_init_f64(const LatticeArgs a, const double* u0) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const long n = a.n;
+    // sites outside every rectangle of a batched atlas take the defaults
+    Frame fr;
+    if (!site_frame(a.rtile, a.rects, a.ntx, a.nx, a.ny, (int)(i % a.nx), (int)(i / a.nx), fr)) fr.r = -1;
     double f[9];
     double uxn, uyn;
     if (u0 == nullptr) {
@@ -192,7 +202,9 @@ __global__ void k_init_f64(const LatticeArgs a, const double* u0) {
 #pragma unroll
         for (int q = 0; q < 9; q++) f[q] = 0.0;
         f[DT] = 1.0;
-        fluid_collide<FM>(f, 0.0, 0.0, 0.0, a.tauI, a.tau, 0.0, 0.0, uxn, uyn);
+        const double tauI = fr.r < 0 ? a.tauI : a.rects[fr.r].tauI;
+        const double tau = fr.r < 0 ? a.tau : a.rects[fr.r].tau;
+        fluid_collide<FM>(f, 0.0, 0.0, 0.0, tauI, tau, 0.0, 0.0, uxn, uyn);
     } else {
         // FluidSolver.cpp:62-101
         const double u = u0[2 * i], v = u0[2 * i + 1];
@@ -225,7 +237,8 @@ __global__ void k_init_f64(const LatticeArgs a, const double* u0) {
         } else {
             // CDESolverD2Q5.cpp:37-40
             g[0] = 1.0; g[1] = 0.0; g[2] = 0.0; g[3] = 0.0; g[4] = 0.0;
-            species_collide<NS>(g, 1.0, uxn, uyn, a.sp[s], dom, a.it, nullptr, nullptr);
+            const double tI = fr.r < 0 ? a.sp[s].tauI : a.rects[fr.r].tauIs[s];
+            species_collide<NS>(g, 1.0, uxn, uyn, a.sp[s], tI, dom, a.it, nullptr, nullptr);
         }
 #pragma unroll
         for (int q = 0; q < 5; q++) a.gd[s * sn + (long)q * n + i] = g[q];
@@ -253,28 +266,35 @@ static LatticeArgs make_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
     a.it = it;
     a.box_outlet = c->cfg.box_outlet;
     a.st = c->st;
+    a.rtile = c->nrect ? c->rtile : nullptr;
+    a.rects = c->rects;
     for (int s = 0; s < LBIB_MAX_SPECIES; s++) a.sp[s] = c->sp[s];
     return a;
 }
 
 template <int FM>
-static void dispatch_sc(lbib_ctx* c, const LatticeArgs& a) {
-    dim3 grid((c->nx + TILE_W - 1) / TILE_W, c->ny);
+static void dispatch_sc(lbib_ctx* c, const LatticeArgs& a, int wg) {
+    dim3 grid((c->nx + wg - 1) / wg, c->ny);
     switch (c->ns) {
-        case 0: k_stream_collide_f64<0, FM><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 1: k_stream_collide_f64<1, FM><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 2: k_stream_collide_f64<2, FM><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 3: k_stream_collide_f64<3, FM><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        default: k_stream_collide_f64<4, FM><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 0: k_stream_collide_f64<0, FM><<<grid, wg, 0, c->stream>>>(a); break;
+        case 1: k_stream_collide_f64<1, FM><<<grid, wg, 0, c->stream>>>(a); break;
+        case 2: k_stream_collide_f64<2, FM><<<grid, wg, 0, c->stream>>>(a); break;
+        case 3: k_stream_collide_f64<3, FM><<<grid, wg, 0, c->stream>>>(a); break;
+        default: k_stream_collide_f64<4, FM><<<grid, wg, 0, c->stream>>>(a); break;
     }
+}
+
+static int launch_f64_shape(lbib_ctx* c, const LatticeArgs& a, int wg) {
+    if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_sc<0>(c, a, wg);
+    else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_sc<1>(c, a, wg);
+    else dispatch_sc<2>(c, a, wg);
+    LBIB_HIP(hipGetLastError());
+    return 0;
 }
 
 int launch_stream_collide_f64(lbib_ctx* c, uint32_t it, bool use_ftile) {
     LatticeArgs a = make_args(c, it, use_ftile);
-    if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_sc<0>(c, a);
-    else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_sc<1>(c, a);
-    else dispatch_sc<2>(c, a);
-    LBIB_HIP(hipGetLastError());
+    if (launch_f64_shape(c, a, lattice_shape(c).wg)) return -1;
     c->cur = 1 - c->cur;
     c->mtab_live = false;
     c->msite_live = false;
@@ -330,6 +350,22 @@ int launch_copy_probe(lbib_ctx* c, const void* src, void* dst, long nbytes) {
     }
     k_copy_probe<<<(unsigned)nb, 256, 0, c->stream>>>((const probe_vec*)src, (probe_vec*)dst, n);
     LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int autotune_f64(lbib_ctx* c, int reps, std::vector<TunedShape>& out) {
+    TuneScratch sc;
+    if (sc.init(c, 2 * (size_t)c->n * sizeof(double))) return -1;
+    LatticeArgs a = make_args(c, 0, true);
+    a.u = (double*)sc.u;
+    a.msite = nullptr;
+    a.mtab = nullptr;
+    a.st = sc.st;
+    for (int wg : {64, 128, 256}) {
+        double ms = 0.0;
+        if (sc.time(c, reps, [&] { return launch_f64_shape(c, a, wg); }, ms)) return -1;
+        out.push_back({{0, wg, 1}, ms});
+    }
     return 0;
 }
 

@@ -66,6 +66,8 @@ struct MixedArgs {
     uint32_t it;
     int box_outlet;
     const DevStatus* st;
+    const int32_t* rtile;   // batched atlas: rectangle per 16x16 tile (null: one domain)
+    const RectDev* rects;
     SpeciesF sp[LBIB_MAX_SPECIES];
 };
 
@@ -139,7 +141,8 @@ __device__ __forceinline__ float reaction_f32(const SpeciesF& p, uint32_t dom, u
 
 template <int NS>
 __device__ __forceinline__ void species_collide_f32(float g[5], float C, float ux, float uy, const SpeciesF& p,
-                                                    uint32_t dom, uint32_t it, const float* cj, const float* cseq) {
+                                                    float tauI, uint32_t dom, uint32_t it, const float* cj,
+                                                    const float* cseq) {
     const float w0 = C / 3.0f, w1 = C / 6.0f;
     float eq[5];
     eq[DT] = w0;
@@ -154,47 +157,49 @@ __device__ __forceinline__ void species_collide_f32(float g[5], float C, float u
         R0 = R / 3.0f;
         R1 = R / 6.0f;
     }
-    g[DT] = g[DT] - (g[DT] - eq[DT]) * p.tauI + R0;
+    g[DT] = g[DT] - (g[DT] - eq[DT]) * tauI + R0;
 #pragma unroll
-    for (int q = 1; q < 5; q++) g[q] = g[q] - (g[q] - eq[q]) * p.tauI + R1;
+    for (int q = 1; q < 5; q++) g[q] = g[q] - (g[q] - eq[q]) * tauI + R1;
 }
 
-// SPT sites per thread, TILE_W apart so that every load stays coalesced; all
-// loads of a thread are issued before its arithmetic (16-bit values carry
-// few bytes per instruction, so the bytes in flight per wavefront matter).
-constexpr int SPT = 2;
-
-template <int NS, int FM, int P>
-__global__ __launch_bounds__(TILE_W) void k_stream_collide_mx(const MixedArgs a) {
+// SPT sites per thread, one workgroup width (blockDim.x) apart so that every
+// load stays coalesced; all loads of a thread are issued before its
+// arithmetic (16-bit values carry few bytes per instruction, so the bytes in
+// flight per wavefront matter).  SPT and the width are the tuned launch shape.
+template <int NS, int FM, int P, int SPT>
+__global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
     using C = Codec<P>;
     using T = typename C::T;
     if (step_stopped(a.st)) return;
+    const int W = (int)blockDim.x;
     const uint32_t wg = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int xb = (int)(wg % gridDim.x) * (TILE_W * SPT) + threadIdx.x;
+    const int xb = (int)(wg % gridDim.x) * (W * SPT) + threadIdx.x;
     const int y = (int)(wg / gridDim.x);
     if (xb >= a.nx) return;
     const long n = a.n;
     const long r0 = (long)y * a.nx;
-    const long rS = (long)((y == 0) ? a.ny - 1 : y - 1) * a.nx;
-    const long rN = (long)((y == a.ny - 1) ? 0 : y + 1) * a.nx;
     const T* fs = static_cast<const T*>(a.fs);
     const T* gsb = static_cast<const T*>(a.gs);
     const long sn = 5 * n;
 
     bool live[SPT];
     int xs[SPT];
+    Frame fr[SPT];
     float d[SPT][9];
     float g[SPT][NS > 0 ? NS : 1][5];
     uint32_t tg[SPT];
     float Fx[SPT], Fy[SPT], m[SPT], pu[SPT], pv[SPT];
 #pragma unroll
     for (int k = 0; k < SPT; k++) {
-        const int x = xb + k * TILE_W;
-        live[k] = x < a.nx;
+        const int x = xb + k * W;
+        live[k] = x < a.nx && site_frame(a.rtile, a.rects, a.ntx, a.nx, a.ny, x, y, fr[k]);
+        if (!live[k]) { fr[k].x0 = 0; fr[k].y0 = 0; fr[k].w = a.nx; fr[k].h = a.ny; fr[k].r = -1; }
         xs[k] = live[k] ? x : xb;
         const int xx = xs[k];
-        const int xm = (xx == 0) ? a.nx - 1 : xx - 1;
-        const int xp = (xx == a.nx - 1) ? 0 : xx + 1;
+        const int xm = (xx == fr[k].x0) ? fr[k].x0 + fr[k].w - 1 : xx - 1;
+        const int xp = (xx == fr[k].x0 + fr[k].w - 1) ? fr[k].x0 : xx + 1;
+        const long rS = (long)((y == fr[k].y0) ? fr[k].y0 + fr[k].h - 1 : y - 1) * a.nx;
+        const long rN = (long)((y == fr[k].y0 + fr[k].h - 1) ? fr[k].y0 : y + 1) * a.nx;
         const long i = r0 + xx;
         // pulled deviations d_q = f_q - w_q
         d[k][DT] = C::dec(fs[i]);
@@ -284,7 +289,9 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_mx(const MixedArgs a)
         }
 
         float uxn, uyn;
-        collide_dev_f32<FM>(d[k], m[k], Fx[k], Fy[k], a.tauI, a.tau, pu[k], pv[k], uxn, uyn);
+        const float tauI = fr[k].r < 0 ? a.tauI : (float)a.rects[fr[k].r].tauI;
+        const float tau = fr[k].r < 0 ? a.tau : (float)a.rects[fr[k].r].tau;
+        collide_dev_f32<FM>(d[k], m[k], Fx[k], Fy[k], tauI, tau, pu[k], pv[k], uxn, uyn);
 
         if (NS > 0) {
             float cj[NS > 0 ? NS : 1], cseq[NS > 0 ? NS : 1];
@@ -295,14 +302,15 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_mx(const MixedArgs a)
             }
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj, cseq);
+                const float tI = fr[k].r < 0 ? a.sp[s].tauI : (float)a.rects[fr[k].r].tauIs[s];
+                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], tI, dom, a.it, cj, cseq);
                 // a later species sees this one's concentration after its collide
                 cseq[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
             }
         }
 
-        if (a.box_outlet && (x == 0 || y == 0)) {
-            const int passes = (x == 0) + (y == 0);
+        if (a.box_outlet && (x == fr[k].x0 || y == fr[k].y0)) {
+            const int passes = (x == fr[k].x0) + (y == fr[k].y0);
             for (int p = 0; p < passes; p++) {
                 float sd = 0.0f;
 #pragma unroll
@@ -338,14 +346,14 @@ struct PairLoad {
     using T = uint16_t;
     const T* base;
     long rowo;          // row offset (population block + row start)
-    int x0, nx, lane;
+    int x0, fx0, fw, lane;   // pair start; periodic frame [fx0, fx0 + fw) along x
     bool lo_edge, hi_edge;
     __device__ uint32_t aligned() const { return *reinterpret_cast<const uint32_t*>(base + rowo + x0); }
     // values at x0-1, x0 (pull from the west)
     __device__ void west(float& a0, float& a1) const {
         const uint32_t own = aligned();
         uint32_t prev = (uint32_t)__shfl_up((int)own, 1, 64);
-        const T lo = lo_edge ? base[rowo + (x0 == 0 ? nx - 1 : x0 - 1)] : (T)(prev >> 16);
+        const T lo = lo_edge ? base[rowo + (x0 == fx0 ? fx0 + fw - 1 : x0 - 1)] : (T)(prev >> 16);
         a0 = Codec<P>::dec(lo);
         a1 = Codec<P>::dec((T)(own & 0xffffu));
     }
@@ -353,7 +361,7 @@ struct PairLoad {
     __device__ void east(float& a0, float& a1) const {
         const uint32_t own = aligned();
         uint32_t next = (uint32_t)__shfl_down((int)own, 1, 64);
-        const T hi = hi_edge ? base[rowo + (x0 + 2 >= nx ? x0 + 2 - nx : x0 + 2)] : (T)(next & 0xffffu);
+        const T hi = hi_edge ? base[rowo + (x0 + 2 >= fx0 + fw ? x0 + 2 - fw : x0 + 2)] : (T)(next & 0xffffu);
         a0 = Codec<P>::dec((T)(own >> 16));
         a1 = Codec<P>::dec(hi);
     }
@@ -365,27 +373,33 @@ struct PairLoad {
 };
 
 template <int NS, int FM, int P>
-__global__ __launch_bounds__(TILE_W) void k_stream_collide_h16(const MixedArgs a) {
+__global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
     using C = Codec<P>;
     using T = uint16_t;
     if (step_stopped(a.st)) return;
+    const int W = (int)blockDim.x;
     const uint32_t wg = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
     const int lane = threadIdx.x & 63;
-    const int x0 = (int)(wg % gridDim.x) * (2 * TILE_W) + 2 * (int)threadIdx.x;
+    const int x0 = (int)(wg % gridDim.x) * (2 * W) + 2 * (int)threadIdx.x;
     const int y = (int)(wg / gridDim.x);
-    // lanes past the row end take part in the shuffles with a valid address
-    const bool live = x0 < a.nx;
-    const int xl = live ? x0 : a.nx - 2;
+    // lanes past the row end or outside every domain take part in the
+    // shuffles with a valid address; both sites of a pair share one 16x16 tile
+    Frame fr;
+    const bool live = x0 < a.nx && site_frame(a.rtile, a.rects, a.ntx, a.nx, a.ny, x0, y, fr);
+    if (!live) { fr.x0 = 0; fr.y0 = 0; fr.w = a.nx; fr.h = a.ny; fr.r = -1; }
+    const int xl = x0 < a.nx ? x0 : a.nx - 2;
     const long n = a.n;
     const long r0 = (long)y * a.nx;
-    const long rS = (long)((y == 0) ? a.ny - 1 : y - 1) * a.nx;
-    const long rN = (long)((y == a.ny - 1) ? 0 : y + 1) * a.nx;
+    const long rS = (long)((y == fr.y0) ? fr.y0 + fr.h - 1 : y - 1) * a.nx;
+    const long rN = (long)((y == fr.y0 + fr.h - 1) ? fr.y0 : y + 1) * a.nx;
     const T* fs = static_cast<const T*>(a.fs);
     const T* gsb = static_cast<const T*>(a.gs);
     const long sn = 5 * n;
-    const bool lo_edge = lane == 0 || xl == 0;
-    const bool hi_edge = lane == 63 || xl + 2 >= a.nx;
-    auto pl = [&](const T* base, long rowo) { return PairLoad<P>{base, rowo, xl, a.nx, lane, lo_edge, hi_edge}; };
+    const bool lo_edge = lane == 0 || xl == fr.x0;
+    const bool hi_edge = lane == 63 || xl + 2 >= fr.x0 + fr.w;
+    auto pl = [&](const T* base, long rowo) {
+        return PairLoad<P>{base, rowo, xl, fr.x0, fr.w, lane, lo_edge, hi_edge};
+    };
 
     float d[2][9];
     pl(fs, 0 * n + r0).here(d[0][DT], d[1][DT]);
@@ -460,7 +474,9 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_h16(const MixedArgs a
             }
         }
         float uxn, uyn;
-        collide_dev_f32<FM>(d[k], m, Fx[k], Fy[k], a.tauI, a.tau, pu[k], pv[k], uxn, uyn);
+        const float tauI = fr.r < 0 ? a.tauI : (float)a.rects[fr.r].tauI;
+        const float tau = fr.r < 0 ? a.tau : (float)a.rects[fr.r].tau;
+        collide_dev_f32<FM>(d[k], m, Fx[k], Fy[k], tauI, tau, pu[k], pv[k], uxn, uyn);
         if (NS > 0) {
             float cj[NS > 0 ? NS : 1], cseq[NS > 0 ? NS : 1];
 #pragma unroll
@@ -470,13 +486,14 @@ __global__ __launch_bounds__(TILE_W) void k_stream_collide_h16(const MixedArgs a
             }
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], dom, a.it, cj, cseq);
+                const float tI = fr.r < 0 ? a.sp[s].tauI : (float)a.rects[fr.r].tauIs[s];
+                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], tI, dom, a.it, cj, cseq);
                 // a later species sees this one's concentration after its collide
                 cseq[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
             }
         }
-        if (a.box_outlet && (x == 0 || y == 0)) {
-            const int passes = (x == 0) + (y == 0);
+        if (a.box_outlet && (x == fr.x0 || y == fr.y0)) {
+            const int passes = (x == fr.x0) + (y == fr.y0);
             for (int p = 0; p < passes; p++) {
                 float sd = 0.0f;
 #pragma unroll
@@ -559,6 +576,8 @@ static MixedArgs make_mixed_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
     a.it = it;
     a.box_outlet = c->cfg.box_outlet;
     a.st = c->st;
+    a.rtile = c->nrect ? c->rtile : nullptr;
+    a.rects = c->rects;
     for (int s = 0; s < LBIB_MAX_SPECIES; s++) {
         const SpeciesDev& p = c->sp[s];
         SpeciesF& q = a.sp[s];
@@ -574,50 +593,53 @@ static MixedArgs make_mixed_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
     return a;
 }
 
-template <int FM, int P>
-static void dispatch_mx(lbib_ctx* c, const MixedArgs& a) {
-    dim3 grid((c->nx + TILE_W * SPT - 1) / (TILE_W * SPT), c->ny);
-    switch (c->ns) {
-        case 0: k_stream_collide_mx<0, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 1: k_stream_collide_mx<1, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 2: k_stream_collide_mx<2, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 3: k_stream_collide_mx<3, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        default: k_stream_collide_mx<4, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+template <int NS, int FM, int P>
+static void launch_mx_spt(lbib_ctx* c, const MixedArgs& a, int wg, int spt) {
+    dim3 grid((c->nx + wg * spt - 1) / (wg * spt), c->ny);
+    if (spt == 1) k_stream_collide_mx<NS, FM, P, 1><<<grid, wg, 0, c->stream>>>(a);
+    else if (spt == 2) k_stream_collide_mx<NS, FM, P, 2><<<grid, wg, 0, c->stream>>>(a);
+    else k_stream_collide_mx<NS, FM, P, 4><<<grid, wg, 0, c->stream>>>(a);
+}
+
+template <int NS, int FM, int P>
+static void launch_mx_shape(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
+    if (sh.variant == 1) {
+        dim3 grid((c->nx + 2 * sh.wg - 1) / (2 * sh.wg), c->ny);
+        k_stream_collide_h16<NS, FM, P><<<grid, sh.wg, 0, c->stream>>>(a);
+    } else {
+        launch_mx_spt<NS, FM, P>(c, a, sh.wg, sh.spt);
     }
 }
 
 template <int FM, int P>
-static void dispatch_h16(lbib_ctx* c, const MixedArgs& a) {
-    dim3 grid((c->nx + 2 * TILE_W - 1) / (2 * TILE_W), c->ny);
+static void dispatch_mx(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
     switch (c->ns) {
-        case 0: k_stream_collide_h16<0, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 1: k_stream_collide_h16<1, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 2: k_stream_collide_h16<2, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        case 3: k_stream_collide_h16<3, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
-        default: k_stream_collide_h16<4, FM, P><<<grid, TILE_W, 0, c->stream>>>(a); break;
+        case 0: launch_mx_shape<0, FM, P>(c, a, sh); break;
+        case 1: launch_mx_shape<1, FM, P>(c, a, sh); break;
+        case 2: launch_mx_shape<2, FM, P>(c, a, sh); break;
+        case 3: launch_mx_shape<3, FM, P>(c, a, sh); break;
+        default: launch_mx_shape<4, FM, P>(c, a, sh); break;
     }
 }
 
 template <int P>
-static void dispatch_mx_fm(lbib_ctx* c, const MixedArgs& a) {
-    // 16-bit storage with an even row length: the paired-lane kernel
-    if (P != LBIB_FP32 && c->nx % 2 == 0) {
-        if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_h16<0, P>(c, a);
-        else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_h16<1, P>(c, a);
-        else dispatch_h16<2, P>(c, a);
-        return;
-    }
-    if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_mx<0, P>(c, a);
-    else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_mx<1, P>(c, a);
-    else dispatch_mx<2, P>(c, a);
+static void dispatch_mx_fm(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
+    if (c->cfg.force_model == LBIB_FORCE_LUO) dispatch_mx<0, P>(c, a, sh);
+    else if (c->cfg.force_model == LBIB_FORCE_GUO_AS_WRITTEN) dispatch_mx<1, P>(c, a, sh);
+    else dispatch_mx<2, P>(c, a, sh);
+}
+
+static int launch_mixed_shape(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
+    if (c->prec == LBIB_FP32) dispatch_mx_fm<LBIB_FP32>(c, a, sh);
+    else if (c->prec == LBIB_FP16_DEV) dispatch_mx_fm<LBIB_FP16_DEV>(c, a, sh);
+    else dispatch_mx_fm<LBIB_BF16_DEV>(c, a, sh);
+    LBIB_HIP(hipGetLastError());
+    return 0;
 }
 
 int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile) {
     const MixedArgs a = make_mixed_args(c, it, use_ftile);
-    if (c->prec == LBIB_FP32) dispatch_mx_fm<LBIB_FP32>(c, a);
-    else if (c->prec == LBIB_FP16_DEV) dispatch_mx_fm<LBIB_FP16_DEV>(c, a);
-    else dispatch_mx_fm<LBIB_BF16_DEV>(c, a);
-    LBIB_HIP(hipGetLastError());
+    if (launch_mixed_shape(c, a, lattice_shape(c))) return -1;
     c->cur = 1 - c->cur;
     c->mtab_live = false;
     c->msite_live = false;
@@ -680,6 +702,27 @@ int launch_init_mixed(lbib_ctx* c, const double* u0_dev) {
     (void)hipFree(g);
     (void)hipFree(u);
     return r;
+}
+
+int autotune_mixed(lbib_ctx* c, int reps, std::vector<TunedShape>& out) {
+    TuneScratch sc;
+    if (sc.init(c, 2 * (size_t)c->n * sizeof(float))) return -1;
+    MixedArgs a = make_mixed_args(c, 0, true);
+    a.u = (float*)sc.u;
+    a.msite = nullptr;
+    a.mtab = nullptr;
+    a.st = sc.st;
+    std::vector<LatticeShape> cand;
+    for (int wg : {64, 128, 256})
+        for (int spt : {1, 2, 4}) cand.push_back({0, wg, spt});
+    if (pairs_apply(c))
+        for (int wg : {64, 128, 256}) cand.push_back({1, wg, 1});
+    for (const LatticeShape& sh : cand) {
+        double ms = 0.0;
+        if (sc.time(c, reps, [&] { return launch_mixed_shape(c, a, sh); }, ms)) return -1;
+        out.push_back({sh, ms});
+    }
+    return 0;
 }
 
 }  // namespace lbib

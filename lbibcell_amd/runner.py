@@ -20,6 +20,7 @@ _SIG = {
     "lbh_destroy": (None, [_P]),
     "lbh_last_error": (C.c_char_p, [_P]),
     "lbh_add_species": (C.c_int, [_P, C.c_char_p, C.POINTER(capi.SpeciesSpec)]),
+    "lbh_set_batch_layout": (C.c_int, [_P, C.c_int, C.POINTER(capi.Rect)]),
     "lbh_load_parameters": (C.c_int, [_P, C.c_char_p]),
     "lbh_load_geometry_txt": (C.c_int, [_P, C.c_char_p]),
     "lbh_add_node": (C.c_int, [_P, C.c_uint, C.c_double, C.c_double]),
@@ -100,6 +101,13 @@ class Simulation:
             spec = capi.species_spec(tau=tau)
             spec.kind = -1
         self._chk(self.L.lbh_add_species(self.h, name.encode(), C.byref(spec)))
+
+    def set_batch_layout(self, rects):
+        """Batched ensemble: rects = [(x0, y0, w, h, tau_fluid, [tau_species...]), ...],
+        one periodic domain each (x0, y0 multiples of 16); before any node is
+        added (each domain draws its own position perturbations)."""
+        arr = capi.make_rects(rects)
+        self._chk(self.L.lbh_set_batch_layout(self.h, len(arr), arr if len(arr) else None))
 
     def add_node(self, nid, x, y):
         self._chk(self.L.lbh_add_node(self.h, nid, x, y))

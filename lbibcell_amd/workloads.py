@@ -69,3 +69,84 @@ def grid_cells(nx, ny, cols, rows, radius, n, seed=1):
             nid += n
             dom += 1
     return nodes, conns
+
+
+def _ceil16(v):
+    return (int(v) + 15) // 16 * 16
+
+
+def shelf_pack(sides, width):
+    """Origins (x0, y0) for square domains of the given sides in an atlas of
+    the given width: tallest first, left to right on shelves, every origin on
+    the 16-site tile grid.  Returns (origins in input order, atlas height)."""
+    order = sorted(range(len(sides)), key=lambda k: -sides[k])
+    origins = [None] * len(sides)
+    x = y = shelf_h = 0
+    for k in order:
+        w = _ceil16(sides[k])
+        if w > width:
+            raise ValueError("domain wider than the atlas")
+        if x + w > width:
+            x, y, shelf_h = 0, y + shelf_h, 0
+        origins[k] = (x, y)
+        x += w
+        shelf_h = max(shelf_h, _ceil16(sides[k]))
+    return origins, y + shelf_h
+
+
+def place_cells(side, count, rmin, rmax, rng, margin=6.0, gap=3.0, tries=400):
+    """Up to `count` non-overlapping circles (cx, cy, r) inside a side x side
+    square, at least `margin` from its edges, by rejection sampling."""
+    cells = []
+    for _ in range(tries):
+        if len(cells) >= count:
+            break
+        r = rng.uniform(rmin, rmax)
+        lo, hi = margin + r, side - margin - r
+        if hi <= lo:
+            continue
+        cx, cy = rng.uniform(lo, hi), rng.uniform(lo, hi)
+        if all(math.hypot(cx - x, cy - y) > r + q + gap for x, y, q in cells):
+            cells.append((cx, cy, r))
+    if not cells:
+        # the smallest cell always fits a side >= 2 (margin + rmin)
+        r = rmin
+        cells.append((side / 2.0, side / 2.0, min(r, side / 2.0 - margin)))
+    return cells
+
+
+def batched_ensemble(ndom=256, lmin=64, lmax=512, cells=(1, 16), radius=(8.0, 30.0), tau_f=(0.6, 1.5),
+                     tau_c=(0.55, 1.2), nspecies=1, width=4096, seg=1.0, seed=1):
+    """Parameter-sweep ensemble: ndom independent periodic square domains with
+    sides drawn log-uniform in [lmin, lmax] (rounded to even sides), each
+    holding 1-16 cells of radius U(radius) placed by rejection, tau_f and
+    tau_c uniform in the given ranges; markers about `seg` lattice units
+    apart.  Domains are shelf-packed into a `width`-wide atlas.
+
+    Returns dict(nx, ny, rects=[(x0, y0, L, L, tau_f, [tau_c]*nspecies)],
+    nodes, conns, domains=[list of cell domain ids per rectangle],
+    cells=[(rect, cx, cy, r)] in local coordinates)."""
+    rng = np.random.default_rng(seed)
+    sides = [2 * int(round(math.exp(rng.uniform(math.log(lmin), math.log(lmax))) / 2.0)) for _ in range(ndom)]
+    origins, height = shelf_pack(sides, width)
+    nodes, conns, rects, domains, allcells = [], [], [], [], []
+    nid, dom = 0, 1
+    for k, L in enumerate(sides):
+        x0, y0 = origins[k]
+        tf = rng.uniform(*tau_f)
+        tc = [rng.uniform(*tau_c) for _ in range(nspecies)]
+        rects.append((x0, y0, L, L, tf, tc))
+        want = int(rng.integers(cells[0], cells[1] + 1))
+        ids = []
+        for cx, cy, r in place_cells(L, want, radius[0], radius[1], rng):
+            n = max(12, int(round(2.0 * math.pi * r / seg)))
+            nd, cn = circle_cell(x0 + cx, y0 + cy, r, n, rng, jitter=0.5, first_id=nid, domain=dom)
+            nodes += nd
+            conns += cn
+            allcells.append((k, cx, cy, r))
+            ids.append(dom)
+            nid += n
+            dom += 1
+        domains.append(ids)
+    return dict(nx=width, ny=_ceil16(height), rects=rects, nodes=nodes, conns=conns, domains=domains,
+                cells=allcells)
