@@ -120,6 +120,56 @@ def measure(nx, ny, nodes, conns, prec_name, steps, warmup, run_ahead):
     return out
 
 
+def build_batched(ens, precision):
+    """The batched ensemble as one atlas context: closed periodic domains
+    (per-domain tau_f and tau_c), the SIGNAL species produced inside cells and
+    the membrane tension.  No area regulator or box outlet: in domains of
+    64-512 sites the outlet's drift carries growing cells to the domain edge
+    within ~150 steps, and a sweep runs thousands."""
+    from lbibcell_amd import capi, runner
+    sim = runner.Simulation(ens["nx"], ens["ny"], TAU_F, precision=precision)
+    sim.add_species("tutorial_01_CDESolverD2Q5_SIGNAL", capi.species_spec(capi.SPECIES_SIGNAL, TAU_C, **SIGNAL))
+    sim.set_batch_layout(ens["rects"])
+    sim.add_geometry(ens["nodes"], ens["conns"])
+    sim.setup()
+    sim.init_solvers()
+    sim.add_tension(TENSION, 10)
+    return sim
+
+
+def measure_batched(ens, prec_name, steps, warmup, run_ahead):
+    """Batched ensemble at one storage precision: autotune the lattice launch
+    shape after the warm-up, then time `steps` coupled steps of all domains."""
+    sim = build_batched(ens, PRECISIONS[prec_name])
+    sim.set_run_ahead(run_ahead)
+    log(f"[batched {prec_name}] setup done; warming up")
+    sim.run(warmup)
+    sim.sync()
+    ctx = sim.context()
+    chosen, rows = ctx.autotune(reps=5)
+    log(f"[batched {prec_name}] tuned lattice shape {chosen[:3]} ({chosen[3] * 1e3:.1f} us)")
+    ctx.profile_enable(True)
+    ctx.profile_read()
+    sim.reset_times()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    ms, launches, bpl = ctx.profile_read()
+    ctx.profile_enable(False)
+    ctx.ns = 1
+    u, v = ctx.get_velocity()
+    live = np.zeros((ens["ny"], ens["nx"]), dtype=bool)
+    for x0, y0, w, h, *_ in ens["rects"]:
+        live[y0:y0 + h, x0:x0 + w] = True
+    fields = dict(speed=np.hypot(u, v)[live], conc=ctx.get_species(0).sum(axis=0)[live], area=sim.areas())
+    out = dict(dt=t1 - t0, kernel_s=ms / 1e3 / max(launches, 1), bytes_per_launch=bpl, tuned=chosen,
+               tried=rows, markers=len(sim.markers()[0]), times=sim.times(), fields=fields)
+    sim.close()
+    return out
+
+
 def l2_rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
 
@@ -137,6 +187,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--run-ahead", type=int, default=16,
                     help="iterations enqueued between device status reads")
+    ap.add_argument("--batched", default="fp16,fp32,fp64",
+                    help="storage precisions of the batched-ensemble measurement ('' = skip); "
+                         "fp64 also serves as the reference of the mixed errors")
+    ap.add_argument("--batched-domains", type=int, default=256)
+    ap.add_argument("--batched-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice path runs on a single MI355X per process; "
@@ -216,6 +271,41 @@ def main():
                 "cell_area": l2_rel(a_m, a_r), "after_steps": args.warmup + args.steps}
     if mixed:
         out["mixed_precision"] = mixed
+    bvars = [p for p in args.batched.split(",") if p]
+    if bvars:
+        from lbibcell_amd import workloads
+        ens = workloads.batched_ensemble(ndom=args.batched_domains, seed=1)
+        nlive = sum(r[2] * r[3] for r in ens["rects"])
+        log(f"batched ensemble: {len(ens['rects'])} domains, {nlive} sites, atlas {ens['nx']}x{ens['ny']}, "
+            f"{sum(len(d) for d in ens['domains'])} cells, {len(ens['nodes'])} initial markers")
+        bres = {p: measure_batched(ens, p, args.batched_steps, 5, args.run_ahead) for p in bvars}
+        batched = {"workload": f"{len(ens['rects'])} periodic domains, sides log-uniform in [64, 512], "
+                               f"1-16 cells r 8-30 each, tau_f U(0.6,1.5), tau_c U(0.55,1.2), one atlas "
+                               f"{ens['nx']}x{ens['ny']} ({nlive} live sites)",
+                   "sites": nlive, "cells": sum(len(d) for d in ens["domains"]), "steps": args.batched_steps}
+        for p, r in bres.items():
+            achieved = r["bytes_per_launch"] / r["kernel_s"] / 1e9
+            e = {"value": nlive * args.batched_steps / r["dt"] / 1e6, "unit": "MLUPS",
+                 "ms_per_step": r["dt"] / args.batched_steps * 1e3, "dtype": DTYPES[p],
+                 "markers_after_remesh": r["markers"],
+                 "lattice_shape": {"variant": r["tuned"][0], "workgroup": r["tuned"][1],
+                                   "sites_per_lane": r["tuned"][2]},
+                 "autotune_us": {f"{t[0]}/{t[1]}/{t[2]}": round(t[3] * 1e3, 2) for t in r["tried"]},
+                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "kernel_ms_avg": r["kernel_s"] * 1e3,
+                              "algorithmic_bytes_per_launch": r["bytes_per_launch"],
+                              "frac_of_copy": achieved / copy_gbs if copy_gbs else None}}
+            ref = bres.get("fp64")
+            if ref is not None and p != "fp64":
+                common = sorted(set(r["fields"]["area"]) & set(ref["fields"]["area"]))
+                e["l2_rel_error_vs_fp64"] = {
+                    "speed": l2_rel(r["fields"]["speed"], ref["fields"]["speed"]),
+                    "concentration": l2_rel(r["fields"]["conc"], ref["fields"]["conc"]),
+                    "cell_area": l2_rel([r["fields"]["area"][d] for d in common],
+                                        [ref["fields"]["area"][d] for d in common]),
+                    "after_steps": 5 + args.batched_steps}
+            batched[p] = e
+        out["batched"] = batched
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(nx, ny, nodes, conns)

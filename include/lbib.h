@@ -284,6 +284,25 @@ int lbib_autotune(lbib_ctx* ctx, int reps, lbib_lattice_shape* tried, int max_tr
                   lbib_lattice_shape* chosen);
 int lbib_set_lattice_shape(lbib_ctx* ctx, const lbib_lattice_shape* shape);
 
+/* Step graphs.  No counterpart in the reference (its loop is a host loop,
+ * SimulationRunner.cpp:69-110).  Between lbib_capture_begin and
+ * lbib_capture_end every per-step call (lbib_step_begin, lbib_step_lattice,
+ * lbib_step_geometry, the force and BioSolver operations) is recorded into a
+ * HIP graph instead of being run; recorded kernels take the iteration from the
+ * device status (one more than the last iteration begun), so the same graph
+ * serves every block of steps with the same launch state.  A call that would
+ * wait for the device or allocate while recording fails, and lbib_capture_end
+ * then reports *ok = 0 and keeps nothing (nothing of the block ran: the caller
+ * rewinds with lbib_rewind and runs it directly).  lbib_graph_launch replays
+ * the recorded block as iterations it_first, it_first + 1, ... (one launch;
+ * run-ahead snapshots are kept as for directly enqueued steps); *stale = 1
+ * and nothing is launched when the launch state has changed since recording
+ * (a reallocation, another launch shape, a topology upload ...). */
+int lbib_capture_begin(lbib_ctx* ctx);
+int lbib_capture_end(lbib_ctx* ctx, int* ok);
+int lbib_graph_launch(lbib_ctx* ctx, uint32_t it_first, int* stale);
+int lbib_graph_reset(lbib_ctx* ctx);
+
 /* ---- measurement ------------------------------------------------------------
  * No counterpart in the reference (it has no profiling): HIP events recorded
  * on the context's stream around every launch of the fused lattice kernel.

@@ -158,6 +158,10 @@ SIGNATURES = {
     "lbib_autotune": (C.c_int, [_P, C.c_int, C.POINTER(LatticeShape), C.c_int, C.POINTER(C.c_int),
                                 C.POINTER(LatticeShape)]),
     "lbib_set_lattice_shape": (C.c_int, [_P, C.POINTER(LatticeShape)]),
+    "lbib_capture_begin": (C.c_int, [_P]),
+    "lbib_capture_end": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "lbib_graph_launch": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int)]),
+    "lbib_graph_reset": (C.c_int, [_P]),
 }
 
 

@@ -439,6 +439,7 @@ namespace {
 struct MembraneTension : BioSolver {
     unsigned freq = 10;
     double force = 0.01;
+    unsigned graphPeriod() const override { return freq; }
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
         r.renewTension(force);
@@ -450,6 +451,7 @@ struct MembraneTension : BioSolver {
 struct AreaRegulator : BioSolver {
     unsigned freq = 1;
     double target = 200.0, kp = 0.00001, maxsource = 0.004, factor = 0.0;
+    unsigned graphPeriod() const override { return freq; }
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
         if (lbib_area_regulator(r.context(), target, kp, maxsource, factor) != 0)
@@ -1020,13 +1022,88 @@ void SimulationRunner::resolveStop(unsigned it) {
 
 void SimulationRunner::step() { run(1); }
 
+void SimulationRunner::setGraphSteps(unsigned steps) {
+    if (steps % 10 != 0) LBIB_FAIL("graph blocks must be a multiple of the coarsening period (10)");
+    graphSteps_ = steps;
+    if (ctx_) check(lbib_graph_reset(ctx_), "graph reset");
+}
+
+// One block of graphSteps_ iterations from it0 (it0 - 1 a multiple of the
+// block length): replayed from the recorded graph, recorded first when there
+// is none or the launch state changed.  false: nothing of the block ran (graph
+// mode is off from now on) and the caller enqueues it step by step.
+bool SimulationRunner::runGraphBlock(unsigned it0) {
+    const unsigned B = graphSteps_;
+    for (auto& b : bio_) {
+        const unsigned p = b->graphPeriod();
+        if (p == 0 || B % p != 0) {
+            graphNote = "BioSolver " + b->name + " cannot be recorded with blocks of " + std::to_string(B);
+            graphSteps_ = 0;
+            return false;
+        }
+    }
+    if (forceSolver_.dirty_) {
+        graphNote = "host force table edited";
+        graphSteps_ = 0;
+        return false;
+    }
+    int stale = 1;
+    check(lbib_graph_launch(ctx_, it0, &stale), "graph launch");
+    if (!stale) {
+        graphLaunches++;
+        pending_ += B;
+        params_.currentIteration = it0 + B;
+        times.steps += B;
+        mirrorValid_ = false;
+        return true;
+    }
+    // record the block; the kernels only run when the graph is launched
+    const unsigned pend0 = pending_;
+    const long steps0 = times.steps;
+    check(lbib_capture_begin(ctx_), "capture");
+    bool failed = false;
+    try {
+        for (unsigned j = 0; j < B; j++) enqueueStep(it0 + j);
+    } catch (const std::exception& e) {
+        failed = true;
+        graphNote = std::string("recording failed: ") + e.what();
+    }
+    int ok = 0;
+    check(lbib_capture_end(ctx_, &ok), "capture");
+    if (!failed && ok) {
+        graphCaptures++;
+        check(lbib_graph_launch(ctx_, it0, &stale), "graph launch");
+    }
+    if (failed || !ok || stale) {
+        // nothing ran: back to the launch state of it0
+        check(lbib_rewind(ctx_, it0), "rewind");
+        pending_ = pend0;
+        times.steps = steps0;
+        params_.currentIteration = it0;
+        forceSolver_.dirty_ = false;
+        if (!failed && !ok) graphNote = "the recorded block could not be instantiated";
+        if (failed || !ok) graphSteps_ = 0;
+        return false;
+    }
+    graphLaunches++;
+    pending_ = pend0 + B;
+    params_.currentIteration = it0 + B;
+    mirrorValid_ = false;
+    return true;
+}
+
 void SimulationRunner::run(unsigned steps) {
     const unsigned end = params_.currentIteration + steps;
     const unsigned window = std::max(1u, runAhead);
     while (params_.currentIteration < end) {
         try {
             const unsigned last = std::min(end, params_.currentIteration + window);
-            while (params_.currentIteration < last) enqueueStep(params_.currentIteration);
+            while (params_.currentIteration < last) {
+                const unsigned it = params_.currentIteration;
+                const unsigned B = graphSteps_;
+                if (B && it % B == 1 && it + B <= end && runGraphBlock(it)) continue;
+                enqueueStep(it);
+            }
             settle();
         } catch (const RunAheadStop& s) {
             resolveStop(s.it);

@@ -149,6 +149,10 @@ class SimulationRunner;
 struct BioSolver {
     virtual ~BioSolver() = default;
     virtual void applyBioProcess(SimulationRunner& runner) = 0;
+    // A BioSolver whose applyBioProcess only enqueues device operations, and
+    // decides on the iteration alone with this period, can be recorded into a
+    // step graph (0: it cannot).
+    virtual unsigned graphPeriod() const { return 0; }
     std::string name;
 };
 
@@ -215,8 +219,18 @@ class SimulationRunner {
     // pending remesh, capacity) is resolved from the iteration that raised it,
     // so results do not depend on this value.
     unsigned runAhead = 16;
+    // Steps per recorded graph (0: every step is enqueued call by call).  A
+    // multiple of 10 (the coarsening period) and of every BioSolver's period;
+    // blocks start after iterations that are multiples of it.  A block that
+    // cannot be recorded (a host-side BioSolver, an allocation or a status
+    // read inside it) turns graph mode off and runs directly.
+    void setGraphSteps(unsigned steps);
+    unsigned graphSteps() const { return graphSteps_; }
+    long graphLaunches = 0, graphCaptures = 0;
+    std::string graphNote;   // why graph mode turned itself off
 
  private:
+    bool runGraphBlock(unsigned it0);
     void check(int rc, const char* what);
     void uploadForces();
     void handleGeometryStatus(unsigned it);
@@ -246,6 +260,7 @@ class SimulationRunner {
     // here the stream is seeded so that runs repeat
     std::mt19937 genInit_{1};
     std::vector<lbib_rect> rects_;
+    unsigned graphSteps_ = 0;
 };
 
 }  // namespace LbmLib

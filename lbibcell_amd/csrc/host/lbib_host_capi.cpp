@@ -3,6 +3,7 @@
 // reference's main() sequence (apps/tutorial_01.cpp:59-145): parameters,
 // geometry, GeometryHandler (setup), SimulationRunner, initSolvers,
 // initForceSolver, addMassSolver / addBioSolver, runSimulation.
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -232,6 +233,29 @@ int lbh_set_run_ahead(void* p, unsigned n) {
         h->runner->runAhead = n < 1 ? 1 : n;
     });
 }
+int lbh_set_graph_steps(void* p, unsigned n) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (!h->runner) LBIB_FAIL("graph steps are set after setup");
+        h->runner->setGraphSteps(n);
+    });
+}
+// why graph mode turned itself off ("" while it is on or was never asked for)
+int lbh_graph_note(void* p, char* buf, int n) {
+    auto* r = static_cast<Handle*>(p)->runner.get();
+    const std::string s = r ? r->graphNote : std::string();
+    if (n > 0) {
+        std::snprintf(buf, (size_t)n, "%s", s.c_str());
+    }
+    return (int)s.size();
+}
+// [steps per graph (0 once graph mode turned itself off), launches, recordings]
+void lbh_graph_stats(void* p, long* out) {
+    auto* r = static_cast<Handle*>(p)->runner.get();
+    out[0] = r ? (long)r->graphSteps() : 0;
+    out[1] = r ? r->graphLaunches : 0;
+    out[2] = r ? r->graphCaptures : 0;
+}
 void lbh_reset_times(void* p) { static_cast<Handle*>(p)->runner->times = StageTimes{}; }
 int lbh_last_status(void* p, lbib_status* st) {
     *st = static_cast<Handle*>(p)->runner->lastStatus;
@@ -259,6 +283,7 @@ namespace {
 struct TensionP : BioSolver {
     unsigned freq;
     double force;
+    unsigned graphPeriod() const override { return freq; }
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
         r.renewTension(force);
@@ -267,6 +292,7 @@ struct TensionP : BioSolver {
 struct AreaP : BioSolver {
     unsigned freq;
     double prm[4];
+    unsigned graphPeriod() const override { return freq; }
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
         if (lbib_area_regulator(r.context(), prm[0], prm[1], prm[2], prm[3]) != 0)

@@ -19,6 +19,11 @@ namespace {
 template <typename T>
 int dev_alloc(lbib_ctx* c, T** p, size_t count, int fill = 0) {
     *p = nullptr;
+    if (c->capturing) {
+        c->err = "device allocation while a step is being recorded into a graph";
+        return -1;
+    }
+    c->alloc_gen++;
     if (count == 0) count = 1;
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) {
@@ -131,7 +136,7 @@ int ensure_domains(lbib_ctx* c, int ndom) {
 int flush_step_begin(lbib_ctx* c) {
     if (!c->begin_pending) return 0;
     c->begin_pending = false;
-    k_step_begin<<<1, 1, 0, c->stream>>>(c->st, c->begin_it);
+    k_step_begin<<<1, 1, 0, c->stream>>>(c->st, c->capturing ? ITER_NEXT : c->begin_it);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
@@ -193,6 +198,20 @@ struct Staging {
     ~Staging() { if (p) (void)hipFree(p); }
 };
 
+// host launch state that the recorded kernel arguments and grids depend on
+std::vector<int64_t> graph_key(const lbib_ctx* c) {
+    const LatticeShape sh = lattice_shape(c);
+    return {(int64_t)c->alloc_gen, c->cur, c->mtab_live, c->msite_live, c->nm == 0, c->nc == 0, c->ndom,
+            sh.variant, sh.wg, sh.spt, c->prof_on, c->nrect};
+}
+void graph_drop(lbib_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->gsnaps.clear();
+    c->gkey.clear();
+}
 }  // namespace
 
 extern "C" {
@@ -304,6 +323,14 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
 
 void lbib_destroy(lbib_ctx* c) {
     if (!c) return;
+    if (c->capturing) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        c->capturing = false;
+    }
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
@@ -633,13 +660,14 @@ static int next_event(lbib_ctx* c, hipEvent_t* ev) {
 
 static int stream_collide(lbib_ctx* c, uint32_t it, bool ftile) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->prof_on) {
+    if (c->capturing) it = ITER_NEXT;
+    if (c->prof_on && !c->capturing) {
         if (next_event(c, &e0) || next_event(c, &e1)) return -1;
         LBIB_HIP(hipEventRecord(e0, c->stream));
     }
     const int r = c->prec == LBIB_FP64 ? launch_stream_collide_f64(c, it, ftile) : launch_stream_collide_mixed(c, it, ftile);
     if (r) return r;
-    if (c->prof_on) LBIB_HIP(hipEventRecord(e1, c->stream));
+    if (c->prof_on && !c->capturing) LBIB_HIP(hipEventRecord(e1, c->stream));
     return 0;
 }
 
@@ -657,7 +685,7 @@ int lbib_step_lattice_only(lbib_ctx* c, uint32_t it) {
 }
 int lbib_step_geometry(lbib_ctx* c, uint32_t it, int coarsen) {
     if (check_ctx(c)) return -1;
-    return launch_geometry(c, it, coarsen ? 2 : 0);
+    return launch_geometry(c, c->capturing ? ITER_NEXT : it, coarsen ? 2 : 0);
 }
 int lbib_step_topology(lbib_ctx* c, int coarsen) {
     if (check_ctx(c)) return -1;
@@ -776,6 +804,7 @@ int lbib_step_begin(lbib_ctx* c, uint32_t it) {
     c->begin_it = it;
     if (!c->snaps.empty() && c->snaps.back().it >= it) c->snaps.clear();   // a new run
     c->snaps.push_back({it, c->cur, c->mtab_live, c->msite_live, c->ev_used});
+    if (c->capturing) c->gsnaps.push_back(c->snaps.back());
     if (c->snaps.size() > 1024) c->snaps.pop_front();
     return 0;
 }
@@ -930,6 +959,66 @@ int lbib_autotune(lbib_ctx* c, int reps, lbib_lattice_shape* tried, int max_trie
         for (int k = 0; k < (int)out.size() && k < max_tried; k++) put(&tried[k], out[k]);
     if (n_tried) *n_tried = (int)out.size();
     if (chosen) put(chosen, out[best]);
+    return 0;
+}
+
+// ------------------------------------------------------------- step graphs --
+
+int lbib_capture_begin(lbib_ctx* c) {
+    if (check_ctx(c)) return -1;
+    if (c->capturing) { c->err = "lbib_capture_begin: already recording"; return -1; }
+    graph_drop(c);
+    c->gkey = graph_key(c);
+    LBIB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    return 0;
+}
+int lbib_capture_end(lbib_ctx* c, int* ok) {
+    if (!c) return -1;
+    if (ok) *ok = 0;
+    if (!c->capturing) { c->err = "lbib_capture_end: not recording"; return -1; }
+    // a step begin still pending belongs to the recorded block
+    const int fl = flush_step_begin(c);
+    c->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (fl != 0 || e != hipSuccess || g == nullptr) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        graph_drop(c);
+        return 0;   // nothing was recorded that can be replayed; *ok = 0
+    }
+    c->graph = g;
+    if (hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        graph_drop(c);
+        return 0;
+    }
+    if (ok) *ok = 1;
+    return 0;
+}
+int lbib_graph_launch(lbib_ctx* c, uint32_t it_first, int* stale) {
+    if (check_ctx(c)) return -1;
+    if (stale) *stale = 0;
+    if (!c->gexec || c->capturing || graph_key(c) != c->gkey || c->gsnaps.empty()) {
+        if (stale) *stale = 1;
+        return 0;
+    }
+    // the launch state of every replayed step, for lbib_rewind
+    const uint32_t it0 = c->gsnaps.front().it;
+    for (const lbib_ctx::Snapshot& s : c->gsnaps) {
+        lbib_ctx::Snapshot t = s;
+        t.it = it_first + (s.it - it0);
+        if (!c->snaps.empty() && c->snaps.back().it >= t.it) c->snaps.clear();
+        c->snaps.push_back(t);
+        if (c->snaps.size() > 1024) c->snaps.pop_front();
+    }
+    LBIB_HIP(hipGraphLaunch(c->gexec, c->stream));
+    return 0;
+}
+int lbib_graph_reset(lbib_ctx* c) {
+    if (!c) return -1;
+    graph_drop(c);
     return 0;
 }
 

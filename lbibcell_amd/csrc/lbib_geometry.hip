@@ -220,6 +220,7 @@ struct ReinitArgs {
     const uint32_t* rmark;
     uint32_t it;
     int stx;
+    const DevStatus* st;    // iteration of ITER_NEXT
     const int32_t* rtile;   // batched atlas (null: one periodic domain)
     const RectDev* rects;
     SpeciesDev sp[LBIB_MAX_SPECIES];
@@ -287,7 +288,7 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         double cj[NS > 0 ? NS : 1];
         for (int j = 0; j < NS; j++) cj[j] = (j == k) ? C : cswap(j, i);
         const double tI = fr.r < 0 ? a.sp[k].tauI : a.rects[fr.r].tauIs[k];
-        species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], tI, nd, a.it, cj, cj);
+        species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], tI, nd, iter_of(a.it, a.st), cj, cj);
         T* gk = gb + k * sn;
         for (int q = 0; q < 5; q++) gk[q * a.n + i] = CD::enc(d[q]);
     }
@@ -510,7 +511,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;
         a.g = c->g[c->cur];
         a.u = c->u; a.tag = c->tag; a.changed = c->changed; a.rmark = c->rmark; a.it = it;
-        a.stx = c->stx; a.rtile = c->nrect ? c->rtile : nullptr; a.rects = c->rects;
+        a.stx = c->stx; a.rtile = c->nrect ? c->rtile : nullptr; a.rects = c->rects; a.st = c->st;
         for (int s = 0; s < LBIB_MAX_SPECIES; s++) a.sp[s] = c->sp[s];
         switch (c->ns) {
             case 0: launch_reinit_t<0>(c, a); break;

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
 template <typename UT>
 __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mvx,
                               double* mvy, double* mox, double* moy, const UT* __restrict__ u,
-                              DevStatus* st) {
+                              const int32_t* rtile, const RectDev* rects, int stx, DevStatus* st) {
     if (step_stopped(st)) return;
     // the grid covers the capacity: renumber over the workgroups with markers
     const uint32_t nm = st->nm, nact = (nm + blockDim.x - 1) / blockDim.x;
@@ -299,7 +299,10 @@ __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mv
     if (p1 || p2 || p3 || p4) atomicOr(&st->bits, LBIB_HALT_PERTURB_NODE);
     const bool within = (Xn >= floor(X)) && (Xn < ceil(X)) && (Yn <= ceil(Y)) && (Yn > floor(Y));
     if (!within) {
-        if (Xn - 2.0 < 0 || Yn - 2.0 < 0 || Xn + 2.0 > nx || Yn + 2.0 > ny) {
+        // the edges of the node's periodic domain (its rectangle in a batched atlas)
+        Frame fr;
+        if (!site_frame(rtile, rects, stx, nx, ny, fx, fy, fr)) { fr.x0 = 0; fr.y0 = 0; fr.w = nx; fr.h = ny; }
+        if (Xn - 2.0 < fr.x0 || Yn - 2.0 < fr.y0 || Xn + 2.0 > fr.x0 + fr.w || Yn + 2.0 > fr.y0 + fr.h) {
             atomicOr(&st->bits, LBIB_ERR_EDGE);
             atomicMin(&st->first_bad, k);
         }
@@ -317,7 +320,7 @@ int launch_force_and_spread(lbib_ctx* c) {
     c->begin_pending = false;
     k_marker_force<<<std::max(nblocks(c->mcap, bs), 1), bs, 0, c->stream>>>(
         c->mx, c->my, c->fs_off, c->fs_type, c->fs_partner, c->fs_prm, c->mfx, c->mfy, ntiles, c->tcnt, c->tact, begin,
-        c->begin_it, c->st);
+        c->capturing ? ITER_NEXT : c->begin_it, c->st);
     if (c->nm == 0) {
         LBIB_HIP(hipGetLastError());
         return 0;
@@ -341,12 +344,14 @@ int launch_clear_force(lbib_ctx*) { return 0; }
 
 int launch_interp_move(lbib_ctx* c) {
     if (c->nm == 0) return 0;
+    const int32_t* rt = c->nrect ? c->rtile : nullptr;
     if (c->prec == LBIB_FP64)
-        k_interp_move<double><<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(c->nx, c->ny, c->mx, c->my, c->mvx,
-                                                                           c->mvy, c->mox, c->moy, c->u, c->st);
+        k_interp_move<double><<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(
+            c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy, c->mox, c->moy, c->u, rt, c->rects, c->stx, c->st);
     else
         k_interp_move<float><<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(
-            c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy, c->mox, c->moy, reinterpret_cast<const float*>(c->u), c->st);
+            c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy, c->mox, c->moy, reinterpret_cast<const float*>(c->u), rt,
+            c->rects, c->stx, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

@@ -70,10 +70,17 @@ constexpr uint32_t STOP_BITS_STEP = STOP_BITS | LBIB_FLAG_REMESH;
 __device__ __forceinline__ bool step_stopped(const DevStatus* st) { return (st->bits & STOP_BITS_STEP) != 0u; }
 // Opens an iteration: records it while the status is clear, otherwise marks
 // the rest of the enqueued work as skipped (one thread).
+// ITER_NEXT (a step recorded into a graph, replayed at many iterations):
+// the iteration after the last one begun.
+constexpr uint32_t ITER_NEXT = 0xffffffffu;
 __device__ __forceinline__ void step_begin_dev(DevStatus* st, uint32_t it) {
     const uint32_t b = st->bits;
     if (b & STOP_BITS_STEP) st->bits = b | LBIB_SKIPPED;
-    else st->step_it = it;
+    else st->step_it = (it == ITER_NEXT) ? st->step_it + 1u : it;
+}
+// the iteration a kernel argument names (ITER_NEXT: the one begun last)
+__device__ __forceinline__ uint32_t iter_of(uint32_t it, const DevStatus* st) {
+    return it == ITER_NEXT ? st->step_it : it;
 }
 
 // One periodic domain of a batched atlas (lbib_rect on the device).
@@ -215,6 +222,13 @@ struct lbib_ctx {
         size_t ev_used;
     };
     std::deque<Snapshot> snaps;
+    // graph of a block of steps (lbib_capture_begin / _end, lbib_graph_launch)
+    bool capturing = false;
+    uint64_t alloc_gen = 0;               // bumped by every device allocation
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<Snapshot> gsnaps;         // launch state of each captured step (it relative)
+    std::vector<int64_t> gkey;            // launch state the graph was recorded from
     // lbib_step_begin is carried out by the next kernel launched (the first
     // kernel of the lattice stage, or a kernel of its own)
     bool begin_pending = false;

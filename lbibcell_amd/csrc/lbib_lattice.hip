@@ -156,7 +156,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_f64(const LatticeArgs a)
 #pragma unroll
         for (int s = 0; s < NS; s++) {
             const double tI = fr.r < 0 ? a.sp[s].tauI : a.rects[fr.r].tauIs[s];
-            species_collide<NS>(g[s], cj[s], uxn, uyn, a.sp[s], tI, dom, a.it, cj, cseq);
+            species_collide<NS>(g[s], cj[s], uxn, uyn, a.sp[s], tI, dom, iter_of(a.it, a.st), cj, cseq);
             // later species read this one's C from its post-collision slots
             cseq[s] = c_swapped(g[s][DT], g[s][DE], g[s][DN], g[s][DW], g[s][DS]);
         }

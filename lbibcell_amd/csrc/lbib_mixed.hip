@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 const float tI = fr[k].r < 0 ? a.sp[s].tauI : (float)a.rects[fr[k].r].tauIs[s];
-                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], tI, dom, a.it, cj, cseq);
+                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], tI, dom, iter_of(a.it, a.st), cj, cseq);
                 // a later species sees this one's concentration after its collide
                 cseq[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
             }
@@ -487,7 +487,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 const float tI = fr.r < 0 ? a.sp[s].tauI : (float)a.rects[fr.r].tauIs[s];
-                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], tI, dom, a.it, cj, cseq);
+                species_collide_f32<NS>(g[k][s], cj[s], uxn, uyn, a.sp[s], tI, dom, iter_of(a.it, a.st), cj, cseq);
                 // a later species sees this one's concentration after its collide
                 cseq[s] = g[k][s][0] + g[k][s][1] + g[k][s][2] + g[k][s][3] + g[k][s][4];
             }

@@ -21,6 +21,9 @@ _SIG = {
     "lbh_last_error": (C.c_char_p, [_P]),
     "lbh_add_species": (C.c_int, [_P, C.c_char_p, C.POINTER(capi.SpeciesSpec)]),
     "lbh_set_batch_layout": (C.c_int, [_P, C.c_int, C.POINTER(capi.Rect)]),
+    "lbh_set_graph_steps": (C.c_int, [_P, C.c_uint]),
+    "lbh_graph_stats": (None, [_P, C.POINTER(C.c_long)]),
+    "lbh_graph_note": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "lbh_load_parameters": (C.c_int, [_P, C.c_char_p]),
     "lbh_load_geometry_txt": (C.c_int, [_P, C.c_char_p]),
     "lbh_add_node": (C.c_int, [_P, C.c_uint, C.c_double, C.c_double]),
@@ -101,6 +104,23 @@ class Simulation:
             spec = capi.species_spec(tau=tau)
             spec.kind = -1
         self._chk(self.L.lbh_add_species(self.h, name.encode(), C.byref(spec)))
+
+    def set_graph_steps(self, n):
+        """Record blocks of n steps (a multiple of 10) into a HIP graph and
+        replay them with one launch each; 0 enqueues step by step."""
+        self._chk(self.L.lbh_set_graph_steps(self.h, int(n)))
+
+    def graph_stats(self):
+        """(steps per graph, 0 once graph mode turned itself off; launches; recordings)"""
+        out = (C.c_long * 3)()
+        self.L.lbh_graph_stats(self.h, out)
+        return tuple(out)
+
+    def graph_note(self):
+        """Why graph mode turned itself off ('' while on)."""
+        buf = C.create_string_buffer(512)
+        self.L.lbh_graph_note(self.h, buf, 512)
+        return buf.value.decode()
 
     def set_batch_layout(self, rects):
         """Batched ensemble: rects = [(x0, y0, w, h, tau_fluid, [tau_species...]), ...],
