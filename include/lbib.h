@@ -303,6 +303,48 @@ int lbib_capture_end(lbib_ctx* ctx, int* ok);
 int lbib_graph_launch(lbib_ctx* ctx, uint32_t it_first, int* stale);
 int lbib_graph_reset(lbib_ctx* ctx);
 
+/* ---- adjoint ----------------------------------------------------------------
+ * No counterpart in the reference (it has no derivatives).  Reverse-mode
+ * gradient of L = sum_x (C_T(x) - C*(x))^2, the final D2Q5 concentration
+ * against a target, through the coupled step restricted to its continuous
+ * part on one periodic nx x ny lattice: type-6 membrane tension of stiffness
+ * kappa on closed marker rings (ForceStructs.hpp:455-514), fastCos-delta
+ * spreading and interpolation with X += U (GeometryNode.cpp:55-254), D2Q9 BGK
+ * with Luo forcing and no mass source (FluidSolver.cpp:103-183,
+ * Luo1993ForceModel.cpp:29-70), and one D2Q5 species on the fresh velocity
+ * with reaction rates[c] inside cell c and -decay*C outside
+ * (tutorial_01_CDESolverD2Q5_SIGNAL.cpp:92-135, one rate per cell).  The cell
+ * map is fixed; cut links, re-initialisation and remeshing are discrete and
+ * not part of the model.  Gradients: dL/dtau_c, dL/dkappa, dL/drates[c].
+ * States are checkpointed every checkpoint_every steps; lbib_adj_gradient
+ * recomputes one segment at a time from its checkpoint and runs the adjoint
+ * kernels backwards over it.  precision: LBIB_FP32 or LBIB_FP64 arithmetic
+ * and storage (gradients are accumulated in FP64 either way). */
+typedef struct lbib_adj lbib_adj;
+typedef struct lbib_adj_config {
+    int32_t nx, ny, precision;
+    double tau_fluid, decay;
+    int32_t n_markers, n_cells, checkpoint_every;
+} lbib_adj_config;
+int lbib_adj_create(const lbib_adj_config* cfg, lbib_adj** out);
+void lbib_adj_destroy(lbib_adj* adj);
+const char* lbib_adj_last_error(const lbib_adj* adj);
+/* Initial populations f0 (9*n) and g0 (5*n) in natural direction order,
+ * plane by plane; marker positions xy0 (2*n_markers); next[k] = the ring
+ * successor of marker k (closed rings); dom (n) = cell of each site, 0
+ * outside, at most n_cells; target (n) = C*. */
+int lbib_adj_set_problem(lbib_adj* adj, const double* f0, const double* g0, const double* xy0, const int32_t* next,
+                         const uint32_t* dom, const double* target);
+/* Runs steps forward steps with the given parameters (rates has n_cells + 1
+ * entries, [0] unused), storing checkpoints; returns the loss and, when
+ * c_final != NULL, the final concentration (n). */
+int lbib_adj_forward(lbib_adj* adj, int steps, double tau_c, double kappa, const double* rates, double* loss,
+                     double* c_final);
+/* Gradient of the last forward's loss (d_rates: n_cells + 1 entries, [0] = 0). */
+int lbib_adj_gradient(lbib_adj* adj, double* d_tau_c, double* d_kappa, double* d_rates);
+/* Device time of the last forward and of the last gradient (recompute + reverse). */
+int lbib_adj_timing(lbib_adj* adj, double* forward_ms, double* gradient_ms);
+
 /* ---- measurement ------------------------------------------------------------
  * No counterpart in the reference (it has no profiling): HIP events recorded
  * on the context's stream around every launch of the fused lattice kernel.

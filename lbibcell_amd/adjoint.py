@@ -1,0 +1,156 @@
+"""Reverse-mode gradients of the coupled D2Q9 + D2Q5 + IB step (C ABI
+lbib_adj_*): the loss L = sum_x (C_T(x) - C*(x))^2 of the final species
+concentration against a target, differentiated with respect to the species
+relaxation time tau_c, the membrane stiffness kappa (type-6 tension,
+ForceStructs.hpp:455-514) and one source rate per cell.
+
+`AdjointProblem` owns one device context; `CoupledLoss` is a thin
+torch.autograd.Function over it, there to check the hand-written adjoint
+against finite differences and FP64 references."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+
+
+class AdjConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("precision", C.c_int32),
+        ("tau_fluid", C.c_double), ("decay", C.c_double),
+        ("n_markers", C.c_int32), ("n_cells", C.c_int32), ("checkpoint_every", C.c_int32),
+    ]
+
+
+SIGNATURES = {
+    "lbib_adj_create": (C.c_int, [C.POINTER(AdjConfig), C.POINTER(_P)]),
+    "lbib_adj_destroy": (None, [_P]),
+    "lbib_adj_last_error": (C.c_char_p, [_P]),
+    "lbib_adj_set_problem": (C.c_int, [_P, _D, _D, _D, C.POINTER(C.c_int32), C.POINTER(C.c_uint32), _D]),
+    "lbib_adj_forward": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _D, _D, _D]),
+    "lbib_adj_gradient": (C.c_int, [_P, _D, _D, _D]),
+    "lbib_adj_timing": (C.c_int, [_P, _D, _D]),
+}
+
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def _lib():
+    lib = capi.load()
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.restype, f.argtypes = res, args
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+class AdjointProblem:
+    """A fixed problem (lattice, initial state, marker rings, cell map,
+    target) whose loss and gradient are evaluated for any parameters.
+
+    rings: list of marker-position arrays (m_r, 2), one closed ring per cell;
+    dom: (ny, nx) cell index per site (0 outside, cells 1..ncell)."""
+
+    def __init__(self, nx, ny, tau_fluid, rings, dom, target=None, f0=None, g0=None, decay=1e-5,
+                 precision=capi.FP32, checkpoint_every=50):
+        self.L = _lib()
+        self.nx, self.ny = nx, ny
+        self.ncell = int(np.max(dom)) if np.size(dom) else 0
+        xy = np.concatenate([np.asarray(r, dtype=np.float64) for r in rings]) if rings else np.zeros((0, 2))
+        nxt = []
+        base = 0
+        for r in rings:
+            m = len(r)
+            nxt += [base + (k + 1) % m for k in range(m)]
+            base += m
+        self.nm = base
+        cfg = AdjConfig(nx, ny, precision, tau_fluid, decay, self.nm, self.ncell, checkpoint_every)
+        self.h = _P()
+        rc = self.L.lbib_adj_create(C.byref(cfg), C.byref(self.h))
+        if rc != 0:
+            msg = self.L.lbib_adj_last_error(self.h).decode() if self.h else "lbib_adj_create failed"
+            self.close()
+            raise capi.LbibError(msg)
+        n = nx * ny
+        f0 = np.tile(W[:, None], (1, n)) if f0 is None else np.ascontiguousarray(f0, dtype=np.float64).reshape(9, n)
+        g0 = np.zeros((5, n)) if g0 is None else np.ascontiguousarray(g0, dtype=np.float64).reshape(5, n)
+        self._xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1)
+        self._next = np.ascontiguousarray(nxt, dtype=np.int32)
+        self._dom = np.ascontiguousarray(dom, dtype=np.uint32).reshape(-1)
+        self._f0, self._g0 = np.ascontiguousarray(f0), np.ascontiguousarray(g0)
+        self.set_target(np.zeros(n) if target is None else target)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise capi.LbibError(self.L.lbib_adj_last_error(self.h).decode())
+
+    def set_target(self, target):
+        self._target = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        self._chk(self.L.lbib_adj_set_problem(
+            self.h, _dp(self._f0), _dp(self._g0), _dp(self._xy) if self.nm else None,
+            self._next.ctypes.data_as(C.POINTER(C.c_int32)) if self.nm else None,
+            self._dom.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(self._target)))
+
+    def forward(self, steps, tau_c, kappa, rates, want_field=False):
+        """Loss (and the final concentration (ny, nx) when want_field)."""
+        r = np.zeros(self.ncell + 1)
+        r[1:] = np.asarray(rates, dtype=np.float64).reshape(-1)[: self.ncell]
+        loss = C.c_double()
+        field = np.empty(self.nx * self.ny) if want_field else None
+        self._chk(self.L.lbib_adj_forward(self.h, int(steps), float(tau_c), float(kappa), _dp(r), C.byref(loss),
+                                          _dp(field) if want_field else None))
+        return (loss.value, field.reshape(self.ny, self.nx)) if want_field else loss.value
+
+    def gradient(self):
+        """(dL/dtau_c, dL/dkappa, dL/drates[ncell]) of the last forward."""
+        dt, dk = C.c_double(), C.c_double()
+        dr = np.zeros(self.ncell + 1)
+        self._chk(self.L.lbib_adj_gradient(self.h, C.byref(dt), C.byref(dk), _dp(dr)))
+        return dt.value, dk.value, dr[1:].copy()
+
+    def timing(self):
+        """(device ms of the last forward, of the last gradient)"""
+        f, g = C.c_double(), C.c_double()
+        self._chk(self.L.lbib_adj_timing(self.h, C.byref(f), C.byref(g)))
+        return f.value, g.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lbib_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def coupled_loss_function():
+    """torch.autograd.Function over AdjointProblem (imported lazily so that
+    the module does not need torch)."""
+    import torch
+
+    class CoupledLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, tau_c, kappa, rates, problem, steps):
+            loss = problem.forward(steps, float(tau_c), float(kappa), rates.detach().cpu().numpy())
+            dt, dk, dr = problem.gradient()
+            ctx.save_for_backward(torch.tensor(dt, dtype=torch.float64), torch.tensor(dk, dtype=torch.float64),
+                                  torch.from_numpy(dr))
+            return torch.tensor(loss, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            dt, dk, dr = ctx.saved_tensors
+            return grad_out * dt, grad_out * dk, grad_out * dr.to(grad_out.dtype), None, None
+
+    return CoupledLoss

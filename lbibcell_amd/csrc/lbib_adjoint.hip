@@ -1,0 +1,893 @@
+// lbib_adjoint.hip -- discrete adjoint of the coupled D2Q9 + D2Q5 + IB step.
+//
+// The differentiable model is the coupled lattice step of SimulationRunner
+// (SimulationRunner.cpp:72-100) restricted to its continuous part, on one
+// periodic nx x ny lattice:
+//   marker forces   F_k = kappa * sum over both ring neighbours (X_j - X_k)/|X_j - X_k|
+//                   (ForceType6, ForceStructs.hpp:455-514, tension renewed on every
+//                   connection as tutorial_01_BioSolverMembraneTension.cpp:42-77)
+//   spread          F(x) = sum_k delta(x - X_k) F_k            (GeometryNode.cpp:228-235)
+//   fluid           pull streaming + BGK with Luo forcing     (FluidSolver.cpp:103-183,
+//                   Luo1993ForceModel.cpp:29-70), no mass source
+//   species         pull streaming + D2Q5 BGK on the fresh velocity with the
+//                   reaction R = s[c] inside cell c, -decay * C outside
+//                   (tutorial_01_CDESolverD2Q5_SIGNAL.cpp:92-135 with one rate per cell)
+//   interpolation   U_k = sum_x delta(x - X_k) u(x), X_k += U_k  (GeometryNode.cpp:216-254)
+// with the fastCos delta of GeometryNode.cpp:55-94.  The cell map (which
+// cell's rate applies at a site) is held at its initial value, and cut links,
+// re-initialisation and remeshing are not part of the model (they are
+// discrete).  The loss is L = sum_x (C_T(x) - C*(x))^2 on the final
+// concentration; the parameters are tau_c, kappa and the per-cell rates s[c].
+//
+// Forward states are checkpointed every K steps; the reverse sweep restores a
+// checkpoint, recomputes its segment storing every intermediate state, and
+// runs the adjoint kernels backwards over it (two-level revolve schedule: the
+// segment store is what HBM capacity buys instead of deeper recursion).
+//
+// Roofline: every kernel here is HBM bound (the per-site adjoint reads the
+// forward state f, g, F, u and the adjoints of step t+1 and writes those of
+// step t: about 2.5x the forward lattice kernel's bytes).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "lbib_device_math.h"
+
+struct lbib_adj;
+
+namespace lbib {
+namespace adj {
+
+template <typename T>
+struct Cst {
+    static constexpr T halfpi = (T)(M_PI / 2.0);
+    static constexpr T pi = (T)M_PI;
+    static constexpr T twopi = (T)(2.0 * M_PI);
+    static constexpr T A = (T)1.27323954;
+    static constexpr T B = (T)0.405284735;
+    static constexpr T W0 = (T)(4.0 / 9.0);
+    static constexpr T W1 = (T)(1.0 / 9.0);
+    static constexpr T W2 = (T)(1.0 / 36.0);
+};
+
+// fastCos (GeometryNode.cpp:55-81) and its derivative.
+template <typename T>
+__device__ __forceinline__ void fast_cos_d(const T x, T& c, T& dc) {
+    T t = x + Cst<T>::halfpi;
+    if (t > Cst<T>::pi) t -= Cst<T>::twopi;
+    T c1, d1;
+    if (t < (T)0) {
+        c1 = Cst<T>::A * t + Cst<T>::B * t * t;
+        d1 = Cst<T>::A + (T)2 * Cst<T>::B * t;
+    } else {
+        c1 = Cst<T>::A * t - Cst<T>::B * t * t;
+        d1 = Cst<T>::A - (T)2 * Cst<T>::B * t;
+    }
+    T d2;
+    if (c1 < (T)0) {
+        c = (T)0.225 * (c1 * -c1 - c1) + c1;
+        d2 = (T)0.225 * ((T)-2 * c1 - (T)1) + (T)1;
+    } else {
+        c = (T)0.225 * (c1 * c1 - c1) + c1;
+        d2 = (T)0.225 * ((T)2 * c1 - (T)1) + (T)1;
+    }
+    dc = d2 * d1;
+}
+
+// The 4 stencil factors h_q = 1 + fastCos(pi/2 (s0 + q - X)) along one axis
+// (delta = hx * hy / 16) and their derivatives with respect to X.
+template <typename T>
+__device__ __forceinline__ void stencil(const T X, int& s0, T h[4], T dh[4]) {
+    s0 = (int)floor((double)X) - 1;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        T c, dc;
+        fast_cos_d<T>(Cst<T>::halfpi * ((T)(s0 + q) - X), c, dc);
+        h[q] = (T)1 + c;
+        dh[q] = -Cst<T>::halfpi * dc;
+    }
+}
+
+__device__ __forceinline__ long wrap_site(int x, int y, int nx, int ny) {
+    x %= nx; if (x < 0) x += nx;
+    y %= ny; if (y < 0) y += ny;
+    return (long)y * nx + x;
+}
+
+// ---------------------------------------------------------------- forward --
+template <typename T>
+__global__ void k_force(int nm, const T* __restrict__ X, const int32_t* __restrict__ nxt,
+                        const int32_t* __restrict__ prv, T kappa, T* __restrict__ Fm) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    const T x1 = X[2 * k], y1 = X[2 * k + 1];
+    T fx = 0, fy = 0;
+    const int nb[2] = {nxt[k], prv[k]};
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        const T x2 = X[2 * nb[s]], y2 = X[2 * nb[s] + 1];
+        const T dist = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+        fx += kappa * (x2 - x1) / dist;
+        fy += kappa * (y2 - y1) / dist;
+    }
+    Fm[2 * k] = fx;
+    Fm[2 * k + 1] = fy;
+}
+
+template <typename T>
+__global__ void k_spread(int nm, const T* __restrict__ X, const T* __restrict__ Fm, T* Fs, int nx, int ny) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    int sx, sy;
+    T hx[4], hy[4], dhx[4], dhy[4];
+    stencil<T>(X[2 * k], sx, hx, dhx);
+    stencil<T>(X[2 * k + 1], sy, hy, dhy);
+    const T fx = Fm[2 * k], fy = Fm[2 * k + 1];
+#pragma unroll
+    for (int qx = 0; qx < 4; qx++)
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const T d = (hx[qx] * hy[qy]) / (T)16;
+            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            atomicAdd(&Fs[2 * s], d * fx);
+            atomicAdd(&Fs[2 * s + 1], d * fy);
+        }
+}
+
+template <typename T>
+struct LatArgs {
+    int nx, ny;
+    long n;
+    const T* fs;
+    const T* gs;
+    T* fd;
+    T* gd;
+    const T* Fs;
+    T* u;
+    const uint32_t* dom;
+    const double* src;
+    T tauI, tauIc, decay;
+};
+
+// pull indices of the 9 directions (natural order) at site (x, y)
+__device__ __forceinline__ void pull_index(int x, int y, int nx, int ny, long idx[9]) {
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+    const long r0 = (long)y * nx, rS = (long)ym * nx, rN = (long)yp * nx;
+    idx[DT] = r0 + x;
+    idx[DE] = r0 + xm;
+    idx[DN] = rS + x;
+    idx[DW] = r0 + xp;
+    idx[DS] = rN + x;
+    idx[DNE] = rS + xm;
+    idx[DNW] = rS + xp;
+    idx[DSW] = rN + xp;
+    idx[DSE] = rN + xm;
+}
+
+// FluidSolver::collide with the Luo force (same operand order as the FP64
+// kernel, m = 0) and the D2Q5 collide with the per-cell reaction.
+template <typename T>
+__device__ __forceinline__ void collide_site(T d[9], T g[5], const T Fx, const T Fy, const T tauI, const T tauIc,
+                                             const uint32_t dm, const double* src, const T decay, T& uxo, T& uyo) {
+    using Q = Cst<T>;
+    T rho0 = 0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) rho0 += d[q];
+    const T rho = rho0;   // no mass source
+    const T rhoI = (T)1 / rho;
+    const T ux = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW])) * rhoI;
+    const T uy = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE])) * rhoI;
+    const T ux2 = ux * ux, uy2 = uy * uy;
+    const T lastTerm = (T)3 / (T)2 * (ux2 + uy2);
+    const T W3 = (T)9 / (T)2;
+    const T W1rho = Q::W1 * rho, W2rho = Q::W2 * rho;
+    const T W1rho3 = rho0 * (T)3 / (T)9, W2rho3 = rho0 * (T)3 / (T)36;
+    T fc[9];
+    fc[DT] = 0;
+    fc[DE] = W1rho3 * Fx;
+    fc[DN] = W1rho3 * Fy;
+    fc[DW] = -W1rho3 * Fx;
+    fc[DS] = -W1rho3 * Fy;
+    fc[DNE] = W2rho3 * (Fx + Fy);
+    fc[DNW] = W2rho3 * (-Fx + Fy);
+    fc[DSW] = W2rho3 * (-Fx - Fy);
+    fc[DSE] = W2rho3 * (Fx - Fy);
+    T tmp[9];
+    tmp[DT] = Q::W0 * rho * ((T)1 - lastTerm) * tauI + fc[DT];
+    tmp[DE] = W1rho * ((T)1 + (T)3 * ux + W3 * ux2 - lastTerm) * tauI + fc[DE];
+    tmp[DNE] = W2rho * ((T)1 + (T)3 * (ux + uy) + W3 * (ux + uy) * (ux + uy) - lastTerm) * tauI + fc[DNE];
+    tmp[DN] = W1rho * ((T)1 + (T)3 * uy + W3 * uy2 - lastTerm) * tauI + fc[DN];
+    tmp[DNW] = W2rho * ((T)1 + (T)3 * (-ux + uy) + W3 * (-ux + uy) * (-ux + uy) - lastTerm) * tauI + fc[DNW];
+    tmp[DW] = W1rho * ((T)1 + (T)3 * (-ux) + W3 * ux2 - lastTerm) * tauI + fc[DW];
+    tmp[DSW] = W2rho * ((T)1 + (T)3 * (-ux - uy) + W3 * (-ux - uy) * (-ux - uy) - lastTerm) * tauI + fc[DSW];
+    tmp[DS] = W1rho * ((T)1 + (T)3 * (-uy) + W3 * uy2 - lastTerm) * tauI + fc[DS];
+    tmp[DSE] = W2rho * ((T)1 + (T)3 * (ux - uy) + W3 * (ux - uy) * (ux - uy) - lastTerm) * tauI + fc[DSE];
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        const T td = d[q];
+        d[q] = td - td * tauI + tmp[q];
+    }
+    uxo = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW])) * rhoI;
+    uyo = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE])) * rhoI;
+    T C = 0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) C += g[q];
+    const T w0 = C / (T)3, w1 = C / (T)6;
+    T te[5];
+    te[DT] = w0;
+    te[DE] = w1 * ((T)1 + uxo * (T)3);
+    te[DN] = w1 * ((T)1 + uyo * (T)3);
+    te[DW] = w1 * ((T)1 + (-uxo) * (T)3);
+    te[DS] = w1 * ((T)1 + (-uyo) * (T)3);
+    const T R = dm != 0u ? (T)src[dm] : -decay * C;
+    const T R0 = (T)1 * R / (T)3, R1 = (T)1 * R / (T)6;
+    g[DT] = g[DT] - g[DT] * tauIc + te[DT] * tauIc + R0;
+#pragma unroll
+    for (int q = 1; q < 5; q++) g[q] = g[q] - g[q] * tauIc + te[q] * tauIc + R1;
+}
+
+template <typename T>
+__global__ void k_lattice(const LatArgs<T> a) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int x = (int)(i % a.nx), y = (int)(i / a.nx);
+    long idx[9];
+    pull_index(x, y, a.nx, a.ny, idx);
+    const long n = a.n;
+    T d[9], g[5];
+#pragma unroll
+    for (int q = 0; q < 9; q++) d[q] = a.fs[q * n + idx[q]];
+#pragma unroll
+    for (int q = 0; q < 5; q++) g[q] = a.gs[q * n + idx[q]];
+    T uxo, uyo;
+    collide_site<T>(d, g, a.Fs[2 * i], a.Fs[2 * i + 1], a.tauI, a.tauIc, a.dom[i], a.src, a.decay, uxo, uyo);
+#pragma unroll
+    for (int q = 0; q < 9; q++) a.fd[q * n + i] = d[q];
+#pragma unroll
+    for (int q = 0; q < 5; q++) a.gd[q * n + i] = g[q];
+    a.u[2 * i] = uxo;
+    a.u[2 * i + 1] = uyo;
+}
+
+template <typename T>
+__global__ void k_interp_move(int nm, T* X, const T* __restrict__ u, int nx, int ny) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    int sx, sy;
+    T hx[4], hy[4], dhx[4], dhy[4];
+    stencil<T>(X[2 * k], sx, hx, dhx);
+    stencil<T>(X[2 * k + 1], sy, hy, dhy);
+    T vx = 0, vy = 0;
+#pragma unroll
+    for (int qx = 0; qx < 4; qx++)
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const T d = (hx[qx] * hy[qy]) / (T)16;
+            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            vx += d * u[2 * s];
+            vy += d * u[2 * s + 1];
+        }
+    X[2 * k] += vx;
+    X[2 * k + 1] += vy;
+}
+
+// loss L = sum (C - C*)^2 and its adjoint on the stored species populations
+template <typename T>
+__global__ void k_loss(const T* __restrict__ g, const double* __restrict__ target, long n, T* lg, double* loss,
+                       T* cfinal) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double r2 = 0.0;
+    if (i < n) {
+        T C = 0;
+#pragma unroll
+        for (int q = 0; q < 5; q++) C += g[q * n + i];
+        const double r = (double)C - target[i];
+        r2 = r * r;
+        if (cfinal) cfinal[i] = C;
+#pragma unroll
+        for (int q = 0; q < 5; q++) lg[q * n + i] = (T)(2.0 * r);
+    }
+    for (int o = 32; o > 0; o >>= 1) r2 += __shfl_down(r2, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(loss, r2);
+}
+
+// ---------------------------------------------------------------- reverse --
+// interpolation + move: lamX (= lambda of X_{t+1}) is lambda_U; adds the
+// derivative of the weights and spreads lambda_U onto lamU (sites).
+template <typename T>
+__global__ void k_interp_bwd(int nm, const T* __restrict__ X, const T* __restrict__ u, T* lamX, T* lamU, int nx,
+                             int ny) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    int sx, sy;
+    T hx[4], hy[4], dhx[4], dhy[4];
+    stencil<T>(X[2 * k], sx, hx, dhx);
+    stencil<T>(X[2 * k + 1], sy, hy, dhy);
+    const T lx = lamX[2 * k], ly = lamX[2 * k + 1];
+    T gx = 0, gy = 0;
+#pragma unroll
+    for (int qx = 0; qx < 4; qx++)
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            const T ul = u[2 * s] * lx + u[2 * s + 1] * ly;
+            gx += (dhx[qx] * hy[qy]) / (T)16 * ul;
+            gy += (hx[qx] * dhy[qy]) / (T)16 * ul;
+            const T d = (hx[qx] * hy[qy]) / (T)16;
+            atomicAdd(&lamU[2 * s], d * lx);
+            atomicAdd(&lamU[2 * s + 1], d * ly);
+        }
+    lamX[2 * k] = lx + gx;
+    lamX[2 * k + 1] = ly + gy;
+}
+
+template <typename T>
+struct BwdArgs {
+    int nx, ny;
+    long n;
+    const T* fs;   // f_t, g_t (pre-stream states of the step)
+    const T* gs;
+    const T* Fs;   // site force of the step
+    const T* lf1;  // adjoints of f_{t+1}, g_{t+1}
+    const T* lg1;
+    T* lamU;       // adjoint of the output velocity from the interpolation (read, cleared)
+    T* lf0;        // adjoints of f_t, g_t (written by push)
+    T* lg0;
+    T* lamF;       // adjoint of the site force
+    const uint32_t* dom;
+    const double* src;
+    T tauI, tauIc, decay;
+    int ncell;
+    double* grad;  // [0] tau_c, [1 + c] rate of cell c
+};
+
+// Per-site adjoint of pull + fluid collide + species collide.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
+    extern __shared__ double sgrad[];   // [ncell + 1]: tau_c at 0, rates at 1..ncell
+    for (int c = threadIdx.x; c <= a.ncell; c += blockDim.x) sgrad[c] = 0.0;
+    __syncthreads();
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.n) {
+        using Q = Cst<T>;
+        const int x = (int)(i % a.nx), y = (int)(i / a.nx);
+        long idx[9];
+        pull_index(x, y, a.nx, a.ny, idx);
+        const long n = a.n;
+        T d[9], g[5];
+#pragma unroll
+        for (int q = 0; q < 9; q++) d[q] = a.fs[q * n + idx[q]];
+#pragma unroll
+        for (int q = 0; q < 5; q++) g[q] = a.gs[q * n + idx[q]];
+        const T Fx = a.Fs[2 * i], Fy = a.Fs[2 * i + 1];
+        // forward values
+        T rho = 0;
+#pragma unroll
+        for (int q = 0; q < 9; q++) rho += d[q];
+        const T rhoI = (T)1 / rho;
+        const T ux = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW])) * rhoI;
+        const T uy = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE])) * rhoI;
+        const T uox = ux + Fx, uoy = uy + Fy;   // post-collision velocity (exact relation)
+        T C = 0;
+#pragma unroll
+        for (int q = 0; q < 5; q++) C += g[q];
+        const uint32_t dm = a.dom[i];
+        // adjoints of step t+1 at this site
+        T lf[9], lg[5];
+#pragma unroll
+        for (int q = 0; q < 9; q++) lf[q] = a.lf1[q * n + i];
+#pragma unroll
+        for (int q = 0; q < 5; q++) lg[q] = a.lg1[q * n + i];
+        T luox = a.lamU[2 * i], luoy = a.lamU[2 * i + 1];
+        a.lamU[2 * i] = 0;
+        a.lamU[2 * i + 1] = 0;
+        // species: g'_p = g_p (1 - tauIc) + geq_p(C, uo) tauIc + R r_p
+        const T ae[5] = {(T)1 / (T)3, ((T)1 + (T)3 * uox) / (T)6, ((T)1 + (T)3 * uoy) / (T)6,
+                         ((T)1 - (T)3 * uox) / (T)6, ((T)1 - (T)3 * uoy) / (T)6};
+        const T rr[5] = {(T)1 / (T)3, (T)1 / (T)6, (T)1 / (T)6, (T)1 / (T)6, (T)1 / (T)6};
+        const T dRdC = dm != 0u ? (T)0 : -a.decay;
+        T S = 0, gtau = 0, gsrc = 0;
+#pragma unroll
+        for (int p = 0; p < 5; p++) {
+            S += lg[p] * (ae[p] * a.tauIc + rr[p] * dRdC);
+            gtau += lg[p] * (g[p] - ae[p] * C);
+            gsrc += lg[p] * rr[p];
+        }
+        T lgh[5];
+#pragma unroll
+        for (int q = 0; q < 5; q++) lgh[q] = lg[q] * ((T)1 - a.tauIc) + S;
+        // d geq_p / d uo = C/2 e_p for p = E, N, W, S
+        luox += C / (T)2 * a.tauIc * (lg[DE] - lg[DW]);
+        luoy += C / (T)2 * a.tauIc * (lg[DN] - lg[DS]);
+        // fluid: f'_q = f_q (1 - tauI) + feq_q(rho, u) tauI + 3 w_q rho (e_q . F), uo = u + F
+        const int ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+        const int ey[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+        const T w[9] = {Q::W0, Q::W1, Q::W1, Q::W1, Q::W1, Q::W2, Q::W2, Q::W2, Q::W2};
+        const T u2 = ux * ux + uy * uy;
+        T Asum = 0, Bx = luox, By = luoy, lFx = luox, lFy = luoy;
+#pragma unroll
+        for (int p = 0; p < 9; p++) {
+            const T eu = (T)ex[p] * ux + (T)ey[p] * uy;
+            const T feq_r = w[p] * ((T)1 + (T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * u2);   // feq / rho
+            const T eF = (T)ex[p] * Fx + (T)ey[p] * Fy;
+            Asum += lf[p] * (a.tauI * feq_r + (T)3 * w[p] * eF);
+            const T cf = lf[p] * a.tauI * w[p] * rho;
+            Bx += cf * ((T)3 * (T)ex[p] + (T)9 * eu * (T)ex[p] - (T)3 * ux);
+            By += cf * ((T)3 * (T)ey[p] + (T)9 * eu * (T)ey[p] - (T)3 * uy);
+            lFx += lf[p] * (T)3 * w[p] * rho * (T)ex[p];
+            lFy += lf[p] * (T)3 * w[p] * rho * (T)ey[p];
+        }
+        // push: the adjoint of the pull from x - e_q lands there
+#pragma unroll
+        for (int q = 0; q < 9; q++) {
+            const T lfh = lf[q] * ((T)1 - a.tauI) + Asum + (Bx * ((T)ex[q] - ux) + By * ((T)ey[q] - uy)) * rhoI;
+            a.lf0[q * n + idx[q]] = lfh;
+        }
+#pragma unroll
+        for (int q = 0; q < 5; q++) a.lg0[q * n + idx[q]] = lgh[q];
+        a.lamF[2 * i] = lFx;
+        a.lamF[2 * i + 1] = lFy;
+        // d g'_p / d tau_c = (g_p - geq_p) / tau_c^2
+        atomicAdd(&sgrad[0], (double)(gtau * a.tauIc * a.tauIc));
+        if (dm != 0u && (int)dm <= a.ncell) atomicAdd(&sgrad[dm], (double)gsrc);
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c <= a.ncell; c += blockDim.x)
+        if (sgrad[c] != 0.0) atomicAdd(&a.grad[c], sgrad[c]);
+}
+
+// spread adjoint: lambda of the marker force and the weight derivative term
+template <typename T>
+__global__ void k_spread_bwd(int nm, const T* __restrict__ X, const T* __restrict__ Fm, const T* __restrict__ lamF,
+                             T* lamFm, T* lamX, int nx, int ny) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nm) return;
+    int sx, sy;
+    T hx[4], hy[4], dhx[4], dhy[4];
+    stencil<T>(X[2 * k], sx, hx, dhx);
+    stencil<T>(X[2 * k + 1], sy, hy, dhy);
+    const T fx = Fm[2 * k], fy = Fm[2 * k + 1];
+    T lx = 0, ly = 0, gx = 0, gy = 0;
+#pragma unroll
+    for (int qx = 0; qx < 4; qx++)
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            const T d = (hx[qx] * hy[qy]) / (T)16;
+            const T Lx = lamF[2 * s], Ly = lamF[2 * s + 1];
+            lx += d * Lx;
+            ly += d * Ly;
+            const T fl = fx * Lx + fy * Ly;
+            gx += (dhx[qx] * hy[qy]) / (T)16 * fl;
+            gy += (hx[qx] * dhy[qy]) / (T)16 * fl;
+        }
+    lamFm[2 * k] = lx;
+    lamFm[2 * k + 1] = ly;
+    lamX[2 * k] += gx;
+    lamX[2 * k + 1] += gy;
+}
+
+// tension adjoint: connection (a, b) gives F_a += kappa n, F_b -= kappa n
+// with n = (X_b - X_a)/|X_b - X_a|; each marker gathers its two connections.
+template <typename T>
+__global__ void k_force_bwd(int nm, const T* __restrict__ X, const int32_t* __restrict__ nxt,
+                            const int32_t* __restrict__ prv, T kappa, const T* __restrict__ lamFm, T* lamX,
+                            double* gkappa) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    double gk = 0.0;
+    if (k < nm) {
+        const T xk = X[2 * k], yk = X[2 * k + 1];
+        T ax = 0, ay = 0;
+        // as the first end of (k, next)
+        {
+            const int b = nxt[k];
+            const T dx = X[2 * b] - xk, dy = X[2 * b + 1] - yk;
+            const T dist = sqrt(dx * dx + dy * dy);
+            const T nxv = dx / dist, nyv = dy / dist;
+            const T vx = lamFm[2 * k] - lamFm[2 * b], vy = lamFm[2 * k + 1] - lamFm[2 * b + 1];
+            const T nv = nxv * vx + nyv * vy;
+            ax -= kappa * (vx - nxv * nv) / dist;
+            ay -= kappa * (vy - nyv * nv) / dist;
+            gk = (double)nv;
+        }
+        // as the second end of (prev, k)
+        {
+            const int p = prv[k];
+            const T dx = xk - X[2 * p], dy = yk - X[2 * p + 1];
+            const T dist = sqrt(dx * dx + dy * dy);
+            const T nxv = dx / dist, nyv = dy / dist;
+            const T vx = lamFm[2 * p] - lamFm[2 * k], vy = lamFm[2 * p + 1] - lamFm[2 * k + 1];
+            const T nv = nxv * vx + nyv * vy;
+            ax += kappa * (vx - nxv * nv) / dist;
+            ay += kappa * (vy - nyv * nv) / dist;
+        }
+        lamX[2 * k] += ax;
+        lamX[2 * k + 1] += ay;
+    }
+    for (int o = 32; o > 0; o >>= 1) gk += __shfl_down(gk, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(gkappa, gk);
+}
+
+}  // namespace adj
+}  // namespace lbib
+
+using namespace lbib;
+
+struct lbib_adj {
+    lbib_adj_config cfg;
+    long n = 0;
+    int nm = 0, ncell = 0, K = 50, prec = LBIB_FP32;
+    size_t tb = 4;                  // bytes of the arithmetic type
+    hipStream_t stream = nullptr;
+    std::string err;
+    // state (arithmetic type T)
+    void *f[2] = {nullptr, nullptr}, *g[2] = {nullptr, nullptr};
+    void *X = nullptr, *Fm = nullptr, *Fs = nullptr, *u = nullptr;
+    // problem
+    void *f0 = nullptr, *g0 = nullptr, *X0 = nullptr;
+    int32_t *nxt = nullptr, *prv = nullptr;
+    uint32_t* dom = nullptr;
+    double *target = nullptr, *src = nullptr, *grad = nullptr, *loss = nullptr;
+    // checkpoints and the segment store
+    void* ck = nullptr;             // S x (14 n + 2 nm)
+    int nck = 0;
+    void* seg = nullptr;            // K x (14 n + 2 nm) states + K x (4 n + 2 nm) forces and velocities
+    // adjoints
+    void *lf[2] = {nullptr, nullptr}, *lg[2] = {nullptr, nullptr};
+    void *lX = nullptr, *lFm = nullptr, *lU = nullptr, *lF = nullptr;
+    int steps = 0;
+    double tau_c = 0.8, kappa = 0.01;
+    std::vector<double> rates;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    double fwd_ms = 0.0, grad_ms = 0.0;
+    bool have_problem = false, have_forward = false;
+};
+
+namespace {
+
+#define ADJ_HIP(call)                                                      \
+    do {                                                                   \
+        hipError_t e_ = (call);                                            \
+        if (e_ != hipSuccess) {                                            \
+            a->err = std::string(#call) + ": " + hipGetErrorString(e_);   \
+            return -1;                                                     \
+        }                                                                  \
+    } while (0)
+
+int adj_alloc(lbib_adj* a, void** p, size_t bytes) {
+    *p = nullptr;
+    ADJ_HIP(hipMalloc(p, std::max<size_t>(bytes, 16)));
+    ADJ_HIP(hipMemsetAsync(*p, 0, std::max<size_t>(bytes, 16), a->stream));
+    return 0;
+}
+void adj_free(void*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+inline int nb(long count) { return (int)((count + 255) / 256); }
+
+template <typename T>
+T* tp(void* p) { return static_cast<T*>(p); }
+
+// state slot offsets (in T elements) of the checkpoint and segment stores
+inline size_t state_len(const lbib_adj* a) { return 14 * (size_t)a->n + 2 * (size_t)a->nm; }
+inline size_t extra_len(const lbib_adj* a) { return 4 * (size_t)a->n + 2 * (size_t)a->nm; }
+
+template <typename T>
+struct Slot {
+    T *f, *g, *X;   // 9n, 5n, 2nm
+};
+template <typename T>
+Slot<T> state_slot(void* base, const lbib_adj* a, int j) {
+    T* s = tp<T>(base) + (size_t)j * state_len(a);
+    return {s, s + 9 * a->n, s + 14 * a->n};
+}
+template <typename T>
+struct Extra {
+    T *Fs, *u, *Fm;   // 2n, 2n, 2nm
+};
+template <typename T>
+Extra<T> extra_slot(lbib_adj* a, int j) {
+    T* s = tp<T>(a->seg) + (size_t)a->K * state_len(a) + (size_t)j * extra_len(a);
+    return {s, s + 2 * a->n, s + 4 * a->n};
+}
+
+template <typename T>
+adj::LatArgs<T> lat_args(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, const T* Fs, T* u) {
+    adj::LatArgs<T> l;
+    l.nx = a->cfg.nx; l.ny = a->cfg.ny; l.n = a->n;
+    l.fs = fs; l.gs = gs; l.fd = fd; l.gd = gd; l.Fs = Fs; l.u = u;
+    l.dom = a->dom; l.src = a->src;
+    l.tauI = (T)(1.0 / a->cfg.tau_fluid);
+    l.tauIc = (T)(1.0 / a->tau_c);
+    l.decay = (T)a->cfg.decay;
+    return l;
+}
+
+// one forward step from (fs, gs, X) into (fd, gd, X); Fs, Fm, u receive the
+// step's site force, marker forces and velocity
+template <typename T>
+int step_fwd(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, T* X, T* Fs, T* Fm, T* u) {
+    const int nx = a->cfg.nx, ny = a->cfg.ny;
+    ADJ_HIP(hipMemsetAsync(Fs, 0, 2 * a->n * sizeof(T), a->stream));
+    if (a->nm > 0) {
+        adj::k_force<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, a->nxt, a->prv, (T)a->kappa, Fm);
+        adj::k_spread<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, Fm, Fs, nx, ny);
+    }
+    adj::k_lattice<T><<<nb(a->n), 256, 0, a->stream>>>(lat_args<T>(a, fs, gs, fd, gd, Fs, u));
+    if (a->nm > 0) adj::k_interp_move<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, u, nx, ny);
+    ADJ_HIP(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
+    const size_t sl = state_len(a);
+    const int need = steps / a->K + 1;
+    if (need > a->nck) {
+        adj_free(a->ck);
+        if (adj_alloc(a, &a->ck, (size_t)need * sl * sizeof(T))) return -1;
+        a->nck = need;
+    }
+    ADJ_HIP(hipMemcpyAsync(a->f[0], a->f0, 9 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+    ADJ_HIP(hipMemcpyAsync(a->g[0], a->g0, 5 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+    ADJ_HIP(hipMemcpyAsync(a->X, a->X0, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+    ADJ_HIP(hipEventRecord(a->e0, a->stream));
+    int cur = 0;
+    for (int t = 0; t < steps; t++) {
+        if (t % a->K == 0) {
+            Slot<T> s = state_slot<T>(a->ck, a, t / a->K);
+            ADJ_HIP(hipMemcpyAsync(s.f, a->f[cur], 9 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+            ADJ_HIP(hipMemcpyAsync(s.g, a->g[cur], 5 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+            ADJ_HIP(hipMemcpyAsync(s.X, a->X, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+        }
+        if (step_fwd<T>(a, tp<T>(a->f[cur]), tp<T>(a->g[cur]), tp<T>(a->f[1 - cur]), tp<T>(a->g[1 - cur]),
+                        tp<T>(a->X), tp<T>(a->Fs), tp<T>(a->Fm), tp<T>(a->u)))
+            return -1;
+        cur = 1 - cur;
+    }
+    ADJ_HIP(hipMemsetAsync(a->loss, 0, sizeof(double), a->stream));
+    T* cf = nullptr;
+    if (cfinal) cf = tp<T>(a->u);   // reuse: n values (u holds 2n)
+    adj::k_loss<T><<<nb(a->n), 256, 0, a->stream>>>(tp<T>(a->g[cur]), a->target, a->n, tp<T>(a->lg[0]), a->loss,
+                                                     cf);
+    ADJ_HIP(hipEventRecord(a->e1, a->stream));
+    ADJ_HIP(hipEventSynchronize(a->e1));
+    float ms = 0.0f;
+    ADJ_HIP(hipEventElapsedTime(&ms, a->e0, a->e1));
+    a->fwd_ms = ms;
+    ADJ_HIP(hipMemcpy(loss, a->loss, sizeof(double), hipMemcpyDeviceToHost));
+    if (cfinal) {
+        std::vector<T> tmp(a->n);
+        ADJ_HIP(hipMemcpy(tmp.data(), cf, a->n * sizeof(T), hipMemcpyDeviceToHost));
+        for (long i = 0; i < a->n; i++) cfinal[i] = (double)tmp[i];
+    }
+    a->steps = steps;
+    return 0;
+}
+
+template <typename T>
+int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_rates) {
+    const int nx = a->cfg.nx, ny = a->cfg.ny;
+    const int steps = a->steps;
+    ADJ_HIP(hipEventRecord(a->e0, a->stream));
+    // lambda at T: lf = 0, lg from the loss (k_loss wrote lg[0]), lX = 0
+    int cur = 0;
+    ADJ_HIP(hipMemsetAsync(a->lf[0], 0, 9 * a->n * sizeof(T), a->stream));
+    ADJ_HIP(hipMemsetAsync(a->lX, 0, 2 * (size_t)a->nm * sizeof(T), a->stream));
+    ADJ_HIP(hipMemsetAsync(a->lU, 0, 2 * a->n * sizeof(T), a->stream));
+    ADJ_HIP(hipMemsetAsync(a->grad, 0, (a->ncell + 2) * sizeof(double), a->stream));
+    const int nseg = (steps + a->K - 1) / a->K;
+    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+        const int t0 = sgi * a->K, len = std::min(a->K, steps - t0);
+        // recompute the segment: slot j holds the state before step t0 + j
+        Slot<T> c = state_slot<T>(a->ck, a, sgi);
+        Slot<T> s0 = state_slot<T>(a->seg, a, 0);
+        ADJ_HIP(hipMemcpyAsync(s0.f, c.f, state_len(a) * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+        for (int j = 0; j < len; j++) {
+            Slot<T> s = state_slot<T>(a->seg, a, j);
+            Extra<T> e = extra_slot<T>(a, j);
+            // next state: the following slot, or scratch after the segment's last step
+            T* fd = j + 1 < len ? state_slot<T>(a->seg, a, j + 1).f : tp<T>(a->f[0]);
+            T* gd = j + 1 < len ? state_slot<T>(a->seg, a, j + 1).g : tp<T>(a->g[0]);
+            T* Xn = j + 1 < len ? state_slot<T>(a->seg, a, j + 1).X : tp<T>(a->X);
+            ADJ_HIP(hipMemcpyAsync(Xn, s.X, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+            if (step_fwd<T>(a, s.f, s.g, fd, gd, Xn, e.Fs, e.Fm, e.u)) return -1;
+        }
+        // reverse over the segment
+        for (int j = len - 1; j >= 0; j--) {
+            Slot<T> s = state_slot<T>(a->seg, a, j);
+            Extra<T> e = extra_slot<T>(a, j);
+            if (a->nm > 0)
+                adj::k_interp_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, e.u, tp<T>(a->lX), tp<T>(a->lU),
+                                                                         nx, ny);
+            adj::BwdArgs<T> b;
+            b.nx = nx; b.ny = ny; b.n = a->n;
+            b.fs = s.f; b.gs = s.g; b.Fs = e.Fs;
+            b.lf1 = tp<T>(a->lf[cur]); b.lg1 = tp<T>(a->lg[cur]);
+            b.lamU = tp<T>(a->lU);
+            b.lf0 = tp<T>(a->lf[1 - cur]); b.lg0 = tp<T>(a->lg[1 - cur]);
+            b.lamF = tp<T>(a->lF);
+            b.dom = a->dom; b.src = a->src;
+            b.tauI = (T)(1.0 / a->cfg.tau_fluid);
+            b.tauIc = (T)(1.0 / a->tau_c);
+            b.decay = (T)a->cfg.decay;
+            b.ncell = a->ncell;
+            b.grad = a->grad;
+            adj::k_lattice_bwd<T><<<nb(a->n), 256, (a->ncell + 1) * sizeof(double), a->stream>>>(b);
+            if (a->nm > 0) {
+                adj::k_spread_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, e.Fm, tp<T>(a->lF), tp<T>(a->lFm),
+                                                                         tp<T>(a->lX), nx, ny);
+                adj::k_force_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, a->nxt, a->prv, (T)a->kappa,
+                                                                        tp<T>(a->lFm), tp<T>(a->lX),
+                                                                        a->grad + a->ncell + 1);
+            }
+            ADJ_HIP(hipGetLastError());
+            cur = 1 - cur;
+        }
+    }
+    ADJ_HIP(hipEventRecord(a->e1, a->stream));
+    ADJ_HIP(hipEventSynchronize(a->e1));
+    float ms = 0.0f;
+    ADJ_HIP(hipEventElapsedTime(&ms, a->e0, a->e1));
+    a->grad_ms = ms;
+    std::vector<double> gr(a->ncell + 2);
+    ADJ_HIP(hipMemcpy(gr.data(), a->grad, gr.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (d_tau) *d_tau = gr[0];
+    if (d_kappa) *d_kappa = gr[a->ncell + 1];
+    if (d_rates) {
+        d_rates[0] = 0.0;
+        for (int c = 1; c <= a->ncell; c++) d_rates[c] = gr[c];
+    }
+    return 0;
+}
+
+template <typename T>
+int upload_T(lbib_adj* a, void* dst, const double* src, size_t count) {
+    std::vector<T> tmp(count);
+    for (size_t k = 0; k < count; k++) tmp[k] = (T)src[k];
+    ADJ_HIP(hipMemcpy(dst, tmp.data(), count * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbib_adj_create(const lbib_adj_config* cfg, lbib_adj** out) {
+    if (!cfg || !out) return -1;
+    *out = nullptr;
+    lbib_adj* a = new (std::nothrow) lbib_adj();
+    if (!a) return -1;
+    *out = a;
+    a->cfg = *cfg;
+    if (cfg->nx < 8 || cfg->ny < 8 || cfg->n_markers < 0 || cfg->n_cells < 0 || cfg->n_cells > 4096 ||
+        !(cfg->tau_fluid > 0.5) || (cfg->precision != LBIB_FP32 && cfg->precision != LBIB_FP64) ||
+        cfg->checkpoint_every < 1) {
+        a->err = "invalid adjoint configuration (nx, ny >= 8, FP32 or FP64, tau_fluid > 0.5, checkpoint_every >= 1)";
+        return -1;
+    }
+    a->n = (long)cfg->nx * cfg->ny;
+    a->nm = cfg->n_markers;
+    a->ncell = cfg->n_cells;
+    a->K = cfg->checkpoint_every;
+    a->prec = cfg->precision;
+    a->tb = a->prec == LBIB_FP64 ? 8 : 4;
+    a->rates.assign(a->ncell + 1, 0.0);
+    ADJ_HIP(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
+    ADJ_HIP(hipEventCreate(&a->e0));
+    ADJ_HIP(hipEventCreate(&a->e1));
+    const size_t tb = a->tb, n = (size_t)a->n, nm = (size_t)a->nm;
+    for (int b = 0; b < 2; b++) {
+        if (adj_alloc(a, &a->f[b], 9 * n * tb) || adj_alloc(a, &a->g[b], 5 * n * tb) ||
+            adj_alloc(a, &a->lf[b], 9 * n * tb) || adj_alloc(a, &a->lg[b], 5 * n * tb))
+            return -1;
+    }
+    if (adj_alloc(a, &a->X, 2 * nm * tb) || adj_alloc(a, &a->Fm, 2 * nm * tb) || adj_alloc(a, &a->Fs, 2 * n * tb) ||
+        adj_alloc(a, &a->u, 2 * n * tb) || adj_alloc(a, &a->f0, 9 * n * tb) || adj_alloc(a, &a->g0, 5 * n * tb) ||
+        adj_alloc(a, &a->X0, 2 * nm * tb) || adj_alloc(a, (void**)&a->nxt, nm * 4) ||
+        adj_alloc(a, (void**)&a->prv, nm * 4) || adj_alloc(a, (void**)&a->dom, n * 4) ||
+        adj_alloc(a, (void**)&a->target, n * 8) || adj_alloc(a, (void**)&a->src, (a->ncell + 1) * 8) ||
+        adj_alloc(a, (void**)&a->grad, (a->ncell + 2) * 8) || adj_alloc(a, (void**)&a->loss, 8) ||
+        adj_alloc(a, &a->lX, 2 * nm * tb) || adj_alloc(a, &a->lFm, 2 * nm * tb) || adj_alloc(a, &a->lU, 2 * n * tb) ||
+        adj_alloc(a, &a->lF, 2 * n * tb))
+        return -1;
+    if (adj_alloc(a, &a->seg, (size_t)a->K * (state_len(a) + extra_len(a)) * tb)) return -1;
+    ADJ_HIP(hipStreamSynchronize(a->stream));
+    return 0;
+}
+
+void lbib_adj_destroy(lbib_adj* a) {
+    if (!a) return;
+    if (a->stream) (void)hipStreamSynchronize(a->stream);
+    void** bufs[] = {&a->f[0], &a->f[1], &a->g[0], &a->g[1], &a->lf[0], &a->lf[1], &a->lg[0], &a->lg[1],
+                     &a->X, &a->Fm, &a->Fs, &a->u, &a->f0, &a->g0, &a->X0, &a->ck, &a->seg, &a->lX, &a->lFm,
+                     &a->lU, &a->lF};
+    for (void** b : bufs) adj_free(*b);
+    void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    if (a->e0) (void)hipEventDestroy(a->e0);
+    if (a->e1) (void)hipEventDestroy(a->e1);
+    if (a->stream) (void)hipStreamDestroy(a->stream);
+    delete a;
+}
+
+const char* lbib_adj_last_error(const lbib_adj* a) { return a ? a->err.c_str() : "null adjoint context"; }
+
+int lbib_adj_set_problem(lbib_adj* a, const double* f0, const double* g0, const double* xy0, const int32_t* next,
+                         const uint32_t* dom, const double* target) {
+    if (!a || !f0 || !g0 || !dom || !target || (a->nm > 0 && (!xy0 || !next))) return -1;
+    const size_t n = (size_t)a->n, nm = (size_t)a->nm;
+    // ring successors must form closed rings (every marker has one predecessor)
+    std::vector<int32_t> prv(nm, -1);
+    for (size_t k = 0; k < nm; k++) {
+        if (next[k] < 0 || next[k] >= (int32_t)nm || next[k] == (int32_t)k || prv[next[k]] >= 0) {
+            a->err = "lbib_adj_set_problem: next[] must describe closed rings of at least two markers";
+            return -1;
+        }
+        prv[next[k]] = (int32_t)k;
+    }
+    for (size_t i = 0; i < n; i++)
+        if ((int)dom[i] > a->ncell) {
+            a->err = "lbib_adj_set_problem: cell map entry above n_cells";
+            return -1;
+        }
+    // markers stay where the periodic stencil does not straddle the edge
+    for (size_t k = 0; k < nm; k++)
+        if (!(xy0[2 * k] >= 3.0 && xy0[2 * k] <= a->cfg.nx - 4.0 && xy0[2 * k + 1] >= 3.0 &&
+              xy0[2 * k + 1] <= a->cfg.ny - 4.0)) {
+            a->err = "lbib_adj_set_problem: markers must start at least 3 sites inside the lattice";
+            return -1;
+        }
+    const bool f64 = a->prec == LBIB_FP64;
+    if ((f64 ? upload_T<double>(a, a->f0, f0, 9 * n) : upload_T<float>(a, a->f0, f0, 9 * n)) ||
+        (f64 ? upload_T<double>(a, a->g0, g0, 5 * n) : upload_T<float>(a, a->g0, g0, 5 * n)))
+        return -1;
+    if (nm > 0) {
+        if (f64 ? upload_T<double>(a, a->X0, xy0, 2 * nm) : upload_T<float>(a, a->X0, xy0, 2 * nm)) return -1;
+        ADJ_HIP(hipMemcpy(a->nxt, next, nm * 4, hipMemcpyHostToDevice));
+        ADJ_HIP(hipMemcpy(a->prv, prv.data(), nm * 4, hipMemcpyHostToDevice));
+    }
+    ADJ_HIP(hipMemcpy(a->dom, dom, n * 4, hipMemcpyHostToDevice));
+    ADJ_HIP(hipMemcpy(a->target, target, n * 8, hipMemcpyHostToDevice));
+    a->have_problem = true;
+    a->have_forward = false;
+    return 0;
+}
+
+int lbib_adj_forward(lbib_adj* a, int steps, double tau_c, double kappa, const double* rates, double* loss,
+                     double* c_final) {
+    if (!a || !loss || steps < 1) return -1;
+    if (!a->have_problem) { a->err = "lbib_adj_forward: no problem set"; return -1; }
+    if (!(tau_c > 0.5)) { a->err = "lbib_adj_forward: tau_c must exceed 0.5"; return -1; }
+    a->tau_c = tau_c;
+    a->kappa = kappa;
+    for (int c = 0; c <= a->ncell; c++) a->rates[c] = rates && c > 0 ? rates[c] : 0.0;
+    ADJ_HIP(hipMemcpy(a->src, a->rates.data(), (a->ncell + 1) * 8, hipMemcpyHostToDevice));
+    const int r = a->prec == LBIB_FP64 ? forward_t<double>(a, steps, loss, c_final)
+                                       : forward_t<float>(a, steps, loss, c_final);
+    a->have_forward = r == 0;
+    return r;
+}
+
+int lbib_adj_gradient(lbib_adj* a, double* d_tau_c, double* d_kappa, double* d_rates) {
+    if (!a) return -1;
+    if (!a->have_forward) { a->err = "lbib_adj_gradient: run lbib_adj_forward first"; return -1; }
+    const int r = a->prec == LBIB_FP64 ? gradient_t<double>(a, d_tau_c, d_kappa, d_rates)
+                                       : gradient_t<float>(a, d_tau_c, d_kappa, d_rates);
+    // the loss adjoint in lg[0] was consumed: a new forward is needed
+    a->have_forward = false;
+    return r;
+}
+
+int lbib_adj_timing(lbib_adj* a, double* forward_ms, double* gradient_ms) {
+    if (!a) return -1;
+    if (forward_ms) *forward_ms = a->fwd_ms;
+    if (gradient_ms) *gradient_ms = a->grad_ms;
+    return 0;
+}
+
+}  // extern "C"

@@ -150,3 +150,35 @@ def batched_ensemble(ndom=256, lmin=64, lmax=512, cells=(1, 16), radius=(8.0, 30
         domains.append(ids)
     return dict(nx=width, ny=_ceil16(height), rects=rects, nodes=nodes, conns=conns, domains=domains,
                 cells=allcells)
+
+
+def inside_polygon(px, py, poly):
+    """Even-odd test of points (px, py) (arrays) against a closed polygon (m, 2)."""
+    x, y = poly[:, 0], poly[:, 1]
+    xj, yj = np.roll(x, 1), np.roll(y, 1)
+    inside = np.zeros(np.shape(px), dtype=bool)
+    for k in range(len(x)):
+        cond = (y[k] > py) != (yj[k] > py)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            xc = (xj[k] - x[k]) * (py - y[k]) / (yj[k] - y[k]) + x[k]
+        inside ^= cond & (px < xc)
+    return inside
+
+
+def adjoint_case(nx=512, ny=512, cols=3, rows=3, radius=40.0, n=150, rate=(2e-4, 1e-3), seed=1):
+    """cols x rows round cells for the adjoint: marker rings (counter-
+    clockwise), the cell map of the sites inside each ring (cells 1..), and
+    one hidden source rate per cell drawn uniformly from `rate`."""
+    rng = np.random.default_rng(seed)
+    px, py = nx / cols, ny / rows
+    rings = []
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    dom = np.zeros((ny, nx), dtype=np.uint32)
+    for j in range(rows):
+        for i in range(cols):
+            nodes, _ = circle_cell((i + 0.5) * px, (j + 0.5) * py, radius, n, rng, jitter=0.5)
+            ring = np.array([[x, y] for _, x, y in nodes])
+            rings.append(ring)
+            dom[inside_polygon(xx, yy, ring)] = len(rings)
+    rates = rng.uniform(rate[0], rate[1], size=len(rings))
+    return rings, dom, rates

@@ -1,0 +1,205 @@
+"""Hand-written adjoint of the coupled D2Q9 + D2Q5 + IB step (lbib_adj_*).
+
+Chain of checks:
+- the adjoint's FP64 forward without markers is the oracle's lattice step
+  (SIGNAL species producing in every cell), bit for bit;
+- with markers its FP64 forward matches the torch restatement to 1e-12;
+- its FP64 gradient (dL/dtau_c, dL/dkappa, dL/ds_c) matches torch.autograd of
+  the restatement to 1e-9 relative and central finite differences of its own
+  forward to 1e-5 relative;
+- the gradient does not depend on the checkpoint interval;
+- the FP32 gradient matches the FP64 one to 1e-3 relative;
+- the torch.autograd.Function wrapper returns the adjoint's gradient."""
+import numpy as np
+import pytest
+
+from lbibcell_amd import adjoint, capi, workloads
+
+pytestmark = pytest.mark.gpu
+
+W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
+
+
+def _small(seed=3, nx=40, ny=36, n=32, radius=8.0):
+    rings, dom, rates = workloads.adjoint_case(nx, ny, 1, 1, radius, n, seed=seed)
+    rng = np.random.default_rng(seed)
+    f0 = W[:, None, None] * (1 + 0.02 * rng.standard_normal((9, ny, nx)))
+    g0 = 0.1 * (1 + 0.1 * rng.standard_normal((5, ny, nx)))
+    target = 0.5 * (1 + 0.1 * rng.standard_normal((ny, nx)))
+    return dict(nx=nx, ny=ny, rings=rings, dom=dom, rates=rates, f0=f0, g0=g0, target=target)
+
+
+def _problem(case, precision=capi.FP64, K=7, tau_f=0.9, decay=1e-3):
+    return adjoint.AdjointProblem(case["nx"], case["ny"], tau_f, case["rings"], case["dom"], target=case["target"],
+                                  f0=case["f0"], g0=case["g0"], decay=decay, precision=precision,
+                                  checkpoint_every=K)
+
+
+PARAMS = dict(tau_c=0.85, kappa=0.02)
+
+
+def test_forward_without_markers_is_the_oracle_step(orc):
+    nx, ny, tau_f, tau_c, prod, decay = 30, 26, 0.9, 0.8, 1e-3, 1e-3
+    rng = np.random.default_rng(1)
+    dom = np.zeros((ny, nx), dtype=np.uint32)
+    dom[5:15, 6:20] = 1
+    dom[17:24, 3:12] = 2
+    f0 = W[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny, nx)))
+    g0 = 0.1 * (1 + 0.1 * rng.standard_normal((5, ny, nx)))
+    p = adjoint.AdjointProblem(nx, ny, tau_f, [], dom, target=np.zeros((ny, nx)), f0=f0, g0=g0, decay=decay,
+                               precision=capi.FP64, checkpoint_every=4)
+    steps = 9
+    loss, C = p.forward(steps, tau_c, 0.0, [prod, prod], want_field=True)
+    o = orc.Oracle(nx, ny, tau_f)
+    o.add_species(orc.SP_SIGNAL, tau_c, [prod, decay, 10 ** 6, 0.0, -1])
+    o.set_domains(dom)
+    o.init_solvers()
+    o.set_f(f0)
+    o.set_g(0, g0)
+    for it in range(steps):
+        o.set_iteration(it)
+        o.lattice_step(1)
+    # the oracle stores locally swapped slots: the concentration is slot-order independent
+    Co = o.get_g(0).sum(axis=0)
+    print("case adjoint forward vs oracle: max |dC|", float(np.max(np.abs(C - Co))), flush=True)
+    np.testing.assert_allclose(C, Co, rtol=0, atol=1e-15)
+    assert loss == pytest.approx(float((Co ** 2).sum()), rel=1e-14)
+    p.close()
+
+
+def _torch_problem(case, tau_f=0.9, decay=1e-3):
+    import torch
+    nxt, prv, xy = [], [], []
+    base = 0
+    for r in case["rings"]:
+        m = len(r)
+        nxt += [base + (k + 1) % m for k in range(m)]
+        prv += [base + (k - 1) % m for k in range(m)]
+        xy.append(r)
+        base += m
+    return dict(nx=case["nx"], ny=case["ny"], tau_f=tau_f, decay=decay, f0=case["f0"], g0=case["g0"],
+                xy=np.concatenate(xy), nxt=nxt, prv=prv, dom=case["dom"].astype(np.int64),
+                target=case["target"]), torch
+
+
+def test_fp64_gradient_matches_torch_autograd():
+    import adjoint_ref
+    case = _small()
+    steps = 12
+    prob, torch = _torch_problem(case)
+    tc = torch.tensor(PARAMS["tau_c"], dtype=torch.float64, requires_grad=True)
+    kp = torch.tensor(PARAMS["kappa"], dtype=torch.float64, requires_grad=True)
+    rt = torch.tensor(case["rates"], dtype=torch.float64, requires_grad=True)
+    L = adjoint_ref.loss(prob, steps, tc, kp, rt)
+    L.backward()
+    p = _problem(case)
+    loss = p.forward(steps, PARAMS["tau_c"], PARAMS["kappa"], case["rates"])
+    dt, dk, dr = p.gradient()
+    print(f"case adjoint vs autograd: loss {loss:.15e} / {L.item():.15e}", flush=True)
+    print(f"  dtau {dt:.10e} / {tc.grad.item():.10e}  dkappa {dk:.10e} / {kp.grad.item():.10e}", flush=True)
+    print(f"  drates {dr} / {rt.grad.numpy()}", flush=True)
+    assert loss == pytest.approx(L.item(), rel=1e-12)
+    assert dt == pytest.approx(tc.grad.item(), rel=1e-9)
+    assert dk == pytest.approx(kp.grad.item(), rel=1e-9, abs=1e-12 * abs(dt))
+    np.testing.assert_allclose(dr, rt.grad.numpy(), rtol=1e-9)
+    p.close()
+
+
+def _fd(p, steps, base, name, h):
+    args = dict(base)
+    args[name] = base[name] + h
+    lp = p.forward(steps, args["tau_c"], args["kappa"], args["rates"])
+    args[name] = base[name] - h
+    lm = p.forward(steps, args["tau_c"], args["kappa"], args["rates"])
+    return (lp - lm) / (2 * h)
+
+
+def test_fp64_gradient_matches_central_differences():
+    case = _small(seed=5)
+    steps = 40
+    p = _problem(case, K=9)
+    base = dict(PARAMS, rates=case["rates"].copy())
+    p.forward(steps, base["tau_c"], base["kappa"], base["rates"])
+    dt, dk, dr = p.gradient()
+    fd_t = _fd(p, steps, base, "tau_c", 1e-5)
+    fd_k = _fd(p, steps, base, "kappa", 1e-6)
+    rp = dict(base)
+    h = 1e-7
+
+    def with_rate(v):
+        r = base["rates"].copy()
+        r[0] = v
+        return p.forward(steps, base["tau_c"], base["kappa"], r)
+    fd_r = (with_rate(base["rates"][0] + h) - with_rate(base["rates"][0] - h)) / (2 * h)
+    print(f"case adjoint vs FD: dtau {dt:.10e} fd {fd_t:.10e}; dkappa {dk:.10e} fd {fd_k:.10e}; "
+          f"drate {dr[0]:.10e} fd {fd_r:.10e}", flush=True)
+    assert dt == pytest.approx(fd_t, rel=1e-5)
+    assert dk == pytest.approx(fd_k, rel=1e-5)
+    assert dr[0] == pytest.approx(fd_r, rel=1e-5)
+    del rp
+    p.close()
+
+
+def test_gradient_does_not_depend_on_the_checkpoint_interval():
+    case = _small(seed=7)
+    steps = 23
+    out = []
+    for K in (1, 5, 23, 40):
+        p = _problem(case, K=K)
+        p.forward(steps, PARAMS["tau_c"], PARAMS["kappa"], case["rates"])
+        out.append(p.gradient())
+        p.close()
+    for dt, dk, dr in out[1:]:
+        assert dt == pytest.approx(out[0][0], rel=1e-12)
+        assert dk == pytest.approx(out[0][1], rel=1e-10)
+        np.testing.assert_allclose(dr, out[0][2], rtol=1e-12)
+
+
+def test_fp32_gradient_close_to_fp64():
+    rings, dom, rates = workloads.adjoint_case(128, 128, 3, 3, 14.0, 60, seed=2)
+    case = dict(nx=128, ny=128, rings=rings, dom=dom, rates=rates, f0=None, g0=None, target=None)
+    steps = 200
+    res = {}
+    for prec in (capi.FP64, capi.FP32):
+        p = adjoint.AdjointProblem(128, 128, 1.0, rings, dom, decay=1e-5, precision=prec, checkpoint_every=50)
+        # target from hidden parameters, evaluated at a guess
+        _, tgt = p.forward(steps, 0.8, 0.01, rates, want_field=True)
+        p.set_target(tgt)
+        p.forward(steps, 0.9, 0.006, rates * 0.7)
+        res[prec] = p.gradient()
+        print("case fp32 vs fp64 gradient", prec, res[prec][0], res[prec][1], res[prec][2][:3],
+              "ms fwd/grad", p.timing(), flush=True)
+        p.close()
+    g64, g32 = res[capi.FP64], res[capi.FP32]
+    assert g32[0] == pytest.approx(g64[0], rel=1e-3)
+    assert g32[1] == pytest.approx(g64[1], rel=1e-2)
+    np.testing.assert_allclose(g32[2], g64[2], rtol=1e-3)
+    del case
+
+
+def test_torch_function_wrapper():
+    import torch
+    case = _small(seed=9)
+    p = _problem(case)
+    Fn = adjoint.coupled_loss_function()
+    tc = torch.tensor(PARAMS["tau_c"], dtype=torch.float64, requires_grad=True)
+    kp = torch.tensor(PARAMS["kappa"], dtype=torch.float64, requires_grad=True)
+    rt = torch.tensor(case["rates"], dtype=torch.float64, requires_grad=True)
+    L = Fn.apply(tc, kp, rt, p, 15)
+    (3.0 * L).backward()
+    p.forward(15, PARAMS["tau_c"], PARAMS["kappa"], case["rates"])
+    dt, dk, dr = p.gradient()
+    assert tc.grad.item() == pytest.approx(3.0 * dt, rel=1e-14)
+    assert kp.grad.item() == pytest.approx(3.0 * dk, rel=1e-14)
+    np.testing.assert_allclose(rt.grad.numpy(), 3.0 * dr, rtol=1e-14)
+    p.close()
+
+
+def test_problem_validation():
+    case = _small()
+    with pytest.raises(capi.LbibError):
+        adjoint.AdjointProblem(40, 36, 0.9, [np.array([[1.0, 1.0], [2.0, 1.0], [2.0, 2.0]])], case["dom"])
+    p = _problem(case)
+    with pytest.raises(capi.LbibError):
+        p.gradient()
+    p.close()
