@@ -170,6 +170,57 @@ def measure_batched(ens, prec_name, steps, warmup, run_ahead):
     return out
 
 
+def measure_adjoint(steps, K, size=512):
+    """Adjoint workload: size^2 lattice, 3x3 cells (radius 40, 150 markers),
+    the target is the final concentration under hidden parameters (tau_c 0.8,
+    kappa 0.01, per-cell rates); loss and gradient are evaluated at a guess
+    (tau_c 0.9, kappa 0.006, 0.7 x rates).  FP32 is the measured path, FP64
+    its gradient reference; central differences check the FP64 gradient."""
+    from lbibcell_amd import adjoint, capi, workloads
+    rings, dom, rates = workloads.adjoint_case(size, size, 3, 3, 40.0, 150, seed=4)
+    guess = (0.9, 0.006, rates * 0.7)
+    out = {}
+    for name, prec in (("fp32", capi.FP32), ("fp64", capi.FP64)):
+        p = adjoint.AdjointProblem(size, size, 1.0, rings, dom, decay=1e-5, precision=prec, checkpoint_every=K)
+        _, tgt = p.forward(steps, 0.8, 0.01, rates, want_field=True)
+        p.set_target(tgt)
+        p.forward(steps, *guess)          # warm-up of the timed pair
+        p.gradient()
+        loss = p.forward(steps, *guess)
+        g = p.gradient()
+        fwd_ms, grad_ms = p.timing()
+        r = {"loss": loss, "d_tau_c": g[0], "d_kappa": g[1], "d_rates": list(g[2]), "forward_ms": fwd_ms,
+             "gradient_ms": grad_ms, "adjoint_to_forward": grad_ms / fwd_ms}
+        if name == "fp64":
+            def fd(idx, h):
+                vals = []
+                for sgn in (1, -1):
+                    tc, kp, rt = guess[0], guess[1], guess[2].copy()
+                    if idx == 0:
+                        tc += sgn * h
+                    elif idx == 1:
+                        kp += sgn * h
+                    else:
+                        rt[0] += sgn * h
+                    vals.append(p.forward(steps, tc, kp, rt))
+                return (vals[0] - vals[1]) / (2 * h)
+            fds = [fd(0, 1e-5), fd(1, 1e-6), fd(2, 1e-7)]
+            r["central_difference_rel_error"] = {
+                "tau_c": abs(g[0] - fds[0]) / abs(fds[0]), "kappa": abs(g[1] - fds[1]) / abs(fds[1]),
+                "rate_cell1": abs(g[2][0] - fds[2]) / abs(fds[2])}
+        out[name] = r
+        p.close()
+    g32, g64 = out["fp32"], out["fp64"]
+    out["fp32_vs_fp64_rel_error"] = {
+        "tau_c": abs(g32["d_tau_c"] - g64["d_tau_c"]) / abs(g64["d_tau_c"]),
+        "kappa": abs(g32["d_kappa"] - g64["d_kappa"]) / abs(g64["d_kappa"]),
+        "rates": float(np.max(np.abs(np.array(g32["d_rates"]) - np.array(g64["d_rates"]))
+                              / np.abs(np.array(g64["d_rates"]))))}
+    return {"workload": f"adjoint {size}x{size}, 9 cells r=40 (150 markers), {steps} steps, checkpoints every {K}, "
+                        "loss = L2 mismatch of the final concentration against a target from hidden parameters",
+            "steps": steps, "checkpoint_every": K, **out}
+
+
 def l2_rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
 
@@ -192,6 +243,7 @@ def main():
                          "fp64 also serves as the reference of the mixed errors")
     ap.add_argument("--batched-domains", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--adjoint-steps", type=int, default=1000, help="0 = skip the adjoint measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice path runs on a single MI355X per process; "
@@ -232,7 +284,7 @@ def main():
     head = res[args.precision]
     times = head["times"]
     out = {
-        "metric": "MLUPS (coupled D2Q9+D2Q5+IB lattice step)",
+        "metric": "MLUPS + %HBM roofline (plain vs mixed/batched); adjoint-to-forward time ratio",
         "value": n * args.steps / head["dt"] / 1e6,
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -306,6 +358,10 @@ def main():
                     "after_steps": 5 + args.batched_steps}
             batched[p] = e
         out["batched"] = batched
+    if args.adjoint_steps > 0:
+        log(f"adjoint: {args.adjoint_steps} steps ...")
+        out["adjoint"] = measure_adjoint(args.adjoint_steps, 50)
+        log(f"adjoint/forward time ratio (fp32): {out['adjoint']['fp32']['adjoint_to_forward']:.2f}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(nx, ny, nodes, conns)
