@@ -236,6 +236,13 @@ int lbib_compute_areas(lbib_ctx* ctx, double* area);
  * areas -> clamped proportional source per domain -> addMass on every site of
  * a cell (MAXSOURCE for cell type 1).  target_factor > 0 sets the target area
  * of a new domain to factor * its first area instead of target_area. */
+/* GeometryHandler::computeAccumulatedDomainConcentrations
+ * (GeometryHandler.cpp:136-147): out[d] = sum of getC() of species s over the
+ * sites of domain d (d < ndom), x from 1 as the reference loops.  The sums
+ * are accumulated per run of equal domain along rows, so the rounding order
+ * differs from the reference's site-by-site sum (relative difference at the
+ * 1e-15 level).  Synchronises. */
+int lbib_domain_concentrations(lbib_ctx* ctx, int s, int ndom, double* out);
 int lbib_area_regulator(lbib_ctx* ctx, double target_area, double kp, double maxsource,
                         double target_factor);
 /* tutorial_01_BioSolverMembraneTension (…MembraneTension.cpp:42-77) as one

@@ -158,6 +158,7 @@ SIGNATURES = {
     "lbib_autotune": (C.c_int, [_P, C.c_int, C.POINTER(LatticeShape), C.c_int, C.POINTER(C.c_int),
                                 C.POINTER(LatticeShape)]),
     "lbib_set_lattice_shape": (C.c_int, [_P, C.POINTER(LatticeShape)]),
+    "lbib_domain_concentrations": (C.c_int, [_P, C.c_int, C.c_int, _D]),
     "lbib_capture_begin": (C.c_int, [_P]),
     "lbib_capture_end": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "lbib_graph_launch": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int)]),
@@ -358,6 +359,13 @@ class Context:
 
     def device_bytes(self):
         return int(self.lib.lbib_device_bytes(self.h))
+
+    def domain_concentrations(self, s, ndom):
+        """Sum of species s's concentration per domain 0..ndom-1 (x from 1, as
+        GeometryHandler::computeAccumulatedDomainConcentrations)."""
+        out = np.zeros(ndom)
+        self._chk(self.lib.lbib_domain_concentrations(self.h, int(s), int(ndom), out.ctypes.data_as(_D)))
+        return out
 
     def set_batch_layout(self, rects):
         """rects: iterable of (x0, y0, w, h, tau_fluid, [tau_species...])."""

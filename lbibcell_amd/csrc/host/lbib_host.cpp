@@ -1124,6 +1124,30 @@ void SimulationRunner::runSimulation() {
     }
 }
 
+std::map<unsigned, double> SimulationRunner::computeAccumulatedDomainConcentrations(const std::string& name) {
+    const int k = speciesIndex(name);
+    if (k < 0) LBIB_FAIL("The CDE Solver with the name " + name + " does not exists.");
+    settle();
+    const size_t n = (size_t)params_.sizeX * params_.sizeY;
+    std::vector<uint32_t> dom(n);
+    check(lbib_get_domains(ctx_, dom.data()), "domains");
+    uint32_t maxd = 0;
+    std::vector<char> present;
+    for (size_t i = 0; i < n; i++) {
+        if (i % params_.sizeX == 0) continue;   // the reference loops x from 1
+        maxd = std::max(maxd, dom[i]);
+    }
+    present.assign(maxd + 1, 0);
+    for (size_t i = 0; i < n; i++)
+        if (i % params_.sizeX != 0) present[dom[i]] = 1;
+    std::vector<double> acc(maxd + 1);
+    check(lbib_domain_concentrations(ctx_, k, (int)maxd + 1, acc.data()), "domain concentrations");
+    std::map<unsigned, double> out;
+    for (uint32_t d = 0; d <= maxd; d++)
+        if (present[d]) out[d] = acc[d];
+    return out;
+}
+
 std::map<unsigned, double> SimulationRunner::computeAreas() {
     syncMirror();
     std::map<unsigned, int> present;

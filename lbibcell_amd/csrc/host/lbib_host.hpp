@@ -212,6 +212,8 @@ class SimulationRunner {
     // holds other force types)
     void renewTension(double force);
     std::map<unsigned, double> computeAreas();            // GeometryHandler.cpp:101-134 (device shoelace)
+    // GeometryHandler.cpp:136-147 (device per-domain sums, domains present on the lattice)
+    std::map<unsigned, double> computeAccumulatedDomainConcentrations(const std::string& name);
     StageTimes times;
     lbib_status lastStatus{};
     // Iterations enqueued before the device status is read (1 reads it after

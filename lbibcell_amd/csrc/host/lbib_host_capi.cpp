@@ -233,6 +233,20 @@ int lbh_set_run_ahead(void* p, unsigned n) {
         h->runner->runAhead = n < 1 ? 1 : n;
     });
 }
+// computeAccumulatedDomainConcentrations into out[0..cap) (domains absent: 0);
+// returns the number of entries written (largest domain + 1)
+int lbh_domain_concentrations(void* p, const char* name, double* out, int cap) {
+    auto* h = static_cast<Handle*>(p);
+    int written = -1;
+    const int rc = guarded(h, [&] {
+        auto m = h->runner->computeAccumulatedDomainConcentrations(name);
+        written = m.empty() ? 0 : (int)m.rbegin()->first + 1;
+        for (int d = 0; d < cap; d++) out[d] = 0.0;
+        for (auto& kv : m)
+            if ((int)kv.first < cap) out[kv.first] = kv.second;
+    });
+    return rc != 0 ? -1 : written;
+}
 int lbh_set_graph_steps(void* p, unsigned n) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {

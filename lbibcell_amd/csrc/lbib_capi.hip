@@ -275,9 +275,10 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     LBIB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     const size_t sb = storage_bytes(c->prec);
     const long n = c->n;
+    const long npad = (n + 63) / 64 * 64;   // whole 64-site blocks for 32- and 16-bit storage
     for (int b = 0; b < 2; b++) {
-        if (dev_alloc(c, (uint8_t**)&c->f[b], 9 * n * sb)) return -1;
-        if (dev_alloc(c, (uint8_t**)&c->g[b], (size_t)std::max(c->ns, 1) * 5 * n * sb)) return -1;
+        if (dev_alloc(c, (uint8_t**)&c->f[b], 9 * npad * sb)) return -1;
+        if (dev_alloc(c, (uint8_t**)&c->g[b], (size_t)std::max(c->ns, 1) * 5 * npad * sb)) return -1;
     }
     if (dev_alloc(c, &c->u, 2 * n) || dev_alloc(c, &c->Fx, n) || dev_alloc(c, &c->Fy, n) ||
         dev_alloc(c, &c->msite, n))
@@ -334,7 +335,7 @@ void lbib_destroy(lbib_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
-    dev_free(c->rtile); dev_free(c->rects);
+    dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
     dev_free(c->tslot); dev_free(c->tlist);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
@@ -849,6 +850,22 @@ int lbib_compute_areas(lbib_ctx* c, double* area) {
     }
     LBIB_HIP(hipStreamSynchronize(c->stream));
     return 0;
+}
+int lbib_domain_concentrations(lbib_ctx* c, int species, int ndom, double* out) {
+    if (check_ctx(c)) return -1;
+    if (species < 0 || species >= c->ns || ndom < 1 || !out) {
+        c->err = "lbib_domain_concentrations: bad species index or output size";
+        return -1;
+    }
+    if (ndom > c->dconc_cap) {
+        dev_free(c->dconc);
+        c->dconc_cap = 0;
+        if (dev_alloc(c, &c->dconc, ndom)) return -1;
+        c->dconc_cap = ndom;
+    }
+    LBIB_HIP(hipMemsetAsync(c->dconc, 0, ndom * sizeof(double), c->stream));
+    if (launch_domain_concentrations(c, species, ndom, c->dconc)) return -1;
+    return download(c, out, c->dconc, ndom * sizeof(double));
 }
 int lbib_area_regulator(lbib_ctx* c, double target_area, double kp, double maxsource, double target_factor) {
     if (check_ctx(c)) return -1;

@@ -244,11 +244,12 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         const int yy = fr.y0 + (y - fr.y0 + EYd[d] + fr.h) % fr.h;
         return (long)yy * a.nx + xx;
     };
-    const long sn = 5 * a.n;
+    constexpr int GQ = NS > 0 ? 5 * NS : 5;   // species planes in the storage blocks
+    auto gat = [&](int k, int q, long j) -> long { return pidx<P>(5 * k + q, j, a.n, GQ); };
     auto cswap = [&](int k, long j) -> double {
-        const T* gk = gb + k * sn;
-        return c_swapped((double)CD::dec(gk[j]), (double)CD::dec(gk[a.n + j]), (double)CD::dec(gk[2 * a.n + j]),
-                         (double)CD::dec(gk[3 * a.n + j]), (double)CD::dec(gk[4 * a.n + j]));
+        return c_swapped((double)CD::dec(gb[gat(k, 0, j)]), (double)CD::dec(gb[gat(k, 1, j)]),
+                         (double)CD::dec(gb[gat(k, 2, j)]), (double)CD::dec(gb[gat(k, 3, j)]),
+                         (double)CD::dec(gb[gat(k, 4, j)]));
     };
     double ux_i, uy_i;
     if (P == LBIB_FP64) {
@@ -289,8 +290,7 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         for (int j = 0; j < NS; j++) cj[j] = (j == k) ? C : cswap(j, i);
         const double tI = fr.r < 0 ? a.sp[k].tauI : a.rects[fr.r].tauIs[k];
         species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], tI, nd, iter_of(a.it, a.st), cj, cj);
-        T* gk = gb + k * sn;
-        for (int q = 0; q < 5; q++) gk[q * a.n + i] = CD::enc(d[q]);
+        for (int q = 0; q < 5; q++) gb[gat(k, q, i)] = CD::enc(d[q]);
     }
 }
 

@@ -152,6 +152,8 @@ struct lbib_ctx {
     lbib::RectDev* rects = nullptr;
     long live_sites = 0;               // sites inside some rectangle (all sites without a layout)
     bool rect_pairs = true;            // every rectangle width even: paired-lane kernel applies
+    double* dconc = nullptr;           // per-domain concentration sums (lbib_domain_concentrations)
+    int dconc_cap = 0;
     // lattice launch shape (lbib_autotune / lbib_set_lattice_shape)
     int lat_variant = -1, lat_wg = 256, lat_spt = 2;
     // spread over 16x16 tiles
@@ -340,6 +342,8 @@ struct AreaMass {
 // status is stopped (an iteration's BioSolver stage)
 int launch_ring_areas(lbib_ctx* c, const AreaMass* mass);
 int launch_remesh(lbib_ctx* c, int rounds);   // also clears the ring counters
+// per-domain sum of one species' concentration into acc[ndom] (zeroed by the caller)
+int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc);
 int launch_coarsen(lbib_ctx* c);
 int launch_forces_clear(lbib_ctx* c);
 int launch_forces_tension(lbib_ctx* c, double f);

@@ -180,7 +180,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
     const long r0 = (long)y * a.nx;
     const T* fs = static_cast<const T*>(a.fs);
     const T* gsb = static_cast<const T*>(a.gs);
-    const long sn = 5 * n;
+    constexpr int GQ = NS > 0 ? 5 * NS : 5;   // species planes in the storage blocks
 
     bool live[SPT];
     int xs[SPT];
@@ -202,23 +202,22 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
         const long rN = (long)((y == fr[k].y0 + fr[k].h - 1) ? fr[k].y0 : y + 1) * a.nx;
         const long i = r0 + xx;
         // pulled deviations d_q = f_q - w_q
-        d[k][DT] = C::dec(fs[i]);
-        d[k][DE] = C::dec(fs[1 * n + r0 + xm]);
-        d[k][DN] = C::dec(fs[2 * n + rS + xx]);
-        d[k][DW] = C::dec(fs[3 * n + r0 + xp]);
-        d[k][DS] = C::dec(fs[4 * n + rN + xx]);
-        d[k][DNE] = C::dec(fs[5 * n + rS + xm]);
-        d[k][DNW] = C::dec(fs[6 * n + rS + xp]);
-        d[k][DSW] = C::dec(fs[7 * n + rN + xp]);
-        d[k][DSE] = C::dec(fs[8 * n + rN + xm]);
+        d[k][DT] = C::dec(fs[pidx<P>(DT, i, n, 9)]);
+        d[k][DE] = C::dec(fs[pidx<P>(DE, r0 + xm, n, 9)]);
+        d[k][DN] = C::dec(fs[pidx<P>(DN, rS + xx, n, 9)]);
+        d[k][DW] = C::dec(fs[pidx<P>(DW, r0 + xp, n, 9)]);
+        d[k][DS] = C::dec(fs[pidx<P>(DS, rN + xx, n, 9)]);
+        d[k][DNE] = C::dec(fs[pidx<P>(DNE, rS + xm, n, 9)]);
+        d[k][DNW] = C::dec(fs[pidx<P>(DNW, rS + xp, n, 9)]);
+        d[k][DSW] = C::dec(fs[pidx<P>(DSW, rN + xp, n, 9)]);
+        d[k][DSE] = C::dec(fs[pidx<P>(DSE, rN + xm, n, 9)]);
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            const T* gs = gsb + s * sn;
-            g[k][s][DT] = C::dec(gs[i]);
-            g[k][s][DE] = C::dec(gs[1 * n + r0 + xm]);
-            g[k][s][DN] = C::dec(gs[2 * n + rS + xx]);
-            g[k][s][DW] = C::dec(gs[3 * n + r0 + xp]);
-            g[k][s][DS] = C::dec(gs[4 * n + rN + xx]);
+            g[k][s][DT] = C::dec(gsb[pidx<P>(5 * s + DT, i, n, GQ)]);
+            g[k][s][DE] = C::dec(gsb[pidx<P>(5 * s + DE, r0 + xm, n, GQ)]);
+            g[k][s][DN] = C::dec(gsb[pidx<P>(5 * s + DN, rS + xx, n, GQ)]);
+            g[k][s][DW] = C::dec(gsb[pidx<P>(5 * s + DW, r0 + xp, n, GQ)]);
+            g[k][s][DS] = C::dec(gsb[pidx<P>(5 * s + DS, rN + xx, n, GQ)]);
         }
         tg[k] = a.tag[i];
         Fx[k] = 0.0f;
@@ -268,9 +267,10 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
                 const int desc = a.bc_desc[e];
                 const int b0 = a.bcd_off[desc], b1 = a.bcd_off[desc + 1];
                 if (b1 <= b0) continue;
-                const T* gs = gsb + a.bcd_sp[b0] * sn;
-                const float Cp = C::dec(gs[i]) + C::dec(gs[1 * n + i]) + C::dec(gs[2 * n + i]) +
-                                 C::dec(gs[3 * n + i]) + C::dec(gs[4 * n + i]);
+                const int s1 = 5 * a.bcd_sp[b0];
+                const float Cp = C::dec(gsb[pidx<P>(s1, i, n, GQ)]) + C::dec(gsb[pidx<P>(s1 + 1, i, n, GQ)]) +
+                                 C::dec(gsb[pidx<P>(s1 + 2, i, n, GQ)]) + C::dec(gsb[pidx<P>(s1 + 3, i, n, GQ)]) +
+                                 C::dec(gsb[pidx<P>(s1 + 4, i, n, GQ)]);
                 const float c6 = Cp / 6.0f;
                 const float bvx = (float)a.bc_v[2 * e], bvy = (float)a.bc_v[2 * e + 1];
                 float feq, eq;
@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
                 else if (dd == DN) { feq = c6 * (1.0f + 3.0f * bvy); eq = c6 * (1.0f + 3.0f * pv[k]); }
                 else if (dd == DW) { feq = c6 * (1.0f - 3.0f * bvx); eq = c6 * (1.0f - 3.0f * pu[k]); }
                 else { feq = c6 * (1.0f - 3.0f * bvy); eq = c6 * (1.0f - 3.0f * pv[k]); }
-                float carry = feq + (C::dec(gs[(long)dd * n + i]) - eq);
+                float carry = feq + (C::dec(gsb[pidx<P>(s1 + dd, i, n, GQ)]) - eq);
                 for (int kk = b0; kk < b1; kk++) {
                     const int sk = a.bcd_sp[kk];
 #pragma unroll
@@ -327,11 +327,12 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
 
 #pragma unroll
         for (int q = 0; q < 9; q++)
-            __builtin_nontemporal_store(C::enc(C::dev ? d[k][q] : d[k][q] + wq(q)), &fd[(long)q * n + i]);
+            __builtin_nontemporal_store(C::enc(C::dev ? d[k][q] : d[k][q] + wq(q)), &fd[pidx<P>(q, i, n, 9)]);
 #pragma unroll
         for (int s = 0; s < NS; s++)
 #pragma unroll
-            for (int q = 0; q < 5; q++) __builtin_nontemporal_store(C::enc(g[k][s][q]), &gd[s * sn + (long)q * n + i]);
+            for (int q = 0; q < 5; q++)
+                __builtin_nontemporal_store(C::enc(g[k][s][q]), &gd[pidx<P>(5 * s + q, i, n, GQ)]);
         *reinterpret_cast<float2*>(a.u + 2 * i) = make_float2(uxn, uyn);
     }
 }
@@ -345,15 +346,18 @@ template <int P>
 struct PairLoad {
     using T = uint16_t;
     const T* base;
-    long rowo;          // row offset (population block + row start)
+    long r, n;               // row start site, sites
+    int q, nq;               // plane, planes per storage block
     int x0, fx0, fw, lane;   // pair start; periodic frame [fx0, fx0 + fw) along x
     bool lo_edge, hi_edge;
-    __device__ uint32_t aligned() const { return *reinterpret_cast<const uint32_t*>(base + rowo + x0); }
+    __device__ long at(int x) const { return pidx<P>(q, r + x, n, nq); }
+    // the pair (x0, x0 + 1) is contiguous: x0 and the row start are even
+    __device__ uint32_t aligned() const { return *reinterpret_cast<const uint32_t*>(base + at(x0)); }
     // values at x0-1, x0 (pull from the west)
     __device__ void west(float& a0, float& a1) const {
         const uint32_t own = aligned();
         uint32_t prev = (uint32_t)__shfl_up((int)own, 1, 64);
-        const T lo = lo_edge ? base[rowo + (x0 == fx0 ? fx0 + fw - 1 : x0 - 1)] : (T)(prev >> 16);
+        const T lo = lo_edge ? base[at(x0 == fx0 ? fx0 + fw - 1 : x0 - 1)] : (T)(prev >> 16);
         a0 = Codec<P>::dec(lo);
         a1 = Codec<P>::dec((T)(own & 0xffffu));
     }
@@ -361,7 +365,7 @@ struct PairLoad {
     __device__ void east(float& a0, float& a1) const {
         const uint32_t own = aligned();
         uint32_t next = (uint32_t)__shfl_down((int)own, 1, 64);
-        const T hi = hi_edge ? base[rowo + (x0 + 2 >= fx0 + fw ? x0 + 2 - fw : x0 + 2)] : (T)(next & 0xffffu);
+        const T hi = hi_edge ? base[at(x0 + 2 >= fx0 + fw ? x0 + 2 - fw : x0 + 2)] : (T)(next & 0xffffu);
         a0 = Codec<P>::dec((T)(own >> 16));
         a1 = Codec<P>::dec(hi);
     }
@@ -394,32 +398,31 @@ __global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
     const long rN = (long)((y == fr.y0 + fr.h - 1) ? fr.y0 : y + 1) * a.nx;
     const T* fs = static_cast<const T*>(a.fs);
     const T* gsb = static_cast<const T*>(a.gs);
-    const long sn = 5 * n;
+    constexpr int GQ = NS > 0 ? 5 * NS : 5;   // species planes in the storage blocks
     const bool lo_edge = lane == 0 || xl == fr.x0;
     const bool hi_edge = lane == 63 || xl + 2 >= fr.x0 + fr.w;
-    auto pl = [&](const T* base, long rowo) {
-        return PairLoad<P>{base, rowo, xl, fr.x0, fr.w, lane, lo_edge, hi_edge};
+    auto pl = [&](const T* base, int q, int nq, long row) {
+        return PairLoad<P>{base, row, n, q, nq, xl, fr.x0, fr.w, lane, lo_edge, hi_edge};
     };
 
     float d[2][9];
-    pl(fs, 0 * n + r0).here(d[0][DT], d[1][DT]);
-    pl(fs, 1 * n + r0).west(d[0][DE], d[1][DE]);
-    pl(fs, 2 * n + rS).here(d[0][DN], d[1][DN]);
-    pl(fs, 3 * n + r0).east(d[0][DW], d[1][DW]);
-    pl(fs, 4 * n + rN).here(d[0][DS], d[1][DS]);
-    pl(fs, 5 * n + rS).west(d[0][DNE], d[1][DNE]);
-    pl(fs, 6 * n + rS).east(d[0][DNW], d[1][DNW]);
-    pl(fs, 7 * n + rN).east(d[0][DSW], d[1][DSW]);
-    pl(fs, 8 * n + rN).west(d[0][DSE], d[1][DSE]);
+    pl(fs, DT, 9, r0).here(d[0][DT], d[1][DT]);
+    pl(fs, DE, 9, r0).west(d[0][DE], d[1][DE]);
+    pl(fs, DN, 9, rS).here(d[0][DN], d[1][DN]);
+    pl(fs, DW, 9, r0).east(d[0][DW], d[1][DW]);
+    pl(fs, DS, 9, rN).here(d[0][DS], d[1][DS]);
+    pl(fs, DNE, 9, rS).west(d[0][DNE], d[1][DNE]);
+    pl(fs, DNW, 9, rS).east(d[0][DNW], d[1][DNW]);
+    pl(fs, DSW, 9, rN).east(d[0][DSW], d[1][DSW]);
+    pl(fs, DSE, 9, rN).west(d[0][DSE], d[1][DSE]);
     float g[2][NS > 0 ? NS : 1][5];
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        const T* gs = gsb + s * sn;
-        pl(gs, 0 * n + r0).here(g[0][s][DT], g[1][s][DT]);
-        pl(gs, 1 * n + r0).west(g[0][s][DE], g[1][s][DE]);
-        pl(gs, 2 * n + rS).here(g[0][s][DN], g[1][s][DN]);
-        pl(gs, 3 * n + r0).east(g[0][s][DW], g[1][s][DW]);
-        pl(gs, 4 * n + rN).here(g[0][s][DS], g[1][s][DS]);
+        pl(gsb, 5 * s + DT, GQ, r0).here(g[0][s][DT], g[1][s][DT]);
+        pl(gsb, 5 * s + DE, GQ, r0).west(g[0][s][DE], g[1][s][DE]);
+        pl(gsb, 5 * s + DN, GQ, rS).here(g[0][s][DN], g[1][s][DN]);
+        pl(gsb, 5 * s + DW, GQ, r0).east(g[0][s][DW], g[1][s][DW]);
+        pl(gsb, 5 * s + DS, GQ, rN).here(g[0][s][DS], g[1][s][DS]);
     }
     if (!live) return;
     const long i0 = r0 + x0;
@@ -454,9 +457,10 @@ __global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
                 const int desc = a.bc_desc[e];
                 const int b0 = a.bcd_off[desc], b1 = a.bcd_off[desc + 1];
                 if (b1 <= b0) continue;
-                const T* gs = gsb + a.bcd_sp[b0] * sn;
-                const float Cp = C::dec(gs[i]) + C::dec(gs[1 * n + i]) + C::dec(gs[2 * n + i]) +
-                                 C::dec(gs[3 * n + i]) + C::dec(gs[4 * n + i]);
+                const int s1 = 5 * a.bcd_sp[b0];
+                const float Cp = C::dec(gsb[pidx<P>(s1, i, n, GQ)]) + C::dec(gsb[pidx<P>(s1 + 1, i, n, GQ)]) +
+                                 C::dec(gsb[pidx<P>(s1 + 2, i, n, GQ)]) + C::dec(gsb[pidx<P>(s1 + 3, i, n, GQ)]) +
+                                 C::dec(gsb[pidx<P>(s1 + 4, i, n, GQ)]);
                 const float c6 = Cp / 6.0f;
                 const float bvx = (float)a.bc_v[2 * e], bvy = (float)a.bc_v[2 * e + 1];
                 float feq, eq;
@@ -464,7 +468,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
                 else if (dd == DN) { feq = c6 * (1.0f + 3.0f * bvy); eq = c6 * (1.0f + 3.0f * pv[k]); }
                 else if (dd == DW) { feq = c6 * (1.0f - 3.0f * bvx); eq = c6 * (1.0f - 3.0f * pu[k]); }
                 else { feq = c6 * (1.0f - 3.0f * bvy); eq = c6 * (1.0f - 3.0f * pv[k]); }
-                float carry = feq + (C::dec(gs[(long)dd * n + i]) - eq);
+                float carry = feq + (C::dec(gsb[pidx<P>(s1 + dd, i, n, GQ)]) - eq);
                 for (int kk = b0; kk < b1; kk++) {
                     const int sk = a.bcd_sp[kk];
 #pragma unroll
@@ -517,36 +521,40 @@ __global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
         uo[2 * k] = uxn;
         uo[2 * k + 1] = uyn;
     }
-    uint32_t* fd = static_cast<uint32_t*>(a.fd);
-    uint32_t* gd = static_cast<uint32_t*>(a.gd);
-    const long h = i0 >> 1;   // pair index
+    T* fd = static_cast<T*>(a.fd);
+    T* gd = static_cast<T*>(a.gd);
 #pragma unroll
-    for (int q = 0; q < 9; q++) __builtin_nontemporal_store(outf[q], &fd[((long)q * n >> 1) + h]);
+    for (int q = 0; q < 9; q++)
+        __builtin_nontemporal_store(outf[q], reinterpret_cast<uint32_t*>(fd + pidx<P>(q, i0, n, 9)));
 #pragma unroll
     for (int s = 0; s < NS; s++)
 #pragma unroll
-        for (int q = 0; q < 5; q++) __builtin_nontemporal_store(outg[s][q], &gd[((s * sn + (long)q * n) >> 1) + h]);
+        for (int q = 0; q < 5; q++)
+            __builtin_nontemporal_store(outg[s][q], reinterpret_cast<uint32_t*>(gd + pidx<P>(5 * s + q, i0, n, GQ)));
     *reinterpret_cast<float4*>(a.u + 2 * i0) = make_float4(uo[0], uo[1], uo[2], uo[3]);
 }
 
 // FP64 natural-order fields <-> storage (current buffers); field 0 = fluid.
+// src/dst planes q0 .. q0 + count/n - 1 of nq in the storage blocks.
 template <int P>
-__global__ void k_pack(const double* __restrict__ src, void* dst, long count, int fluid, long n) {
+__global__ void k_pack(const double* __restrict__ src, void* dst, long count, int fluid, long n, int q0, int nq) {
     using C = Codec<P>;
     typename C::T* out = static_cast<typename C::T*>(dst);
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (long)gridDim.x * blockDim.x) {
+        const int q = (int)(k / n);
         float v = (float)src[k];
-        if (fluid && C::dev) v = (float)(src[k] - (double)wq((int)(k / n)));
-        out[k] = C::enc(v);
+        if (fluid && C::dev) v = (float)(src[k] - (double)wq(q));
+        out[pidx<P>(q0 + q, k - (long)q * n, n, nq)] = C::enc(v);
     }
 }
 template <int P>
-__global__ void k_unpack(const void* src, double* __restrict__ dst, long count, int fluid, long n) {
+__global__ void k_unpack(const void* src, double* __restrict__ dst, long count, int fluid, long n, int q0, int nq) {
     using C = Codec<P>;
     const typename C::T* in = static_cast<const typename C::T*>(src);
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (long)gridDim.x * blockDim.x) {
-        double v = (double)C::dec(in[k]);
-        if (fluid && C::dev) v += (double)wq((int)(k / n));
+        const int q = (int)(k / n);
+        double v = (double)C::dec(in[pidx<P>(q0 + q, k - (long)q * n, n, nq)]);
+        if (fluid && C::dev) v += (double)wq(q);
         dst[k] = v;
     }
 }
@@ -649,29 +657,31 @@ int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile) {
 static int grid_of(long count) { return (int)std::min<long>((count + 255) / 256, 8192); }
 
 int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq) {
-    void* base = field == 0 ? c->f[c->cur] : c->g[c->cur];
-    const size_t sb = c->prec == LBIB_FP32 ? 4 : 2;
-    void* dst = static_cast<char*>(base) + (field == 0 ? 0 : (size_t)(field - 1) * 5 * c->n * sb);
+    void* dst = field == 0 ? c->f[c->cur] : c->g[c->cur];
+    const int q0 = field == 0 ? 0 : 5 * (field - 1);
+    const int nqt = field == 0 ? 9 : 5 * std::max(c->ns, 1);
     const long count = (long)nq * c->n;
     const int fl = field == 0 ? 1 : 0;
-    if (c->prec == LBIB_FP32) k_pack<LBIB_FP32><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n);
+    if (c->prec == LBIB_FP32)
+        k_pack<LBIB_FP32><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n, q0, nqt);
     else if (c->prec == LBIB_FP16_DEV)
-        k_pack<LBIB_FP16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n);
-    else k_pack<LBIB_BF16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n);
+        k_pack<LBIB_FP16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n, q0, nqt);
+    else k_pack<LBIB_BF16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src_dev, dst, count, fl, c->n, q0, nqt);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
 
 int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq) {
-    const void* base = field == 0 ? c->f[c->cur] : c->g[c->cur];
-    const size_t sb = c->prec == LBIB_FP32 ? 4 : 2;
-    const void* src = static_cast<const char*>(base) + (field == 0 ? 0 : (size_t)(field - 1) * 5 * c->n * sb);
+    const void* src = field == 0 ? c->f[c->cur] : c->g[c->cur];
+    const int q0 = field == 0 ? 0 : 5 * (field - 1);
+    const int nqt = field == 0 ? 9 : 5 * std::max(c->ns, 1);
     const long count = (long)nq * c->n;
     const int fl = field == 0 ? 1 : 0;
-    if (c->prec == LBIB_FP32) k_unpack<LBIB_FP32><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n);
+    if (c->prec == LBIB_FP32)
+        k_unpack<LBIB_FP32><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n, q0, nqt);
     else if (c->prec == LBIB_FP16_DEV)
-        k_unpack<LBIB_FP16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n);
-    else k_unpack<LBIB_BF16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n);
+        k_unpack<LBIB_FP16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n, q0, nqt);
+    else k_unpack<LBIB_BF16_DEV><<<grid_of(count), 256, 0, c->stream>>>(src, dst_dev, count, fl, c->n, q0, nqt);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

@@ -819,4 +819,68 @@ int launch_forces_tension(lbib_ctx* c, double f) {
     return 0;
 }
 
+// GeometryHandler::computeAccumulatedDomainConcentrations
+// (GeometryHandler.cpp:136-147): the sum of getC() of one species over the
+// sites of each domain, x from 1 (the reference skips the x = 0 column).
+// One workgroup per row; runs of equal domain inside a wavefront are summed
+// with a segmented scan and added once per run (domain 0 in LDS, one global
+// atomic per row).
+template <int P>
+__global__ __launch_bounds__(256) void k_domain_conc(int nx, long n, const void* g, int species, int nq,
+                                                    const uint32_t* __restrict__ tag, int ndom, double* acc) {
+    using CD = Codec<P>;
+    const typename CD::T* gs = static_cast<const typename CD::T*>(g);
+    const int q0 = 5 * species;
+    __shared__ double out0;
+    if (threadIdx.x == 0) out0 = 0.0;
+    __syncthreads();
+    const long y = blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    for (int x0 = 1; x0 < nx; x0 += blockDim.x) {
+        const int x = x0 + (int)threadIdx.x;
+        uint32_t d = 0xffffffffu;
+        double s = 0.0;
+        if (x < nx) {
+            const long i = y * nx + x;
+            d = tag[i] & DOM_MASK;
+            s = c_swapped((double)CD::dec(gs[pidx<P>(q0, i, n, nq)]), (double)CD::dec(gs[pidx<P>(q0 + 1, i, n, nq)]),
+                          (double)CD::dec(gs[pidx<P>(q0 + 2, i, n, nq)]), (double)CD::dec(gs[pidx<P>(q0 + 3, i, n, nq)]),
+                          (double)CD::dec(gs[pidx<P>(q0 + 4, i, n, nq)]));
+        }
+        const uint32_t dprev = (uint32_t)__shfl_up((int)d, 1, 64);
+        int head = (lane == 0 || dprev != d) ? 1 : 0;
+        for (int o = 1; o < 64; o <<= 1) {
+            const double ts = __shfl_up(s, o, 64);
+            const int th = __shfl_up(head, o, 64);
+            if (lane >= o) {
+                if (!head) s += ts;
+                head |= th;
+            }
+        }
+        const uint32_t dnext = (uint32_t)__shfl_down((int)d, 1, 64);
+        const bool tail = lane == 63 || dnext != d;
+        if (tail && d != 0xffffffffu) {
+            if (d == 0u) atomicAdd(&out0, s);
+            else if ((int)d < ndom) atomicAdd(&acc[d], s);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&acc[0], out0);
+}
+
+int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc) {
+    const void* g = c->g[c->cur];
+    const int nq = 5 * std::max(c->ns, 1);
+    switch (c->prec) {
+        case LBIB_FP64: k_domain_conc<LBIB_FP64><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc); break;
+        case LBIB_FP32: k_domain_conc<LBIB_FP32><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc); break;
+        case LBIB_FP16_DEV:
+            k_domain_conc<LBIB_FP16_DEV><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc);
+            break;
+        default: k_domain_conc<LBIB_BF16_DEV><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc); break;
+    }
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace lbib

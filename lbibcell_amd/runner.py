@@ -24,6 +24,7 @@ _SIG = {
     "lbh_set_graph_steps": (C.c_int, [_P, C.c_uint]),
     "lbh_graph_stats": (None, [_P, C.POINTER(C.c_long)]),
     "lbh_graph_note": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "lbh_domain_concentrations": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.c_int]),
     "lbh_load_parameters": (C.c_int, [_P, C.c_char_p]),
     "lbh_load_geometry_txt": (C.c_int, [_P, C.c_char_p]),
     "lbh_add_node": (C.c_int, [_P, C.c_uint, C.c_double, C.c_double]),
@@ -104,6 +105,15 @@ class Simulation:
             spec = capi.species_spec(tau=tau)
             spec.kind = -1
         self._chk(self.L.lbh_add_species(self.h, name.encode(), C.byref(spec)))
+
+    def domain_concentrations(self, name, cap=65536):
+        """GeometryHandler::computeAccumulatedDomainConcentrations: array indexed
+        by domain id (0 for ids absent from the lattice)."""
+        out = np.zeros(cap)
+        w = self.L.lbh_domain_concentrations(self.h, name.encode(), capi.dptr(out), cap)
+        if w < 0:
+            raise capi.LbibError(self.L.lbh_last_error(self.h).decode())
+        return out[:w]
 
     def set_graph_steps(self, n):
         """Record blocks of n steps (a multiple of 10) into a HIP graph and

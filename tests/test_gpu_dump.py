@@ -76,3 +76,25 @@ def test_solver_dump_round_trip(orc, tmp_path):
             assert [g[slot_of5[q], y, x] for q in range(5)] == v
     b.close()
     sim.close()
+
+
+def test_domain_concentration_sums(orc):
+    """GeometryHandler::computeAccumulatedDomainConcentrations on the device:
+    the per-domain sum of getC() over x >= 1, against the oracle's stored
+    species summed in (y, x) order."""
+    sim, o = _case(orc)
+    sim.run(9)
+    o.run(9)
+    names = sorted(s[0] for s in SPECIES)
+    dom_o, _ = o.get_domains()
+    for k, name in enumerate(names):
+        got = sim.domain_concentrations(name)
+        C = o.get_g(k).sum(axis=0)
+        want = np.zeros(got.shape)
+        for d in range(len(want)):
+            want[d] = C[:, 1:][dom_o[:, 1:] == d].sum()
+        print(f"case domain sums {name}: {got}", flush=True)
+        np.testing.assert_allclose(got, want, rtol=1e-13, atol=1e-13)
+    with pytest.raises(capi.LbibError):
+        sim.domain_concentrations("NoSuchSolver")
+    sim.close()
