@@ -63,9 +63,13 @@ namespace geometry {
 
 // ------------------------------------------------------------- geometry ----
 Geometry::Geometry(const std::string& file) {
-    if (file.size() >= 4 && file.compare(file.size() - 4, 4, ".txt") == 0) loadGeometryTXT(file);
-    else LBIB_FAIL("only .txt geometries are supported by this loader");
+    auto ends = [&](const char* e) { return file.size() >= 4 && file.compare(file.size() - 4, 4, e) == 0; };
+    if (ends(".txt")) loadGeometryTXT(file);
+    else if (ends(".vtm") || ends(".vtp")) loadGeometryVTK(file);
+    else LBIB_FAIL("geometry files are .txt, .vtm or .vtp");
 }
+
+void Geometry::loadGeometryVTK(const std::string& file) { vtkxml::load_geometry(file, *this); }
 
 void Geometry::setBatchRects(const std::vector<lbib_rect>& rects) {
     rects_ = rects;
@@ -1122,6 +1126,59 @@ void SimulationRunner::runSimulation() {
                         "output/CRASH_parameterOut_" + t + ".txt", "output/CRASH_solverOut_" + t + ".txt");
         std::cout << exp.what() << std::endl;
     }
+}
+
+// vtkFluidReporter.cpp:44-108: density (getRho, slot order of the stored
+// state) and velocity (z = 0) at every coarsening-th site
+void SimulationRunner::writeVtkFluid(const std::string& prefix, unsigned time, int coarse) {
+    if (coarse < 1) LBIB_FAIL("coarsening factor must be positive");
+    settle();
+    const int nx = (int)params_.sizeX, ny = (int)params_.sizeY;
+    const size_t n = (size_t)nx * ny;
+    std::vector<double> f(9 * n), ux(n), uy(n);
+    check(lbib_get_populations(ctx_, f.data()), "populations");
+    check(lbib_get_velocity(ctx_, ux.data(), uy.data()), "velocity");
+    std::vector<double> rho, vel;
+    // dimensions nx/coarse x ny/coarse, sites (jx coarse, jy coarse)
+    for (int jy = 0; jy < ny / coarse; jy++)
+        for (int jx = 0; jx < nx / coarse; jx++) {
+            const size_t i = (size_t)jy * coarse * nx + (size_t)jx * coarse;
+            double r = 0.0;
+            for (int q = 0; q < 9; q++) r += f[(size_t)kSlotOfD2Q9[q] * n + i];
+            rho.push_back(r);
+            vel.push_back(ux[i]);
+            vel.push_back(uy[i]);
+            vel.push_back(0.0);
+        }
+    vtkxml::write_image(prefix, time, nx, ny, coarse, {{"density", {1, rho}}, {"velocity", {3, vel}}});
+}
+
+// vtkCDEReporter.cpp:44-100: getC of every CDE solver, one array per name
+void SimulationRunner::writeVtkCDE(const std::string& prefix, unsigned time, int coarse) {
+    if (coarse < 1) LBIB_FAIL("coarsening factor must be positive");
+    settle();
+    const int nx = (int)params_.sizeX, ny = (int)params_.sizeY;
+    const size_t n = (size_t)nx * ny;
+    std::vector<std::pair<std::string, std::pair<int, std::vector<double>>>> fields;
+    std::vector<double> g(5 * n);
+    for (size_t k = 0; k < speciesNames_.size(); k++) {
+        check(lbib_get_species(ctx_, (int)k, g.data()), "species");
+        std::vector<double> c;
+        for (int jy = 0; jy < ny / coarse; jy++)
+            for (int jx = 0; jx < nx / coarse; jx++) {
+                const size_t i = (size_t)jy * coarse * nx + (size_t)jx * coarse;
+                double s = 0.0;
+                for (int q = 0; q < 5; q++) s += g[(size_t)kSlotOfD2Q5[q] * n + i];
+                c.push_back(s);
+            }
+        fields.push_back({speciesNames_[k], {1, c}});
+    }
+    vtkxml::write_image(prefix, time, nx, ny, coarse, fields);
+}
+
+void SimulationRunner::writeVtkCells(const std::string& prefix, unsigned time) {
+    syncMirror();
+    vtkxml::write_cells(prefix, time, geo_, geo_.cellTypeTrackerMap());
 }
 
 std::map<unsigned, double> SimulationRunner::computeAccumulatedDomainConcentrations(const std::string& name) {

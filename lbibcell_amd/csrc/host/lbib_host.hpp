@@ -65,8 +65,9 @@ struct Connection {
 class Geometry {
  public:
     Geometry() = default;
-    explicit Geometry(const std::string& txtFile);     // Geometry.cpp:56-72 (.txt only)
+    explicit Geometry(const std::string& file);        // Geometry.cpp:56-72 (.txt, .vtm or .vtp)
     void loadGeometryTXT(const std::string& file);      // Geometry.cpp:99-160
+    void loadGeometryVTK(const std::string& file);      // Geometry.cpp:162-306 (ASCII VTK XML)
     void writeGeometry(const std::string& file) const;  // Geometry.cpp:77-95
     // GeometryNode ctor + Geometry::addGeometryNode (Geometry.cpp:381-399)
     unsigned addGeometryNode(unsigned id, double x, double y);
@@ -230,6 +231,11 @@ class SimulationRunner {
     unsigned graphSteps() const { return graphSteps_; }
     long graphLaunches = 0, graphCaptures = 0;
     std::string graphNote;   // why graph mode turned itself off
+    // vtkFluidReporter / vtkCDEReporter / vtkCellReporter (ASCII VTK XML, no
+    // VTK library): append the time step's block to <prefix>_<time>.vtm
+    void writeVtkFluid(const std::string& prefix, unsigned time, int coarsening = 1);
+    void writeVtkCDE(const std::string& prefix, unsigned time, int coarsening = 1);
+    void writeVtkCells(const std::string& prefix, unsigned time);
 
  private:
     bool runGraphBlock(unsigned it0);
@@ -264,5 +270,14 @@ class SimulationRunner {
     std::vector<lbib_rect> rects_;
     unsigned graphSteps_ = 0;
 };
+
+namespace vtkxml {
+// ASCII VTK XML without the VTK library (lbib_host_vtk.cpp)
+void load_geometry(const std::string& file, geometry::Geometry& geo);
+void write_image(const std::string& prefix, unsigned time, int nx, int ny, int coarse,
+                 const std::vector<std::pair<std::string, std::pair<int, std::vector<double>>>>& fields);
+void write_cells(const std::string& prefix, unsigned time, const geometry::Geometry& geo,
+                 const std::map<unsigned, unsigned>& cellTypes);
+}  // namespace vtkxml
 
 }  // namespace LbmLib

@@ -77,6 +77,30 @@ int lbh_load_parameters(void* p, const char* file) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] { h->params.loadGlobalSimulationParameters(file); });
 }
+int lbh_load_geometry_vtk(void* p, const char* file) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->geo.loadGeometryVTK(file); });
+}
+// which: bit 0 fluid, bit 1 CDE solvers, bit 2 cells
+int lbh_write_vtk(void* p, const char* prefix, unsigned time, int coarsening, int which) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        if (!h->runner) LBIB_FAIL("write_vtk needs a set-up runner");
+        if (which & 1) h->runner->writeVtkFluid(prefix, time, coarsening);
+        if (which & 2) h->runner->writeVtkCDE(prefix, time, coarsening);
+        if (which & 4) h->runner->writeVtkCells(prefix, time);
+    });
+}
+// the host geometry as Geometry::writeGeometry text, or as a VTK cell block
+// (vtkCellReporter layout): both work before setup
+int lbh_write_geometry_txt(void* p, const char* file) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { h->geo.writeGeometry(file); });
+}
+int lbh_write_geometry_vtk(void* p, const char* prefix, unsigned time) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { vtkxml::write_cells(prefix, time, h->geo, h->geo.cellTypeTrackerMap()); });
+}
 int lbh_load_geometry_txt(void* p, const char* file) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] { h->geo.loadGeometryTXT(file); });

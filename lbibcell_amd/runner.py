@@ -25,6 +25,10 @@ _SIG = {
     "lbh_graph_stats": (None, [_P, C.POINTER(C.c_long)]),
     "lbh_graph_note": (C.c_int, [_P, C.c_char_p, C.c_int]),
     "lbh_domain_concentrations": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.c_int]),
+    "lbh_load_geometry_vtk": (C.c_int, [_P, C.c_char_p]),
+    "lbh_write_geometry_txt": (C.c_int, [_P, C.c_char_p]),
+    "lbh_write_geometry_vtk": (C.c_int, [_P, C.c_char_p, C.c_uint]),
+    "lbh_write_vtk": (C.c_int, [_P, C.c_char_p, C.c_uint, C.c_int, C.c_int]),
     "lbh_load_parameters": (C.c_int, [_P, C.c_char_p]),
     "lbh_load_geometry_txt": (C.c_int, [_P, C.c_char_p]),
     "lbh_add_node": (C.c_int, [_P, C.c_uint, C.c_double, C.c_double]),
@@ -114,6 +118,26 @@ class Simulation:
         if w < 0:
             raise capi.LbibError(self.L.lbh_last_error(self.h).decode())
         return out[:w]
+
+    def load_geometry_vtk(self, path):
+        """Geometry::loadGeometryVTK: the polygon block of an ASCII VTK XML
+        .vtm (or a .vtp) with domain_identifier, cell_type and
+        boundarydescriptors point arrays."""
+        self._chk(self.L.lbh_load_geometry_vtk(self.h, str(path).encode()))
+
+    def write_geometry_txt(self, path):
+        """Geometry::writeGeometry of the host geometry (no device needed)."""
+        self._chk(self.L.lbh_write_geometry_txt(self.h, str(path).encode()))
+
+    def write_geometry_vtk(self, prefix, time=0):
+        """The host geometry as a vtkCellReporter block (no device needed)."""
+        self._chk(self.L.lbh_write_geometry_vtk(self.h, str(prefix).encode(), int(time)))
+
+    def write_vtk(self, prefix, time, coarsening=1, fluid=True, species=True, cells=True):
+        """vtkFluidReporter / vtkCDEReporter / vtkCellReporter: append this
+        step's blocks to <prefix>_<time>.vtm (ASCII VTK XML)."""
+        which = (1 if fluid else 0) | (2 if species else 0) | (4 if cells else 0)
+        self._chk(self.L.lbh_write_vtk(self.h, str(prefix).encode(), int(time), int(coarsening), which))
 
     def set_graph_steps(self, n):
         """Record blocks of n steps (a multiple of 10) into a HIP graph and
