@@ -249,6 +249,18 @@ int lbib_area_regulator(lbib_ctx* ctx, double target_area, double kp, double max
  * device op: deleteForceType(6) + two type-6 structs of magnitude f per
  * connection.  Only valid while the force table holds no other types. */
 int lbib_forces_tension(lbib_ctx* ctx, double f);
+/* tutorial_02_BioSolverCellJunction / BioSolverCellJunction
+ * (tutorial_02_BioSolverCellJunction.cpp:47-99) as one device op:
+ * deleteForceType(7), then for each marker j in id order the closest marker c
+ * within `radius` whose adjacent connections belong to another domain
+ * (Geometry.cpp:351-373, FastNeighborList.hpp:404-467) gives the springs
+ * (7, j, c, k, l0) and (7, c, j, k, l0), appended to the rows of j and c.
+ * Other struct types keep their order.  k and l0 are used as given (the
+ * reference streams them through a text descriptor: round them first).
+ * Equidistant candidates resolve to the smaller id.  At most 16 markers may
+ * choose one marker; more, or a table beyond its capacity, sets
+ * LBIB_ERR_CAPACITY.  Stream-ordered. */
+int lbib_forces_junctions(lbib_ctx* ctx, double radius, double k, double l0);
 /* ForceSolver::deleteAllForces (ForceSolver.cpp:158-166). */
 int lbib_forces_clear(lbib_ctx* ctx);
 /* FluidSolver::addMass for every site of each domain: m_site = table[dom]. */

@@ -140,6 +140,7 @@ SIGNATURES = {
     "lbib_step_geometry": (C.c_int, [_P, C.c_uint32, C.c_int]),
     "lbib_step_topology": (C.c_int, [_P, C.c_int]),
     "lbib_forces_tension": (C.c_int, [_P, C.c_double]),
+    "lbib_forces_junctions": (C.c_int, [_P, C.c_double, C.c_double, C.c_double]),
     "lbib_forces_clear": (C.c_int, [_P]),
     "lbib_cure_lattice": (C.c_int, [_P]),
     "lbib_set_box_outlet": (C.c_int, [_P, C.c_int]),
@@ -307,6 +308,22 @@ class Context:
     def set_domains(self, d):
         d = np.ascontiguousarray(d, dtype=np.uint32)
         self._chk(self.lib.lbib_set_domains(self.h, u32ptr(d)))
+
+    def get_forces(self, nm):
+        """The device force table of nm markers as CSR by dense marker index:
+        (off[nm + 1], type[nf], partner[nf], prm[nf, 4])."""
+        off = np.empty(nm + 1, dtype=np.int32)
+        self._chk(self.lib.lbib_get_forces(self.h, i32ptr(off), None, None, None))
+        nf = int(off[nm])
+        t = np.empty(max(nf, 1), dtype=np.int32)
+        q = np.empty(max(nf, 1), dtype=np.int32)
+        prm = np.empty((max(nf, 1), 4), dtype=np.float64)
+        self._chk(self.lib.lbib_get_forces(self.h, i32ptr(off), i32ptr(t), i32ptr(q), dptr(prm)))
+        return off, t[:nf], q[:nf], prm[:nf]
+
+    def forces_junctions(self, radius=1.0, k=0.02, l0=0.5):
+        """deleteForceType(7) and the cell-junction springs (lbib_forces_junctions)."""
+        self._chk(self.lib.lbib_forces_junctions(self.h, float(radius), float(k), float(l0)))
 
     def set_force_field(self, fx, fy):
         fx = np.ascontiguousarray(fx, dtype=np.float64)

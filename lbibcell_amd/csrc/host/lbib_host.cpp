@@ -450,6 +450,20 @@ struct MembraneTension : BioSolver {
     }
 };
 
+// tutorial_02_BioSolverCellJunction, tutorial_01_BioSolverCellJunction and
+// BioSolverCellJunction (tutorial_02_BioSolverCellJunction.cpp:37-99): every
+// FREQUENCY = 10 iterations, junction springs K = 0.02, L0 = 0.5 between each
+// node and its closest node of another cell within RADIUS = 1.0.
+struct CellJunction : BioSolver {
+    unsigned freq = 10;
+    double radius = 1.0, k = 0.02, l0 = 0.5;
+    unsigned graphPeriod() const override { return freq; }
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        r.renewJunctions(radius, k, l0);
+    }
+};
+
 // tutorial_01_BioSolverAreaRegulator (tutorial_01_BioSolverAreaRegulator.cpp:43-100)
 // served by one device op (areas, clamped source, addMass on every cell site).
 struct AreaRegulator : BioSolver {
@@ -698,6 +712,9 @@ void SimulationRunner::addBioSolver(const std::string& name) {
     std::unique_ptr<BioSolver> b;
     if (name == "tutorial_01_BioSolverMembraneTension") b.reset(new MembraneTension());
     else if (name == "tutorial_01_BioSolverAreaRegulator") b.reset(new AreaRegulator());
+    else if (name == "tutorial_02_BioSolverCellJunction" || name == "tutorial_01_BioSolverCellJunction" ||
+             name == "BioSolverCellJunction")
+        b.reset(new CellJunction());
     else throw std::bad_typeid();
     b->name = name;
     bio_.push_back(std::move(b));
@@ -746,7 +763,7 @@ void SimulationRunner::renewTension(double force) {
     ss << force;
     double f;
     ss >> f;
-    if (forceSolver_.dirty_ || !forceSolver_.onlyType(6)) {
+    if (forceSolver_.dirty_ || !forceSolver_.onlyType(6) || (devForceTypes_ & ~(1u << 6))) {
         syncMirror();
         forceSolver_.deleteForceType(6);
         for (auto& c : geo_.getConnections()) {
@@ -757,7 +774,26 @@ void SimulationRunner::renewTension(double force) {
         return;
     }
     check(lbib_forces_tension(ctx_, f), "tension");
+    devForceTypes_ |= 1u << 6;
     mirrorValid_ = false;   // the host forcemap no longer matches the device table
+}
+
+void SimulationRunner::renewJunctions(double radius, double k, double l0) {
+    // K and L0 travel through a text descriptor in the reference
+    auto via_text = [](double v) {
+        std::stringstream ss;
+        ss << v;
+        double r;
+        ss >> r;
+        return r;
+    };
+    if (forceSolver_.dirty_) {
+        syncMirror();
+        uploadForces();
+    }
+    check(lbib_forces_junctions(ctx_, radius, via_text(k), via_text(l0)), "cell junctions");
+    devForceTypes_ |= 1u << 7;
+    mirrorValid_ = false;
 }
 
 void SimulationRunner::uploadTopology() {
@@ -857,6 +893,8 @@ void SimulationRunner::uploadForces() {
         off[k + 1] = (int32_t)type.size();
     }
     check(lbib_set_forces(ctx_, nm, off.data(), type.data(), partner.data(), prm.data()), "set_forces");
+    devForceTypes_ = 0;
+    for (int t : type) devForceTypes_ |= 1u << t;
     forceSolver_.dirty_ = false;
 }
 
@@ -949,6 +987,7 @@ void SimulationRunner::handleGeometryStatus(unsigned it) {
         forceSolver_.deleteAllForces();
         forceSolver_.dirty_ = false;
         check(lbib_forces_clear(ctx_), "deleteAllForces");
+        devForceTypes_ = 0;
     }
     check(lbib_clear_status(ctx_), "status");
 }
@@ -982,6 +1021,7 @@ void SimulationRunner::enqueueStep(unsigned it) {
         forceSolver_.deleteAllForces();
         forceSolver_.dirty_ = false;
         check(lbib_forces_clear(ctx_), "deleteAllForces");
+        devForceTypes_ = 0;
     }
     auto t2 = std::chrono::steady_clock::now();
     times.lattice += std::chrono::duration<double>(t1 - t0).count();
@@ -1025,6 +1065,11 @@ void SimulationRunner::resolveStop(unsigned it) {
 }
 
 void SimulationRunner::step() { run(1); }
+
+void SimulationRunner::applyBioSolvers() {
+    settle();
+    for (auto& b : bio_) b->applyBioProcess(*this);
+}
 
 void SimulationRunner::setGraphSteps(unsigned steps) {
     if (steps % 10 != 0) LBIB_FAIL("graph blocks must be a multiple of the coarsening period (10)");

@@ -198,7 +198,12 @@ class SimulationRunner {
     void runSimulation();                                 // :69-110 (Parameters.iterations steps)
     void run(unsigned steps);
     void step();
-    solver::ForceSolver& getForceSolver() { return forceSolver_; }
+    // the host table is brought up to date first, so that edits through it
+    // extend what the device holds (they are uploaded before the next step)
+    solver::ForceSolver& getForceSolver() {
+        syncMirror();
+        return forceSolver_;
+    }
     geometry::Geometry& getGeometry() { return geo_; }
     lbib_ctx* context() { return ctx_; }
     GlobalSimulationParameters& parameters() { return params_; }
@@ -212,6 +217,12 @@ class SimulationRunner {
     // device-resident tension renewal (falls back to the host table when it
     // holds other force types)
     void renewTension(double force);
+    // device-resident cell-junction renewal (tutorial_02_BioSolverCellJunction):
+    // deleteForceType(7) and one spring pair per marker with a closest marker of
+    // another cell within radius; k and l0 pass through the text descriptor
+    void renewJunctions(double radius, double k, double l0);
+    // the BioSolvers of the current iteration, outside a step
+    void applyBioSolvers();
     std::map<unsigned, double> computeAreas();            // GeometryHandler.cpp:101-134 (device shoelace)
     // GeometryHandler.cpp:136-147 (device per-domain sums, domains present on the lattice)
     std::map<unsigned, double> computeAccumulatedDomainConcentrations(const std::string& name);
@@ -261,6 +272,9 @@ class SimulationRunner {
     std::vector<std::vector<int>> bcDescriptors_;
     std::vector<std::map<std::string, std::vector<std::string>>> bcDescriptorText_;
     bool mirrorValid_ = true;
+    // force types the device table may hold (bit t: type t); device-side
+    // renewals that rewrite whole rows are valid only over their own type
+    uint32_t devForceTypes_ = 0;
     size_t nm_ = 0, nc_ = 0;
     unsigned pending_ = 0;   // iterations enqueued since the last clear status
     // tutorial_02_CDESolverD2Q5_R/_L::initSolver draw (1 + U(-0.01, 0.01)) / 5

@@ -36,6 +36,12 @@ int guarded(Handle* h, F&& f) {
     }
     return -1;
 }
+
+// the runner exists from lbh_setup on
+SimulationRunner& R(Handle* h) {
+    if (!h->runner) throw Exception("the simulation is not set up yet (call lbh_setup first)");
+    return *h->runner;
+}
 }  // namespace
 
 extern "C" {
@@ -86,9 +92,9 @@ int lbh_write_vtk(void* p, const char* prefix, unsigned time, int coarsening, in
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
         if (!h->runner) LBIB_FAIL("write_vtk needs a set-up runner");
-        if (which & 1) h->runner->writeVtkFluid(prefix, time, coarsening);
-        if (which & 2) h->runner->writeVtkCDE(prefix, time, coarsening);
-        if (which & 4) h->runner->writeVtkCells(prefix, time);
+        if (which & 1) R(h).writeVtkFluid(prefix, time, coarsening);
+        if (which & 2) R(h).writeVtkCDE(prefix, time, coarsening);
+        if (which & 4) R(h).writeVtkCells(prefix, time);
     });
 }
 // the host geometry as Geometry::writeGeometry text, or as a VTK cell block
@@ -134,33 +140,33 @@ int lbh_setup(void* p) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
         h->runner.reset(new SimulationRunner(h->params, h->geo, h->precision, h->forceModel));
-        for (auto& kv : h->specs) h->runner->setSpeciesSpec(kv.first, kv.second);
-        h->runner->setBatchLayout(h->rects);
-        h->runner->setup();
+        for (auto& kv : h->specs) R(h).setSpeciesSpec(kv.first, kv.second);
+        R(h).setBatchLayout(h->rects);
+        R(h).setup();
     });
 }
 int lbh_init_solvers(void* p, const double* u0) {
     auto* h = static_cast<Handle*>(p);
-    return guarded(h, [&] { h->runner->initSolvers(u0); });
+    return guarded(h, [&] { R(h).initSolvers(u0); });
 }
 int lbh_add_force(void* p, int type, unsigned id1, unsigned id2, double a, double b, double c, double d) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
         solver::ForceStruct f{type, id1, id2, {a, b, c, d}};
-        h->runner->getForceSolver().addForce(f);
+        R(h).getForceSolver().addForce(f);
     });
 }
 int lbh_load_force_file(void* p, const char* file) {
     auto* h = static_cast<Handle*>(p);
-    return guarded(h, [&] { h->runner->initForceSolver(file); });
+    return guarded(h, [&] { R(h).initForceSolver(file); });
 }
 int lbh_add_mass_solver(void* p, const char* name) {
     auto* h = static_cast<Handle*>(p);
-    return guarded(h, [&] { h->runner->addMassSolver(name); });
+    return guarded(h, [&] { R(h).addMassSolver(name); });
 }
 int lbh_add_bio_solver(void* p, const char* name) {
     auto* h = static_cast<Handle*>(p);
-    return guarded(h, [&] { h->runner->addBioSolver(name); });
+    return guarded(h, [&] { R(h).addBioSolver(name); });
 }
 // Parameterised tutorial BioSolvers: kind 0 tension (prm[0] = force), kind 1
 // area regulator (target, kp, maxsource, target factor).
@@ -168,38 +174,43 @@ int lbh_add_bio_solver_params(void* p, int kind, const double* prm, unsigned fre
 int lbh_write_simulation(void* p, const char* force, const char* geometry, const char* parameters,
                          const char* solvers) {
     auto* h = static_cast<Handle*>(p);
-    return guarded(h, [&] { h->runner->writeSimulation(force, geometry, parameters, solvers); });
+    return guarded(h, [&] { R(h).writeSimulation(force, geometry, parameters, solvers); });
 }
 int lbh_init_solvers_file(void* p, const char* solvers) {
     auto* h = static_cast<Handle*>(p);
-    return guarded(h, [&] { h->runner->initSolvers(std::string(solvers)); });
+    return guarded(h, [&] { R(h).initSolvers(std::string(solvers)); });
+}
+// the BioSolvers of the current iteration alone (SimulationRunner.cpp:95-97)
+int lbh_apply_bio(void* p) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] { R(h).applyBioSolvers(); });
 }
 int lbh_run(void* p, unsigned steps) {
     auto* h = static_cast<Handle*>(p);
-    return guarded(h, [&] { h->runner->run(steps); });
+    return guarded(h, [&] { R(h).run(steps); });
 }
 int lbh_sync(void* p) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
-        if (lbib_synchronize(h->runner->context()) != 0) throw Exception(lbib_last_error(h->runner->context()));
+        if (lbib_synchronize(R(h).context()) != 0) throw Exception(lbib_last_error(R(h).context()));
     });
 }
 void* lbh_context(void* p) {
     auto* h = static_cast<Handle*>(p);
-    return h->runner ? h->runner->context() : nullptr;
+    return h->runner ? R(h).context() : nullptr;
 }
 unsigned lbh_iteration(void* p) {
     auto* h = static_cast<Handle*>(p);
-    return h->runner ? h->runner->parameters().currentIteration : h->params.currentIteration;
+    return h->runner ? R(h).parameters().currentIteration : h->params.currentIteration;
 }
 int lbh_set_iteration(void* p, unsigned it) {
     auto* h = static_cast<Handle*>(p);
-    if (h->runner) h->runner->parameters().currentIteration = it;
+    if (h->runner) R(h).parameters().currentIteration = it;
     else h->params.currentIteration = it;
     return 0;
 }
 static void mirror(Handle* h) {
-    if (h->runner) h->runner->syncMirror();
+    if (h->runner) R(h).syncMirror();
 }
 int lbh_num_markers(void* p) {
     auto* h = static_cast<Handle*>(p);
@@ -244,7 +255,11 @@ long lbh_perturb_counts(void* p, int which) {
 // waits), the step count and the number of resolved stops
 void lbh_times(void* p, double* out) {
     auto* h = static_cast<Handle*>(p);
-    const auto& t = h->runner->times;
+    if (!h->runner) {
+        for (int k = 0; k < 7; k++) out[k] = 0.0;
+        return;
+    }
+    const auto& t = R(h).times;
     out[0] = t.lattice; out[1] = t.geometry; out[2] = t.host; out[3] = t.bio; out[4] = (double)t.steps;
     out[5] = t.wait;
     out[6] = (double)t.stops;
@@ -254,7 +269,7 @@ int lbh_set_run_ahead(void* p, unsigned n) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
         if (!h->runner) LBIB_FAIL("run-ahead is set after setup");
-        h->runner->runAhead = n < 1 ? 1 : n;
+        R(h).runAhead = n < 1 ? 1 : n;
     });
 }
 // computeAccumulatedDomainConcentrations into out[0..cap) (domains absent: 0);
@@ -263,7 +278,7 @@ int lbh_domain_concentrations(void* p, const char* name, double* out, int cap) {
     auto* h = static_cast<Handle*>(p);
     int written = -1;
     const int rc = guarded(h, [&] {
-        auto m = h->runner->computeAccumulatedDomainConcentrations(name);
+        auto m = R(h).computeAccumulatedDomainConcentrations(name);
         written = m.empty() ? 0 : (int)m.rbegin()->first + 1;
         for (int d = 0; d < cap; d++) out[d] = 0.0;
         for (auto& kv : m)
@@ -275,7 +290,7 @@ int lbh_set_graph_steps(void* p, unsigned n) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
         if (!h->runner) LBIB_FAIL("graph steps are set after setup");
-        h->runner->setGraphSteps(n);
+        R(h).setGraphSteps(n);
     });
 }
 // why graph mode turned itself off ("" while it is on or was never asked for)
@@ -309,7 +324,7 @@ int lbh_num_areas(void* p) {
 int lbh_get_areas(void* p, unsigned* dom, double* area) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {
-        auto a = h->runner->computeAreas();
+        auto a = R(h).computeAreas();
         size_t k = 0;
         for (auto& kv : a) { dom[k] = kv.first; area[k] = kv.second; k++; }
     });
@@ -325,6 +340,15 @@ struct TensionP : BioSolver {
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
         r.renewTension(force);
+    }
+};
+struct JunctionP : BioSolver {
+    unsigned freq;
+    double prm[3];
+    unsigned graphPeriod() const override { return freq; }
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        r.renewJunctions(prm[0], prm[1], prm[2]);
     }
 };
 struct AreaP : BioSolver {
@@ -348,13 +372,20 @@ extern "C" int lbh_add_bio_solver_params(void* p, int kind, const double* prm, u
             b->freq = freq;
             b->force = prm[0];
             b->name = "tutorial_01_BioSolverMembraneTension";
-            h->runner->addBioSolver(std::move(b));
+            R(h).addBioSolver(std::move(b));
         } else if (kind == 1) {
             std::unique_ptr<AreaP> b(new AreaP());
             b->freq = freq;
             for (int k = 0; k < 4; k++) b->prm[k] = prm[k];
             b->name = "tutorial_01_BioSolverAreaRegulator";
-            h->runner->addBioSolver(std::move(b));
+            R(h).addBioSolver(std::move(b));
+        } else if (kind == 2) {
+            if (!(prm[0] >= 0.0)) LBIB_FAIL("junction radius must be >= 0");
+            std::unique_ptr<JunctionP> b(new JunctionP());
+            b->freq = freq;
+            for (int k = 0; k < 3; k++) b->prm[k] = prm[k];
+            b->name = "tutorial_02_BioSolverCellJunction";
+            R(h).addBioSolver(std::move(b));
         } else {
             throw std::bad_typeid();
         }

@@ -62,6 +62,8 @@ int ensure_scan(lbib_ctx* c) {
     return 0;
 }
 
+int ensure_junctions(lbib_ctx* c);
+
 // Marker-indexed arrays (positions, links, spread work lists, coarsening
 // scratch, force CSR offsets).  Reallocation drops the contents: callers
 // re-upload the topology afterwards.
@@ -83,7 +85,8 @@ int ensure_markers(lbib_ctx* c, int nm) {
     for (auto a : iarr) if (dev_alloc(c, a, cap + 1)) return -1;
     if (dev_alloc(c, &c->tslot, cap) || dev_alloc(c, &c->tlist, cap)) return -1;
     c->mcap = cap;
-    return ensure_scan(c);
+    if (ensure_scan(c)) return -1;
+    return ensure_junctions(c);
 }
 
 int ensure_connections(lbib_ctx* c, int nc) {
@@ -160,13 +163,64 @@ int grow_connections(lbib_ctx* c, int nc) {
     return 0;
 }
 
+// Force structs: growth keeps the table's contents (a BioSolver may add to
+// structs written earlier in the same iteration).
 int ensure_force_cap(lbib_ctx* c, int nf) {
     if (nf <= c->fcap) return 0;
-    dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
-    c->fcap = std::max(nf, 2 * c->fcap);
-    if (dev_alloc(c, &c->fs_type, c->fcap) || dev_alloc(c, &c->fs_partner, c->fcap) ||
-        dev_alloc(c, &c->fs_prm, 4 * (size_t)c->fcap))
+    const int old = c->fcap;
+    const int cap = std::max(nf, 2 * c->fcap);
+    int32_t *t = nullptr, *p = nullptr;
+    double* q = nullptr;
+    if (dev_alloc(c, &t, cap) || dev_alloc(c, &p, cap) || dev_alloc(c, &q, 4 * (size_t)cap)) {
+        dev_free(t); dev_free(p); dev_free(q);
         return -1;
+    }
+    if (old > 0) {
+        LBIB_HIP(hipMemcpyAsync(t, c->fs_type, old * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+        LBIB_HIP(hipMemcpyAsync(p, c->fs_partner, old * sizeof(int32_t), hipMemcpyDeviceToDevice, c->stream));
+        LBIB_HIP(hipMemcpyAsync(q, c->fs_prm, 4 * (size_t)old * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        LBIB_HIP(hipStreamSynchronize(c->stream));
+    }
+    dev_free(c->fs_type); dev_free(c->fs_partner); dev_free(c->fs_prm);
+    c->fs_type = t; c->fs_partner = p; c->fs_prm = q;
+    c->fcap = cap;
+    if (c->junc_on && c->jfcap < c->fcap) {
+        dev_free(c->j_type); dev_free(c->j_partner); dev_free(c->j_prm);
+        if (dev_alloc(c, &c->j_type, c->fcap) || dev_alloc(c, &c->j_partner, c->fcap) ||
+            dev_alloc(c, &c->j_prm, 4 * (size_t)c->fcap))
+            return -1;
+        c->jfcap = c->fcap;
+    }
+    return 0;
+}
+
+// Scratch of the cell-junction rebuild, sized with the markers once the
+// junction BioSolver has run: later growth happens with the marker arrays,
+// never inside a step recorded into a graph.  The force table keeps room for
+// two tension structs, a junction and an average of one incoming junction per
+// marker; a fuller table is an LBIB_ERR_CAPACITY at run time.
+int ensure_junctions(lbib_ctx* c) {
+    if (!c->junc_on) return 0;
+    if (c->jcap < c->mcap) {
+        int32_t** iarr[3] = {&c->j_part, &c->j_src, &c->j_off};
+        uint32_t** uarr[3] = {&c->j_in, &c->j_cnt, &c->j_ofs};
+        for (auto a : iarr) dev_free(*a);
+        for (auto a : uarr) dev_free(*a);
+        if (dev_alloc(c, &c->j_part, c->mcap) || dev_alloc(c, &c->j_src, (size_t)c->mcap * JUNCTION_MAXIN) ||
+            dev_alloc(c, &c->j_off, c->mcap + 1) || dev_alloc(c, &c->j_in, c->mcap) ||
+            dev_alloc(c, &c->j_cnt, c->mcap) || dev_alloc(c, &c->j_ofs, c->mcap))
+            return -1;
+        c->jcap = c->mcap;
+    }
+    if (!c->j_total && dev_alloc(c, &c->j_total, 1)) return -1;
+    if (ensure_force_cap(c, 4 * c->mcap)) return -1;
+    if (c->jfcap < c->fcap) {
+        dev_free(c->j_type); dev_free(c->j_partner); dev_free(c->j_prm);
+        if (dev_alloc(c, &c->j_type, c->fcap) || dev_alloc(c, &c->j_partner, c->fcap) ||
+            dev_alloc(c, &c->j_prm, 4 * (size_t)c->fcap))
+            return -1;
+        c->jfcap = c->fcap;
+    }
     return 0;
 }
 
@@ -350,6 +404,8 @@ void lbib_destroy(lbib_ctx* c) {
     for (auto a : uarr) dev_free(*a);
     for (auto a : iarr) dev_free(*a);
     dev_free(c->fs_prm); dev_free(c->bcd_off); dev_free(c->bcd_sp);
+    dev_free(c->j_part); dev_free(c->j_src); dev_free(c->j_off); dev_free(c->j_in); dev_free(c->j_cnt);
+    dev_free(c->j_ofs); dev_free(c->j_total); dev_free(c->j_type); dev_free(c->j_partner); dev_free(c->j_prm);
     dev_free(c->cand); dev_free(c->best_dist); dev_free(c->best_seq); dev_free(c->bn_dom); dev_free(c->bc_v);
     dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rmark); dev_free(c->rstate);
     dev_free(c->ring_start); dev_free(c->ring_cnt); dev_free(c->ring_off_u); dev_free(c->area);
@@ -696,6 +752,17 @@ int lbib_forces_tension(lbib_ctx* c, double f) {
     if (check_ctx(c)) return -1;
     if (ensure_force_cap(c, 2 * c->mcap)) return -1;
     return launch_forces_tension(c, f);
+}
+int lbib_forces_junctions(lbib_ctx* c, double radius, double k, double l0) {
+    if (check_ctx(c)) return -1;
+    if (!std::isfinite(radius) || radius < 0.0 || !std::isfinite(k) || !std::isfinite(l0)) {
+        c->err = "lbib_forces_junctions: radius must be finite and >= 0, k and l0 finite";
+        return -1;
+    }
+    if (c->mcap == 0) return 0;
+    c->junc_on = true;
+    if (ensure_junctions(c)) return -1;
+    return launch_forces_junctions(c, radius, k, l0);
 }
 int lbib_forces_clear(lbib_ctx* c) {
     if (check_ctx(c)) return -1;

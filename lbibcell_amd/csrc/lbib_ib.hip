@@ -340,6 +340,24 @@ int launch_force_and_spread(lbib_ctx* c) {
     return 0;
 }
 
+__global__ void k_tile_clear(uint32_t* tcnt, uint32_t n, const DevStatus* st) {
+    if (step_stopped(st)) return;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) tcnt[t] = 0u;
+}
+
+int launch_tile_bins(lbib_ctx* c) {
+    const int bs = 256;
+    const uint32_t nt = (uint32_t)c->stx * (uint32_t)c->sty;
+    k_tile_clear<<<std::min(nblocks((long)nt, bs), 1024), bs, 0, c->stream>>>(c->tcnt, nt, c->st);
+    k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
+                                                             c->tact, c->st);
+    if (scan_exclusive(c, c->tcnt, c->toff, nullptr, nt, nt, nullptr)) return -1;
+    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist,
+                                                               c->st);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
 int launch_clear_force(lbib_ctx*) { return 0; }
 
 int launch_interp_move(lbib_ctx* c) {

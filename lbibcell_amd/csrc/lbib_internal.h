@@ -176,6 +176,15 @@ struct lbib_ctx {
     int fcap = 0;
     int32_t *fs_off = nullptr, *fs_type = nullptr, *fs_partner = nullptr;
     double* fs_prm = nullptr;
+    // cell-junction rebuild (lbib_forces_junctions): per-marker closest
+    // partner, incoming finders, new row counts/offsets, and a scratch CSR
+    // of fcap structs the new table is assembled in
+    bool junc_on = false;
+    int jcap = 0, jfcap = 0;
+    int32_t *j_part = nullptr, *j_src = nullptr, *j_off = nullptr;
+    uint32_t *j_in = nullptr, *j_cnt = nullptr, *j_ofs = nullptr, *j_total = nullptr;
+    int32_t *j_type = nullptr, *j_partner = nullptr;
+    double* j_prm = nullptr;
     // connections
     int ccap = 0, nc = 0;
     int32_t *ca = nullptr, *cb = nullptr, *cbc = nullptr;
@@ -347,6 +356,13 @@ int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc
 int launch_coarsen(lbib_ctx* c);
 int launch_forces_clear(lbib_ctx* c);
 int launch_forces_tension(lbib_ctx* c, double f);
+// markers that may name one marker as their closest junction partner
+constexpr int JUNCTION_MAXIN = 16;
+// deleteForceType(7) and one type-7 pair per marker with a closest marker of
+// another domain within radius (needs the junction scratch of ensure_junctions)
+int launch_forces_junctions(lbib_ctx* c, double radius, double k, double l0);
+// markers binned by the 16x16 tile of floor(X), floor(Y) (tcnt/toff/tlist)
+int launch_tile_bins(lbib_ctx* c);
 int launch_topology(lbib_ctx* c, bool coarsen);
 // FP64 <-> storage conversion of the current buffers (mixed precision only)
 int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq);

@@ -41,6 +41,7 @@ _SIG = {
     "lbh_add_mass_solver": (C.c_int, [_P, C.c_char_p]),
     "lbh_add_bio_solver": (C.c_int, [_P, C.c_char_p]),
     "lbh_add_bio_solver_params": (C.c_int, [_P, C.c_int, _D, C.c_uint]),
+    "lbh_apply_bio": (C.c_int, [_P]),
     "lbh_run": (C.c_int, [_P, C.c_uint]),
     "lbh_write_simulation": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
     "lbh_init_solvers_file": (C.c_int, [_P, C.c_char_p]),
@@ -200,12 +201,24 @@ class Simulation:
         p = np.array([force, 0, 0, 0], dtype=np.float64)
         self._chk(self.L.lbh_add_bio_solver_params(self.h, 0, capi.dptr(p), freq))
 
+    def add_junctions(self, radius=1.0, k=0.02, l0=0.5, freq=10):
+        """tutorial_02_BioSolverCellJunction (tutorial_02_BioSolverCellJunction.cpp:37-99)
+        with its constants as defaults: every freq iterations the type-7 springs
+        are rebuilt between each node and its closest node of another cell
+        within radius."""
+        p = np.array([radius, k, l0, 0], dtype=np.float64)
+        self._chk(self.L.lbh_add_bio_solver_params(self.h, 2, capi.dptr(p), freq))
+
     def add_area_regulator(self, target=200.0, kp=1e-5, maxsource=0.004, factor=0.0, freq=1):
         p = np.array([target, kp, maxsource, factor], dtype=np.float64)
         self._chk(self.L.lbh_add_bio_solver_params(self.h, 1, capi.dptr(p), freq))
 
     def run(self, steps):
         self._chk(self.L.lbh_run(self.h, steps))
+
+    def apply_bio(self):
+        """The BioSolvers of the current iteration alone, outside a step."""
+        self._chk(self.L.lbh_apply_bio(self.h))
 
     def write_simulation(self, force, geometry, parameters, solvers):
         """SimulationRunner::writeSimulation: the reference's four text files."""

@@ -928,6 +928,32 @@ static void applyBio(Sim& s) {
                 const unsigned d = s.dom[i];
                 if (d != 0) s.madd[i] = (s.ctype[i] == 1) ? b.p[2] : src[d];   // addMass overwrites
             }
+        } else if (b.kind == 2) {
+            // tutorial_02_BioSolverCellJunction.cpp:47-99 (RADIUS, K, L0 = p[0..2]), one
+            // thread: deleteForceType(7); for each node in id order the closest node
+            // within RADIUS (FastNeighborList.hpp:404-467: removed when distance >
+            // radius) whose getDomainIdOfAdjacentConnections (GeometryNode.cpp:267-283)
+            // differs; K and L0 pass through the text descriptor.  Brute force over
+            // all nodes in id order, so equidistant candidates resolve to the smaller id.
+            deleteForceType(s, 7);
+            const double R = b.p[0], K = via_text(b.p[1]), L0 = via_text(b.p[2]);
+            std::vector<const Marker*> nodes;
+            for (auto& kv : s.markers) nodes.push_back(kv.second.get());
+            for (const Marker* m : nodes) {
+                const unsigned avoid = m->c0->domain;
+                const Marker* best = nullptr;
+                double bd = 0.0;
+                for (const Marker* q : nodes) {
+                    const double d = std::sqrt((q->x - m->x) * (q->x - m->x) + (q->y - m->y) * (q->y - m->y));
+                    if (d > R) continue;
+                    if (q->c0->domain == avoid) continue;
+                    if (!best || d < bd) { best = q; bd = d; }
+                }
+                if (best) {
+                    addForceStruct(s, ForceStruct{7, m->id, best->id, K, L0});
+                    addForceStruct(s, ForceStruct{7, best->id, m->id, K, L0});
+                }
+            }
         }
     }
 }
@@ -1052,6 +1078,29 @@ void orc_add_bio(void* p, int kind, const double* prm, unsigned freq) {
     Sim::Bio b; b.kind = kind; b.freq = freq;
     for (int k = 0; k < 4; k++) b.p[k] = prm ? prm[k] : 0.0;
     static_cast<Sim*>(p)->bio.push_back(b);
+}
+// the BioSolvers of the current iteration alone (SimulationRunner.cpp:95-97)
+int orc_apply_bio(void* p) {
+    Sim* s = static_cast<Sim*>(p);
+    try {
+        applyBio(*s);
+    } catch (const std::exception& e) { s->last_error = e.what(); return -1; }
+    return 0;
+}
+// ForceSolver::forcemap_ flattened in map order (node id, then row order)
+int orc_num_forces(void* p) {
+    long n = 0;
+    for (auto& kv : static_cast<Sim*>(p)->forcemap) n += (long)kv.second.size();
+    return (int)n;
+}
+void orc_get_forces(void* p, int32_t* type, uint32_t* id1, uint32_t* id2, double* prm) {
+    long j = 0;
+    for (auto& kv : static_cast<Sim*>(p)->forcemap)
+        for (auto& f : kv.second) {
+            type[j] = f.type; id1[j] = f.id1; id2[j] = f.id2;
+            prm[4 * j] = f.a; prm[4 * j + 1] = f.b; prm[4 * j + 2] = f.c; prm[4 * j + 3] = f.d;
+            j++;
+        }
 }
 int orc_run(void* p, int steps) {
     Sim* s = static_cast<Sim*>(p);

@@ -37,6 +37,9 @@ _SIG = {
     "orc_enable_box_outlet": (None, [_P, C.c_int]),
     "orc_add_bio": (None, [_P, C.c_int, _D, C.c_uint]),
     "orc_run": (C.c_int, [_P, C.c_int]),
+    "orc_apply_bio": (C.c_int, [_P]),
+    "orc_num_forces": (C.c_int, [_P]),
+    "orc_get_forces": (None, [_P, _I32, _U32, _U32, _D]),
     "orc_set_force_field": (None, [_P, _D, _D]),
     "orc_set_mass": (None, [_P, _D]),
     "orc_set_domains": (None, [_P, _U32]),
@@ -148,6 +151,20 @@ class Oracle:
         p = np.zeros(4)
         p[:len(prm)] = prm
         self.L.orc_add_bio(self.h, kind, _d(p), freq)
+
+    def apply_bio(self):
+        self._chk(self.L.orc_apply_bio(self.h))
+
+    def forces(self):
+        """ForceSolver::forcemap_ in map order: (type, id1, id2, prm (n, 4))."""
+        n = self.L.orc_num_forces(self.h)
+        t = np.empty(max(n, 1), dtype=np.int32)
+        a = np.empty(max(n, 1), dtype=np.uint32)
+        b = np.empty(max(n, 1), dtype=np.uint32)
+        prm = np.empty((max(n, 1), 4))
+        self.L.orc_get_forces(self.h, t.ctypes.data_as(_I32), a.ctypes.data_as(_U32), b.ctypes.data_as(_U32),
+                              _d(prm))
+        return t[:n], a[:n], b[:n], prm[:n]
 
     def run(self, steps):
         self._chk(self.L.orc_run(self.h, steps))
