@@ -170,6 +170,65 @@ def measure_batched(ens, prec_name, steps, warmup, run_ahead):
     return out
 
 
+MS_TAUS = (0.8, 0.9, 1.0, 1.1)
+MS_SOURCE = 1e-3
+JUNCTION = dict(radius=2.0, k=0.02, l0=0.5, freq=10)
+
+
+def multispecies_case(size=4096, ncell=1600, seed=1):
+    """1600 cells r = 30 +- 5 (120 markers each) grown as one touching
+    cluster, and a 4x4 linear reaction matrix with entries N(0, (1e-4)^2)."""
+    from lbibcell_amd import workloads
+    nodes, conns = workloads.packed_tissue(size, size, ncell, 30.0, 5.0, 120, seed=seed)
+    K = np.random.default_rng(seed + 100).normal(0.0, 1e-4, size=(4, 4))
+    return nodes, conns, K
+
+
+def measure_multispecies(size, nodes, conns, K, prec_name, steps, warmup, run_ahead):
+    """Four coupled D2Q5 species (LINEAR kind: R_s = sum_j K[s][j] C_j, plus
+    a source inside cells for the first), membrane tension and cell
+    junctions within 2 lattice units, box outlet; times `steps` coupled
+    steps after `warmup`."""
+    from lbibcell_amd import capi, runner
+    sim = runner.Simulation(size, size, TAU_F, precision=PRECISIONS[prec_name])
+    for s in range(4):
+        sim.add_species(f"LINEAR_{s}", capi.species_spec(capi.SPECIES_LINEAR, MS_TAUS[s], init_inside=0.1,
+                                                          source_inside=MS_SOURCE if s == 0 else 0.0,
+                                                          krow=tuple(K[s])))
+    sim.add_geometry(nodes, conns)
+    sim.setup()
+    sim.init_solvers()
+    sim.add_mass_solver("MassSolverBoxOutlet")
+    sim.add_tension(TENSION, 10)
+    sim.add_junctions(**JUNCTION)
+    sim.set_run_ahead(run_ahead)
+    log(f"[multispecies {prec_name}] setup done; warming up")
+    sim.run(warmup)
+    sim.sync()
+    ctx = sim.context()
+    chosen, rows = ctx.autotune(reps=5)
+    log(f"[multispecies {prec_name}] tuned lattice shape {chosen[:3]} ({chosen[3] * 1e3:.1f} us)")
+    ctx.profile_enable(True)
+    ctx.profile_read()
+    sim.reset_times()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    ms, launches, bpl = ctx.profile_read()
+    ctx.profile_enable(False)
+    ids, _, _ = sim.markers()
+    off, t, _, _ = ctx.get_forces(len(ids))
+    ctx.ns = 4
+    u, v = ctx.get_velocity()
+    fields = dict(speed=np.hypot(u, v), conc=[ctx.get_species(s).sum(axis=0) for s in range(4)], area=sim.areas())
+    out = dict(dt=t1 - t0, kernel_s=ms / 1e3 / max(launches, 1), bytes_per_launch=bpl, markers=len(ids),
+               junction_structs=int(np.sum(t == 7)), times=sim.times(), fields=fields, tuned=chosen, tried=rows)
+    sim.close()
+    return out
+
+
 def measure_adjoint(steps, K, size=512):
     """Adjoint workload: size^2 lattice, 3x3 cells (radius 40, 150 markers),
     the target is the final concentration under hidden parameters (tau_c 0.8,
@@ -244,6 +303,11 @@ def main():
     ap.add_argument("--batched-domains", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--adjoint-steps", type=int, default=1000, help="0 = skip the adjoint measurement")
+    ap.add_argument("--multispecies", default="fp16,fp64",
+                    help="storage precisions of the 4096^2 four-species junction workload ('' = skip); "
+                         "fp64 also serves as the reference of the mixed errors")
+    ap.add_argument("--multispecies-size", type=int, default=4096)
+    ap.add_argument("--multispecies-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the lattice path runs on a single MI355X per process; "
@@ -358,6 +422,45 @@ def main():
                     "after_steps": 5 + args.batched_steps}
             batched[p] = e
         out["batched"] = batched
+    mvars = [p for p in args.multispecies.split(",") if p]
+    if mvars:
+        msz = args.multispecies_size
+        ncell = max(1, round(1600 * (msz / 4096) ** 2))
+        mnodes, mconns, K = multispecies_case(msz, ncell)
+        log(f"multispecies: {msz}x{msz}, {ncell} cells, {len(mnodes)} initial markers")
+        mres = {p: measure_multispecies(msz, mnodes, mconns, K, p, args.multispecies_steps, 5, args.run_ahead)
+                for p in mvars}
+        ms_out = {"workload": f"{msz}x{msz} D2Q9 + 4 coupled D2Q5 species (linear reaction matrix N(0, 1e-4^2), "
+                              f"tau_c {MS_TAUS}, source {MS_SOURCE} inside cells for species 0), {ncell} cells "
+                              f"r=30+-5 (120 markers, remeshed) in one touching cluster, tension {TENSION}, "
+                              f"junctions within {JUNCTION['radius']} lu (k {JUNCTION['k']}, l0 {JUNCTION['l0']}, "
+                              f"every {JUNCTION['freq']} steps), box outlet",
+                  "reaction_matrix": K.tolist(), "steps": args.multispecies_steps, "sites": msz * msz}
+        for p, r in mres.items():
+            achieved = r["bytes_per_launch"] / r["kernel_s"] / 1e9
+            e = {"value": msz * msz * args.multispecies_steps / r["dt"] / 1e6, "unit": "MLUPS",
+                 "ms_per_step": r["dt"] / args.multispecies_steps * 1e3, "dtype": DTYPES[p],
+                 "markers_after_remesh": r["markers"], "junction_structs": r["junction_structs"],
+                 "lattice_shape": {"variant": r["tuned"][0], "workgroup": r["tuned"][1],
+                                   "sites_per_lane": r["tuned"][2]},
+                 "autotune_us": {f"{t[0]}/{t[1]}/{t[2]}": round(t[3] * 1e3, 2) for t in r["tried"]},
+                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": achieved / HBM_PEAK_GBS, "kernel_ms_avg": r["kernel_s"] * 1e3,
+                              "algorithmic_bytes_per_launch": r["bytes_per_launch"],
+                              "frac_of_copy": achieved / copy_gbs if copy_gbs else None},
+                 "host_seconds_per_step": {k: v / max(r["times"]["steps"], 1) for k, v in r["times"].items()
+                                           if k not in ("steps", "stops")}}
+            ref = mres.get("fp64")
+            if ref is not None and p != "fp64":
+                common = sorted(set(r["fields"]["area"]) & set(ref["fields"]["area"]))
+                e["l2_rel_error_vs_fp64"] = {
+                    "speed": l2_rel(r["fields"]["speed"], ref["fields"]["speed"]),
+                    "concentration": [l2_rel(a, b) for a, b in zip(r["fields"]["conc"], ref["fields"]["conc"])],
+                    "cell_area": l2_rel([r["fields"]["area"][d] for d in common],
+                                        [ref["fields"]["area"][d] for d in common]),
+                    "after_steps": 5 + args.multispecies_steps}
+            ms_out[p] = e
+        out["multispecies"] = ms_out
     if args.adjoint_steps > 0:
         log(f"adjoint: {args.adjoint_steps} steps ...")
         out["adjoint"] = measure_adjoint(args.adjoint_steps, 50)

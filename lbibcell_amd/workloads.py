@@ -54,6 +54,47 @@ def hex_tissue(nx, ny, cols, rows, radius=40.0, rjit=4.0, n=150, pos_jit=2.0, se
     return nodes, conns
 
 
+def packed_tissue(nx, ny, ncell, radius=30.0, rjit=5.0, n=120, gap=(0.8, 1.6), margin=12.0, seed=1):
+    """ncell cells of radius radius +- rjit grown as one cluster from the
+    lattice centre: each new cell sits at a gap U(gap) from the membrane of a
+    cell on the cluster front and at least gap[0] from every other, so
+    neighbouring membranes are within junction reach.  Marker radii are
+    jittered by +-0.2.  Domains 1..ncell."""
+    rng = np.random.default_rng(seed)
+    r = radius + rng.uniform(-rjit, rjit, size=ncell)
+    cx = np.empty(ncell)
+    cy = np.empty(ncell)
+    cx[0], cy[0] = nx / 2.0, ny / 2.0
+    fails = np.zeros(ncell, dtype=np.int64)
+    front = [0]
+    placed = 1
+    while placed < ncell:
+        if not front:
+            raise RuntimeError("packed_tissue: no room left for the next cell")
+        j = front[rng.integers(len(front))]
+        th = rng.uniform(0.0, 2.0 * math.pi)
+        d = r[j] + r[placed] + rng.uniform(*gap)
+        x, y = cx[j] + d * math.cos(th), cy[j] + d * math.sin(th)
+        ok = margin + r[placed] <= x <= nx - margin - r[placed] and margin + r[placed] <= y <= ny - margin - r[placed]
+        if ok:
+            sep = np.hypot(cx[:placed] - x, cy[:placed] - y) - r[:placed] - r[placed]
+            ok = sep.min() >= gap[0] - 1e-9
+        if ok:
+            cx[placed], cy[placed] = x, y
+            front.append(placed)
+            placed += 1
+        else:
+            fails[j] += 1
+            if fails[j] > 40:
+                front.remove(j)
+    nodes, conns = [], []
+    for c in range(ncell):
+        nd, cn = circle_cell(cx[c], cy[c], r[c], n, rng, jitter=0.2, first_id=c * n, domain=c + 1)
+        nodes += nd
+        conns += cn
+    return nodes, conns
+
+
 def grid_cells(nx, ny, cols, rows, radius, n, seed=1):
     """cols x rows cells on a square grid (used for small multi-cell cases)."""
     rng = np.random.default_rng(seed)
