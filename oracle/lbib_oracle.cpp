@@ -171,7 +171,12 @@ struct Sim {
     long n_perturb_node = 0, n_perturb_conn = 0;
     // plugins
     bool box_outlet = false;
-    struct Bio { int kind; double p[4]; unsigned freq; std::map<unsigned, double> target; };
+    struct Bio {
+        int kind; double p[4]; unsigned freq;
+        std::map<unsigned, double> target, integ;   // area regulators
+        bool inited = false;                        // differentiation's call_once
+        std::mt19937 gen{1};                        // random-direction division
+    };
     std::vector<Bio> bio;
     int threads = 1;
     std::vector<unsigned> reinit_log;  // sites reinitialised during the last step
@@ -901,6 +906,96 @@ static std::map<unsigned, double> computeAreas(const Sim& s) {
     }
     return out;
 }
+// ------------------------------------------------------- cell division ------
+// BioSolverCellDivision / tutorial_0{1,2}_BioSolverCellDivision (…CellDivision.cpp:42-256)
+using CellDef = std::map<unsigned, std::vector<std::shared_ptr<Conn>>>;
+static CellDef cellDefinition(const Sim& s) {        // updateCellDefinition (:250-256)
+    CellDef d;
+    for (auto& c : s.conns) d[c->domain].push_back(c);
+    return d;
+}
+static bool belowAxis(const Conn& c, double invslope, double intercept) {   // (:184-190)
+    const bool C1 = c.a->y > invslope * c.a->x + intercept;
+    const bool C2 = c.b->y < invslope * c.b->x + intercept;
+    return (C1 && C2) || (!C1 && !C2);
+}
+// getTwoConnectionsLongestAxis (:151-198): the farthest pair of first nodes
+// (boost::geometry::distance, pythagoras: the sum of two squares is
+// commutative, so its term order cannot matter), strict > keeps the first;
+// the cut is the perpendicular bisector of that pair.
+static std::vector<std::shared_ptr<Conn>> longestAxis(const std::vector<std::shared_ptr<Conn>>& cell) {
+    double dist = 0.0;
+    Marker *N1 = nullptr, *N2 = nullptr;
+    for (auto& c1 : cell)
+        for (auto& c2 : cell) {
+            const double dx = c1->a->x - c2->a->x, dy = c1->a->y - c2->a->y;
+            const double t = std::sqrt(dx * dx + dy * dy);
+            if (t > dist) { dist = t; N1 = c1->a; N2 = c2->a; }
+        }
+    if (!N1) throw Fail("could not find cell division plane");
+    const double mx = 0.5 * (N1->x + N2->x), my = 0.5 * (N1->y + N2->y);
+    const double invslope = -(N2->x - N1->x) / (N2->y - N1->y);
+    const double intercept = my - invslope * mx;
+    if (!std::isfinite(invslope) || !std::isfinite(intercept)) throw Fail("could not find cell division plane");
+    std::vector<std::shared_ptr<Conn>> out;
+    for (auto& c : cell)
+        if (belowAxis(*c, invslope, intercept)) out.push_back(c);
+    if (out.size() != 2)
+        throw Fail("More or less than 2 connections found in BioSolver::getTwoConnectionsLongestAxis().");
+    return out;
+}
+// getTwoConnectionsRandomDirection (:200-248): a line through the mean of the
+// first nodes at angle U(0, 2 pi) from the solver's mt19937(1), retried up to
+// 10 times until it cuts exactly two connections.
+static std::vector<std::shared_ptr<Conn>> randomDirection(const std::vector<std::shared_ptr<Conn>>& cell,
+                                                          std::mt19937& gen) {
+    std::uniform_real_distribution<> dis(0.0, 2.0 * M_PI);
+    double cx = 0.0, cy = 0.0;
+    for (auto& c : cell) { cx += c->a->x; cy += c->a->y; }
+    cx /= (double)cell.size();
+    cy /= (double)cell.size();
+    std::vector<std::shared_ptr<Conn>> out;
+    unsigned counter = 0;
+    while (out.size() != 2) {
+        const double invslope = std::tan(dis(gen));
+        const double intercept = cy - invslope * cx;
+        out.clear();
+        for (auto& c : cell)
+            if (belowAxis(*c, invslope, intercept)) out.push_back(c);
+        counter++;
+        if (counter > 10) throw Fail("could not find cell division plane");
+    }
+    return out;
+}
+// divideCell (:94-149): seal the two cut connections, close both halves with
+// new connections (appended), erase the cut ones, and give the ring through
+// the second cut's end node the id max(domain) + 1.
+static void divideCell(Sim& s, Sim::Bio& b, unsigned dom, CellDef& def) {
+    const unsigned newID = def.rbegin()->first + 1;
+    auto aff = (b.kind == 7) ? longestAxis(def[dom]) : randomDirection(def[dom], b.gen);
+    aff[0]->a->c1 = nullptr; aff[0]->b->c0 = nullptr;
+    aff[1]->a->c1 = nullptr; aff[1]->b->c0 = nullptr;
+    if (aff[0]->bc_species != aff[1]->bc_species)
+        throw Fail("The method cannot handle cell division cases with heterogeneous boundary conditions.");
+    const std::vector<int> bc = aff[0]->bc_species;
+    addConnection(s, aff[0]->a, aff[1]->b, dom, bc);
+    addConnection(s, aff[1]->a, aff[0]->b, dom, bc);
+    for (auto& c : aff) {                  // Geometry::eraseConnection (Geometry.cpp:471-491)
+        if (c->a->c1 == c.get()) c->a->c1 = nullptr;
+        if (c->b->c0 == c.get()) c->b->c0 = nullptr;
+        s.conns.erase(std::remove(s.conns.begin(), s.conns.end(), c), s.conns.end());
+    }
+    Conn* start = aff[1]->b->c1;
+    Conn* c1 = start;
+    Conn* c2 = nullptr;
+    start->domain = newID;
+    while (c2 != start) {
+        c2 = c1->b->c1;
+        c2->domain = newID;
+        c1 = c2;
+    }
+}
+
 static void applyBio(Sim& s) {
     for (auto& b : s.bio) {
         if (s.it % b.freq != 0) continue;
@@ -953,6 +1048,91 @@ static void applyBio(Sim& s) {
                     addForceStruct(s, ForceStruct{7, m->id, best->id, K, L0});
                     addForceStruct(s, ForceStruct{7, best->id, m->id, K, L0});
                 }
+            }
+        } else if (b.kind == 3) {
+            // BioSolverHookeanMembraneTension.cpp:47-80 (K, l_resting = p[0..1]): deletes
+            // type 6, adds type-0 springs along both directions of every connection
+            deleteForceType(s, 6);
+            const double K = via_text(b.p[0]), L = via_text(b.p[1]);
+            for (auto& c : s.conns) {
+                addForceStruct(s, ForceStruct{0, c->a->id, c->b->id, K, L});
+                addForceStruct(s, ForceStruct{0, c->b->id, c->a->id, K, L});
+            }
+        } else if (b.kind == 4) {
+            // BioSolverGrowth.cpp:44-66: addMass(source = p[0]) on every cell site
+            for (long i = 0; i < (long)s.f.size(); i++)
+                if (s.dom[i] != 0) s.madd[i] = b.p[0];
+        } else if (b.kind == 5) {
+            // tutorial_02_BioSolverGrowth.cpp:44-82 (MAXSOURCE, THRESHOLD = p[0..1]):
+            // R*R*L > THRESHOLD adds MAXSOURCE, otherwise MAXSOURCE/10
+            int iR = -1, iL = -1;
+            for (size_t k = 0; k < s.sp.size(); k++) {
+                if (s.sp[k].kind == SP_SCHNAKENBERG_R) iR = (int)k;
+                if (s.sp[k].kind == SP_SCHNAKENBERG_L) iL = (int)k;
+            }
+            if (iR < 0 || iL < 0) throw Fail("The CDE Solver with the name tutorial_02_CDESolverD2Q5_R does not exists.");
+            for (long i = 0; i < (long)s.f.size(); i++) {
+                if (s.dom[i] == 0) continue;
+                const double R = cOfSlots(s.g[iR][i]), L = cOfSlots(s.g[iL][i]);
+                s.madd[i] = (R * R * L > b.p[1]) ? b.p[0] : b.p[0] / 10.0;
+            }
+        } else if (b.kind == 6) {
+            // tutorial_01_BioSolverDifferentiation.cpp:42-83 (THRESHOLD = p[0], species
+            // index p[1]): accumulated concentration per domain
+            // (GeometryHandler.cpp:136-147, x from 1), tracker reset once, every
+            // domain after the first below THRESHOLD becomes type 2
+            const int k = (int)b.p[1];
+            std::map<unsigned, double> acc;
+            for (long y = 0; y < s.ny; y++)
+                for (long x = 1; x < s.nx; x++) acc[s.dom[s.idx(x, y)]] += cOfSlots(s.g[k][s.idx(x, y)]);
+            if (!b.inited) {
+                s.celltype_map.clear();
+                s.celltype_map[0] = 0;
+                for (auto& kv : cellDefinition(s)) s.celltype_map[kv.first] = 1;
+                b.inited = true;
+            }
+            if (!acc.empty())
+                for (auto it = ++acc.begin(); it != acc.end(); ++it)
+                    if (it->second < b.p[0]) s.celltype_map[it->first] = 2;
+            copyCellType(s);
+        } else if (b.kind == 7 || b.kind == 8) {
+            // cell division, longest axis (7) or random direction (8); MAXCELLSIZE = p[0]
+            CellDef def = cellDefinition(s);
+            if (s.celltype_map.empty()) {
+                s.celltype_map[0] = 0;
+                for (auto& kv : def) s.celltype_map[kv.first] = 1;
+            }
+            auto areas = computeAreas(s);
+            areas[0] = 0;
+            for (auto it = ++areas.begin(); it != areas.end(); ++it) {
+                if (it->second > b.p[0]) {
+                    divideCell(s, b, it->first, def);
+                    def = cellDefinition(s);
+                    s.celltype_map[def.rbegin()->first] = s.celltype_map[it->first];
+                    cureLattice(s);
+                }
+            }
+            copyCellType(s);
+        } else if (b.kind == 9) {
+            // BioSolverAreaRegulator.cpp:44-130 (TARGETAREA, TARGETAREA2, KP, MAXSOURCE =
+            // p[0..3], KI = 0): cell 1 aims at TARGETAREA2 before iteration 5000;
+            // sources clamped to [0, MAXSOURCE]; cell type 1 sites take MAXSOURCE
+            const double KI = 0.0;
+            if (s.it < 5000) { b.target[1] = b.p[1]; b.integ[1] = 0.0; }
+            else b.target[1] = b.p[0];
+            auto areas = computeAreas(s);
+            for (auto& kv : areas)
+                if (b.target.find(kv.first) == b.target.end()) { b.target[kv.first] = b.p[0]; b.integ[kv.first] = 0.0; }
+            std::map<unsigned, double> err, src;
+            for (auto& kv : areas) err[kv.first] = b.target[kv.first] - areas[kv.first];
+            for (auto& kv : areas) b.integ[kv.first] += err[kv.first];
+            for (auto& kv : areas) {
+                const double u = b.p[2] * err[kv.first] + KI * b.integ[kv.first];
+                src[kv.first] = (u > b.p[3]) ? b.p[3] : ((u < 0.0) ? 0.0 : u);
+            }
+            for (long i = 0; i < (long)s.f.size(); i++) {
+                const unsigned d = s.dom[i];
+                if (d != 0) s.madd[i] = (s.ctype[i] == 1) ? b.p[3] : src[d];
             }
         }
     }
@@ -1086,6 +1266,13 @@ int orc_apply_bio(void* p) {
         applyBio(*s);
     } catch (const std::exception& e) { s->last_error = e.what(); return -1; }
     return 0;
+}
+// GeometryHandler::copyCellTypeToPhysicalNodes with the current tracker map
+void orc_copy_cell_types(void* p) { copyCellType(*static_cast<Sim*>(p)); }
+// FluidSolver::rhoToAdd_ of every site (the mass the next collide adds)
+void orc_get_mass(void* p, double* m) {
+    Sim* s = static_cast<Sim*>(p);
+    for (size_t i = 0; i < s->madd.size(); i++) m[i] = s->madd[i];
 }
 // ForceSolver::forcemap_ flattened in map order (node id, then row order)
 int orc_num_forces(void* p) {

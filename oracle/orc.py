@@ -38,6 +38,8 @@ _SIG = {
     "orc_add_bio": (None, [_P, C.c_int, _D, C.c_uint]),
     "orc_run": (C.c_int, [_P, C.c_int]),
     "orc_apply_bio": (C.c_int, [_P]),
+    "orc_get_mass": (None, [_P, _D]),
+    "orc_copy_cell_types": (None, [_P]),
     "orc_num_forces": (C.c_int, [_P]),
     "orc_get_forces": (None, [_P, _I32, _U32, _U32, _D]),
     "orc_set_force_field": (None, [_P, _D, _D]),
@@ -154,6 +156,14 @@ class Oracle:
 
     def apply_bio(self):
         self._chk(self.L.orc_apply_bio(self.h))
+
+    def copy_cell_types(self):
+        self.L.orc_copy_cell_types(self.h)
+
+    def get_mass(self):
+        m = np.empty((self.ny, self.nx))
+        self.L.orc_get_mass(self.h, _d(m))
+        return m
 
     def forces(self):
         """ForceSolver::forcemap_ in map order: (type, id1, id2, prm (n, 4))."""
