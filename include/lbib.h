@@ -249,6 +249,17 @@ int lbib_area_regulator(lbib_ctx* ctx, double target_area, double kp, double max
  * device op: deleteForceType(6) + two type-6 structs of magnitude f per
  * connection.  Only valid while the force table holds no other types. */
 int lbib_forces_tension(lbib_ctx* ctx, double f);
+/* BioSolverHookeanMembraneTension (BioSolverHookeanMembraneTension.cpp:47-80)
+ * as one device op: two type-0 springs (k, l0) per connection, one per
+ * direction.  The reference deletes type 6 and appends; this op rewrites the
+ * table, so it is valid only while the table is empty (after
+ * ForceSolver::deleteAllForces). */
+int lbib_forces_springs(lbib_ctx* ctx, double k, double l0);
+/* tutorial_02_BioSolverGrowth (tutorial_02_BioSolverGrowth.cpp:44-82) as one
+ * device op: on every cell site addMass(R*R*L > threshold ? maxsource :
+ * maxsource / 10), R and L the concentrations (getC) of species species_r and
+ * species_l; other sites keep this iteration's earlier addMass. */
+int lbib_mass_growth(lbib_ctx* ctx, int species_r, int species_l, double threshold, double maxsource);
 /* tutorial_02_BioSolverCellJunction / BioSolverCellJunction
  * (tutorial_02_BioSolverCellJunction.cpp:47-99) as one device op:
  * deleteForceType(7), then for each marker j in id order the closest marker c

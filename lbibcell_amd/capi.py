@@ -141,6 +141,8 @@ SIGNATURES = {
     "lbib_step_topology": (C.c_int, [_P, C.c_int]),
     "lbib_forces_tension": (C.c_int, [_P, C.c_double]),
     "lbib_forces_junctions": (C.c_int, [_P, C.c_double, C.c_double, C.c_double]),
+    "lbib_forces_springs": (C.c_int, [_P, C.c_double, C.c_double]),
+    "lbib_mass_growth": (C.c_int, [_P, C.c_int, C.c_int, C.c_double, C.c_double]),
     "lbib_forces_clear": (C.c_int, [_P]),
     "lbib_cure_lattice": (C.c_int, [_P]),
     "lbib_set_box_outlet": (C.c_int, [_P, C.c_int]),

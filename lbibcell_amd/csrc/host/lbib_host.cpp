@@ -437,12 +437,14 @@ void ForceSolver::deleteForceType(unsigned type) {
 // ------------------------------------------------------------ bio solvers --
 namespace {
 
-// tutorial_01_BioSolverMembraneTension (tutorial_01_BioSolverMembraneTension.cpp:42-77):
+// tutorial_01_BioSolverMembraneTension (tutorial_01_BioSolverMembraneTension.cpp:42-77;
+// tutorial_02_: FORCE 0.005, BioSolverMembraneTension: FORCE 0.02):
 // every FREQUENCY iterations, type-6 tension of FORCE along both directions of
 // every connection.  FORCE goes through a text descriptor (6 significant digits).
 struct MembraneTension : BioSolver {
     unsigned freq = 10;
     double force = 0.01;
+    explicit MembraneTension(double f = 0.01) : force(f) {}
     unsigned graphPeriod() const override { return freq; }
     void applyBioProcess(SimulationRunner& r) override {
         if (r.parameters().currentIteration % freq != 0) return;
@@ -462,6 +464,133 @@ struct CellJunction : BioSolver {
         if (r.parameters().currentIteration % freq != 0) return;
         r.renewJunctions(radius, k, l0);
     }
+};
+
+// BioSolverHookeanMembraneTension (BioSolverHookeanMembraneTension.cpp:34-80):
+// every 10 iterations type-0 springs K = 0.01, l_resting = 0.05.
+struct HookeanTension : BioSolver {
+    unsigned freq = 10;
+    double k = 0.01, l0 = 0.05;
+    unsigned graphPeriod() const override { return freq; }
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        r.renewSprings(k, l0);
+    }
+};
+
+// BioSolverGrowth (BioSolverGrowth.cpp:33-66): addMass(0.0025) on every cell site.
+struct Growth : BioSolver {
+    unsigned freq = 1;
+    double source = 0.0025;
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        unsigned maxdom = 0;
+        for (auto& c : r.getGeometry().getConnections()) maxdom = std::max(maxdom, c->domain);
+        for (auto& kv : r.getGeometry().cellTypeTrackerMap()) maxdom = std::max(maxdom, kv.first);
+        std::vector<double> t(maxdom + 1, source);
+        t[0] = 0.0;
+        r.addMassByDomain(t);
+    }
+};
+
+// tutorial_02_BioSolverGrowth (tutorial_02_BioSolverGrowth.cpp:33-82): one device op.
+struct ThresholdGrowth : BioSolver {
+    unsigned freq = 1;
+    double maxsource = 0.001, threshold = 1.0;
+    unsigned graphPeriod() const override { return freq; }
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        r.thresholdGrowth(threshold, maxsource);
+    }
+};
+
+// BioSolverAreaRegulator (BioSolverAreaRegulator.cpp:33-130): PI form with
+// KI = 0, cell 1 aiming at TARGETAREA2 = 1000 before iteration 5000, sources
+// clamped to [0, MAXSOURCE]; cell type 1 sites take MAXSOURCE.
+struct AreaRegulatorPI : BioSolver {
+    unsigned freq = 1;
+    double target = 200.0, target2 = 1000.0, kp = 0.00001, maxsource = 0.004;
+    std::map<unsigned, double> targets, integ;
+    void applyBioProcess(SimulationRunner& r) override {
+        const unsigned it = r.parameters().currentIteration;
+        if (it % freq != 0) return;
+        const double KI = 0.0;
+        if (it < 5000) { targets[1] = target2; integ[1] = 0.0; }
+        else targets[1] = target;
+        auto areas = r.computeAreas();
+        for (auto& kv : areas)
+            if (targets.find(kv.first) == targets.end()) { targets[kv.first] = target; integ[kv.first] = 0.0; }
+        std::map<unsigned, double> err, src;
+        for (auto& kv : areas) err[kv.first] = targets[kv.first] - areas[kv.first];
+        for (auto& kv : areas) integ[kv.first] += err[kv.first];
+        for (auto& kv : areas) {
+            const double u = kp * err[kv.first] + KI * integ[kv.first];
+            src[kv.first] = (u > maxsource) ? maxsource : ((u < 0.0) ? 0.0 : u);
+        }
+        auto& ctm = r.getGeometry().cellTypeTrackerMap();
+        unsigned maxdom = 0;
+        for (auto& kv : areas) maxdom = std::max(maxdom, kv.first);
+        for (auto& kv : ctm) maxdom = std::max(maxdom, kv.first);
+        std::vector<double> t(maxdom + 1, 0.0);
+        for (unsigned d = 1; d <= maxdom; d++) {
+            auto c = ctm.find(d);
+            const unsigned type = (c == ctm.end()) ? 1u : c->second;   // copyCellTypeToPhysicalNodes default
+            auto sv = src.find(d);
+            t[d] = (type == 1) ? maxsource : (sv == src.end() ? 0.0 : sv->second);
+        }
+        r.addMassByDomain(t);
+    }
+};
+
+// tutorial_01_BioSolverDifferentiation / BioSolverDifferentiation
+// (…Differentiation.cpp:33-83): every 100 iterations, cells whose accumulated
+// concentration (GeometryHandler.cpp:136-147) is below THRESHOLD = 1.6 become
+// cell type 2.  The tracker is reset once per solver (the reference's
+// std::call_once flag is shared by all instances of its class).
+struct Differentiation : BioSolver {
+    unsigned freq = 100;
+    double threshold = 1.6;
+    std::string species = "tutorial_01_CDESolverD2Q5_SIGNAL";
+    bool inited = false;
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        auto acc = r.computeAccumulatedDomainConcentrations(species);
+        auto& ctm = r.getGeometry().cellTypeTrackerMap();
+        if (!inited) {
+            std::map<unsigned, int> cells;
+            for (auto& c : r.getGeometry().getConnections()) cells[c->domain] = 1;
+            ctm.clear();
+            ctm[0] = 0;
+            for (auto& kv : cells) ctm[kv.first] = 1;
+            inited = true;
+        }
+        if (!acc.empty())
+            for (auto it = ++acc.begin(); it != acc.end(); ++it)
+                if (it->second < threshold) ctm[it->first] = 2;
+        r.copyCellTypeToPhysicalNodes();
+    }
+};
+
+// tutorial_01_/tutorial_02_BioSolverCellDivision (longest axis) and
+// BioSolverCellDivision (random direction): every 100 iterations, cells above
+// MAXCELLSIZE = 380 divide.  The random direction draws from this solver's
+// mt19937(1) (the reference's generator is a function-static shared by all
+// instances of the class).
+struct CellDivision : BioSolver {
+    unsigned freq = 100;
+    double maxsize = 380.0;
+    bool longest = true;
+    std::mt19937 gen{1};
+    void applyBioProcess(SimulationRunner& r) override {
+        if (r.parameters().currentIteration % freq != 0) return;
+        r.divideCells(maxsize, longest, gen);
+    }
+};
+
+// BioSolverEmpty (BioSolverEmpty.cpp:37-48)
+struct Empty : BioSolver {
+    void applyBioProcess(SimulationRunner&) override { std::cout << "I'm BioSolverEmpty" << std::endl; }
+    unsigned graphPeriod() const override { return 0; }
 };
 
 // tutorial_01_BioSolverAreaRegulator (tutorial_01_BioSolverAreaRegulator.cpp:43-100)
@@ -710,17 +839,89 @@ void SimulationRunner::addMassSolver(const std::string& name) {
 
 void SimulationRunner::addBioSolver(const std::string& name) {
     std::unique_ptr<BioSolver> b;
-    if (name == "tutorial_01_BioSolverMembraneTension") b.reset(new MembraneTension());
+    if (name == "tutorial_01_BioSolverMembraneTension") b.reset(new MembraneTension(0.01));
+    else if (name == "tutorial_02_BioSolverMembraneTension") b.reset(new MembraneTension(0.005));
+    else if (name == "BioSolverMembraneTension") b.reset(new MembraneTension(0.02));
+    else if (name == "BioSolverHookeanMembraneTension") b.reset(new HookeanTension());
     else if (name == "tutorial_01_BioSolverAreaRegulator") b.reset(new AreaRegulator());
+    else if (name == "BioSolverAreaRegulator") b.reset(new AreaRegulatorPI());
     else if (name == "tutorial_02_BioSolverCellJunction" || name == "tutorial_01_BioSolverCellJunction" ||
              name == "BioSolverCellJunction")
         b.reset(new CellJunction());
+    else if (name == "BioSolverGrowth") b.reset(new Growth());
+    else if (name == "tutorial_02_BioSolverGrowth") b.reset(new ThresholdGrowth());
+    else if (name == "tutorial_01_BioSolverDifferentiation") b.reset(new Differentiation());
+    else if (name == "BioSolverDifferentiation") {
+        auto* d = new Differentiation();
+        d->species = "CDESolverD2Q5HH";
+        b.reset(d);
+    } else if (name == "tutorial_01_BioSolverCellDivision" || name == "tutorial_02_BioSolverCellDivision")
+        b.reset(new CellDivision());
+    else if (name == "BioSolverCellDivision") {
+        auto* d = new CellDivision();
+        d->longest = false;
+        b.reset(d);
+    } else if (name == "BioSolverEmpty") b.reset(new Empty());
     else throw std::bad_typeid();
     b->name = name;
     bio_.push_back(std::move(b));
 }
 
 void SimulationRunner::addBioSolver(std::unique_ptr<BioSolver> b) { bio_.push_back(std::move(b)); }
+
+void SimulationRunner::addBioSolverWith(int kind, const double* prm, unsigned freq) {
+    if (freq == 0) LBIB_FAIL("frequency must be positive");
+    std::unique_ptr<BioSolver> b;
+    switch (kind) {
+        case 3: {
+            auto* h = new HookeanTension();
+            h->k = prm[0]; h->l0 = prm[1]; h->freq = freq;
+            h->name = "BioSolverHookeanMembraneTension";
+            b.reset(h);
+            break;
+        }
+        case 4: {
+            auto* g = new Growth();
+            g->source = prm[0]; g->freq = freq;
+            g->name = "BioSolverGrowth";
+            b.reset(g);
+            break;
+        }
+        case 5: {
+            auto* g = new ThresholdGrowth();
+            g->maxsource = prm[0]; g->threshold = prm[1]; g->freq = freq;
+            g->name = "tutorial_02_BioSolverGrowth";
+            b.reset(g);
+            break;
+        }
+        case 6: {
+            const int k = (int)prm[1];
+            if (k < 0 || k >= (int)speciesNames_.size()) LBIB_FAIL("differentiation: species index out of range");
+            auto* d = new Differentiation();
+            d->threshold = prm[0]; d->species = speciesNames_[k]; d->freq = freq;
+            d->name = "tutorial_01_BioSolverDifferentiation";
+            b.reset(d);
+            break;
+        }
+        case 7:
+        case 8: {
+            auto* d = new CellDivision();
+            d->maxsize = prm[0]; d->longest = kind == 7; d->freq = freq;
+            d->name = kind == 7 ? "tutorial_01_BioSolverCellDivision" : "BioSolverCellDivision";
+            b.reset(d);
+            break;
+        }
+        case 9: {
+            auto* a = new AreaRegulatorPI();
+            a->target = prm[0]; a->target2 = prm[1]; a->kp = prm[2]; a->maxsource = prm[3]; a->freq = freq;
+            a->name = "BioSolverAreaRegulator";
+            b.reset(a);
+            break;
+        }
+        default: throw std::bad_typeid();
+    }
+    bio_.push_back(std::move(b));
+}
 
 void SimulationRunner::syncMirror() {
     if (mirrorValid_) return;
@@ -1065,6 +1266,163 @@ void SimulationRunner::resolveStop(unsigned it) {
 }
 
 void SimulationRunner::step() { run(1); }
+
+void SimulationRunner::renewSprings(double k, double l0) {
+    auto via_text = [](double v) {
+        std::stringstream ss;
+        ss << v;
+        double r;
+        ss >> r;
+        return r;
+    };
+    const double K = via_text(k), L = via_text(l0);
+    if (forceSolver_.dirty_ || devForceTypes_ != 0) {
+        syncMirror();
+        forceSolver_.deleteForceType(6);
+        for (auto& c : geo_.getConnections()) {
+            forceSolver_.addForce(solver::ForceStruct{0, c->first->id, c->second->id, {K, L, 0, 0}});
+            forceSolver_.addForce(solver::ForceStruct{0, c->second->id, c->first->id, {K, L, 0, 0}});
+        }
+        uploadForces();
+        return;
+    }
+    check(lbib_forces_springs(ctx_, K, L), "Hookean tension");
+    devForceTypes_ |= 1u;
+    mirrorValid_ = false;
+}
+
+void SimulationRunner::addMassByDomain(const std::vector<double>& table) {
+    check(lbib_add_mass_by_domain(ctx_, (int)table.size(), table.data()), "addMass");
+}
+
+void SimulationRunner::thresholdGrowth(double threshold, double maxsource) {
+    const int r = speciesIndex("tutorial_02_CDESolverD2Q5_R"), l = speciesIndex("tutorial_02_CDESolverD2Q5_L");
+    if (r < 0) LBIB_FAIL("The CDE Solver with the name tutorial_02_CDESolverD2Q5_R does not exists.");
+    if (l < 0) LBIB_FAIL("The CDE Solver with the name tutorial_02_CDESolverD2Q5_L does not exists.");
+    check(lbib_mass_growth(ctx_, r, l, threshold, maxsource), "growth");
+}
+
+void SimulationRunner::copyCellTypeToPhysicalNodes() {
+    auto& ctm = geo_.cellTypeTrackerMap();
+    unsigned maxdom = 0;
+    for (auto& c : geo_.getConnections()) {
+        maxdom = std::max(maxdom, c->domain);
+        if (!ctm.count(c->domain)) ctm[c->domain] = 1;
+    }
+    for (auto& kv : ctm) maxdom = std::max(maxdom, kv.first);
+    std::vector<uint32_t> ct(maxdom + 1, 1u);
+    for (auto& kv : ctm) ct[kv.first] = kv.second;
+    settle();
+    check(lbib_set_celltypes(ctx_, (int)ct.size(), ct.data()), "copyCellTypeToPhysicalNodes");
+}
+
+namespace {
+using CellDef = std::map<unsigned, std::vector<std::shared_ptr<geometry::Connection>>>;
+CellDef cellDefinition(const geometry::Geometry& g) {   // updateCellDefinition (…CellDivision.cpp:250-256)
+    CellDef d;
+    for (auto& c : g.getConnections()) d[c->domain].push_back(c);
+    return d;
+}
+bool cutByLine(const geometry::Connection& c, double invslope, double intercept) {   // (…CellDivision.cpp:184-190)
+    const bool C1 = c.first->y > invslope * c.first->x + intercept;
+    const bool C2 = c.second->y < invslope * c.second->x + intercept;
+    return (C1 && C2) || (!C1 && !C2);
+}
+// getTwoConnectionsLongestAxis (…CellDivision.cpp:151-198)
+std::vector<std::shared_ptr<geometry::Connection>> longestAxisCut(
+    const std::vector<std::shared_ptr<geometry::Connection>>& cell) {
+    double dist = 0.0;
+    const geometry::GeometryNode *N1 = nullptr, *N2 = nullptr;
+    for (auto& c1 : cell)
+        for (auto& c2 : cell) {
+            // boost::geometry::distance (pythagoras): a sum of two squares
+            const double dx = c1->first->x - c2->first->x, dy = c1->first->y - c2->first->y;
+            const double t = std::sqrt(dx * dx + dy * dy);
+            if (t > dist) { dist = t; N1 = c1->first; N2 = c2->first; }
+        }
+    if (!N1) LBIB_FAIL("could not find cell division plane");
+    const double mx = 0.5 * (N1->x + N2->x), my = 0.5 * (N1->y + N2->y);
+    const double invslope = -(N2->x - N1->x) / (N2->y - N1->y);
+    const double intercept = my - invslope * mx;
+    if (!std::isfinite(invslope) || !std::isfinite(intercept)) LBIB_FAIL("could not find cell division plane");
+    std::vector<std::shared_ptr<geometry::Connection>> out;
+    for (auto& c : cell)
+        if (cutByLine(*c, invslope, intercept)) out.push_back(c);
+    if (out.size() != 2) LBIB_FAIL("More or less than 2 connections found in BioSolver::getTwoConnectionsLongestAxis().");
+    return out;
+}
+// getTwoConnectionsRandomDirection (…CellDivision.cpp:200-248)
+std::vector<std::shared_ptr<geometry::Connection>> randomDirectionCut(
+    const std::vector<std::shared_ptr<geometry::Connection>>& cell, std::mt19937& gen) {
+    std::uniform_real_distribution<> dis(0.0, 2.0 * M_PI);
+    double cx = 0.0, cy = 0.0;
+    for (auto& c : cell) { cx += c->first->x; cy += c->first->y; }
+    cx /= (double)cell.size();
+    cy /= (double)cell.size();
+    std::vector<std::shared_ptr<geometry::Connection>> out;
+    unsigned counter = 0;
+    while (out.size() != 2) {
+        const double invslope = std::tan(dis(gen));
+        const double intercept = cy - invslope * cx;
+        out.clear();
+        for (auto& c : cell)
+            if (cutByLine(*c, invslope, intercept)) out.push_back(c);
+        counter++;
+        if (counter > 10) LBIB_FAIL("could not find cell division plane");
+    }
+    return out;
+}
+}  // namespace
+
+void SimulationRunner::divideCells(double maxCellSize, bool longestAxis, std::mt19937& gen) {
+    syncMirror();
+    auto& ctm = geo_.cellTypeTrackerMap();
+    CellDef def = cellDefinition(geo_);
+    if (ctm.empty()) {
+        ctm[0] = 0;
+        for (auto& kv : def) ctm[kv.first] = 1;
+    }
+    auto areas = computeAreas();
+    areas[0] = 0;
+    bool divided = false;
+    for (auto it = ++areas.begin(); it != areas.end(); ++it) {
+        if (!(it->second > maxCellSize)) continue;
+        // divideCell (…CellDivision.cpp:94-149)
+        const unsigned dom = it->first;
+        const unsigned newID = def.rbegin()->first + 1;
+        auto aff = longestAxis ? longestAxisCut(def[dom]) : randomDirectionCut(def[dom], gen);
+        aff[0]->first->out = nullptr; aff[0]->second->in = nullptr;
+        aff[1]->first->out = nullptr; aff[1]->second->in = nullptr;
+        if (aff[0]->descriptor != aff[1]->descriptor)
+            LBIB_FAIL("The method cannot handle cell division cases with heterogeneous boundary conditions.");
+        const auto desc = aff[0]->descriptor;
+        geo_.addConnection(aff[0]->first->id, aff[1]->second->id, dom, desc);
+        geo_.addConnection(aff[1]->first->id, aff[0]->second->id, dom, desc);
+        for (auto& c : aff) geo_.eraseConnection(c.get());
+        geometry::Connection* start = aff[1]->second->out;
+        geometry::Connection* c1 = start;
+        geometry::Connection* c2 = nullptr;
+        start->domain = newID;
+        while (c2 != start) {
+            c2 = c1->second->out;
+            c2->domain = newID;
+            c1 = c2;
+        }
+        def = cellDefinition(geo_);
+        ctm[def.rbegin()->first] = ctm[dom];
+        geo_.perturbConnections();   // GeometryHandler::cureLattice step 1
+        divided = true;
+    }
+    if (divided) {
+        // the rest of cureLattice (boundary nodes, scanline) on the device
+        uploadTopology();
+        check(lbib_cure_lattice(ctx_), "cureLattice");
+        check(lbib_read_status(ctx_, &lastStatus), "status");
+        if (lastStatus.bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) throw Exception("cureLattice: " + describe(lastStatus));
+        check(lbib_clear_status(ctx_), "status");
+    }
+    copyCellTypeToPhysicalNodes();
+}
 
 void SimulationRunner::applyBioSolvers() {
     settle();

@@ -223,6 +223,29 @@ class SimulationRunner {
     void renewJunctions(double radius, double k, double l0);
     // the BioSolvers of the current iteration, outside a step
     void applyBioSolvers();
+    // a registry BioSolver with its constants given: 3 Hookean tension (k, l0),
+    // 4 growth (source), 5 threshold growth (maxsource, threshold),
+    // 6 differentiation (threshold, species index), 7 / 8 cell division along
+    // the longest axis / a random direction (max cell size), 9 PI area
+    // regulator (target, target of cell 1, kp, maxsource)
+    void addBioSolverWith(int kind, const double* prm, unsigned freq);
+    // BioSolverHookeanMembraneTension: deleteForceType(6), then type-0 springs
+    // (k, l0 through the text descriptor) along both directions of every connection
+    void renewSprings(double k, double l0);
+    // FluidSolver::addMass on every site of each domain d: table[d] (table[0]
+    // for the extracellular sites), applied by the next collide
+    void addMassByDomain(const std::vector<double>& table);
+    // tutorial_02_BioSolverGrowth on the device (species tutorial_02_CDESolverD2Q5_R / _L)
+    void thresholdGrowth(double threshold, double maxsource);
+    // GeometryHandler::copyCellTypeToPhysicalNodes (GeometryHandler.cpp:267-301)
+    // with the cell-type tracker map; domains missing from it are added as type 1
+    void copyCellTypeToPhysicalNodes();
+    // tutorial_01_BioSolverCellDivision / BioSolverCellDivision
+    // (…CellDivision.cpp:42-256): every cell with an area above maxCellSize is
+    // cut along the perpendicular bisector of its farthest node pair (or a
+    // random direction through its centroid drawn from gen), the new ring
+    // takes the next domain id, then cureLattice
+    void divideCells(double maxCellSize, bool longestAxis, std::mt19937& gen);
     std::map<unsigned, double> computeAreas();            // GeometryHandler.cpp:101-134 (device shoelace)
     // GeometryHandler.cpp:136-147 (device per-domain sums, domains present on the lattice)
     std::map<unsigned, double> computeAccumulatedDomainConcentrations(const std::string& name);

@@ -387,7 +387,7 @@ extern "C" int lbh_add_bio_solver_params(void* p, int kind, const double* prm, u
             b->name = "tutorial_02_BioSolverCellJunction";
             R(h).addBioSolver(std::move(b));
         } else {
-            throw std::bad_typeid();
+            R(h).addBioSolverWith(kind, prm, freq);
         }
     });
 }

@@ -751,7 +751,20 @@ int lbib_step_topology(lbib_ctx* c, int coarsen) {
 int lbib_forces_tension(lbib_ctx* c, double f) {
     if (check_ctx(c)) return -1;
     if (ensure_force_cap(c, 2 * c->mcap)) return -1;
-    return launch_forces_tension(c, f);
+    return launch_forces_pairs(c, 6, f, 0.0);
+}
+int lbib_forces_springs(lbib_ctx* c, double k, double l0) {
+    if (check_ctx(c)) return -1;
+    if (ensure_force_cap(c, 2 * c->mcap)) return -1;
+    return launch_forces_pairs(c, 0, k, l0);
+}
+int lbib_mass_growth(lbib_ctx* c, int species_r, int species_l, double threshold, double maxsource) {
+    if (check_ctx(c)) return -1;
+    if (species_r < 0 || species_r >= c->ns || species_l < 0 || species_l >= c->ns) {
+        c->err = "lbib_mass_growth: species index out of range";
+        return -1;
+    }
+    return launch_mass_growth(c, species_r, species_l, threshold, maxsource);
 }
 int lbib_forces_junctions(lbib_ctx* c, double radius, double k, double l0) {
     if (check_ctx(c)) return -1;

@@ -355,7 +355,11 @@ int launch_remesh(lbib_ctx* c, int rounds);   // also clears the ring counters
 int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc);
 int launch_coarsen(lbib_ctx* c);
 int launch_forces_clear(lbib_ctx* c);
-int launch_forces_tension(lbib_ctx* c, double f);
+// two structs of `type` per marker, one per adjacent connection (type 6:
+// p0 = force; type 0: p0 = K, p1 = resting length)
+int launch_forces_pairs(lbib_ctx* c, int type, double p0, double p1);
+// per-site mass of tutorial_02_BioSolverGrowth from species sR and sL
+int launch_mass_growth(lbib_ctx* c, int sR, int sL, double threshold, double maxsource);
 // markers that may name one marker as their closest junction partner
 constexpr int JUNCTION_MAXIN = 16;
 // deleteForceType(7) and one type-7 pair per marker with a closest marker of

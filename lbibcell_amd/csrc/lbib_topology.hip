@@ -691,24 +691,58 @@ __global__ void k_forces_clear(int32_t* fs_off, DevStatus* st) {
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k <= nm; k += gridDim.x * blockDim.x) fs_off[k] = 0;
 }
 
-// tutorial_01_BioSolverMembraneTension (…MembraneTension.cpp:53-76): after
-// deleteForceType(6) every marker holds exactly its two type-6 structs, one per
-// adjacent connection; with two terms the summation order does not matter.
-__global__ void k_forces_tension(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
-                                 const int32_t* __restrict__ min_c, const int32_t* __restrict__ mout_c, double f,
-                                 int32_t* fs_off, int32_t* fs_type, int32_t* fs_partner, double* fs_prm,
-                                 DevStatus* st) {
+// tutorial_01_BioSolverMembraneTension (…MembraneTension.cpp:53-76; type 6,
+// p0 = f) and BioSolverHookeanMembraneTension (…HookeanMembraneTension.cpp:
+// 55-80; type 0, p0 = K, p1 = l_resting) over a table without other structs:
+// every marker holds exactly its two structs, one per adjacent connection;
+// with two terms the summation order does not matter.
+__global__ void k_forces_pairs(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
+                               const int32_t* __restrict__ min_c, const int32_t* __restrict__ mout_c, int type,
+                               double p0, double p1, int32_t* fs_off, int32_t* fs_type, int32_t* fs_partner,
+                               double* fs_prm, DevStatus* st) {
     if (step_stopped(st)) return;
     const uint32_t nm = st->nm;
     for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
         fs_off[k] = (int32_t)(2 * k);
         if (k + 1 == nm) fs_off[nm] = (int32_t)(2 * nm);
         const uint32_t e = 2 * k;
-        fs_type[e] = 6; fs_partner[e] = cb[mout_c[k]];
-        fs_type[e + 1] = 6; fs_partner[e + 1] = ca[min_c[k]];
+        fs_type[e] = type; fs_partner[e] = cb[mout_c[k]];
+        fs_type[e + 1] = type; fs_partner[e + 1] = ca[min_c[k]];
         for (int q = 0; q < 4; q++) { fs_prm[4 * e + q] = 0.0; fs_prm[4 * e + 4 + q] = 0.0; }
-        fs_prm[4 * e] = f;
-        fs_prm[4 * e + 4] = f;
+        fs_prm[4 * e] = p0;
+        fs_prm[4 * e + 1] = p1;
+        fs_prm[4 * e + 4] = p0;
+        fs_prm[4 * e + 5] = p1;
+    }
+}
+
+// tutorial_02_BioSolverGrowth (tutorial_02_BioSolverGrowth.cpp:54-80): on
+// every cell site addMass(R*R*L > threshold ? maxsource : maxsource/10) with
+// R, L the getC() of the two species (summed in the locally swapped slot
+// order); other sites keep the mass an earlier BioSolver of this iteration
+// gave them (addMass overwrites).
+template <int P>
+__global__ void k_mass_growth(long n, const void* g, int sR, int sL, int nq, const uint32_t* __restrict__ tag,
+                              double threshold, double maxsource, int keep_site, const double* __restrict__ mtab,
+                              int ntab, double* msite, const DevStatus* st) {
+    if (step_stopped(st)) return;
+    using CD = Codec<P>;
+    const typename CD::T* gs = static_cast<const typename CD::T*>(g);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const uint32_t d = tag[i] & DOM_MASK;
+        if (d == 0) {
+            if (!keep_site) msite[i] = (mtab && (int)d < ntab) ? mtab[d] : 0.0;
+            continue;
+        }
+        auto conc = [&](int s) {
+            const int q0 = 5 * s;
+            return c_swapped((double)CD::dec(gs[pidx<P>(q0, i, n, nq)]), (double)CD::dec(gs[pidx<P>(q0 + 1, i, n, nq)]),
+                             (double)CD::dec(gs[pidx<P>(q0 + 2, i, n, nq)]),
+                             (double)CD::dec(gs[pidx<P>(q0 + 3, i, n, nq)]),
+                             (double)CD::dec(gs[pidx<P>(q0 + 4, i, n, nq)]));
+        };
+        const double R = conc(sR), L = conc(sL);
+        msite[i] = (R * R * L > threshold) ? maxsource : maxsource / 10.0;
     }
 }
 
@@ -812,10 +846,39 @@ int launch_forces_clear(lbib_ctx* c) {
     return 0;
 }
 
-int launch_forces_tension(lbib_ctx* c, double f) {
-    k_forces_tension<<<grid_for(c->mcap), 256, 0, c->stream>>>(c->ca, c->cb, c->min_c, c->mout_c, f, c->fs_off,
-                                                               c->fs_type, c->fs_partner, c->fs_prm, c->st);
+int launch_forces_pairs(lbib_ctx* c, int type, double p0, double p1) {
+    k_forces_pairs<<<grid_for(c->mcap), 256, 0, c->stream>>>(c->ca, c->cb, c->min_c, c->mout_c, type, p0, p1,
+                                                             c->fs_off, c->fs_type, c->fs_partner, c->fs_prm, c->st);
     LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_mass_growth(lbib_ctx* c, int sR, int sL, double threshold, double maxsource) {
+    const void* g = c->g[c->cur];
+    const int nq = 5 * std::max(c->ns, 1);
+    const int keep = c->msite_live ? 1 : 0;
+    const double* mt = c->mtab_live ? c->mtab : nullptr;
+    const int gs = (int)std::min<long>((c->n + 255) / 256, 8192);
+    switch (c->prec) {
+        case LBIB_FP64:
+            k_mass_growth<LBIB_FP64><<<gs, 256, 0, c->stream>>>(c->n, g, sR, sL, nq, c->tag, threshold, maxsource, keep,
+                                                                mt, c->ndom, c->msite, c->st);
+            break;
+        case LBIB_FP32:
+            k_mass_growth<LBIB_FP32><<<gs, 256, 0, c->stream>>>(c->n, g, sR, sL, nq, c->tag, threshold, maxsource, keep,
+                                                                mt, c->ndom, c->msite, c->st);
+            break;
+        case LBIB_FP16_DEV:
+            k_mass_growth<LBIB_FP16_DEV><<<gs, 256, 0, c->stream>>>(c->n, g, sR, sL, nq, c->tag, threshold, maxsource,
+                                                                    keep, mt, c->ndom, c->msite, c->st);
+            break;
+        default:
+            k_mass_growth<LBIB_BF16_DEV><<<gs, 256, 0, c->stream>>>(c->n, g, sR, sL, nq, c->tag, threshold, maxsource,
+                                                                    keep, mt, c->ndom, c->msite, c->st);
+            break;
+    }
+    LBIB_HIP(hipGetLastError());
+    c->msite_live = true;
     return 0;
 }
 

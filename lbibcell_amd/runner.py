@@ -209,6 +209,17 @@ class Simulation:
         p = np.array([radius, k, l0, 0], dtype=np.float64)
         self._chk(self.L.lbh_add_bio_solver_params(self.h, 2, capi.dptr(p), freq))
 
+    def add_bio_solver_with(self, kind, prm, freq):
+        """A BioSolver with its constants given (kinds: 3 Hookean tension (k, l0),
+        4 growth (source), 5 threshold growth (maxsource, threshold),
+        6 differentiation (threshold, species index in name order),
+        7 / 8 cell division along the longest axis / a random direction
+        (max cell size), 9 PI area regulator (target, target of cell 1, kp,
+        maxsource))."""
+        p = np.zeros(4)
+        p[:len(prm)] = prm
+        self._chk(self.L.lbh_add_bio_solver_params(self.h, int(kind), capi.dptr(p), freq))
+
     def add_area_regulator(self, target=200.0, kp=1e-5, maxsource=0.004, factor=0.0, freq=1):
         p = np.array([target, kp, maxsource, factor], dtype=np.float64)
         self._chk(self.L.lbh_add_bio_solver_params(self.h, 1, capi.dptr(p), freq))
