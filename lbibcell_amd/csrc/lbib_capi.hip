@@ -389,7 +389,7 @@ void lbib_destroy(lbib_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
-    dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc);
+    dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc); dev_free(c->vsegs);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
     dev_free(c->tslot); dev_free(c->tlist);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
@@ -1004,10 +1004,18 @@ int lbib_set_batch_layout(lbib_ctx* c, int nrect, const lbib_rect* rects) {
     LBIB_HIP(hipStreamSynchronize(c->stream));
     dev_free(c->rtile);
     dev_free(c->rects);
+    dev_free(c->vsegs);
+    c->vseg_len = 0;
+    c->nvseg = 0;
     c->nrect = 0;
     c->live_sites = c->n;
     c->rect_pairs = true;
+    c->rect_host.clear();
     if (nrect == 0) return 0;
+    for (int r = 0; r < nrect; r++) {
+        const lbib_rect& q = rects[r];
+        c->rect_host.insert(c->rect_host.end(), {q.x0, q.y0, q.w, q.h});
+    }
     if (dev_alloc(c, &c->rtile, nt) || dev_alloc(c, &c->rects, nrect)) return -1;
     if (upload(c, c->rtile, owner.data(), nt * sizeof(int32_t)) ||
         upload(c, c->rects, rd.data(), nrect * sizeof(RectDev)))
@@ -1023,6 +1031,7 @@ int lbib_set_batch_layout(lbib_ctx* c, int nrect, const lbib_rect* rects) {
 static bool shape_valid(const lbib_ctx* c, const LatticeShape& s) {
     const bool wg_ok = s.wg == 64 || s.wg == 128 || s.wg == 256;
     if (s.variant == 1) return wg_ok && pairs_apply(c);
+    if (s.variant == 2) return wg_ok && vec_apply(c, s.spt);
     if (s.variant != 0 || !wg_ok) return false;
     return c->prec == LBIB_FP64 ? s.spt == 1 : (s.spt == 1 || s.spt == 2 || s.spt == 4);
 }

@@ -152,6 +152,9 @@ struct lbib_ctx {
     lbib::RectDev* rects = nullptr;
     long live_sites = 0;               // sites inside some rectangle (all sites without a layout)
     bool rect_pairs = true;            // every rectangle width even: paired-lane kernel applies
+    std::vector<int32_t> rect_host;    // x0, y0, w, h per rectangle
+    void* vsegs = nullptr;             // vector lattice kernel: row segments of the atlas
+    int vseg_len = 0, nvseg = 0;
     double* dconc = nullptr;           // per-domain concentration sums (lbib_domain_concentrations)
     int dconc_cap = 0;
     // lattice launch shape (lbib_autotune / lbib_set_lattice_shape)
@@ -269,7 +272,8 @@ int launch_copy_probe(lbib_ctx* c, const void* src, void* dst, long nbytes);
 int launch_init_mixed(lbib_ctx* c, const double* u0_dev);
 int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile);
 // Lattice launch shape: variant 0 = wg-wide row tiles with spt sites per lane
-// (FP64: spt 1), variant 1 = paired-lane 16-bit kernel.
+// (FP64: spt 1), variant 1 = paired-lane 16-bit kernel, variant 2 = vector
+// kernel with spt consecutive sites per lane (8- or 16-byte plane accesses).
 struct LatticeShape {
     int variant, wg, spt;
 };
@@ -278,8 +282,17 @@ inline bool pairs_apply(const lbib_ctx* c) {
     return (c->prec == LBIB_FP16_DEV || c->prec == LBIB_BF16_DEV) && c->nx % 2 == 0 &&
            (c->nrect == 0 || c->rect_pairs);
 }
+// the vector kernel: mixed storage, v sites of 8 or 16 bytes per lane, rows
+// made of whole lane vectors (rectangle origins are multiples of 16)
+inline bool vec_apply(const lbib_ctx* c, int v) {
+    if (c->prec == LBIB_FP64 || c->nx % v) return false;
+    if (c->prec == LBIB_FP32) return v == 2 || v == 4;
+    return v == 4 || v == 8;
+}
 inline LatticeShape lattice_shape(const lbib_ctx* c) {
-    if (c->lat_variant >= 0 && (c->lat_variant == 0 || pairs_apply(c))) return {c->lat_variant, c->lat_wg, c->lat_spt};
+    if (c->lat_variant >= 0 && (c->lat_variant == 0 || (c->lat_variant == 1 && pairs_apply(c)) ||
+                                (c->lat_variant == 2 && vec_apply(c, c->lat_spt))))
+        return {c->lat_variant, c->lat_wg, c->lat_spt};
     if (c->prec == LBIB_FP64) return {0, 256, 1};
     return pairs_apply(c) ? LatticeShape{1, 256, 1} : LatticeShape{0, 256, 2};
 }

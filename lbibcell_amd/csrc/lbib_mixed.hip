@@ -534,6 +534,352 @@ __global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
     *reinterpret_cast<float4*>(a.u + 2 * i0) = make_float4(uo[0], uo[1], uo[2], uo[3]);
 }
 
+// Vector lattice kernel.  Each lane owns V consecutive sites of one row, so
+// one population plane moves in one 8- or 16-byte load and one store per
+// lane.  A pull shifted along x takes the missing site from the neighbouring
+// lane with one wavefront shuffle per plane; lanes 0 and 63 of a wavefront
+// and the lanes holding a periodic frame's first or last site load that one
+// site themselves (all such loads in two branch regions, after every vector
+// load has been issued).
+//
+// A workgroup serves one row segment [xs, xe) of one periodic frame: a chunk
+// of a lattice row, or in a batched atlas a chunk of a row of one rectangle
+// taken from a segment table, so the frame and its relaxation times are
+// uniform over the workgroup and no lane is launched outside a rectangle.
+// Rectangle origins are multiples of 16, so a frame always starts at a lane's
+// first site; a frame may end inside a lane (odd multiple of 2 widths), whose
+// sites past the end are neither stored nor used.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+template <int P, int V>
+struct LaneVec {
+    using C = Codec<P>;
+    using T = typename C::T;
+    static constexpr int NW = V * (int)sizeof(T) / 4;   // dwords per lane
+    uint32_t w[NW];
+    uint32_t edge;   // storage bits of the site shifted in from outside the lane
+    __device__ void load(const T* p) {
+        const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+        if constexpr (NW % 4 == 0) {
+#pragma unroll
+            for (int k = 0; k < NW; k += 4) {
+                const uint4 v = *reinterpret_cast<const uint4*>(q + k);
+                w[k] = v.x; w[k + 1] = v.y; w[k + 2] = v.z; w[k + 3] = v.w;
+            }
+        } else {
+            static_assert(NW == 2, "lane vectors of 8 or a multiple of 16 bytes");
+            const uint2 v = *reinterpret_cast<const uint2*>(q);
+            w[0] = v.x; w[1] = v.y;
+        }
+    }
+    __device__ uint32_t bits(int j) const {
+        if constexpr (sizeof(T) == 2) return (w[j >> 1] >> (16 * (j & 1))) & 0xffffu;
+        else return w[j];
+    }
+    static __device__ float dec(uint32_t b) {
+        if constexpr (sizeof(T) == 2) return C::dec((T)b);
+        else return __uint_as_float(b);
+    }
+    static __device__ uint32_t enc(float x) {
+        if constexpr (sizeof(T) == 2) return (uint32_t)C::enc(x);
+        else return __float_as_uint(x);
+    }
+    static __device__ uint32_t ld1(const T* p) {
+        if constexpr (sizeof(T) == 2) return (uint32_t)*reinterpret_cast<const uint16_t*>(p);
+        else return *reinterpret_cast<const uint32_t*>(p);
+    }
+    // site j of the plane pulled with shift s (0 here, 1 from the west, 2
+    // from the east; je: the lane's last site whose east neighbour is edge)
+    __device__ float pulled(int s, int j, int je) const {
+        if (s == 0) return dec(bits(j));
+        if (s == 1) return dec(j == 0 ? edge : bits(j - 1));
+        if (j == V - 1) return dec(edge);
+        return dec(j >= je ? edge : bits(j + 1));
+    }
+    __device__ void store(T* p) const {
+        uint32_t* q = reinterpret_cast<uint32_t*>(p);
+        if constexpr (NW % 4 == 0) {
+#pragma unroll
+            for (int k = 0; k < NW; k += 4)
+                __builtin_nontemporal_store(u32x4{w[k], w[k + 1], w[k + 2], w[k + 3]},
+                                            reinterpret_cast<u32x4*>(q + k));
+        } else {
+            __builtin_nontemporal_store(u32x2{w[0], w[1]}, reinterpret_cast<u32x2*>(q));
+        }
+    }
+    __device__ void store_first(T* p, int count) const {   // the first count sites
+#pragma unroll
+        for (int j = 0; j < V; j++) {
+            if (j >= count) break;
+            if constexpr (sizeof(T) == 2) reinterpret_cast<uint16_t*>(p)[j] = (uint16_t)bits(j);
+            else reinterpret_cast<uint32_t*>(p)[j] = bits(j);
+        }
+    }
+    __device__ void put(int j, uint32_t b) {
+        if constexpr (sizeof(T) == 2) {
+            if (j & 1) w[j >> 1] |= b << 16;
+            else w[j >> 1] = b;
+        } else {
+            w[j] = b;
+        }
+    }
+};
+
+struct SegDev {
+    int32_t y, xs, xe, r;
+};
+
+// plane -> source row (0 same, 1 south, 2 north) and x shift (0, 1 west, 2 east)
+__device__ constexpr int row_of(int q) {
+    return (q == DN || q == DNE || q == DNW) ? 1 : ((q == DS || q == DSW || q == DSE) ? 2 : 0);
+}
+__device__ constexpr int shift_of(int q) {
+    return (q == DE || q == DNE || q == DSE) ? 1 : ((q == DW || q == DNW || q == DSW) ? 2 : 0);
+}
+
+template <int NS, int FM, int P, int V>
+__global__ __launch_bounds__(256) void k_stream_collide_vec(const MixedArgs a, const SegDev* __restrict__ segs,
+                                                            int nchunk) {
+    using LV = LaneVec<P, V>;
+    using T = typename LV::T;
+    if (step_stopped(a.st)) return;
+    const int L = (int)blockDim.x * V;
+    const uint32_t wg = xcd_contiguous(blockIdx.x, gridDim.x);
+    int y, xs, xe, fx0, fy0, fw, fh, r;
+    if (segs) {
+        const SegDev sg = segs[wg];
+        y = sg.y; xs = sg.xs; xe = sg.xe; r = sg.r;
+        const RectDev& R = a.rects[r];
+        fx0 = R.x0; fy0 = R.y0; fw = R.w; fh = R.h;
+    } else {
+        y = (int)(wg / (uint32_t)nchunk);
+        xs = (int)(wg % (uint32_t)nchunk) * L;
+        xe = min(a.nx, xs + L);
+        r = -1; fx0 = 0; fy0 = 0; fw = a.nx; fh = a.ny;
+    }
+    const int fxe = fx0 + fw;
+    const int lane = (int)threadIdx.x & 63;
+    const int x = xs + V * (int)threadIdx.x;
+    const int nvalid = min(max(xe - x, 0), V);
+    const int xl = min(x, a.nx - V);   // dead lanes keep a valid address
+    const long n = a.n;
+    const long r0 = (long)y * a.nx;
+    const long rS = (long)((y == fy0) ? fy0 + fh - 1 : y - 1) * a.nx;
+    const long rN = (long)((y == fy0 + fh - 1) ? fy0 : y + 1) * a.nx;
+    auto rsel = [&](int k) { return k == 0 ? r0 : (k == 1 ? rS : rN); };
+    const bool w_need = lane == 0 || xl == fx0;
+    const int w_x = (xl == fx0) ? fxe - 1 : xl - 1;
+    const bool e_fend = xl + V - 1 >= fxe - 1;
+    const bool e_need = lane == 63 || e_fend;
+    const int e_x = e_fend ? fx0 : xl + V;
+    const int je = e_fend ? min(max(fxe - 1 - xl, 0), V - 1) : V - 1;
+    const T* fs = static_cast<const T*>(a.fs);
+    const T* gsb = static_cast<const T*>(a.gs);
+    constexpr int NSS = NS > 0 ? NS : 1;
+
+    LV f[9], g[NSS][5];
+#pragma unroll
+    for (int q = 0; q < 9; q++) f[q].load(fs + (long)q * n + rsel(row_of(q)) + xl);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) g[s][q].load(gsb + (long)(5 * s + q) * n + rsel(row_of(q)) + xl);
+    const long i0 = r0 + xl;
+    uint32_t tg[V];
+#pragma unroll
+    for (int k = 0; k < V; k += 2) {
+        const uint2 t = *reinterpret_cast<const uint2*>(a.tag + i0 + k);
+        tg[k] = t.x; tg[k + 1] = t.y;
+    }
+    float Fx[V], Fy[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) { Fx[k] = 0.0f; Fy[k] = 0.0f; }
+    if (a.ftile == nullptr || a.ftile[(long)(y >> 4) * a.ntx + (xl >> 4)]) {
+#pragma unroll
+        for (int k = 0; k < V; k += 2) {
+            const float4 F = *reinterpret_cast<const float4*>(a.F + i0 + k);
+            Fx[k] = F.x; Fy[k] = F.y; Fx[k + 1] = F.z; Fy[k + 1] = F.w;
+        }
+    }
+    // sites shifted in from outside the lane
+    uint32_t wh[9 + 5 * NSS], eh[9 + 5 * NSS];
+    if (w_need) {
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            if (shift_of(q) == 1) wh[q] = LV::ld1(fs + (long)q * n + rsel(row_of(q)) + w_x);
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            wh[9 + 5 * s + DE] = LV::ld1(gsb + (long)(5 * s + DE) * n + r0 + w_x);
+    }
+    if (e_need) {
+#pragma unroll
+        for (int q = 0; q < 9; q++)
+            if (shift_of(q) == 2) eh[q] = LV::ld1(fs + (long)q * n + rsel(row_of(q)) + e_x);
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            eh[9 + 5 * s + DW] = LV::ld1(gsb + (long)(5 * s + DW) * n + r0 + e_x);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+        if (shift_of(q) == 1) {
+            const uint32_t sh = (uint32_t)__shfl_up((int)f[q].bits(V - 1), 1, 64);
+            f[q].edge = w_need ? wh[q] : sh;
+        } else if (shift_of(q) == 2) {
+            const uint32_t sh = (uint32_t)__shfl_down((int)f[q].bits(0), 1, 64);
+            f[q].edge = e_need ? eh[q] : sh;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const uint32_t shw = (uint32_t)__shfl_up((int)g[s][DE].bits(V - 1), 1, 64);
+        g[s][DE].edge = w_need ? wh[9 + 5 * s + DE] : shw;
+        const uint32_t she = (uint32_t)__shfl_down((int)g[s][DW].bits(0), 1, 64);
+        g[s][DW].edge = e_need ? eh[9 + 5 * s + DW] : she;
+    }
+
+    bool anybc = false;
+#pragma unroll
+    for (int k = 0; k < V; k++) anybc |= (tg[k] >> BC_SHIFT) != 0u;
+    float pu[V], pv[V];
+#pragma unroll
+    for (int k = 0; k < V; k++) { pu[k] = 0.0f; pv[k] = 0.0f; }
+    if (FM != 0 || (NS > 0 && anybc)) {
+#pragma unroll
+        for (int k = 0; k < V; k += 2) {
+            const float4 up = *reinterpret_cast<const float4*>(a.u + 2 * (i0 + k));
+            pu[k] = up.x; pv[k] = up.y; pu[k + 1] = up.z; pv[k + 1] = up.w;
+        }
+    }
+    const float tauI = r < 0 ? a.tauI : (float)a.rects[r].tauI;
+    const float tau = r < 0 ? a.tau : (float)a.rects[r].tau;
+    float tIs[NSS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) tIs[s] = r < 0 ? a.sp[s].tauI : (float)a.rects[r].tauIs[s];
+    const uint32_t it = iter_of(a.it, a.st);
+
+    LV of[9], og[NSS][5];
+    float uo[2 * V];
+#pragma unroll
+    for (int k = 0; k < V; k++) {
+        const int xk = xl + k;
+        const long i = i0 + k;
+        const uint32_t dom = tg[k] & DOM_MASK;
+        float d[9];
+#pragma unroll
+        for (int q = 0; q < 9; q++) d[q] = f[q].pulled(shift_of(q), k, je);
+        float gg[NSS][5];
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) gg[s][q] = g[s][q].pulled(shift_of(q), k, je);
+        if (!LV::C::dev) {
+#pragma unroll
+            for (int q = 0; q < 9; q++) d[q] -= wq(q);
+        }
+        float m = 0.0f;
+        if (a.msite) { m = (float)a.msite[i]; if (k < nvalid) a.msite[i] = 0.0; }
+        else if (a.mtab) m = (dom < (uint32_t)a.ntab) ? (float)a.mtab[dom] : 0.0f;
+        if (NS > 0 && (tg[k] >> BC_SHIFT)) {
+            // no-flux cut links: incoming population from the first listed
+            // species' pre-stream state, swapped into every listed species
+            const int inv[5] = {DT, DW, DS, DE, DN};
+            for (int o = 1; o <= 4; o++) {
+                if (!((tg[k] >> (BC_SHIFT + o - 1)) & 1u)) continue;
+                const int dd = inv[o];
+                const long e = (long)(o - 1) * n + i;
+                const int desc = a.bc_desc[e];
+                const int b0 = a.bcd_off[desc], b1 = a.bcd_off[desc + 1];
+                if (b1 <= b0) continue;
+                const long s1 = (long)(5 * a.bcd_sp[b0]) * n + i;
+                const float Cp = LV::dec(LV::ld1(gsb + s1)) + LV::dec(LV::ld1(gsb + s1 + n)) +
+                                 LV::dec(LV::ld1(gsb + s1 + 2 * n)) + LV::dec(LV::ld1(gsb + s1 + 3 * n)) +
+                                 LV::dec(LV::ld1(gsb + s1 + 4 * n));
+                const float c6 = Cp / 6.0f;
+                const float bvx = (float)a.bc_v[2 * e], bvy = (float)a.bc_v[2 * e + 1];
+                float feq, eq;
+                if (dd == DE) { feq = c6 * (1.0f + 3.0f * bvx); eq = c6 * (1.0f + 3.0f * pu[k]); }
+                else if (dd == DN) { feq = c6 * (1.0f + 3.0f * bvy); eq = c6 * (1.0f + 3.0f * pv[k]); }
+                else if (dd == DW) { feq = c6 * (1.0f - 3.0f * bvx); eq = c6 * (1.0f - 3.0f * pu[k]); }
+                else { feq = c6 * (1.0f - 3.0f * bvy); eq = c6 * (1.0f - 3.0f * pv[k]); }
+                float carry = feq + (LV::dec(LV::ld1(gsb + s1 + (long)dd * n)) - eq);
+                for (int kk = b0; kk < b1; kk++) {
+                    const int sk = a.bcd_sp[kk];
+#pragma unroll
+                    for (int s = 0; s < NS; s++)
+                        if (s == sk) { const float t = gg[s][dd]; gg[s][dd] = carry; carry = t; }
+                }
+            }
+        }
+        float uxn, uyn;
+        collide_dev_f32<FM>(d, m, Fx[k], Fy[k], tauI, tau, pu[k], pv[k], uxn, uyn);
+        if (NS > 0) {
+            float cj[NSS], cseq[NSS];
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                cj[s] = gg[s][0] + gg[s][1] + gg[s][2] + gg[s][3] + gg[s][4];
+                cseq[s] = cj[s];
+            }
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                species_collide_f32<NS>(gg[s], cj[s], uxn, uyn, a.sp[s], tIs[s], dom, it, cj, cseq);
+                // a later species sees this one's concentration after its collide
+                cseq[s] = gg[s][0] + gg[s][1] + gg[s][2] + gg[s][3] + gg[s][4];
+            }
+        }
+        if (a.box_outlet && (xk == fx0 || y == fy0)) {
+            const int passes = (xk == fx0) + (y == fy0);
+            for (int p = 0; p < passes; p++) {
+                float sd = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 9; q++) sd += d[q];
+                const float sf = 1.0f / (1.0f + sd);
+#pragma unroll
+                for (int q = 0; q < 9; q++) d[q] = (wq(q) + d[q]) * sf - wq(q);
+#pragma unroll
+                for (int s = 0; s < NS; s++)
+#pragma unroll
+                    for (int q = 0; q < 5; q++) gg[s][q] *= sf;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 9; q++) of[q].put(k, LV::enc(LV::C::dev ? d[q] : d[q] + wq(q)));
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) og[s][q].put(k, LV::enc(gg[s][q]));
+        uo[2 * k] = uxn;
+        uo[2 * k + 1] = uyn;
+    }
+    if (nvalid <= 0) return;
+    T* fd = static_cast<T*>(a.fd);
+    T* gd = static_cast<T*>(a.gd);
+    if (nvalid == V) {
+#pragma unroll
+        for (int q = 0; q < 9; q++) of[q].store(fd + (long)q * n + i0);
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) og[s][q].store(gd + (long)(5 * s + q) * n + i0);
+#pragma unroll
+        for (int k = 0; k < V; k += 2)
+            *reinterpret_cast<float4*>(a.u + 2 * (i0 + k)) = make_float4(uo[2 * k], uo[2 * k + 1], uo[2 * k + 2],
+                                                                          uo[2 * k + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; q++) of[q].store_first(fd + (long)q * n + i0, nvalid);
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) og[s][q].store_first(gd + (long)(5 * s + q) * n + i0, nvalid);
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+            if (k >= nvalid) break;
+            *reinterpret_cast<float2*>(a.u + 2 * (i0 + k)) = make_float2(uo[2 * k], uo[2 * k + 1]);
+        }
+    }
+}
+
 // FP64 natural-order fields <-> storage (current buffers); field 0 = fluid.
 // src/dst planes q0 .. q0 + count/n - 1 of nq in the storage blocks.
 template <int P>
@@ -609,9 +955,55 @@ static void launch_mx_spt(lbib_ctx* c, const MixedArgs& a, int wg, int spt) {
     else k_stream_collide_mx<NS, FM, P, 4><<<grid, wg, 0, c->stream>>>(a);
 }
 
+// Segment table of the vector kernel for chunks of len sites: every row of
+// every rectangle cut into chunks, rectangle-major and row-major inside a
+// rectangle so that the rows one chunk pulls from sit in the same XCD's
+// range of workgroups.  Not needed without an atlas.
+static int ensure_segments(lbib_ctx* c, int len) {
+    if (c->nrect == 0 || (c->vsegs && c->vseg_len == len)) return 0;
+    if (c->capturing) {
+        c->err = "vector lattice kernel: segment table must be built before a step is recorded";
+        return -1;
+    }
+    std::vector<SegDev> t;
+    for (int r = 0; r < c->nrect; r++) {
+        const int x0 = c->rect_host[4 * r], y0 = c->rect_host[4 * r + 1];
+        const int w = c->rect_host[4 * r + 2], h = c->rect_host[4 * r + 3];
+        for (int y = y0; y < y0 + h; y++)
+            for (int xs = x0; xs < x0 + w; xs += len) t.push_back({y, xs, std::min(xs + len, x0 + w), r});
+    }
+    if (c->vsegs) { (void)hipFree(c->vsegs); c->vsegs = nullptr; c->nvseg = 0; c->vseg_len = 0; }
+    c->alloc_gen++;
+    LBIB_HIP(hipMalloc(&c->vsegs, t.size() * sizeof(SegDev)));
+    LBIB_HIP(hipMemcpyAsync(c->vsegs, t.data(), t.size() * sizeof(SegDev), hipMemcpyHostToDevice, c->stream));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    c->nvseg = (int)t.size();
+    c->vseg_len = len;
+    return 0;
+}
+
+template <int NS, int FM, int P, int V>
+static void launch_vec(lbib_ctx* c, const MixedArgs& a, int wg) {
+    if (c->nrect) {
+        k_stream_collide_vec<NS, FM, P, V><<<c->nvseg, wg, 0, c->stream>>>(
+            a, static_cast<const SegDev*>(c->vsegs), 0);
+    } else {
+        const int nchunk = (c->nx + wg * V - 1) / (wg * V);
+        k_stream_collide_vec<NS, FM, P, V><<<nchunk * c->ny, wg, 0, c->stream>>>(a, nullptr, nchunk);
+    }
+}
+
 template <int NS, int FM, int P>
 static void launch_mx_shape(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
-    if (sh.variant == 1) {
+    if (sh.variant == 2) {
+        if constexpr (sizeof(typename Codec<P>::T) == 2) {
+            if (sh.spt == 8) launch_vec<NS, FM, P, 8>(c, a, sh.wg);
+            else launch_vec<NS, FM, P, 4>(c, a, sh.wg);
+        } else {
+            if (sh.spt == 4) launch_vec<NS, FM, P, 4>(c, a, sh.wg);
+            else launch_vec<NS, FM, P, 2>(c, a, sh.wg);
+        }
+    } else if (sh.variant == 1) {
         dim3 grid((c->nx + 2 * sh.wg - 1) / (2 * sh.wg), c->ny);
         k_stream_collide_h16<NS, FM, P><<<grid, sh.wg, 0, c->stream>>>(a);
     } else {
@@ -638,6 +1030,7 @@ static void dispatch_mx_fm(lbib_ctx* c, const MixedArgs& a, const LatticeShape& 
 }
 
 static int launch_mixed_shape(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
+    if (sh.variant == 2 && ensure_segments(c, sh.wg * sh.spt)) return -1;
     if (c->prec == LBIB_FP32) dispatch_mx_fm<LBIB_FP32>(c, a, sh);
     else if (c->prec == LBIB_FP16_DEV) dispatch_mx_fm<LBIB_FP16_DEV>(c, a, sh);
     else dispatch_mx_fm<LBIB_BF16_DEV>(c, a, sh);
@@ -727,6 +1120,9 @@ int autotune_mixed(lbib_ctx* c, int reps, std::vector<TunedShape>& out) {
         for (int spt : {1, 2, 4}) cand.push_back({0, wg, spt});
     if (pairs_apply(c))
         for (int wg : {64, 128, 256}) cand.push_back({1, wg, 1});
+    for (int v : {2, 4, 8})
+        if (vec_apply(c, v))
+            for (int wg : {64, 128, 256}) cand.push_back({2, wg, v});
     for (const LatticeShape& sh : cand) {
         double ms = 0.0;
         if (sc.time(c, reps, [&] { return launch_mixed_shape(c, a, sh); }, ms)) return -1;

@@ -35,6 +35,8 @@ def _shapes(prec, even):
     s = [(0, wg, spt) for wg in (64, 128, 256) for spt in (1, 2, 4)]
     if prec in ("fp16", "bf16") and even:
         s += [(1, wg, 1) for wg in (64, 128, 256)]
+    # vector kernel (every atlas here is 112 or 512 sites wide)
+    s += [(2, wg, v) for v in ((2, 4) if prec == "fp32" else (4, 8)) for wg in (64, 128, 256)]
     return s
 
 
