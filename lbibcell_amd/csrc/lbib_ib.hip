@@ -141,7 +141,12 @@ __global__ void k_tile_scatter(int ntx, const double* __restrict__ mx, const dou
     tlist[toff[(fy / ST) * ntx + fx / ST] + tslot[k]] = k;
 }
 
-// one workgroup per tile; tiles no marker stencil reaches return at once.
+// One workgroup per tile; tiles no marker stencil reaches return at once.
+// The gathered markers are ranked by index and staged SP_CH at a time with
+// their eight delta factors; each staged marker sets its bit in a per-site
+// mask of the sites its stencil covers in this tile, and every lane then
+// walks only the set bits of its site's mask, lowest first, so it adds
+// delta*F of exactly the covering markers in rank order.
 // F32OUT: mixed-precision lattices take the force as FP32 pairs in Fx's
 // allocation (the sums themselves are FP64 either way).
 template <bool F32OUT>
@@ -151,15 +156,18 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
                                                      const double* __restrict__ my, const double* __restrict__ mfx,
                                                      const double* __restrict__ mfy, double* Fx, double* Fy,
                                                      DevStatus* st) {
+    static_assert(SP_CH == 128, "two 64-bit site masks per staged chunk");
     const int t = (int)xcd_contiguous(blockIdx.x, gridDim.x);
     if (!tact[t] || step_stopped(st)) return;
     __shared__ int32_t idx[SP_IDX];
     __shared__ int32_t sorted[SP_IDX];
     __shared__ double fxa[SP_CH][4], fya[SP_CH][4], sfx[SP_CH], sfy[SP_CH];
     __shared__ int32_t s0x[SP_CH], s0y[SP_CH];
+    __shared__ unsigned long long smask[2][ST * ST];
     __shared__ uint32_t hcnt[9], hoff[9], hpre[10];
     __shared__ uint32_t n;
     const int tx = t % ntx, ty = t / ntx;
+    const int bx = tx * ST, by = ty * ST;
     // the 3x3 home tiles' lists, loaded at once
     if (threadIdx.x < 9) {
         const int hx = tx + (int)(threadIdx.x % 3) - 1, hy = ty + (int)(threadIdx.x / 3) - 1;
@@ -169,6 +177,8 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
         hoff[threadIdx.x] = in ? toff[h] : 0u;
     }
     if (threadIdx.x == 0) n = 0;
+    smask[0][threadIdx.x] = 0ull;
+    smask[1][threadIdx.x] = 0ull;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = 0;
@@ -182,7 +192,7 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
         const int k = tlist[hoff[h] + (j - hpre[h])];
         // keep only markers whose 4x4 stencil overlaps this tile
         const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
-        if (fx + 2 < tx * ST || fx - 1 > tx * ST + ST - 1 || fy + 2 < ty * ST || fy - 1 > ty * ST + ST - 1) continue;
+        if (fx + 2 < bx || fx - 1 > bx + ST - 1 || fy + 2 < by || fy - 1 > by + ST - 1) continue;
         const uint32_t p = atomicAdd(&n, 1u);
         if (p < (uint32_t)SP_IDX) idx[p] = k;
     }
@@ -199,7 +209,8 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
         sorted[r] = v;
     }
     __syncthreads();
-    const int px = tx * ST + (int)(threadIdx.x % ST), py = ty * ST + (int)(threadIdx.x / ST);
+    const int lx = (int)(threadIdx.x % ST), ly = (int)(threadIdx.x / ST);
+    const int px = bx + lx, py = by + ly;
     double ax = 0.0, ay = 0.0;
     for (uint32_t c0 = 0; c0 < m; c0 += SP_CH) {
         const uint32_t mc = min((uint32_t)SP_CH, m - c0);
@@ -213,14 +224,26 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
             ib_factors(y0, Y, fya[j]);
             sfx[j] = mfx[k];
             sfy[j] = mfy[k];
+            const unsigned long long bit = 1ull << (j & 63u);
+            for (int qy = 0; qy < 4; qy++)
+                for (int qx = 0; qx < 4; qx++) {
+                    const int sx = x0 + qx - bx, sy = y0 + qy - by;
+                    if ((unsigned)sx < (unsigned)ST && (unsigned)sy < (unsigned)ST)
+                        atomicOr(&smask[j >> 6][sy * ST + sx], bit);
+                }
         }
         __syncthreads();
-        for (uint32_t j = 0; j < mc; j++) {
-            const int qx = px - s0x[j], qy = py - s0y[j];
-            if ((unsigned)qx > 3u || (unsigned)qy > 3u) continue;
-            const double dd = (fxa[j][qx] * fya[j][qy]) / 16.0;
-            ax += dd * sfx[j];
-            ay += dd * sfy[j];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            unsigned long long mk = smask[h][threadIdx.x];
+            while (mk) {
+                const int j = 64 * h + __ffsll((long long)mk) - 1;
+                mk &= mk - 1ull;
+                const double dd = (fxa[j][px - s0x[j]] * fya[j][py - s0y[j]]) / 16.0;
+                ax += dd * sfx[j];
+                ay += dd * sfy[j];
+            }
+            smask[h][threadIdx.x] = 0ull;
         }
         __syncthreads();
     }
