@@ -84,11 +84,13 @@ def cpu_baseline(nx, ny, nodes, conns, budget_s=15.0, max_steps=50):
                       f"oracle with {threads} OpenMP threads on the lattice loops, serial IB/geometry"}
 
 
-PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3}
+PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3, "fp16r": 4}
 DTYPES = {"fp64": "f64", "fp32": "f32", "fp16": "f32 arithmetic, f16 storage (f - w)",
-          "bf16": "f32 arithmetic, bf16 storage (f - w)"}
+          "bf16": "f32 arithmetic, bf16 storage (f - w)",
+          "fp16r": "f32 arithmetic, f32 moments + f16 remainders (f - feq) storage"}
 KERNELS = {"fp64": "k_stream_collide_f64<1,0>", "fp32": "k_stream_collide_mx<1,0,FP32>",
-           "fp16": "k_stream_collide_mx<1,0,FP16_DEV>", "bf16": "k_stream_collide_mx<1,0,BF16_DEV>"}
+           "fp16": "k_stream_collide_mx<1,0,FP16_DEV>", "bf16": "k_stream_collide_mx<1,0,BF16_DEV>",
+           "fp16r": "k_stream_collide_rem<1,0>"}
 
 
 def measure(nx, ny, nodes, conns, prec_name, steps, warmup, run_ahead):
@@ -291,19 +293,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--precision", default="fp64", choices=list(PRECISIONS),
                     help="storage precision of the headline line (fp64 is the parity path)")
-    ap.add_argument("--mixed", default="fp32,fp16,bf16",
+    ap.add_argument("--mixed", default="fp32,fp16,bf16,fp16r",
                     help="comma-separated mixed precisions also measured and compared with fp64 ('' = none)")
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--run-ahead", type=int, default=16,
                     help="iterations enqueued between device status reads")
-    ap.add_argument("--batched", default="fp16,fp32,fp64",
+    ap.add_argument("--batched", default="fp16,fp16r,fp32,fp64",
                     help="storage precisions of the batched-ensemble measurement ('' = skip); "
                          "fp64 also serves as the reference of the mixed errors")
     ap.add_argument("--batched-domains", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--adjoint-steps", type=int, default=1000, help="0 = skip the adjoint measurement")
-    ap.add_argument("--multispecies", default="fp16,fp64",
+    ap.add_argument("--multispecies", default="fp16,fp16r,fp64",
                     help="storage precisions of the 4096^2 four-species junction workload ('' = skip); "
                          "fp64 also serves as the reference of the mixed errors")
     ap.add_argument("--multispecies-size", type=int, default=4096)

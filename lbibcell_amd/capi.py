@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liblbib_hip.so")
 
 MAX_SPECIES = 4
-FP64, FP32, FP16_DEV, BF16_DEV = 0, 1, 2, 3
+FP64, FP32, FP16_DEV, BF16_DEV, FP16_REM = 0, 1, 2, 3, 4
 FORCE_LUO, FORCE_GUO_AS_WRITTEN, FORCE_GUO = 0, 1, 2
 SPECIES_PLAIN, SPECIES_SIGNAL, SPECIES_LINEAR = 0, 1, 2
 SPECIES_SCHNAKENBERG_R, SPECIES_SCHNAKENBERG_L = 3, 4

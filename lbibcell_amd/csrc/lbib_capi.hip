@@ -51,6 +51,11 @@ size_t storage_bytes(int prec) {
         default: return 2;
     }
 }
+// bytes of one site's fluid (planes 9) or species (planes 5) populations
+size_t field_bytes(int prec, int planes) {
+    if (prec == LBIB_FP16_REM) return planes == 9 ? 3 * 4 + 9 * 2 : 4 + 5 * 2;
+    return (size_t)planes * storage_bytes(prec);
+}
 
 int ensure_scan(lbib_ctx* c) {
     const uint32_t need =
@@ -283,7 +288,7 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
         return -1;
     }
     if (cfg->precision != LBIB_FP64 && cfg->precision != LBIB_FP32 && cfg->precision != LBIB_FP16_DEV &&
-        cfg->precision != LBIB_BF16_DEV) {
+        cfg->precision != LBIB_BF16_DEV && cfg->precision != LBIB_FP16_REM) {
         c->err = "unknown precision";
         *out = c;
         return -1;
@@ -327,12 +332,11 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
         }
     }
     LBIB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    const size_t sb = storage_bytes(c->prec);
     const long n = c->n;
     const long npad = (n + 63) / 64 * 64;   // whole 64-site blocks for 32- and 16-bit storage
     for (int b = 0; b < 2; b++) {
-        if (dev_alloc(c, (uint8_t**)&c->f[b], 9 * npad * sb)) return -1;
-        if (dev_alloc(c, (uint8_t**)&c->g[b], (size_t)std::max(c->ns, 1) * 5 * npad * sb)) return -1;
+        if (dev_alloc(c, (uint8_t**)&c->f[b], npad * field_bytes(c->prec, 9))) return -1;
+        if (dev_alloc(c, (uint8_t**)&c->g[b], (size_t)std::max(c->ns, 1) * npad * field_bytes(c->prec, 5))) return -1;
     }
     if (dev_alloc(c, &c->u, 2 * n) || dev_alloc(c, &c->Fx, n) || dev_alloc(c, &c->Fy, n) ||
         dev_alloc(c, &c->msite, n))
@@ -802,7 +806,8 @@ int lbib_profile_read(lbib_ctx* c, double* kernel_ms, int64_t* launches, double*
     if (kernel_ms) *kernel_ms = ms;
     if (launches) *launches = (int64_t)(c->ev_used / 2);
     if (bytes_per_launch)
-        *bytes_per_launch = (double)(2 * 9 + 2 * 5 * c->ns) * (double)storage_bytes(c->prec) * (double)c->live_sites;
+        *bytes_per_launch = 2.0 * (double)(field_bytes(c->prec, 9) + c->ns * field_bytes(c->prec, 5)) *
+                            (double)c->live_sites;
     c->ev_used = 0;
     return 0;
 }
@@ -1031,9 +1036,9 @@ int lbib_set_batch_layout(lbib_ctx* c, int nrect, const lbib_rect* rects) {
 static bool shape_valid(const lbib_ctx* c, const LatticeShape& s) {
     const bool wg_ok = s.wg == 64 || s.wg == 128 || s.wg == 256;
     if (s.variant == 1) return wg_ok && pairs_apply(c);
-    if (s.variant == 2) return wg_ok && vec_apply(c, s.spt);
+    if (s.variant == 2) return wg_ok && (c->prec == LBIB_FP16_REM ? s.spt == 1 : vec_apply(c, s.spt));
     if (s.variant != 0 || !wg_ok) return false;
-    return c->prec == LBIB_FP64 ? s.spt == 1 : (s.spt == 1 || s.spt == 2 || s.spt == 4);
+    return (c->prec == LBIB_FP64 || c->prec == LBIB_FP16_REM) ? s.spt == 1 : (s.spt == 1 || s.spt == 2 || s.spt == 4);
 }
 int lbib_set_lattice_shape(lbib_ctx* c, const lbib_lattice_shape* shape) {
     if (check_ctx(c) || !shape) return -1;

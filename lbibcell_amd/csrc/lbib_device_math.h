@@ -69,6 +69,74 @@ __device__ __forceinline__ long pidx(int q, long i, long n, int nq) {
     return (long)q * n + i;
 }
 
+// LBIB_FP16_REM: FP32 moments plus FP16 remainders.
+//   fluid block:   float M[3][n] = (rho - 1, jx, jy) of the stored populations,
+//                  then uint16 R[9][n] = FP16 of (f_q - w_q - deq_q(M)) * REM_SCALE
+//                  with deq_q = w_q (rho - 1) + w_q rho (3 e.u + 9/2 (e.u)^2 - 3/2 u^2)
+//   species block: float C[ns][n] (sum of the five populations), then uint16
+//                  R[ns][5][n] = FP16 of (g_q - t_q C) * REM_SCALE, t = (1/3, 1/6 x 4)
+// The moments carry FP32 precision; a remainder is small (the
+// non-equilibrium part), so its FP16 rounding is small in absolute terms.
+// Decoding evaluates the same FP32 expressions as encoding.  REM_SCALE lifts
+// remainders of order 1e-8 out of the FP16 subnormal range; a remainder
+// beyond 15.9 overflows to infinity (populations are O(1)).
+constexpr float REM_SCALE = 4096.0f;
+constexpr float REM_INV = 1.0f / 4096.0f;
+__device__ __forceinline__ float rem_w(int q) { return q == 0 ? 4.0f / 9.0f : (q < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
+__device__ __forceinline__ float rem_t(int q) { return q == 0 ? 1.0f / 3.0f : 1.0f / 6.0f; }
+__device__ __forceinline__ int rem_ex(int q) { return (q == 1 || q == 5 || q == 8) ? 1 : ((q == 3 || q == 6 || q == 7) ? -1 : 0); }
+__device__ __forceinline__ int rem_ey(int q) { return (q == 2 || q == 5 || q == 6) ? 1 : ((q == 4 || q == 7 || q == 8) ? -1 : 0); }
+// equilibrium deviation deq_q of the moments (rho - 1, jx, jy)
+__device__ __forceinline__ float rem_deq(int q, float rm1, float jx, float jy) {
+    const float rho = 1.0f + rm1;
+    const float rhoI = 1.0f / rho;
+    const float ux = jx * rhoI, uy = jy * rhoI;
+    const float u2 = 1.5f * (ux * ux + uy * uy);
+    const float eu = (float)rem_ex(q) * ux + (float)rem_ey(q) * uy;
+    const float w = rem_w(q);
+    return w * rm1 + w * rho * (3.0f * eu + 4.5f * eu * eu - u2);
+}
+__device__ __forceinline__ uint16_t rem_enc(float r) { return __builtin_bit_cast(uint16_t, (_Float16)(r * REM_SCALE)); }
+__device__ __forceinline__ float rem_dec(uint16_t v) { return (float)__builtin_bit_cast(_Float16, v) * REM_INV; }
+// moments of fluid deviations d[9] (the sums the encoder stores)
+__device__ __forceinline__ void rem_moments(const float d[9], float& rm1, float& jx, float& jy) {
+    rm1 = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7])) + d[8];
+    jx = (d[1] + d[5] + d[8]) - (d[3] + d[6] + d[7]);
+    jy = (d[2] + d[5] + d[6]) - (d[4] + d[7] + d[8]);
+}
+__device__ __forceinline__ float rem_csum(const float g[5]) { return (g[0] + g[1]) + (g[2] + g[3]) + g[4]; }
+
+// Species populations of one storage block of nq = 5 * max(ns, 1) planes, for
+// code that handles one site at a time (re-initialisation, per-domain sums):
+// get returns a population in FP64, put5 stores the five of one species.
+template <int P>
+struct GStore {
+    __device__ static double get(const void* g, int s, int q, long i, long n, int nq) {
+        return (double)Codec<P>::dec(static_cast<const typename Codec<P>::T*>(g)[pidx<P>(5 * s + q, i, n, nq)]);
+    }
+    __device__ static void put5(void* g, int s, long i, long n, int nq, const double d[5]) {
+        typename Codec<P>::T* gb = static_cast<typename Codec<P>::T*>(g);
+        for (int q = 0; q < 5; q++) gb[pidx<P>(5 * s + q, i, n, nq)] = Codec<P>::enc(d[q]);
+    }
+};
+template <>
+struct GStore<LBIB_FP16_REM> {
+    __device__ static double get(const void* g, int s, int q, long i, long n, int nq) {
+        const float* C = static_cast<const float*>(g);
+        const uint16_t* R = reinterpret_cast<const uint16_t*>(C + (long)(nq / 5) * n);
+        return (double)(rem_t(q) * C[(long)s * n + i] + rem_dec(R[(long)(5 * s + q) * n + i]));
+    }
+    __device__ static void put5(void* g, int s, long i, long n, int nq, const double d[5]) {
+        float* C = static_cast<float*>(g);
+        uint16_t* R = reinterpret_cast<uint16_t*>(C + (long)(nq / 5) * n);
+        float gf[5];
+        for (int q = 0; q < 5; q++) gf[q] = (float)d[q];
+        const float c = rem_csum(gf);
+        C[(long)s * n + i] = c;
+        for (int q = 0; q < 5; q++) R[(long)(5 * s + q) * n + i] = rem_enc(gf[q] - rem_t(q) * c);
+    }
+};
+
 // GeometryNode.cpp:55-81 (parabolic sine approximation, not cos)
 __device__ __forceinline__ double fast_cos(const double x) {
     double c;

@@ -229,9 +229,6 @@ struct ReinitArgs {
 // P: storage format; the re-initialisation itself is FP64 arithmetic.
 template <int NS, int P>
 __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd) {
-    using CD = Codec<P>;
-    using T = typename CD::T;
-    T* const gb = static_cast<T*>(a.g);
     const int x = (int)(i % a.nx), y = (int)(i / a.nx);
     const int EXd[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
     const int EYd[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -245,11 +242,10 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         return (long)yy * a.nx + xx;
     };
     constexpr int GQ = NS > 0 ? 5 * NS : 5;   // species planes in the storage blocks
-    auto gat = [&](int k, int q, long j) -> long { return pidx<P>(5 * k + q, j, a.n, GQ); };
+    using GS = GStore<P>;
     auto cswap = [&](int k, long j) -> double {
-        return c_swapped((double)CD::dec(gb[gat(k, 0, j)]), (double)CD::dec(gb[gat(k, 1, j)]),
-                         (double)CD::dec(gb[gat(k, 2, j)]), (double)CD::dec(gb[gat(k, 3, j)]),
-                         (double)CD::dec(gb[gat(k, 4, j)]));
+        return c_swapped(GS::get(a.g, k, 0, j, a.n, GQ), GS::get(a.g, k, 1, j, a.n, GQ), GS::get(a.g, k, 2, j, a.n, GQ),
+                         GS::get(a.g, k, 3, j, a.n, GQ), GS::get(a.g, k, 4, j, a.n, GQ));
     };
     double ux_i, uy_i;
     if (P == LBIB_FP64) {
@@ -290,7 +286,7 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         for (int j = 0; j < NS; j++) cj[j] = (j == k) ? C : cswap(j, i);
         const double tI = fr.r < 0 ? a.sp[k].tauI : a.rects[fr.r].tauIs[k];
         species_collide<NS>(d, C, ux_i, uy_i, a.sp[k], tI, nd, iter_of(a.it, a.st), cj, cj);
-        for (int q = 0; q < 5; q++) gb[gat(k, q, i)] = CD::enc(d[q]);
+        GS::put5(a.g, k, i, a.n, GQ, d);
     }
 }
 
@@ -480,6 +476,7 @@ static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
         case LBIB_FP64: launch_reinit_p<NS, LBIB_FP64>(c, a); break;
         case LBIB_FP32: launch_reinit_p<NS, LBIB_FP32>(c, a); break;
         case LBIB_FP16_DEV: launch_reinit_p<NS, LBIB_FP16_DEV>(c, a); break;
+        case LBIB_FP16_REM: launch_reinit_p<NS, LBIB_FP16_REM>(c, a); break;
         default: launch_reinit_p<NS, LBIB_BF16_DEV>(c, a); break;
     }
 }

@@ -285,15 +285,15 @@ inline bool pairs_apply(const lbib_ctx* c) {
 // the vector kernel: mixed storage, v sites of 8 or 16 bytes per lane, rows
 // made of whole lane vectors (rectangle origins are multiples of 16)
 inline bool vec_apply(const lbib_ctx* c, int v) {
-    if (c->prec == LBIB_FP64 || c->nx % v) return false;
+    if (c->prec == LBIB_FP64 || c->prec == LBIB_FP16_REM || c->nx % v) return false;
     if (c->prec == LBIB_FP32) return v == 2 || v == 4;
     return v == 4 || v == 8;
 }
 inline LatticeShape lattice_shape(const lbib_ctx* c) {
     if (c->lat_variant >= 0 && (c->lat_variant == 0 || (c->lat_variant == 1 && pairs_apply(c)) ||
-                                (c->lat_variant == 2 && vec_apply(c, c->lat_spt))))
+                                (c->lat_variant == 2 && (vec_apply(c, c->lat_spt) || c->prec == LBIB_FP16_REM))))
         return {c->lat_variant, c->lat_wg, c->lat_spt};
-    if (c->prec == LBIB_FP64) return {0, 256, 1};
+    if (c->prec == LBIB_FP64 || c->prec == LBIB_FP16_REM) return {0, 256, 1};
     return pairs_apply(c) ? LatticeShape{1, 256, 1} : LatticeShape{0, 256, 2};
 }
 struct TunedShape {

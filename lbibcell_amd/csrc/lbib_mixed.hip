@@ -18,149 +18,9 @@
 //
 // Roofline: HBM bound.  Algorithmic bytes per site = (2*9 + 2*5*ns) * b with
 // b = 4 (FP32) or 2 (FP16/BF16).
-#include "lbib_device_math.h"
+#include "lbib_mixed.h"
 
 namespace lbib {
-
-// lattice weights and velocities in natural order T,E,N,W,S,NE,NW,SW,SE
-__device__ __forceinline__ float wq(int q) { return q == 0 ? 4.0f / 9.0f : (q < 5 ? 1.0f / 9.0f : 1.0f / 36.0f); }
-__device__ __forceinline__ int exq(int q) {
-    const int e[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-    return e[q];
-}
-__device__ __forceinline__ int eyq(int q) {
-    const int e[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
-    return e[q];
-}
-
-struct SpeciesF {
-    int kind;
-    float tauI;
-    float production, decay;
-    uint32_t switchoff;
-    int prod_domain;
-    float source_inside;
-    float krow[LBIB_MAX_SPECIES];
-};
-
-struct MixedArgs {
-    int nx, ny;
-    long n;
-    const void* fs;
-    void* fd;
-    const void* gs;
-    void* gd;
-    float* u;            // velocity pairs
-    const float2* F;     // force pairs (FP32)
-    const uint8_t* ftile;
-    const uint32_t* tag;
-    const double* mtab;
-    int ntab;
-    double* msite;
-    const double* bc_v;
-    const int32_t* bc_desc;
-    const int32_t* bcd_off;
-    const int32_t* bcd_sp;
-    int ntx;
-    float tauI, tau;
-    uint32_t it;
-    int box_outlet;
-    const DevStatus* st;
-    const int32_t* rtile;   // batched atlas: rectangle per 16x16 tile (null: one domain)
-    const RectDev* rects;
-    SpeciesF sp[LBIB_MAX_SPECIES];
-};
-
-// Deviation-form BGK with Luo (FM 0) or Guo (FM 1 as written, FM 2 textbook)
-// forcing; d[] in, d[] out, post-collision velocity out.
-template <int FM>
-__device__ __forceinline__ void collide_dev_f32(float d[9], float m, float Fx, float Fy, float tauI, float tau,
-                                                float gx, float gy, float& uxo, float& uyo) {
-    float sd = 0.0f;
-#pragma unroll
-    for (int q = 0; q < 9; q++) sd += d[q];
-    const float rho0 = 1.0f + sd;
-    float rho = rho0;
-    if (isfinite(m)) rho += m;
-    const float rhoI = 1.0f / rho;
-    float jx = d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW]);
-    float jy = d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE]);
-    if (FM != 0) { jx += 0.5f * Fx; jy += 0.5f * Fy; }
-    const float ux = jx * rhoI, uy = jy * rhoI;
-    const float u2 = 1.5f * (ux * ux + uy * uy);
-    const float G = (FM != 0) ? (1.0f - 0.5f / tau) : 0.0f;
-#pragma unroll
-    for (int q = 0; q < 9; q++) {
-        const float w = wq(q);
-        const float ex = (float)exq(q), ey = (float)eyq(q);
-        const float eu = ex * ux + ey * uy;
-        const float deq = w * (rho - 1.0f) + w * rho * (3.0f * eu + 4.5f * eu * eu - u2);
-        float fc;
-        if (FM == 0) {
-            fc = 3.0f * w * rho0 * (ex * Fx + ey * Fy);
-        } else {
-            // w (1 - 1/(2 tau)) [3 (e - u) + 9 (e.u) e] . F with the stored velocity
-            const float eg = ex * gx + ey * gy;
-            float cx = 3.0f * (ex - gx) + 9.0f * eg * ex;
-            float cy = 3.0f * (ey - gy) + 9.0f * eg * ey;
-            if (FM == 1 && q == DNE) cy = 9.0f * gx - 3.0f * gy + 3.0f;   // GuoZhengShi2002ForceModel.cpp NE y-term
-            fc = G * w * (cx * Fx + cy * Fy);
-        }
-        d[q] = d[q] - (d[q] - deq) * tauI + fc;
-    }
-    float px = d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW]);
-    float py = d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE]);
-    if (FM != 0) { px += 0.5f * Fx; py += 0.5f * Fy; }
-    uxo = px * rhoI;
-    uyo = py * rhoI;
-}
-
-template <int NS>
-__device__ __forceinline__ float reaction_f32(const SpeciesF& p, uint32_t dom, uint32_t it, float C, const float* cj,
-                                              const float* cseq) {
-    if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R || p.kind == LBIB_SPECIES_SCHNAKENBERG_L) {
-        if (dom == 0u) return 0.0f;
-        const float Cp = cseq[p.prod_domain];
-        if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R) return p.production * (p.source_inside - C + C * C * Cp);
-        return p.production * (p.source_inside - Cp * Cp * C);
-    }
-    if (p.kind == LBIB_SPECIES_SIGNAL) {
-        const bool prod = (p.prod_domain < 0) ? (dom != 0u) : (dom == (uint32_t)p.prod_domain);
-        if (prod && it < p.switchoff) return p.production;
-        return -p.decay * C;
-    }
-    if (p.kind == LBIB_SPECIES_LINEAR) {
-        float r = 0.0f;
-#pragma unroll
-        for (int j = 0; j < NS; j++) r += p.krow[j] * cj[j];
-        if (dom != 0u) r += p.source_inside;
-        return r;
-    }
-    return 0.0f;
-}
-
-template <int NS>
-__device__ __forceinline__ void species_collide_f32(float g[5], float C, float ux, float uy, const SpeciesF& p,
-                                                    float tauI, uint32_t dom, uint32_t it, const float* cj,
-                                                    const float* cseq) {
-    const float w0 = C / 3.0f, w1 = C / 6.0f;
-    float eq[5];
-    eq[DT] = w0;
-    eq[DE] = w1 * (1.0f + 3.0f * ux);
-    eq[DN] = w1 * (1.0f + 3.0f * uy);
-    eq[DW] = w1 * (1.0f - 3.0f * ux);
-    eq[DS] = w1 * (1.0f - 3.0f * uy);
-    float R0 = 0.0f, R1 = 0.0f;
-    if (p.kind != LBIB_SPECIES_PLAIN) {
-        float R = reaction_f32<NS>(p, dom, it, C, cj, cseq);
-        if (p.kind == LBIB_SPECIES_SCHNAKENBERG_R || p.kind == LBIB_SPECIES_SCHNAKENBERG_L) R *= p.decay;
-        R0 = R / 3.0f;
-        R1 = R / 6.0f;
-    }
-    g[DT] = g[DT] - (g[DT] - eq[DT]) * tauI + R0;
-#pragma unroll
-    for (int q = 1; q < 5; q++) g[q] = g[q] - (g[q] - eq[q]) * tauI + R1;
-}
 
 // SPT sites per thread, one workgroup width (blockDim.x) apart so that every
 // load stays coalesced; all loads of a thread are issued before its
@@ -909,43 +769,6 @@ __global__ void k_velocity_to_f32(const double* __restrict__ src, float* dst, lo
         dst[k] = (float)src[k];
 }
 
-static MixedArgs make_mixed_args(lbib_ctx* c, uint32_t it, bool use_ftile) {
-    MixedArgs a;
-    a.nx = c->nx; a.ny = c->ny; a.n = c->n;
-    a.fs = c->f[c->cur];
-    a.fd = c->f[1 - c->cur];
-    a.gs = c->g[c->cur];
-    a.gd = c->g[1 - c->cur];
-    a.u = reinterpret_cast<float*>(c->u);
-    a.F = reinterpret_cast<const float2*>(c->Fx);
-    a.ftile = use_ftile ? c->tact : nullptr;
-    a.tag = c->tag;
-    a.mtab = c->mtab_live ? c->mtab : nullptr;
-    a.ntab = c->ndom;
-    a.msite = c->msite_live ? c->msite : nullptr;
-    a.bc_v = c->bc_v; a.bc_desc = c->bc_desc; a.bcd_off = c->bcd_off; a.bcd_sp = c->bcd_sp;
-    a.ntx = c->stx;
-    a.tauI = (float)(1.0 / c->cfg.tau_fluid);
-    a.tau = (float)c->cfg.tau_fluid;
-    a.it = it;
-    a.box_outlet = c->cfg.box_outlet;
-    a.st = c->st;
-    a.rtile = c->nrect ? c->rtile : nullptr;
-    a.rects = c->rects;
-    for (int s = 0; s < LBIB_MAX_SPECIES; s++) {
-        const SpeciesDev& p = c->sp[s];
-        SpeciesF& q = a.sp[s];
-        q.kind = p.kind;
-        q.tauI = (float)p.tauI;
-        q.production = (float)p.production;
-        q.decay = (float)p.decay;
-        q.switchoff = p.switchoff;
-        q.prod_domain = p.prod_domain;
-        q.source_inside = (float)p.source_inside;
-        for (int j = 0; j < LBIB_MAX_SPECIES; j++) q.krow[j] = (float)p.krow[j];
-    }
-    return a;
-}
 
 template <int NS, int FM, int P>
 static void launch_mx_spt(lbib_ctx* c, const MixedArgs& a, int wg, int spt) {
@@ -1030,6 +853,7 @@ static void dispatch_mx_fm(lbib_ctx* c, const MixedArgs& a, const LatticeShape& 
 }
 
 static int launch_mixed_shape(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
+    if (c->prec == LBIB_FP16_REM) return launch_rem_shape(c, a, sh.variant, sh.wg);
     if (sh.variant == 2 && ensure_segments(c, sh.wg * sh.spt)) return -1;
     if (c->prec == LBIB_FP32) dispatch_mx_fm<LBIB_FP32>(c, a, sh);
     else if (c->prec == LBIB_FP16_DEV) dispatch_mx_fm<LBIB_FP16_DEV>(c, a, sh);
@@ -1050,6 +874,7 @@ int launch_stream_collide_mixed(lbib_ctx* c, uint32_t it, bool use_ftile) {
 static int grid_of(long count) { return (int)std::min<long>((count + 255) / 256, 8192); }
 
 int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq) {
+    if (c->prec == LBIB_FP16_REM) return launch_rem_pack(c, src_dev, field);
     void* dst = field == 0 ? c->f[c->cur] : c->g[c->cur];
     const int q0 = field == 0 ? 0 : 5 * (field - 1);
     const int nqt = field == 0 ? 9 : 5 * std::max(c->ns, 1);
@@ -1065,6 +890,7 @@ int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq) {
 }
 
 int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq) {
+    if (c->prec == LBIB_FP16_REM) return launch_rem_unpack(c, dst_dev, field);
     const void* src = field == 0 ? c->f[c->cur] : c->g[c->cur];
     const int q0 = field == 0 ? 0 : 5 * (field - 1);
     const int nqt = field == 0 ? 9 : 5 * std::max(c->ns, 1);
@@ -1117,12 +943,15 @@ int autotune_mixed(lbib_ctx* c, int reps, std::vector<TunedShape>& out) {
     a.st = sc.st;
     std::vector<LatticeShape> cand;
     for (int wg : {64, 128, 256})
-        for (int spt : {1, 2, 4}) cand.push_back({0, wg, spt});
+        for (int spt : {1, 2, 4})
+            if (spt == 1 || c->prec != LBIB_FP16_REM) cand.push_back({0, wg, spt});
     if (pairs_apply(c))
         for (int wg : {64, 128, 256}) cand.push_back({1, wg, 1});
     for (int v : {2, 4, 8})
         if (vec_apply(c, v))
             for (int wg : {64, 128, 256}) cand.push_back({2, wg, v});
+    if (c->prec == LBIB_FP16_REM)
+        for (int wg : {64, 128, 256}) cand.push_back({2, wg, 1});
     for (const LatticeShape& sh : cand) {
         double ms = 0.0;
         if (sc.time(c, reps, [&] { return launch_mixed_shape(c, a, sh); }, ms)) return -1;

@@ -726,8 +726,7 @@ __global__ void k_mass_growth(long n, const void* g, int sR, int sL, int nq, con
                               double threshold, double maxsource, int keep_site, const double* __restrict__ mtab,
                               int ntab, double* msite, const DevStatus* st) {
     if (step_stopped(st)) return;
-    using CD = Codec<P>;
-    const typename CD::T* gs = static_cast<const typename CD::T*>(g);
+    using GS = GStore<P>;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         const uint32_t d = tag[i] & DOM_MASK;
         if (d == 0) {
@@ -735,11 +734,8 @@ __global__ void k_mass_growth(long n, const void* g, int sR, int sL, int nq, con
             continue;
         }
         auto conc = [&](int s) {
-            const int q0 = 5 * s;
-            return c_swapped((double)CD::dec(gs[pidx<P>(q0, i, n, nq)]), (double)CD::dec(gs[pidx<P>(q0 + 1, i, n, nq)]),
-                             (double)CD::dec(gs[pidx<P>(q0 + 2, i, n, nq)]),
-                             (double)CD::dec(gs[pidx<P>(q0 + 3, i, n, nq)]),
-                             (double)CD::dec(gs[pidx<P>(q0 + 4, i, n, nq)]));
+            return c_swapped(GS::get(g, s, 0, i, n, nq), GS::get(g, s, 1, i, n, nq), GS::get(g, s, 2, i, n, nq),
+                             GS::get(g, s, 3, i, n, nq), GS::get(g, s, 4, i, n, nq));
         };
         const double R = conc(sR), L = conc(sL);
         msite[i] = (R * R * L > threshold) ? maxsource : maxsource / 10.0;
@@ -872,6 +868,10 @@ int launch_mass_growth(lbib_ctx* c, int sR, int sL, double threshold, double max
             k_mass_growth<LBIB_FP16_DEV><<<gs, 256, 0, c->stream>>>(c->n, g, sR, sL, nq, c->tag, threshold, maxsource,
                                                                     keep, mt, c->ndom, c->msite, c->st);
             break;
+        case LBIB_FP16_REM:
+            k_mass_growth<LBIB_FP16_REM><<<gs, 256, 0, c->stream>>>(c->n, g, sR, sL, nq, c->tag, threshold, maxsource,
+                                                                    keep, mt, c->ndom, c->msite, c->st);
+            break;
         default:
             k_mass_growth<LBIB_BF16_DEV><<<gs, 256, 0, c->stream>>>(c->n, g, sR, sL, nq, c->tag, threshold, maxsource,
                                                                     keep, mt, c->ndom, c->msite, c->st);
@@ -891,9 +891,7 @@ int launch_mass_growth(lbib_ctx* c, int sR, int sL, double threshold, double max
 template <int P>
 __global__ __launch_bounds__(256) void k_domain_conc(int nx, long n, const void* g, int species, int nq,
                                                     const uint32_t* __restrict__ tag, int ndom, double* acc) {
-    using CD = Codec<P>;
-    const typename CD::T* gs = static_cast<const typename CD::T*>(g);
-    const int q0 = 5 * species;
+    using GS = GStore<P>;
     __shared__ double out0;
     if (threadIdx.x == 0) out0 = 0.0;
     __syncthreads();
@@ -906,9 +904,9 @@ __global__ __launch_bounds__(256) void k_domain_conc(int nx, long n, const void*
         if (x < nx) {
             const long i = y * nx + x;
             d = tag[i] & DOM_MASK;
-            s = c_swapped((double)CD::dec(gs[pidx<P>(q0, i, n, nq)]), (double)CD::dec(gs[pidx<P>(q0 + 1, i, n, nq)]),
-                          (double)CD::dec(gs[pidx<P>(q0 + 2, i, n, nq)]), (double)CD::dec(gs[pidx<P>(q0 + 3, i, n, nq)]),
-                          (double)CD::dec(gs[pidx<P>(q0 + 4, i, n, nq)]));
+            s = c_swapped(GS::get(g, species, 0, i, n, nq), GS::get(g, species, 1, i, n, nq),
+                          GS::get(g, species, 2, i, n, nq), GS::get(g, species, 3, i, n, nq),
+                          GS::get(g, species, 4, i, n, nq));
         }
         const uint32_t dprev = (uint32_t)__shfl_up((int)d, 1, 64);
         int head = (lane == 0 || dprev != d) ? 1 : 0;
@@ -939,6 +937,9 @@ int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc
         case LBIB_FP32: k_domain_conc<LBIB_FP32><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc); break;
         case LBIB_FP16_DEV:
             k_domain_conc<LBIB_FP16_DEV><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc);
+            break;
+        case LBIB_FP16_REM:
+            k_domain_conc<LBIB_FP16_REM><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc);
             break;
         default: k_domain_conc<LBIB_BF16_DEV><<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, g, species, nq, c->tag, ndom, acc); break;
     }

@@ -19,7 +19,8 @@ from lbibcell_amd import capi, runner, workloads
 
 pytestmark = pytest.mark.gpu
 
-PRECS = {"fp64": capi.FP64, "fp32": capi.FP32, "fp16": capi.FP16_DEV, "bf16": capi.BF16_DEV}
+PRECS = {"fp64": capi.FP64, "fp32": capi.FP32, "fp16": capi.FP16_DEV, "bf16": capi.BF16_DEV,
+         "fp16r": capi.FP16_REM}
 W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
 SIG = dict(production=1e-3, decay=1e-5, switchoff=7, init_inside=0.5, prod_domain=-1)
 
@@ -32,6 +33,8 @@ RECTS = [(0, 0, 37, 30, 0.83, [0.7]), (48, 0, 64, 44, 1.27, [1.05]), (0, 48, 26,
 def _shapes(prec, even):
     if prec == "fp64":
         return [(0, wg, 1) for wg in (64, 128, 256)]
+    if prec == "fp16r":   # one load per pulled moment, or moments shared by shuffles
+        return [(v, wg, 1) for v in (0, 2) for wg in (64, 128, 256)]
     s = [(0, wg, spt) for wg in (64, 128, 256) for spt in (1, 2, 4)]
     if prec in ("fp16", "bf16") and even:
         s += [(1, wg, 1) for wg in (64, 128, 256)]

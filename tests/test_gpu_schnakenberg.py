@@ -28,10 +28,10 @@ def _oracle_pair(orc, o, tau=0.9):
     o.add_species(orc.SP_SCHNAKENBERG_R, tau, [100.0, 0.1, 1e-4, 0, 1.0])
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "fp16", "fp16r"])
 def test_schnakenberg_lattice_steps_match_oracle(orc, prec):
     nx, ny, tau = 52, 38, 0.85
-    precision = {"fp64": capi.FP64, "fp32": capi.FP32, "fp16": capi.FP16_DEV}[prec]
+    precision = {"fp64": capi.FP64, "fp32": capi.FP32, "fp16": capi.FP16_DEV, "fp16r": capi.FP16_REM}[prec]
     print(f"case schnakenberg lattice {prec} {nx}x{ny}", flush=True)
     ctx = capi.Context(nx, ny, tau, species=_specs(), precision=precision)
     o = orc.Oracle(nx, ny, tau)
@@ -67,7 +67,8 @@ def test_schnakenberg_lattice_steps_match_oracle(orc, prec):
             np.testing.assert_array_equal(gd, go)
         else:
             # storage roundoff of values near 0.2 per step, 25 steps
-            assert err <= {"fp32": 1e-5, "fp16": 5e-3}[prec]
+            # fp16r: remainders of the 5 % noise (about 0.01) rounded to 11 bits
+            assert err <= {"fp32": 1e-5, "fp16": 5e-3, "fp16r": 2e-4}[prec]
     ctx.close()
 
 

@@ -1,6 +1,8 @@
 """Vector lattice kernel (launch shape variant 2: V consecutive sites per
 lane, 8- or 16-byte plane accesses, shifted pulls through wavefront
-shuffles) against the one-site-per-lane kernel (variant 0, spt 1).
+shuffles) against the one-site-per-lane kernel (variant 0, spt 1); for the
+moments-and-remainders storage (fp16r), variant 2 is the kernel that shares
+each loaded moment with the neighbouring lanes by shuffles.
 
 Both kernels evaluate the same FP32 expressions in the same order on the
 same decoded values, so every result must be bit-identical: populations,
@@ -19,11 +21,13 @@ from lbibcell_amd import capi, runner, workloads
 pytestmark = pytest.mark.gpu
 
 W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
-PRECS = {"fp32": capi.FP32, "fp16": capi.FP16_DEV, "bf16": capi.BF16_DEV}
+PRECS = {"fp32": capi.FP32, "fp16": capi.FP16_DEV, "bf16": capi.BF16_DEV, "fp16r": capi.FP16_REM}
 SIG = dict(production=1e-3, decay=1e-5, switchoff=7, init_inside=0.5, prod_domain=1)
 
 
 def vec_shapes(prec, nx):
+    if prec == "fp16r":
+        return [(2, wg, 1) for wg in (64, 128, 256)]
     vs = (2, 4) if prec == "fp32" else (4, 8)
     return [(2, wg, v) for v in vs if nx % v == 0 for wg in (64, 128, 256)]
 
@@ -103,7 +107,7 @@ def _noflux_run(prec, shape):
     return out
 
 
-@pytest.mark.parametrize("prec", ["fp32", "fp16"])
+@pytest.mark.parametrize("prec", ["fp32", "fp16", "fp16r"])
 def test_vector_kernel_noflux_coupled_run(prec):
     ref = _noflux_run(prec, (0, 64, 1))
     for shape in vec_shapes(prec, 64)[::3]:
