@@ -376,6 +376,27 @@ int lbib_adj_forward(lbib_adj* adj, int steps, double tau_c, double kappa, const
 int lbib_adj_gradient(lbib_adj* adj, double* d_tau_c, double* d_kappa, double* d_rates);
 /* Device time of the last forward and of the last gradient (recompute + reverse). */
 int lbib_adj_timing(lbib_adj* adj, double* forward_ms, double* gradient_ms);
+/* Multi-species model (after lbib_adj_set_problem; replaces its species):
+ * nspecies (1-4) species with initial populations g0 (nspecies*5*n), targets
+ * (nspecies*n; the loss sums over species) and decay[s] outside every cell.
+ * The reaction is R_s = sum_j K_sj C_j (all concentrations before the
+ * collides, as the linear-matrix species of the multi-species runs), plus
+ * rates[c] inside cell c for species 0, plus -decay_s C_s outside cells. */
+int lbib_adj_set_species(lbib_adj* adj, int nspecies, const double* g0, const double* target, const double* decay);
+/* Junction springs between marker pairs (pairs: 2*npairs marker indices, each
+ * pair once): F_a += k_j (|d| - l0) d/|d| with d = X_b - X_a, and the
+ * opposite force on b (ForceType7, ForceStructs.hpp:521-588, as the
+ * cell-junction BioSolver creates them); the pair list is fixed over the run. */
+int lbib_adj_set_junctions(lbib_adj* adj, int npairs, const int32_t* pairs, double l0);
+/* Forward with every parameter: tau_c[nspecies], kappa, the junction
+ * stiffness k_j, K (nspecies*nspecies row-major, NULL = 0), rates (n_cells + 1,
+ * [0] unused); c_final (nspecies*n) optional. */
+int lbib_adj_forward_ms(lbib_adj* adj, int steps, const double* tau_c, double kappa, double k_junction,
+                        const double* K, const double* rates, double* loss, double* c_final);
+/* Gradient of the last forward's loss with respect to every parameter of
+ * lbib_adj_forward_ms (any output may be NULL). */
+int lbib_adj_gradient_ms(lbib_adj* adj, double* d_tau_c, double* d_kappa, double* d_k_junction, double* d_K,
+                         double* d_rates);
 
 /* ---- measurement ------------------------------------------------------------
  * No counterpart in the reference (it has no profiling): HIP events recorded

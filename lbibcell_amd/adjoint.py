@@ -2,7 +2,10 @@
 lbib_adj_*): the loss L = sum_x (C_T(x) - C*(x))^2 of the final species
 concentration against a target, differentiated with respect to the species
 relaxation time tau_c, the membrane stiffness kappa (type-6 tension,
-ForceStructs.hpp:455-514) and one source rate per cell.
+ForceStructs.hpp:455-514) and one source rate per cell; with several species
+(set_species), also the reaction matrix K, and with junction springs between
+cells (set_junctions, ForceType7 of ForceStructs.hpp:521-588), their
+stiffness.
 
 `AdjointProblem` owns one device context; `CoupledLoss` is a thin
 torch.autograd.Function over it, there to check the hand-written adjoint
@@ -35,6 +38,10 @@ SIGNATURES = {
     "lbib_adj_forward": (C.c_int, [_P, C.c_int, C.c_double, C.c_double, _D, _D, _D]),
     "lbib_adj_gradient": (C.c_int, [_P, _D, _D, _D]),
     "lbib_adj_timing": (C.c_int, [_P, _D, _D]),
+    "lbib_adj_set_species": (C.c_int, [_P, C.c_int, _D, _D, _D]),
+    "lbib_adj_set_junctions": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32), C.c_double]),
+    "lbib_adj_forward_ms": (C.c_int, [_P, C.c_int, _D, C.c_double, C.c_double, _D, _D, _D, _D]),
+    "lbib_adj_gradient_ms": (C.c_int, [_P, _D, _D, _D, _D, _D]),
 }
 
 W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
@@ -115,6 +122,53 @@ class AdjointProblem:
         dr = np.zeros(self.ncell + 1)
         self._chk(self.L.lbib_adj_gradient(self.h, C.byref(dt), C.byref(dk), _dp(dr)))
         return dt.value, dk.value, dr[1:].copy()
+
+    def set_species(self, g0, target, decay):
+        """Several species: g0 (ns, 5, ny, nx), target (ns, ny, nx), decay (ns)
+        outside every cell; replaces the species given at construction."""
+        g0 = np.ascontiguousarray(g0, dtype=np.float64)
+        ns = g0.shape[0]
+        self._gms = g0.reshape(-1)
+        self._tms = np.ascontiguousarray(target, dtype=np.float64).reshape(-1)
+        self._dec = np.ascontiguousarray(decay, dtype=np.float64).reshape(-1)
+        if self._tms.size != ns * self.nx * self.ny or self._dec.size != ns:
+            raise capi.LbibError("set_species: target and decay must cover every species")
+        self._chk(self.L.lbib_adj_set_species(self.h, ns, _dp(self._gms), _dp(self._tms), _dp(self._dec)))
+        self.ns = ns
+
+    def set_junctions(self, pairs, l0=0.5):
+        """Junction springs between marker pairs (P, 2) (global marker indices,
+        rings concatenated in order), rest length l0."""
+        self._pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        self._chk(self.L.lbib_adj_set_junctions(self.h, self._pairs.size // 2,
+                                                self._pairs.ctypes.data_as(C.POINTER(C.c_int32)), float(l0)))
+
+    def forward_ms(self, steps, tau_c, kappa, k_junction=0.0, K=None, rates=None, want_field=False):
+        """Loss with every parameter: tau_c (ns), kappa, k_junction, K (ns, ns),
+        rates (ncell); the final concentrations (ns, ny, nx) when want_field."""
+        ns = getattr(self, "ns", 1)
+        tc = np.ascontiguousarray(np.broadcast_to(np.asarray(tau_c, dtype=np.float64), (ns,)))
+        Km = None if K is None else np.ascontiguousarray(K, dtype=np.float64).reshape(-1)
+        r = np.zeros(self.ncell + 1)
+        if rates is not None:
+            r[1:] = np.asarray(rates, dtype=np.float64).reshape(-1)[: self.ncell]
+        loss = C.c_double()
+        field = np.empty(ns * self.nx * self.ny) if want_field else None
+        self._chk(self.L.lbib_adj_forward_ms(self.h, int(steps), _dp(tc), float(kappa), float(k_junction),
+                                             _dp(Km) if Km is not None else None, _dp(r), C.byref(loss),
+                                             _dp(field) if want_field else None))
+        return (loss.value, field.reshape(ns, self.ny, self.nx)) if want_field else loss.value
+
+    def gradient_ms(self):
+        """dict(tau_c (ns), kappa, k_junction, K (ns, ns), rates (ncell)) of the
+        last forward."""
+        ns = getattr(self, "ns", 1)
+        dt = np.zeros(ns)
+        dk, dj = C.c_double(), C.c_double()
+        dK = np.zeros(ns * ns)
+        dr = np.zeros(self.ncell + 1)
+        self._chk(self.L.lbib_adj_gradient_ms(self.h, _dp(dt), C.byref(dk), C.byref(dj), _dp(dK), _dp(dr)))
+        return dict(tau_c=dt, kappa=dk.value, k_junction=dj.value, K=dK.reshape(ns, ns), rates=dr[1:].copy())
 
     def timing(self):
         """(device ms of the last forward, of the last gradient)"""

@@ -9,15 +9,21 @@
 //   spread          F(x) = sum_k delta(x - X_k) F_k            (GeometryNode.cpp:228-235)
 //   fluid           pull streaming + BGK with Luo forcing     (FluidSolver.cpp:103-183,
 //                   Luo1993ForceModel.cpp:29-70), no mass source
-//   species         pull streaming + D2Q5 BGK on the fresh velocity with the
-//                   reaction R = s[c] inside cell c, -decay * C outside
-//                   (tutorial_01_CDESolverD2Q5_SIGNAL.cpp:92-135 with one rate per cell)
+//   junctions       F_k += k_j (|d| - l0) d / |d|, d = X_m - X_k, over a fixed list
+//                   of marker pairs of different cells (ForceType7,
+//                   ForceStructs.hpp:521-588, as the cell-junction BioSolver sets it)
+//   species         1-4 species; pull streaming + D2Q5 BGK on the fresh velocity
+//                   with the reaction R_s = sum_j K_sj C_j, plus s[c] for species 0
+//                   inside cell c, and -decay_s * C_s outside every cell
+//                   (tutorial_01_CDESolverD2Q5_SIGNAL.cpp:92-135 with one rate per
+//                   cell; the linear reaction matrix of the multi-species runs)
 //   interpolation   U_k = sum_x delta(x - X_k) u(x), X_k += U_k  (GeometryNode.cpp:216-254)
 // with the fastCos delta of GeometryNode.cpp:55-94.  The cell map (which
 // cell's rate applies at a site) is held at its initial value, and cut links,
 // re-initialisation and remeshing are not part of the model (they are
 // discrete).  The loss is L = sum_x (C_T(x) - C*(x))^2 on the final
-// concentration; the parameters are tau_c, kappa and the per-cell rates s[c].
+// concentrations of all species; the parameters are tau_c per species,
+// kappa, k_j, the reaction matrix K and the per-cell rates s[c].
 //
 // Forward states are checkpointed every K steps; the reverse sweep restores a
 // checkpoint, recomputes its segment storing every intermediate state, and
@@ -96,9 +102,17 @@ __device__ __forceinline__ long wrap_site(int x, int y, int nx, int ny) {
 }
 
 // ---------------------------------------------------------------- forward --
+// Junction springs of marker k: partners jpart[jofs[k] .. jofs[k + 1]) (a pair
+// is listed at both ends, so every force is gathered, never scattered).
+struct Junctions {
+    const int32_t* ofs;
+    const int32_t* part;
+    double kj, l0;
+};
+
 template <typename T>
 __global__ void k_force(int nm, const T* __restrict__ X, const int32_t* __restrict__ nxt,
-                        const int32_t* __restrict__ prv, T kappa, T* __restrict__ Fm) {
+                        const int32_t* __restrict__ prv, T kappa, T* __restrict__ Fm, const Junctions jn) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nm) return;
     const T x1 = X[2 * k], y1 = X[2 * k + 1];
@@ -110,6 +124,17 @@ __global__ void k_force(int nm, const T* __restrict__ X, const int32_t* __restri
         const T dist = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
         fx += kappa * (x2 - x1) / dist;
         fy += kappa * (y2 - y1) / dist;
+    }
+    if (jn.ofs) {
+        const T kj = (T)jn.kj, l0 = (T)jn.l0;
+        for (int p = jn.ofs[k]; p < jn.ofs[k + 1]; p++) {
+            const int m = jn.part[p];
+            const T dx = X[2 * m] - x1, dy = X[2 * m + 1] - y1;
+            const T dist = sqrt(dx * dx + dy * dy);
+            const T c = kj * (dist - l0) / dist;
+            fx += c * dx;
+            fy += c * dy;
+        }
     }
     Fm[2 * k] = fx;
     Fm[2 * k + 1] = fy;
@@ -147,7 +172,10 @@ struct LatArgs {
     T* u;
     const uint32_t* dom;
     const double* src;
-    T tauI, tauIc, decay;
+    T tauI;
+    T tauIc[LBIB_MAX_SPECIES], decay[LBIB_MAX_SPECIES];
+    T Km[LBIB_MAX_SPECIES * LBIB_MAX_SPECIES];   // reaction matrix, row-major (stride LBIB_MAX_SPECIES)
+    int hasK;                                     // any nonzero entry
 };
 
 // pull indices of the 9 directions (natural order) at site (x, y)
@@ -167,10 +195,12 @@ __device__ __forceinline__ void pull_index(int x, int y, int nx, int ny, long id
 }
 
 // FluidSolver::collide with the Luo force (same operand order as the FP64
-// kernel, m = 0) and the D2Q5 collide with the per-cell reaction.
-template <typename T>
-__device__ __forceinline__ void collide_site(T d[9], T g[5], const T Fx, const T Fy, const T tauI, const T tauIc,
-                                             const uint32_t dm, const double* src, const T decay, T& uxo, T& uyo) {
+// kernel, m = 0) and the D2Q5 collides with the per-cell reaction and the
+// reaction matrix (concentrations of every species before the collides).
+template <typename T, int NS>
+__device__ __forceinline__ void collide_site(T d[9], T g[NS][5], const T Fx, const T Fy, const LatArgs<T>& a,
+                                             const uint32_t dm, T& uxo, T& uyo) {
+    const T tauI = a.tauI;
     using Q = Cst<T>;
     T rho0 = 0;
 #pragma unroll
@@ -211,24 +241,37 @@ __device__ __forceinline__ void collide_site(T d[9], T g[5], const T Fx, const T
     }
     uxo = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW])) * rhoI;
     uyo = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE])) * rhoI;
-    T C = 0;
+    T Cs[NS];
 #pragma unroll
-    for (int q = 0; q < 5; q++) C += g[q];
-    const T w0 = C / (T)3, w1 = C / (T)6;
-    T te[5];
-    te[DT] = w0;
-    te[DE] = w1 * ((T)1 + uxo * (T)3);
-    te[DN] = w1 * ((T)1 + uyo * (T)3);
-    te[DW] = w1 * ((T)1 + (-uxo) * (T)3);
-    te[DS] = w1 * ((T)1 + (-uyo) * (T)3);
-    const T R = dm != 0u ? (T)src[dm] : -decay * C;
-    const T R0 = (T)1 * R / (T)3, R1 = (T)1 * R / (T)6;
-    g[DT] = g[DT] - g[DT] * tauIc + te[DT] * tauIc + R0;
+    for (int s = 0; s < NS; s++) {
+        T C = 0;
 #pragma unroll
-    for (int q = 1; q < 5; q++) g[q] = g[q] - g[q] * tauIc + te[q] * tauIc + R1;
+        for (int q = 0; q < 5; q++) C += g[s][q];
+        Cs[s] = C;
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const T C = Cs[s], tauIc = a.tauIc[s];
+        const T w0 = C / (T)3, w1 = C / (T)6;
+        T te[5];
+        te[DT] = w0;
+        te[DE] = w1 * ((T)1 + uxo * (T)3);
+        te[DN] = w1 * ((T)1 + uyo * (T)3);
+        te[DW] = w1 * ((T)1 + (-uxo) * (T)3);
+        te[DS] = w1 * ((T)1 + (-uyo) * (T)3);
+        T R = dm != 0u ? (s == 0 ? (T)a.src[dm] : (T)0) : -a.decay[s] * C;
+        if (a.hasK) {
+#pragma unroll
+            for (int j = 0; j < NS; j++) R += a.Km[s * LBIB_MAX_SPECIES + j] * Cs[j];
+        }
+        const T R0 = (T)1 * R / (T)3, R1 = (T)1 * R / (T)6;
+        g[s][DT] = g[s][DT] - g[s][DT] * tauIc + te[DT] * tauIc + R0;
+#pragma unroll
+        for (int q = 1; q < 5; q++) g[s][q] = g[s][q] - g[s][q] * tauIc + te[q] * tauIc + R1;
+    }
 }
 
-template <typename T>
+template <typename T, int NS>
 __global__ void k_lattice(const LatArgs<T> a) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -236,17 +279,21 @@ __global__ void k_lattice(const LatArgs<T> a) {
     long idx[9];
     pull_index(x, y, a.nx, a.ny, idx);
     const long n = a.n;
-    T d[9], g[5];
+    T d[9], g[NS][5];
 #pragma unroll
     for (int q = 0; q < 9; q++) d[q] = a.fs[q * n + idx[q]];
 #pragma unroll
-    for (int q = 0; q < 5; q++) g[q] = a.gs[q * n + idx[q]];
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) g[s][q] = a.gs[(5 * s + q) * n + idx[q]];
     T uxo, uyo;
-    collide_site<T>(d, g, a.Fs[2 * i], a.Fs[2 * i + 1], a.tauI, a.tauIc, a.dom[i], a.src, a.decay, uxo, uyo);
+    collide_site<T, NS>(d, g, a.Fs[2 * i], a.Fs[2 * i + 1], a, a.dom[i], uxo, uyo);
 #pragma unroll
     for (int q = 0; q < 9; q++) a.fd[q * n + i] = d[q];
 #pragma unroll
-    for (int q = 0; q < 5; q++) a.gd[q * n + i] = g[q];
+    for (int s = 0; s < NS; s++)
+#pragma unroll
+        for (int q = 0; q < 5; q++) a.gd[(5 * s + q) * n + i] = g[s][q];
     a.u[2 * i] = uxo;
     a.u[2 * i + 1] = uyo;
 }
@@ -273,21 +320,23 @@ __global__ void k_interp_move(int nm, T* X, const T* __restrict__ u, int nx, int
     X[2 * k + 1] += vy;
 }
 
-// loss L = sum (C - C*)^2 and its adjoint on the stored species populations
+// loss L = sum over species and sites of (C - C*)^2 and its adjoint on the
+// stored species populations; cfinal (optional) receives every C
 template <typename T>
-__global__ void k_loss(const T* __restrict__ g, const double* __restrict__ target, long n, T* lg, double* loss,
-                       T* cfinal) {
+__global__ void k_loss(const T* __restrict__ g, const double* __restrict__ target, long n, int ns, T* lg,
+                       double* loss, T* cfinal) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double r2 = 0.0;
-    if (i < n) {
+    for (int s = 0; s < ns && i < n; s++) {
+        const T* gs = g + (long)5 * s * n;
         T C = 0;
 #pragma unroll
-        for (int q = 0; q < 5; q++) C += g[q * n + i];
-        const double r = (double)C - target[i];
-        r2 = r * r;
-        if (cfinal) cfinal[i] = C;
+        for (int q = 0; q < 5; q++) C += gs[q * n + i];
+        const double r = (double)C - target[(long)s * n + i];
+        r2 += r * r;
+        if (cfinal) cfinal[(long)s * n + i] = C;
 #pragma unroll
-        for (int q = 0; q < 5; q++) lg[q * n + i] = (T)(2.0 * r);
+        for (int q = 0; q < 5; q++) lg[(long)(5 * s + q) * n + i] = (T)(2.0 * r);
     }
     for (int o = 32; o > 0; o >>= 1) r2 += __shfl_down(r2, o, 64);
     if ((threadIdx.x & 63) == 0) atomicAdd(loss, r2);
@@ -338,16 +387,21 @@ struct BwdArgs {
     T* lamF;       // adjoint of the site force
     const uint32_t* dom;
     const double* src;
-    T tauI, tauIc, decay;
+    T tauI;
+    T tauIc[LBIB_MAX_SPECIES], decay[LBIB_MAX_SPECIES];
+    T Km[LBIB_MAX_SPECIES * LBIB_MAX_SPECIES];
+    int hasK;
     int ncell;
-    double* grad;  // [0] tau_c, [1 + c] rate of cell c
+    double* grad;  // [s] tau_c of species s, [ns + c] rate of cell c, [ns + ncell + 1 + s ns + j] K_sj
 };
 
-// Per-site adjoint of pull + fluid collide + species collide.
-template <typename T>
+// Per-site adjoint of pull + fluid collide + species collides.
+template <typename T, int NS>
 __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
-    extern __shared__ double sgrad[];   // [ncell + 1]: tau_c at 0, rates at 1..ncell
-    for (int c = threadIdx.x; c <= a.ncell; c += blockDim.x) sgrad[c] = 0.0;
+    // [ns + ncell + 1 + ns^2] gradient slots, laid out like a.grad
+    extern __shared__ double sgrad[];
+    const int ng = NS + a.ncell + 1 + NS * NS;
+    for (int c = threadIdx.x; c < ng; c += blockDim.x) sgrad[c] = 0.0;
     __syncthreads();
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.n) {
@@ -356,11 +410,13 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
         long idx[9];
         pull_index(x, y, a.nx, a.ny, idx);
         const long n = a.n;
-        T d[9], g[5];
+        T d[9], g[NS][5];
 #pragma unroll
         for (int q = 0; q < 9; q++) d[q] = a.fs[q * n + idx[q]];
 #pragma unroll
-        for (int q = 0; q < 5; q++) g[q] = a.gs[q * n + idx[q]];
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) g[s][q] = a.gs[(5 * s + q) * n + idx[q]];
         const T Fx = a.Fs[2 * i], Fy = a.Fs[2 * i + 1];
         // forward values
         T rho = 0;
@@ -370,37 +426,60 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
         const T ux = (d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW])) * rhoI;
         const T uy = (d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE])) * rhoI;
         const T uox = ux + Fx, uoy = uy + Fy;   // post-collision velocity (exact relation)
-        T C = 0;
+        T C[NS];
 #pragma unroll
-        for (int q = 0; q < 5; q++) C += g[q];
+        for (int s = 0; s < NS; s++) {
+            C[s] = 0;
+#pragma unroll
+            for (int q = 0; q < 5; q++) C[s] += g[s][q];
+        }
         const uint32_t dm = a.dom[i];
         // adjoints of step t+1 at this site
-        T lf[9], lg[5];
+        T lf[9], lg[NS][5];
 #pragma unroll
         for (int q = 0; q < 9; q++) lf[q] = a.lf1[q * n + i];
 #pragma unroll
-        for (int q = 0; q < 5; q++) lg[q] = a.lg1[q * n + i];
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) lg[s][q] = a.lg1[(5 * s + q) * n + i];
         T luox = a.lamU[2 * i], luoy = a.lamU[2 * i + 1];
         a.lamU[2 * i] = 0;
         a.lamU[2 * i + 1] = 0;
-        // species: g'_p = g_p (1 - tauIc) + geq_p(C, uo) tauIc + R r_p
+        // species s: g'_p = g_p (1 - tauIc_s) + geq_p(C_s, uo) tauIc_s + R_s r_p with
+        // R_s = sum_j K_sj C_j + (inside: rate of the cell for s = 0; outside: -decay_s C_s)
         const T ae[5] = {(T)1 / (T)3, ((T)1 + (T)3 * uox) / (T)6, ((T)1 + (T)3 * uoy) / (T)6,
                          ((T)1 - (T)3 * uox) / (T)6, ((T)1 - (T)3 * uoy) / (T)6};
         const T rr[5] = {(T)1 / (T)3, (T)1 / (T)6, (T)1 / (T)6, (T)1 / (T)6, (T)1 / (T)6};
-        const T dRdC = dm != 0u ? (T)0 : -a.decay;
-        T S = 0, gtau = 0, gsrc = 0;
+        T P[NS];   // adjoint of R_s
 #pragma unroll
-        for (int p = 0; p < 5; p++) {
-            S += lg[p] * (ae[p] * a.tauIc + rr[p] * dRdC);
-            gtau += lg[p] * (g[p] - ae[p] * C);
-            gsrc += lg[p] * rr[p];
+        for (int s = 0; s < NS; s++) {
+            P[s] = 0;
+#pragma unroll
+            for (int p = 0; p < 5; p++) P[s] += lg[s][p] * rr[p];
         }
-        T lgh[5];
+        T lgh[NS][5];
 #pragma unroll
-        for (int q = 0; q < 5; q++) lgh[q] = lg[q] * ((T)1 - a.tauIc) + S;
-        // d geq_p / d uo = C/2 e_p for p = E, N, W, S
-        luox += C / (T)2 * a.tauIc * (lg[DE] - lg[DW]);
-        luoy += C / (T)2 * a.tauIc * (lg[DN] - lg[DS]);
+        for (int j = 0; j < NS; j++) {
+            const T tauIc = a.tauIc[j];
+            const T dRdC = dm != 0u ? (T)0 : -a.decay[j];
+            T S = 0, gtau = 0;
+#pragma unroll
+            for (int p = 0; p < 5; p++) {
+                S += lg[j][p] * (ae[p] * tauIc + rr[p] * dRdC);
+                gtau += lg[j][p] * (g[j][p] - ae[p] * C[j]);
+            }
+            if (a.hasK) {
+#pragma unroll
+                for (int s = 0; s < NS; s++) S += P[s] * a.Km[s * LBIB_MAX_SPECIES + j];
+            }
+#pragma unroll
+            for (int q = 0; q < 5; q++) lgh[j][q] = lg[j][q] * ((T)1 - tauIc) + S;
+            // d geq_p / d uo = C/2 e_p for p = E, N, W, S
+            luox += C[j] / (T)2 * tauIc * (lg[j][DE] - lg[j][DW]);
+            luoy += C[j] / (T)2 * tauIc * (lg[j][DN] - lg[j][DS]);
+            // d g'_p / d tau_c = (g_p - geq_p) / tau_c^2
+            atomicAdd(&sgrad[j], (double)(gtau * tauIc * tauIc));
+        }
         // fluid: f'_q = f_q (1 - tauI) + feq_q(rho, u) tauI + 3 w_q rho (e_q . F), uo = u + F
         const int ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
         const int ey[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
@@ -426,15 +505,22 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
             a.lf0[q * n + idx[q]] = lfh;
         }
 #pragma unroll
-        for (int q = 0; q < 5; q++) a.lg0[q * n + idx[q]] = lgh[q];
+        for (int s = 0; s < NS; s++)
+#pragma unroll
+            for (int q = 0; q < 5; q++) a.lg0[(5 * s + q) * n + idx[q]] = lgh[s][q];
         a.lamF[2 * i] = lFx;
         a.lamF[2 * i + 1] = lFy;
-        // d g'_p / d tau_c = (g_p - geq_p) / tau_c^2
-        atomicAdd(&sgrad[0], (double)(gtau * a.tauIc * a.tauIc));
-        if (dm != 0u && (int)dm <= a.ncell) atomicAdd(&sgrad[dm], (double)gsrc);
+        if (dm != 0u && (int)dm <= a.ncell) atomicAdd(&sgrad[NS + dm], (double)P[0]);
+        if (a.hasK) {
+#pragma unroll
+            for (int s = 0; s < NS; s++)
+#pragma unroll
+                for (int j = 0; j < NS; j++)
+                    atomicAdd(&sgrad[NS + a.ncell + 1 + s * NS + j], (double)(P[s] * C[j]));
+        }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c <= a.ncell; c += blockDim.x)
+    for (int c = threadIdx.x; c < ng; c += blockDim.x)
         if (sgrad[c] != 0.0) atomicAdd(&a.grad[c], sgrad[c]);
 }
 
@@ -471,12 +557,16 @@ __global__ void k_spread_bwd(int nm, const T* __restrict__ X, const T* __restric
 
 // tension adjoint: connection (a, b) gives F_a += kappa n, F_b -= kappa n
 // with n = (X_b - X_a)/|X_b - X_a|; each marker gathers its two connections.
+// Junction (k, m): F_k += k_j phi(d), d = X_m - X_k, phi(d) = (|d| - l0) d/|d|,
+// listed at both ends; J = d phi / d d = (1 - l0/|d|) I + (l0/|d|) n n^T is
+// symmetric, so lambda_X_k += k_j J (lambda_F_m - lambda_F_k) and
+// dL/dk_j = sum_k lambda_F_k . phi(d).
 template <typename T>
 __global__ void k_force_bwd(int nm, const T* __restrict__ X, const int32_t* __restrict__ nxt,
                             const int32_t* __restrict__ prv, T kappa, const T* __restrict__ lamFm, T* lamX,
-                            double* gkappa) {
+                            double* gkappa, const Junctions jn, double* gkj) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    double gk = 0.0;
+    double gk = 0.0, gj = 0.0;
     if (k < nm) {
         const T xk = X[2 * k], yk = X[2 * k + 1];
         T ax = 0, ay = 0;
@@ -503,11 +593,32 @@ __global__ void k_force_bwd(int nm, const T* __restrict__ X, const int32_t* __re
             ax += kappa * (vx - nxv * nv) / dist;
             ay += kappa * (vy - nyv * nv) / dist;
         }
+        if (jn.ofs) {
+            const T kj = (T)jn.kj, l0 = (T)jn.l0;
+            const T lkx = lamFm[2 * k], lky = lamFm[2 * k + 1];
+            for (int p = jn.ofs[k]; p < jn.ofs[k + 1]; p++) {
+                const int m = jn.part[p];
+                const T dx = X[2 * m] - xk, dy = X[2 * m + 1] - yk;
+                const T dist = sqrt(dx * dx + dy * dy);
+                const T nxv = dx / dist, nyv = dy / dist;
+                const T vx = lamFm[2 * m] - lkx, vy = lamFm[2 * m + 1] - lky;
+                const T a1 = (T)1 - l0 / dist, a2 = l0 / dist * (nxv * vx + nyv * vy);
+                ax += kj * (a1 * vx + a2 * nxv);
+                ay += kj * (a1 * vy + a2 * nyv);
+                gj += (double)((dist - l0) * (lkx * nxv + lky * nyv));
+            }
+        }
         lamX[2 * k] += ax;
         lamX[2 * k + 1] += ay;
     }
-    for (int o = 32; o > 0; o >>= 1) gk += __shfl_down(gk, o, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(gkappa, gk);
+    for (int o = 32; o > 0; o >>= 1) {
+        gk += __shfl_down(gk, o, 64);
+        gj += __shfl_down(gj, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(gkappa, gk);
+        if (jn.ofs) atomicAdd(gkj, gj);
+    }
 }
 
 }  // namespace adj
@@ -538,8 +649,17 @@ struct lbib_adj {
     void *lf[2] = {nullptr, nullptr}, *lg[2] = {nullptr, nullptr};
     void *lX = nullptr, *lFm = nullptr, *lU = nullptr, *lF = nullptr;
     int steps = 0;
-    double tau_c = 0.8, kappa = 0.01;
+    double kappa = 0.01;
     std::vector<double> rates;
+    // species (lbib_adj_set_species; one by default) and the reaction matrix
+    int ns = 1;
+    double tau_c[LBIB_MAX_SPECIES] = {0.8, 0.8, 0.8, 0.8};
+    double decay[LBIB_MAX_SPECIES] = {0, 0, 0, 0};
+    double Km[LBIB_MAX_SPECIES * LBIB_MAX_SPECIES] = {};
+    void* cfin = nullptr;           // final concentrations [ns][n] (on request)
+    // junction springs (lbib_adj_set_junctions): partners of each marker
+    int32_t *jofs = nullptr, *jpart = nullptr;
+    double kj = 0.0, l0 = 0.5;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double fwd_ms = 0.0, grad_ms = 0.0;
     bool have_problem = false, have_forward = false;
@@ -572,17 +692,20 @@ template <typename T>
 T* tp(void* p) { return static_cast<T*>(p); }
 
 // state slot offsets (in T elements) of the checkpoint and segment stores
-inline size_t state_len(const lbib_adj* a) { return 14 * (size_t)a->n + 2 * (size_t)a->nm; }
+inline size_t gplanes(const lbib_adj* a) { return 5 * (size_t)a->ns; }
+inline size_t state_len(const lbib_adj* a) { return (9 + gplanes(a)) * (size_t)a->n + 2 * (size_t)a->nm; }
+// gradient slots: tau_c [ns], rates [ncell + 1], K [ns^2], then kappa and k_j
+inline int grad_len(int ns, int ncell) { return ns + ncell + 1 + ns * ns + 2; }
 inline size_t extra_len(const lbib_adj* a) { return 4 * (size_t)a->n + 2 * (size_t)a->nm; }
 
 template <typename T>
 struct Slot {
-    T *f, *g, *X;   // 9n, 5n, 2nm
+    T *f, *g, *X;   // 9n, 5 ns n, 2nm
 };
 template <typename T>
 Slot<T> state_slot(void* base, const lbib_adj* a, int j) {
     T* s = tp<T>(base) + (size_t)j * state_len(a);
-    return {s, s + 9 * a->n, s + 14 * a->n};
+    return {s, s + 9 * a->n, s + (9 + gplanes(a)) * a->n};
 }
 template <typename T>
 struct Extra {
@@ -601,9 +724,31 @@ adj::LatArgs<T> lat_args(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, co
     l.fs = fs; l.gs = gs; l.fd = fd; l.gd = gd; l.Fs = Fs; l.u = u;
     l.dom = a->dom; l.src = a->src;
     l.tauI = (T)(1.0 / a->cfg.tau_fluid);
-    l.tauIc = (T)(1.0 / a->tau_c);
-    l.decay = (T)a->cfg.decay;
+    l.hasK = 0;
+    for (int s = 0; s < LBIB_MAX_SPECIES; s++) {
+        l.tauIc[s] = (T)(1.0 / a->tau_c[s]);
+        l.decay[s] = (T)a->decay[s];
+        for (int j = 0; j < LBIB_MAX_SPECIES; j++) {
+            l.Km[s * LBIB_MAX_SPECIES + j] = (T)a->Km[s * LBIB_MAX_SPECIES + j];
+            if (a->Km[s * LBIB_MAX_SPECIES + j] != 0.0) l.hasK = 1;
+        }
+    }
     return l;
+}
+
+template <typename T>
+adj::Junctions junctions(const lbib_adj* a) {
+    return adj::Junctions{a->jofs, a->jpart, a->kj, a->l0};
+}
+
+template <typename T>
+void launch_lattice(lbib_adj* a, const adj::LatArgs<T>& l) {
+    switch (a->ns) {
+        case 1: adj::k_lattice<T, 1><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+        case 2: adj::k_lattice<T, 2><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+        case 3: adj::k_lattice<T, 3><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+        default: adj::k_lattice<T, 4><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+    }
 }
 
 // one forward step from (fs, gs, X) into (fd, gd, X); Fs, Fm, u receive the
@@ -613,10 +758,11 @@ int step_fwd(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, T* X, T* Fs, T
     const int nx = a->cfg.nx, ny = a->cfg.ny;
     ADJ_HIP(hipMemsetAsync(Fs, 0, 2 * a->n * sizeof(T), a->stream));
     if (a->nm > 0) {
-        adj::k_force<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, a->nxt, a->prv, (T)a->kappa, Fm);
+        adj::k_force<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, a->nxt, a->prv, (T)a->kappa, Fm,
+                                                          junctions<T>(a));
         adj::k_spread<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, Fm, Fs, nx, ny);
     }
-    adj::k_lattice<T><<<nb(a->n), 256, 0, a->stream>>>(lat_args<T>(a, fs, gs, fd, gd, Fs, u));
+    launch_lattice<T>(a, lat_args<T>(a, fs, gs, fd, gd, Fs, u));
     if (a->nm > 0) adj::k_interp_move<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, u, nx, ny);
     ADJ_HIP(hipGetLastError());
     return 0;
@@ -632,7 +778,7 @@ int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
         a->nck = need;
     }
     ADJ_HIP(hipMemcpyAsync(a->f[0], a->f0, 9 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
-    ADJ_HIP(hipMemcpyAsync(a->g[0], a->g0, 5 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+    ADJ_HIP(hipMemcpyAsync(a->g[0], a->g0, gplanes(a) * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
     ADJ_HIP(hipMemcpyAsync(a->X, a->X0, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
     ADJ_HIP(hipEventRecord(a->e0, a->stream));
     int cur = 0;
@@ -640,7 +786,8 @@ int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
         if (t % a->K == 0) {
             Slot<T> s = state_slot<T>(a->ck, a, t / a->K);
             ADJ_HIP(hipMemcpyAsync(s.f, a->f[cur], 9 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
-            ADJ_HIP(hipMemcpyAsync(s.g, a->g[cur], 5 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+            ADJ_HIP(hipMemcpyAsync(s.g, a->g[cur], gplanes(a) * a->n * sizeof(T), hipMemcpyDeviceToDevice,
+                                   a->stream));
             ADJ_HIP(hipMemcpyAsync(s.X, a->X, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
         }
         if (step_fwd<T>(a, tp<T>(a->f[cur]), tp<T>(a->g[cur]), tp<T>(a->f[1 - cur]), tp<T>(a->g[1 - cur]),
@@ -650,9 +797,12 @@ int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
     }
     ADJ_HIP(hipMemsetAsync(a->loss, 0, sizeof(double), a->stream));
     T* cf = nullptr;
-    if (cfinal) cf = tp<T>(a->u);   // reuse: n values (u holds 2n)
-    adj::k_loss<T><<<nb(a->n), 256, 0, a->stream>>>(tp<T>(a->g[cur]), a->target, a->n, tp<T>(a->lg[0]), a->loss,
-                                                     cf);
+    if (cfinal) {
+        if (!a->cfin && adj_alloc(a, &a->cfin, (size_t)a->ns * a->n * sizeof(T))) return -1;
+        cf = tp<T>(a->cfin);
+    }
+    adj::k_loss<T><<<nb(a->n), 256, 0, a->stream>>>(tp<T>(a->g[cur]), a->target, a->n, a->ns, tp<T>(a->lg[0]),
+                                                     a->loss, cf);
     ADJ_HIP(hipEventRecord(a->e1, a->stream));
     ADJ_HIP(hipEventSynchronize(a->e1));
     float ms = 0.0f;
@@ -660,16 +810,30 @@ int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
     a->fwd_ms = ms;
     ADJ_HIP(hipMemcpy(loss, a->loss, sizeof(double), hipMemcpyDeviceToHost));
     if (cfinal) {
-        std::vector<T> tmp(a->n);
-        ADJ_HIP(hipMemcpy(tmp.data(), cf, a->n * sizeof(T), hipMemcpyDeviceToHost));
-        for (long i = 0; i < a->n; i++) cfinal[i] = (double)tmp[i];
+        const size_t cnt = (size_t)a->ns * a->n;
+        std::vector<T> tmp(cnt);
+        ADJ_HIP(hipMemcpy(tmp.data(), cf, cnt * sizeof(T), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < cnt; i++) cfinal[i] = (double)tmp[i];
     }
     a->steps = steps;
     return 0;
 }
 
 template <typename T>
-int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_rates) {
+void launch_lattice_bwd(lbib_adj* a, const adj::BwdArgs<T>& b) {
+    const size_t lds = (size_t)(a->ns + a->ncell + 1 + a->ns * a->ns) * sizeof(double);
+    switch (a->ns) {
+        case 1: adj::k_lattice_bwd<T, 1><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+        case 2: adj::k_lattice_bwd<T, 2><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+        case 3: adj::k_lattice_bwd<T, 3><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+        default: adj::k_lattice_bwd<T, 4><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+    }
+}
+
+// gradient slots into the caller's arrays (any may be null): tau_c [ns],
+// kappa, k_j, K [ns ns] row-major, rates [ncell + 1] of species 0 (entry 0 = 0)
+template <typename T>
+int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_kj, double* d_K, double* d_rates) {
     const int nx = a->cfg.nx, ny = a->cfg.ny;
     const int steps = a->steps;
     ADJ_HIP(hipEventRecord(a->e0, a->stream));
@@ -678,7 +842,8 @@ int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_rates) {
     ADJ_HIP(hipMemsetAsync(a->lf[0], 0, 9 * a->n * sizeof(T), a->stream));
     ADJ_HIP(hipMemsetAsync(a->lX, 0, 2 * (size_t)a->nm * sizeof(T), a->stream));
     ADJ_HIP(hipMemsetAsync(a->lU, 0, 2 * a->n * sizeof(T), a->stream));
-    ADJ_HIP(hipMemsetAsync(a->grad, 0, (a->ncell + 2) * sizeof(double), a->stream));
+    const int gl = grad_len(a->ns, a->ncell);
+    ADJ_HIP(hipMemsetAsync(a->grad, 0, gl * sizeof(double), a->stream));
     const int nseg = (steps + a->K - 1) / a->K;
     for (int sgi = nseg - 1; sgi >= 0; sgi--) {
         const int t0 = sgi * a->K, len = std::min(a->K, steps - t0);
@@ -711,18 +876,20 @@ int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_rates) {
             b.lf0 = tp<T>(a->lf[1 - cur]); b.lg0 = tp<T>(a->lg[1 - cur]);
             b.lamF = tp<T>(a->lF);
             b.dom = a->dom; b.src = a->src;
-            b.tauI = (T)(1.0 / a->cfg.tau_fluid);
-            b.tauIc = (T)(1.0 / a->tau_c);
-            b.decay = (T)a->cfg.decay;
+            const adj::LatArgs<T> l = lat_args<T>(a, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+            b.tauI = l.tauI;
+            for (int q = 0; q < LBIB_MAX_SPECIES; q++) { b.tauIc[q] = l.tauIc[q]; b.decay[q] = l.decay[q]; }
+            for (int q = 0; q < LBIB_MAX_SPECIES * LBIB_MAX_SPECIES; q++) b.Km[q] = l.Km[q];
+            b.hasK = l.hasK;
             b.ncell = a->ncell;
             b.grad = a->grad;
-            adj::k_lattice_bwd<T><<<nb(a->n), 256, (a->ncell + 1) * sizeof(double), a->stream>>>(b);
+            launch_lattice_bwd<T>(a, b);
             if (a->nm > 0) {
                 adj::k_spread_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, e.Fm, tp<T>(a->lF), tp<T>(a->lFm),
                                                                          tp<T>(a->lX), nx, ny);
                 adj::k_force_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, a->nxt, a->prv, (T)a->kappa,
-                                                                        tp<T>(a->lFm), tp<T>(a->lX),
-                                                                        a->grad + a->ncell + 1);
+                                                                        tp<T>(a->lFm), tp<T>(a->lX), a->grad + gl - 2,
+                                                                        junctions<T>(a), a->grad + gl - 1);
             }
             ADJ_HIP(hipGetLastError());
             cur = 1 - cur;
@@ -733,14 +900,19 @@ int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_rates) {
     float ms = 0.0f;
     ADJ_HIP(hipEventElapsedTime(&ms, a->e0, a->e1));
     a->grad_ms = ms;
-    std::vector<double> gr(a->ncell + 2);
+    std::vector<double> gr(gl);
     ADJ_HIP(hipMemcpy(gr.data(), a->grad, gr.size() * sizeof(double), hipMemcpyDeviceToHost));
-    if (d_tau) *d_tau = gr[0];
-    if (d_kappa) *d_kappa = gr[a->ncell + 1];
+    const int ns = a->ns;
+    if (d_tau)
+        for (int s = 0; s < ns; s++) d_tau[s] = gr[s];
     if (d_rates) {
         d_rates[0] = 0.0;
-        for (int c = 1; c <= a->ncell; c++) d_rates[c] = gr[c];
+        for (int c = 1; c <= a->ncell; c++) d_rates[c] = gr[ns + c];
     }
+    if (d_K)
+        for (int q = 0; q < ns * ns; q++) d_K[q] = gr[ns + a->ncell + 1 + q];
+    if (d_kappa) *d_kappa = gr[gl - 2];
+    if (d_kj) *d_kj = gr[gl - 1];
     return 0;
 }
 
@@ -776,6 +948,7 @@ int lbib_adj_create(const lbib_adj_config* cfg, lbib_adj** out) {
     a->prec = cfg->precision;
     a->tb = a->prec == LBIB_FP64 ? 8 : 4;
     a->rates.assign(a->ncell + 1, 0.0);
+    a->decay[0] = cfg->decay;
     ADJ_HIP(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
     ADJ_HIP(hipEventCreate(&a->e0));
     ADJ_HIP(hipEventCreate(&a->e1));
@@ -790,7 +963,7 @@ int lbib_adj_create(const lbib_adj_config* cfg, lbib_adj** out) {
         adj_alloc(a, &a->X0, 2 * nm * tb) || adj_alloc(a, (void**)&a->nxt, nm * 4) ||
         adj_alloc(a, (void**)&a->prv, nm * 4) || adj_alloc(a, (void**)&a->dom, n * 4) ||
         adj_alloc(a, (void**)&a->target, n * 8) || adj_alloc(a, (void**)&a->src, (a->ncell + 1) * 8) ||
-        adj_alloc(a, (void**)&a->grad, (a->ncell + 2) * 8) || adj_alloc(a, (void**)&a->loss, 8) ||
+        adj_alloc(a, (void**)&a->grad, grad_len(LBIB_MAX_SPECIES, a->ncell) * 8) || adj_alloc(a, (void**)&a->loss, 8) ||
         adj_alloc(a, &a->lX, 2 * nm * tb) || adj_alloc(a, &a->lFm, 2 * nm * tb) || adj_alloc(a, &a->lU, 2 * n * tb) ||
         adj_alloc(a, &a->lF, 2 * n * tb))
         return -1;
@@ -804,9 +977,9 @@ void lbib_adj_destroy(lbib_adj* a) {
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     void** bufs[] = {&a->f[0], &a->f[1], &a->g[0], &a->g[1], &a->lf[0], &a->lf[1], &a->lg[0], &a->lg[1],
                      &a->X, &a->Fm, &a->Fs, &a->u, &a->f0, &a->g0, &a->X0, &a->ck, &a->seg, &a->lX, &a->lFm,
-                     &a->lU, &a->lF};
+                     &a->lU, &a->lF, &a->cfin};
     for (void** b : bufs) adj_free(*b);
-    void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss};
+    void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss, a->jofs, a->jpart};
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (a->e0) (void)hipEventDestroy(a->e0);
@@ -820,6 +993,10 @@ const char* lbib_adj_last_error(const lbib_adj* a) { return a ? a->err.c_str() :
 int lbib_adj_set_problem(lbib_adj* a, const double* f0, const double* g0, const double* xy0, const int32_t* next,
                          const uint32_t* dom, const double* target) {
     if (!a || !f0 || !g0 || !dom || !target || (a->nm > 0 && (!xy0 || !next))) return -1;
+    if (a->ns != 1) {
+        a->err = "lbib_adj_set_problem: set the problem before lbib_adj_set_species";
+        return -1;
+    }
     const size_t n = (size_t)a->n, nm = (size_t)a->nm;
     // ring successors must form closed rings (every marker has one predecessor)
     std::vector<int32_t> prv(nm, -1);
@@ -858,13 +1035,87 @@ int lbib_adj_set_problem(lbib_adj* a, const double* f0, const double* g0, const 
     return 0;
 }
 
-int lbib_adj_forward(lbib_adj* a, int steps, double tau_c, double kappa, const double* rates, double* loss,
-                     double* c_final) {
-    if (!a || !loss || steps < 1) return -1;
+int lbib_adj_set_species(lbib_adj* a, int nspecies, const double* g0, const double* target, const double* decay) {
+    if (!a || !g0 || !target || !decay) return -1;
+    if (!a->have_problem) { a->err = "lbib_adj_set_species: set the problem first"; return -1; }
+    if (nspecies < 1 || nspecies > LBIB_MAX_SPECIES) {
+        a->err = "lbib_adj_set_species: 1 to LBIB_MAX_SPECIES species";
+        return -1;
+    }
+    ADJ_HIP(hipStreamSynchronize(a->stream));
+    a->ns = nspecies;
+    const size_t tb = a->tb, n = (size_t)a->n, gp = gplanes(a);
+    for (int b = 0; b < 2; b++) {
+        adj_free(a->g[b]);
+        adj_free(a->lg[b]);
+        if (adj_alloc(a, &a->g[b], gp * n * tb) || adj_alloc(a, &a->lg[b], gp * n * tb)) return -1;
+    }
+    adj_free(a->g0);
+    adj_free(a->seg);
+    adj_free(a->ck);
+    adj_free(a->cfin);
+    a->nck = 0;
+    if (a->target) (void)hipFree(a->target);
+    a->target = nullptr;
+    if (adj_alloc(a, &a->g0, gp * n * tb) || adj_alloc(a, (void**)&a->target, nspecies * n * 8) ||
+        adj_alloc(a, &a->seg, (size_t)a->K * (state_len(a) + extra_len(a)) * tb))
+        return -1;
+    if (a->prec == LBIB_FP64 ? upload_T<double>(a, a->g0, g0, gp * n) : upload_T<float>(a, a->g0, g0, gp * n))
+        return -1;
+    ADJ_HIP(hipMemcpy(a->target, target, nspecies * n * 8, hipMemcpyHostToDevice));
+    for (int s = 0; s < LBIB_MAX_SPECIES; s++) a->decay[s] = s < nspecies ? decay[s] : 0.0;
+    a->have_forward = false;
+    return 0;
+}
+
+int lbib_adj_set_junctions(lbib_adj* a, int npairs, const int32_t* pairs, double l0) {
+    if (!a || npairs < 0 || (npairs > 0 && !pairs)) return -1;
+    if (!(l0 >= 0.0)) { a->err = "lbib_adj_set_junctions: l0 must be non-negative"; return -1; }
+    const int nm = a->nm;
+    std::vector<int32_t> cnt(nm + 1, 0);
+    for (int p = 0; p < npairs; p++) {
+        const int u = pairs[2 * p], v = pairs[2 * p + 1];
+        if (u < 0 || v < 0 || u >= nm || v >= nm || u == v) {
+            a->err = "lbib_adj_set_junctions: pair of invalid or equal marker indices";
+            return -1;
+        }
+        cnt[u + 1]++;
+        cnt[v + 1]++;
+    }
+    for (int k = 0; k < nm; k++) cnt[k + 1] += cnt[k];
+    std::vector<int32_t> part(std::max(2 * npairs, 1)), cur(cnt.begin(), cnt.end() - 1);
+    for (int p = 0; p < npairs; p++) {
+        const int u = pairs[2 * p], v = pairs[2 * p + 1];
+        part[cur[u]++] = v;
+        part[cur[v]++] = u;
+    }
+    ADJ_HIP(hipStreamSynchronize(a->stream));
+    if (a->jofs) (void)hipFree(a->jofs);
+    if (a->jpart) (void)hipFree(a->jpart);
+    a->jofs = a->jpart = nullptr;
+    if (npairs > 0) {
+        if (adj_alloc(a, (void**)&a->jofs, (nm + 1) * 4) || adj_alloc(a, (void**)&a->jpart, part.size() * 4))
+            return -1;
+        ADJ_HIP(hipMemcpy(a->jofs, cnt.data(), (nm + 1) * 4, hipMemcpyHostToDevice));
+        ADJ_HIP(hipMemcpy(a->jpart, part.data(), part.size() * 4, hipMemcpyHostToDevice));
+    }
+    a->l0 = l0;
+    a->have_forward = false;
+    return 0;
+}
+
+int lbib_adj_forward_ms(lbib_adj* a, int steps, const double* tau_c, double kappa, double k_junction,
+                        const double* K, const double* rates, double* loss, double* c_final) {
+    if (!a || !loss || !tau_c || steps < 1) return -1;
     if (!a->have_problem) { a->err = "lbib_adj_forward: no problem set"; return -1; }
-    if (!(tau_c > 0.5)) { a->err = "lbib_adj_forward: tau_c must exceed 0.5"; return -1; }
-    a->tau_c = tau_c;
+    for (int s = 0; s < a->ns; s++)
+        if (!(tau_c[s] > 0.5)) { a->err = "lbib_adj_forward: tau_c must exceed 0.5"; return -1; }
+    for (int s = 0; s < LBIB_MAX_SPECIES; s++) a->tau_c[s] = s < a->ns ? tau_c[s] : 0.8;
+    for (int s = 0; s < LBIB_MAX_SPECIES; s++)
+        for (int j = 0; j < LBIB_MAX_SPECIES; j++)
+            a->Km[s * LBIB_MAX_SPECIES + j] = (K && s < a->ns && j < a->ns) ? K[s * a->ns + j] : 0.0;
     a->kappa = kappa;
+    a->kj = k_junction;
     for (int c = 0; c <= a->ncell; c++) a->rates[c] = rates && c > 0 ? rates[c] : 0.0;
     ADJ_HIP(hipMemcpy(a->src, a->rates.data(), (a->ncell + 1) * 8, hipMemcpyHostToDevice));
     const int r = a->prec == LBIB_FP64 ? forward_t<double>(a, steps, loss, c_final)
@@ -873,14 +1124,28 @@ int lbib_adj_forward(lbib_adj* a, int steps, double tau_c, double kappa, const d
     return r;
 }
 
-int lbib_adj_gradient(lbib_adj* a, double* d_tau_c, double* d_kappa, double* d_rates) {
+int lbib_adj_gradient_ms(lbib_adj* a, double* d_tau_c, double* d_kappa, double* d_k_junction, double* d_K,
+                         double* d_rates) {
     if (!a) return -1;
     if (!a->have_forward) { a->err = "lbib_adj_gradient: run lbib_adj_forward first"; return -1; }
-    const int r = a->prec == LBIB_FP64 ? gradient_t<double>(a, d_tau_c, d_kappa, d_rates)
-                                       : gradient_t<float>(a, d_tau_c, d_kappa, d_rates);
+    const int r = a->prec == LBIB_FP64 ? gradient_t<double>(a, d_tau_c, d_kappa, d_k_junction, d_K, d_rates)
+                                       : gradient_t<float>(a, d_tau_c, d_kappa, d_k_junction, d_K, d_rates);
     // the loss adjoint in lg[0] was consumed: a new forward is needed
     a->have_forward = false;
     return r;
+}
+
+int lbib_adj_forward(lbib_adj* a, int steps, double tau_c, double kappa, const double* rates, double* loss,
+                     double* c_final) {
+    if (!a) return -1;
+    if (a->ns != 1) { a->err = "lbib_adj_forward: several species; use lbib_adj_forward_ms"; return -1; }
+    return lbib_adj_forward_ms(a, steps, &tau_c, kappa, 0.0, nullptr, rates, loss, c_final);
+}
+
+int lbib_adj_gradient(lbib_adj* a, double* d_tau_c, double* d_kappa, double* d_rates) {
+    if (!a) return -1;
+    if (a->ns != 1) { a->err = "lbib_adj_gradient: several species; use lbib_adj_gradient_ms"; return -1; }
+    return lbib_adj_gradient_ms(a, d_tau_c, d_kappa, nullptr, nullptr, d_rates);
 }
 
 int lbib_adj_timing(lbib_adj* a, double* forward_ms, double* gradient_ms) {

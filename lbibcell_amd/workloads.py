@@ -223,3 +223,46 @@ def adjoint_case(nx=512, ny=512, cols=3, rows=3, radius=40.0, n=150, rate=(2e-4,
             dom[inside_polygon(xx, yy, ring)] = len(rings)
     rates = rng.uniform(rate[0], rate[1], size=len(rings))
     return rings, dom, rates
+
+
+def cell_map(rings, nx, ny):
+    """Cell index of every site (0 outside, ring c -> c + 1), each ring tested
+    only over its bounding box."""
+    dom = np.zeros((ny, nx), dtype=np.uint32)
+    for c, ring in enumerate(rings):
+        x0, y0 = max(int(np.floor(ring[:, 0].min())), 0), max(int(np.floor(ring[:, 1].min())), 0)
+        x1, y1 = min(int(np.ceil(ring[:, 0].max())) + 1, nx), min(int(np.ceil(ring[:, 1].max())) + 1, ny)
+        yy, xx = np.mgrid[y0:y1, x0:x1]
+        sub = dom[y0:y1, x0:x1]
+        sub[inside_polygon(xx, yy, ring)] = c + 1
+    return dom
+
+
+def junction_pairs(rings, radius):
+    """Marker pairs for junction springs: every marker with its closest marker
+    of another ring within `radius` (the pairing the cell-junction BioSolver
+    makes), each pair once, as global indices of the concatenated rings."""
+    from scipy.spatial import cKDTree
+    xy = np.concatenate(rings)
+    owner = np.concatenate([np.full(len(r), c) for c, r in enumerate(rings)])
+    k = min(16, len(xy))
+    dist, idx = cKDTree(xy).query(xy, k=k, distance_upper_bound=radius)
+    pairs = set()
+    for a in range(len(xy)):
+        for d, b in zip(dist[a], idx[a]):
+            if not np.isfinite(d):
+                break
+            if owner[b] != owner[a]:
+                pairs.add((min(a, int(b)), max(a, int(b))))
+                break
+    return np.array(sorted(pairs), dtype=np.int32).reshape(-1, 2)
+
+
+def tissue_adjoint_case(nx, ny, ncell, radius=30.0, rjit=5.0, n=120, rate=(2e-4, 1e-3), seed=1):
+    """The packed touching tissue of packed_tissue as an adjoint problem:
+    marker rings, the cell map and one hidden source rate per cell."""
+    nodes, _ = packed_tissue(nx, ny, ncell, radius, rjit, n, seed=seed)
+    xy = np.array([[x, y] for _, x, y in nodes])
+    rings = [xy[c * n:(c + 1) * n] for c in range(ncell)]
+    rates = np.random.default_rng(seed + 7).uniform(rate[0], rate[1], size=ncell)
+    return rings, cell_map(rings, nx, ny), rates

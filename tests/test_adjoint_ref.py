@@ -53,3 +53,34 @@ def test_ring_tension_is_internal_and_interpolation_weights_sum_near_one():
     w, site = adjoint_ref._weights(X, 32, 32)
     # the fastCos delta integrates to 1 within its approximation error
     assert float((w.sum((1, 2)) - 1).abs().max()) < 5e-3
+
+
+def test_multispecies_step_reduces_to_the_single_species_step():
+    """step_ms with one species, K = 0 and no junction pairs is step; with
+    several species and K = 0 the species evolve independently."""
+    nx, ny, tau_f = 14, 12, 0.9
+    rng = np.random.default_rng(8)
+    dom = np.zeros((ny, nx), dtype=np.int64)
+    dom[3:8, 4:10] = 1
+    f0 = torch.tensor(W[:, None, None] * (1 + 0.05 * rng.standard_normal((9, ny, nx))))
+    g0 = torch.tensor(0.1 * (1 + 0.1 * rng.standard_normal((2, 5, ny, nx))))
+    m = 16
+    th = 2 * np.pi * np.arange(m) / m
+    X = torch.tensor(np.stack([7.2 + 2.5 * np.cos(th), 5.9 + 2.5 * np.sin(th)], 1))
+    nxt = torch.tensor([(k + 1) % m for k in range(m)])
+    prv = torch.tensor([(k - 1) % m for k in range(m)])
+    none = torch.zeros((0, 2), dtype=torch.long)
+    rates = torch.tensor([0.0, 1e-3], dtype=torch.float64)
+    dt = torch.tensor(dom)
+    f1, g1, X1 = adjoint_ref.step(f0, g0[0], X, nxt, prv, dt, tau_f, 0.8, 0.01, rates, 1e-3)
+    f2, g2, X2 = adjoint_ref.step_ms(f0, g0[:1], X, nxt, prv, none, dt, tau_f, [0.8], 0.01, 0.0, 0.5,
+                                     torch.zeros((1, 1), dtype=torch.float64), rates, [1e-3])
+    assert torch.equal(f1, f2) and torch.equal(g1, g2[0]) and torch.equal(X1, X2)
+    # species 1 with zero rate inside: its own decay outside only
+    _, g3, _ = adjoint_ref.step_ms(f0, g0, X, nxt, prv, none, dt, tau_f, [0.8, 0.8], 0.01, 0.0, 0.5,
+                                   torch.zeros((2, 2), dtype=torch.float64), rates, [1e-3, 1e-3])
+    assert torch.equal(g3[0], g2[0])
+    r0 = torch.zeros(2, dtype=torch.float64)
+    _, g4, _ = adjoint_ref.step(f0, g0[1], X, nxt, prv, dt, tau_f, 0.8, 0.01, r0, 1e-3)
+    assert torch.equal(g3[1], g4)
+

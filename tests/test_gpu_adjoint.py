@@ -203,3 +203,145 @@ def test_problem_validation():
     with pytest.raises(capi.LbibError):
         p.gradient()
     p.close()
+
+
+# ---------------------------------------------------------------- several species + junctions
+MS = dict(tau_c=[0.85, 0.95, 1.1], kappa=0.02, k_junction=0.004, decay=[1e-3, 2e-3, 5e-4], l0=0.5)
+
+
+def _ms_case(seed=11, nx=60, ny=36, ns=3):
+    """Two cells about 8 lu apart (junction pairs between them), three species
+    with a random reaction matrix."""
+    rings, dom, rates = workloads.adjoint_case(nx, ny, 2, 1, 10.5, 44, seed=seed)
+    pairs = workloads.junction_pairs(rings, 9.5)
+    rng = np.random.default_rng(seed)
+    f0 = W[:, None, None] * (1 + 0.02 * rng.standard_normal((9, ny, nx)))
+    g0 = 0.1 * (1 + 0.1 * rng.standard_normal((ns, 5, ny, nx)))
+    target = 0.5 * (1 + 0.1 * rng.standard_normal((ns, ny, nx)))
+    K = 1e-2 * rng.standard_normal((ns, ns))
+    return dict(nx=nx, ny=ny, rings=rings, dom=dom, rates=rates, pairs=pairs, f0=f0, g0=g0, target=target, K=K,
+                ns=ns)
+
+
+def _ms_problem(case, precision=capi.FP64, K=7, tau_f=0.9):
+    p = adjoint.AdjointProblem(case["nx"], case["ny"], tau_f, case["rings"], case["dom"], f0=case["f0"],
+                               g0=case["g0"][0], decay=MS["decay"][0], precision=precision, checkpoint_every=K)
+    p.set_species(case["g0"], case["target"], MS["decay"][: case["ns"]])
+    p.set_junctions(case["pairs"], MS["l0"])
+    return p
+
+
+def _ms_forward(p, case, steps, **over):
+    prm = dict(tau_c=MS["tau_c"][: case["ns"]], kappa=MS["kappa"], k_junction=MS["k_junction"], K=case["K"],
+               rates=case["rates"])
+    prm.update(over)
+    return p.forward_ms(steps, prm["tau_c"], prm["kappa"], prm["k_junction"], prm["K"], prm["rates"])
+
+
+def test_multispecies_junction_gradient_matches_torch_autograd():
+    import adjoint_ref
+    import torch
+    case = _ms_case(seed=13)
+    assert len(case["pairs"]) >= 4
+    steps = 12
+    prob, _ = _torch_problem(case)
+    prob.update(g0=case["g0"], target=case["target"], decay=MS["decay"], l0=MS["l0"], pairs=case["pairs"])
+    tc = torch.tensor(MS["tau_c"], dtype=torch.float64, requires_grad=True)
+    kp = torch.tensor(MS["kappa"], dtype=torch.float64, requires_grad=True)
+    kj = torch.tensor(MS["k_junction"], dtype=torch.float64, requires_grad=True)
+    Km = torch.tensor(case["K"], dtype=torch.float64, requires_grad=True)
+    rt = torch.tensor(case["rates"], dtype=torch.float64, requires_grad=True)
+    L = adjoint_ref.loss_ms(prob, steps, tc, kp, kj, Km, rt)
+    L.backward()
+    p = _ms_problem(case)
+    loss = _ms_forward(p, case, steps)
+    g = p.gradient_ms()
+    print(f"case multi-species adjoint vs autograd: loss {loss:.15e} / {L.item():.15e}", flush=True)
+    print(f"  dtau {g['tau_c']} / {tc.grad.numpy()}", flush=True)
+    print(f"  dkappa {g['kappa']:.10e} / {kp.grad.item():.10e}  dkj {g['k_junction']:.10e} / {kj.grad.item():.10e}",
+          flush=True)
+    print(f"  dK {g['K'].ravel()} / {Km.grad.numpy().ravel()}", flush=True)
+    assert loss == pytest.approx(L.item(), rel=1e-12)
+    np.testing.assert_allclose(g["tau_c"], tc.grad.numpy(), rtol=1e-9)
+    assert g["kappa"] == pytest.approx(kp.grad.item(), rel=1e-9)
+    assert g["k_junction"] == pytest.approx(kj.grad.item(), rel=1e-9)
+    np.testing.assert_allclose(g["K"], Km.grad.numpy(), rtol=1e-9, atol=1e-12 * np.abs(Km.grad.numpy()).max())
+    np.testing.assert_allclose(g["rates"], rt.grad.numpy(), rtol=1e-9)
+    p.close()
+
+
+def test_multispecies_junction_gradient_matches_central_differences():
+    case = _ms_case(seed=13)
+    steps = 30
+    p = _ms_problem(case, K=8)
+    _ms_forward(p, case, steps)
+    g = p.gradient_ms()
+
+    def fd(h, **shift):
+        return (_ms_forward(p, case, steps, **{k: v(+h) for k, v in shift.items()}) -
+                _ms_forward(p, case, steps, **{k: v(-h) for k, v in shift.items()})) / (2 * h)
+
+    def kshift(i, j):
+        def f(h):
+            K = case["K"].copy()
+            K[i, j] += h
+            return K
+        return f
+
+    def tshift(s):
+        def f(h):
+            t = list(MS["tau_c"][: case["ns"]])
+            t[s] += h
+            return t
+        return f
+    fd_kj = fd(1e-6, k_junction=lambda h: MS["k_junction"] + h)
+    fd_k01 = fd(1e-6, K=kshift(0, 1))
+    fd_k20 = fd(1e-6, K=kshift(2, 0))
+    fd_t2 = fd(1e-5, tau_c=tshift(2))
+    print(f"case multi-species adjoint vs FD: dkj {g['k_junction']:.10e} fd {fd_kj:.10e}; dK01 {g['K'][0, 1]:.10e} "
+          f"fd {fd_k01:.10e}; dK20 {g['K'][2, 0]:.10e} fd {fd_k20:.10e}; dtau2 {g['tau_c'][2]:.10e} fd {fd_t2:.10e}",
+          flush=True)
+    assert g["k_junction"] == pytest.approx(fd_kj, rel=1e-5)
+    assert g["K"][0, 1] == pytest.approx(fd_k01, rel=1e-5)
+    assert g["K"][2, 0] == pytest.approx(fd_k20, rel=1e-5)
+    assert g["tau_c"][2] == pytest.approx(fd_t2, rel=1e-5)
+    p.close()
+
+
+def test_single_species_through_the_multispecies_entry_points():
+    """One species, no junctions, K = 0: the _ms entry points give the
+    single-species results."""
+    case = _small(seed=17)
+    steps = 14
+    p = _problem(case)
+    loss = p.forward(steps, PARAMS["tau_c"], PARAMS["kappa"], case["rates"])
+    dt, dk, dr = p.gradient()
+    loss2 = p.forward_ms(steps, [PARAMS["tau_c"]], PARAMS["kappa"], 0.0, None, case["rates"])
+    g = p.gradient_ms()
+    # the loss and the gradients are sums over workgroups in atomic order:
+    # equal to the last bits
+    assert loss2 == pytest.approx(loss, rel=1e-14)
+    assert g["tau_c"][0] == pytest.approx(dt, rel=1e-12) and g["kappa"] == pytest.approx(dk, rel=1e-10)
+    assert g["k_junction"] == 0.0
+    np.testing.assert_allclose(g["rates"], dr, rtol=1e-12)
+    p.close()
+
+
+def test_multispecies_fp32_gradient_close_to_fp64():
+    case = _ms_case(seed=19, nx=60, ny=40)
+    steps = 60
+    res = {}
+    for prec in (capi.FP64, capi.FP32):
+        p = _ms_problem(case, precision=prec, K=20)
+        _ms_forward(p, case, steps)
+        res[prec] = p.gradient_ms()
+        p.close()
+    g64, g32 = res[capi.FP64], res[capi.FP32]
+    print("case multi-species fp32 vs fp64:", g32["tau_c"], g64["tau_c"], g32["k_junction"], g64["k_junction"],
+          flush=True)
+    # FP32 rounding of the states over 60 steps, relative to the largest entry
+    np.testing.assert_allclose(g32["tau_c"], g64["tau_c"], rtol=1e-3, atol=1e-3 * np.abs(g64["tau_c"]).max())
+    np.testing.assert_allclose(g32["K"], g64["K"], rtol=1e-3, atol=1e-3 * np.abs(g64["K"]).max())
+    np.testing.assert_allclose(g32["rates"], g64["rates"], rtol=1e-3)
+    assert g64["k_junction"] != 0.0
+    assert g32["k_junction"] == pytest.approx(g64["k_junction"], rel=1e-2)
