@@ -282,6 +282,46 @@ def measure_adjoint(steps, K, size=512):
             "steps": steps, "checkpoint_every": K, **out}
 
 
+def measure_adjoint_ms(size, steps, K, ncell=1600):
+    """Adjoint of the four-species junction workload: size^2 lattice, ncell
+    touching cells (r = 30 +- 5, 120 markers), junction springs between the
+    closest markers of different cells within 2 lu, four species with a
+    linear reaction matrix N(0, (1e-4)^2) and per-cell sources of species 0.
+    The target is the final concentration of every species under hidden
+    parameters; loss and gradient are evaluated at a guess.  FP32 arithmetic
+    and checkpoint storage."""
+    from lbibcell_amd import adjoint, capi, workloads
+    ns = 4
+    rings, dom, rates = workloads.tissue_adjoint_case(size, size, ncell, 30.0, 5.0, 120, seed=1)
+    pairs = workloads.junction_pairs(rings, JUNCTION["radius"])
+    Km = np.random.default_rng(101).normal(0.0, 1e-4, size=(ns, ns))
+    g0 = np.zeros((ns, 5, size, size))
+    decay = [1e-5] * ns
+    p = adjoint.AdjointProblem(size, size, 1.0, rings, dom, decay=decay[0], precision=capi.FP32, checkpoint_every=K)
+    p.set_species(g0, np.zeros((ns, size, size)), decay)
+    p.set_junctions(pairs, JUNCTION["l0"])
+    hidden = dict(tau_c=list(MS_TAUS), kappa=0.01, k_junction=JUNCTION["k"], K=Km, rates=rates)
+    guess = dict(tau_c=[t + 0.05 for t in MS_TAUS], kappa=0.006, k_junction=0.6 * JUNCTION["k"], K=0.7 * Km,
+                 rates=0.7 * rates)
+    _, target = p.forward_ms(steps, want_field=True, **hidden)
+    p.set_species(g0, target, decay)
+    del target
+    p.forward_ms(steps, **guess)          # warm-up of the timed pair
+    p.gradient_ms()
+    loss = p.forward_ms(steps, **guess)
+    g = p.gradient_ms()
+    fwd_ms, grad_ms = p.timing()
+    p.close()
+    return {"workload": f"adjoint {size}x{size}, {ncell} touching cells r=30+-5 (120 markers), {len(pairs)} junction "
+                        f"springs within {JUNCTION['radius']} lu, 4 species with a linear reaction matrix, {steps} "
+                        f"steps, checkpoints every {K}, FP32; loss = L2 mismatch of the final concentrations against "
+                        "a target from hidden parameters",
+            "steps": steps, "checkpoint_every": K, "markers": int(sum(len(r) for r in rings)),
+            "junction_springs": int(len(pairs)), "loss": loss, "forward_ms": fwd_ms, "gradient_ms": grad_ms,
+            "adjoint_to_forward": grad_ms / fwd_ms, "d_tau_c": list(g["tau_c"]), "d_kappa": g["kappa"],
+            "d_k_junction": g["k_junction"], "d_K": g["K"].tolist(), "d_rates_norm": float(np.linalg.norm(g["rates"]))}
+
+
 def l2_rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
 
@@ -305,6 +345,9 @@ def main():
     ap.add_argument("--batched-domains", type=int, default=256)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--adjoint-steps", type=int, default=1000, help="0 = skip the adjoint measurement")
+    ap.add_argument("--adjoint-ms-steps", type=int, default=200,
+                    help="steps of the four-species junction adjoint window (0 = skip)")
+    ap.add_argument("--adjoint-ms-size", type=int, default=4096)
     ap.add_argument("--multispecies", default="fp16,fp16r,fp64",
                     help="storage precisions of the 4096^2 four-species junction workload ('' = skip); "
                          "fp64 also serves as the reference of the mixed errors")
@@ -467,6 +510,10 @@ def main():
         log(f"adjoint: {args.adjoint_steps} steps ...")
         out["adjoint"] = measure_adjoint(args.adjoint_steps, 50)
         log(f"adjoint/forward time ratio (fp32): {out['adjoint']['fp32']['adjoint_to_forward']:.2f}")
+    if args.adjoint_ms_steps > 0:
+        log(f"four-species adjoint: {args.adjoint_ms_size}^2, {args.adjoint_ms_steps} steps ...")
+        out["adjoint_multispecies"] = measure_adjoint_ms(args.adjoint_ms_size, args.adjoint_ms_steps, 50)
+        log(f"four-species adjoint/forward time ratio: {out['adjoint_multispecies']['adjoint_to_forward']:.2f}")
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(nx, ny, nodes, conns)

@@ -404,6 +404,15 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
     for (int c = threadIdx.x; c < ng; c += blockDim.x) sgrad[c] = 0.0;
     __syncthreads();
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    // this site's gradient terms (zero past the lattice end), summed over the
+    // wavefront before one LDS atomic per slot: every lane adds to the same
+    // few slots, so per-lane atomics would serialise
+    double gt[NS], gK[NS * NS], grate = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) gt[s] = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS * NS; s++) gK[s] = 0.0;
+    uint32_t dmw = 0;
     if (i < a.n) {
         using Q = Cst<T>;
         const int x = (int)(i % a.nx), y = (int)(i / a.nx);
@@ -478,7 +487,7 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
             luox += C[j] / (T)2 * tauIc * (lg[j][DE] - lg[j][DW]);
             luoy += C[j] / (T)2 * tauIc * (lg[j][DN] - lg[j][DS]);
             // d g'_p / d tau_c = (g_p - geq_p) / tau_c^2
-            atomicAdd(&sgrad[j], (double)(gtau * tauIc * tauIc));
+            gt[j] = (double)(gtau * tauIc * tauIc);
         }
         // fluid: f'_q = f_q (1 - tauI) + feq_q(rho, u) tauI + 3 w_q rho (e_q . F), uo = u + F
         const int ex[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
@@ -510,14 +519,42 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
             for (int q = 0; q < 5; q++) a.lg0[(5 * s + q) * n + idx[q]] = lgh[s][q];
         a.lamF[2 * i] = lFx;
         a.lamF[2 * i + 1] = lFy;
-        if (dm != 0u && (int)dm <= a.ncell) atomicAdd(&sgrad[NS + dm], (double)P[0]);
+        if (dm != 0u && (int)dm <= a.ncell) {
+            dmw = dm;
+            grate = (double)P[0];
+        }
         if (a.hasK) {
 #pragma unroll
             for (int s = 0; s < NS; s++)
 #pragma unroll
-                for (int j = 0; j < NS; j++)
-                    atomicAdd(&sgrad[NS + a.ncell + 1 + s * NS + j], (double)(P[s] * C[j]));
+                for (int j = 0; j < NS; j++) gK[s * NS + j] = (double)(P[s] * C[j]);
         }
+    }
+    const int lane = (int)threadIdx.x & 63;
+    auto wsum = [](double v) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    };
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double v = wsum(gt[s]);
+        if (lane == 0) atomicAdd(&sgrad[s], v);
+    }
+    if (a.hasK) {
+#pragma unroll
+        for (int q = 0; q < NS * NS; q++) {
+            const double v = wsum(gK[q]);
+            if (lane == 0) atomicAdd(&sgrad[NS + a.ncell + 1 + q], v);
+        }
+    }
+    // per-cell rates: one sum when the whole wavefront lies in one cell
+    const uint32_t dm0 = (uint32_t)__shfl((int)dmw, 0, 64);
+    if (__all(dmw == dm0)) {
+        const double v = wsum(grate);
+        if (lane == 0 && dm0 != 0u) atomicAdd(&sgrad[NS + dm0], v);
+    } else if (dmw != 0u) {
+        atomicAdd(&sgrad[NS + dmw], grate);
     }
     __syncthreads();
     for (int c = threadIdx.x; c < ng; c += blockDim.x)
