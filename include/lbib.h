@@ -388,6 +388,14 @@ int lbib_adj_set_species(lbib_adj* adj, int nspecies, const double* g0, const do
  * opposite force on b (ForceType7, ForceStructs.hpp:521-588, as the
  * cell-junction BioSolver creates them); the pair list is fixed over the run. */
 int lbib_adj_set_junctions(lbib_adj* adj, int npairs, const int32_t* pairs, double l0);
+/* A batch of independent problems in one lattice: nrect periodic rectangles
+ * (rects: x0, y0, w, h each; origins on the 16-site tile grid, no 16x16 tile
+ * shared), each holding its own cells (markers at least 3 sites inside their
+ * rectangle).  Parameters are shared by the batch and the loss sums over it,
+ * so the gradient is the sum of the problems' gradients; sites outside every
+ * rectangle are not updated.  Call after lbib_adj_set_problem; nrect = 0
+ * returns to one periodic lattice. */
+int lbib_adj_set_batch_layout(lbib_adj* adj, int nrect, const int32_t* rects);
 /* Forward with every parameter: tau_c[nspecies], kappa, the junction
  * stiffness k_j, K (nspecies*nspecies row-major, NULL = 0), rates (n_cells + 1,
  * [0] unused); c_final (nspecies*n) optional. */

@@ -42,6 +42,7 @@ SIGNATURES = {
     "lbib_adj_set_junctions": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32), C.c_double]),
     "lbib_adj_forward_ms": (C.c_int, [_P, C.c_int, _D, C.c_double, C.c_double, _D, _D, _D, _D]),
     "lbib_adj_gradient_ms": (C.c_int, [_P, _D, _D, _D, _D, _D]),
+    "lbib_adj_set_batch_layout": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32)]),
 }
 
 W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
@@ -142,6 +143,14 @@ class AdjointProblem:
         self._pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
         self._chk(self.L.lbib_adj_set_junctions(self.h, self._pairs.size // 2,
                                                 self._pairs.ctypes.data_as(C.POINTER(C.c_int32)), float(l0)))
+
+    def set_batch_layout(self, rects):
+        """A batch of independent periodic problems in this lattice: rects =
+        [(x0, y0, w, h)] on the 16-site tile grid; shared parameters, summed
+        loss."""
+        self._rects = np.ascontiguousarray(rects, dtype=np.int32).reshape(-1)
+        self._chk(self.L.lbib_adj_set_batch_layout(self.h, self._rects.size // 4,
+                                                   self._rects.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def forward_ms(self, steps, tau_c, kappa, k_junction=0.0, K=None, rates=None, want_field=False):
         """Loss with every parameter: tau_c (ns), kappa, k_junction, K (ns, ns),

@@ -25,6 +25,9 @@
 // concentrations of all species; the parameters are tau_c per species,
 // kappa, k_j, the reaction matrix K and the per-cell rates s[c].
 //
+// A batch of independent problems may share the lattice as periodic
+// rectangles (shared parameters, summed loss).
+//
 // Forward states are checkpointed every K steps; the reverse sweep restores a
 // checkpoint, recomputes its segment storing every intermediate state, and
 // runs the adjoint kernels backwards over it (two-level revolve schedule: the
@@ -99,6 +102,27 @@ __device__ __forceinline__ long wrap_site(int x, int y, int nx, int ny) {
     x %= nx; if (x < 0) x += nx;
     y %= ny; if (y < 0) y += ny;
     return (long)y * nx + x;
+}
+
+// A batch of independent problems: periodic rectangles of one lattice (an
+// atlas).  rtile holds the rectangle of every 16x16 tile (-1: none); sites
+// outside every rectangle are not updated and carry no loss.  Null rtile:
+// one periodic lattice.
+struct AdjFrames {
+    const int32_t* rtile;
+    const int4* rects;   // x0, y0, w, h
+    int ntx;
+};
+// periodic frame of site (x, y); false when no rectangle holds the site
+__device__ __forceinline__ bool adj_frame(const AdjFrames& f, int x, int y, int nx, int ny, int4& fr) {
+    if (!f.rtile) {
+        fr = make_int4(0, 0, nx, ny);
+        return true;
+    }
+    const int r = f.rtile[(y >> 4) * f.ntx + (x >> 4)];
+    if (r < 0) return false;
+    fr = f.rects[r];
+    return x - fr.x < fr.z && y - fr.y < fr.w;
 }
 
 // ---------------------------------------------------------------- forward --
@@ -176,12 +200,14 @@ struct LatArgs {
     T tauIc[LBIB_MAX_SPECIES], decay[LBIB_MAX_SPECIES];
     T Km[LBIB_MAX_SPECIES * LBIB_MAX_SPECIES];   // reaction matrix, row-major (stride LBIB_MAX_SPECIES)
     int hasK;                                     // any nonzero entry
+    AdjFrames fr;
 };
 
-// pull indices of the 9 directions (natural order) at site (x, y)
-__device__ __forceinline__ void pull_index(int x, int y, int nx, int ny, long idx[9]) {
-    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
-    const int ym = y == 0 ? ny - 1 : y - 1, yp = y == ny - 1 ? 0 : y + 1;
+// pull indices of the 9 directions (natural order) at site (x, y), periodic
+// in the frame fr = (x0, y0, w, h) of a lattice nx wide
+__device__ __forceinline__ void pull_index(int x, int y, int nx, const int4 fr, long idx[9]) {
+    const int xm = x == fr.x ? fr.x + fr.z - 1 : x - 1, xp = x == fr.x + fr.z - 1 ? fr.x : x + 1;
+    const int ym = y == fr.y ? fr.y + fr.w - 1 : y - 1, yp = y == fr.y + fr.w - 1 ? fr.y : y + 1;
     const long r0 = (long)y * nx, rS = (long)ym * nx, rN = (long)yp * nx;
     idx[DT] = r0 + x;
     idx[DE] = r0 + xm;
@@ -276,8 +302,10 @@ __global__ void k_lattice(const LatArgs<T> a) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int x = (int)(i % a.nx), y = (int)(i / a.nx);
+    int4 fr;
+    if (!adj_frame(a.fr, x, y, a.nx, a.ny, fr)) return;
     long idx[9];
-    pull_index(x, y, a.nx, a.ny, idx);
+    pull_index(x, y, a.nx, fr, idx);
     const long n = a.n;
     T d[9], g[NS][5];
 #pragma unroll
@@ -324,10 +352,18 @@ __global__ void k_interp_move(int nm, T* X, const T* __restrict__ u, int nx, int
 // stored species populations; cfinal (optional) receives every C
 template <typename T>
 __global__ void k_loss(const T* __restrict__ g, const double* __restrict__ target, long n, int ns, T* lg,
-                       double* loss, T* cfinal) {
+                       double* loss, T* cfinal, const AdjFrames fa, int nx, int ny) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double r2 = 0.0;
+    int4 fr;
+    const bool live = i < n && adj_frame(fa, (int)(i % nx), (int)(i / nx), nx, ny, fr);
     for (int s = 0; s < ns && i < n; s++) {
+        if (!live) {   // outside every problem of a batch: no loss
+#pragma unroll
+            for (int q = 0; q < 5; q++) lg[(long)(5 * s + q) * n + i] = (T)0;
+            if (cfinal) cfinal[(long)s * n + i] = (T)0;
+            continue;
+        }
         const T* gs = g + (long)5 * s * n;
         T C = 0;
 #pragma unroll
@@ -393,6 +429,7 @@ struct BwdArgs {
     int hasK;
     int ncell;
     double* grad;  // [s] tau_c of species s, [ns + c] rate of cell c, [ns + ncell + 1 + s ns + j] K_sj
+    AdjFrames fr;
 };
 
 // Per-site adjoint of pull + fluid collide + species collides.
@@ -413,11 +450,12 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
 #pragma unroll
     for (int s = 0; s < NS * NS; s++) gK[s] = 0.0;
     uint32_t dmw = 0;
-    if (i < a.n) {
+    int4 fr;
+    if (i < a.n && adj_frame(a.fr, (int)(i % a.nx), (int)(i / a.nx), a.nx, a.ny, fr)) {
         using Q = Cst<T>;
         const int x = (int)(i % a.nx), y = (int)(i / a.nx);
         long idx[9];
-        pull_index(x, y, a.nx, a.ny, idx);
+        pull_index(x, y, a.nx, fr, idx);
         const long n = a.n;
         T d[9], g[NS][5];
 #pragma unroll
@@ -697,6 +735,11 @@ struct lbib_adj {
     // junction springs (lbib_adj_set_junctions): partners of each marker
     int32_t *jofs = nullptr, *jpart = nullptr;
     double kj = 0.0, l0 = 0.5;
+    // batch of problems (lbib_adj_set_batch_layout)
+    int32_t* rtile = nullptr;
+    int4* rects = nullptr;
+    int ntx = 0;
+    std::vector<double> xy_host;    // initial marker positions (layout checks)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double fwd_ms = 0.0, grad_ms = 0.0;
     bool have_problem = false, have_forward = false;
@@ -761,6 +804,7 @@ adj::LatArgs<T> lat_args(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, co
     l.fs = fs; l.gs = gs; l.fd = fd; l.gd = gd; l.Fs = Fs; l.u = u;
     l.dom = a->dom; l.src = a->src;
     l.tauI = (T)(1.0 / a->cfg.tau_fluid);
+    l.fr = adj::AdjFrames{a->rtile, a->rects, a->ntx};
     l.hasK = 0;
     for (int s = 0; s < LBIB_MAX_SPECIES; s++) {
         l.tauIc[s] = (T)(1.0 / a->tau_c[s]);
@@ -839,7 +883,8 @@ int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
         cf = tp<T>(a->cfin);
     }
     adj::k_loss<T><<<nb(a->n), 256, 0, a->stream>>>(tp<T>(a->g[cur]), a->target, a->n, a->ns, tp<T>(a->lg[0]),
-                                                     a->loss, cf);
+                                                     a->loss, cf, adj::AdjFrames{a->rtile, a->rects, a->ntx},
+                                                     a->cfg.nx, a->cfg.ny);
     ADJ_HIP(hipEventRecord(a->e1, a->stream));
     ADJ_HIP(hipEventSynchronize(a->e1));
     float ms = 0.0f;
@@ -918,6 +963,7 @@ int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_kj, double
             for (int q = 0; q < LBIB_MAX_SPECIES; q++) { b.tauIc[q] = l.tauIc[q]; b.decay[q] = l.decay[q]; }
             for (int q = 0; q < LBIB_MAX_SPECIES * LBIB_MAX_SPECIES; q++) b.Km[q] = l.Km[q];
             b.hasK = l.hasK;
+            b.fr = l.fr;
             b.ncell = a->ncell;
             b.grad = a->grad;
             launch_lattice_bwd<T>(a, b);
@@ -1016,7 +1062,7 @@ void lbib_adj_destroy(lbib_adj* a) {
                      &a->X, &a->Fm, &a->Fs, &a->u, &a->f0, &a->g0, &a->X0, &a->ck, &a->seg, &a->lX, &a->lFm,
                      &a->lU, &a->lF, &a->cfin};
     for (void** b : bufs) adj_free(*b);
-    void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss, a->jofs, a->jpart};
+    void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss, a->jofs, a->jpart, a->rtile, a->rects};
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (a->e0) (void)hipEventDestroy(a->e0);
@@ -1067,6 +1113,7 @@ int lbib_adj_set_problem(lbib_adj* a, const double* f0, const double* g0, const 
     }
     ADJ_HIP(hipMemcpy(a->dom, dom, n * 4, hipMemcpyHostToDevice));
     ADJ_HIP(hipMemcpy(a->target, target, n * 8, hipMemcpyHostToDevice));
+    a->xy_host.assign(xy0 ? xy0 : (const double*)nullptr, xy0 ? xy0 + 2 * nm : (const double*)nullptr);
     a->have_problem = true;
     a->have_forward = false;
     return 0;
@@ -1137,6 +1184,51 @@ int lbib_adj_set_junctions(lbib_adj* a, int npairs, const int32_t* pairs, double
         ADJ_HIP(hipMemcpy(a->jpart, part.data(), part.size() * 4, hipMemcpyHostToDevice));
     }
     a->l0 = l0;
+    a->have_forward = false;
+    return 0;
+}
+
+int lbib_adj_set_batch_layout(lbib_adj* a, int nrect, const int32_t* rects) {
+    if (!a || nrect < 0 || (nrect > 0 && !rects)) return -1;
+    if (!a->have_problem) { a->err = "lbib_adj_set_batch_layout: set the problem first"; return -1; }
+    const int nx = a->cfg.nx, ny = a->cfg.ny, ntx = (nx + 15) / 16, nty = (ny + 15) / 16;
+    std::vector<int32_t> owner((size_t)ntx * nty, -1);
+    for (int r = 0; r < nrect; r++) {
+        const int x0 = rects[4 * r], y0 = rects[4 * r + 1], w = rects[4 * r + 2], h = rects[4 * r + 3];
+        if (x0 < 0 || y0 < 0 || x0 % 16 || y0 % 16 || w < 8 || h < 8 || x0 + w > nx || y0 + h > ny) {
+            a->err = "lbib_adj_set_batch_layout: origins must be multiples of 16, sides >= 8, inside the lattice";
+            return -1;
+        }
+        for (int ty = y0 / 16; ty < (y0 + h + 15) / 16; ty++)
+            for (int tx = x0 / 16; tx < (x0 + w + 15) / 16; tx++) {
+                int32_t& o = owner[(size_t)ty * ntx + tx];
+                if (o >= 0) { a->err = "lbib_adj_set_batch_layout: rectangles share a 16x16 tile"; return -1; }
+                o = r;
+            }
+    }
+    // every marker's stencil and every cell stay inside one rectangle
+    for (size_t k = 0; 2 * k < a->xy_host.size() && nrect > 0; k++) {
+        const double X = a->xy_host[2 * k], Y = a->xy_host[2 * k + 1];
+        const int r = owner[(size_t)((int)Y / 16) * ntx + (int)X / 16];
+        const bool in = r >= 0 && X >= rects[4 * r] + 3.0 && X <= rects[4 * r] + rects[4 * r + 2] - 4.0 &&
+                        Y >= rects[4 * r + 1] + 3.0 && Y <= rects[4 * r + 1] + rects[4 * r + 3] - 4.0;
+        if (!in) {
+            a->err = "lbib_adj_set_batch_layout: markers must start at least 3 sites inside their rectangle";
+            return -1;
+        }
+    }
+    ADJ_HIP(hipStreamSynchronize(a->stream));
+    if (a->rtile) (void)hipFree(a->rtile);
+    if (a->rects) (void)hipFree(a->rects);
+    a->rtile = nullptr;
+    a->rects = nullptr;
+    a->ntx = ntx;
+    if (nrect > 0) {
+        if (adj_alloc(a, (void**)&a->rtile, owner.size() * 4) || adj_alloc(a, (void**)&a->rects, (size_t)nrect * 16))
+            return -1;
+        ADJ_HIP(hipMemcpy(a->rtile, owner.data(), owner.size() * 4, hipMemcpyHostToDevice));
+        ADJ_HIP(hipMemcpy(a->rects, rects, (size_t)nrect * 16, hipMemcpyHostToDevice));
+    }
     a->have_forward = false;
     return 0;
 }

@@ -345,3 +345,53 @@ def test_multispecies_fp32_gradient_close_to_fp64():
     np.testing.assert_allclose(g32["rates"], g64["rates"], rtol=1e-3)
     assert g64["k_junction"] != 0.0
     assert g32["k_junction"] == pytest.approx(g64["k_junction"], rel=1e-2)
+
+
+def test_batched_problems_give_the_sum_of_the_separate_gradients():
+    """Two independent problems in one lattice (periodic rectangles, shared
+    parameters): the loss and the gradient of every shared parameter are
+    the sums of the separate problems', the per-cell rate gradients their
+    concatenation; rows outside both rectangles carry no loss."""
+    cases = [_ms_case(seed=13, nx=60, ny=40), _ms_case(seed=19, nx=60, ny=40)]
+    origins = [(0, 0), (64, 0)]
+    NX, NY, ns, steps = 128, 48, 3, 16
+    prm = dict(tau_c=MS["tau_c"], kappa=MS["kappa"], k_junction=MS["k_junction"], K=cases[0]["K"])
+    sep = []
+    for c in cases:
+        p = _ms_problem(c, K=6)
+        loss = p.forward_ms(steps, rates=c["rates"], **prm)
+        sep.append((loss, p.gradient_ms()))
+        p.close()
+    f0 = np.tile(W[:, None, None], (1, NY, NX))
+    g0 = np.zeros((ns, 5, NY, NX))
+    tgt = np.zeros((ns, NY, NX))
+    dom = np.zeros((NY, NX), dtype=np.uint32)
+    rings, pairs, rates = [], [], []
+    ncell = nmark = 0
+    for c, (x0, y0) in zip(cases, origins):
+        win = (slice(y0, y0 + c["ny"]), slice(x0, x0 + c["nx"]))
+        f0[(slice(None),) + win] = c["f0"]
+        g0[(slice(None), slice(None)) + win] = c["g0"]
+        tgt[(slice(None),) + win] = c["target"]
+        dom[win] = np.where(c["dom"] > 0, c["dom"] + ncell, 0)
+        rings += [r + np.array([x0, y0]) for r in c["rings"]]
+        pairs.append(c["pairs"] + nmark)
+        rates += list(c["rates"])
+        ncell += len(c["rings"])
+        nmark += sum(len(r) for r in c["rings"])
+    p = adjoint.AdjointProblem(NX, NY, 0.9, rings, dom, f0=f0, g0=g0[0], decay=MS["decay"][0],
+                               precision=capi.FP64, checkpoint_every=6)
+    p.set_species(g0, tgt, MS["decay"][:ns])
+    p.set_junctions(np.concatenate(pairs), MS["l0"])
+    p.set_batch_layout([(x0, y0, c["nx"], c["ny"]) for c, (x0, y0) in zip(cases, origins)])
+    loss = p.forward_ms(steps, rates=np.array(rates), **prm)
+    g = p.gradient_ms()
+    p.close()
+    print(f"case batched adjoint: loss {loss:.15e} vs {sep[0][0] + sep[1][0]:.15e}", flush=True)
+    assert loss == pytest.approx(sep[0][0] + sep[1][0], rel=1e-12)
+    np.testing.assert_allclose(g["tau_c"], sep[0][1]["tau_c"] + sep[1][1]["tau_c"], rtol=1e-10)
+    assert g["kappa"] == pytest.approx(sep[0][1]["kappa"] + sep[1][1]["kappa"], rel=1e-10)
+    assert g["k_junction"] == pytest.approx(sep[0][1]["k_junction"] + sep[1][1]["k_junction"], rel=1e-10)
+    Ks = sep[0][1]["K"] + sep[1][1]["K"]
+    np.testing.assert_allclose(g["K"], Ks, rtol=1e-10, atol=1e-13 * np.abs(Ks).max())
+    np.testing.assert_allclose(g["rates"], np.concatenate([sep[0][1]["rates"], sep[1][1]["rates"]]), rtol=1e-10)
