@@ -120,6 +120,50 @@ def test_noflux_boundary_two_species_match_oracle(orc):
     sim.close()
 
 
+def test_reference_tutorial_geometry_matches_oracle(orc):
+    """The reference's own tutorial_01 geometry fixture (apps/config/vtk/
+    Cells_0.vtm, copied as data under tests/golden/vtk: 361 Float32 markers
+    on a ring of radius 10 about (200, 200), first and last coinciding, many
+    on lattice lines) loaded by Geometry::loadGeometryVTK on the host, run
+    with the tutorial_01 pipeline (SIGNAL species, membrane tension 0.01
+    renewed every 10 steps, box outlet) against the oracle fed the same
+    points: bit-identical, including the perturbation draws of the markers
+    that sit on lattice lines or cell diagonals."""
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vtk")
+    text = open(os.path.join(gold, "Cells_0", "Cells_0_0.vtp")).read()
+    i = text.index('Name="Points"')
+    pts = np.array(text[text.index(">", i) + 1: text.index("</DataArray>", i)].split(),
+                   dtype=np.float64).reshape(-1, 3)[:, :2].astype(np.float32).astype(np.float64)
+    nx = ny = 400
+    sig = dict(production=1e-4, decay=1e-5, switchoff=5000, init_inside=0.0, prod_domain=1)
+    print("case reference tutorial_01 geometry fixture 400x400", flush=True)
+    sim = runner.Simulation(nx, ny, 1.0)
+    o = orc.Oracle(nx, ny, 1.0)
+    sim.add_species("tutorial_01_CDESolverD2Q5_SIGNAL", capi.species_spec(capi.SPECIES_SIGNAL, 1.0, **sig))
+    o.add_species(orc.SP_SIGNAL, 1.0, [sig["production"], sig["decay"], sig["switchoff"], sig["init_inside"],
+                                      sig["prod_domain"]])
+    sim.load_geometry_vtk(os.path.join(gold, "Cells_0.vtm"))
+    m = len(pts)
+    for k in range(m):
+        o.add_node(k, pts[k, 0], pts[k, 1])
+    for k in range(m):
+        o.add_connection(k, (k + 1) % m, 1, [])
+    sim.setup()
+    o.cure()
+    sim.init_solvers()
+    o.init_solvers()
+    sim.add_mass_solver("MassSolverBoxOutlet")
+    o.box_outlet(True)
+    sim.add_tension(0.01, 10)
+    o.add_bio(0, [0.01], 10)
+    sim.run(60)
+    o.run(60)
+    _compare(sim, o, 1, "tutorial_01 fixture")
+    print("perturbations oracle", o.perturb_counts(), "device", sim.perturb_counts(), flush=True)
+    sim.close()
+
+
 @pytest.mark.parametrize("window", [1, 5, 16])
 def test_run_ahead_resolves_stops_like_the_synchronous_loop(orc, window):
     """Iterations enqueued ahead of the status read: a stop (here a remesh that
