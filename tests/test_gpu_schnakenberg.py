@@ -127,3 +127,59 @@ def test_schnakenberg_partner_must_be_registered():
     with pytest.raises(capi.LbibError, match="tutorial_02_CDESolverD2Q5_L"):
         sim.setup()
     sim.close()
+
+
+def test_reference_tutorial02_geometry_matches_oracle(orc):
+    """The reference's tutorial_02 geometry (apps/config/tutorial_02_geometry.txt,
+    copied as data under tests/golden/txt: 361 markers on a ring about
+    (200, 200), every connection carrying BoundarySolverNoFluxD2Q5 for
+    tutorial_02_CDESolverD2Q5_R) read by Geometry::loadGeometry's TXT
+    format, with the Schnakenberg pair, tension and the box outlet, against
+    the oracle fed the same nodes and descriptors: bit-identical."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "txt", "tutorial_02_geometry.txt")
+    nodes, conns, mode = [], [], None
+    for line in open(path):
+        if line.startswith("#Nodes"):
+            mode = "n"
+            continue
+        if line.startswith("#Connection"):
+            mode = "c"
+            continue
+        t = line.split()
+        if not t:
+            continue
+        if mode == "n":
+            nodes.append((int(t[0]), float(t[1]), float(t[2])))
+        else:
+            conns.append((int(t[0]), int(t[1]), int(t[2]), t[3:]))
+    assert len(nodes) == 361 and len(conns) == 361
+    nx = ny = 400
+    print("case reference tutorial_02 geometry fixture 400x400", flush=True)
+    sim = runner.Simulation(nx, ny, 1.0)
+    o = orc.Oracle(nx, ny, 1.0)
+    for name in NAMES:
+        sim.add_species(name, tau=0.9)          # the registry's constants
+    _oracle_pair(orc, o)
+    sim.load_geometry_txt(path)
+    for nid, x, y in nodes:
+        o.add_node(nid, x, y)
+    for a, b, d, desc in conns:
+        # species in name order: L = 0, R = 1; the descriptors name R only
+        o.add_connection(a, b, d, [1] if "tutorial_02_CDESolverD2Q5_R" in desc else [])
+    sim.setup()
+    o.cure()
+    sim.init_solvers()
+    o.init_solvers()
+    ctx = sim.context()
+    ctx.ns = 2
+    for k in range(2):
+        o.set_g(k, ctx.get_species(k))
+    sim.add_mass_solver("MassSolverBoxOutlet")
+    o.box_outlet(True)
+    sim.add_tension(0.01, 10)
+    o.add_bio(0, [0.01], 10)
+    sim.run(40)
+    o.run(40)
+    _compare(sim, o, 2, "tutorial_02 fixture")
+    sim.close()
