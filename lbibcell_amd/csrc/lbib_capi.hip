@@ -84,11 +84,11 @@ int ensure_markers(lbib_ctx* c, int nm) {
     for (auto a : darr) dev_free(*a);
     for (auto a : uarr) dev_free(*a);
     for (auto a : iarr) dev_free(*a);
-    dev_free(c->tslot); dev_free(c->tlist);
+    dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
     for (auto a : darr) if (dev_alloc(c, a, cap)) return -1;
     for (auto a : uarr) if (dev_alloc(c, a, cap)) return -1;
     for (auto a : iarr) if (dev_alloc(c, a, cap + 1)) return -1;
-    if (dev_alloc(c, &c->tslot, cap) || dev_alloc(c, &c->tlist, cap)) return -1;
+    if (dev_alloc(c, &c->tslot, cap) || dev_alloc(c, &c->tlist, cap) || dev_alloc(c, &c->tpos, cap)) return -1;
     c->mcap = cap;
     if (ensure_scan(c)) return -1;
     return ensure_junctions(c);
@@ -281,9 +281,9 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     lbib_ctx* c = new (std::nothrow) lbib_ctx();
     if (!c) return -1;
     c->cfg = *cfg;
-    if (cfg->nx < 4 || cfg->ny < 4 || cfg->nspecies < 0 || cfg->nspecies > LBIB_MAX_SPECIES ||
-        !(cfg->tau_fluid > 0.5)) {
-        c->err = "invalid configuration (nx, ny >= 4, 0 <= nspecies <= 4, tau_fluid > 0.5)";
+    if (cfg->nx < 4 || cfg->ny < 4 || cfg->nx > 65535 || cfg->ny > 65535 || cfg->nspecies < 0 ||
+        cfg->nspecies > LBIB_MAX_SPECIES || !(cfg->tau_fluid > 0.5)) {
+        c->err = "invalid configuration (4 <= nx, ny <= 65535, 0 <= nspecies <= 4, tau_fluid > 0.5)";
         *out = c;
         return -1;
     }
@@ -395,7 +395,7 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
     dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc); dev_free(c->vsegs);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
-    dev_free(c->tslot); dev_free(c->tlist);
+    dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
                          &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
     uint32_t** uarr[18] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,

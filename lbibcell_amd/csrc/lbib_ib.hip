@@ -133,12 +133,15 @@ __global__ void k_tile_count(int ntx, int nty, const double* __restrict__ mx, co
 
 __global__ void k_tile_scatter(int ntx, const double* __restrict__ mx, const double* __restrict__ my,
                                const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tslot, int32_t* tlist,
+                               uint32_t* tpos,
                                const DevStatus* st) {
     if (step_stopped(st)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (int)st->nm) return;
     const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
-    tlist[toff[(fy / ST) * ntx + fx / ST] + tslot[k]] = k;
+    const uint32_t e = toff[(fy / ST) * ntx + fx / ST] + tslot[k];
+    tlist[e] = k;
+    tpos[e] = (uint32_t)fx | ((uint32_t)fy << 16);   // markers lie inside the lattice: 0 <= fx, fy < 65536
 }
 
 // One workgroup per tile; tiles no marker stencil reaches return at once.
@@ -152,7 +155,8 @@ __global__ void k_tile_scatter(int ntx, const double* __restrict__ mx, const dou
 template <bool F32OUT>
 __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
                                                      const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
-                                                     const int32_t* __restrict__ tlist, const double* __restrict__ mx,
+                                                     const int32_t* __restrict__ tlist,
+                                                     const uint32_t* __restrict__ tpos, const double* __restrict__ mx,
                                                      const double* __restrict__ my, const double* __restrict__ mfx,
                                                      const double* __restrict__ mfy, double* Fx, double* Fy,
                                                      DevStatus* st) {
@@ -189,9 +193,12 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
     for (uint32_t j = threadIdx.x; j < hpre[9]; j += blockDim.x) {
         int h = 0;
         while (j >= hpre[h + 1]) h++;
-        const int k = tlist[hoff[h] + (j - hpre[h])];
-        // keep only markers whose 4x4 stencil overlaps this tile
-        const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
+        const uint32_t e = hoff[h] + (j - hpre[h]);
+        const int k = tlist[e];
+        // keep only markers whose 4x4 stencil overlaps this tile (the binning
+        // stored floor(X), floor(Y) with the list entry)
+        const uint32_t pp = tpos[e];
+        const int fx = (int)(pp & 0xffffu), fy = (int)(pp >> 16);
         if (fx + 2 < bx || fx - 1 > bx + ST - 1 || fy + 2 < by || fy - 1 > by + ST - 1) continue;
         const uint32_t p = atomicAdd(&n, 1u);
         if (p < (uint32_t)SP_IDX) idx[p] = k;
@@ -351,14 +358,14 @@ int launch_force_and_spread(lbib_ctx* c) {
     k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
                                                              c->tact, c->st);
     if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;
-    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist,
+    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist, c->tpos,
                                                                c->st);
     if (c->prec == LBIB_FP64)
         k_tile_spread<false><<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                            c->tlist, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
+                                                            c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
     else
         k_tile_spread<true><<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                           c->tlist, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
+                                                           c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
@@ -375,7 +382,7 @@ int launch_tile_bins(lbib_ctx* c) {
     k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
                                                              c->tact, c->st);
     if (scan_exclusive(c, c->tcnt, c->toff, nullptr, nt, nt, nullptr)) return -1;
-    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist,
+    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist, c->tpos,
                                                                c->st);
     LBIB_HIP(hipGetLastError());
     return 0;

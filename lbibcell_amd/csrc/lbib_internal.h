@@ -164,6 +164,7 @@ struct lbib_ctx {
     uint32_t *tcnt = nullptr, *toff = nullptr, *tslot = nullptr;
     uint8_t* tact = nullptr;
     int32_t* tlist = nullptr;
+    uint32_t* tpos = nullptr;          // floor(X) | floor(Y) << 16 of each tlist entry
     // markers
     int mcap = 0, nm = 0;
     double *mx = nullptr, *my = nullptr, *mvx = nullptr, *mvy = nullptr;
