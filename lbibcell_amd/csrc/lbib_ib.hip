@@ -163,8 +163,7 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
     static_assert(SP_CH == 128, "two 64-bit site masks per staged chunk");
     const int t = (int)xcd_contiguous(blockIdx.x, gridDim.x);
     if (!tact[t] || step_stopped(st)) return;
-    __shared__ int32_t idx[SP_IDX];
-    __shared__ int32_t sorted[SP_IDX];
+    __shared__ int32_t idx[SP_IDX];   // gathered, then in rank order
     __shared__ double fxa[SP_CH][4], fya[SP_CH][4], sfx[SP_CH], sfy[SP_CH];
     __shared__ int32_t s0x[SP_CH], s0y[SP_CH];
     __shared__ unsigned long long smask[2][ST * ST];
@@ -209,12 +208,24 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
         if (threadIdx.x == 0) atomicOr(&st->bits, LBIB_ERR_CAPACITY);
         return;
     }
-    for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) {
-        const int32_t v = idx[j];
+    // rank sort in place: ranks into registers, then one scatter (the
+    // workgroup is 256 wide, so each lane ranks at most SP_IDX / 256 markers)
+    constexpr int RPL = SP_IDX / 256;
+    int32_t rv[RPL];
+    uint32_t rr[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; q++) {
+        const uint32_t j = threadIdx.x + 256u * q;
+        rv[q] = j < m ? idx[j] : 0;
         uint32_t r = 0;
-        for (uint32_t i = 0; i < m; i++) r += (idx[i] < v) ? 1u : 0u;
-        sorted[r] = v;
+        if (j < m)
+            for (uint32_t i = 0; i < m; i++) r += (idx[i] < rv[q]) ? 1u : 0u;
+        rr[q] = r;
     }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RPL; q++)
+        if (threadIdx.x + 256u * q < m) idx[rr[q]] = rv[q];
     __syncthreads();
     const int lx = (int)(threadIdx.x % ST), ly = (int)(threadIdx.x / ST);
     const int px = bx + lx, py = by + ly;
@@ -222,7 +233,7 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
     for (uint32_t c0 = 0; c0 < m; c0 += SP_CH) {
         const uint32_t mc = min((uint32_t)SP_CH, m - c0);
         for (uint32_t j = threadIdx.x; j < mc; j += blockDim.x) {
-            const int k = sorted[c0 + j];
+            const int k = idx[c0 + j];
             const double X = mx[k], Y = my[k];
             const int x0 = (int)floor(X) - 1, y0 = (int)floor(Y) - 1;
             s0x[j] = x0;
