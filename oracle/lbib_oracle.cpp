@@ -23,22 +23,31 @@
 //
 // Parity status (see the tests named in brackets):
 //  * D2Q9 collide/stream, Luo force  -- pinned: equilibrium fixed point, mass and
-//    momentum balance, shear-wave decay, swap==pull permutation, Guo terms vs.
-//    the textbook formula [tests/test_oracle_lattice.py]
+//    momentum balance, shear-wave decay, textbook BGK + Luo step with pull
+//    streaming [tests/test_oracle_lattice.py]
+//  * Guo force, as written and fixed -- pinned: Guo, Zheng & Shi's source term
+//    with the stored velocity, NE coefficient difference [test_oracle_pins.py]
 //  * D2Q5 species                     -- pinned: mass balance, Gaussian spreading
-//    with D=(tau-1/2)/3 against the reference's DiracD2Q5 closed form
+//    with D=(tau-1/2)/3 against the reference's DiracD2Q5 closed form; SIGNAL
+//    source/decay balances; LINEAR species as the (I+K) matrix map on a
+//    uniform field; the Schnakenberg explicit-map recurrence
+//  * no-flux D2Q5 boundary            -- pinned: hand-derived incoming values,
+//    first species only and the swap into every listed species [pins]
+//  * CDE re-initialisation            -- pinned: hand-derived means (direct,
+//    diagonal, SIGNAL fallback, NaN of the plain solver) then collide [pins]
 //  * fastCos / delta                 -- pinned: table vs (1+cos)/4 within the
 //    approximation error, partition of unity, evenness defect
 //  * spread / interpolate             -- pinned: adjointness <S F,u> = <F,I u>
 //  * cut links + scanline             -- pinned: brute-force point-in-polygon on
-//    small grids, count inside a circle ~ pi R^2
+//    small grids, count inside a circle ~ pi R^2; closest-wins tie goes to the
+//    earlier connection; boundary-node velocity linear along the segment [pins]
 //  * shoelace area                    -- pinned: regular n-gon closed form;
 //    parity against boost::geometry::area itself is UNPINNED (Boost absent)
 //  * tension force type 6             -- pinned: regular n-gon net inward force
-//  * mt19937(1) + uniform_real        -- pinned: [rand.predef] KAT 4123659995
-//  * reinit-on-domain-change, closest-wins ties, perturbation draw order,
-//    remesh/coarsen interleaving      -- parity unpinned beyond line-by-line
-//    review (LbmLib has no tests and the full reference needs Boost+VTK)
+//  * mt19937(1) + uniform_real        -- pinned: [rand.predef] KAT 4123659995 and
+//    an independent MT19937; node and connection draw order [pins]
+//  * remesh / coarsen                 -- pinned: hand-built rings with known ids,
+//    midpoints, removal sequence and connection order [pins]
 
 #include <algorithm>
 #include <array>
@@ -136,6 +145,7 @@ struct BNode {                        // BoundaryNode.cpp:35-69
     std::vector<int> bc_species;      // boundary solvers built from the descriptor (BoundaryNode.cpp:43-46)
     double dist = 0;                  // BC scratch (BoundarySolverNoFluxD2Q5 distribution_)
     bool alive = true;
+    int conn = -1;                    // index of the generating connection in Sim::conns
 };
 
 struct ForceStruct {                  // ForceStructs.hpp:49-588
@@ -316,7 +326,7 @@ static void insertBN(Sim& s, const Conn& c, BNode nb, long site, int d) {
     }
 }
 // Connection::generateBoundaryNodes, Connection.cpp:224-324
-static void generateBN(Sim& s, const Conn& c) {
+static void generateBN(Sim& s, const Conn& c, int ci) {
     const double x1 = c.a->x, y1 = c.a->y, x2 = c.b->x, y2 = c.b->y;
     const double maxX = std::max(x1, x2), maxY = std::max(y1, y2);
     const double minX = std::min(x1, x2), minY = std::min(y1, y2);
@@ -327,6 +337,7 @@ static void generateBN(Sim& s, const Conn& c) {
     for (int i = (int)std::ceil(minX); i <= std::floor(maxX); i++) {
         const double y = m * i + n;
         BNode bN{(double)i, y, 0}, bS{(double)i, y, 0};
+        bN.conn = bS.conn = ci;
         if (x1 < x2) { bN.domain = c.domain; bS.domain = 0; }
         else         { bN.domain = 0; bS.domain = c.domain; }
         const long sS = s.idx(i, (long)std::floor(y));
@@ -337,6 +348,7 @@ static void generateBN(Sim& s, const Conn& c) {
     for (int i = (int)std::ceil(minY); i <= std::floor(maxY); i++) {
         const double x = (i - n) / m;
         BNode bW{x, (double)i, 0}, bE{x, (double)i, 0};
+        bW.conn = bE.conn = ci;
         if (y1 > y2) { bW.domain = 0; bE.domain = c.domain; }
         else         { bW.domain = c.domain; bE.domain = 0; }
         const long sW = s.idx((long)std::floor(x), i);
@@ -348,7 +360,8 @@ static void generateBN(Sim& s, const Conn& c) {
 static void regenerateBoundaryNodes(Sim& s) {
     for (auto& a : s.bn) a = {-1, -1, -1, -1, -1};
     s.bnodes.clear();
-    for (auto& c : s.conns) generateBN(s, *c);   // GeometryHandler.cpp:646-659 (serial)
+    for (size_t k = 0; k < s.conns.size(); k++)   // GeometryHandler.cpp:646-659 (serial)
+        generateBN(s, *s.conns[k], (int)k);
 }
 
 // ------------------------------------------------------------ fluid solver --
@@ -1467,5 +1480,54 @@ int orc_reinit_count(void* p) { return (int)static_cast<Sim*>(p)->reinit_log.siz
 int orc_rect_within(double n, double s_, double w, double e, double x, double y) {
     return (x >= w) && (x < e) && (y <= n) && (y > s_);
 }
+
+// --- single stages, for the hand-derived tests ------------------------------
+// Alive boundary nodes in creation order: site, direction to the physical
+// neighbour, domain, generating connection index, and x, y, vx, vy.
+int orc_num_bnodes(void* p) {
+    int n = 0;
+    for (auto& b : static_cast<Sim*>(p)->bnodes) n += b.alive ? 1 : 0;
+    return n;
+}
+void orc_get_bnodes(void* p, int* site, int* dir, unsigned* dom, int* conn, double* xyv) {
+    Sim* s = static_cast<Sim*>(p);
+    size_t k = 0;
+    for (auto& b : s->bnodes) {
+        if (!b.alive) continue;
+        site[k] = (int)b.site; dir[k] = b.dir; dom[k] = b.domain; conn[k] = b.conn;
+        xyv[4 * k] = b.x; xyv[4 * k + 1] = b.y; xyv[4 * k + 2] = b.vx; xyv[4 * k + 3] = b.vy;
+        k++;
+    }
+}
+// LagrangianPoint::velocity_ of every marker (id order), as collectVelocity sets it.
+void orc_set_marker_velocities(void* p, const double* v) {
+    Sim* s = static_cast<Sim*>(p);
+    size_t k = 0;
+    for (auto& kv : s->markers) { kv.second->vx = v[2 * k]; kv.second->vy = v[2 * k + 1]; k++; }
+}
+// preAdvect + advect + postAdvect only: afterwards slot q of every site holds
+// the population arriving in direction q (read with orc_get_g_slots).
+void orc_stream_bc(void* p) {
+    Sim* s = static_cast<Sim*>(p);
+    preAdvect(*s); advect(*s); postAdvect(*s);
+}
+// Stored slots as they are: out[q*n + i] = slot q of site i.
+void orc_get_g_slots(void* p, int k, double* out) {
+    Sim* s = static_cast<Sim*>(p);
+    const size_t n = s->f.size();
+    for (size_t i = 0; i < n; i++) for (int q = 0; q < 5; q++) out[q * n + i] = s->g[k][i][q];
+}
+// A boundary node of domain `dom` in slot `slot` (E,N,W,S) of a site, without
+// a connection (for CDE re-initialisation cases).
+void orc_mark_boundary(void* p, long site, int slot, unsigned dom) {
+    Sim* s = static_cast<Sim*>(p);
+    BNode b{(double)(site % s->nx), (double)(site / s->nx), dom};
+    b.site = site; b.dir = INV[slot];
+    s->bnodes.push_back(b);
+    s->bn[site][slot] = (long)s->bnodes.size() - 1;
+}
+void orc_reinit_site(void* p, int k, long site) { speciesReinit(*static_cast<Sim*>(p), k, site); }
+int orc_remesh(void* p) { return (int)remeshBoundary(*static_cast<Sim*>(p)); }
+void orc_coarsen(void* p) { coarsenBoundary(*static_cast<Sim*>(p)); }
 
 }  // extern "C"

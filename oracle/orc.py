@@ -73,6 +73,15 @@ _SIG = {
     "orc_mt19937_default_10000": (C.c_uint, []),
     "orc_reinit_count": (C.c_int, [_P]),
     "orc_rect_within": (C.c_int, [C.c_double] * 6),
+    "orc_num_bnodes": (C.c_int, [_P]),
+    "orc_get_bnodes": (None, [_P, _I32, _I32, _U32, _I32, _D]),
+    "orc_set_marker_velocities": (None, [_P, _D]),
+    "orc_stream_bc": (None, [_P]),
+    "orc_get_g_slots": (None, [_P, C.c_int, _D]),
+    "orc_mark_boundary": (None, [_P, C.c_long, C.c_int, C.c_uint]),
+    "orc_reinit_site": (None, [_P, C.c_int, C.c_long]),
+    "orc_remesh": (C.c_int, [_P]),
+    "orc_coarsen": (None, [_P]),
 }
 
 _lib = None
@@ -298,6 +307,44 @@ class Oracle:
 
     def reinit_count(self):
         return self.L.orc_reinit_count(self.h)
+
+    # single stages for the hand-derived tests
+    def bnodes(self):
+        """Alive boundary nodes: site, dir (to the physical neighbour), domain,
+        generating connection index, (x, y, vx, vy)."""
+        k = self.L.orc_num_bnodes(self.h)
+        site = np.empty(max(k, 1), dtype=np.int32)
+        d = np.empty(max(k, 1), dtype=np.int32)
+        dom = np.empty(max(k, 1), dtype=np.uint32)
+        conn = np.empty(max(k, 1), dtype=np.int32)
+        xyv = np.empty((max(k, 1), 4))
+        self.L.orc_get_bnodes(self.h, site.ctypes.data_as(_I32), d.ctypes.data_as(_I32), dom.ctypes.data_as(_U32),
+                              conn.ctypes.data_as(_I32), _d(xyv))
+        return site[:k], d[:k], dom[:k], conn[:k], xyv[:k]
+
+    def set_marker_velocities(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self.L.orc_set_marker_velocities(self.h, _d(v))
+
+    def stream_bc(self):
+        self.L.orc_stream_bc(self.h)
+
+    def get_g_slots(self, k):
+        g = np.empty((5, self.ny, self.nx))
+        self.L.orc_get_g_slots(self.h, k, _d(g))
+        return g
+
+    def mark_boundary(self, x, y, slot, dom):
+        self.L.orc_mark_boundary(self.h, y * self.nx + x, slot, dom)
+
+    def reinit_site(self, k, x, y):
+        self.L.orc_reinit_site(self.h, k, y * self.nx + x)
+
+    def remesh(self):
+        return self.L.orc_remesh(self.h)
+
+    def coarsen(self):
+        self.L.orc_coarsen(self.h)
 
 
 def fastcos(x):
