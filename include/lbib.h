@@ -154,7 +154,13 @@ int lbib_set_mass_field(lbib_ctx* ctx, const double* m);
 
 /* ---- membrane (GeometryNode) and connections ------------------------------
  * Markers are dense indices 0..nm-1 in increasing GeometryNode id order (the
- * std::map order of Geometry::geometryNodes_).  xy, vxy, oxy are interleaved. */
+ * std::map order of Geometry::geometryNodes_).  xy, vxy, oxy are interleaved.
+ * lbib_set_markers rejects (status -1, LBIB_ERR_EDGE with first_bad_marker in
+ * the status word) any position that is not finite or lies within 2 sites of
+ * the lattice edge, or of its rectangle's edge in a batched atlas: the check
+ * GeometryHandler::connectGeometryNodesToPhysicalNodes makes for every new
+ * node (GeometryHandler.cpp:613-636).  After a change of the marker count,
+ * upload ids, connections and forces for the new count. */
 int lbib_set_markers(lbib_ctx* ctx, int nm, const double* xy, const double* vxy, const double* oxy);
 int lbib_get_markers(lbib_ctx* ctx, double* xy, double* vxy, double* fxy, double* oxy);
 /* Connections in Geometry::connections_ vector order: first/second marker
@@ -165,7 +171,8 @@ int lbib_set_connections(lbib_ctx* ctx, int nc, const int32_t* a, const int32_t*
  * BoundarySolverNoFluxD2Q5 is connected to (BoundaryNode.cpp:43-46). */
 int lbib_set_bc_descriptors(lbib_ctx* ctx, int nd, const int32_t* off, const int32_t* species);
 /* Capacity for markers and connections (device topology grows by remeshing).
- * Reallocation discards the device topology: upload it again afterwards. */
+ * Growth keeps the uploaded topology (positions, ids, links, connections and
+ * the force offsets). */
 int lbib_reserve(lbib_ctx* ctx, int max_markers, int max_connections);
 /* GeometryNode ids per dense marker index (needed once remeshing has added
  * markers on the device) and the current connection list. */
@@ -278,6 +285,13 @@ int lbib_forces_junctions(lbib_ctx* ctx, double radius, double k, double l0);
 int lbib_forces_clear(lbib_ctx* ctx);
 /* FluidSolver::addMass for every site of each domain: m_site = table[dom]. */
 int lbib_add_mass_by_domain(lbib_ctx* ctx, int ndom, const double* table);
+/* FluidSolver::addMass (FluidSolver.cpp:253-257) at count listed sites
+ * (y * nx + x): each call overwrites the site's mass for the next collide;
+ * sites not listed keep what earlier calls of this iteration gave them (a
+ * per-domain table from lbib_add_mass_by_domain or a device BioSolver op, or
+ * zero).  A site listed twice takes its last value.  This is what a host-side
+ * BioSolver's getPhysicalNodes()[y][x]->getFluidSolver().addMass(m) reaches. */
+int lbib_add_mass_sites(lbib_ctx* ctx, int64_t count, const int64_t* site, const double* m);
 
 /* ---- batched ensemble -------------------------------------------------------
  * No single counterpart in the reference: a parameter sweep there is one
@@ -390,10 +404,13 @@ int lbib_adj_set_species(lbib_adj* adj, int nspecies, const double* g0, const do
 int lbib_adj_set_junctions(lbib_adj* adj, int npairs, const int32_t* pairs, double l0);
 /* A batch of independent problems in one lattice: nrect periodic rectangles
  * (rects: x0, y0, w, h each; origins on the 16-site tile grid, no 16x16 tile
- * shared), each holding its own cells (markers at least 3 sites inside their
- * rectangle).  Parameters are shared by the batch and the loss sums over it,
- * so the gradient is the sum of the problems' gradients; sites outside every
- * rectangle are not updated.  Call after lbib_adj_set_problem; nrect = 0
+ * shared), each holding its own cells (every marker inside its rectangle).
+ * Parameters are shared by the batch and the loss sums over it, so the
+ * gradient is the sum of the problems' gradients; sites outside every
+ * rectangle are not updated.  Each marker keeps the rectangle it starts in,
+ * and its delta stencil wraps inside that rectangle, as the lattice wraps each
+ * problem, so a marker drifting to its rectangle's edge never touches another
+ * problem.  Call after lbib_adj_set_problem (which drops the layout); nrect = 0
  * returns to one periodic lattice. */
 int lbib_adj_set_batch_layout(lbib_adj* adj, int nrect, const int32_t* rects);
 /* Forward with every parameter: tau_c[nspecies], kappa, the junction

@@ -154,6 +154,7 @@ SIGNATURES = {
     "lbib_compute_areas": (C.c_int, [_P, _D]),
     "lbib_area_regulator": (C.c_int, [_P, C.c_double, C.c_double, C.c_double, C.c_double]),
     "lbib_add_mass_by_domain": (C.c_int, [_P, C.c_int, _D]),
+    "lbib_add_mass_sites": (C.c_int, [_P, C.c_int64, C.POINTER(C.c_int64), _D]),
     "lbib_profile_enable": (C.c_int, [_P, C.c_int]),
     "lbib_profile_read": (C.c_int, [_P, _D, C.POINTER(C.c_int64), _D]),
     "lbib_profile_copy": (C.c_int, [_P, C.c_int64, C.c_int, _D]),

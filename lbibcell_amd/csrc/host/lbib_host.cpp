@@ -1,5 +1,6 @@
 // lbib_host.cpp -- host runtime (see lbib_host.hpp).
 #include "lbib_host.hpp"
+#include "lbib_compat.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -44,6 +45,14 @@ void GlobalSimulationParameters::loadGlobalSimulationParameters(const std::strin
             throw Exception("An unknown parameter was passed in the global Parameter file");
         }
     }
+}
+// GlobalSimulationParameters::printParameters: the same fields as the file
+void GlobalSimulationParameters::printParameters() const {
+    std::cout << "CurrentIterations:\t" << currentIteration << "\nIterations:\t" << iterations << "\nReportStep:\t"
+              << reportSteps << "\nSizeX:\t" << sizeX << "\nSizeY:\t" << sizeY << "\ntauFluid:\t" << tauFluid
+              << "\nCDESolvers:";
+    for (auto& kv : cdeSolvers) std::cout << '\t' << kv.first << '\t' << kv.second;
+    std::cout << std::endl;
 }
 void GlobalSimulationParameters::writeGlobalSimulationParameters(const std::string& file) const {
     std::ofstream out(file);
@@ -654,6 +663,7 @@ SimulationRunner::SimulationRunner(const GlobalSimulationParameters& params, geo
 }
 
 SimulationRunner::~SimulationRunner() {
+    if (gh_ && gh_->runner_ == this) gh_->runner_ = nullptr;
     if (ctx_) lbib_destroy(ctx_);
 }
 
@@ -712,6 +722,7 @@ void SimulationRunner::setup() {
 
 void SimulationRunner::initSolvers(const double* u0) {
     check(lbib_init_solvers(ctx_, u0), "initSolvers");
+    gen_++;
     std::vector<uint32_t> dom;
     std::vector<double> g;
     const size_t n = (size_t)params_.sizeX * params_.sizeY;
@@ -793,6 +804,7 @@ void SimulationRunner::writeSimulation(const std::string& forceFile, const std::
 }
 
 void SimulationRunner::initSolvers(const std::string& solverFile) {
+    gen_++;
     std::ifstream in(solverFile);
     if (!in.is_open()) LBIB_FAIL("Cannot find the force file.");
     const int nx = (int)params_.sizeX, ny = (int)params_.sizeY;
@@ -838,6 +850,14 @@ void SimulationRunner::addMassSolver(const std::string& name) {
 }
 
 void SimulationRunner::addBioSolver(const std::string& name) {
+    // a BioSolver registered through BioBaseSolver<T> (BioSolverFactory)
+    if (gh_) {
+        auto user = solver::BioSolverFactory::create(name);
+        if (user) {
+            addBioSolver(std::move(user));
+            return;
+        }
+    }
     std::unique_ptr<BioSolver> b;
     if (name == "tutorial_01_BioSolverMembraneTension") b.reset(new MembraneTension(0.01));
     else if (name == "tutorial_02_BioSolverMembraneTension") b.reset(new MembraneTension(0.005));
@@ -999,6 +1019,7 @@ void SimulationRunner::renewJunctions(double radius, double k, double l0) {
 
 void SimulationRunner::uploadTopology() {
     settle();
+    gen_++;
     const auto& nodes = geo_.getGeometryNodes();
     const int nm = (int)nodes.size();
     ids_.clear();
@@ -1206,6 +1227,7 @@ constexpr uint32_t kStopBits = LBIB_ERR_MASK | LBIB_HALT_MASK | LBIB_FLAG_REMESH
 // deleteAllForces every 10th iteration, BioSolvers.
 void SimulationRunner::enqueueStep(unsigned it) {
     params_.currentIteration = it;
+    gen_++;
     auto t0 = std::chrono::steady_clock::now();
     check(lbib_step_begin(ctx_, it), "step begin");
     pending_++;
@@ -1253,6 +1275,7 @@ void SimulationRunner::settle() {
 // does, and drop host-side edits made while enqueueing the skipped iterations.
 void SimulationRunner::resolveStop(unsigned it) {
     check(lbib_rewind(ctx_, it + 1), "rewind");
+    gen_++;
     pending_ = 0;
     times.stops++;
     params_.currentIteration = it;
@@ -1424,6 +1447,17 @@ void SimulationRunner::divideCells(double maxCellSize, bool longestAxis, std::mt
     copyCellTypeToPhysicalNodes();
 }
 
+void SimulationRunner::cureLattice() {
+    syncMirror();
+    geo_.perturbConnections();
+    uploadTopology();
+    check(lbib_cure_lattice(ctx_), "cureLattice");
+    check(lbib_read_status(ctx_, &lastStatus), "status");
+    if (lastStatus.bits & (LBIB_ERR_MASK | LBIB_HALT_MASK)) throw Exception("cureLattice: " + describe(lastStatus));
+    check(lbib_clear_status(ctx_), "status");
+    gen_++;
+}
+
 void SimulationRunner::applyBioSolvers() {
     settle();
     for (auto& b : bio_) b->applyBioProcess(*this);
@@ -1441,6 +1475,7 @@ void SimulationRunner::setGraphSteps(unsigned steps) {
 // mode is off from now on) and the caller enqueues it step by step.
 bool SimulationRunner::runGraphBlock(unsigned it0) {
     const unsigned B = graphSteps_;
+    gen_++;
     for (auto& b : bio_) {
         const unsigned p = b->graphPeriod();
         if (p == 0 || B % p != 0) {
@@ -1516,17 +1551,42 @@ void SimulationRunner::run(unsigned steps) {
             resolveStop(s.it);
         }
     }
+    if (gh_) Parameters.currentIteration = params_.currentIteration;
 }
 
-// SimulationRunner::runSimulation (SimulationRunner.cpp:69-110): on failure
+// SimulationRunner::runSimulation (SimulationRunner.cpp:69-110): Parameters'
+// Iterations steps; after step t (1, 2, ...) the reports are written when t is
+// a multiple of the report step.  On failure the reports are written for t,
 // the state is dumped to output/CRASH_*_<t>.txt and the message printed.
 void SimulationRunner::runSimulation() {
+    const unsigned start = params_.currentIteration;
+    const bool reports = report_ && report_->hasReporters();
+    const unsigned rs = std::max(1u, params_.reportSteps);
     try {
-        run(params_.iterations);
+        unsigned t = 0;
+        while (t < params_.iterations) {
+            // run to the next report (or the end) without reading the status
+            unsigned next = params_.iterations;
+            if (reports) next = std::min(next, (t / rs + 1) * rs);
+            run(next - t);
+            t = next;
+            if (reports && t % rs == 0) {
+                settle();
+                syncMirror();
+                report_->writeReport(t);
+            }
+        }
     } catch (const std::exception& exp) {
-        const std::string t = std::to_string(params_.currentIteration);
-        writeSimulation("output/CRASH_forceOut_" + t + ".txt", "output/CRASH_geoOut_" + t + ".txt",
-                        "output/CRASH_parameterOut_" + t + ".txt", "output/CRASH_solverOut_" + t + ".txt");
+        const unsigned t = params_.currentIteration - start + 1;
+        if (report_) {
+            try {
+                report_->writeCrashReport(t);
+            } catch (const std::exception&) {
+            }
+        }
+        const std::string ts = std::to_string(t);
+        writeSimulation("output/CRASH_forceOut_" + ts + ".txt", "output/CRASH_geoOut_" + ts + ".txt",
+                        "output/CRASH_parameterOut_" + ts + ".txt", "output/CRASH_solverOut_" + ts + ".txt");
         std::cout << exp.what() << std::endl;
     }
 }

@@ -40,7 +40,36 @@ struct GlobalSimulationParameters {
     std::map<std::string, double> cdeSolvers;   // name -> tau (std::map: name order)
     void loadGlobalSimulationParameters(const std::string& file);   // :34-72
     void writeGlobalSimulationParameters(const std::string& file) const;
+    // the reference's accessors (GlobalSimulationParameters.hpp:58-150)
+    unsigned getCurrentIteration() const { return currentIteration; }
+    unsigned getIterations() const { return iterations; }
+    unsigned getReportSteps() const { return reportSteps; }
+    void setReportSteps(unsigned r) { reportSteps = r; }
+    unsigned getSizeX() const { return sizeX; }
+    unsigned getSizeY() const { return sizeY; }
+    double getTauFluid() const { return tauFluid; }
+    const std::map<std::string, double>& getCdeSolvers() const { return cdeSolvers; }
+    void printParameters() const;
+    GlobalSimulationParameters& operator++() {
+        ++currentIteration;
+        return *this;
+    }
+    GlobalSimulationParameters operator++(int) {
+        GlobalSimulationParameters old = *this;
+        ++currentIteration;
+        return old;
+    }
 };
+
+namespace geometry {
+class GeometryHandler;
+}
+namespace reportHandler {
+class ReportHandler;
+}
+namespace solver {
+class BioAbstractSolver;
+}
 
 namespace geometry {
 
@@ -171,6 +200,14 @@ class SimulationRunner {
  public:
     SimulationRunner(const GlobalSimulationParameters& params, geometry::Geometry& geo, int precision = LBIB_FP64,
                      int forceModel = LBIB_FORCE_LUO);
+    // SimulationRunner(const GeometryHandler&, const ReportHandler&)
+    // (SimulationRunner.hpp:51, SimulationRunner.cpp:50-64): the global
+    // Parameters, the handler's geometry and its CDE solvers by name; the
+    // lattice is set up at once (the GeometryHandler ctor's cureLattice,
+    // GeometryHandler.cpp:51-56) and the handler serves its host view from it.
+    // Defined in lbib_compat.cpp.
+    SimulationRunner(const geometry::GeometryHandler& geometryHandler, const reportHandler::ReportHandler& reports,
+                     int precision = LBIB_FP64);
     ~SimulationRunner();
     // CDE solver registry (CDESolverFactory): name -> reaction spec; called
     // before the context is created, in Parameters::cdeSolvers order.
@@ -195,6 +232,9 @@ class SimulationRunner {
     void addMassSolver(const std::string& name);          // :401-403
     void addBioSolver(const std::string& name);           // :405-408
     void addBioSolver(std::unique_ptr<BioSolver> b);
+    // a BioSolver written against BioAbstractSolver::applyBioProcess(
+    // GeometryHandler&, ForceSolver&); needs the GeometryHandler ctor
+    void addBioSolver(std::unique_ptr<solver::BioAbstractSolver> b);
     void runSimulation();                                 // :69-110 (Parameters.iterations steps)
     void run(unsigned steps);
     void step();
@@ -249,6 +289,17 @@ class SimulationRunner {
     std::map<unsigned, double> computeAreas();            // GeometryHandler.cpp:101-134 (device shoelace)
     // GeometryHandler.cpp:136-147 (device per-domain sums, domains present on the lattice)
     std::map<unsigned, double> computeAccumulatedDomainConcentrations(const std::string& name);
+    // GeometryHandler::cureLattice (GeometryHandler.cpp:149-170) after host
+    // edits of the topology: upload, boundary nodes, scanline
+    void cureLattice();
+    // bumped whenever device state may have changed (a step enqueued or
+    // resolved, an upload); host views re-read what they cache
+    uint64_t stateGeneration() const { return gen_; }
+    // waits for the enqueued iterations (a stop is resolved by run())
+    void settleDevice() { settle(); }
+    geometry::GeometryHandler* geometryHandler() { return gh_; }
+    void detachGeometryHandler();
+    const std::vector<std::string>& speciesNames() const { return speciesNames_; }
     StageTimes times;
     lbib_status lastStatus{};
     // Iterations enqueued before the device status is read (1 reads it after
@@ -306,6 +357,9 @@ class SimulationRunner {
     std::mt19937 genInit_{1};
     std::vector<lbib_rect> rects_;
     unsigned graphSteps_ = 0;
+    uint64_t gen_ = 0;
+    geometry::GeometryHandler* gh_ = nullptr;
+    const reportHandler::ReportHandler* report_ = nullptr;
 };
 
 namespace vtkxml {
@@ -314,6 +368,9 @@ void load_geometry(const std::string& file, geometry::Geometry& geo);
 void write_image(const std::string& prefix, unsigned time, int nx, int ny, int coarse,
                  const std::vector<std::pair<std::string, std::pair<int, std::vector<double>>>>& fields);
 void write_cells(const std::string& prefix, unsigned time, const geometry::Geometry& geo,
+                 const std::map<unsigned, unsigned>& cellTypes);
+void write_cells(const std::string& prefix, unsigned time,
+                 const std::vector<std::shared_ptr<geometry::Connection>>& connections,
                  const std::map<unsigned, unsigned>& cellTypes);
 }  // namespace vtkxml
 

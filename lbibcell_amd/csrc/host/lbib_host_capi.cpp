@@ -189,6 +189,16 @@ int lbh_run(void* p, unsigned steps) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] { R(h).run(steps); });
 }
+// SimulationRunner::runSimulation (SimulationRunner.cpp:69-110) for `iterations`
+// steps: a failure is caught, the reports and output/CRASH_*_<t>.txt files
+// written (relative to the working directory) and the message printed
+int lbh_run_simulation(void* p, unsigned iterations) {
+    auto* h = static_cast<Handle*>(p);
+    return guarded(h, [&] {
+        R(h).parameters().iterations = iterations;
+        R(h).runSimulation();
+    });
+}
 int lbh_sync(void* p) {
     auto* h = static_cast<Handle*>(p);
     return guarded(h, [&] {

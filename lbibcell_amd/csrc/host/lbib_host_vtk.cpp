@@ -250,12 +250,18 @@ void write_image(const std::string& prefix, unsigned time, int nx, int ny, int c
 
 void write_cells(const std::string& prefix, unsigned time, const geometry::Geometry& geo,
                  const std::map<unsigned, unsigned>& cellTypes) {
+    write_cells(prefix, time, geo.getConnections(), cellTypes);
+}
+
+void write_cells(const std::string& prefix, unsigned time,
+                 const std::vector<std::shared_ptr<geometry::Connection>>& connections,
+                 const std::map<unsigned, unsigned>& cellTypes) {
     std::string vtm;
     const std::string rel = next_block(prefix, time, "vtp", vtm);
     // vtkCellReporter.cpp:77-120: one polygon per domain, walked from the
     // domain's first connection (Geometry::connections_ order)
     std::map<unsigned, const geometry::Connection*> first;
-    for (auto& c : geo.getConnections())
+    for (auto& c : connections)
         if (!first.count(c->domain)) first[c->domain] = c.get();
     std::vector<double> pts, dom, ct, desc;
     std::vector<double> conn, offs;

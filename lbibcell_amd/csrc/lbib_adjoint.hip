@@ -125,6 +125,30 @@ __device__ __forceinline__ bool adj_frame(const AdjFrames& f, int x, int y, int 
     return x - fr.x < fr.z && y - fr.y < fr.w;
 }
 
+// Stencil sites of marker k: in a batch each marker wraps inside its own
+// rectangle (mrect[k]), as the lattice kernels wrap each problem's pulls, so a
+// stencil near a rectangle edge never reaches the gap or another problem.
+struct MarkerFrames {
+    const int32_t* mrect;   // rectangle per marker, null: one periodic lattice
+    const int4* rects;
+};
+struct MarkerWrap {
+    int x0, y0, w, h, nx;
+    __device__ MarkerWrap(const MarkerFrames& mf, int k, int nx_, int ny_) : nx(nx_) {
+        if (mf.mrect) {
+            const int4 r = mf.rects[mf.mrect[k]];
+            x0 = r.x; y0 = r.y; w = r.z; h = r.w;
+        } else {
+            x0 = 0; y0 = 0; w = nx_; h = ny_;
+        }
+    }
+    __device__ long site(int x, int y) const {
+        int xr = (x - x0) % w; if (xr < 0) xr += w;
+        int yr = (y - y0) % h; if (yr < 0) yr += h;
+        return (long)(y0 + yr) * nx + (x0 + xr);
+    }
+};
+
 // ---------------------------------------------------------------- forward --
 // Junction springs of marker k: partners jpart[jofs[k] .. jofs[k + 1]) (a pair
 // is listed at both ends, so every force is gathered, never scattered).
@@ -165,9 +189,11 @@ __global__ void k_force(int nm, const T* __restrict__ X, const int32_t* __restri
 }
 
 template <typename T>
-__global__ void k_spread(int nm, const T* __restrict__ X, const T* __restrict__ Fm, T* Fs, int nx, int ny) {
+__global__ void k_spread(int nm, const T* __restrict__ X, const T* __restrict__ Fm, T* Fs, int nx, int ny,
+                         const MarkerFrames mf) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nm) return;
+    const MarkerWrap wr(mf, k, nx, ny);
     int sx, sy;
     T hx[4], hy[4], dhx[4], dhy[4];
     stencil<T>(X[2 * k], sx, hx, dhx);
@@ -178,7 +204,7 @@ __global__ void k_spread(int nm, const T* __restrict__ X, const T* __restrict__ 
 #pragma unroll
         for (int qy = 0; qy < 4; qy++) {
             const T d = (hx[qx] * hy[qy]) / (T)16;
-            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            const long s = wr.site(sx + qx, sy + qy);
             atomicAdd(&Fs[2 * s], d * fx);
             atomicAdd(&Fs[2 * s + 1], d * fy);
         }
@@ -327,9 +353,10 @@ __global__ void k_lattice(const LatArgs<T> a) {
 }
 
 template <typename T>
-__global__ void k_interp_move(int nm, T* X, const T* __restrict__ u, int nx, int ny) {
+__global__ void k_interp_move(int nm, T* X, const T* __restrict__ u, int nx, int ny, const MarkerFrames mf) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nm) return;
+    const MarkerWrap wr(mf, k, nx, ny);
     int sx, sy;
     T hx[4], hy[4], dhx[4], dhy[4];
     stencil<T>(X[2 * k], sx, hx, dhx);
@@ -340,7 +367,7 @@ __global__ void k_interp_move(int nm, T* X, const T* __restrict__ u, int nx, int
 #pragma unroll
         for (int qy = 0; qy < 4; qy++) {
             const T d = (hx[qx] * hy[qy]) / (T)16;
-            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            const long s = wr.site(sx + qx, sy + qy);
             vx += d * u[2 * s];
             vy += d * u[2 * s + 1];
         }
@@ -383,9 +410,10 @@ __global__ void k_loss(const T* __restrict__ g, const double* __restrict__ targe
 // derivative of the weights and spreads lambda_U onto lamU (sites).
 template <typename T>
 __global__ void k_interp_bwd(int nm, const T* __restrict__ X, const T* __restrict__ u, T* lamX, T* lamU, int nx,
-                             int ny) {
+                             int ny, const MarkerFrames mf) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nm) return;
+    const MarkerWrap wr(mf, k, nx, ny);
     int sx, sy;
     T hx[4], hy[4], dhx[4], dhy[4];
     stencil<T>(X[2 * k], sx, hx, dhx);
@@ -396,7 +424,7 @@ __global__ void k_interp_bwd(int nm, const T* __restrict__ X, const T* __restric
     for (int qx = 0; qx < 4; qx++)
 #pragma unroll
         for (int qy = 0; qy < 4; qy++) {
-            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            const long s = wr.site(sx + qx, sy + qy);
             const T ul = u[2 * s] * lx + u[2 * s + 1] * ly;
             gx += (dhx[qx] * hy[qy]) / (T)16 * ul;
             gy += (hx[qx] * dhy[qy]) / (T)16 * ul;
@@ -602,9 +630,10 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
 // spread adjoint: lambda of the marker force and the weight derivative term
 template <typename T>
 __global__ void k_spread_bwd(int nm, const T* __restrict__ X, const T* __restrict__ Fm, const T* __restrict__ lamF,
-                             T* lamFm, T* lamX, int nx, int ny) {
+                             T* lamFm, T* lamX, int nx, int ny, const MarkerFrames mf) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nm) return;
+    const MarkerWrap wr(mf, k, nx, ny);
     int sx, sy;
     T hx[4], hy[4], dhx[4], dhy[4];
     stencil<T>(X[2 * k], sx, hx, dhx);
@@ -615,7 +644,7 @@ __global__ void k_spread_bwd(int nm, const T* __restrict__ X, const T* __restric
     for (int qx = 0; qx < 4; qx++)
 #pragma unroll
         for (int qy = 0; qy < 4; qy++) {
-            const long s = wrap_site(sx + qx, sy + qy, nx, ny);
+            const long s = wr.site(sx + qx, sy + qy);
             const T d = (hx[qx] * hy[qy]) / (T)16;
             const T Lx = lamF[2 * s], Ly = lamF[2 * s + 1];
             lx += d * Lx;
@@ -738,6 +767,7 @@ struct lbib_adj {
     // batch of problems (lbib_adj_set_batch_layout)
     int32_t* rtile = nullptr;
     int4* rects = nullptr;
+    int32_t* mrect = nullptr;   // rectangle of every marker (batches)
     int ntx = 0;
     std::vector<double> xy_host;    // initial marker positions (layout checks)
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -767,6 +797,7 @@ void adj_free(void*& p) {
     p = nullptr;
 }
 inline int nb(long count) { return (int)((count + 255) / 256); }
+adj::MarkerFrames mframes(const lbib_adj* a) { return adj::MarkerFrames{a->rtile ? a->mrect : nullptr, a->rects}; }
 
 template <typename T>
 T* tp(void* p) { return static_cast<T*>(p); }
@@ -841,10 +872,10 @@ int step_fwd(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, T* X, T* Fs, T
     if (a->nm > 0) {
         adj::k_force<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, a->nxt, a->prv, (T)a->kappa, Fm,
                                                           junctions<T>(a));
-        adj::k_spread<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, Fm, Fs, nx, ny);
+        adj::k_spread<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, Fm, Fs, nx, ny, mframes(a));
     }
     launch_lattice<T>(a, lat_args<T>(a, fs, gs, fd, gd, Fs, u));
-    if (a->nm > 0) adj::k_interp_move<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, u, nx, ny);
+    if (a->nm > 0) adj::k_interp_move<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, u, nx, ny, mframes(a));
     ADJ_HIP(hipGetLastError());
     return 0;
 }
@@ -949,7 +980,7 @@ int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_kj, double
             Extra<T> e = extra_slot<T>(a, j);
             if (a->nm > 0)
                 adj::k_interp_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, e.u, tp<T>(a->lX), tp<T>(a->lU),
-                                                                         nx, ny);
+                                                                         nx, ny, mframes(a));
             adj::BwdArgs<T> b;
             b.nx = nx; b.ny = ny; b.n = a->n;
             b.fs = s.f; b.gs = s.g; b.Fs = e.Fs;
@@ -969,7 +1000,7 @@ int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_kj, double
             launch_lattice_bwd<T>(a, b);
             if (a->nm > 0) {
                 adj::k_spread_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, e.Fm, tp<T>(a->lF), tp<T>(a->lFm),
-                                                                         tp<T>(a->lX), nx, ny);
+                                                                         tp<T>(a->lX), nx, ny, mframes(a));
                 adj::k_force_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, a->nxt, a->prv, (T)a->kappa,
                                                                         tp<T>(a->lFm), tp<T>(a->lX), a->grad + gl - 2,
                                                                         junctions<T>(a), a->grad + gl - 1);
@@ -1062,7 +1093,7 @@ void lbib_adj_destroy(lbib_adj* a) {
                      &a->X, &a->Fm, &a->Fs, &a->u, &a->f0, &a->g0, &a->X0, &a->ck, &a->seg, &a->lX, &a->lFm,
                      &a->lU, &a->lF, &a->cfin};
     for (void** b : bufs) adj_free(*b);
-    void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss, a->jofs, a->jpart, a->rtile, a->rects};
+    void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss, a->jofs, a->jpart, a->rtile, a->rects, a->mrect};
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (a->e0) (void)hipEventDestroy(a->e0);
@@ -1095,13 +1126,22 @@ int lbib_adj_set_problem(lbib_adj* a, const double* f0, const double* g0, const 
             a->err = "lbib_adj_set_problem: cell map entry above n_cells";
             return -1;
         }
-    // markers stay where the periodic stencil does not straddle the edge
+    // markers start on the periodic lattice (their stencils wrap; a ring must
+    // not straddle the periodic edge, since the tension uses plain differences)
     for (size_t k = 0; k < nm; k++)
-        if (!(xy0[2 * k] >= 3.0 && xy0[2 * k] <= a->cfg.nx - 4.0 && xy0[2 * k + 1] >= 3.0 &&
-              xy0[2 * k + 1] <= a->cfg.ny - 4.0)) {
-            a->err = "lbib_adj_set_problem: markers must start at least 3 sites inside the lattice";
+        if (!(xy0[2 * k] >= 0.0 && xy0[2 * k] < a->cfg.nx && xy0[2 * k + 1] >= 0.0 && xy0[2 * k + 1] < a->cfg.ny)) {
+            a->err = "lbib_adj_set_problem: markers must start inside the lattice";
             return -1;
         }
+    // a new problem drops the batch layout (its per-marker rectangles refer to
+    // the old positions): set it again afterwards
+    ADJ_HIP(hipStreamSynchronize(a->stream));
+    for (int32_t** q : {&a->rtile, &a->mrect}) {
+        if (*q) (void)hipFree(*q);
+        *q = nullptr;
+    }
+    if (a->rects) (void)hipFree(a->rects);
+    a->rects = nullptr;
     const bool f64 = a->prec == LBIB_FP64;
     if ((f64 ? upload_T<double>(a, a->f0, f0, 9 * n) : upload_T<float>(a, a->f0, f0, 9 * n)) ||
         (f64 ? upload_T<double>(a, a->g0, g0, 5 * n) : upload_T<float>(a, a->g0, g0, 5 * n)))
@@ -1206,28 +1246,36 @@ int lbib_adj_set_batch_layout(lbib_adj* a, int nrect, const int32_t* rects) {
                 o = r;
             }
     }
-    // every marker's stencil and every cell stay inside one rectangle
+    // every marker starts inside one rectangle, which it keeps: its stencil
+    // wraps inside that rectangle from then on
+    std::vector<int32_t> mrect(a->xy_host.size() / 2, 0);
     for (size_t k = 0; 2 * k < a->xy_host.size() && nrect > 0; k++) {
         const double X = a->xy_host[2 * k], Y = a->xy_host[2 * k + 1];
-        const int r = owner[(size_t)((int)Y / 16) * ntx + (int)X / 16];
-        const bool in = r >= 0 && X >= rects[4 * r] + 3.0 && X <= rects[4 * r] + rects[4 * r + 2] - 4.0 &&
-                        Y >= rects[4 * r + 1] + 3.0 && Y <= rects[4 * r + 1] + rects[4 * r + 3] - 4.0;
+        const int r = (X >= 0 && Y >= 0 && X < nx && Y < ny) ? owner[(size_t)((int)Y / 16) * ntx + (int)X / 16] : -1;
+        mrect[k] = r;
+        const bool in = r >= 0 && X >= rects[4 * r] && X < rects[4 * r] + rects[4 * r + 2] &&
+                        Y >= rects[4 * r + 1] && Y < rects[4 * r + 1] + rects[4 * r + 3];
         if (!in) {
-            a->err = "lbib_adj_set_batch_layout: markers must start at least 3 sites inside their rectangle";
+            a->err = "lbib_adj_set_batch_layout: every marker must start inside a rectangle";
             return -1;
         }
     }
     ADJ_HIP(hipStreamSynchronize(a->stream));
     if (a->rtile) (void)hipFree(a->rtile);
     if (a->rects) (void)hipFree(a->rects);
+    if (a->mrect) (void)hipFree(a->mrect);
     a->rtile = nullptr;
     a->rects = nullptr;
+    a->mrect = nullptr;
     a->ntx = ntx;
     if (nrect > 0) {
-        if (adj_alloc(a, (void**)&a->rtile, owner.size() * 4) || adj_alloc(a, (void**)&a->rects, (size_t)nrect * 16))
+        if (adj_alloc(a, (void**)&a->rtile, owner.size() * 4) || adj_alloc(a, (void**)&a->rects, (size_t)nrect * 16) ||
+            adj_alloc(a, (void**)&a->mrect, std::max<size_t>(mrect.size(), 1) * 4))
             return -1;
         ADJ_HIP(hipMemcpy(a->rtile, owner.data(), owner.size() * 4, hipMemcpyHostToDevice));
         ADJ_HIP(hipMemcpy(a->rects, rects, (size_t)nrect * 16, hipMemcpyHostToDevice));
+        if (!mrect.empty())
+            ADJ_HIP(hipMemcpy(a->mrect, mrect.data(), mrect.size() * 4, hipMemcpyHostToDevice));
     }
     a->have_forward = false;
     return 0;

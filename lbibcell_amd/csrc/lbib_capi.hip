@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <unordered_map>
 #include <vector>
 #include "lbib_internal.h"
 
@@ -70,45 +71,62 @@ int ensure_scan(lbib_ctx* c) {
 int ensure_junctions(lbib_ctx* c);
 
 // Marker-indexed arrays (positions, links, spread work lists, coarsening
-// scratch, force CSR offsets).  Reallocation drops the contents: callers
-// re-upload the topology afterwards.
+// scratch, force CSR offsets).  Growth keeps every entry below the old
+// capacity, so ids, links and the force offsets survive lbib_reserve and a
+// set_markers call that crosses the capacity.
+template <typename T>
+static int regrow(lbib_ctx* c, T** p, size_t old_count, size_t count) {
+    T* q = nullptr;
+    if (dev_alloc(c, &q, count)) return -1;
+    if (*p && old_count) {
+        if (hipMemcpyAsync(q, *p, old_count * sizeof(T), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            c->err = "copy of marker arrays failed";
+            dev_free(q);
+            return -1;
+        }
+    }
+    dev_free(*p);
+    *p = q;
+    return 0;
+}
 int ensure_markers(lbib_ctx* c, int nm) {
     if (nm + 1 <= c->mcap) return 0;
     int cap = std::max(nm + 1, c->mcap * 2);
     cap = std::max(cap, 1024);
+    const size_t old = (size_t)c->mcap;
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
                          &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
     uint32_t** uarr[9] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
                           &c->pos_r, &c->mdom};
     int32_t** iarr[6] = {&c->min_c, &c->mout_c, &c->rec_a, &c->rec_b, &c->rec_bc, &c->fs_off};
-    for (auto a : darr) dev_free(*a);
-    for (auto a : uarr) dev_free(*a);
-    for (auto a : iarr) dev_free(*a);
+    for (auto a : darr) if (regrow(c, a, old, cap)) return -1;
+    for (auto a : uarr) if (regrow(c, a, old, cap)) return -1;
+    for (auto a : iarr) if (regrow(c, a, old ? old + 1 : 0, cap + 1)) return -1;
+    // per-step work lists: rebuilt every step
     dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
-    for (auto a : darr) if (dev_alloc(c, a, cap)) return -1;
-    for (auto a : uarr) if (dev_alloc(c, a, cap)) return -1;
-    for (auto a : iarr) if (dev_alloc(c, a, cap + 1)) return -1;
     if (dev_alloc(c, &c->tslot, cap) || dev_alloc(c, &c->tlist, cap) || dev_alloc(c, &c->tpos, cap)) return -1;
     c->mcap = cap;
     if (ensure_scan(c)) return -1;
     return ensure_junctions(c);
 }
 
+// Connection-indexed arrays and the cut-link candidate list; growth keeps the
+// entries below the old capacity (the connections and this step's candidates).
 int ensure_connections(lbib_ctx* c, int nc) {
     if (nc <= c->ccap && c->cand) return 0;
     int cap = std::max(nc, c->ccap * 2);
     cap = std::max(cap, 1024);
+    const size_t old = c->cand ? (size_t)c->ccap : 0;
     int32_t** iarr[8] = {&c->ca, &c->cb, &c->cbc, &c->ca2, &c->cb2, &c->cbc2, &c->ring_bucket, &c->ring_marker};
     uint32_t** uarr[8] = {&c->cdom, &c->cdom2, &c->flag_c, &c->rank_c, &c->cdead, &c->keep_c, &c->pos_c, &c->ring_slot};
-    for (auto a : iarr) dev_free(*a);
-    for (auto a : uarr) dev_free(*a);
-    dev_free(c->cand);
-    for (auto a : iarr) if (dev_alloc(c, a, cap)) return -1;
-    for (auto a : uarr) if (dev_alloc(c, a, cap)) return -1;
+    for (auto a : iarr) if (regrow(c, a, old, cap)) return -1;
+    for (auto a : uarr) if (regrow(c, a, old, cap)) return -1;
     // a connection of length L crosses at most L+1 lines of each family; keep
     // room for 8 candidates per connection plus slack for long first segments.
-    c->candcap = cap * 8 + 65536;
-    if (dev_alloc(c, &c->cand, c->candcap)) return -1;
+    const int candcap = cap * 8 + 65536;
+    if (regrow(c, &c->cand, c->cand ? (size_t)c->candcap : 0, candcap)) return -1;
+    c->candcap = candcap;
     c->ccap = cap;
     return ensure_scan(c);
 }
@@ -155,18 +173,7 @@ int download(lbib_ctx* c, void* dst, const void* src, size_t bytes);
 
 // ensure_connections that keeps the previous cure's candidate list (the next
 // stage B clears those boundary-node bits) valid across the reallocation
-int grow_connections(lbib_ctx* c, int nc) {
-    std::vector<Cand> keep;
-    DevStatus s;
-    if (download(c, &s, c->st, sizeof(s))) return -1;
-    keep.resize(s.ncand);
-    if (s.ncand && download(c, keep.data(), c->cand, s.ncand * sizeof(Cand))) return -1;
-    if (ensure_connections(c, nc)) return -1;
-    if ((int)keep.size() > c->candcap) { c->err = "candidate list too long"; return -1; }
-    if (!keep.empty() && upload(c, c->cand, keep.data(), keep.size() * sizeof(Cand))) return -1;
-    LBIB_HIP(hipStreamSynchronize(c->stream));
-    return 0;
-}
+int grow_connections(lbib_ctx* c, int nc) { return ensure_connections(c, nc); }
 
 // Force structs: growth keeps the table's contents (a BioSolver may add to
 // structs written earlier in the same iteration).
@@ -393,6 +400,7 @@ void lbib_destroy(lbib_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
+    dev_free(c->mass_site); dev_free(c->mass_val);
     dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc); dev_free(c->vsegs);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
     dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
@@ -562,8 +570,54 @@ int lbib_set_mass_field(lbib_ctx* c, const double* m) {
 }
 
 // ---------------------------------------------------------------- markers --
+// GeometryHandler::connectGeometryNodesToPhysicalNodes (GeometryHandler.cpp:
+// 613-636) fails for a node whose 4x4 stencil leaves the lattice, and every
+// node is connected when it is created.  The same bound, 2 <= X <= nx - 2 and
+// 2 <= Y <= ny - 2 (inside the owning rectangle in a batched atlas), is
+// required of every uploaded position before anything is launched: the spread
+// tiles and the interpolation index the lattice without wrapping.
+static int check_marker_edges(lbib_ctx* c, int nm, const double* xy) {
+    std::vector<int32_t> tmap;
+    const int stx = (c->nx + 15) / 16, sty = (c->ny + 15) / 16;
+    if (c->nrect > 0) {
+        tmap.assign((size_t)stx * sty, -1);
+        for (int r = 0; r < c->nrect; r++) {
+            const int32_t* q = &c->rect_host[4 * r];
+            for (int ty = q[1] / 16; ty < (q[1] + q[3] + 15) / 16 && ty < sty; ty++)
+                for (int tx = q[0] / 16; tx < (q[0] + q[2] + 15) / 16 && tx < stx; tx++) tmap[(size_t)ty * stx + tx] = r;
+        }
+    }
+    for (int k = 0; k < nm; k++) {
+        const double X = xy[2 * k], Y = xy[2 * k + 1];
+        double x0 = 0, y0 = 0, w = c->nx, h = c->ny;
+        bool ok = std::isfinite(X) && std::isfinite(Y) && X >= 0 && Y >= 0 && X < c->nx && Y < c->ny;
+        if (ok && c->nrect > 0) {
+            const int r = tmap[(size_t)((int)Y / 16) * stx + (int)X / 16];
+            if (r < 0) {
+                ok = false;
+            } else {
+                const int32_t* q = &c->rect_host[4 * r];
+                x0 = q[0]; y0 = q[1]; w = q[2]; h = q[3];
+            }
+        }
+        if (ok) ok = X - 2.0 >= x0 && Y - 2.0 >= y0 && X + 2.0 <= x0 + w && Y + 2.0 <= y0 + h;
+        if (!ok) {
+            struct { uint32_t bits; int32_t first; } z = {LBIB_ERR_EDGE, k};
+            LBIB_HIP(hipMemcpyAsync(c->st, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+            LBIB_HIP(hipStreamSynchronize(c->stream));
+            char msg[192];
+            snprintf(msg, sizeof msg, "geometry point %d at (%.17g, %.17g) lies within 2 sites of the edge of its domain",
+                     k, X, Y);
+            c->err = msg;
+            return -1;
+        }
+    }
+    return 0;
+}
+
 int lbib_set_markers(lbib_ctx* c, int nm, const double* xy, const double* vxy, const double* oxy) {
     if (check_ctx(c) || nm < 0 || (nm > 0 && !xy)) return -1;
+    if (check_marker_edges(c, nm, xy)) return -1;
     if (ensure_markers(c, nm)) return -1;
     std::vector<double> a(nm), b(nm);
     auto split = [&](const double* src, double* dx, double* dy) -> int {
@@ -964,6 +1018,48 @@ int lbib_add_mass_by_domain(lbib_ctx* c, int ndom, const double* table) {
     for (int d = 0; d < ndom; d++) t[d] = table[d];
     if (upload(c, c->mtab, t.data(), c->ndom * sizeof(double))) return -1;
     c->mtab_live = true;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int lbib_add_mass_sites(lbib_ctx* c, int64_t count, const int64_t* site, const double* m) {
+    if (check_ctx(c) || count < 0 || (count > 0 && (!site || !m))) return -1;
+    // a site listed twice keeps its last value (addMass overwrites)
+    std::vector<int64_t> s;
+    std::vector<double> v;
+    s.reserve((size_t)count);
+    v.reserve((size_t)count);
+    {
+        for (int64_t k = 0; k < count; k++) {
+            if (site[k] < 0 || site[k] >= c->n) {
+                c->err = "lbib_add_mass_sites: site index outside the lattice";
+                return -1;
+            }
+        }
+        std::unordered_map<int64_t, size_t> at;
+        at.reserve((size_t)count * 2);
+        for (int64_t k = 0; k < count; k++) {
+            auto it = at.find(site[k]);
+            if (it == at.end()) {
+                at.emplace(site[k], s.size());
+                s.push_back(site[k]);
+                v.push_back(m[k]);
+            } else {
+                v[it->second] = m[k];
+            }
+        }
+    }
+    const long cnt = (long)s.size();
+    if (cnt > c->mass_cap) {
+        dev_free(c->mass_site);
+        dev_free(c->mass_val);
+        c->mass_cap = 0;
+        if (dev_alloc(c, &c->mass_site, (size_t)cnt) || dev_alloc(c, &c->mass_val, (size_t)cnt)) return -1;
+        c->mass_cap = cnt;
+    }
+    if (upload(c, c->mass_site, s.data(), cnt * sizeof(int64_t)) || upload(c, c->mass_val, v.data(), cnt * sizeof(double)))
+        return -1;
+    if (launch_mass_sites(c, cnt, c->mass_site, c->mass_val)) return -1;
     LBIB_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }

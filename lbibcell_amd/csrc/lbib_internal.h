@@ -184,6 +184,10 @@ struct lbib_ctx {
     // partner, incoming finders, new row counts/offsets, and a scratch CSR
     // of fcap structs the new table is assembled in
     bool junc_on = false;
+    // staging of lbib_add_mass_sites (site indices, then values)
+    int64_t* mass_site = nullptr;
+    double* mass_val = nullptr;
+    long mass_cap = 0;
     int jcap = 0, jfcap = 0;
     int32_t *j_part = nullptr, *j_src = nullptr, *j_off = nullptr;
     uint32_t *j_in = nullptr, *j_cnt = nullptr, *j_ofs = nullptr, *j_total = nullptr;
@@ -374,6 +378,7 @@ int launch_forces_clear(lbib_ctx* c);
 int launch_forces_pairs(lbib_ctx* c, int type, double p0, double p1);
 // per-site mass of tutorial_02_BioSolverGrowth from species sR and sL
 int launch_mass_growth(lbib_ctx* c, int sR, int sL, double threshold, double maxsource);
+int launch_mass_sites(lbib_ctx* c, long cnt, const int64_t* site, const double* val);
 // markers that may name one marker as their closest junction partner
 constexpr int JUNCTION_MAXIN = 16;
 // deleteForceType(7) and one type-7 pair per marker with a closest marker of

@@ -742,6 +742,28 @@ __global__ void k_mass_growth(long n, const void* g, int sR, int sL, int nq, con
     }
 }
 
+// FluidSolver::addMass at a list of sites (FluidSolver.cpp:253-257: the call
+// overwrites, so the last value given for a site this iteration counts).  The
+// pending mass of the iteration is first made per-site (k_mass_fill: the
+// per-domain table, or zero, where no per-site field is pending yet), then the
+// listed sites are overwritten in list order by one thread (k_mass_scatter's
+// grid-stride loop keeps the last write of a repeated site when sites are
+// unique per list; duplicates are resolved on the host).
+__global__ void k_mass_fill(long n, const uint32_t* __restrict__ tag, const double* __restrict__ mtab, int ntab,
+                            double* msite, const DevStatus* st) {
+    if (step_stopped(st)) return;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const uint32_t d = tag[i] & DOM_MASK;
+        msite[i] = (mtab && (int)d < ntab) ? mtab[d] : 0.0;
+    }
+}
+__global__ void k_mass_scatter(long cnt, const int64_t* __restrict__ site, const double* __restrict__ val, double* msite,
+                               const DevStatus* st) {
+    if (step_stopped(st)) return;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < cnt; k += (long)gridDim.x * blockDim.x)
+        msite[site[k]] = val[k];
+}
+
 // --------------------------------------------------------------- drivers ---
 static inline int grid_for(long n) { return (int)std::min<long>(std::max<long>((n + 255) / 256, 1), 4096); }
 
@@ -846,6 +868,20 @@ int launch_forces_pairs(lbib_ctx* c, int type, double p0, double p1) {
     k_forces_pairs<<<grid_for(c->mcap), 256, 0, c->stream>>>(c->ca, c->cb, c->min_c, c->mout_c, type, p0, p1,
                                                              c->fs_off, c->fs_type, c->fs_partner, c->fs_prm, c->st);
     LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_mass_sites(lbib_ctx* c, long cnt, const int64_t* site, const double* val) {
+    if (!c->msite_live) {
+        const double* mt = c->mtab_live ? c->mtab : nullptr;
+        k_mass_fill<<<grid_for(c->n), 256, 0, c->stream>>>(c->n, c->tag, mt, c->ndom, c->msite, c->st);
+        LBIB_HIP(hipGetLastError());
+    }
+    if (cnt > 0) {
+        k_mass_scatter<<<grid_for(cnt), 256, 0, c->stream>>>(cnt, site, val, c->msite, c->st);
+        LBIB_HIP(hipGetLastError());
+    }
+    c->msite_live = true;
     return 0;
 }
 

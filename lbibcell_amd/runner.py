@@ -43,6 +43,7 @@ _SIG = {
     "lbh_add_bio_solver_params": (C.c_int, [_P, C.c_int, _D, C.c_uint]),
     "lbh_apply_bio": (C.c_int, [_P]),
     "lbh_run": (C.c_int, [_P, C.c_uint]),
+    "lbh_run_simulation": (C.c_int, [_P, C.c_uint]),
     "lbh_write_simulation": (C.c_int, [_P, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p]),
     "lbh_init_solvers_file": (C.c_int, [_P, C.c_char_p]),
     "lbh_sync": (C.c_int, [_P]),
@@ -230,6 +231,11 @@ class Simulation:
     def apply_bio(self):
         """The BioSolvers of the current iteration alone, outside a step."""
         self._chk(self.L.lbh_apply_bio(self.h))
+
+    def run_simulation(self, iterations):
+        """SimulationRunner::runSimulation: `iterations` steps; a failure is
+        caught and dumped to output/CRASH_*_<t>.txt under the working directory."""
+        self._chk(self.L.lbh_run_simulation(self.h, int(iterations)))
 
     def write_simulation(self, force, geometry, parameters, solvers):
         """SimulationRunner::writeSimulation: the reference's four text files."""

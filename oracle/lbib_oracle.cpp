@@ -1045,13 +1045,30 @@ static void applyBio(Sim& s) {
             // all nodes in id order, so equidistant candidates resolve to the smaller id.
             deleteForceType(s, 7);
             const double R = b.p[0], K = via_text(b.p[1]), L0 = via_text(b.p[2]);
+            // candidates within R lie in the 3x3 bins of side R around a node;
+            // visiting them in id order keeps the all-pairs result (first
+            // strictly closer node in id order wins)
             std::vector<const Marker*> nodes;
-            for (auto& kv : s.markers) nodes.push_back(kv.second.get());
+            std::map<std::pair<long, long>, std::vector<const Marker*>> bins;
+            auto key = [R](double x, double y) { return std::make_pair((long)std::floor(x / R), (long)std::floor(y / R)); };
+            for (auto& kv : s.markers) {
+                nodes.push_back(kv.second.get());
+                bins[key(kv.second->x, kv.second->y)].push_back(kv.second.get());
+            }
+            std::vector<const Marker*> cand;
             for (const Marker* m : nodes) {
                 const unsigned avoid = m->c0->domain;
                 const Marker* best = nullptr;
                 double bd = 0.0;
-                for (const Marker* q : nodes) {
+                cand.clear();
+                const auto k0 = key(m->x, m->y);
+                for (long by = k0.second - 1; by <= k0.second + 1; by++)
+                    for (long bx = k0.first - 1; bx <= k0.first + 1; bx++) {
+                        auto it = bins.find(std::make_pair(bx, by));
+                        if (it != bins.end()) cand.insert(cand.end(), it->second.begin(), it->second.end());
+                    }
+                std::sort(cand.begin(), cand.end(), [](const Marker* a, const Marker* c) { return a->id < c->id; });
+                for (const Marker* q : cand) {
                     const double d = std::sqrt((q->x - m->x) * (q->x - m->x) + (q->y - m->y) * (q->y - m->y));
                     if (d > R) continue;
                     if (q->c0->domain == avoid) continue;

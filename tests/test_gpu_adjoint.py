@@ -197,8 +197,11 @@ def test_torch_function_wrapper():
 
 def test_problem_validation():
     case = _small()
+    # a marker off the periodic lattice
     with pytest.raises(capi.LbibError):
-        adjoint.AdjointProblem(40, 36, 0.9, [np.array([[1.0, 1.0], [2.0, 1.0], [2.0, 2.0]])], case["dom"])
+        adjoint.AdjointProblem(40, 36, 0.9, [np.array([[-1.0, 1.0], [2.0, 1.0], [2.0, 2.0]])], case["dom"])
+    with pytest.raises(capi.LbibError):
+        adjoint.AdjointProblem(40, 36, 0.9, [np.array([[5.0, 5.0], [40.0, 5.0], [6.0, 7.0]])], case["dom"])
     p = _problem(case)
     with pytest.raises(capi.LbibError):
         p.gradient()
@@ -347,12 +350,35 @@ def test_multispecies_fp32_gradient_close_to_fp64():
     assert g32["k_junction"] == pytest.approx(g64["k_junction"], rel=1e-2)
 
 
-def test_batched_problems_give_the_sum_of_the_separate_gradients():
+def _shift_x(case, dx):
+    """The same periodic problem moved by dx sites along x."""
+    c = dict(case)
+    c["f0"] = np.roll(case["f0"], dx, axis=-1)
+    c["g0"] = np.roll(case["g0"], dx, axis=-1)
+    c["target"] = np.roll(case["target"], dx, axis=-1)
+    c["dom"] = np.roll(case["dom"], dx, axis=-1)
+    c["rings"] = [r + np.array([dx, 0]) for r in case["rings"]]
+    allx = np.concatenate([r[:, 0] for r in c["rings"]])
+    assert allx.min() >= 0 and allx.max() < c["nx"]
+    return c
+
+
+@pytest.mark.parametrize("edge", [False, True])
+def test_batched_problems_give_the_sum_of_the_separate_gradients(edge):
     """Two independent problems in one lattice (periodic rectangles, shared
     parameters): the loss and the gradient of every shared parameter are
     the sums of the separate problems', the per-cell rate gradients their
-    concatenation; rows outside both rectangles carry no loss."""
+    concatenation; rows outside both rectangles carry no loss.  With `edge`
+    the first problem's cells touch its rectangle's west edge and the second's
+    its east edge, so delta stencils wrap inside their own rectangle, as they
+    wrap on the separate periodic lattices."""
     cases = [_ms_case(seed=13, nx=60, ny=40), _ms_case(seed=19, nx=60, ny=40)]
+    if edge:
+        lo = min(r[:, 0].min() for r in cases[0]["rings"])
+        hi = max(r[:, 0].max() for r in cases[1]["rings"])
+        cases = [_shift_x(cases[0], -int(np.floor(lo))), _shift_x(cases[1], 59 - int(np.floor(hi)))]
+        print(f"case batched adjoint at rectangle edges: x from {min(r[:, 0].min() for r in cases[0]['rings']):.3f}"
+              f" to {max(r[:, 0].max() for r in cases[1]['rings']):.3f}", flush=True)
     origins = [(0, 0), (64, 0)]
     NX, NY, ns, steps = 128, 48, 3, 16
     prm = dict(tau_c=MS["tau_c"], kappa=MS["kappa"], k_junction=MS["k_junction"], K=cases[0]["K"])
