@@ -1132,8 +1132,13 @@ std::string SimulationRunner::describe(const lbib_status& s) {
     snprintf(buf, sizeof(buf), "[status bits 0x%x, markers %d, connections %d, iteration %u] ", s.bits, s.n_markers,
              s.n_connections, params_.currentIteration);
     m += buf;
-    if (s.first_bad_marker >= 0 && s.first_bad_marker < (int)ids_.size())
-        m += "GeometryNode ID=" + std::to_string(ids_[s.first_bad_marker]);
+    // the device's ids: remeshing there may have renumbered the dense indices
+    // since the host last copied the topology
+    if (s.first_bad_marker >= 0 && s.first_bad_marker < s.n_markers) {
+        std::vector<uint32_t> ids((size_t)s.n_markers);
+        if (lbib_get_marker_ids(ctx_, ids.data()) == 0)
+            m += "GeometryNode ID=" + std::to_string(ids[(size_t)s.first_bad_marker]);
+    }
     return m;
 }
 

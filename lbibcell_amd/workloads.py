@@ -32,19 +32,22 @@ def tutorial_cell(nx=200, ny=200, radius=20.0, n=120, seed=1):
     return circle_cell(nx / 2.0, ny / 2.0, radius, n, rng)
 
 
-def hex_tissue(nx, ny, cols, rows, radius=40.0, rjit=4.0, n=150, pos_jit=2.0, seed=1):
+def hex_tissue(nx, ny, cols, rows, radius=40.0, rjit=4.0, n=150, pos_jit=2.0, seed=1, margin=0.0):
     """cols x rows cells on a hex grid (odd rows shifted half a pitch), radius
-    radius +- rjit, centre jitter +- pos_jit; domains 1..cols*rows."""
+    radius +- rjit, centre jitter +- pos_jit; domains 1..cols*rows.  The grid
+    fills [margin, n - margin) in x and y: growing cells push fluid out
+    through the box outlet on the lattice edges, and the tissue drifts
+    towards them."""
     rng = np.random.default_rng(seed)
-    px = nx / (cols + 0.5)
-    py = ny / rows
+    px = (nx - 2 * margin) / (cols + 0.5)
+    py = (ny - 2 * margin) / rows
     nodes, conns = [], []
     dom = 1
     nid = 0
     for j in range(rows):
         for i in range(cols):
-            cx = (i + 0.5 + 0.5 * (j % 2)) * px + rng.uniform(-pos_jit, pos_jit)
-            cy = (j + 0.5) * py + rng.uniform(-pos_jit, pos_jit)
+            cx = margin + (i + 0.5 + 0.5 * (j % 2)) * px + rng.uniform(-pos_jit, pos_jit)
+            cy = margin + (j + 0.5) * py + rng.uniform(-pos_jit, pos_jit)
             r = radius + rng.uniform(-rjit, rjit)
             nd, cn = circle_cell(cx, cy, r, n, rng, jitter=0.5, first_id=nid, domain=dom)
             nodes += nd
