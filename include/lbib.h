@@ -252,6 +252,16 @@ int lbib_compute_areas(lbib_ctx* ctx, double* area);
  * differs from the reference's site-by-site sum (relative difference at the
  * 1e-15 level).  Synchronises. */
 int lbib_domain_concentrations(lbib_ctx* ctx, int s, int ndom, double* out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory or wait for the device (code under HIP graph capture, serving loops):
+ * the same sums as lbib_domain_concentrations, accumulated in a static device
+ * scratch array of the library (65536 doubles, one per GPU, held until the
+ * process ends) and written to d_out[0 .. ndom), a DEVICE array of the caller,
+ * on the context's stream: no allocation, no host copy, no synchronisation
+ * (read d_out after lbib_synchronize or a stream-ordered consumer).  ndom is at
+ * most 65536 (-1 otherwise).  All calls on one GPU share its scratch array:
+ * calls that overlap on a GPU, from any context or thread, race. */
+int lbib_domain_concentrations_noalloc(lbib_ctx* ctx, int s, int ndom, double* d_out);
 int lbib_area_regulator(lbib_ctx* ctx, double target_area, double kp, double maxsource,
                         double target_factor);
 /* tutorial_01_BioSolverMembraneTension (…MembraneTension.cpp:42-77) as one

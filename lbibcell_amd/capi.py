@@ -163,6 +163,7 @@ SIGNATURES = {
                                 C.POINTER(LatticeShape)]),
     "lbib_set_lattice_shape": (C.c_int, [_P, C.POINTER(LatticeShape)]),
     "lbib_domain_concentrations": (C.c_int, [_P, C.c_int, C.c_int, _D]),
+    "lbib_domain_concentrations_noalloc": (C.c_int, [_P, C.c_int, C.c_int, C.c_void_p]),
     "lbib_capture_begin": (C.c_int, [_P]),
     "lbib_capture_end": (C.c_int, [_P, C.POINTER(C.c_int)]),
     "lbib_graph_launch": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int)]),

@@ -280,6 +280,10 @@ void graph_drop(lbib_ctx* c) {
 }
 }  // namespace
 
+namespace lbib {
+int ctx_ready(lbib_ctx* c) { return check_ctx(c); }
+}  // namespace lbib
+
 extern "C" {
 
 int lbib_create(const lbib_config* cfg, lbib_ctx** out) {

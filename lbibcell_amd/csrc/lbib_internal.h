@@ -371,6 +371,8 @@ int launch_ring_areas(lbib_ctx* c, const AreaMass* mass);
 int launch_remesh(lbib_ctx* c, int rounds);   // also clears the ring counters
 // per-domain sum of one species' concentration into acc[ndom] (zeroed by the caller)
 int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc);
+// a live context with any deferred step begin enqueued (-1: null context)
+int ctx_ready(lbib_ctx* c);
 int launch_coarsen(lbib_ctx* c);
 int launch_forces_clear(lbib_ctx* c);
 // two structs of `type` per marker, one per adjacent connection (type 6:
