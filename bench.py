@@ -56,7 +56,9 @@ def cpu_baseline(nx, ny, nodes, conns, budget_s=15.0, max_steps=50):
     """The oracle (a CPU restatement of the reference path, OpenMP on the
     race-free lattice loops) on the same workload, timed for a bounded sample."""
     from oracle import orc
-    threads = min(16, os.cpu_count() or 1)
+    # one GPU's share of the box's host: 16 threads (OMP_NUM_THREADS on the box)
+    box_cpus = os.cpu_count() or 1
+    threads = min(int(os.environ.get("OMP_NUM_THREADS", "16") or 16), box_cpus)
     o = orc.Oracle(nx, ny, TAU_F)
     o.set_threads(threads)
     o.add_species(orc.SP_SIGNAL, TAU_C, [SIGNAL["production"], SIGNAL["decay"], SIGNAL["switchoff"],
@@ -73,15 +75,20 @@ def cpu_baseline(nx, ny, nodes, conns, budget_s=15.0, max_steps=50):
     o.add_bio(1, list(AREA), 1)
     o.add_bio(0, [TENSION], 10)
     o.run(2)            # first steps remesh the initial polygons
+    before = o.stage_times()
     steps = 0
     t0 = time.perf_counter()
     while steps < max_steps and time.perf_counter() - t0 < budget_s:
         o.run(1)
         steps += 1
     dt = time.perf_counter() - t0
+    after = o.stage_times()
     return {"value": nx * ny * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "host_cpus_visible": box_cpus,
+            "seconds_per_step_by_stage": {k: (after[k] - before[k]) / max(steps, 1) for k in after},
             "sample": f"{steps} coupled steps (after 2 warm-up steps) of the same {nx}x{ny} 400-cell workload, "
-                      f"oracle with {threads} OpenMP threads on the lattice loops, serial IB/geometry"}
+                      f"oracle with {threads} OpenMP threads (one GPU's share of the box's host) on the "
+                      f"lattice loops, serial IB/geometry"}
 
 
 PRECISIONS = {"fp64": 0, "fp32": 1, "fp16": 2, "bf16": 3, "fp16r": 4}
@@ -139,9 +146,11 @@ def build_batched(ens, precision):
     return sim
 
 
-def measure_batched(ens, prec_name, steps, warmup, run_ahead):
+def measure_batched(ens, prec_name, steps, warmup, run_ahead, graph_steps=10):
     """Batched ensemble at one storage precision: autotune the lattice launch
-    shape after the warm-up, then time `steps` coupled steps of all domains."""
+    shape after the warm-up, then time `steps` coupled steps of all domains,
+    replayed as recorded step graphs of graph_steps iterations (blocks start
+    after iterations that are multiples of graph_steps)."""
     sim = build_batched(ens, PRECISIONS[prec_name])
     sim.set_run_ahead(run_ahead)
     log(f"[batched {prec_name}] setup done; warming up")
@@ -150,6 +159,11 @@ def measure_batched(ens, prec_name, steps, warmup, run_ahead):
     ctx = sim.context()
     chosen, rows = ctx.autotune(reps=5)
     log(f"[batched {prec_name}] tuned lattice shape {chosen[:3]} ({chosen[3] * 1e3:.1f} us)")
+    if graph_steps:
+        sim.set_graph_steps(graph_steps)
+        sim.run(graph_steps)          # records the block once, outside the timed region
+        sim.sync()
+    g0 = sim.graph_stats()
     ctx.profile_enable(True)
     ctx.profile_read()
     sim.reset_times()
@@ -166,8 +180,15 @@ def measure_batched(ens, prec_name, steps, warmup, run_ahead):
     for x0, y0, w, h, *_ in ens["rects"]:
         live[y0:y0 + h, x0:x0 + w] = True
     fields = dict(speed=np.hypot(u, v)[live], conc=ctx.get_species(0).sum(axis=0)[live], area=sim.areas())
-    out = dict(dt=t1 - t0, kernel_s=ms / 1e3 / max(launches, 1), bytes_per_launch=bpl, tuned=chosen,
-               tried=rows, markers=len(sim.markers()[0]), times=sim.times(), fields=fields)
+    g1 = sim.graph_stats()
+    # inside step graphs the per-launch events see no lattice launch: the
+    # kernel time is then the tuner's average over launches on the live state
+    kernel_s = ms / 1e3 / launches if launches else chosen[3] / 1e3
+    out = dict(dt=t1 - t0, kernel_s=kernel_s, kernel_time_source="events" if launches else "autotuner",
+               bytes_per_launch=bpl, tuned=chosen,
+               tried=rows, markers=len(sim.markers()[0]), times=sim.times(), fields=fields,
+               graph={"steps_per_graph": g1[0], "launches_in_timed_region": g1[1] - g0[1],
+                      "recordings": g1[2], "note": sim.graph_note()})
     sim.close()
     return out
 
@@ -322,6 +343,71 @@ def measure_adjoint_ms(size, steps, K, ncell=1600):
             "d_k_junction": g["k_junction"], "d_K": g["K"].tolist(), "d_rates_norm": float(np.linalg.norm(g["rates"]))}
 
 
+LONG_CHECKS = (100, 1000, 5000)
+
+
+def long_window_errors(kind, precisions, steps):
+    """L2 errors of every storage precision against the FP64 run over a long
+    window, sampled at LONG_CHECKS (<= steps).  kind "tissue": the 2048^2 cells
+    of the headline laid out 100 sites from the edges, area regulator KP 1e-6
+    and clamp 4e-4 (the headline's own regulator carries the grown tissue into
+    the lattice edge in iteration 88, as it does in the reference), box outlet;
+    kind "ensemble": the 256-domain batched ensemble.  Returns {prec: {step:
+    {speed, concentration, cell_area}}} plus the seconds each run took."""
+    from lbibcell_amd import capi, runner, workloads
+    checks = [c for c in LONG_CHECKS if c <= steps]
+    fields, secs = {}, {}
+    for p in ["fp64"] + [q for q in precisions if q != "fp64"]:
+        if kind == "tissue":
+            nodes, conns = workloads.hex_tissue(2048, 2048, 20, 20, radius=40.0, rjit=4.0, n=150, pos_jit=2.0,
+                                                seed=1, margin=100.0)
+            sim = runner.Simulation(2048, 2048, TAU_F, precision=PRECISIONS[p])
+            sim.add_species("tutorial_01_CDESolverD2Q5_SIGNAL",
+                            capi.species_spec(capi.SPECIES_SIGNAL, TAU_C, **SIGNAL))
+            sim.add_geometry(nodes, conns)
+            for d in sorted({c[2] for c in conns}):
+                sim.set_celltype(d, 2)
+            sim.setup()
+            sim.init_solvers()
+            sim.add_mass_solver("MassSolverBoxOutlet")
+            sim.add_area_regulator(200.0, 1e-6, 4e-4, 1.2, freq=1)
+            sim.add_tension(TENSION, 10)
+            live = None
+        else:
+            ens = workloads.batched_ensemble(ndom=256, seed=1)
+            sim = build_batched(ens, PRECISIONS[p])
+            live = np.zeros((ens["ny"], ens["nx"]), dtype=bool)
+            for x0, y0, w, h, *_ in ens["rects"]:
+                live[y0:y0 + h, x0:x0 + w] = True
+        log(f"[long window {kind} {p}] {steps} steps")
+        t0 = time.perf_counter()
+        fields[p], done = {}, 0
+        for c in checks:
+            sim.run(c - done)
+            done = c
+            ctx = sim.context()
+            ctx.ns = 1
+            u, v = ctx.get_velocity()
+            sp, C = np.hypot(u, v), ctx.get_species(0).sum(axis=0)
+            if live is not None:
+                sp, C = sp[live], C[live]
+            fields[p][c] = (sp, C, sim.areas())
+        secs[p] = time.perf_counter() - t0
+        sim.close()
+    out = {}
+    for p in fields:
+        if p == "fp64":
+            continue
+        out[p] = {}
+        for c in checks:
+            sp, C, a = fields[p][c]
+            sr, Cr, ar = fields["fp64"][c]
+            common = sorted(set(a) & set(ar))
+            out[p][str(c)] = {"speed": l2_rel(sp, sr), "concentration": l2_rel(C, Cr),
+                              "cell_area": l2_rel([a[d] for d in common], [ar[d] for d in common])}
+    return {"errors_vs_fp64": out, "seconds": secs, "steps": steps}
+
+
 def l2_rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
 
@@ -352,6 +438,10 @@ def main():
                     help="storage precisions of the 4096^2 four-species junction workload ('' = skip); "
                          "fp64 also serves as the reference of the mixed errors")
     ap.add_argument("--multispecies-size", type=int, default=4096)
+    ap.add_argument("--long-window", default="tissue,ensemble",
+                    help="workloads whose mixed-precision errors are also measured over a long window ('' = skip)")
+    ap.add_argument("--long-steps", type=int, default=5000)
+    ap.add_argument("--long-precisions", default="fp32,fp16r,fp16")
     ap.add_argument("--multispecies-steps", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -439,7 +529,8 @@ def main():
         nlive = sum(r[2] * r[3] for r in ens["rects"])
         log(f"batched ensemble: {len(ens['rects'])} domains, {nlive} sites, atlas {ens['nx']}x{ens['ny']}, "
             f"{sum(len(d) for d in ens['domains'])} cells, {len(ens['nodes'])} initial markers")
-        bres = {p: measure_batched(ens, p, args.batched_steps, 5, args.run_ahead) for p in bvars}
+        # warm-up to iteration 11, one recorded block, then timed blocks from 21
+        bres = {p: measure_batched(ens, p, args.batched_steps, 11, args.run_ahead) for p in bvars}
         batched = {"workload": f"{len(ens['rects'])} periodic domains, sides log-uniform in [64, 512], "
                                f"1-16 cells r 8-30 each, tau_f U(0.6,1.5), tau_c U(0.55,1.2), one atlas "
                                f"{ens['nx']}x{ens['ny']} ({nlive} live sites)",
@@ -448,7 +539,11 @@ def main():
             achieved = r["bytes_per_launch"] / r["kernel_s"] / 1e9
             e = {"value": nlive * args.batched_steps / r["dt"] / 1e6, "unit": "MLUPS",
                  "ms_per_step": r["dt"] / args.batched_steps * 1e3, "dtype": DTYPES[p],
-                 "markers_after_remesh": r["markers"],
+                 "markers_after_remesh": r["markers"], "step_graphs": r["graph"],
+                 "lattice_kernel_time_from": r["kernel_time_source"],
+                 "host_seconds_per_step": {k: v / max(r["times"]["steps"], 1) for k, v in r["times"].items()
+                                           if k not in ("steps", "stops")},
+                 "resolved_stops": r["times"]["stops"],
                  "lattice_shape": {"variant": r["tuned"][0], "workgroup": r["tuned"][1],
                                    "sites_per_lane": r["tuned"][2]},
                  "autotune_us": {f"{t[0]}/{t[1]}/{t[2]}": round(t[3] * 1e3, 2) for t in r["tried"]},
@@ -464,7 +559,7 @@ def main():
                     "concentration": l2_rel(r["fields"]["conc"], ref["fields"]["conc"]),
                     "cell_area": l2_rel([r["fields"]["area"][d] for d in common],
                                         [ref["fields"]["area"][d] for d in common]),
-                    "after_steps": 5 + args.batched_steps}
+                    "after_steps": 11 + 10 + args.batched_steps}
             batched[p] = e
         out["batched"] = batched
     mvars = [p for p in args.multispecies.split(",") if p]
@@ -506,6 +601,10 @@ def main():
                     "after_steps": 5 + args.multispecies_steps}
             ms_out[p] = e
         out["multispecies"] = ms_out
+    lw = [k for k in args.long_window.split(",") if k]
+    if lw and args.long_steps > 0:
+        precs = [p for p in args.long_precisions.split(",") if p]
+        out["mixed_precision_long_window"] = {k: long_window_errors(k, precs, args.long_steps) for k in lw}
     if args.adjoint_steps > 0:
         log(f"adjoint: {args.adjoint_steps} steps ...")
         out["adjoint"] = measure_adjoint(args.adjoint_steps, 50)

@@ -163,6 +163,33 @@ int lbib_set_mass_field(lbib_ctx* ctx, const double* m);
  * upload ids, connections and forces for the new count. */
 int lbib_set_markers(lbib_ctx* ctx, int nm, const double* xy, const double* vxy, const double* oxy);
 int lbib_get_markers(lbib_ctx* ctx, double* xy, double* vxy, double* fxy, double* oxy);
+/* The markers whose current position meets a condition of
+ * GeometryNode::perturbatePosition (GeometryNode.cpp:153-214): the node lies
+ * within kEPSILON of an integer coordinate or of a lattice-cell diagonal.
+ * *count receives how many there are; the first cap of them (in no set order)
+ * get their dense index, GeometryNode id and position written.  Together with
+ * lbib_set_marker_positions it replaces the whole-geometry download and upload
+ * around GeometryNode::setPos's perturbation when the device halts with
+ * LBIB_HALT_PERTURB_NODE: only the flagged nodes draw perturbations. */
+int lbib_perturbation_candidates(lbib_ctx* ctx, int cap, int* count, int32_t* index, uint32_t* id, double* xy);
+/* The rings (membranes, one per domain id) that hold a connection meeting the
+ * condition of Connection::perturbConnection (Connection.cpp:332-387): its two
+ * markers within kEPSILON in x or in y.  *nrings receives how many there are;
+ * when that is at most cap_rings, domain[0..nrings) lists them in increasing
+ * domain id (the order GeometryHandler::perturbConnections visits rings,
+ * GeometryHandler.cpp:326-356), *nnodes the total of their markers, and
+ * off[0..nrings] the offsets of each ring's markers; when *nnodes is at most
+ * cap_nodes, marker[] and xy[] hold each ring's markers (dense index and
+ * position) in the walk order of perturbConnections: node j of a ring is the
+ * first marker of its j-th connection counted from the ring's first
+ * connection in vector order.  Rings are disjoint in markers, so walking
+ * these rings alone draws the same perturbations as walking all of them. */
+int lbib_perturbation_rings(lbib_ctx* ctx, int cap_rings, int cap_nodes, int* nrings, int* nnodes,
+                            uint32_t* domain, int32_t* off, int32_t* marker, double* xy);
+/* New positions of n markers given by dense index; no other marker changes.
+ * The positions pass the edge check of lbib_set_markers (first_bad_marker is
+ * the failing dense index). */
+int lbib_set_marker_positions(lbib_ctx* ctx, int n, const int32_t* index, const double* xy);
 /* Connections in Geometry::connections_ vector order: first/second marker
  * index, domain id, boundary-condition descriptor index (-1 = none). */
 int lbib_set_connections(lbib_ctx* ctx, int nc, const int32_t* a, const int32_t* b,

@@ -127,6 +127,10 @@ SIGNATURES = {
     "lbib_set_mass_field": (C.c_int, [_P, _D]),
     "lbib_set_markers": (C.c_int, [_P, C.c_int, _D, _D, _D]),
     "lbib_get_markers": (C.c_int, [_P, _D, _D, _D, _D]),
+    "lbib_perturbation_candidates": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int), _I32, _U32, _D]),
+    "lbib_set_marker_positions": (C.c_int, [_P, C.c_int, _I32, _D]),
+    "lbib_perturbation_rings": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _U32, _I32,
+                                          _I32, _D]),
     "lbib_set_connections": (C.c_int, [_P, C.c_int, _I32, _I32, _U32, _I32]),
     "lbib_set_bc_descriptors": (C.c_int, [_P, C.c_int, _I32, _I32]),
     "lbib_reserve": (C.c_int, [_P, C.c_int, C.c_int]),

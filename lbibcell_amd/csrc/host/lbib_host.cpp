@@ -1162,6 +1162,94 @@ void SimulationRunner::hostTopologyFallback(unsigned it, bool grow) {
     times.host += seconds_since(t0);
 }
 
+// Geometry::moveGeometryNodes -> setPos -> perturbatePosition draws only for
+// the nodes that meet one of its conditions, in id order: fetch just those
+// from the device, perturb them on the host and write their positions back,
+// then redo stage B.  Returns false (nothing changed) when more nodes are
+// flagged than one fetch holds; the caller then takes the whole-geometry path.
+bool SimulationRunner::perturbFlagged(unsigned it) {
+    constexpr int kCap = 4096;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<int32_t> idx(kCap);
+    std::vector<uint32_t> ids(kCap);
+    std::vector<double> xy(2 * kCap);
+    int cnt = 0;
+    check(lbib_perturbation_candidates(ctx_, kCap, &cnt, idx.data(), ids.data(), xy.data()), "perturbation candidates");
+    if (cnt > kCap) return false;
+    std::vector<int> order(cnt);
+    for (int i = 0; i < cnt; i++) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return ids[a] < ids[b]; });
+    std::vector<int32_t> at(cnt);
+    std::vector<double> pos(2 * cnt);
+    for (int j = 0; j < cnt; j++) {
+        const int i = order[j];
+        geometry::GeometryNode n;
+        n.id = ids[i];
+        n.x = xy[2 * i];
+        n.y = xy[2 * i + 1];
+        geo_.perturbatePosition(n);
+        at[j] = idx[i];
+        pos[2 * j] = n.x;
+        pos[2 * j + 1] = n.y;
+    }
+    check(lbib_set_marker_positions(ctx_, cnt, at.data(), pos.data()), "set_marker_positions");
+    mirrorValid_ = false;
+    check(lbib_clear_status(ctx_), "status");
+    times.sparsePerturbations++;
+    times.host += seconds_since(t0);
+    check(lbib_step_geometry(ctx_, it, it % 10 == 0), "moveLattice");
+    return true;
+}
+
+// GeometryHandler::perturbConnections walks every ring from its first
+// connection, in increasing domain id, and draws only for connections whose
+// markers line up within kEPSILON (a draw moves the connection's first marker,
+// which can line up the ring's last connection in turn).  Rings share no
+// markers, so only the rings the device flags need the walk: fetch those in
+// walk order, walk them on the host and write back the markers that moved.
+// Returns false (nothing changed) when they do not fit one fetch.
+bool SimulationRunner::perturbRingsFlagged(unsigned it) {
+    constexpr int kRings = 1024, kNodes = 1 << 17;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> dom(kRings);
+    std::vector<int32_t> off(kRings + 1), mk(kNodes);
+    std::vector<double> xy(2 * (size_t)kNodes);
+    int nr = 0, nn = 0;
+    check(lbib_perturbation_rings(ctx_, kRings, kNodes, &nr, &nn, dom.data(), off.data(), mk.data(), xy.data()),
+          "perturbation rings");
+    if (nr > kRings || nn > kNodes) return false;
+    std::vector<int32_t> at;
+    std::vector<double> pos;
+    for (int r = 0; r < nr; r++) {
+        const int o = off[r], n = off[r + 1] - off[r];
+        std::vector<geometry::GeometryNode> ring(n);
+        for (int j = 0; j < n; j++) {
+            ring[j].x = xy[2 * (size_t)(o + j)];
+            ring[j].y = xy[2 * (size_t)(o + j) + 1];
+        }
+        for (int j = 0; j < n; j++) {
+            geometry::Connection cn;
+            cn.first = &ring[j];
+            cn.second = &ring[(j + 1) % n];
+            cn.domain = dom[r];
+            geo_.perturbConnection(cn);
+        }
+        for (int j = 0; j < n; j++) {
+            if (ring[j].x == xy[2 * (size_t)(o + j)] && ring[j].y == xy[2 * (size_t)(o + j) + 1]) continue;
+            at.push_back(mk[o + j]);
+            pos.push_back(ring[j].x);
+            pos.push_back(ring[j].y);
+        }
+    }
+    check(lbib_set_marker_positions(ctx_, (int)at.size(), at.data(), pos.data()), "set_marker_positions");
+    mirrorValid_ = false;
+    check(lbib_clear_status(ctx_), "status");
+    times.sparsePerturbations++;
+    times.host += seconds_since(t0);
+    check(lbib_step_geometry(ctx_, it, it % 10 == 0), "moveLattice");
+    return true;
+}
+
 void SimulationRunner::handleGeometryStatus(unsigned it) {
     bool coarsened = false;
     for (int guard = 0; guard < 64; guard++) {
@@ -1170,9 +1258,12 @@ void SimulationRunner::handleGeometryStatus(unsigned it) {
         nc_ = (size_t)lastStatus.n_connections;
         const uint32_t bits = lastStatus.bits;
         if (bits & LBIB_ERR_MASK) throw Exception(describe(lastStatus));
+        if ((bits & LBIB_HALT_PERTURB_NODE) && !(bits & LBIB_HALT_PERTURB_CONN) && perturbFlagged(it)) continue;
+        if ((bits & LBIB_HALT_PERTURB_CONN) && !(bits & LBIB_HALT_PERTURB_NODE) && perturbRingsFlagged(it)) continue;
         if (bits & (LBIB_HALT_PERTURB_NODE | LBIB_HALT_PERTURB_CONN)) {
             // the move ran, nothing after it: perturb on the host, redo stage B
             auto t0 = std::chrono::steady_clock::now();
+            times.fullPerturbations++;
             mirrorValid_ = false;
             syncMirror();
             if (bits & LBIB_HALT_PERTURB_NODE) {

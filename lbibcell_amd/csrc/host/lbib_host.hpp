@@ -192,6 +192,9 @@ struct StageTimes {
     double lattice = 0, geometry = 0, host = 0, bio = 0, wait = 0;
     long steps = 0;
     long stops = 0;   // enqueued iterations that stopped and were resolved
+    // perturbation halts resolved from the flagged nodes or rings alone, and
+    // perturbation halts resolved through the whole host geometry
+    long sparsePerturbations = 0, fullPerturbations = 0;
 };
 
 // SimulationRunner (LbmLib/include/SimulationRunner.hpp:43-231) fused with the
@@ -331,6 +334,8 @@ class SimulationRunner {
     void settle();
     void resolveStop(unsigned it);
     void hostTopologyFallback(unsigned it, bool grow);
+    bool perturbFlagged(unsigned it);
+    bool perturbRingsFlagged(unsigned it);
     std::string describe(const lbib_status& s);
 
     GlobalSimulationParameters params_;

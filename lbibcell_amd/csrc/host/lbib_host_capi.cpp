@@ -262,17 +262,20 @@ long lbh_perturb_counts(void* p, int which) {
     return which == 0 ? h->geo.nodePerturbations : h->geo.connectionPerturbations;
 }
 // host seconds (lattice, geometry, host fallbacks, bio enqueue; status
-// waits), the step count and the number of resolved stops
+// waits), the step count, the number of resolved stops, and the node
+// perturbations resolved from the flagged nodes / through the host geometry
 void lbh_times(void* p, double* out) {
     auto* h = static_cast<Handle*>(p);
     if (!h->runner) {
-        for (int k = 0; k < 7; k++) out[k] = 0.0;
+        for (int k = 0; k < 9; k++) out[k] = 0.0;
         return;
     }
     const auto& t = R(h).times;
     out[0] = t.lattice; out[1] = t.geometry; out[2] = t.host; out[3] = t.bio; out[4] = (double)t.steps;
     out[5] = t.wait;
     out[6] = (double)t.stops;
+    out[7] = (double)t.sparsePerturbations;
+    out[8] = (double)t.fullPerturbations;
 }
 // iterations enqueued between status reads (1 = after every iteration)
 int lbh_set_run_ahead(void* p, unsigned n) {

@@ -404,7 +404,7 @@ void lbib_destroy(lbib_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
-    dev_free(c->mass_site); dev_free(c->mass_val);
+    dev_free(c->mass_site); dev_free(c->mass_val); dev_free(c->pert_buf);
     dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc); dev_free(c->vsegs);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
     dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
@@ -580,7 +580,7 @@ int lbib_set_mass_field(lbib_ctx* c, const double* m) {
 // 2 <= Y <= ny - 2 (inside the owning rectangle in a batched atlas), is
 // required of every uploaded position before anything is launched: the spread
 // tiles and the interpolation index the lattice without wrapping.
-static int check_marker_edges(lbib_ctx* c, int nm, const double* xy) {
+static int check_marker_edges(lbib_ctx* c, int nm, const double* xy, const int32_t* index = nullptr) {
     std::vector<int32_t> tmap;
     const int stx = (c->nx + 15) / 16, sty = (c->ny + 15) / 16;
     if (c->nrect > 0) {
@@ -606,12 +606,13 @@ static int check_marker_edges(lbib_ctx* c, int nm, const double* xy) {
         }
         if (ok) ok = X - 2.0 >= x0 && Y - 2.0 >= y0 && X + 2.0 <= x0 + w && Y + 2.0 <= y0 + h;
         if (!ok) {
-            struct { uint32_t bits; int32_t first; } z = {LBIB_ERR_EDGE, k};
+            const int32_t at = index ? index[k] : k;
+            struct { uint32_t bits; int32_t first; } z = {LBIB_ERR_EDGE, at};
             LBIB_HIP(hipMemcpyAsync(c->st, &z, sizeof(z), hipMemcpyHostToDevice, c->stream));
             LBIB_HIP(hipStreamSynchronize(c->stream));
             char msg[192];
             snprintf(msg, sizeof msg, "geometry point %d at (%.17g, %.17g) lies within 2 sites of the edge of its domain",
-                     k, X, Y);
+                     at, X, Y);
             c->err = msg;
             return -1;
         }
@@ -637,6 +638,141 @@ int lbib_set_markers(lbib_ctx* c, int nm, const double* xy, const double* vxy, c
     }
     c->nm = nm;
     return put_counts(c);
+}
+// the device marker and connection counts, read after everything enqueued
+static int sync_counts(lbib_ctx* c) {
+    LBIB_HIP(hipMemcpyAsync(c->st_host, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    c->nm = (int)c->st_host->nm;
+    c->nc = (int)c->st_host->nc;
+    return 0;
+}
+// Carves typed, 256-byte aligned arrays out of the perturbation staging: a
+// first pass with base == nullptr sizes it.
+extern "C++" {
+struct Carve {
+    char* base;
+    size_t at = 0;
+    template <typename T>
+    T* take(size_t n) {
+        at = (at + 255) & ~size_t(255);
+        T* r = reinterpret_cast<T*>(base + at);
+        at += n * sizeof(T);
+        return r;
+    }
+};
+}
+static char* pert_staging(lbib_ctx* c, size_t bytes) {
+    if (bytes > c->pert_bytes) {
+        dev_free(c->pert_buf);
+        c->pert_bytes = 0;
+        if (dev_alloc(c, &c->pert_buf, bytes)) return nullptr;
+        c->pert_bytes = bytes;
+    }
+    return c->pert_buf;
+}
+int lbib_perturbation_candidates(lbib_ctx* c, int cap, int* count, int32_t* index, uint32_t* id, double* xy) {
+    if (check_ctx(c) || cap < 0 || !count || (cap > 0 && (!index || !id || !xy))) return -1;
+    if (sync_counts(c)) return -1;
+    const size_t w = (size_t)std::max(cap, 1);
+    Carve sz{nullptr};
+    sz.take<uint32_t>(1); sz.take<double>(2 * w); sz.take<int32_t>(w); sz.take<uint32_t>(w);
+    Carve cv{pert_staging(c, sz.at)};
+    if (!cv.base) return -1;
+    uint32_t* cnt = cv.take<uint32_t>(1);
+    double* dxy = cv.take<double>(2 * w);
+    int32_t* dix = cv.take<int32_t>(w);
+    uint32_t* did = cv.take<uint32_t>(w);
+    LBIB_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+    if (launch_perturbation_candidates(c, cap, cnt, dix, did, dxy)) return -1;
+    uint32_t n = 0;
+    if (download(c, &n, cnt, sizeof n)) return -1;
+    *count = (int)n;
+    const size_t got = std::min<size_t>(n, (size_t)cap);
+    if (got > 0 && (download(c, index, dix, got * sizeof(int32_t)) || download(c, id, did, got * sizeof(uint32_t)) ||
+                    download(c, xy, dxy, 2 * got * sizeof(double))))
+        return -1;
+    return 0;
+}
+int lbib_perturbation_rings(lbib_ctx* c, int cap_rings, int cap_nodes, int* nrings, int* nnodes, uint32_t* domain,
+                            int32_t* off, int32_t* marker, double* xy) {
+    if (check_ctx(c) || cap_rings < 0 || cap_nodes < 0 || !nrings || !nnodes ||
+        (cap_rings > 0 && (!domain || !off)) || (cap_nodes > 0 && (!marker || !xy)))
+        return -1;
+    *nrings = 0;
+    *nnodes = 0;
+    if (sync_counts(c)) return -1;
+    if (c->nc == 0 || c->ndom == 0) return 0;
+    const size_t wr = (size_t)std::max(cap_rings, 1), wn = (size_t)std::max(cap_nodes, 1);
+    Carve sz{nullptr};
+    sz.take<uint32_t>(c->ndom); sz.take<uint32_t>(1); sz.take<uint32_t>(wr); sz.take<uint32_t>(wr);
+    sz.take<uint32_t>(wr); sz.take<int32_t>(wr + 1); sz.take<int32_t>(wn); sz.take<double>(2 * wn);
+    Carve cv{pert_staging(c, sz.at)};
+    if (!cv.base) return -1;
+    uint32_t* dflag = cv.take<uint32_t>(c->ndom);
+    uint32_t* cnt = cv.take<uint32_t>(1);
+    uint32_t* dlist = cv.take<uint32_t>(wr);
+    uint32_t* dlen = cv.take<uint32_t>(wr);
+    uint32_t* dbase = cv.take<uint32_t>(wr);
+    int32_t* doff = cv.take<int32_t>(wr + 1);
+    int32_t* dmarker = cv.take<int32_t>(wn);
+    double* dxy = cv.take<double>(2 * wn);
+    // ring order of the current topology (the in/out links first)
+    if (launch_inout(c) || launch_rings(c)) return -1;
+    LBIB_HIP(hipMemsetAsync(dflag, 0, (size_t)c->ndom * sizeof(uint32_t), c->stream));
+    LBIB_HIP(hipMemsetAsync(cnt, 0, sizeof(uint32_t), c->stream));
+    if (launch_conn_perturbation_domains(c, dflag, cap_rings, cnt, dlist)) return -1;
+    uint32_t nd = 0;
+    if (download(c, &nd, cnt, sizeof nd)) return -1;
+    *nrings = (int)nd;
+    if (nd == 0 || nd > (uint32_t)cap_rings) return 0;
+    std::vector<uint32_t> doms(nd), len(nd), base(nd);
+    if (download(c, doms.data(), dlist, nd * sizeof(uint32_t))) return -1;
+    std::sort(doms.begin(), doms.end());
+    if (upload(c, dlist, doms.data(), nd * sizeof(uint32_t)) || launch_ring_info(c, (int)nd, dlist, dlen, dbase) ||
+        download(c, len.data(), dlen, nd * sizeof(uint32_t)))
+        return -1;
+    std::vector<int32_t> o(nd + 1, 0);
+    for (uint32_t i = 0; i < nd; i++) {
+        if (len[i] == 0) {
+            c->err = "lbib_perturbation_rings: a flagged connection's domain has no ring";
+            return -1;
+        }
+        o[i + 1] = o[i] + (int32_t)len[i];
+        if ((long)o[i + 1] > (long)INT_MAX / 2) { c->err = "lbib_perturbation_rings: rings too long"; return -1; }
+    }
+    *nnodes = o[nd];
+    for (uint32_t i = 0; i < nd; i++) domain[i] = doms[i];
+    for (uint32_t i = 0; i <= nd; i++) off[i] = o[i];
+    if (o[nd] > cap_nodes) return 0;
+    if (upload(c, doff, o.data(), (nd + 1) * sizeof(int32_t)) ||
+        launch_ring_gather(c, (int)nd, dbase, doff, dmarker, dxy) ||
+        download(c, marker, dmarker, (size_t)o[nd] * sizeof(int32_t)) ||
+        download(c, xy, dxy, 2 * (size_t)o[nd] * sizeof(double)))
+        return -1;
+    return 0;
+}
+int lbib_set_marker_positions(lbib_ctx* c, int n, const int32_t* index, const double* xy) {
+    if (check_ctx(c) || n < 0 || (n > 0 && (!index || !xy))) return -1;
+    if (n == 0) return 0;
+    if (sync_counts(c)) return -1;
+    for (int i = 0; i < n; i++)
+        if (index[i] < 0 || index[i] >= c->nm) {
+            c->err = "lbib_set_marker_positions: marker index outside the live markers";
+            return -1;
+        }
+    if (check_marker_edges(c, n, xy, index)) return -1;
+    Carve sz{nullptr};
+    sz.take<double>(2 * (size_t)n); sz.take<int32_t>((size_t)n);
+    Carve cv{pert_staging(c, sz.at)};
+    if (!cv.base) return -1;
+    double* dxy = cv.take<double>(2 * (size_t)n);
+    int32_t* dix = cv.take<int32_t>((size_t)n);
+    if (upload(c, dxy, xy, 2 * (size_t)n * sizeof(double)) || upload(c, dix, index, (size_t)n * sizeof(int32_t)) ||
+        launch_set_positions(c, n, dix, dxy))
+        return -1;
+    LBIB_HIP(hipStreamSynchronize(c->stream));
+    return 0;
 }
 int lbib_set_marker_ids(lbib_ctx* c, const uint32_t* ids) {
     if (check_ctx(c) || (c->nm > 0 && !ids)) return -1;

@@ -184,6 +184,10 @@ struct lbib_ctx {
     // partner, incoming finders, new row counts/offsets, and a scratch CSR
     // of fcap structs the new table is assembled in
     bool junc_on = false;
+    // device staging of the perturbation entry points (lbib_perturbation_*,
+    // lbib_set_marker_positions), grown on demand
+    char* pert_buf = nullptr;
+    size_t pert_bytes = 0;
     // staging of lbib_add_mass_sites (site indices, then values)
     int64_t* mass_site = nullptr;
     double* mass_val = nullptr;
@@ -381,6 +385,21 @@ int launch_forces_pairs(lbib_ctx* c, int type, double p0, double p1);
 // per-site mass of tutorial_02_BioSolverGrowth from species sR and sL
 int launch_mass_growth(lbib_ctx* c, int sR, int sL, double threshold, double maxsource);
 int launch_mass_sites(lbib_ctx* c, long cnt, const int64_t* site, const double* val);
+// the markers (of the first c->nm) whose position meets a perturbation
+// condition of GeometryNode::perturbatePosition: *cnt (zeroed by the caller)
+// counts them, the first cap get their dense index, id and position written
+int launch_perturbation_candidates(lbib_ctx* c, int cap, uint32_t* cnt, int32_t* ix, uint32_t* id, double* xy);
+// mx/my[ix[i]] = xy[2i], xy[2i+1] for i < n
+int launch_set_positions(lbib_ctx* c, int n, const int32_t* ix, const double* xy);
+// the domains (dflag[ndom] zeroed by the caller) holding a connection that
+// meets the Connection::perturbConnection condition: *cnt counts them, the
+// first cap are listed in dlist
+int launch_conn_perturbation_domains(lbib_ctx* c, uint32_t* dflag, int cap, uint32_t* cnt, uint32_t* dlist);
+// ring length and ring_marker offset of each of the nd listed domains
+int launch_ring_info(lbib_ctx* c, int nd, const uint32_t* dlist, uint32_t* len, uint32_t* base);
+// the markers of each listed ring in walk order from its first connection
+// (node j is the first marker of ring connection j), at off[i]..off[i+1)
+int launch_ring_gather(lbib_ctx* c, int nd, const uint32_t* base, const int32_t* off, int32_t* marker, double* xy);
 // markers that may name one marker as their closest junction partner
 constexpr int JUNCTION_MAXIN = 16;
 // deleteForceType(7) and one type-7 pair per marker with a closest marker of

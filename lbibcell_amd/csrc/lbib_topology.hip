@@ -871,6 +871,102 @@ int launch_forces_pairs(lbib_ctx* c, int type, double p0, double p1) {
     return 0;
 }
 
+namespace {
+__global__ void k_pert_collect(int nm, const double* __restrict__ mx, const double* __restrict__ my,
+                               const uint32_t* __restrict__ mid, int cap, uint32_t* cnt, int32_t* ix, uint32_t* id,
+                               double* xy) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nm; k += gridDim.x * blockDim.x) {
+        const double X = mx[k], Y = my[k];
+        if (!needs_perturbation(X, Y)) continue;
+        const uint32_t s = atomicAdd(cnt, 1u);
+        if (s < (uint32_t)cap) {
+            ix[s] = k;
+            id[s] = mid[k];
+            xy[2 * s] = X;
+            xy[2 * s + 1] = Y;
+        }
+    }
+}
+__global__ void k_conn_pert_domains(int nc, int ndom, const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
+                                    const uint32_t* __restrict__ cdom, const double* __restrict__ mx,
+                                    const double* __restrict__ my, uint32_t* dflag, int cap, uint32_t* cnt,
+                                    uint32_t* dlist) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const int a = ca[c], b = cb[c];
+        if (!(fabs(mx[a] - mx[b]) < kEPS || fabs(my[a] - my[b]) < kEPS)) continue;
+        const uint32_t d = cdom[c];
+        if (d >= (uint32_t)ndom || atomicExch(&dflag[d], 1u) != 0u) continue;
+        const uint32_t s = atomicAdd(cnt, 1u);
+        if (s < (uint32_t)cap) dlist[s] = d;
+    }
+}
+__global__ void k_ring_info(int nd, const uint32_t* __restrict__ dlist, const uint32_t* __restrict__ ring_cnt,
+                            const uint32_t* __restrict__ ring_off, uint32_t* len, uint32_t* base) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nd; i += gridDim.x * blockDim.x) {
+        len[i] = ring_cnt[dlist[i]];
+        base[i] = ring_off[dlist[i]];
+    }
+}
+// one workgroup per listed ring; ring_marker[base + p] is the second marker of
+// the p-th connection from the ring's first, so the first marker of
+// connection j is ring_marker[base + j - 1] (j = 0: the last entry)
+__global__ void k_ring_gather(const uint32_t* __restrict__ base, const int32_t* __restrict__ off,
+                              const int32_t* __restrict__ ring_marker, const double* __restrict__ mx,
+                              const double* __restrict__ my, int32_t* marker, double* xy) {
+    const int i = blockIdx.x;
+    const int o = off[i], n = off[i + 1] - off[i];
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const int32_t m = ring_marker[base[i] + (uint32_t)(j == 0 ? n - 1 : j - 1)];
+        marker[o + j] = m;
+        xy[2 * (o + j)] = mx[m];
+        xy[2 * (o + j) + 1] = my[m];
+    }
+}
+__global__ void k_set_positions(int n, const int32_t* __restrict__ ix, const double* __restrict__ xy, double* mx,
+                                double* my) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        mx[ix[i]] = xy[2 * i];
+        my[ix[i]] = xy[2 * i + 1];
+    }
+}
+}  // namespace
+
+int launch_perturbation_candidates(lbib_ctx* c, int cap, uint32_t* cnt, int32_t* ix, uint32_t* id, double* xy) {
+    if (c->nm <= 0) return 0;
+    k_pert_collect<<<grid_for(c->nm), 256, 0, c->stream>>>(c->nm, c->mx, c->my, c->mid, cap, cnt, ix, id, xy);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_conn_perturbation_domains(lbib_ctx* c, uint32_t* dflag, int cap, uint32_t* cnt, uint32_t* dlist) {
+    if (c->nc <= 0) return 0;
+    k_conn_pert_domains<<<grid_for(c->nc), 256, 0, c->stream>>>(c->nc, c->ndom, c->ca, c->cb, c->cdom, c->mx, c->my,
+                                                                dflag, cap, cnt, dlist);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_ring_info(lbib_ctx* c, int nd, const uint32_t* dlist, uint32_t* len, uint32_t* base) {
+    if (nd <= 0) return 0;
+    k_ring_info<<<grid_for(nd), 256, 0, c->stream>>>(nd, dlist, c->ring_cnt, c->ring_off_u, len, base);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_ring_gather(lbib_ctx* c, int nd, const uint32_t* base, const int32_t* off, int32_t* marker, double* xy) {
+    if (nd <= 0) return 0;
+    k_ring_gather<<<nd, 256, 0, c->stream>>>(base, off, c->ring_marker, c->mx, c->my, marker, xy);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
+int launch_set_positions(lbib_ctx* c, int n, const int32_t* ix, const double* xy) {
+    if (n <= 0) return 0;
+    k_set_positions<<<grid_for(n), 256, 0, c->stream>>>(n, ix, xy, c->mx, c->my);
+    LBIB_HIP(hipGetLastError());
+    return 0;
+}
+
 int launch_mass_sites(lbib_ctx* c, long cnt, const int64_t* site, const double* val) {
     if (!c->msite_live) {
         const double* mt = c->mtab_live ? c->mtab : nullptr;

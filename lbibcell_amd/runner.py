@@ -293,10 +293,10 @@ class Simulation:
         return self.L.lbh_perturb_counts(self.h, 0), self.L.lbh_perturb_counts(self.h, 1)
 
     def times(self):
-        out = np.zeros(7)
+        out = np.zeros(9)
         self.L.lbh_times(self.h, capi.dptr(out))
         return dict(lattice=out[0], geometry=out[1], host=out[2], bio=out[3], wait=out[5], steps=int(out[4]),
-                    stops=int(out[6]))
+                    stops=int(out[6]), sparse_perturbations=int(out[7]), full_perturbations=int(out[8]))
 
     def reset_times(self):
         self.L.lbh_reset_times(self.h)

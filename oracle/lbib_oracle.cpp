@@ -51,6 +51,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -190,6 +191,10 @@ struct Sim {
     std::vector<Bio> bio;
     int threads = 1;
     std::vector<unsigned> reinit_log;  // sites reinitialised during the last step
+    // seconds per stage: forces + spread, boundary + streaming, collide,
+    // interpolation + move + cut links + domain ids, remesh + coarsen,
+    // mass solver + BioSolvers
+    double stage_s[6] = {0, 0, 0, 0, 0, 0};
     std::string last_error;
 
     long idx(long x, long y) const { return y * nx + x; }
@@ -1189,23 +1194,36 @@ static void collide(Sim& s) {       // SimulationRunner.cpp:244-261
         for (size_t k = 0; k < s.sp.size(); k++) speciesCollide(s, (int)k, i, cj);
     }
 }
-// SimulationRunner::runSimulation loop body (SimulationRunner.cpp:72-100)
+// SimulationRunner::runSimulation loop body (SimulationRunner.cpp:72-100),
+// with the wall time of each stage accumulated in Sim::stage_s
 static void step(Sim& s) {
+    auto t = std::chrono::steady_clock::now();
+    auto lap = [&](int k) {
+        const auto now = std::chrono::steady_clock::now();
+        s.stage_s[k] += std::chrono::duration<double>(now - t).count();
+        t = now;
+    };
     calculateForce(s);
     distributeForce(s);
+    lap(0);
     preAdvect(s);
     advect(s);
     postAdvect(s);
+    lap(1);
     collide(s);
+    lap(2);
     collectVelocity(s);
     moveLattice(s);
+    lap(3);
     while (remeshBoundary(s)) {}
     if (s.it % 10 == 0) {
         s.forcemap.clear();
         coarsenBoundary(s);
     }
+    lap(4);
     if (s.box_outlet) boxOutlet(s);
     applyBio(s);
+    lap(5);
     s.it++;
 }
 
@@ -1493,6 +1511,9 @@ unsigned orc_mt19937_default_10000(void) {
     return v;
 }
 int orc_reinit_count(void* p) { return (int)static_cast<Sim*>(p)->reinit_log.size(); }
+void orc_stage_times(void* p, double* out) {
+    for (int k = 0; k < 6; k++) out[k] = static_cast<Sim*>(p)->stage_s[k];
+}
 // Half-open point test of UtilLib::geometry::Rectangle::pointWithinBounds.
 int orc_rect_within(double n, double s_, double w, double e, double x, double y) {
     return (x >= w) && (x < e) && (y <= n) && (y > s_);

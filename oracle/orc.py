@@ -74,6 +74,7 @@ _SIG = {
     "orc_reinit_count": (C.c_int, [_P]),
     "orc_rect_within": (C.c_int, [C.c_double] * 6),
     "orc_num_bnodes": (C.c_int, [_P]),
+    "orc_stage_times": (None, [_P, _D]),
     "orc_get_bnodes": (None, [_P, _I32, _I32, _U32, _I32, _D]),
     "orc_set_marker_velocities": (None, [_P, _D]),
     "orc_stream_bc": (None, [_P]),
@@ -307,6 +308,15 @@ class Oracle:
 
     def reinit_count(self):
         return self.L.orc_reinit_count(self.h)
+
+    STAGES = ("forces_and_spread", "boundary_and_streaming", "collide", "interpolate_move_cut_links_domains",
+              "remesh_coarsen", "mass_and_biosolvers")
+
+    def stage_times(self):
+        """Seconds spent in each stage of the coupled steps run so far."""
+        out = np.zeros(6)
+        self.L.orc_stage_times(self.h, _d(out))
+        return dict(zip(self.STAGES, out.tolist()))
 
     # single stages for the hand-derived tests
     def bnodes(self):
