@@ -106,6 +106,13 @@ int ensure_markers(lbib_ctx* c, int nm) {
     // per-step work lists: rebuilt every step
     dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
     if (dev_alloc(c, &c->tslot, cap) || dev_alloc(c, &c->tlist, cap) || dev_alloc(c, &c->tpos, cap)) return -1;
+    if (c->prec != LBIB_FP64) {
+        dev_free(c->pslot); dev_free(c->pe_x); dev_free(c->pe_y); dev_free(c->pe_fx); dev_free(c->pe_fy);
+        const size_t ne = 4 * (size_t)cap;
+        if (dev_alloc(c, &c->pslot, ne) || dev_alloc(c, &c->pe_x, ne) || dev_alloc(c, &c->pe_y, ne) ||
+            dev_alloc(c, &c->pe_fx, ne) || dev_alloc(c, &c->pe_fy, ne))
+            return -1;
+    }
     c->mcap = cap;
     if (ensure_scan(c)) return -1;
     return ensure_junctions(c);
@@ -408,6 +415,7 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc); dev_free(c->vsegs);
     dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
     dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
+    dev_free(c->pslot); dev_free(c->pe_x); dev_free(c->pe_y); dev_free(c->pe_fx); dev_free(c->pe_fy);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
                          &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
     uint32_t** uarr[18] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,

@@ -339,6 +339,15 @@ __device__ __forceinline__ uint32_t xcd_contiguous(uint32_t lid, uint32_t total)
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + local;
 }
 
+// Tile rows dealt to the 8 XCDs in turn (row r to XCD r % 8), consecutive
+// workgroups of one XCD walking along a row, for a grid of
+// 8 * ceil(nty / 8) * ntx workgroups; -1 for the workgroups past the last row.
+__device__ __forceinline__ int row_interleaved_tile(uint32_t lid, int ntx, int nty) {
+    const int x = (int)(lid % 8u), local = (int)(lid / 8u);
+    const int row = x + 8 * (local / ntx), col = local % ntx;
+    return row < nty ? row * ntx + col : -1;
+}
+
 // Inclusive prefix sum across a 64-lane wavefront.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     const int lane = threadIdx.x & 63;

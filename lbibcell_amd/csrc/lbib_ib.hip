@@ -9,12 +9,13 @@
 // GeometryNode::perturbatePosition) that set is x = floor(X)-1 .. floor(X)+2,
 // so the kernels derive it from the current position instead of storing it.
 //
-// Ordered spread.  The reference adds delta*F into each site in marker-id
-// order (SimulationRunner.cpp:449-452).  To reproduce those FP64 sums exactly
-// the spread is a 4-kernel bucket sort: count contributions per site (first
-// toucher lists the site), allocate per-site pool ranges, scatter
-// (marker, contribution) records, then per site sort by marker and add in
-// that order.  Atomics only count; no floating-point atomics.
+// Ordered spread (FP64 lattices).  The reference adds delta*F into each site
+// in marker-id order (SimulationRunner.cpp:449-452).  To reproduce those FP64
+// sums exactly, markers are binned by the 16x16 tile of their position (count,
+// scan, scatter), and one workgroup per tile gathers the markers of the 3x3
+// tiles around it, ranks them by index and adds each site's contributions in
+// that order.  Atomics only count; no floating-point atomics.  Mixed-precision
+// lattices take an order-free fixed-point sum instead (see k_spread_fixed).
 #include <algorithm>
 #include <climits>
 #include "lbib_device_math.h"
@@ -144,31 +145,29 @@ __global__ void k_tile_scatter(int ntx, const double* __restrict__ mx, const dou
     tpos[e] = (uint32_t)fx | ((uint32_t)fy << 16);   // markers lie inside the lattice: 0 <= fx, fy < 65536
 }
 
-// One workgroup per tile; tiles no marker stencil reaches return at once.
-// The gathered markers are ranked by index and staged SP_CH at a time with
-// their eight delta factors; each staged marker sets its bit in a per-site
-// mask of the sites its stencil covers in this tile, and every lane then
-// walks only the set bits of its site's mask, lowest first, so it adds
-// delta*F of exactly the covering markers in rank order.
-// F32OUT: mixed-precision lattices take the force as FP32 pairs in Fx's
-// allocation (the sums themselves are FP64 either way).
-template <bool F32OUT>
-__global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
-                                                     const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
-                                                     const int32_t* __restrict__ tlist,
-                                                     const uint32_t* __restrict__ tpos, const double* __restrict__ mx,
-                                                     const double* __restrict__ my, const double* __restrict__ mfx,
-                                                     const double* __restrict__ mfy, double* Fx, double* Fy,
-                                                     DevStatus* st) {
-    static_assert(SP_CH == 128, "two 64-bit site masks per staged chunk");
-    const int t = (int)xcd_contiguous(blockIdx.x, gridDim.x);
-    if (!tact[t] || step_stopped(st)) return;
-    __shared__ int32_t idx[SP_IDX];   // gathered, then in rank order
-    __shared__ double fxa[SP_CH][4], fya[SP_CH][4], sfx[SP_CH], sfy[SP_CH];
-    __shared__ int32_t s0x[SP_CH], s0y[SP_CH];
-    __shared__ unsigned long long smask[2][ST * ST];
-    __shared__ uint32_t hcnt[9], hoff[9], hpre[10];
-    __shared__ uint32_t n;
+// Shared staging of one tile's spread.
+struct SpreadLds {
+    int32_t idx[SP_IDX];   // gathered, then in rank order
+    double fxa[SP_CH][4], fya[SP_CH][4], sfx[SP_CH], sfy[SP_CH];
+    int32_t s0x[SP_CH], s0y[SP_CH];
+    unsigned long long smask[2][ST * ST];
+    uint32_t hcnt[9], hoff[9], hpre[10];
+    uint32_t n;
+};
+
+// One tile of the ordered spread.  The gathered markers are ranked by index
+// and staged SP_CH at a time with their eight delta factors; each staged
+// marker sets its bit in a per-site mask of the sites its stencil covers in
+// this tile, and every lane then walks only the set bits of its site's mask,
+// lowest first, so it adds delta*F of exactly the covering markers in rank
+// order.  Called by the whole workgroup with a workgroup-uniform tile;
+// returns false when the tile's marker count exceeds SP_IDX.
+__device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny, int ntx, int nty,
+                                            const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
+                                            const int32_t* __restrict__ tlist, const uint32_t* __restrict__ tpos,
+                                            const double* __restrict__ mx, const double* __restrict__ my,
+                                            const double* __restrict__ mfx, const double* __restrict__ mfy,
+                                            double* Fx, double* Fy) {
     const int tx = t % ntx, ty = t / ntx;
     const int bx = tx * ST, by = ty * ST;
     // the 3x3 home tiles' lists, loaded at once
@@ -176,38 +175,35 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
         const int hx = tx + (int)(threadIdx.x % 3) - 1, hy = ty + (int)(threadIdx.x / 3) - 1;
         const bool in = hx >= 0 && hy >= 0 && hx < ntx && hy < nty;
         const int h = hy * ntx + hx;
-        hcnt[threadIdx.x] = in ? tcnt[h] : 0u;
-        hoff[threadIdx.x] = in ? toff[h] : 0u;
+        L.hcnt[threadIdx.x] = in ? tcnt[h] : 0u;
+        L.hoff[threadIdx.x] = in ? toff[h] : 0u;
     }
-    if (threadIdx.x == 0) n = 0;
-    smask[0][threadIdx.x] = 0ull;
-    smask[1][threadIdx.x] = 0ull;
+    if (threadIdx.x == 0) L.n = 0;
+    L.smask[0][threadIdx.x] = 0ull;
+    L.smask[1][threadIdx.x] = 0ull;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = 0;
-        for (int h = 0; h < 9; h++) { hpre[h] = a; a += hcnt[h]; }
-        hpre[9] = a;
+        for (int h = 0; h < 9; h++) { L.hpre[h] = a; a += L.hcnt[h]; }
+        L.hpre[9] = a;
     }
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < hpre[9]; j += blockDim.x) {
+    for (uint32_t j = threadIdx.x; j < L.hpre[9]; j += blockDim.x) {
         int h = 0;
-        while (j >= hpre[h + 1]) h++;
-        const uint32_t e = hoff[h] + (j - hpre[h]);
+        while (j >= L.hpre[h + 1]) h++;
+        const uint32_t e = L.hoff[h] + (j - L.hpre[h]);
         const int k = tlist[e];
         // keep only markers whose 4x4 stencil overlaps this tile (the binning
         // stored floor(X), floor(Y) with the list entry)
         const uint32_t pp = tpos[e];
         const int fx = (int)(pp & 0xffffu), fy = (int)(pp >> 16);
         if (fx + 2 < bx || fx - 1 > bx + ST - 1 || fy + 2 < by || fy - 1 > by + ST - 1) continue;
-        const uint32_t p = atomicAdd(&n, 1u);
-        if (p < (uint32_t)SP_IDX) idx[p] = k;
+        const uint32_t p = atomicAdd(&L.n, 1u);
+        if (p < (uint32_t)SP_IDX) L.idx[p] = k;
     }
     __syncthreads();
-    const uint32_t m = n;
-    if (m > (uint32_t)SP_IDX) {
-        if (threadIdx.x == 0) atomicOr(&st->bits, LBIB_ERR_CAPACITY);
-        return;
-    }
+    const uint32_t m = L.n;
+    if (m > (uint32_t)SP_IDX) return false;
     // rank sort in place: ranks into registers, then one scatter (the
     // workgroup is 256 wide, so each lane ranks at most SP_IDX / 256 markers)
     constexpr int RPL = SP_IDX / 256;
@@ -216,16 +212,16 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
 #pragma unroll
     for (int q = 0; q < RPL; q++) {
         const uint32_t j = threadIdx.x + 256u * q;
-        rv[q] = j < m ? idx[j] : 0;
+        rv[q] = j < m ? L.idx[j] : 0;
         uint32_t r = 0;
         if (j < m)
-            for (uint32_t i = 0; i < m; i++) r += (idx[i] < rv[q]) ? 1u : 0u;
+            for (uint32_t i = 0; i < m; i++) r += (L.idx[i] < rv[q]) ? 1u : 0u;
         rr[q] = r;
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RPL; q++)
-        if (threadIdx.x + 256u * q < m) idx[rr[q]] = rv[q];
+        if (threadIdx.x + 256u * q < m) L.idx[rr[q]] = rv[q];
     __syncthreads();
     const int lx = (int)(threadIdx.x % ST), ly = (int)(threadIdx.x / ST);
     const int px = bx + lx, py = by + ly;
@@ -233,46 +229,216 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
     for (uint32_t c0 = 0; c0 < m; c0 += SP_CH) {
         const uint32_t mc = min((uint32_t)SP_CH, m - c0);
         for (uint32_t j = threadIdx.x; j < mc; j += blockDim.x) {
-            const int k = idx[c0 + j];
+            const int k = L.idx[c0 + j];
             const double X = mx[k], Y = my[k];
             const int x0 = (int)floor(X) - 1, y0 = (int)floor(Y) - 1;
-            s0x[j] = x0;
-            s0y[j] = y0;
-            ib_factors(x0, X, fxa[j]);
-            ib_factors(y0, Y, fya[j]);
-            sfx[j] = mfx[k];
-            sfy[j] = mfy[k];
+            L.s0x[j] = x0;
+            L.s0y[j] = y0;
+            ib_factors(x0, X, L.fxa[j]);
+            ib_factors(y0, Y, L.fya[j]);
+            L.sfx[j] = mfx[k];
+            L.sfy[j] = mfy[k];
             const unsigned long long bit = 1ull << (j & 63u);
             for (int qy = 0; qy < 4; qy++)
                 for (int qx = 0; qx < 4; qx++) {
                     const int sx = x0 + qx - bx, sy = y0 + qy - by;
                     if ((unsigned)sx < (unsigned)ST && (unsigned)sy < (unsigned)ST)
-                        atomicOr(&smask[j >> 6][sy * ST + sx], bit);
+                        atomicOr(&L.smask[j >> 6][sy * ST + sx], bit);
                 }
         }
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < 2; h++) {
-            unsigned long long mk = smask[h][threadIdx.x];
+            unsigned long long mk = L.smask[h][threadIdx.x];
             while (mk) {
                 const int j = 64 * h + __ffsll((long long)mk) - 1;
                 mk &= mk - 1ull;
-                const double dd = (fxa[j][px - s0x[j]] * fya[j][py - s0y[j]]) / 16.0;
-                ax += dd * sfx[j];
-                ay += dd * sfy[j];
+                const double dd = (L.fxa[j][px - L.s0x[j]] * L.fya[j][py - L.s0y[j]]) / 16.0;
+                ax += dd * L.sfx[j];
+                ay += dd * L.sfy[j];
             }
-            smask[h][threadIdx.x] = 0ull;
+            L.smask[h][threadIdx.x] = 0ull;
         }
         __syncthreads();
     }
     if (px < nx && py < ny) {
         const long site = (long)py * nx + px;
-        if (F32OUT) {
-            reinterpret_cast<float2*>(Fx)[site] = make_float2((float)ax, (float)ay);
-        } else {
-            Fx[site] = ax;
-            Fy[site] = ay;
+        Fx[site] = ax;
+        Fy[site] = ay;
+    }
+    return true;
+}
+
+// One workgroup per tile; tiles no marker stencil reaches return at once.
+// MAP 0: consecutive tiles on one XCD (xcd_contiguous), for one lattice whose
+// active tiles spread evenly; MAP 1: tile rows dealt to the XCDs in turn
+// (row_interleaved_tile), for a batched atlas whose occupied rectangles would
+// otherwise load some XCDs far more than others.
+template <int MAP>
+__global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
+                                                     const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
+                                                     const int32_t* __restrict__ tlist,
+                                                     const uint32_t* __restrict__ tpos, const double* __restrict__ mx,
+                                                     const double* __restrict__ my, const double* __restrict__ mfx,
+                                                     const double* __restrict__ mfy, double* Fx, double* Fy,
+                                                     DevStatus* st) {
+    static_assert(SP_CH == 128, "two 64-bit site masks per staged chunk");
+    const int t = MAP == 0 ? (int)xcd_contiguous(blockIdx.x, gridDim.x) : row_interleaved_tile(blockIdx.x, ntx, nty);
+    if (t < 0 || !tact[t] || step_stopped(st)) return;
+    __shared__ SpreadLds L;
+    if (!spread_tile(L, t, nx, ny, ntx, nty, toff, tcnt, tlist, tpos, mx, my, mfx, mfy, Fx, Fy) &&
+        threadIdx.x == 0)
+        atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+}
+
+// ---- mixed-precision spread ------------------------------------------------
+// Lattices stored in FP32 or 16-bit formats do not reproduce the reference's
+// FP64 sums anyway, so their force field is summed without an order: each
+// tile lists every marker whose stencil reaches it (with its position and
+// force), and a workgroup per tile adds the FP32 products delta*F as int64
+// multiples of 2^-36 with LDS atomics.  Integer addition is associative, so
+// the result does not depend on the order of the atomics (runs are
+// reproducible); the quantum 2^-36 is far below FP32 resolution of the
+// forces involved.
+constexpr float kFixScale = 68719476736.0f;    // 2^36
+constexpr double kFixInv = 1.0 / 68719476736.0;
+
+// fastCos (GeometryNode.cpp:28-45) in FP32
+__device__ __forceinline__ float fast_cos_f32(float x) {
+    float t = x + 1.57079632679f;
+    if (t > 3.14159265359f) t -= 6.28318530718f;
+    float c = t < 0.0f ? 1.27323954f * t + 0.405284735f * t * t : 1.27323954f * t - 0.405284735f * t * t;
+    c = c < 0.0f ? 0.225f * (c * -c - c) + c : 0.225f * (c * c - c) + c;
+    return c;
+}
+
+// The tiles a marker's 4x4 stencil reaches: quadrant q takes the tile of the
+// stencil's corner (q & 1: right column, q & 2: top row) when it differs from
+// the tiles of the lower quadrants.
+struct StencilTiles {
+    int tx0, tx1, ty0, ty1;
+    __device__ StencilTiles(int x0, int y0) : tx0(x0 / ST), tx1((x0 + 3) / ST), ty0(y0 / ST), ty1((y0 + 3) / ST) {}
+    __device__ bool has(int q) const { return ((q & 1) == 0 || tx1 != tx0) && ((q & 2) == 0 || ty1 != ty0); }
+    __device__ int tile(int q, int ntx) const { return ((q & 2) ? ty1 : ty0) * ntx + ((q & 1) ? tx1 : tx0); }
+};
+
+// Counts each marker into the tiles its stencil reaches (tcnt, zeroed by
+// k_marker_force) and keeps its slot per quadrant; runs of lanes with the
+// same tile in a wavefront take one atomic.
+__global__ void k_push_count(int ntx, const double* __restrict__ mx, const double* __restrict__ my, uint32_t* tcnt,
+                             uint32_t* pslot, int pcap, uint8_t* tact, const DevStatus* st) {
+    if (step_stopped(st)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = k < (int)st->nm;
+    int x0 = 0, y0 = 0;
+    if (live) {
+        x0 = (int)floor(mx[k]) - 1;
+        y0 = (int)floor(my[k]) - 1;
+    }
+    const StencilTiles T(x0, y0);
+    const int lane = threadIdx.x & 63;
+    const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const bool part = live && T.has(q);
+        const int key = part ? T.tile(q, ntx) : -1 - lane;   // lanes not taking part end every run
+        const int kprev = __shfl_up(key, 1, 64);
+        const bool head = lane == 0 || kprev != key;
+        const unsigned long long heads = __ballot(head);
+        const int hl = 63 - __clzll((long long)(heads & le));
+        const unsigned long long gt = heads & ~le;
+        const int nextl = gt ? __ffsll((long long)gt) - 1 : 64;
+        uint32_t base = 0;
+        if (head && part) base = atomicAdd(&tcnt[key], (uint32_t)(nextl - lane));
+        base = __shfl(base, hl, 64);
+        if (part) {
+            pslot[(size_t)q * pcap + k] = base + (uint32_t)(lane - hl);
+            tact[key] = 1;
         }
+    }
+}
+
+__global__ void k_push_scatter(int ntx, const double* __restrict__ mx, const double* __restrict__ my,
+                               const double* __restrict__ mfx, const double* __restrict__ mfy,
+                               const uint32_t* __restrict__ toff, const uint32_t* __restrict__ pslot, int pcap,
+                               double* ex, double* ey, double* efx, double* efy, const DevStatus* st) {
+    if (step_stopped(st)) return;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int)st->nm) return;
+    const double X = mx[k], Y = my[k], fx = mfx[k], fy = mfy[k];
+    const StencilTiles T((int)floor(X) - 1, (int)floor(Y) - 1);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (!T.has(q)) continue;
+        const uint32_t e = toff[T.tile(q, ntx)] + pslot[(size_t)q * pcap + k];
+        ex[e] = X;
+        ey[e] = Y;
+        efx[e] = fx;
+        efy[e] = fy;
+    }
+}
+
+template <int MAP>
+__global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, int nty, const uint32_t* __restrict__ toff,
+                                                      const uint32_t* __restrict__ tcnt, const double* __restrict__ ex,
+                                                      const double* __restrict__ ey, const double* __restrict__ efx,
+                                                      const double* __restrict__ efy, float2* F, DevStatus* st) {
+    const int t = MAP == 0 ? (int)xcd_contiguous(blockIdx.x, gridDim.x) : row_interleaved_tile(blockIdx.x, ntx, nty);
+    if (t < 0 || step_stopped(st)) return;
+    const uint32_t n = tcnt[t];
+    if (n == 0) return;
+    const uint32_t o = toff[t];
+    __shared__ unsigned long long acc[2][ST * ST];
+    __shared__ uint32_t bad[ST * ST];   // a non-finite or out-of-range product reached the site
+    acc[0][threadIdx.x] = 0ull;
+    acc[1][threadIdx.x] = 0ull;
+    bad[threadIdx.x] = 0u;
+    __syncthreads();
+    const int tx = t % ntx, ty = t / ntx;
+    const int bx = tx * ST, by = ty * ST;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double X = ex[o + i], Y = ey[o + i];
+        const int x0 = (int)floor(X) - 1, y0 = (int)floor(Y) - 1;
+        const float rx = (float)(X - (double)x0), ry = (float)(Y - (double)y0);
+        float fa[4], fb[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            fa[q] = 1.0f + fast_cos_f32(1.57079632679f * ((float)q - rx));
+            fb[q] = 1.0f + fast_cos_f32(1.57079632679f * ((float)q - ry));
+        }
+        // delta = fa * fb / 16: the 1/16 goes into the scale
+        const float gx = (float)efx[o + i] * (kFixScale / 16.0f), gy = (float)efy[o + i] * (kFixScale / 16.0f);
+#pragma unroll
+        for (int qy = 0; qy < 4; qy++) {
+            const int sy = y0 + qy - by;
+            if ((unsigned)sy >= (unsigned)ST) continue;
+#pragma unroll
+            for (int qx = 0; qx < 4; qx++) {
+                const int sx = x0 + qx - bx;
+                if ((unsigned)sx >= (unsigned)ST) continue;
+                const float w = fa[qx] * fb[qy];
+                const float vx = w * gx, vy = w * gy;
+                const int site = sy * ST + sx;
+                if (!(fabsf(vx) < 4.0e18f && fabsf(vy) < 4.0e18f)) {
+                    bad[site] = 1u;
+                    continue;
+                }
+                atomicAdd(&acc[0][site], (unsigned long long)__float2ll_rn(vx));
+                atomicAdd(&acc[1][site], (unsigned long long)__float2ll_rn(vy));
+            }
+        }
+    }
+    __syncthreads();
+    const int px = bx + (int)(threadIdx.x % ST), py = by + (int)(threadIdx.x / ST);
+    if (px < nx && py < ny) {
+        float2 v;
+        if (bad[threadIdx.x]) {
+            v = make_float2(__int_as_float(0x7fc00000), __int_as_float(0x7fc00000));
+        } else {
+            v = make_float2((float)((double)(long long)acc[0][threadIdx.x] * kFixInv),
+                            (float)((double)(long long)acc[1][threadIdx.x] * kFixInv));
+        }
+        F[(long)py * nx + px] = v;
     }
 }
 
@@ -366,17 +532,40 @@ int launch_force_and_spread(lbib_ctx* c) {
         LBIB_HIP(hipGetLastError());
         return 0;
     }
+    // a batched atlas deals tile rows to the XCDs in turn (its occupied
+    // rectangles are unevenly spread); one lattice keeps tiles together
+    const int map = c->nrect > 0 ? 1 : 0;
+    const int grid = map ? 8 * ((c->sty + 7) / 8) * c->stx : ntiles;
+    if (c->prec != LBIB_FP64) {
+        k_push_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->tcnt, c->pslot, c->mcap,
+                                                                 c->tact, c->st);
+        if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;
+        k_push_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->mfx, c->mfy, c->toff,
+                                                                   c->pslot, c->mcap, c->pe_x, c->pe_y, c->pe_fx,
+                                                                   c->pe_fy, c->st);
+        float2* F = reinterpret_cast<float2*>(c->Fx);
+        if (map)
+            k_spread_fixed<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->toff, c->tcnt, c->pe_x,
+                                                          c->pe_y, c->pe_fx, c->pe_fy, F, c->st);
+        else
+            k_spread_fixed<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->toff, c->tcnt, c->pe_x,
+                                                          c->pe_y, c->pe_fx, c->pe_fy, F, c->st);
+        LBIB_HIP(hipGetLastError());
+        return 0;
+    }
     k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
                                                              c->tact, c->st);
     if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;
     k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist, c->tpos,
                                                                c->st);
-    if (c->prec == LBIB_FP64)
-        k_tile_spread<false><<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                            c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
+    if (map)
+        k_tile_spread<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
+                                                            c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx,
+                                                            c->Fy, c->st);
     else
-        k_tile_spread<true><<<ntiles, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                           c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->st);
+        k_tile_spread<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
+                                                            c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx,
+                                                            c->Fy, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

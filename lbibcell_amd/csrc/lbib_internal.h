@@ -165,6 +165,10 @@ struct lbib_ctx {
     uint8_t* tact = nullptr;
     int32_t* tlist = nullptr;
     uint32_t* tpos = nullptr;          // floor(X) | floor(Y) << 16 of each tlist entry
+    // mixed-precision spread: every tile a marker's stencil reaches lists the
+    // marker with its position and force (at most 4 entries per marker)
+    uint32_t* pslot = nullptr;         // [4][mcap] slot of the marker in each reached tile
+    double *pe_x = nullptr, *pe_y = nullptr, *pe_fx = nullptr, *pe_fy = nullptr;   // [4 * mcap]
     // markers
     int mcap = 0, nm = 0;
     double *mx = nullptr, *my = nullptr, *mvx = nullptr, *mvy = nullptr;
