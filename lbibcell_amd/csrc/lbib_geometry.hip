@@ -52,6 +52,9 @@ __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ c
         st->nchanged = 0;
         st->rm_total = 0;
         st->co_removed = 0;
+        st->rm_any = 0;
+        st->rm_L = 0;
+        st->topo_changed = 0;
         st->reinit_levels = 0;
     }
 }
@@ -481,7 +484,7 @@ static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
     }
 }
 
-int launch_topology(lbib_ctx* c, bool coarsen);
+int launch_topology(lbib_ctx* c, bool coarsen, bool in_step);
 
 int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     // mode: 0 = moveLattice (local update + reinit), 1 = cureLattice (scanline only),
@@ -521,18 +524,20 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     k_scanline<<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, c->tag, c->bn_dom, c->st);
     LBIB_HIP(hipGetLastError());
     if (cure) return 0;
-    return launch_topology(c, coarsen);
+    return launch_topology(c, coarsen, true);
 }
 
 // remeshBoundary, then at it % 10 == 0 deleteAllForces + coarsenBoundary
 // (SimulationRunner.cpp:83-87); rings are rebuilt for the area helpers.
-int launch_topology(lbib_ctx* c, bool coarsen) {
+int launch_topology(lbib_ctx* c, bool coarsen, bool in_step) {
     if (c->nc == 0) return 0;
-    if (launch_remesh(c, 1)) return -1;
-    if (launch_rings(c, true)) return -1;
+    if (launch_remesh(c, 1, in_step)) return -1;
+    // the ring order is rebuilt only when remeshing or coarsening changed
+    // the topology in this step (st->topo_changed)
+    if (launch_rings(c, true, true)) return -1;
     if (coarsen) {
         if (launch_coarsen(c)) return -1;
-        if (launch_rings(c)) return -1;
+        if (launch_rings(c, false, true)) return -1;
     }
     return 0;
 }

@@ -60,6 +60,8 @@ struct DevStatus {
     uint32_t step_it;       // iteration of the last step begun with a clear status
     uint32_t rm_nc0, rm_nm0;   // counts before the current remesh round
     uint32_t reinit_levels;    // dependency levels of this step's CDE re-initialisation
+    uint32_t rm_any;           // the first remesh round of this step found a long connection
+    uint32_t topo_changed;     // remesh or coarsening changed the topology in this step
 };
 
 // A stopped status makes later kernels return without work.  Geometry and
@@ -360,12 +362,13 @@ int launch_interp_move(lbib_ctx* c);
 int launch_geometry(lbib_ctx* c, uint32_t it, int mode);
 int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor);
 int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
-                   uint32_t* total);
+                   uint32_t* total, const uint32_t* gate = nullptr);
 // exclusive scan of the per-tile marker counts (tcnt -> toff); for short tile
 // arrays also the number of nonzero active flags into *nactive
 int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive);
 int launch_inout(lbib_ctx* c);
-int launch_rings(lbib_ctx* c, bool counters_cleared = false);
+// gated: skip (keeping the current ring order) unless st->topo_changed
+int launch_rings(lbib_ctx* c, bool counters_cleared = false, bool gated = false);
 // per-domain inputs and outputs of the area regulator's mass source
 struct AreaMass {
     const uint32_t* celltype;
@@ -376,7 +379,9 @@ struct AreaMass {
 // mass != nullptr: also the area regulator's mass table, skipped while the
 // status is stopped (an iteration's BioSolver stage)
 int launch_ring_areas(lbib_ctx* c, const AreaMass* mass);
-int launch_remesh(lbib_ctx* c, int rounds);   // also clears the ring counters
+// also clears the ring counters when the topology changed; in_step: the
+// geometry stage's own pass (its scan is skipped when no connection is long)
+int launch_remesh(lbib_ctx* c, int rounds, bool in_step = false);
 // per-domain sum of one species' concentration into acc[ndom] (zeroed by the caller)
 int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc);
 // a live context with any deferred step begin enqueued (-1: null context)
@@ -411,7 +416,7 @@ constexpr int JUNCTION_MAXIN = 16;
 int launch_forces_junctions(lbib_ctx* c, double radius, double k, double l0);
 // markers binned by the 16x16 tile of floor(X), floor(Y) (tcnt/toff/tlist)
 int launch_tile_bins(lbib_ctx* c);
-int launch_topology(lbib_ctx* c, bool coarsen);
+int launch_topology(lbib_ctx* c, bool coarsen, bool in_step = false);
 // FP64 <-> storage conversion of the current buffers (mixed precision only)
 int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq);
 int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq);

@@ -31,8 +31,10 @@ __device__ __forceinline__ bool topo_halted(const DevStatus* st) { return (st->b
 // Exclusive scan of n (device-resident count) uint32 values, blocks of 1024.
 constexpr int SCAN_B = 1024;
 
+// gate != nullptr: *gate == 0 skips the whole scan (outputs and total untouched)
 __global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* __restrict__ in, const uint32_t* nptr, uint32_t nfix,
-                                                     uint32_t* __restrict__ bsum) {
+                                                     uint32_t* __restrict__ bsum, const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     const uint32_t n = nptr ? *nptr : nfix;
     const uint32_t b0 = blockIdx.x * SCAN_B;
     __shared__ uint32_t wsum[16];
@@ -48,7 +50,8 @@ __global__ __launch_bounds__(256) void k_scan_blocks(const uint32_t* __restrict_
 }
 
 // single block: exclusive scan of the block sums; total -> *total
-__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* bsum, uint32_t nb, uint32_t* total) {
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* bsum, uint32_t nb, uint32_t* total, const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     __shared__ uint32_t wsum[16];
     uint32_t carry = 0;
     for (uint32_t base = 0; base < nb; base += 1024) {
@@ -63,7 +66,9 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* bsum, uint32_t nb, 
 }
 
 __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, const uint32_t* nptr, uint32_t nfix,
-                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ out) {
+                                                    const uint32_t* __restrict__ bsum, uint32_t* __restrict__ out,
+                                                    const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     const uint32_t n = nptr ? *nptr : nfix;
     const uint32_t b0 = blockIdx.x * SCAN_B;
     if (b0 >= n) return;
@@ -94,7 +99,9 @@ constexpr uint32_t SCAN_SINGLE_MAX = 1u << 16;
 // stores from LDS.
 __global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict__ in, const uint32_t* nptr,
                                                       uint32_t nfix, uint32_t* __restrict__ out, uint32_t* total,
-                                                      const uint8_t* __restrict__ nz, uint32_t* nz_total) {
+                                                      const uint8_t* __restrict__ nz, uint32_t* nz_total,
+                                                      const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     constexpr int PER = 16, PASS = 1024 * PER, ROW = PER + 1;
     const uint32_t n = nptr ? *nptr : nfix;
     __shared__ uint32_t wsum[16];
@@ -144,17 +151,17 @@ __global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict
 int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive) {
     const uint32_t nt = (uint32_t)c->stx * (uint32_t)c->sty;
     if (nt > SCAN_SINGLE_MAX) return scan_exclusive(c, c->tcnt, c->toff, nullptr, nt, nt, nullptr);
-    k_scan_single<<<1, 1024, 0, c->stream>>>(c->tcnt, nullptr, nt, c->toff, nullptr, active, nactive);
+    k_scan_single<<<1, 1024, 0, c->stream>>>(c->tcnt, nullptr, nt, c->toff, nullptr, active, nactive, nullptr);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
 
 int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
-                   uint32_t* total) {
+                   uint32_t* total, const uint32_t* gate) {
     const uint32_t nb = (cap + SCAN_B - 1) / SCAN_B;
     if (nb == 0) return 0;
     if (cap <= SCAN_SINGLE_MAX) {
-        k_scan_single<<<1, 1024, 0, c->stream>>>(in, nptr, nfix, out, total, nullptr, nullptr);
+        k_scan_single<<<1, 1024, 0, c->stream>>>(in, nptr, nfix, out, total, nullptr, nullptr, gate);
         LBIB_HIP(hipGetLastError());
         return 0;
     }
@@ -162,9 +169,9 @@ int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_
         c->err = "scan scratch too small";
         return -1;
     }
-    k_scan_blocks<<<nb, 256, 0, c->stream>>>(in, nptr, nfix, c->scan_tmp);
-    k_scan_top<<<1, 1024, 0, c->stream>>>(c->scan_tmp, nb, total);
-    k_scan_apply<<<nb, 256, 0, c->stream>>>(in, nptr, nfix, c->scan_tmp, out);
+    k_scan_blocks<<<nb, 256, 0, c->stream>>>(in, nptr, nfix, c->scan_tmp, gate);
+    k_scan_top<<<1, 1024, 0, c->stream>>>(c->scan_tmp, nb, total, gate);
+    k_scan_apply<<<nb, 256, 0, c->stream>>>(in, nptr, nfix, c->scan_tmp, out, gate);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
@@ -190,14 +197,15 @@ __device__ __forceinline__ bool needs_perturbation(const double X, const double 
 }
 
 // flag != nullptr: mark this round's long connections (and the perturbation
-// halt of their midpoints); flag == nullptr: only raise LBIB_FLAG_REMESH if a
-// connection is still long after the rounds of this pass.  ring_start != nullptr:
-// also clear the per-domain ring counters for the ring rebuild that follows
-// (always, like k_ring_reset).
+// halt of their midpoints), and note in rm_any that there was one;
+// flag == nullptr: only raise LBIB_FLAG_REMESH if a connection is still long
+// after the rounds of this pass.  ring_start != nullptr: also clear the
+// per-domain ring counters for the ring rebuild that follows, when the
+// topology changed (like k_ring_reset).
 __global__ void k_rm_flag(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb, const double* __restrict__ mx,
                           const double* __restrict__ my, uint32_t* flag, int ndom, int32_t* ring_start,
                           uint32_t* ring_cnt, DevStatus* st) {
-    if (ring_start)
+    if (ring_start && st->topo_changed)
         for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < ndom; d += gridDim.x * blockDim.x) {
             ring_start[d] = INT_MAX;
             ring_cnt[d] = 0;
@@ -216,6 +224,7 @@ __global__ void k_rm_flag(const int32_t* __restrict__ ca, const int32_t* __restr
             atomicOr(&st->bits, LBIB_HALT_PERTURB_REMESH);
     }
     if (!flag && __any(any) && (threadIdx.x & 63) == 0) atomicOr(&st->bits, LBIB_FLAG_REMESH);
+    if (flag && __any(any) && (threadIdx.x & 63) == 0) atomicOr(&st->rm_any, 1u);
 }
 
 struct TopoArrays {
@@ -291,6 +300,7 @@ __global__ void k_rm_commit(TopoArrays t, int32_t* min_c, int32_t* mout_c, DevSt
         t.fs_off[nm + 1 + r] = t.fs_off[nm];
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->rm_total += L;
+        st->topo_changed = 1u;
         st->nm = nm + L;
         st->nc = nn;
         st->maxid += L;
@@ -298,7 +308,9 @@ __global__ void k_rm_commit(TopoArrays t, int32_t* min_c, int32_t* mout_c, DevSt
 }
 
 // ------------------------------------------------------------- rings -------
-__global__ void k_ring_reset(int ndom, int32_t* start, uint32_t* cnt) {
+// gate != nullptr: rebuild only when *gate (the step changed the topology)
+__global__ void k_ring_reset(int ndom, int32_t* start, uint32_t* cnt, const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= ndom) return;
     start[d] = INT_MAX;
@@ -307,7 +319,8 @@ __global__ void k_ring_reset(int ndom, int32_t* start, uint32_t* cnt) {
 // Connections of one ring mostly sit in contiguous runs of the vector: the
 // head lane of each run inside a wavefront does one atomic for the whole run.
 __global__ void k_ring_count(const uint32_t* __restrict__ cdom, int32_t* start, uint32_t* cnt, uint32_t* slot,
-                             int ndom, DevStatus* st) {
+                             int ndom, DevStatus* st, const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     const uint32_t nc = st->nc;
     const uint32_t stride = gridDim.x * blockDim.x;
     const int lane = threadIdx.x & 63;
@@ -334,7 +347,8 @@ __global__ void k_ring_count(const uint32_t* __restrict__ cdom, int32_t* start, 
     }
 }
 __global__ void k_ring_bucket(const uint32_t* __restrict__ cdom, const uint32_t* __restrict__ slot,
-                              const uint32_t* __restrict__ roff, int32_t* bucket, DevStatus* st) {
+                              const uint32_t* __restrict__ roff, int32_t* bucket, DevStatus* st, const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     const uint32_t nc = st->nc;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
         bucket[roff[cdom[c]] + slot[c]] = (int32_t)c;
@@ -347,7 +361,8 @@ __global__ __launch_bounds__(1024) void k_ring_order(const int32_t* __restrict__
                                                      const uint32_t* __restrict__ roff, const int32_t* __restrict__ bucket,
                                                      const uint32_t* __restrict__ slot, const int32_t* __restrict__ cb,
                                                      const int32_t* __restrict__ mout_c, int32_t* ring_marker,
-                                                     DevStatus* st) {
+                                                     DevStatus* st, const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
     const int d = blockIdx.x;
     const uint32_t n = cnt[d];
     if (n == 0) return;
@@ -592,7 +607,10 @@ __global__ __launch_bounds__(256) void k_co_ring(const CoarsenArgs a, DevStatus*
             removed++;
         }
     }
-    if (lane == 0 && removed) atomicAdd(&st->co_removed, removed);
+    if (lane == 0 && removed) {
+        atomicAdd(&st->co_removed, removed);
+        atomicOr(&st->topo_changed, 1u);
+    }
 }
 
 // keep flags for the compaction scans
@@ -782,17 +800,19 @@ int launch_inout(lbib_ctx* c) {
     return 0;
 }
 
-int launch_rings(lbib_ctx* c, bool counters_cleared) {
+int launch_rings(lbib_ctx* c, bool counters_cleared, bool gated) {
     if (c->ndom == 0) return 0;
+    const uint32_t* gate = gated ? &c->st->topo_changed : nullptr;
     if (!counters_cleared)
-        k_ring_reset<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->ring_start, c->ring_cnt);
+        k_ring_reset<<<(c->ndom + 255) / 256, 256, 0, c->stream>>>(c->ndom, c->ring_start, c->ring_cnt, gate);
     k_ring_count<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->cdom, c->ring_start, c->ring_cnt, c->ring_slot, c->ndom,
-                                                          c->st);
-    if (scan_exclusive(c, c->ring_cnt, c->ring_off_u, nullptr, (uint32_t)c->ndom, (uint32_t)c->ndom, nullptr)) return -1;
+                                                          c->st, gate);
+    if (scan_exclusive(c, c->ring_cnt, c->ring_off_u, nullptr, (uint32_t)c->ndom, (uint32_t)c->ndom, nullptr, gate))
+        return -1;
     k_ring_bucket<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->cdom, c->ring_slot, c->ring_off_u, c->ring_bucket,
-                                                           c->st);
+                                                           c->st, gate);
     k_ring_order<<<c->ndom, 1024, 0, c->stream>>>(c->ring_start, c->ring_cnt, c->ring_off_u, c->ring_bucket,
-                                                  c->ring_slot, c->cb, c->mout_c, c->ring_marker, c->st);
+                                                  c->ring_slot, c->cb, c->mout_c, c->ring_marker, c->st, gate);
     LBIB_HIP(hipGetLastError());
     return 0;
 }
@@ -809,12 +829,15 @@ int launch_ring_areas(lbib_ctx* c, const AreaMass* mass) {
 // Rounds of remeshing; LBIB_FLAG_REMESH asks for another pass when a
 // connection is still long (a marker moves far less than MAXLENGTH per
 // iteration, so one round is the steady-state case).
-int launch_remesh(lbib_ctx* c, int rounds) {
+int launch_remesh(lbib_ctx* c, int rounds, bool in_step) {
     TopoArrays t = topo_arrays(c);
     for (int r = 0; r < rounds; r++) {
         k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, 0, nullptr,
                                                             nullptr, c->st);
-        if (scan_exclusive(c, c->flag_c, c->rank_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->rm_L)) return -1;
+        // the geometry stage's first round (rm_any and rm_L start at zero)
+        // skips the scan when no connection is long
+        const uint32_t* gate = (in_step && r == 0) ? &c->st->rm_any : nullptr;
+        if (scan_exclusive(c, c->flag_c, c->rank_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->rm_L, gate)) return -1;
         k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->nx, c->ny,
                                                              (uint32_t)c->mcap, (uint32_t)c->ccap, c->st);
         k_rm_commit<<<grid_for(2 * (long)c->ccap), 256, 0, c->stream>>>(t, c->min_c, c->mout_c, c->st);
