@@ -350,6 +350,9 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
         }
     }
     LBIB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    LBIB_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    LBIB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+    LBIB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
     const long n = c->n;
     const long npad = (n + 63) / 64 * 64;   // whole 64-site blocks for 32- and 16-bit storage
     for (int b = 0; b < 2; b++) {
@@ -436,6 +439,9 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1070,7 +1076,7 @@ int lbib_read_status(lbib_ctx* c, lbib_status* out) {
     LBIB_HIP(hipMemcpyAsync(c->st_host, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
     LBIB_HIP(hipStreamSynchronize(c->stream));
     const DevStatus& s = *c->st_host;
-    out->bits = s.bits;
+    out->bits = (s.bits & ERR_EDGE_REMESH) ? ((s.bits & ~ERR_EDGE_REMESH) | LBIB_ERR_EDGE) : s.bits;
     out->first_bad_marker = s.first_bad == INT_MAX ? -1 : s.first_bad;
     out->n_touched_sites = (int32_t)s.ntouched;
     out->n_boundary_nodes = (int32_t)s.nbn;

@@ -63,15 +63,23 @@ __device__ __forceinline__ double sqd(double x1, double y1, double x2, double y2
     return (x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2);   // EulerianPoint.hpp:88-96
 }
 
+// Every candidate carries what its winner needs (the connection's boundary
+// descriptor and, for one with a descriptor, the boundary-node velocity of
+// Connection::calculateVelocity, Connection.cpp:146-168), so that no later
+// kernel of the stage reads the connections or markers that the remesh
+// rewrites beside it.
 __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const int32_t* __restrict__ ca,
                           const int32_t* __restrict__ cb, const uint32_t* __restrict__ cdom,
-                          const double* __restrict__ mx, const double* __restrict__ my, Cand* cand,
+                          const int32_t* __restrict__ cbc, const double* __restrict__ mx,
+                          const double* __restrict__ my, const double* __restrict__ mvx,
+                          const double* __restrict__ mvy, int want_bc, Cand* cand,
                           int candcap, unsigned long long* best_dist, DevStatus* st) {
     if (halted(st)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = c < (int)st->nc;
     double x1 = 0, y1 = 0, x2 = 1, y2 = 1;
-    if (live) { x1 = mx[ca[c]]; y1 = my[ca[c]]; x2 = mx[cb[c]]; y2 = my[cb[c]]; }
+    int ia = 0, ib = 0;
+    if (live) { ia = ca[c]; ib = cb[c]; x1 = mx[ia]; y1 = my[ia]; x2 = mx[ib]; y2 = my[ib]; }
     const double maxX = fmax(x1, x2), maxY = fmax(y1, y2);
     const double minX = fmin(x1, x2), minY = fmin(y1, y2);
     const double m = (y2 - y1) / (x2 - x1);
@@ -97,6 +105,9 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
         return;
     }
     const uint32_t dom = cdom[c];
+    const int desc = want_bc ? cbc[c] : -1;
+    double vxa = 0, vya = 0, vxb = 0, vyb = 0;
+    if (desc >= 0) { vxa = mvx[ia]; vya = mvy[ia]; vxb = mvx[ib]; vyb = mvy[ib]; }
     int k = 0;
     auto put = [&](int sx, int sy, int slot, uint32_t d, double bx, double by) {
         Cand e;
@@ -111,7 +122,15 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
         e.dist2 = sqd((double)sx, (double)sy, bx, by);
         e.bx = bx;
         e.by = by;
-        e.conn = c;
+        e.desc = desc;
+        e.vx = 0.0;
+        e.vy = 0.0;
+        if (desc >= 0) {
+            const double l = sqrt(sqd(x1, y1, bx, by));
+            const double L = sqrt(sqd(x2, y2, x1, y1));
+            e.vx = (vxb - vxa) * l / L + vxa;
+            e.vy = (vyb - vya) * l / L + vya;
+        }
         e.win = 0;
         cand[base + k] = e;
         atomicMin(&best_dist[sel_index(e.site, slot)], (unsigned long long)__double_as_longlong(e.dist2));
@@ -135,7 +154,7 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
 
 __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned long long* __restrict__ best_dist,
                           uint32_t* best_seq, DevStatus* st) {
-    if (halted(st)) return;
+    if (halted_beside_topology(st)) return;
     const uint32_t nc = st->ncand;
     if (blockIdx.x == 0 && threadIdx.x == 0) st->ncand_prev = nc;   // this cure's reset list
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
@@ -153,13 +172,10 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
 // winner in the site's highest slot with a candidate (a set best distance),
 // which reports the site when its domain changes.
 __global__ __launch_bounds__(1024) void k_cut_win(const Cand* cand, long n, const unsigned long long* __restrict__ best_dist,
-                          const uint32_t* __restrict__ best_seq, const int32_t* __restrict__ ca,
-                          const int32_t* __restrict__ cb, const int32_t* __restrict__ cbc,
-                          const double* __restrict__ mx, const double* __restrict__ my,
-                          const double* __restrict__ mvx, const double* __restrict__ mvy, uint32_t* tag,
+                          const uint32_t* __restrict__ best_seq, uint32_t* tag,
                           uint32_t* bn_dom, double* bc_v, int32_t* bc_desc, uint32_t* changed, uint32_t* rmark,
                           uint8_t* rstate, int local_update, DevStatus* st) {
-    if (halted(st)) return;
+    if (halted_beside_topology(st)) return;
     const uint32_t nc = st->ncand;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
@@ -179,15 +195,11 @@ __global__ __launch_bounds__(1024) void k_cut_win(const Cand* cand, long n, cons
                 dom = e.domain;
                 bn_dom[slotix] = e.domain;
                 uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
-                const int desc = cbc[e.conn];
-                if (desc >= 0 && bc_v != nullptr) {
+                if (e.desc >= 0 && bc_v != nullptr) {
                     bits |= 1u << (BC_SHIFT + e.slot - 1);
-                    const int a = ca[e.conn], b = cb[e.conn];
-                    const double l = sqrt(sqd(mx[a], my[a], e.bx, e.by));
-                    const double L = sqrt(sqd(mx[b], my[b], mx[a], my[a]));
-                    bc_v[2 * slotix] = (mvx[b] - mvx[a]) * l / L + mvx[a];
-                    bc_v[2 * slotix + 1] = (mvy[b] - mvy[a]) * l / L + mvy[a];
-                    bc_desc[slotix] = desc;
+                    bc_v[2 * slotix] = e.vx;
+                    bc_v[2 * slotix + 1] = e.vy;
+                    bc_desc[slotix] = e.desc;
                 }
                 atomicOr(&tag[e.site], bits);
                 if (local_update && dom_now != e.domain) {
@@ -336,7 +348,7 @@ __device__ __forceinline__ bool reinit_ready(const ReinitArgs& a, uint32_t cnt, 
 
 template <int NS, int P>
 __global__ __launch_bounds__(256) void k_reinit_level(const ReinitArgs a, uint8_t* state, int level, DevStatus* st) {
-    if (halted(st)) return;
+    if (halted_beside_topology(st)) return;
     const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
     bool any = false;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
@@ -352,7 +364,7 @@ __global__ __launch_bounds__(256) void k_reinit_level(const ReinitArgs a, uint8_
 
 template <int NS, int P>
 __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gstate, DevStatus* st) {
-    if (halted(st)) return;
+    if (halted_beside_topology(st)) return;
     __shared__ uint8_t lstate[REINIT_LDS];
     __shared__ int pending;
     const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
@@ -398,7 +410,7 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gs
 // positions and the wavefronts before it.
 __global__ __launch_bounds__(256) void k_scanline(int nx, long n, uint32_t* tag, const uint32_t* __restrict__ bn_dom,
                                                   DevStatus* st) {
-    if (halted(st)) return;
+    if (halted_beside_topology(st)) return;
     constexpr int PER = 8;
     __shared__ int wpos[4];
     __shared__ uint32_t wdom[4];
@@ -499,13 +511,26 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
                                           c->st);
     // 1024-wide workgroups: one candidate-counter atomic per 1024 connections
     if (c->ccap > 0)
-        k_cut_gen<<<(c->ccap + 1023) / 1024, 1024, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom,
-                                                               c->mx, c->my, c->cand, c->candcap, c->best_dist,
-                                                               c->st);
+        k_cut_gen<<<(c->ccap + 1023) / 1024, 1024, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom, c->cbc,
+                                                               c->mx, c->my, c->mvx, c->mvy, c->bc_v ? 1 : 0,
+                                                               c->cand, c->candcap, c->best_dist, c->st);
+    // From here on nothing of this stage reads the connections or markers:
+    // the topology stage (remesh, rings, coarsening), which reads and
+    // rewrites them, runs beside the rest on the second stream.
+    const bool beside = !cure && c->nc > 0;
+    if (beside) {
+        LBIB_HIP(hipEventRecord(c->ev_fork, c->stream));
+        LBIB_HIP(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+        hipStream_t main = c->stream;
+        c->stream = c->stream2;
+        const int r = launch_topology(c, coarsen, true);
+        c->stream = main;
+        if (r) return -1;
+        LBIB_HIP(hipEventRecord(c->ev_join, c->stream2));
+    }
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
-    k_cut_win<<<256, 1024, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->ca, c->cb, c->cbc, c->mx, c->my,
-                                        c->mvx, c->mvy, c->tag, c->bn_dom, c->bc_v, c->bc_desc, c->changed, c->rmark,
-                                        c->rstate, cure ? 0 : 1, c->st);
+    k_cut_win<<<256, 1024, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->tag, c->bn_dom, c->bc_v,
+                                        c->bc_desc, c->changed, c->rmark, c->rstate, cure ? 0 : 1, c->st);
     if (!cure) {
         ReinitArgs a;
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;
@@ -523,8 +548,8 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     }
     k_scanline<<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, c->tag, c->bn_dom, c->st);
     LBIB_HIP(hipGetLastError());
-    if (cure) return 0;
-    return launch_topology(c, coarsen, true);
+    if (beside) LBIB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return 0;
 }
 
 // remeshBoundary, then at it % 10 == 0 deleteAllForces + coarsenBoundary

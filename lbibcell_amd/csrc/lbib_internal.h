@@ -67,7 +67,18 @@ struct DevStatus {
 // A stopped status makes later kernels return without work.  Geometry and
 // topology kernels stop on errors, halts and skipped steps; the lattice, IB,
 // force and area-regulator kernels also while a remesh round is pending.
+// Raised by remeshing for a new node within 2 sites of the edge; reported as
+// LBIB_ERR_EDGE by lbib_read_status.  A bit of its own keeps it apart from the
+// edge errors of the move and the cut links for the kernels below.
+constexpr uint32_t ERR_EDGE_REMESH = 0x00000080u;
 constexpr uint32_t STOP_BITS = LBIB_ERR_MASK | LBIB_HALT_MASK | LBIB_SKIPPED;
+// The cut-link winners, the re-initialisation and the scanline run beside the
+// remesh / ring / coarsening kernels of the same iteration, which the
+// reference runs after them: they ignore what only those kernels raise.
+constexpr uint32_t TOPO_ONLY_BITS = LBIB_HALT_PERTURB_REMESH | LBIB_HALT_GROW | LBIB_ERR_TOPOLOGY | ERR_EDGE_REMESH;
+__device__ __forceinline__ bool halted_beside_topology(const DevStatus* st) {
+    return (st->bits & (STOP_BITS & ~TOPO_ONLY_BITS)) != 0u;
+}
 constexpr uint32_t STOP_BITS_STEP = STOP_BITS | LBIB_FLAG_REMESH;
 __device__ __forceinline__ bool step_stopped(const DevStatus* st) { return (st->bits & STOP_BITS_STEP) != 0u; }
 // Opens an iteration: records it while the status is clear, otherwise marks
@@ -124,7 +135,8 @@ struct Cand {
     uint32_t domain;
     double dist2;
     double bx, by;      // boundary-node position
-    int32_t conn;
+    double vx, vy;      // boundary-node velocity (connections with a boundary condition)
+    int32_t desc;       // boundary-condition descriptor of the connection (-1: none)
     int32_t win;
 };
 
@@ -163,6 +175,10 @@ struct lbib_ctx {
     int lat_variant = -1, lat_wg = 256, lat_spt = 2;
     // spread over 16x16 tiles
     int stx = 0, sty = 0;
+    // second stream: the topology stage (remesh, rings, coarsening) runs on it
+    // beside the cut-link winners, re-initialisation and scanline
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint32_t *tcnt = nullptr, *toff = nullptr, *tslot = nullptr;
     uint8_t* tact = nullptr;
     int32_t* tlist = nullptr;

@@ -272,7 +272,7 @@ __global__ void k_rm_apply(TopoArrays t, const uint32_t* __restrict__ flag, cons
         // the new node is linked at once (update needed): boundary checks of
         // GeometryHandler.cpp:613-636
         if (x - 2.0 < 0 || y - 2.0 < 0 || x + 2.0 > nx || y + 2.0 > ny) {
-            atomicOr(&st->bits, LBIB_ERR_EDGE);
+            atomicOr(&st->bits, ERR_EDGE_REMESH);
             atomicMin(&st->first_bad, (int)k);
         }
         const uint32_t p = (nc - L) + 2 * r;
