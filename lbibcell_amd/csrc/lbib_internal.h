@@ -64,13 +64,13 @@ struct DevStatus {
     uint32_t topo_changed;     // remesh or coarsening changed the topology in this step
 };
 
-// A stopped status makes later kernels return without work.  Geometry and
-// topology kernels stop on errors, halts and skipped steps; the lattice, IB,
-// force and area-regulator kernels also while a remesh round is pending.
 // Raised by remeshing for a new node within 2 sites of the edge; reported as
 // LBIB_ERR_EDGE by lbib_read_status.  A bit of its own keeps it apart from the
 // edge errors of the move and the cut links for the kernels below.
 constexpr uint32_t ERR_EDGE_REMESH = 0x00000080u;
+// A stopped status makes later kernels return without work.  Geometry and
+// topology kernels stop on errors, halts and skipped steps; the lattice, IB,
+// force and area-regulator kernels also while a remesh round is pending.
 constexpr uint32_t STOP_BITS = LBIB_ERR_MASK | LBIB_HALT_MASK | LBIB_SKIPPED;
 // The cut-link winners, the re-initialisation and the scanline run beside the
 // remesh / ring / coarsening kernels of the same iteration, which the
@@ -325,6 +325,11 @@ inline LatticeShape lattice_shape(const lbib_ctx* c) {
                                 (c->lat_variant == 2 && (vec_apply(c, c->lat_spt) || c->prec == LBIB_FP16_REM))))
         return {c->lat_variant, c->lat_wg, c->lat_spt};
     if (c->prec == LBIB_FP64 || c->prec == LBIB_FP16_REM) return {0, 256, 1};
+    // one lattice: four sites per lane (every population plane one 8- or
+    // 16-byte load per lane, all issued before the arithmetic) measured
+    // fastest on 2048^2 in both FP32 and 16-bit storage; a batched atlas keeps
+    // the per-site shapes until the tuner has timed them on its segments
+    if (c->nrect == 0 && vec_apply(c, 4)) return {2, 128, 4};
     return pairs_apply(c) ? LatticeShape{1, 256, 1} : LatticeShape{0, 256, 2};
 }
 struct TunedShape {
