@@ -425,6 +425,25 @@ int lbib_adj_forward(lbib_adj* adj, int steps, double tau_c, double kappa, const
                      double* c_final);
 /* Gradient of the last forward's loss (d_rates: n_cells + 1 entries, [0] = 0). */
 int lbib_adj_gradient(lbib_adj* adj, double* d_tau_c, double* d_kappa, double* d_rates);
+/* Population storage of the adjoint's states (call before
+ * lbib_adj_set_problem): the arithmetic precision (default), or with FP32
+ * arithmetic LBIB_FP16_DEV / LBIB_BF16_DEV, 16-bit deviations from the rest
+ * state (fluid f - w_q, species as they are) like the mixed-precision lattice;
+ * the reverse sweep reads the stored states and takes the rounding's
+ * derivative as 1.  No reference counterpart: part of the adjoint. */
+int lbib_adj_set_storage(lbib_adj* a, int storage);
+/* Checkpointing: LBIB_ADJ_CHECKPOINT_EVERY, param = K (states every K steps,
+ * a K-state segment store; the default with config.checkpoint_every), or
+ * LBIB_ADJ_REVOLVE, param = s snapshot slots (binomial checkpointing,
+ * Griewank & Walther, ACM TOMS 26 (2000), Algorithm 799: T steps with every
+ * step recomputed at most r times when T <= C(s + r, s)).  Both give the same
+ * gradient; they trade device memory for recomputation. */
+#define LBIB_ADJ_CHECKPOINT_EVERY 0
+#define LBIB_ADJ_REVOLVE 1
+int lbib_adj_set_checkpointing(lbib_adj* a, int scheme, int param);
+/* Forward steps the last forward and gradient calls ran (the gradient's count
+ * includes every recomputed step) and the bytes of the checkpoint store. */
+int lbib_adj_stats(const lbib_adj* a, int64_t* forward_steps, int64_t* gradient_steps, int64_t* store_bytes);
 /* Device time of the last forward and of the last gradient (recompute + reverse). */
 int lbib_adj_timing(lbib_adj* adj, double* forward_ms, double* gradient_ms);
 /* Multi-species model (after lbib_adj_set_problem; replaces its species):

@@ -43,7 +43,14 @@ SIGNATURES = {
     "lbib_adj_forward_ms": (C.c_int, [_P, C.c_int, _D, C.c_double, C.c_double, _D, _D, _D, _D]),
     "lbib_adj_gradient_ms": (C.c_int, [_P, _D, _D, _D, _D, _D]),
     "lbib_adj_set_batch_layout": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int32)]),
+    "lbib_adj_set_storage": (C.c_int, [_P, C.c_int]),
+    "lbib_adj_set_checkpointing": (C.c_int, [_P, C.c_int, C.c_int]),
+    "lbib_adj_stats": (C.c_int, [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
 }
+
+# checkpointing schemes (lbib_adj_set_checkpointing)
+CHECKPOINT_EVERY = 0
+REVOLVE = 1
 
 W = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4)
 
@@ -68,8 +75,9 @@ class AdjointProblem:
     dom: (ny, nx) cell index per site (0 outside, cells 1..ncell)."""
 
     def __init__(self, nx, ny, tau_fluid, rings, dom, target=None, f0=None, g0=None, decay=1e-5,
-                 precision=capi.FP32, checkpoint_every=50):
+                 precision=capi.FP32, checkpoint_every=50, storage=None, revolve_snapshots=None):
         self.L = _lib()
+        self.serial = 0   # counts forwards and gradients (the autograd wrappers check it)
         self.nx, self.ny = nx, ny
         self.ncell = int(np.max(dom)) if np.size(dom) else 0
         xy = np.concatenate([np.asarray(r, dtype=np.float64) for r in rings]) if rings else np.zeros((0, 2))
@@ -87,6 +95,10 @@ class AdjointProblem:
             msg = self.L.lbib_adj_last_error(self.h).decode() if self.h else "lbib_adj_create failed"
             self.close()
             raise capi.LbibError(msg)
+        if storage is not None:
+            self._chk(self.L.lbib_adj_set_storage(self.h, int(storage)))
+        if revolve_snapshots is not None:
+            self.set_checkpointing(REVOLVE, revolve_snapshots)
         n = nx * ny
         f0 = np.tile(W[:, None], (1, n)) if f0 is None else np.ascontiguousarray(f0, dtype=np.float64).reshape(9, n)
         g0 = np.zeros((5, n)) if g0 is None else np.ascontiguousarray(g0, dtype=np.float64).reshape(5, n)
@@ -113,6 +125,7 @@ class AdjointProblem:
         r[1:] = np.asarray(rates, dtype=np.float64).reshape(-1)[: self.ncell]
         loss = C.c_double()
         field = np.empty(self.nx * self.ny) if want_field else None
+        self.serial += 1
         self._chk(self.L.lbib_adj_forward(self.h, int(steps), float(tau_c), float(kappa), _dp(r), C.byref(loss),
                                           _dp(field) if want_field else None))
         return (loss.value, field.reshape(self.ny, self.nx)) if want_field else loss.value
@@ -121,6 +134,7 @@ class AdjointProblem:
         """(dL/dtau_c, dL/dkappa, dL/drates[ncell]) of the last forward."""
         dt, dk = C.c_double(), C.c_double()
         dr = np.zeros(self.ncell + 1)
+        self.serial += 1   # the adjoint consumes the forward's state
         self._chk(self.L.lbib_adj_gradient(self.h, C.byref(dt), C.byref(dk), _dp(dr)))
         return dt.value, dk.value, dr[1:].copy()
 
@@ -163,6 +177,7 @@ class AdjointProblem:
             r[1:] = np.asarray(rates, dtype=np.float64).reshape(-1)[: self.ncell]
         loss = C.c_double()
         field = np.empty(ns * self.nx * self.ny) if want_field else None
+        self.serial += 1
         self._chk(self.L.lbib_adj_forward_ms(self.h, int(steps), _dp(tc), float(kappa), float(k_junction),
                                              _dp(Km) if Km is not None else None, _dp(r), C.byref(loss),
                                              _dp(field) if want_field else None))
@@ -176,8 +191,21 @@ class AdjointProblem:
         dk, dj = C.c_double(), C.c_double()
         dK = np.zeros(ns * ns)
         dr = np.zeros(self.ncell + 1)
+        self.serial += 1
         self._chk(self.L.lbib_adj_gradient_ms(self.h, _dp(dt), C.byref(dk), C.byref(dj), _dp(dK), _dp(dr)))
         return dict(tau_c=dt, kappa=dk.value, k_junction=dj.value, K=dK.reshape(ns, ns), rates=dr[1:].copy())
+
+    def set_checkpointing(self, scheme, param):
+        """CHECKPOINT_EVERY with K steps, or REVOLVE with that many snapshot
+        slots (binomial checkpointing); the gradient does not depend on it."""
+        self._chk(self.L.lbib_adj_set_checkpointing(self.h, int(scheme), int(param)))
+
+    def stats(self):
+        """dict(forward_steps, gradient_steps (recomputed steps included),
+        store_bytes of the checkpoints) of the last forward / gradient"""
+        f, g, b = C.c_int64(), C.c_int64(), C.c_int64()
+        self._chk(self.L.lbib_adj_stats(self.h, C.byref(f), C.byref(g), C.byref(b)))
+        return dict(forward_steps=f.value, gradient_steps=g.value, store_bytes=b.value)
 
     def timing(self):
         """(device ms of the last forward, of the last gradient)"""
@@ -197,23 +225,81 @@ class AdjointProblem:
             pass
 
 
+def _needs_gradient(ctx):
+    return any(ctx.needs_input_grad)
+
+
 def coupled_loss_function():
     """torch.autograd.Function over AdjointProblem (imported lazily so that
-    the module does not need torch)."""
+    the module does not need torch).  The forward runs the device forward
+    only; backward runs the adjoint when a gradient is asked for (again
+    running the forward first if the problem has seen another forward in the
+    meantime) and returns the gradients on the inputs' device and dtype."""
     import torch
 
     class CoupledLoss(torch.autograd.Function):
         @staticmethod
         def forward(ctx, tau_c, kappa, rates, problem, steps):
-            loss = problem.forward(steps, float(tau_c), float(kappa), rates.detach().cpu().numpy())
-            dt, dk, dr = problem.gradient()
-            ctx.save_for_backward(torch.tensor(dt, dtype=torch.float64), torch.tensor(dk, dtype=torch.float64),
-                                  torch.from_numpy(dr))
-            return torch.tensor(loss, dtype=torch.float64)
+            params = (float(tau_c), float(kappa), rates.detach().cpu().numpy().astype(np.float64))
+            loss = problem.forward(steps, *params)
+            ctx.problem, ctx.steps, ctx.params = problem, steps, params
+            ctx.serial = problem.serial
+            ctx.like = (tau_c, kappa, rates)
+            return torch.tensor(loss, dtype=torch.float64, device=rates.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            dt, dk, dr = ctx.saved_tensors
-            return grad_out * dt, grad_out * dk, grad_out * dr.to(grad_out.dtype), None, None
+            if not _needs_gradient(ctx):
+                return None, None, None, None, None
+            p = ctx.problem
+            if p.serial != ctx.serial:   # another forward or gradient ran since: redo this forward
+                p.forward(ctx.steps, *ctx.params)
+            dt, dk, dr = p.gradient()
+            tc, kp, rt = ctx.like
+            g = grad_out.to(torch.float64)
+            out = (g * dt, g * dk, g * torch.from_numpy(dr).to(g.device))
+            return (out[0].to(tc.dtype) if ctx.needs_input_grad[0] else None,
+                    out[1].to(kp.dtype) if ctx.needs_input_grad[1] else None,
+                    out[2].to(rt.dtype) if ctx.needs_input_grad[2] else None, None, None)
 
     return CoupledLoss
+
+
+def multispecies_loss_function():
+    """torch.autograd.Function over AdjointProblem.forward_ms / gradient_ms:
+    apply(tau_c (ns), kappa, k_junction, K (ns, ns), rates (ncell), problem,
+    steps) -> loss."""
+    import torch
+
+    class MultiSpeciesLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, tau_c, kappa, k_junction, K, rates, problem, steps):
+            params = dict(tau_c=tau_c.detach().cpu().numpy().astype(np.float64), kappa=float(kappa),
+                          k_junction=float(k_junction), K=K.detach().cpu().numpy().astype(np.float64),
+                          rates=rates.detach().cpu().numpy().astype(np.float64))
+            loss = problem.forward_ms(steps, **params)
+            ctx.problem, ctx.steps, ctx.params = problem, steps, params
+            ctx.serial = problem.serial
+            ctx.like = (tau_c, kappa, k_junction, K, rates)
+            return torch.tensor(loss, dtype=torch.float64, device=rates.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            if not _needs_gradient(ctx):
+                return (None,) * 7
+            p = ctx.problem
+            if p.serial != ctx.serial:
+                p.forward_ms(ctx.steps, **ctx.params)
+            gr = p.gradient_ms()
+            g = grad_out.to(torch.float64)
+            vals = [gr["tau_c"], gr["kappa"], gr["k_junction"], gr["K"], gr["rates"]]
+            out = []
+            for need, like, v in zip(ctx.needs_input_grad[:5], ctx.like, vals):
+                if not need:
+                    out.append(None)
+                    continue
+                t = torch.as_tensor(np.asarray(v, dtype=np.float64), device=g.device) * g
+                out.append(t.to(like.dtype).reshape(like.shape))
+            return (*out, None, None)
+
+    return MultiSpeciesLoss

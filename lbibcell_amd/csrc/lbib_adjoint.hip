@@ -28,10 +28,14 @@
 // A batch of independent problems may share the lattice as periodic
 // rectangles (shared parameters, summed loss).
 //
-// Forward states are checkpointed every K steps; the reverse sweep restores a
-// checkpoint, recomputes its segment storing every intermediate state, and
-// runs the adjoint kernels backwards over it (two-level revolve schedule: the
-// segment store is what HBM capacity buys instead of deeper recursion).
+// Checkpointing: either states every K steps, the reverse sweep restoring a
+// checkpoint, recomputing its segment with every intermediate state stored
+// and running the adjoint kernels backwards over it; or binomial
+// checkpointing (revolve, Griewank & Walther's Algorithm 799) in a fixed
+// number of snapshot slots, recomputing each step at most r times for
+// C(s + r, s) steps.  Populations may be stored as 16-bit deviations with FP32
+// arithmetic (the forward model of the mixed-precision lattice), which halves
+// every snapshot.
 //
 // Roofline: every kernel here is HBM bound (the per-site adjoint reads the
 // forward state f, g, F, u and the adjoints of step t+1 and writes those of
@@ -40,6 +44,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 #include "lbib_device_math.h"
 
@@ -210,14 +215,40 @@ __global__ void k_spread(int nm, const T* __restrict__ X, const T* __restrict__ 
         }
 }
 
+// Population storage of the states: SP == 0 keeps them in the arithmetic
+// type; LBIB_FP16_DEV / LBIB_BF16_DEV keep 16-bit deviations (fluid f - w_q,
+// species as they are, their rest state being 0), decoded to the arithmetic
+// type on load and rounded to nearest on store.  The reverse sweep reads the
+// same stored states and takes the rounding's derivative as 1.
+template <typename T, int SP>
+struct PopStore {
+    using R = uint16_t;
+    __device__ static T wgt(int q) { return q == 0 ? Cst<T>::W0 : (q < 5 ? Cst<T>::W1 : Cst<T>::W2); }
+    __device__ static T f(const void* p, long k, int q) {
+        return (T)Codec<SP>::dec(static_cast<const R*>(p)[k]) + wgt(q);
+    }
+    __device__ static void put_f(void* p, long k, int q, T v) {
+        static_cast<R*>(p)[k] = Codec<SP>::enc((float)(v - wgt(q)));
+    }
+    __device__ static T g(const void* p, long k) { return (T)Codec<SP>::dec(static_cast<const R*>(p)[k]); }
+    __device__ static void put_g(void* p, long k, T v) { static_cast<R*>(p)[k] = Codec<SP>::enc((float)v); }
+};
+template <typename T>
+struct PopStore<T, 0> {
+    __device__ static T f(const void* p, long k, int) { return static_cast<const T*>(p)[k]; }
+    __device__ static void put_f(void* p, long k, int, T v) { static_cast<T*>(p)[k] = v; }
+    __device__ static T g(const void* p, long k) { return static_cast<const T*>(p)[k]; }
+    __device__ static void put_g(void* p, long k, T v) { static_cast<T*>(p)[k] = v; }
+};
+
 template <typename T>
 struct LatArgs {
     int nx, ny;
     long n;
-    const T* fs;
-    const T* gs;
-    T* fd;
-    T* gd;
+    const void* fs;   // states in the population storage (PopStore)
+    const void* gs;
+    void* fd;
+    void* gd;
     const T* Fs;
     T* u;
     const uint32_t* dom;
@@ -323,8 +354,9 @@ __device__ __forceinline__ void collide_site(T d[9], T g[NS][5], const T Fx, con
     }
 }
 
-template <typename T, int NS>
+template <typename T, int NS, int SP>
 __global__ void k_lattice(const LatArgs<T> a) {
+    using PS = PopStore<T, SP>;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int x = (int)(i % a.nx), y = (int)(i / a.nx);
@@ -335,19 +367,19 @@ __global__ void k_lattice(const LatArgs<T> a) {
     const long n = a.n;
     T d[9], g[NS][5];
 #pragma unroll
-    for (int q = 0; q < 9; q++) d[q] = a.fs[q * n + idx[q]];
+    for (int q = 0; q < 9; q++) d[q] = PS::f(a.fs, q * n + idx[q], q);
 #pragma unroll
     for (int s = 0; s < NS; s++)
 #pragma unroll
-        for (int q = 0; q < 5; q++) g[s][q] = a.gs[(5 * s + q) * n + idx[q]];
+        for (int q = 0; q < 5; q++) g[s][q] = PS::g(a.gs, (5 * s + q) * n + idx[q]);
     T uxo, uyo;
     collide_site<T, NS>(d, g, a.Fs[2 * i], a.Fs[2 * i + 1], a, a.dom[i], uxo, uyo);
 #pragma unroll
-    for (int q = 0; q < 9; q++) a.fd[q * n + i] = d[q];
+    for (int q = 0; q < 9; q++) PS::put_f(a.fd, q * n + i, q, d[q]);
 #pragma unroll
     for (int s = 0; s < NS; s++)
 #pragma unroll
-        for (int q = 0; q < 5; q++) a.gd[(5 * s + q) * n + i] = g[s][q];
+        for (int q = 0; q < 5; q++) PS::put_g(a.gd, (5 * s + q) * n + i, g[s][q]);
     a.u[2 * i] = uxo;
     a.u[2 * i + 1] = uyo;
 }
@@ -377,8 +409,8 @@ __global__ void k_interp_move(int nm, T* X, const T* __restrict__ u, int nx, int
 
 // loss L = sum over species and sites of (C - C*)^2 and its adjoint on the
 // stored species populations; cfinal (optional) receives every C
-template <typename T>
-__global__ void k_loss(const T* __restrict__ g, const double* __restrict__ target, long n, int ns, T* lg,
+template <typename T, int SP>
+__global__ void k_loss(const void* __restrict__ g, const double* __restrict__ target, long n, int ns, T* lg,
                        double* loss, T* cfinal, const AdjFrames fa, int nx, int ny) {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     double r2 = 0.0;
@@ -391,10 +423,9 @@ __global__ void k_loss(const T* __restrict__ g, const double* __restrict__ targe
             if (cfinal) cfinal[(long)s * n + i] = (T)0;
             continue;
         }
-        const T* gs = g + (long)5 * s * n;
         T C = 0;
 #pragma unroll
-        for (int q = 0; q < 5; q++) C += gs[q * n + i];
+        for (int q = 0; q < 5; q++) C += PopStore<T, SP>::g(g, (long)(5 * s + q) * n + i);
         const double r = (double)C - target[(long)s * n + i];
         r2 += r * r;
         if (cfinal) cfinal[(long)s * n + i] = C;
@@ -440,8 +471,8 @@ template <typename T>
 struct BwdArgs {
     int nx, ny;
     long n;
-    const T* fs;   // f_t, g_t (pre-stream states of the step)
-    const T* gs;
+    const void* fs;   // f_t, g_t (pre-stream states of the step, in the population storage)
+    const void* gs;
     const T* Fs;   // site force of the step
     const T* lf1;  // adjoints of f_{t+1}, g_{t+1}
     const T* lg1;
@@ -461,8 +492,9 @@ struct BwdArgs {
 };
 
 // Per-site adjoint of pull + fluid collide + species collides.
-template <typename T, int NS>
+template <typename T, int NS, int SP>
 __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
+    using PS = PopStore<T, SP>;
     // [ns + ncell + 1 + ns^2] gradient slots, laid out like a.grad
     extern __shared__ double sgrad[];
     const int ng = NS + a.ncell + 1 + NS * NS;
@@ -487,11 +519,11 @@ __global__ __launch_bounds__(256) void k_lattice_bwd(const BwdArgs<T> a) {
         const long n = a.n;
         T d[9], g[NS][5];
 #pragma unroll
-        for (int q = 0; q < 9; q++) d[q] = a.fs[q * n + idx[q]];
+        for (int q = 0; q < 9; q++) d[q] = PS::f(a.fs, q * n + idx[q], q);
 #pragma unroll
         for (int s = 0; s < NS; s++)
 #pragma unroll
-            for (int q = 0; q < 5; q++) g[s][q] = a.gs[(5 * s + q) * n + idx[q]];
+            for (int q = 0; q < 5; q++) g[s][q] = PS::g(a.gs, (5 * s + q) * n + idx[q]);
         const T Fx = a.Fs[2 * i], Fy = a.Fs[2 * i + 1];
         // forward values
         T rho = 0;
@@ -735,20 +767,24 @@ struct lbib_adj {
     long n = 0;
     int nm = 0, ncell = 0, K = 50, prec = LBIB_FP32;
     size_t tb = 4;                  // bytes of the arithmetic type
+    int sprec = LBIB_FP32;          // population storage (the arithmetic type, or 16-bit deviations)
+    size_t sb = 4;                  // bytes of one stored population value
+    int scheme = 0;                 // 0: checkpoint every K steps + segment store; 1: binomial (revolve)
+    int snaps = 0;                  // revolve: snapshot slots
     hipStream_t stream = nullptr;
     std::string err;
-    // state (arithmetic type T)
+    // state: populations in the storage type, markers etc. in the arithmetic type
     void *f[2] = {nullptr, nullptr}, *g[2] = {nullptr, nullptr};
-    void *X = nullptr, *Fm = nullptr, *Fs = nullptr, *u = nullptr;
+    void *X = nullptr, *Xs = nullptr, *Fm = nullptr, *Fs = nullptr, *u = nullptr;
     // problem
     void *f0 = nullptr, *g0 = nullptr, *X0 = nullptr;
     int32_t *nxt = nullptr, *prv = nullptr;
     uint32_t* dom = nullptr;
     double *target = nullptr, *src = nullptr, *grad = nullptr, *loss = nullptr;
-    // checkpoints and the segment store
-    void* ck = nullptr;             // S x (14 n + 2 nm)
+    // checkpoints (or revolve snapshots) and the segment store
+    void* ck = nullptr;             // nck state slots
     int nck = 0;
-    void* seg = nullptr;            // K x (14 n + 2 nm) states + K x (4 n + 2 nm) forces and velocities
+    void* seg = nullptr;            // K states + K x (4 n + 2 nm) forces and velocities
     // adjoints
     void *lf[2] = {nullptr, nullptr}, *lg[2] = {nullptr, nullptr};
     void *lX = nullptr, *lFm = nullptr, *lU = nullptr, *lF = nullptr;
@@ -772,6 +808,7 @@ struct lbib_adj {
     std::vector<double> xy_host;    // initial marker positions (layout checks)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     double fwd_ms = 0.0, grad_ms = 0.0;
+    long fwd_steps_run = 0, grad_steps_run = 0;   // forward steps taken by the last forward / gradient
     bool have_problem = false, have_forward = false;
 };
 
@@ -802,21 +839,28 @@ adj::MarkerFrames mframes(const lbib_adj* a) { return adj::MarkerFrames{a->rtile
 template <typename T>
 T* tp(void* p) { return static_cast<T*>(p); }
 
-// state slot offsets (in T elements) of the checkpoint and segment stores
+// state slots (bytes): fluid populations, species populations (storage
+// type), then the markers (arithmetic type) at a 256-byte boundary
 inline size_t gplanes(const lbib_adj* a) { return 5 * (size_t)a->ns; }
-inline size_t state_len(const lbib_adj* a) { return (9 + gplanes(a)) * (size_t)a->n + 2 * (size_t)a->nm; }
+inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+inline size_t fbytes(const lbib_adj* a) { return 9 * (size_t)a->n * a->sb; }
+inline size_t gbytes(const lbib_adj* a) { return gplanes(a) * (size_t)a->n * a->sb; }
+inline size_t xbytes(const lbib_adj* a) { return 2 * (size_t)a->nm * a->tb; }
+inline size_t xoff(const lbib_adj* a) { return align256(fbytes(a) + gbytes(a)); }
+inline size_t state_bytes(const lbib_adj* a) { return align256(xoff(a) + xbytes(a)); }
 // gradient slots: tau_c [ns], rates [ncell + 1], K [ns^2], then kappa and k_j
 inline int grad_len(int ns, int ncell) { return ns + ncell + 1 + ns * ns + 2; }
-inline size_t extra_len(const lbib_adj* a) { return 4 * (size_t)a->n + 2 * (size_t)a->nm; }
+inline size_t extra_bytes(const lbib_adj* a) { return align256((4 * (size_t)a->n + 2 * (size_t)a->nm) * a->tb); }
 
 template <typename T>
 struct Slot {
-    T *f, *g, *X;   // 9n, 5 ns n, 2nm
+    void *f, *g;   // 9n, 5 ns n stored values
+    T* X;          // 2nm
 };
 template <typename T>
 Slot<T> state_slot(void* base, const lbib_adj* a, int j) {
-    T* s = tp<T>(base) + (size_t)j * state_len(a);
-    return {s, s + 9 * a->n, s + (9 + gplanes(a)) * a->n};
+    char* s = static_cast<char*>(base) + (size_t)j * state_bytes(a);
+    return {s, s + fbytes(a), reinterpret_cast<T*>(s + xoff(a))};
 }
 template <typename T>
 struct Extra {
@@ -824,12 +868,20 @@ struct Extra {
 };
 template <typename T>
 Extra<T> extra_slot(lbib_adj* a, int j) {
-    T* s = tp<T>(a->seg) + (size_t)a->K * state_len(a) + (size_t)j * extra_len(a);
+    T* s = reinterpret_cast<T*>(static_cast<char*>(a->seg) + (size_t)a->K * state_bytes(a) +
+                                (size_t)j * extra_bytes(a));
     return {s, s + 2 * a->n, s + 4 * a->n};
+}
+template <typename T>
+int copy_state(lbib_adj* a, const Slot<T>& dst, const Slot<T>& srcs) {
+    ADJ_HIP(hipMemcpyAsync(dst.f, srcs.f, fbytes(a), hipMemcpyDeviceToDevice, a->stream));
+    ADJ_HIP(hipMemcpyAsync(dst.g, srcs.g, gbytes(a), hipMemcpyDeviceToDevice, a->stream));
+    if (a->nm > 0) ADJ_HIP(hipMemcpyAsync(dst.X, srcs.X, xbytes(a), hipMemcpyDeviceToDevice, a->stream));
+    return 0;
 }
 
 template <typename T>
-adj::LatArgs<T> lat_args(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, const T* Fs, T* u) {
+adj::LatArgs<T> lat_args(lbib_adj* a, const void* fs, const void* gs, void* fd, void* gd, const T* Fs, T* u) {
     adj::LatArgs<T> l;
     l.nx = a->cfg.nx; l.ny = a->cfg.ny; l.n = a->n;
     l.fs = fs; l.gs = gs; l.fd = fd; l.gd = gd; l.Fs = Fs; l.u = u;
@@ -853,20 +905,52 @@ adj::Junctions junctions(const lbib_adj* a) {
     return adj::Junctions{a->jofs, a->jpart, a->kj, a->l0};
 }
 
-template <typename T>
-void launch_lattice(lbib_adj* a, const adj::LatArgs<T>& l) {
-    switch (a->ns) {
-        case 1: adj::k_lattice<T, 1><<<nb(a->n), 256, 0, a->stream>>>(l); break;
-        case 2: adj::k_lattice<T, 2><<<nb(a->n), 256, 0, a->stream>>>(l); break;
-        case 3: adj::k_lattice<T, 3><<<nb(a->n), 256, 0, a->stream>>>(l); break;
-        default: adj::k_lattice<T, 4><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+// the storage variants: the arithmetic type itself, or (FP32 arithmetic
+// only) 16-bit deviations
+template <typename T, template <typename, int> class Launch, typename... Args>
+void by_storage(lbib_adj* a, Args&&... args) {
+    if constexpr (std::is_same<T, float>::value) {
+        if (a->sprec == LBIB_FP16_DEV) { Launch<T, LBIB_FP16_DEV>::go(a, args...); return; }
+        if (a->sprec == LBIB_BF16_DEV) { Launch<T, LBIB_BF16_DEV>::go(a, args...); return; }
     }
+    Launch<T, 0>::go(a, args...);
 }
+template <typename T, int SP>
+struct LatLaunch {
+    static void go(lbib_adj* a, const adj::LatArgs<T>& l) {
+        switch (a->ns) {
+            case 1: adj::k_lattice<T, 1, SP><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+            case 2: adj::k_lattice<T, 2, SP><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+            case 3: adj::k_lattice<T, 3, SP><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+            default: adj::k_lattice<T, 4, SP><<<nb(a->n), 256, 0, a->stream>>>(l); break;
+        }
+    }
+};
+template <typename T, int SP>
+struct BwdLaunch {
+    static void go(lbib_adj* a, const adj::BwdArgs<T>& b) {
+        const size_t lds = (size_t)(a->ns + a->ncell + 1 + a->ns * a->ns) * sizeof(double);
+        switch (a->ns) {
+            case 1: adj::k_lattice_bwd<T, 1, SP><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+            case 2: adj::k_lattice_bwd<T, 2, SP><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+            case 3: adj::k_lattice_bwd<T, 3, SP><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+            default: adj::k_lattice_bwd<T, 4, SP><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+        }
+    }
+};
+template <typename T, int SP>
+struct LossLaunch {
+    static void go(lbib_adj* a, const void* g, T* cf) {
+        adj::k_loss<T, SP><<<nb(a->n), 256, 0, a->stream>>>(g, a->target, a->n, a->ns, tp<T>(a->lg[0]), a->loss, cf,
+                                                            adj::AdjFrames{a->rtile, a->rects, a->ntx}, a->cfg.nx,
+                                                            a->cfg.ny);
+    }
+};
 
 // one forward step from (fs, gs, X) into (fd, gd, X); Fs, Fm, u receive the
 // step's site force, marker forces and velocity
 template <typename T>
-int step_fwd(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, T* X, T* Fs, T* Fm, T* u) {
+int step_fwd(lbib_adj* a, const void* fs, const void* gs, void* fd, void* gd, T* X, T* Fs, T* Fm, T* u) {
     const int nx = a->cfg.nx, ny = a->cfg.ny;
     ADJ_HIP(hipMemsetAsync(Fs, 0, 2 * a->n * sizeof(T), a->stream));
     if (a->nm > 0) {
@@ -874,36 +958,73 @@ int step_fwd(lbib_adj* a, const T* fs, const T* gs, T* fd, T* gd, T* X, T* Fs, T
                                                           junctions<T>(a));
         adj::k_spread<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, Fm, Fs, nx, ny, mframes(a));
     }
-    launch_lattice<T>(a, lat_args<T>(a, fs, gs, fd, gd, Fs, u));
+    by_storage<T, LatLaunch>(a, lat_args<T>(a, fs, gs, fd, gd, Fs, u));
     if (a->nm > 0) adj::k_interp_move<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, X, u, nx, ny, mframes(a));
     ADJ_HIP(hipGetLastError());
     return 0;
 }
 
+// ---- binomial checkpointing (revolve) ----------------------------------------
+// beta(s, r) = C(s + r, s): the most steps s snapshots reverse when every
+// step is recomputed at most r times (Griewank & Walther, Algorithm 799),
+// saturated far above any step count.
+double rev_beta(int s, int r) {
+    double b = 1.0;
+    for (int k = 1; k <= s; k++) b = b * (double)(r + k) / (double)k;
+    return b;
+}
+// First snapshot position (steps after t0) for l steps with s free slots:
+// the least r with beta(s, r) >= l, then the smallest advance that leaves
+// the right part within beta(s - 1, r) (the left part then fits beta(s, r - 1)).
+int rev_split(int l, int s) {
+    int r = 0;
+    while (rev_beta(s, r) < (double)l) r++;
+    double m = (double)l - rev_beta(s - 1, r);
+    if (m < 1.0) m = 1.0;
+    const double cap = rev_beta(s, r - 1);
+    if (m > cap) m = cap;
+    if (m > l - 1) m = l - 1;
+    return (int)m;
+}
+// snapshot positions the forward sweep stores: the right spine of the
+// recursion (slot k holds the state before step pos[k])
+std::vector<int> rev_spine(int steps, int s) {
+    std::vector<int> pos;
+    int t0 = 0, free = s;
+    while (free > 0 && steps - t0 > 1) {
+        t0 += rev_split(steps - t0, free);
+        pos.push_back(t0);
+        free--;
+    }
+    return pos;
+}
+
 template <typename T>
 int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
-    const size_t sl = state_len(a);
-    const int need = steps / a->K + 1;
+    const std::vector<int> spine = a->scheme == 1 ? rev_spine(steps, a->snaps) : std::vector<int>();
+    const int need = a->scheme == 1 ? std::max(a->snaps, 1) : steps / a->K + 1;
     if (need > a->nck) {
         adj_free(a->ck);
-        if (adj_alloc(a, &a->ck, (size_t)need * sl * sizeof(T))) return -1;
+        a->nck = 0;
+        if (adj_alloc(a, &a->ck, (size_t)need * state_bytes(a))) return -1;
         a->nck = need;
     }
-    ADJ_HIP(hipMemcpyAsync(a->f[0], a->f0, 9 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
-    ADJ_HIP(hipMemcpyAsync(a->g[0], a->g0, gplanes(a) * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
-    ADJ_HIP(hipMemcpyAsync(a->X, a->X0, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+    ADJ_HIP(hipMemcpyAsync(a->f[0], a->f0, fbytes(a), hipMemcpyDeviceToDevice, a->stream));
+    ADJ_HIP(hipMemcpyAsync(a->g[0], a->g0, gbytes(a), hipMemcpyDeviceToDevice, a->stream));
+    if (a->nm > 0) ADJ_HIP(hipMemcpyAsync(a->X, a->X0, xbytes(a), hipMemcpyDeviceToDevice, a->stream));
     ADJ_HIP(hipEventRecord(a->e0, a->stream));
     int cur = 0;
+    size_t sp = 0;
     for (int t = 0; t < steps; t++) {
-        if (t % a->K == 0) {
-            Slot<T> s = state_slot<T>(a->ck, a, t / a->K);
-            ADJ_HIP(hipMemcpyAsync(s.f, a->f[cur], 9 * a->n * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
-            ADJ_HIP(hipMemcpyAsync(s.g, a->g[cur], gplanes(a) * a->n * sizeof(T), hipMemcpyDeviceToDevice,
-                                   a->stream));
-            ADJ_HIP(hipMemcpyAsync(s.X, a->X, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
+        int slot = -1;
+        if (a->scheme == 0 && t % a->K == 0) slot = t / a->K;
+        if (a->scheme == 1 && sp < spine.size() && spine[sp] == t) slot = (int)sp++;
+        if (slot >= 0) {
+            const Slot<T> w{a->f[cur], a->g[cur], tp<T>(a->X)};
+            if (copy_state<T>(a, state_slot<T>(a->ck, a, slot), w)) return -1;
         }
-        if (step_fwd<T>(a, tp<T>(a->f[cur]), tp<T>(a->g[cur]), tp<T>(a->f[1 - cur]), tp<T>(a->g[1 - cur]),
-                        tp<T>(a->X), tp<T>(a->Fs), tp<T>(a->Fm), tp<T>(a->u)))
+        if (step_fwd<T>(a, a->f[cur], a->g[cur], a->f[1 - cur], a->g[1 - cur], tp<T>(a->X), tp<T>(a->Fs),
+                        tp<T>(a->Fm), tp<T>(a->u)))
             return -1;
         cur = 1 - cur;
     }
@@ -913,14 +1034,13 @@ int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
         if (!a->cfin && adj_alloc(a, &a->cfin, (size_t)a->ns * a->n * sizeof(T))) return -1;
         cf = tp<T>(a->cfin);
     }
-    adj::k_loss<T><<<nb(a->n), 256, 0, a->stream>>>(tp<T>(a->g[cur]), a->target, a->n, a->ns, tp<T>(a->lg[0]),
-                                                     a->loss, cf, adj::AdjFrames{a->rtile, a->rects, a->ntx},
-                                                     a->cfg.nx, a->cfg.ny);
+    by_storage<T, LossLaunch>(a, (const void*)a->g[cur], cf);
     ADJ_HIP(hipEventRecord(a->e1, a->stream));
     ADJ_HIP(hipEventSynchronize(a->e1));
     float ms = 0.0f;
     ADJ_HIP(hipEventElapsedTime(&ms, a->e0, a->e1));
     a->fwd_ms = ms;
+    a->fwd_steps_run = steps;
     ADJ_HIP(hipMemcpy(loss, a->loss, sizeof(double), hipMemcpyDeviceToHost));
     if (cfinal) {
         const size_t cnt = (size_t)a->ns * a->n;
@@ -932,24 +1052,106 @@ int forward_t(lbib_adj* a, int steps, double* loss, double* cfinal) {
     return 0;
 }
 
+// The reverse of one step: state t in st (pre-move markers), its forces and
+// velocity in e; lambda of t+1 in lf/lg[cur] becomes lambda of t in lf/lg[1-cur].
 template <typename T>
-void launch_lattice_bwd(lbib_adj* a, const adj::BwdArgs<T>& b) {
-    const size_t lds = (size_t)(a->ns + a->ncell + 1 + a->ns * a->ns) * sizeof(double);
-    switch (a->ns) {
-        case 1: adj::k_lattice_bwd<T, 1><<<nb(a->n), 256, lds, a->stream>>>(b); break;
-        case 2: adj::k_lattice_bwd<T, 2><<<nb(a->n), 256, lds, a->stream>>>(b); break;
-        case 3: adj::k_lattice_bwd<T, 3><<<nb(a->n), 256, lds, a->stream>>>(b); break;
-        default: adj::k_lattice_bwd<T, 4><<<nb(a->n), 256, lds, a->stream>>>(b); break;
+int reverse_step(lbib_adj* a, const Slot<T>& st, const Extra<T>& e, int& cur) {
+    const int nx = a->cfg.nx, ny = a->cfg.ny;
+    const int gl = grad_len(a->ns, a->ncell);
+    if (a->nm > 0)
+        adj::k_interp_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, st.X, e.u, tp<T>(a->lX), tp<T>(a->lU), nx, ny,
+                                                                 mframes(a));
+    adj::BwdArgs<T> b;
+    b.nx = nx; b.ny = ny; b.n = a->n;
+    b.fs = st.f; b.gs = st.g; b.Fs = e.Fs;
+    b.lf1 = tp<T>(a->lf[cur]); b.lg1 = tp<T>(a->lg[cur]);
+    b.lamU = tp<T>(a->lU);
+    b.lf0 = tp<T>(a->lf[1 - cur]); b.lg0 = tp<T>(a->lg[1 - cur]);
+    b.lamF = tp<T>(a->lF);
+    b.dom = a->dom; b.src = a->src;
+    const adj::LatArgs<T> l = lat_args<T>(a, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    b.tauI = l.tauI;
+    for (int q = 0; q < LBIB_MAX_SPECIES; q++) { b.tauIc[q] = l.tauIc[q]; b.decay[q] = l.decay[q]; }
+    for (int q = 0; q < LBIB_MAX_SPECIES * LBIB_MAX_SPECIES; q++) b.Km[q] = l.Km[q];
+    b.hasK = l.hasK;
+    b.fr = l.fr;
+    b.ncell = a->ncell;
+    b.grad = a->grad;
+    by_storage<T, BwdLaunch>(a, b);
+    if (a->nm > 0) {
+        adj::k_spread_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, st.X, e.Fm, tp<T>(a->lF), tp<T>(a->lFm),
+                                                                 tp<T>(a->lX), nx, ny, mframes(a));
+        adj::k_force_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, st.X, a->nxt, a->prv, (T)a->kappa,
+                                                                tp<T>(a->lFm), tp<T>(a->lX), a->grad + gl - 2,
+                                                                junctions<T>(a), a->grad + gl - 1);
     }
+    ADJ_HIP(hipGetLastError());
+    cur = 1 - cur;
+    return 0;
 }
+
+// Revolve: the working state ping-pongs between f/g[0] and f/g[1] (index
+// w), markers in X; slot -1 is the initial state.
+template <typename T>
+struct Revolver {
+    lbib_adj* a;
+    int w = 0, cur = 0;
+    Slot<T> work(int k) const { return {a->f[k], a->g[k], tp<T>(a->X)}; }
+    Slot<T> source(int slot) const {
+        return slot < 0 ? Slot<T>{a->f0, a->g0, tp<T>(a->X0)} : state_slot<T>(a->ck, a, slot);
+    }
+    int restore(int slot) {
+        w = 0;
+        return copy_state<T>(a, work(0), source(slot));
+    }
+    int advance(int nsteps) {
+        for (int k = 0; k < nsteps; k++) {
+            if (step_fwd<T>(a, a->f[w], a->g[w], a->f[1 - w], a->g[1 - w], tp<T>(a->X), tp<T>(a->Fs), tp<T>(a->Fm),
+                            tp<T>(a->u)))
+                return -1;
+            w = 1 - w;
+            a->grad_steps_run++;
+        }
+        return 0;
+    }
+    // reverse of the step from the working state: recompute it into the
+    // other buffers with the markers moved in Xs, keeping X at time t
+    int reverse_one() {
+        if (a->nm > 0) ADJ_HIP(hipMemcpyAsync(a->Xs, a->X, xbytes(a), hipMemcpyDeviceToDevice, a->stream));
+        if (step_fwd<T>(a, a->f[w], a->g[w], a->f[1 - w], a->g[1 - w], tp<T>(a->Xs), tp<T>(a->Fs), tp<T>(a->Fm),
+                        tp<T>(a->u)))
+            return -1;
+        a->grad_steps_run++;
+        const Extra<T> e{tp<T>(a->Fs), tp<T>(a->u), tp<T>(a->Fm)};
+        return reverse_step<T>(a, work(w), e, cur);
+    }
+    // adjoint from t1 back to t0, the state of t0 being in `slot`; `free`
+    // snapshot slots from a->snaps - free up; on the right spine the
+    // forward sweep already stored the snapshots
+    int run(int t0, int slot, int t1, int free, bool spine) {
+        const int l = t1 - t0;
+        if (l == 1) return restore(slot) || reverse_one();
+        if (free == 0) {
+            for (int t = t1 - 1; t >= t0; t--)
+                if (restore(slot) || advance(t - t0) || reverse_one()) return -1;
+            return 0;
+        }
+        const int m = t0 + rev_split(l, free);
+        const int k = a->snaps - free;
+        if (!spine && (restore(slot) || advance(m - t0) || copy_state<T>(a, state_slot<T>(a->ck, a, k), work(w))))
+            return -1;
+        if (run(m, k, t1, free - 1, spine)) return -1;
+        return run(t0, slot, m, free, false);
+    }
+};
 
 // gradient slots into the caller's arrays (any may be null): tau_c [ns],
 // kappa, k_j, K [ns ns] row-major, rates [ncell + 1] of species 0 (entry 0 = 0)
 template <typename T>
 int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_kj, double* d_K, double* d_rates) {
-    const int nx = a->cfg.nx, ny = a->cfg.ny;
     const int steps = a->steps;
     ADJ_HIP(hipEventRecord(a->e0, a->stream));
+    a->grad_steps_run = 0;
     // lambda at T: lf = 0, lg from the loss (k_loss wrote lg[0]), lX = 0
     int cur = 0;
     ADJ_HIP(hipMemsetAsync(a->lf[0], 0, 9 * a->n * sizeof(T), a->stream));
@@ -957,56 +1159,30 @@ int gradient_t(lbib_adj* a, double* d_tau, double* d_kappa, double* d_kj, double
     ADJ_HIP(hipMemsetAsync(a->lU, 0, 2 * a->n * sizeof(T), a->stream));
     const int gl = grad_len(a->ns, a->ncell);
     ADJ_HIP(hipMemsetAsync(a->grad, 0, gl * sizeof(double), a->stream));
-    const int nseg = (steps + a->K - 1) / a->K;
-    for (int sgi = nseg - 1; sgi >= 0; sgi--) {
-        const int t0 = sgi * a->K, len = std::min(a->K, steps - t0);
-        // recompute the segment: slot j holds the state before step t0 + j
-        Slot<T> c = state_slot<T>(a->ck, a, sgi);
-        Slot<T> s0 = state_slot<T>(a->seg, a, 0);
-        ADJ_HIP(hipMemcpyAsync(s0.f, c.f, state_len(a) * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
-        for (int j = 0; j < len; j++) {
-            Slot<T> s = state_slot<T>(a->seg, a, j);
-            Extra<T> e = extra_slot<T>(a, j);
-            // next state: the following slot, or scratch after the segment's last step
-            T* fd = j + 1 < len ? state_slot<T>(a->seg, a, j + 1).f : tp<T>(a->f[0]);
-            T* gd = j + 1 < len ? state_slot<T>(a->seg, a, j + 1).g : tp<T>(a->g[0]);
-            T* Xn = j + 1 < len ? state_slot<T>(a->seg, a, j + 1).X : tp<T>(a->X);
-            ADJ_HIP(hipMemcpyAsync(Xn, s.X, 2 * (size_t)a->nm * sizeof(T), hipMemcpyDeviceToDevice, a->stream));
-            if (step_fwd<T>(a, s.f, s.g, fd, gd, Xn, e.Fs, e.Fm, e.u)) return -1;
-        }
-        // reverse over the segment
-        for (int j = len - 1; j >= 0; j--) {
-            Slot<T> s = state_slot<T>(a->seg, a, j);
-            Extra<T> e = extra_slot<T>(a, j);
-            if (a->nm > 0)
-                adj::k_interp_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, e.u, tp<T>(a->lX), tp<T>(a->lU),
-                                                                         nx, ny, mframes(a));
-            adj::BwdArgs<T> b;
-            b.nx = nx; b.ny = ny; b.n = a->n;
-            b.fs = s.f; b.gs = s.g; b.Fs = e.Fs;
-            b.lf1 = tp<T>(a->lf[cur]); b.lg1 = tp<T>(a->lg[cur]);
-            b.lamU = tp<T>(a->lU);
-            b.lf0 = tp<T>(a->lf[1 - cur]); b.lg0 = tp<T>(a->lg[1 - cur]);
-            b.lamF = tp<T>(a->lF);
-            b.dom = a->dom; b.src = a->src;
-            const adj::LatArgs<T> l = lat_args<T>(a, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
-            b.tauI = l.tauI;
-            for (int q = 0; q < LBIB_MAX_SPECIES; q++) { b.tauIc[q] = l.tauIc[q]; b.decay[q] = l.decay[q]; }
-            for (int q = 0; q < LBIB_MAX_SPECIES * LBIB_MAX_SPECIES; q++) b.Km[q] = l.Km[q];
-            b.hasK = l.hasK;
-            b.fr = l.fr;
-            b.ncell = a->ncell;
-            b.grad = a->grad;
-            launch_lattice_bwd<T>(a, b);
-            if (a->nm > 0) {
-                adj::k_spread_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, e.Fm, tp<T>(a->lF), tp<T>(a->lFm),
-                                                                         tp<T>(a->lX), nx, ny, mframes(a));
-                adj::k_force_bwd<T><<<nb(a->nm), 256, 0, a->stream>>>(a->nm, s.X, a->nxt, a->prv, (T)a->kappa,
-                                                                        tp<T>(a->lFm), tp<T>(a->lX), a->grad + gl - 2,
-                                                                        junctions<T>(a), a->grad + gl - 1);
+    if (a->scheme == 1) {
+        Revolver<T> rv{a};
+        rv.cur = 0;
+        if (rv.run(0, -1, steps, a->snaps, true)) return -1;
+    } else {
+        const int nseg = (steps + a->K - 1) / a->K;
+        for (int sgi = nseg - 1; sgi >= 0; sgi--) {
+            const int t0 = sgi * a->K, len = std::min(a->K, steps - t0);
+            // recompute the segment: slot j holds the state before step t0 + j
+            if (copy_state<T>(a, state_slot<T>(a->seg, a, 0), state_slot<T>(a->ck, a, sgi))) return -1;
+            for (int j = 0; j < len; j++) {
+                Slot<T> st = state_slot<T>(a->seg, a, j);
+                Extra<T> e = extra_slot<T>(a, j);
+                // next state: the following slot, or scratch after the segment's last step
+                const bool last = j + 1 == len;
+                const Slot<T> nx = last ? Slot<T>{a->f[0], a->g[0], tp<T>(a->X)} : state_slot<T>(a->seg, a, j + 1);
+                if (a->nm > 0)
+                    ADJ_HIP(hipMemcpyAsync(nx.X, st.X, xbytes(a), hipMemcpyDeviceToDevice, a->stream));
+                if (step_fwd<T>(a, st.f, st.g, nx.f, nx.g, nx.X, e.Fs, e.Fm, e.u)) return -1;
+                a->grad_steps_run++;
             }
-            ADJ_HIP(hipGetLastError());
-            cur = 1 - cur;
+            // reverse over the segment
+            for (int j = len - 1; j >= 0; j--)
+                if (reverse_step<T>(a, state_slot<T>(a->seg, a, j), extra_slot<T>(a, j), cur)) return -1;
         }
     }
     ADJ_HIP(hipEventRecord(a->e1, a->stream));
@@ -1038,6 +1214,53 @@ int upload_T(lbib_adj* a, void* dst, const double* src, size_t count) {
     return 0;
 }
 
+// 16-bit deviation of one value (host side of Codec<SP>::enc)
+uint16_t encode16(int sp, float v) {
+    if (sp == LBIB_FP16_DEV) {
+        const _Float16 h = (_Float16)v;
+        uint16_t r;
+        std::memcpy(&r, &h, 2);
+        return r;
+    }
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+// populations into the storage: planes of n values; fluid planes store f - w_q
+int upload_pop(lbib_adj* a, void* dst, const double* src, size_t planes, bool fluid) {
+    const size_t count = planes * (size_t)a->n;
+    if (a->sprec == a->prec)
+        return a->prec == LBIB_FP64 ? upload_T<double>(a, dst, src, count) : upload_T<float>(a, dst, src, count);
+    std::vector<uint16_t> tmp(count);
+    for (size_t k = 0; k < count; k++) {
+        const int q = (int)(k / (size_t)a->n);
+        const float w = !fluid ? 0.0f : (q == 0 ? 4.0f / 9.0f : (q < 5 ? 1.0f / 9.0f : 1.0f / 36.0f));
+        tmp[k] = encode16(a->sprec, (float)src[k] - w);
+    }
+    ADJ_HIP(hipMemcpy(dst, tmp.data(), count * 2, hipMemcpyHostToDevice));
+    return 0;
+}
+// population buffers of the storage type (f, g ping-pong, initial state) and
+// the store of the checkpointing scheme
+int alloc_states(lbib_adj* a) {
+    const size_t n = (size_t)a->n;
+    for (int b = 0; b < 2; b++) {
+        adj_free(a->f[b]);
+        adj_free(a->g[b]);
+        if (adj_alloc(a, &a->f[b], 9 * n * a->sb) || adj_alloc(a, &a->g[b], gplanes(a) * n * a->sb)) return -1;
+    }
+    adj_free(a->f0);
+    adj_free(a->g0);
+    adj_free(a->seg);
+    adj_free(a->ck);
+    a->nck = 0;
+    if (adj_alloc(a, &a->f0, 9 * n * a->sb) || adj_alloc(a, &a->g0, gplanes(a) * n * a->sb)) return -1;
+    if (a->scheme == 0 && adj_alloc(a, &a->seg, (size_t)a->K * (state_bytes(a) + extra_bytes(a)))) return -1;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1066,14 +1289,15 @@ int lbib_adj_create(const lbib_adj_config* cfg, lbib_adj** out) {
     ADJ_HIP(hipStreamCreateWithFlags(&a->stream, hipStreamNonBlocking));
     ADJ_HIP(hipEventCreate(&a->e0));
     ADJ_HIP(hipEventCreate(&a->e1));
+    a->sprec = a->prec;
+    a->sb = a->tb;
     const size_t tb = a->tb, n = (size_t)a->n, nm = (size_t)a->nm;
     for (int b = 0; b < 2; b++) {
-        if (adj_alloc(a, &a->f[b], 9 * n * tb) || adj_alloc(a, &a->g[b], 5 * n * tb) ||
-            adj_alloc(a, &a->lf[b], 9 * n * tb) || adj_alloc(a, &a->lg[b], 5 * n * tb))
-            return -1;
+        if (adj_alloc(a, &a->lf[b], 9 * n * tb) || adj_alloc(a, &a->lg[b], 5 * n * tb)) return -1;
     }
-    if (adj_alloc(a, &a->X, 2 * nm * tb) || adj_alloc(a, &a->Fm, 2 * nm * tb) || adj_alloc(a, &a->Fs, 2 * n * tb) ||
-        adj_alloc(a, &a->u, 2 * n * tb) || adj_alloc(a, &a->f0, 9 * n * tb) || adj_alloc(a, &a->g0, 5 * n * tb) ||
+    if (alloc_states(a)) return -1;
+    if (adj_alloc(a, &a->X, 2 * nm * tb) || adj_alloc(a, &a->Xs, 2 * nm * tb) || adj_alloc(a, &a->Fm, 2 * nm * tb) ||
+        adj_alloc(a, &a->Fs, 2 * n * tb) || adj_alloc(a, &a->u, 2 * n * tb) ||
         adj_alloc(a, &a->X0, 2 * nm * tb) || adj_alloc(a, (void**)&a->nxt, nm * 4) ||
         adj_alloc(a, (void**)&a->prv, nm * 4) || adj_alloc(a, (void**)&a->dom, n * 4) ||
         adj_alloc(a, (void**)&a->target, n * 8) || adj_alloc(a, (void**)&a->src, (a->ncell + 1) * 8) ||
@@ -1081,7 +1305,6 @@ int lbib_adj_create(const lbib_adj_config* cfg, lbib_adj** out) {
         adj_alloc(a, &a->lX, 2 * nm * tb) || adj_alloc(a, &a->lFm, 2 * nm * tb) || adj_alloc(a, &a->lU, 2 * n * tb) ||
         adj_alloc(a, &a->lF, 2 * n * tb))
         return -1;
-    if (adj_alloc(a, &a->seg, (size_t)a->K * (state_len(a) + extra_len(a)) * tb)) return -1;
     ADJ_HIP(hipStreamSynchronize(a->stream));
     return 0;
 }
@@ -1090,7 +1313,7 @@ void lbib_adj_destroy(lbib_adj* a) {
     if (!a) return;
     if (a->stream) (void)hipStreamSynchronize(a->stream);
     void** bufs[] = {&a->f[0], &a->f[1], &a->g[0], &a->g[1], &a->lf[0], &a->lf[1], &a->lg[0], &a->lg[1],
-                     &a->X, &a->Fm, &a->Fs, &a->u, &a->f0, &a->g0, &a->X0, &a->ck, &a->seg, &a->lX, &a->lFm,
+                     &a->X, &a->Xs, &a->Fm, &a->Fs, &a->u, &a->f0, &a->g0, &a->X0, &a->ck, &a->seg, &a->lX, &a->lFm,
                      &a->lU, &a->lF, &a->cfin};
     for (void** b : bufs) adj_free(*b);
     void* p[] = {a->nxt, a->prv, a->dom, a->target, a->src, a->grad, a->loss, a->jofs, a->jpart, a->rtile, a->rects, a->mrect};
@@ -1143,9 +1366,7 @@ int lbib_adj_set_problem(lbib_adj* a, const double* f0, const double* g0, const 
     if (a->rects) (void)hipFree(a->rects);
     a->rects = nullptr;
     const bool f64 = a->prec == LBIB_FP64;
-    if ((f64 ? upload_T<double>(a, a->f0, f0, 9 * n) : upload_T<float>(a, a->f0, f0, 9 * n)) ||
-        (f64 ? upload_T<double>(a, a->g0, g0, 5 * n) : upload_T<float>(a, a->g0, g0, 5 * n)))
-        return -1;
+    if (upload_pop(a, a->f0, f0, 9, true) || upload_pop(a, a->g0, g0, 5, false)) return -1;
     if (nm > 0) {
         if (f64 ? upload_T<double>(a, a->X0, xy0, 2 * nm) : upload_T<float>(a, a->X0, xy0, 2 * nm)) return -1;
         ADJ_HIP(hipMemcpy(a->nxt, next, nm * 4, hipMemcpyHostToDevice));
@@ -1172,7 +1393,7 @@ int lbib_adj_set_species(lbib_adj* a, int nspecies, const double* g0, const doub
     for (int b = 0; b < 2; b++) {
         adj_free(a->g[b]);
         adj_free(a->lg[b]);
-        if (adj_alloc(a, &a->g[b], gp * n * tb) || adj_alloc(a, &a->lg[b], gp * n * tb)) return -1;
+        if (adj_alloc(a, &a->g[b], gp * n * a->sb) || adj_alloc(a, &a->lg[b], gp * n * tb)) return -1;
     }
     adj_free(a->g0);
     adj_free(a->seg);
@@ -1181,11 +1402,10 @@ int lbib_adj_set_species(lbib_adj* a, int nspecies, const double* g0, const doub
     a->nck = 0;
     if (a->target) (void)hipFree(a->target);
     a->target = nullptr;
-    if (adj_alloc(a, &a->g0, gp * n * tb) || adj_alloc(a, (void**)&a->target, nspecies * n * 8) ||
-        adj_alloc(a, &a->seg, (size_t)a->K * (state_len(a) + extra_len(a)) * tb))
+    if (adj_alloc(a, &a->g0, gp * n * a->sb) || adj_alloc(a, (void**)&a->target, nspecies * n * 8) ||
+        (a->scheme == 0 && adj_alloc(a, &a->seg, (size_t)a->K * (state_bytes(a) + extra_bytes(a)))))
         return -1;
-    if (a->prec == LBIB_FP64 ? upload_T<double>(a, a->g0, g0, gp * n) : upload_T<float>(a, a->g0, g0, gp * n))
-        return -1;
+    if (upload_pop(a, a->g0, g0, gp, false)) return -1;
     ADJ_HIP(hipMemcpy(a->target, target, nspecies * n * 8, hipMemcpyHostToDevice));
     for (int s = 0; s < LBIB_MAX_SPECIES; s++) a->decay[s] = s < nspecies ? decay[s] : 0.0;
     a->have_forward = false;
@@ -1323,6 +1543,56 @@ int lbib_adj_gradient(lbib_adj* a, double* d_tau_c, double* d_kappa, double* d_r
     if (!a) return -1;
     if (a->ns != 1) { a->err = "lbib_adj_gradient: several species; use lbib_adj_gradient_ms"; return -1; }
     return lbib_adj_gradient_ms(a, d_tau_c, d_kappa, nullptr, nullptr, d_rates);
+}
+
+int lbib_adj_set_storage(lbib_adj* a, int storage) {
+    if (!a) return -1;
+    if (a->have_problem) { a->err = "lbib_adj_set_storage: choose the storage before lbib_adj_set_problem"; return -1; }
+    const bool same = storage == a->prec;
+    const bool half = (storage == LBIB_FP16_DEV || storage == LBIB_BF16_DEV) && a->prec == LBIB_FP32;
+    if (!same && !half) {
+        a->err = "lbib_adj_set_storage: the arithmetic precision, or FP16 / BF16 deviations with FP32 arithmetic";
+        return -1;
+    }
+    ADJ_HIP(hipStreamSynchronize(a->stream));
+    a->sprec = storage;
+    a->sb = same ? a->tb : 2;
+    a->have_forward = false;
+    return alloc_states(a);
+}
+
+int lbib_adj_set_checkpointing(lbib_adj* a, int scheme, int param) {
+    if (!a) return -1;
+    if ((scheme != LBIB_ADJ_CHECKPOINT_EVERY && scheme != LBIB_ADJ_REVOLVE) || param < 1) {
+        a->err = "lbib_adj_set_checkpointing: LBIB_ADJ_CHECKPOINT_EVERY with K >= 1 steps, or LBIB_ADJ_REVOLVE "
+                 "with >= 1 snapshot slots";
+        return -1;
+    }
+    ADJ_HIP(hipStreamSynchronize(a->stream));
+    adj_free(a->seg);
+    adj_free(a->ck);
+    a->nck = 0;
+    a->scheme = scheme == LBIB_ADJ_REVOLVE ? 1 : 0;
+    if (a->scheme == 1) {
+        a->snaps = param;
+    } else {
+        a->K = param;
+        if (adj_alloc(a, &a->seg, (size_t)a->K * (state_bytes(a) + extra_bytes(a)))) return -1;
+    }
+    a->have_forward = false;
+    return 0;
+}
+
+int lbib_adj_stats(const lbib_adj* a, int64_t* forward_steps, int64_t* gradient_steps, int64_t* store_bytes) {
+    if (!a) return -1;
+    if (forward_steps) *forward_steps = a->fwd_steps_run;
+    if (gradient_steps) *gradient_steps = a->grad_steps_run;
+    if (store_bytes) {
+        const size_t ck = (size_t)(a->scheme == 1 ? std::max(a->snaps, 1) : a->steps / a->K + 1) * state_bytes(a);
+        const size_t seg = a->scheme == 0 ? (size_t)a->K * (state_bytes(a) + extra_bytes(a)) : 0;
+        *store_bytes = (int64_t)(ck + seg);
+    }
+    return 0;
 }
 
 int lbib_adj_timing(lbib_adj* a, double* forward_ms, double* gradient_ms) {
