@@ -305,44 +305,109 @@ def measure_adjoint(steps, K, size=512):
             "steps": steps, "checkpoint_every": K, **out}
 
 
-def measure_adjoint_ms(size, steps, K, ncell=1600):
-    """Adjoint of the four-species junction workload: size^2 lattice, ncell
-    touching cells (r = 30 +- 5, 120 markers), junction springs between the
-    closest markers of different cells within 2 lu, four species with a
-    linear reaction matrix N(0, (1e-4)^2) and per-cell sources of species 0.
-    The target is the final concentration of every species under hidden
-    parameters; loss and gradient are evaluated at a guess.  FP32 arithmetic
-    and checkpoint storage."""
-    from lbibcell_amd import adjoint, capi, workloads
+def adjoint_ms_case(size, ncell, batch):
+    """The four-species junction workload as `batch` x `batch` independent
+    problems of (size / batch)^2 in one size^2 lattice (ncell cells in all):
+    marker rings, cell map, per-cell rates, junction pairs (within each
+    problem) and the rectangles."""
+    from lbibcell_amd import workloads
+    side = size // batch
+    rings, rates, pairs, rects = [], [], [], []
+    dom = np.zeros((size, size), dtype=np.uint32)
+    base_m = base_c = 0
+    for by in range(batch):
+        for bx in range(batch):
+            k = by * batch + bx
+            r, d, rt = workloads.tissue_adjoint_case(side, side, ncell // (batch * batch), 30.0, 5.0, 120, seed=1 + k)
+            pr = workloads.junction_pairs(r, JUNCTION["radius"])
+            off = np.array([bx * side, by * side], dtype=np.float64)
+            rings += [q + off for q in r]
+            dd = d.astype(np.uint32)
+            dom[by * side:(by + 1) * side, bx * side:(bx + 1) * side] = np.where(dd > 0, dd + base_c, 0)
+            rates.append(rt)
+            pairs.append(pr + base_m)
+            rects.append((bx * side, by * side, side, side))
+            base_m += sum(len(q) for q in r)
+            base_c += len(r)
+    return dict(rings=rings, dom=dom, rates=np.concatenate(rates), pairs=np.concatenate(pairs),
+                rects=rects if batch > 1 else None)
+
+
+def measure_adjoint_ms(size, steps, K, ncell=1600, batch=2, snapshots=20):
+    """Adjoint of the four-species junction workload: size^2 lattice holding
+    batch x batch independent problems (one summed loss, shared parameters),
+    ncell touching cells in all (r = 30 +- 5, 120 markers), junction springs
+    between the closest markers of different cells within 2 lu, four species
+    with a linear reaction matrix N(0, (1e-4)^2) and per-cell sources of
+    species 0.  The target is the final concentration of every species under
+    hidden parameters (FP64); loss and gradient are evaluated at a guess in
+    three set-ups: FP32 with checkpoints every K steps, the mixed-precision
+    one (FP16 deviation storage, FP32 arithmetic, revolve with `snapshots`
+    slots), and the FP64 reference gradient (revolve) that both are compared
+    with."""
+    from lbibcell_amd import adjoint, capi
     ns = 4
-    rings, dom, rates = workloads.tissue_adjoint_case(size, size, ncell, 30.0, 5.0, 120, seed=1)
-    pairs = workloads.junction_pairs(rings, JUNCTION["radius"])
+    case = adjoint_ms_case(size, ncell, batch)
     Km = np.random.default_rng(101).normal(0.0, 1e-4, size=(ns, ns))
     g0 = np.zeros((ns, 5, size, size))
     decay = [1e-5] * ns
-    p = adjoint.AdjointProblem(size, size, 1.0, rings, dom, decay=decay[0], precision=capi.FP32, checkpoint_every=K)
-    p.set_species(g0, np.zeros((ns, size, size)), decay)
-    p.set_junctions(pairs, JUNCTION["l0"])
-    hidden = dict(tau_c=list(MS_TAUS), kappa=0.01, k_junction=JUNCTION["k"], K=Km, rates=rates)
+    hidden = dict(tau_c=list(MS_TAUS), kappa=0.01, k_junction=JUNCTION["k"], K=Km, rates=case["rates"])
     guess = dict(tau_c=[t + 0.05 for t in MS_TAUS], kappa=0.006, k_junction=0.6 * JUNCTION["k"], K=0.7 * Km,
-                 rates=0.7 * rates)
+                 rates=0.7 * case["rates"])
+
+    def problem(precision, target, storage=None, revolve=None):
+        p = adjoint.AdjointProblem(size, size, 1.0, case["rings"], case["dom"], decay=decay[0], precision=precision,
+                                   checkpoint_every=K, storage=storage, revolve_snapshots=revolve)
+        p.set_species(g0, target, decay)
+        p.set_junctions(case["pairs"], JUNCTION["l0"])
+        if case["rects"]:
+            p.set_batch_layout(case["rects"])
+        return p
+
+    p = problem(capi.FP64, np.zeros((ns, size, size)), revolve=snapshots)
     _, target = p.forward_ms(steps, want_field=True, **hidden)
-    p.set_species(g0, target, decay)
-    del target
-    p.forward_ms(steps, **guess)          # warm-up of the timed pair
-    p.gradient_ms()
-    loss = p.forward_ms(steps, **guess)
-    g = p.gradient_ms()
-    fwd_ms, grad_ms = p.timing()
     p.close()
-    return {"workload": f"adjoint {size}x{size}, {ncell} touching cells r=30+-5 (120 markers), {len(pairs)} junction "
-                        f"springs within {JUNCTION['radius']} lu, 4 species with a linear reaction matrix, {steps} "
-                        f"steps, checkpoints every {K}, FP32; loss = L2 mismatch of the final concentrations against "
-                        "a target from hidden parameters",
-            "steps": steps, "checkpoint_every": K, "markers": int(sum(len(r) for r in rings)),
-            "junction_springs": int(len(pairs)), "loss": loss, "forward_ms": fwd_ms, "gradient_ms": grad_ms,
-            "adjoint_to_forward": grad_ms / fwd_ms, "d_tau_c": list(g["tau_c"]), "d_kappa": g["kappa"],
-            "d_k_junction": g["k_junction"], "d_K": g["K"].tolist(), "d_rates_norm": float(np.linalg.norm(g["rates"]))}
+    runs = {}
+    for name, prec, storage, rev in (("fp64_revolve", capi.FP64, None, snapshots), ("fp32", capi.FP32, None, None),
+                                     ("fp16_revolve", capi.FP32, capi.FP16_DEV, snapshots)):
+        log(f"  adjoint four species: {name}")
+        p = problem(prec, target, storage, rev)
+        p.forward_ms(steps, **guess)          # warm-up of the timed pair
+        p.gradient_ms()
+        loss = p.forward_ms(steps, **guess)
+        g = p.gradient_ms()
+        fwd_ms, grad_ms = p.timing()
+        st = p.stats()
+        p.close()
+        runs[name] = dict(loss=loss, g=g, forward_ms=fwd_ms, gradient_ms=grad_ms, stats=st)
+    ref = runs["fp64_revolve"]["g"]
+
+    def rel(g):
+        return {k: float(np.max(np.abs(np.asarray(g[k]) - np.asarray(ref[k]))) /
+                         max(float(np.max(np.abs(np.asarray(ref[k])))), 1e-300)) for k in ref}
+
+    out = {"workload": f"adjoint {size}x{size} holding {batch * batch} independent {size // batch}^2 problems "
+                       f"(one batched lattice, summed loss), {ncell} touching cells r=30+-5 (120 markers), "
+                       f"{len(case['pairs'])} junction springs within {JUNCTION['radius']} lu, 4 species with a linear "
+                       f"reaction matrix, {steps} steps; loss = L2 mismatch of the final concentrations against a "
+                       "target from hidden parameters",
+           "steps": steps, "markers": int(sum(len(r) for r in case["rings"])),
+           "junction_springs": int(len(case["pairs"])), "problems": batch * batch}
+    for name, r in runs.items():
+        e = {"loss": r["loss"], "forward_ms": r["forward_ms"], "gradient_ms": r["gradient_ms"],
+             "adjoint_to_forward": r["gradient_ms"] / r["forward_ms"],
+             "checkpointing": (f"revolve, {snapshots} snapshot slots" if name.endswith("revolve")
+                               else f"every {K} steps + {K}-state segment store"),
+             "storage": {"fp64_revolve": "fp64", "fp32": "fp32", "fp16_revolve": "fp16 deviations, fp32 arithmetic"}[name],
+             "checkpoint_store_gb": r["stats"]["store_bytes"] / 1e9,
+             "gradient_recomputed_steps": r["stats"]["gradient_steps"]}
+        if name != "fp64_revolve":
+            e["gradient_rel_error_vs_fp64"] = rel(r["g"])
+        out[name] = e
+    g = ref
+    out["fp64_gradient"] = {"d_tau_c": list(g["tau_c"]), "d_kappa": g["kappa"], "d_k_junction": g["k_junction"],
+                            "d_K": g["K"].tolist(), "d_rates_norm": float(np.linalg.norm(g["rates"]))}
+    return out
 
 
 LONG_CHECKS = (100, 1000, 5000)
@@ -436,6 +501,9 @@ def main():
     ap.add_argument("--adjoint-ms-steps", type=int, default=200,
                     help="steps of the four-species junction adjoint window (0 = skip)")
     ap.add_argument("--adjoint-ms-size", type=int, default=4096)
+    ap.add_argument("--adjoint-ms-batch", type=int, default=2,
+                    help="the four-species adjoint lattice holds batch x batch independent problems")
+    ap.add_argument("--adjoint-ms-snapshots", type=int, default=20, help="revolve snapshot slots")
     ap.add_argument("--multispecies", default="fp16,fp16r,fp64",
                     help="storage precisions of the 4096^2 four-species junction workload ('' = skip); "
                          "fp64 also serves as the reference of the mixed errors")
@@ -613,8 +681,12 @@ def main():
         log(f"adjoint/forward time ratio (fp32): {out['adjoint']['fp32']['adjoint_to_forward']:.2f}")
     if args.adjoint_ms_steps > 0:
         log(f"four-species adjoint: {args.adjoint_ms_size}^2, {args.adjoint_ms_steps} steps ...")
-        out["adjoint_multispecies"] = measure_adjoint_ms(args.adjoint_ms_size, args.adjoint_ms_steps, 50)
-        log(f"four-species adjoint/forward time ratio: {out['adjoint_multispecies']['adjoint_to_forward']:.2f}")
+        out["adjoint_multispecies"] = measure_adjoint_ms(args.adjoint_ms_size, args.adjoint_ms_steps, 50,
+                                                         batch=args.adjoint_ms_batch,
+                                                         snapshots=args.adjoint_ms_snapshots)
+        am = out["adjoint_multispecies"]
+        log("four-species adjoint/forward time ratio: " +
+            ", ".join(f"{k} {am[k]['adjoint_to_forward']:.2f}" for k in ("fp64_revolve", "fp32", "fp16_revolve")))
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(nx, ny, nodes, conns)

@@ -1090,56 +1090,72 @@ int reverse_step(lbib_adj* a, const Slot<T>& st, const Extra<T>& e, int& cur) {
     return 0;
 }
 
-// Revolve: the working state ping-pongs between f/g[0] and f/g[1] (index
-// w), markers in X; slot -1 is the initial state.
+// Revolve.  The current state is a view: a snapshot slot (or the initial
+// state, slot -1), read in place, or one of the working buffers f/g[0..1]
+// after steps were taken from it.  Markers move in place, so a view of a
+// slot first copies its markers to X (2 nm values, not the populations).
 template <typename T>
 struct Revolver {
     lbib_adj* a;
-    int w = 0, cur = 0;
+    int cur = 0;
+    Slot<T> st{};   // current state
+    int w = -1;     // working buffer holding st, -1: a slot
     Slot<T> work(int k) const { return {a->f[k], a->g[k], tp<T>(a->X)}; }
     Slot<T> source(int slot) const {
         return slot < 0 ? Slot<T>{a->f0, a->g0, tp<T>(a->X0)} : state_slot<T>(a->ck, a, slot);
     }
-    int restore(int slot) {
-        w = 0;
-        return copy_state<T>(a, work(0), source(slot));
+    void view(int slot) {
+        st = source(slot);
+        w = -1;
     }
+    // working buffer that does not hold the current state
+    int other() const { return w == 0 ? 1 : 0; }
     int advance(int nsteps) {
         for (int k = 0; k < nsteps; k++) {
-            if (step_fwd<T>(a, a->f[w], a->g[w], a->f[1 - w], a->g[1 - w], tp<T>(a->X), tp<T>(a->Fs), tp<T>(a->Fm),
-                            tp<T>(a->u)))
+            if (w < 0 && a->nm > 0)
+                ADJ_HIP(hipMemcpyAsync(a->X, st.X, xbytes(a), hipMemcpyDeviceToDevice, a->stream));
+            const int d = other();
+            if (step_fwd<T>(a, st.f, st.g, a->f[d], a->g[d], tp<T>(a->X), tp<T>(a->Fs), tp<T>(a->Fm), tp<T>(a->u)))
                 return -1;
-            w = 1 - w;
+            st = work(d);
+            w = d;
             a->grad_steps_run++;
         }
         return 0;
     }
-    // reverse of the step from the working state: recompute it into the
-    // other buffers with the markers moved in Xs, keeping X at time t
+    // reverse of the step from the current state: recompute it into the
+    // other working buffer with the markers moved in Xs, st.X staying at t
     int reverse_one() {
-        if (a->nm > 0) ADJ_HIP(hipMemcpyAsync(a->Xs, a->X, xbytes(a), hipMemcpyDeviceToDevice, a->stream));
-        if (step_fwd<T>(a, a->f[w], a->g[w], a->f[1 - w], a->g[1 - w], tp<T>(a->Xs), tp<T>(a->Fs), tp<T>(a->Fm),
-                        tp<T>(a->u)))
+        if (a->nm > 0) ADJ_HIP(hipMemcpyAsync(a->Xs, st.X, xbytes(a), hipMemcpyDeviceToDevice, a->stream));
+        const int d = other();
+        if (step_fwd<T>(a, st.f, st.g, a->f[d], a->g[d], tp<T>(a->Xs), tp<T>(a->Fs), tp<T>(a->Fm), tp<T>(a->u)))
             return -1;
         a->grad_steps_run++;
         const Extra<T> e{tp<T>(a->Fs), tp<T>(a->u), tp<T>(a->Fm)};
-        return reverse_step<T>(a, work(w), e, cur);
+        return reverse_step<T>(a, st, e, cur);
     }
     // adjoint from t1 back to t0, the state of t0 being in `slot`; `free`
     // snapshot slots from a->snaps - free up; on the right spine the
     // forward sweep already stored the snapshots
     int run(int t0, int slot, int t1, int free, bool spine) {
         const int l = t1 - t0;
-        if (l == 1) return restore(slot) || reverse_one();
+        if (l == 1) {
+            view(slot);
+            return reverse_one();
+        }
         if (free == 0) {
-            for (int t = t1 - 1; t >= t0; t--)
-                if (restore(slot) || advance(t - t0) || reverse_one()) return -1;
+            for (int t = t1 - 1; t >= t0; t--) {
+                view(slot);
+                if (advance(t - t0) || reverse_one()) return -1;
+            }
             return 0;
         }
         const int m = t0 + rev_split(l, free);
         const int k = a->snaps - free;
-        if (!spine && (restore(slot) || advance(m - t0) || copy_state<T>(a, state_slot<T>(a->ck, a, k), work(w))))
-            return -1;
+        if (!spine) {
+            view(slot);
+            if (advance(m - t0) || copy_state<T>(a, state_slot<T>(a->ck, a, k), st)) return -1;
+        }
         if (run(m, k, t1, free - 1, spine)) return -1;
         return run(t0, slot, m, free, false);
     }
