@@ -152,7 +152,13 @@ def test_mixed_coupled_run_error_against_fp64(prec):
     e_x = rel(mx["xy"], ref["xy"]) if same_topology else float("nan")
     e_a = abs(mx["area"][1] - ref["area"][1]) / ref["area"][1]
     print(f"  L2 rel error: speed {e_u:.2e} concentration {e_c:.2e} markers {e_x:.2e} area {e_a:.2e}", flush=True)
-    # measured on MI355X: FP32 9e-5 / 9e-6 / 4e-7, FP16 3e-4 / 4e-4 / 4e-7,
-    # BF16 5e-3 / 2e-2 / 7e-6 (speed / concentration / area)
-    bound = {"fp32": 5e-4, "fp16": 3e-3, "bf16": 1e-1, "fp16r": 5e-4}[prec]
-    assert e_u <= bound and e_c <= bound and e_a <= bound
+    # Each format pinned at about twice its measured error (the run is
+    # deterministic: the mixed force field is an order-free integer sum).
+    # Measured on MI355X (speed / concentration / markers / area):
+    #   FP32  9.6e-5 / 9.2e-6 / 1.1e-7 / 3.6e-7    FP16R 9.6e-5 / 9.5e-6 / 1.1e-7 / 3.7e-7
+    #   FP16  3.5e-4 / 4.5e-4 / 9.2e-7 / 4.4e-7    BF16  4.9e-3 / 2.2e-2 / 6.9e-6 / 7.1e-6
+    # FP32 and the remainder store keep speed and concentration within 1e-4.
+    bounds = {"fp32": (1e-4, 2e-5, 3e-7, 1e-6), "fp16r": (1e-4, 2e-5, 3e-7, 1e-6),
+              "fp16": (7e-4, 9e-4, 2e-6, 1e-6), "bf16": (1e-2, 4e-2, 1.5e-5, 1.5e-5)}[prec]
+    assert e_u <= bounds[0] and e_c <= bounds[1] and e_a <= bounds[3]
+    assert not same_topology or e_x <= bounds[2]
