@@ -13,6 +13,15 @@ using namespace lbib;
 
 namespace lbib {
 __global__ void k_step_begin(DevStatus* st, uint32_t it) { step_begin_dev(st, it); }
+
+// the main stream waits for the topology stage still running on the second
+// stream (its kernels raise stops the next iteration's begin must see)
+int join_topology(lbib_ctx* c) {
+    if (!c->topo_pending) return 0;
+    c->topo_pending = false;
+    LBIB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    return 0;
+}
 }  // namespace lbib
 
 namespace {
@@ -61,6 +70,7 @@ size_t field_bytes(int prec, int planes) {
 int ensure_scan(lbib_ctx* c) {
     const uint32_t need =
         (uint32_t)((std::max({c->mcap, c->ccap, c->dcap, c->stx * c->sty, 1}) + 1023) / 1024 + 1);
+    if (!c->rm_list && dev_alloc(c, &c->rm_list, RM_LIST_CAP)) return -1;
     if (need <= c->scan_cap) return 0;
     dev_free(c->scan_tmp);
     if (dev_alloc(c, &c->scan_tmp, need)) return -1;
@@ -173,7 +183,22 @@ int flush_step_begin(lbib_ctx* c) {
     LBIB_HIP(hipGetLastError());
     return 0;
 }
-int check_ctx(lbib_ctx* c) { return c ? flush_step_begin(c) : -1; }
+// Every entry point: later work sees the topology stage and the pending step
+// begin; the marker forces of the next lattice stage then run on the main
+// stream (the call may change what they read).
+int check_ctx(lbib_ctx* c) {
+    if (!c) return -1;
+    c->forces_beside = false;
+    if (join_topology(c)) return -1;
+    return flush_step_begin(c);
+}
+// Entry points that change nothing the marker forces and binning read (the
+// area regulator, the step begin) keep them on the second stream.
+int check_ctx_keep_forces(lbib_ctx* c) {
+    if (!c) return -1;
+    if (join_topology(c)) return -1;
+    return flush_step_begin(c);
+}
 int check_ctx_keep_begin(lbib_ctx* c) { return c ? 0 : -1; }
 int upload(lbib_ctx* c, void* dst, const void* src, size_t bytes);
 int download(lbib_ctx* c, void* dst, const void* src, size_t bytes);
@@ -351,8 +376,10 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     }
     LBIB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     LBIB_HIP(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-    LBIB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-    LBIB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    // the fork and join only order work on this device: a device-scope
+    // release is enough (the default system-scope fence writes back caches)
+    LBIB_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming | hipEventReleaseToDevice));
+    LBIB_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming | hipEventReleaseToDevice));
     const long n = c->n;
     const long npad = (n + 63) / 64 * 64;   // whole 64-site blocks for 32- and 16-bit storage
     for (int b = 0; b < 2; b++) {
@@ -411,6 +438,7 @@ void lbib_destroy(lbib_ctx* c) {
     }
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (int b = 0; b < 2; b++) { dev_free(c->f[b]); dev_free(c->g[b]); }
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
@@ -421,9 +449,9 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->pslot); dev_free(c->pe_x); dev_free(c->pe_y); dev_free(c->pe_fx); dev_free(c->pe_fy);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
                          &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
-    uint32_t** uarr[18] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
+    uint32_t** uarr[19] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
                            &c->pos_r, &c->mdom, &c->cdom, &c->cdom2, &c->flag_c, &c->rank_c, &c->cdead, &c->keep_c,
-                           &c->pos_c, &c->ring_slot, &c->scan_tmp};
+                           &c->pos_c, &c->ring_slot, &c->scan_tmp, &c->rm_list};
     int32_t** iarr[16] = {&c->min_c, &c->mout_c, &c->rec_a, &c->rec_b, &c->rec_bc, &c->fs_off, &c->ca, &c->cb,
                           &c->cbc, &c->ca2, &c->cb2, &c->cbc2, &c->ring_bucket, &c->ring_marker, &c->fs_type,
                           &c->fs_partner};
@@ -919,8 +947,10 @@ int lbib_set_forces(lbib_ctx* c, int nm, const int32_t* off, const int32_t* type
 // ------------------------------------------------------------------- step --
 static int next_event(lbib_ctx* c, hipEvent_t* ev) {
     if (c->ev_used == c->ev_pool.size()) {
+        // device-scope release: the timing pair brackets one kernel on this
+        // device without a system-scope cache write-back around it
         hipEvent_t e;
-        LBIB_HIP(hipEventCreate(&e));
+        LBIB_HIP(hipEventCreateWithFlags(&e, hipEventReleaseToDevice));
         c->ev_pool.push_back(e);
     }
     *ev = c->ev_pool[c->ev_used++];
@@ -1093,7 +1123,7 @@ int lbib_read_status(lbib_ctx* c, lbib_status* out) {
     return 0;
 }
 int lbib_step_begin(lbib_ctx* c, uint32_t it) {
-    if (check_ctx(c)) return -1;
+    if (check_ctx_keep_forces(c)) return -1;
     c->begin_pending = true;
     c->begin_it = it;
     if (!c->snaps.empty() && c->snaps.back().it >= it) c->snaps.clear();   // a new run
@@ -1105,6 +1135,8 @@ int lbib_step_begin(lbib_ctx* c, uint32_t it) {
 int lbib_rewind(lbib_ctx* c, uint32_t it) {
     if (check_ctx_keep_begin(c)) return -1;
     if (c->begin_pending && c->begin_it >= it) c->begin_pending = false;   // never reached the device
+    c->forces_beside = false;
+    if (join_topology(c)) return -1;
     if (flush_step_begin(c)) return -1;
     if (c->snaps.empty() || it > c->snaps.back().it) return 0;   // iteration it never began
     while (!c->snaps.empty() && c->snaps.back().it > it) c->snaps.pop_back();
@@ -1161,7 +1193,7 @@ int lbib_domain_concentrations(lbib_ctx* c, int species, int ndom, double* out) 
     return download(c, out, c->dconc, ndom * sizeof(double));
 }
 int lbib_area_regulator(lbib_ctx* c, double target_area, double kp, double maxsource, double target_factor) {
-    if (check_ctx(c)) return -1;
+    if (check_ctx_keep_forces(c)) return -1;
     return launch_area_regulator(c, target_area, kp, maxsource, target_factor);
 }
 int lbib_add_mass_by_domain(lbib_ctx* c, int ndom, const double* table) {
@@ -1338,8 +1370,10 @@ int lbib_capture_end(lbib_ctx* c, int* ok) {
     if (!c) return -1;
     if (ok) *ok = 0;
     if (!c->capturing) { c->err = "lbib_capture_end: not recording"; return -1; }
-    // a step begin still pending belongs to the recorded block
-    const int fl = flush_step_begin(c);
+    // the second stream rejoins the recorded stream; a step begin still
+    // pending belongs to the recorded block
+    c->forces_beside = false;
+    const int fl = (join_topology(c) != 0 || flush_step_begin(c) != 0) ? -1 : 0;
     c->capturing = false;
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(c->stream, &g);

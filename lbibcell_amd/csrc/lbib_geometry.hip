@@ -10,7 +10,10 @@
 // keeps per (site, slot) the closest node, the first inserted on ties.  Here
 // each connection writes its candidates with a global insertion number
 // seq = conn * 256 + k; three passes pick min distance, then min seq among
-// equal distances, which is exactly the sequential outcome.
+// equal distances, which is exactly the sequential outcome.  The distance
+// pass marks the entries where two candidates met at the same distance; the
+// sequence pass runs its atomics only there (distances rarely tie), and an
+// unmarked entry's winner is its one candidate at the minimum distance.
 #include "lbib_device_math.h"
 
 namespace lbib {
@@ -20,6 +23,9 @@ __device__ __forceinline__ bool halted(const DevStatus* st) { return (st->bits &
 // closest-wins selection entries, site-major: the four slots of a site share
 // one 32-byte (distance) / 16-byte (sequence) span
 __device__ __forceinline__ long sel_index(int site, int slot) { return (long)site * 4 + (slot - 1); }
+// best_seq of a selection entry where two candidates met at the same distance
+// (above every insertion number, below the cleared value)
+constexpr uint32_t SEQ_TIE = 0xfffffffeu;
 
 // Start of the geometry stage, one launch: the Connection::perturbConnection
 // trigger (Connection.cpp:339-340, 363-364), deletion of the boundary nodes of
@@ -41,7 +47,9 @@ __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ c
     }
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nprev; j += stride) {
         const int site = cand[j].site, slot = cand[j].slot;
-        atomicAnd(&tag[site], DOM_MASK);
+        // every thread touching a site stores the same value: no atomic
+        const uint32_t t0 = tag[site];
+        if (t0 & ~DOM_MASK) tag[site] = t0 & DOM_MASK;
         const long b = sel_index(site, slot);
         best_dist[b] = ~0ull;
         best_seq[b] = ~0u;
@@ -52,7 +60,7 @@ __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ c
         st->nchanged = 0;
         st->rm_total = 0;
         st->co_removed = 0;
-        st->rm_any = 0;
+        st->rm_K = 0;
         st->rm_L = 0;
         st->topo_changed = 0;
         st->reinit_levels = 0;
@@ -73,7 +81,7 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
                           const int32_t* __restrict__ cbc, const double* __restrict__ mx,
                           const double* __restrict__ my, const double* __restrict__ mvx,
                           const double* __restrict__ mvy, int want_bc, Cand* cand,
-                          int candcap, unsigned long long* best_dist, DevStatus* st) {
+                          int candcap, unsigned long long* best_dist, uint32_t* best_seq, DevStatus* st) {
     if (halted(st)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = c < (int)st->nc;
@@ -133,7 +141,11 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
         }
         e.win = 0;
         cand[base + k] = e;
-        atomicMin(&best_dist[sel_index(e.site, slot)], (unsigned long long)__double_as_longlong(e.dist2));
+        const long b = sel_index(e.site, slot);
+        const unsigned long long key = (unsigned long long)__double_as_longlong(e.dist2);
+        // a candidate meeting an equal distance already in place marks the
+        // entry: only such entries need the insertion-number pass
+        if (atomicMin(&best_dist[b], key) == key) best_seq[b] = SEQ_TIE;
         k++;
     };
     // vertical lattice lines: node S at (i, floor y) [its slot N], node N at (i, ceil y) [slot S]
@@ -160,7 +172,8 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
         const Cand e = cand[j];
         const long b = sel_index(e.site, e.slot);
-        if ((unsigned long long)__double_as_longlong(e.dist2) == best_dist[b]) atomicMin(&best_seq[b], e.seq);
+        if (best_seq[b] != ~0u && (unsigned long long)__double_as_longlong(e.dist2) == best_dist[b])
+            atomicMin(&best_seq[b], e.seq);
     }
 }
 
@@ -189,7 +202,8 @@ __global__ __launch_bounds__(1024) void k_cut_win(const Cand* cand, long n, cons
             const long slotix = (long)(e.slot - 1) * n + e.site;
             const long b = sel_index(e.site, e.slot);
             const uint32_t dom_now = tag[e.site] & DOM_MASK;   // domain bits are not changed here
-            if (best_seq[b] == e.seq) {
+            const uint32_t bs = best_seq[b];
+            if (bs == ~0u ? (unsigned long long)__double_as_longlong(e.dist2) == best_dist[b] : bs == e.seq) {
                 won = 1;
                 site = e.site;
                 dom = e.domain;
@@ -496,7 +510,7 @@ static void launch_reinit_t(lbib_ctx* c, const ReinitArgs& a) {
     }
 }
 
-int launch_topology(lbib_ctx* c, bool coarsen, bool in_step);
+int launch_topology(lbib_ctx* c, bool coarsen);
 
 int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     // mode: 0 = moveLattice (local update + reinit), 1 = cureLattice (scanline only),
@@ -513,7 +527,8 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     if (c->ccap > 0)
         k_cut_gen<<<(c->ccap + 1023) / 1024, 1024, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom, c->cbc,
                                                                c->mx, c->my, c->mvx, c->mvy, c->bc_v ? 1 : 0,
-                                                               c->cand, c->candcap, c->best_dist, c->st);
+                                                               c->cand, c->candcap, c->best_dist, c->best_seq,
+                                                               c->st);
     // From here on nothing of this stage reads the connections or markers:
     // the topology stage (remesh, rings, coarsening), which reads and
     // rewrites them, runs beside the rest on the second stream.
@@ -523,7 +538,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
         LBIB_HIP(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
         hipStream_t main = c->stream;
         c->stream = c->stream2;
-        const int r = launch_topology(c, coarsen, true);
+        const int r = launch_topology(c, coarsen);
         c->stream = main;
         if (r) return -1;
         LBIB_HIP(hipEventRecord(c->ev_join, c->stream2));
@@ -548,15 +563,21 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     }
     k_scanline<<<c->ny, 256, 0, c->stream>>>(c->nx, c->n, c->tag, c->bn_dom, c->st);
     LBIB_HIP(hipGetLastError());
-    if (beside) LBIB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    // the main stream waits for the topology stage at the next entry point
+    // (join_topology), or after the next lattice stage's marker forces and
+    // binning, which then follow the topology stage on the second stream
+    if (beside) {
+        c->topo_pending = true;
+        c->forces_beside = true;
+    }
     return 0;
 }
 
 // remeshBoundary, then at it % 10 == 0 deleteAllForces + coarsenBoundary
 // (SimulationRunner.cpp:83-87); rings are rebuilt for the area helpers.
-int launch_topology(lbib_ctx* c, bool coarsen, bool in_step) {
+int launch_topology(lbib_ctx* c, bool coarsen) {
     if (c->nc == 0) return 0;
-    if (launch_remesh(c, 1, in_step)) return -1;
+    if (launch_remesh(c, 1)) return -1;
     // the ring order is rebuilt only when remeshing or coarsening changed
     // the topology in this step (st->topo_changed)
     if (launch_rings(c, true, true)) return -1;

@@ -24,8 +24,8 @@ namespace lbib {
 
 // ForceSolver::calculateForce (ForceSolver.cpp:42-85) with the force structs of
 // ForceStructs.hpp:49-588; per marker, structs in map/push order.
-// Also the first kernel of the lattice stage: opens a pending iteration
-// (lbib_step_begin) and clears the spread tiles.
+// Also the first kernel of the lattice stage: clears the spread tiles and,
+// when it runs on the main stream, opens a pending iteration (lbib_step_begin).
 __global__ void k_marker_force(const double* __restrict__ mx, const double* __restrict__ my,
                                const int32_t* __restrict__ off, const int32_t* __restrict__ type,
                                const int32_t* __restrict__ partner, const double* __restrict__ prm,
@@ -281,8 +281,9 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
                                                      const uint32_t* __restrict__ tpos, const double* __restrict__ mx,
                                                      const double* __restrict__ my, const double* __restrict__ mfx,
                                                      const double* __restrict__ mfy, double* Fx, double* Fy,
-                                                     DevStatus* st) {
+                                                     int begin, uint32_t begin_it, DevStatus* st) {
     static_assert(SP_CH == 128, "two 64-bit site masks per staged chunk");
+    if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     const int t = MAP == 0 ? (int)xcd_contiguous(blockIdx.x, gridDim.x) : row_interleaved_tile(blockIdx.x, ntx, nty);
     if (t < 0 || !tact[t] || step_stopped(st)) return;
     __shared__ SpreadLds L;
@@ -382,7 +383,9 @@ template <int MAP>
 __global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, int nty, const uint32_t* __restrict__ toff,
                                                       const uint32_t* __restrict__ tcnt, const double* __restrict__ ex,
                                                       const double* __restrict__ ey, const double* __restrict__ efx,
-                                                      const double* __restrict__ efy, float2* F, DevStatus* st) {
+                                                      const double* __restrict__ efy, float2* F, int begin,
+                                                      uint32_t begin_it, DevStatus* st) {
+    if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     const int t = MAP == 0 ? (int)xcd_contiguous(blockIdx.x, gridDim.x) : row_interleaved_tile(blockIdx.x, ntx, nty);
     if (t < 0 || step_stopped(st)) return;
     const uint32_t n = tcnt[t];
@@ -518,16 +521,30 @@ __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mv
 
 static inline int nblocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// The marker forces and the tile binning read the topology, the marker
+// positions and the force structs only.  When nothing enqueued since the last
+// geometry stage changed those (forces_beside), they follow the topology stage
+// on the second stream, beside the main stream's cut links, re-initialisation
+// and scanline; the main stream waits for them before the spread, whose first
+// workgroup then opens the iteration (after every stop of the last one is
+// raised).  Otherwise all of it runs on the main stream, the force kernel
+// opening the iteration.
 int launch_force_and_spread(lbib_ctx* c) {
     const int bs = 256;
     const int ntiles = c->stx * c->sty;
+    const int begin = c->begin_pending ? 1 : 0;
+    const uint32_t begin_it = c->capturing ? ITER_NEXT : c->begin_it;
+    c->begin_pending = false;
+    const bool beside = c->forces_beside && c->nm > 0;
+    c->forces_beside = false;
+    if (!beside && join_topology(c)) return -1;
+    const hipStream_t main = c->stream;
+    if (beside) c->stream = c->stream2;
     // marker counts are device-resident (remeshing changes them): grids cover
     // the capacity and the kernels read st->nm
-    const int begin = c->begin_pending ? 1 : 0;
-    c->begin_pending = false;
     k_marker_force<<<std::max(nblocks(c->mcap, bs), 1), bs, 0, c->stream>>>(
-        c->mx, c->my, c->fs_off, c->fs_type, c->fs_partner, c->fs_prm, c->mfx, c->mfy, ntiles, c->tcnt, c->tact, begin,
-        c->capturing ? ITER_NEXT : c->begin_it, c->st);
+        c->mx, c->my, c->fs_off, c->fs_type, c->fs_partner, c->fs_prm, c->mfx, c->mfy, ntiles, c->tcnt, c->tact,
+        beside ? 0 : begin, begin_it, c->st);
     if (c->nm == 0) {
         LBIB_HIP(hipGetLastError());
         return 0;
@@ -536,36 +553,49 @@ int launch_force_and_spread(lbib_ctx* c) {
     // rectangles are unevenly spread); one lattice keeps tiles together
     const int map = c->nrect > 0 ? 1 : 0;
     const int grid = map ? 8 * ((c->sty + 7) / 8) * c->stx : ntiles;
-    if (c->prec != LBIB_FP64) {
+    const bool mixed = c->prec != LBIB_FP64;
+    int r = 0;
+    if (mixed) {
         k_push_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->tcnt, c->pslot, c->mcap,
                                                                  c->tact, c->st);
-        if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;
-        k_push_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->mfx, c->mfy, c->toff,
-                                                                   c->pslot, c->mcap, c->pe_x, c->pe_y, c->pe_fx,
-                                                                   c->pe_fy, c->st);
+        r = scan_tiles(c, c->tact, &c->st->ntouched);
+        if (!r)
+            k_push_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->mfx, c->mfy, c->toff,
+                                                                       c->pslot, c->mcap, c->pe_x, c->pe_y, c->pe_fx,
+                                                                       c->pe_fy, c->st);
+    } else {
+        k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
+                                                                 c->tact, c->st);
+        r = scan_tiles(c, c->tact, &c->st->ntouched);
+        if (!r)
+            k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot,
+                                                                       c->tlist, c->tpos, c->st);
+    }
+    c->stream = main;
+    if (r) return -1;
+    const int sbegin = beside ? begin : 0;
+    if (beside) {
+        LBIB_HIP(hipEventRecord(c->ev_join, c->stream2));
+        LBIB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        c->topo_pending = false;
+    }
+    if (mixed) {
         float2* F = reinterpret_cast<float2*>(c->Fx);
         if (map)
             k_spread_fixed<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->toff, c->tcnt, c->pe_x,
-                                                          c->pe_y, c->pe_fx, c->pe_fy, F, c->st);
+                                                          c->pe_y, c->pe_fx, c->pe_fy, F, sbegin, begin_it, c->st);
         else
             k_spread_fixed<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->toff, c->tcnt, c->pe_x,
-                                                          c->pe_y, c->pe_fx, c->pe_fy, F, c->st);
-        LBIB_HIP(hipGetLastError());
-        return 0;
-    }
-    k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
-                                                             c->tact, c->st);
-    if (scan_tiles(c, c->tact, &c->st->ntouched)) return -1;
-    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist, c->tpos,
-                                                               c->st);
-    if (map)
+                                                          c->pe_y, c->pe_fx, c->pe_fy, F, sbegin, begin_it, c->st);
+    } else if (map) {
         k_tile_spread<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                            c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx,
-                                                            c->Fy, c->st);
-    else
+                                                      c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy,
+                                                      sbegin, begin_it, c->st);
+    } else {
         k_tile_spread<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                            c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx,
-                                                            c->Fy, c->st);
+                                                      c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy,
+                                                      sbegin, begin_it, c->st);
+    }
     LBIB_HIP(hipGetLastError());
     return 0;
 }

@@ -30,6 +30,9 @@ constexpr int BC_SHIFT = 28;
 constexpr int TILE_W = 256;         // lattice kernel block width = force tile width
 constexpr int MAX_BC_SPECIES = LBIB_MAX_SPECIES;
 constexpr int REINIT_LDS = 8192;          // level state of the changed sites kept in LDS
+// Long connections of one remesh round ranked by a sort of their list; more
+// of them fall back to a scan of the per-connection flags.
+constexpr uint32_t RM_LIST_CAP = 4096;
 constexpr int REINIT_CAP = 1 << 20;       // changed sites per step (global-memory state beyond REINIT_LDS)
 constexpr int SEQ_PER_CONN = 256;   // candidates per connection (segment < 127 lu)
 
@@ -60,7 +63,8 @@ struct DevStatus {
     uint32_t step_it;       // iteration of the last step begun with a clear status
     uint32_t rm_nc0, rm_nm0;   // counts before the current remesh round
     uint32_t reinit_levels;    // dependency levels of this step's CDE re-initialisation
-    uint32_t rm_any;           // the first remesh round of this step found a long connection
+    uint32_t rm_K;             // long connections listed by the current remesh round's flag pass
+    uint32_t rm_sparse;        // the round's ranks come from the sorted list (else from the scan)
     uint32_t topo_changed;     // remesh or coarsening changed the topology in this step
 };
 
@@ -176,9 +180,14 @@ struct lbib_ctx {
     // spread over 16x16 tiles
     int stx = 0, sty = 0;
     // second stream: the topology stage (remesh, rings, coarsening) runs on it
-    // beside the cut-link winners, re-initialisation and scanline
+    // beside the cut-link winners, re-initialisation and scanline, and the
+    // next lattice stage's marker forces and tile binning follow it there.
+    // topo_pending: the main stream has not yet waited for the topology stage
+    // (ev_join); forces_beside: nothing enqueued since that stage changed what
+    // the marker forces and binning read, so they may run on the second stream
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool topo_pending = false, forces_beside = false;
     uint32_t *tcnt = nullptr, *toff = nullptr, *tslot = nullptr;
     uint8_t* tact = nullptr;
     int32_t* tlist = nullptr;
@@ -231,6 +240,7 @@ struct lbib_ctx {
     int32_t* ring_marker = nullptr;
     uint32_t* scan_tmp = nullptr;
     uint32_t scan_cap = 0;
+    uint32_t* rm_list = nullptr;       // a remesh round's long connections (RM_LIST_CAP entries)
     int bcd_cap = 0;
     int32_t *bcd_off = nullptr, *bcd_sp = nullptr;
     int nbcd = 0;
@@ -400,13 +410,13 @@ struct AreaMass {
 // mass != nullptr: also the area regulator's mass table, skipped while the
 // status is stopped (an iteration's BioSolver stage)
 int launch_ring_areas(lbib_ctx* c, const AreaMass* mass);
-// also clears the ring counters when the topology changed; in_step: the
-// geometry stage's own pass (its scan is skipped when no connection is long)
-int launch_remesh(lbib_ctx* c, int rounds, bool in_step = false);
+// also clears the ring counters when the topology changed
+int launch_remesh(lbib_ctx* c, int rounds);
 // per-domain sum of one species' concentration into acc[ndom] (zeroed by the caller)
 int launch_domain_concentrations(lbib_ctx* c, int species, int ndom, double* acc);
 // a live context with any deferred step begin enqueued (-1: null context)
 int ctx_ready(lbib_ctx* c);
+int join_topology(lbib_ctx* c);
 int launch_coarsen(lbib_ctx* c);
 int launch_forces_clear(lbib_ctx* c);
 // two structs of `type` per marker, one per adjacent connection (type 6:
@@ -437,7 +447,7 @@ constexpr int JUNCTION_MAXIN = 16;
 int launch_forces_junctions(lbib_ctx* c, double radius, double k, double l0);
 // markers binned by the 16x16 tile of floor(X), floor(Y) (tcnt/toff/tlist)
 int launch_tile_bins(lbib_ctx* c);
-int launch_topology(lbib_ctx* c, bool coarsen, bool in_step = false);
+int launch_topology(lbib_ctx* c, bool coarsen);
 // FP64 <-> storage conversion of the current buffers (mixed precision only)
 int launch_pack(lbib_ctx* c, const double* src_dev, int field, int nq);
 int launch_unpack(lbib_ctx* c, double* dst_dev, int field, int nq);

@@ -33,7 +33,8 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
     if (step_stopped(a.st)) return;
     const int W = (int)blockDim.x;
     const uint32_t wg = xcd_contiguous(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
-    const int xb = (int)(wg % gridDim.x) * (W * SPT) + threadIdx.x;
+    const int x0w = (int)(wg % gridDim.x) * (W * SPT);   // the workgroup's first site
+    const int xb = x0w + threadIdx.x;
     const int y = (int)(wg / gridDim.x);
     if (xb >= a.nx) return;
     const long n = a.n;
@@ -54,7 +55,9 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
         const int x = xb + k * W;
         live[k] = x < a.nx && site_frame(a.rtile, a.rects, a.ntx, a.nx, a.ny, x, y, fr[k]);
         if (!live[k]) { fr[k].x0 = 0; fr[k].y0 = 0; fr[k].w = a.nx; fr[k].h = a.ny; fr[k].r = -1; }
-        xs[k] = live[k] ? x : xb;
+        // a site outside every rectangle reads the workgroup's first site,
+        // one address for all such lanes, and writes nothing
+        xs[k] = live[k] ? x : x0w;
         const int xx = xs[k];
         const int xm = (xx == fr[k].x0) ? fr[k].x0 + fr[k].w - 1 : xx - 1;
         const int xp = (xx == fr[k].x0 + fr[k].w - 1) ? fr[k].x0 : xx + 1;
@@ -486,8 +489,13 @@ struct LaneVec {
     }
 };
 
+// A batched atlas for the vector kernel: rectangle r's sites as groups of V
+// consecutive sites along its rows, numbered row-major (a group never crosses
+// a row); one workgroup takes groups l0 .. l0 + blockDim - 1, so a workgroup
+// may cover the ends of several short rows and only a rectangle's last
+// workgroup has lanes without sites.
 struct SegDev {
-    int32_t y, xs, xe, r;
+    int32_t r, l0;
 };
 
 // plane -> source row (0 same, 1 south, 2 north) and x shift (0, 1 west, 2 east)
@@ -504,25 +512,33 @@ __global__ __launch_bounds__(256) void k_stream_collide_vec(const MixedArgs a, c
     using LV = LaneVec<P, V>;
     using T = typename LV::T;
     if (step_stopped(a.st)) return;
-    const int L = (int)blockDim.x * V;
     const uint32_t wg = xcd_contiguous(blockIdx.x, gridDim.x);
-    int y, xs, xe, fx0, fy0, fw, fh, r;
+    int y, x, nvalid, fx0, fy0, fw, fh, r;
     if (segs) {
         const SegDev sg = segs[wg];
-        y = sg.y; xs = sg.xs; xe = sg.xe; r = sg.r;
+        r = sg.r;
         const RectDev& R = a.rects[r];
         fx0 = R.x0; fy0 = R.y0; fw = R.w; fh = R.h;
+        const int G = (fw + V - 1) / V;   // groups per row
+        const int l = sg.l0 + (int)threadIdx.x;
+        const bool dead = l >= G * fh;
+        const int row = dead ? 0 : l / G;
+        y = fy0 + row;
+        x = dead ? fx0 : fx0 + V * (l - row * G);
+        nvalid = dead ? 0 : min(fx0 + fw - x, V);
     } else {
+        const int L = (int)blockDim.x * V;
         y = (int)(wg / (uint32_t)nchunk);
-        xs = (int)(wg % (uint32_t)nchunk) * L;
-        xe = min(a.nx, xs + L);
+        const int xs = (int)(wg % (uint32_t)nchunk) * L;
+        x = xs + V * (int)threadIdx.x;
+        nvalid = min(max(min(a.nx, xs + L) - x, 0), V);
         r = -1; fx0 = 0; fy0 = 0; fw = a.nx; fh = a.ny;
     }
     const int fxe = fx0 + fw;
     const int lane = (int)threadIdx.x & 63;
-    const int x = xs + V * (int)threadIdx.x;
-    const int nvalid = min(max(xe - x, 0), V);
-    const int xl = min(x, a.nx - V);   // dead lanes keep a valid address
+    // dead lanes keep a valid address; a live group lies inside the lattice
+    // (origins on the 16-site grid, even widths, nx a multiple of 16)
+    const int xl = min(x, a.nx - V);
     const long n = a.n;
     const long r0 = (long)y * a.nx;
     const long rS = (long)((y == fy0) ? fy0 + fh - 1 : y - 1) * a.nx;
@@ -778,22 +794,22 @@ static void launch_mx_spt(lbib_ctx* c, const MixedArgs& a, int wg, int spt) {
     else k_stream_collide_mx<NS, FM, P, 4><<<grid, wg, 0, c->stream>>>(a);
 }
 
-// Segment table of the vector kernel for chunks of len sites: every row of
-// every rectangle cut into chunks, rectangle-major and row-major inside a
-// rectangle so that the rows one chunk pulls from sit in the same XCD's
-// range of workgroups.  Not needed without an atlas.
-static int ensure_segments(lbib_ctx* c, int len) {
-    if (c->nrect == 0 || (c->vsegs && c->vseg_len == len)) return 0;
+// Segment table of the vector kernel for workgroups of wg lanes with v sites
+// each: every rectangle's row-major groups of v sites cut into runs of wg
+// groups, rectangle-major so that the rows one workgroup pulls from sit in the
+// same XCD's range of workgroups.  Not needed without an atlas.
+static int ensure_segments(lbib_ctx* c, int wg, int v) {
+    const int key = wg * 64 + v;
+    if (c->nrect == 0 || (c->vsegs && c->vseg_len == key)) return 0;
     if (c->capturing) {
         c->err = "vector lattice kernel: segment table must be built before a step is recorded";
         return -1;
     }
     std::vector<SegDev> t;
     for (int r = 0; r < c->nrect; r++) {
-        const int x0 = c->rect_host[4 * r], y0 = c->rect_host[4 * r + 1];
         const int w = c->rect_host[4 * r + 2], h = c->rect_host[4 * r + 3];
-        for (int y = y0; y < y0 + h; y++)
-            for (int xs = x0; xs < x0 + w; xs += len) t.push_back({y, xs, std::min(xs + len, x0 + w), r});
+        const long groups = (long)((w + v - 1) / v) * h;
+        for (long l0 = 0; l0 < groups; l0 += wg) t.push_back({r, (int32_t)l0});
     }
     if (c->vsegs) { (void)hipFree(c->vsegs); c->vsegs = nullptr; c->nvseg = 0; c->vseg_len = 0; }
     c->alloc_gen++;
@@ -801,7 +817,7 @@ static int ensure_segments(lbib_ctx* c, int len) {
     LBIB_HIP(hipMemcpyAsync(c->vsegs, t.data(), t.size() * sizeof(SegDev), hipMemcpyHostToDevice, c->stream));
     LBIB_HIP(hipStreamSynchronize(c->stream));
     c->nvseg = (int)t.size();
-    c->vseg_len = len;
+    c->vseg_len = key;
     return 0;
 }
 
@@ -854,7 +870,7 @@ static void dispatch_mx_fm(lbib_ctx* c, const MixedArgs& a, const LatticeShape& 
 
 static int launch_mixed_shape(lbib_ctx* c, const MixedArgs& a, const LatticeShape& sh) {
     if (c->prec == LBIB_FP16_REM) return launch_rem_shape(c, a, sh.variant, sh.wg);
-    if (sh.variant == 2 && ensure_segments(c, sh.wg * sh.spt)) return -1;
+    if (sh.variant == 2 && ensure_segments(c, sh.wg, sh.spt)) return -1;
     if (c->prec == LBIB_FP32) dispatch_mx_fm<LBIB_FP32>(c, a, sh);
     else if (c->prec == LBIB_FP16_DEV) dispatch_mx_fm<LBIB_FP16_DEV>(c, a, sh);
     else dispatch_mx_fm<LBIB_BF16_DEV>(c, a, sh);

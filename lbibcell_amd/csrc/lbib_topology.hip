@@ -97,13 +97,10 @@ constexpr uint32_t SCAN_SINGLE_MAX = 1u << 16;
 // A pass covers 16384 values: coalesced loads into LDS, each thread then scans
 // 16 consecutive ones (row stride 17 words avoids bank conflicts), coalesced
 // stores from LDS.
-__global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict__ in, const uint32_t* nptr,
-                                                      uint32_t nfix, uint32_t* __restrict__ out, uint32_t* total,
-                                                      const uint8_t* __restrict__ nz, uint32_t* nz_total,
-                                                      const uint32_t* gate) {
-    if (gate && *gate == 0u) return;
+// Called by a whole 1024-wide workgroup.
+__device__ void scan_single_dev(const uint32_t* __restrict__ in, uint32_t n, uint32_t* __restrict__ out,
+                                uint32_t* total, const uint8_t* __restrict__ nz, uint32_t* nz_total) {
     constexpr int PER = 16, PASS = 1024 * PER, ROW = PER + 1;
-    const uint32_t n = nptr ? *nptr : nfix;
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t buf[1024 * ROW];
     uint32_t carry = 0, nzc = 0;
@@ -146,6 +143,14 @@ __global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict
         if (threadIdx.x == 0) *nz_total = t;
     }
     if (threadIdx.x == 0 && total) *total = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict__ in, const uint32_t* nptr,
+                                                      uint32_t nfix, uint32_t* __restrict__ out, uint32_t* total,
+                                                      const uint8_t* __restrict__ nz, uint32_t* nz_total,
+                                                      const uint32_t* gate) {
+    if (gate && *gate == 0u) return;
+    scan_single_dev(in, nptr ? *nptr : nfix, out, total, nz, nz_total);
 }
 
 int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive) {
@@ -197,13 +202,14 @@ __device__ __forceinline__ bool needs_perturbation(const double X, const double 
 }
 
 // flag != nullptr: mark this round's long connections (and the perturbation
-// halt of their midpoints), and note in rm_any that there was one;
+// halt of their midpoints) and list them, in any order, in list (st->rm_K
+// counts them; the first RM_LIST_CAP are stored);
 // flag == nullptr: only raise LBIB_FLAG_REMESH if a connection is still long
 // after the rounds of this pass.  ring_start != nullptr: also clear the
 // per-domain ring counters for the ring rebuild that follows, when the
 // topology changed (like k_ring_reset).
 __global__ void k_rm_flag(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb, const double* __restrict__ mx,
-                          const double* __restrict__ my, uint32_t* flag, int ndom, int32_t* ring_start,
+                          const double* __restrict__ my, uint32_t* flag, uint32_t* list, int ndom, int32_t* ring_start,
                           uint32_t* ring_cnt, DevStatus* st) {
     if (ring_start && st->topo_changed)
         for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < ndom; d += gridDim.x * blockDim.x) {
@@ -220,11 +226,62 @@ __global__ void k_rm_flag(const int32_t* __restrict__ ca, const int32_t* __restr
         any |= lg;
         if (!flag) continue;
         flag[c] = lg ? 1u : 0u;
-        if (lg && needs_perturbation((mx[a] + mx[b]) / 2, (my[a] + my[b]) / 2))
-            atomicOr(&st->bits, LBIB_HALT_PERTURB_REMESH);
+        // one list slot range per wavefront: the lowest long lane takes it
+        const unsigned long long m = __ballot(lg);
+        if (lg) {
+            const int lane = threadIdx.x & 63;
+            const int lead = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == lead) base = atomicAdd(&st->rm_K, (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, lead, 64);
+            const uint32_t p = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (p < RM_LIST_CAP) list[p] = c;
+            if (needs_perturbation((mx[a] + mx[b]) / 2, (my[a] + my[b]) / 2))
+                atomicOr(&st->bits, LBIB_HALT_PERTURB_REMESH);
+        }
     }
     if (!flag && __any(any) && (threadIdx.x & 63) == 0) atomicOr(&st->bits, LBIB_FLAG_REMESH);
-    if (flag && __any(any) && (threadIdx.x & 63) == 0) atomicOr(&st->rm_any, 1u);
+}
+
+// One workgroup of 1024: the round's ranks.  Up to RM_LIST_CAP long
+// connections: their list sorted ascending (a connection's rank is then the
+// number of entries below its index); more: an exclusive scan of the flags
+// into rank.  Sets rm_L and rm_sparse and empties the list count for the next
+// round, also when the topology stage is stopped.
+__global__ __launch_bounds__(1024) void k_rm_rank(const uint32_t* __restrict__ flag, uint32_t* rank, uint32_t* list,
+                                                  DevStatus* st) {
+    __shared__ uint32_t key[RM_LIST_CAP];
+    const uint32_t K = st->rm_K;
+    const bool run = !topo_halted(st);
+    if (run && K > 0 && K <= RM_LIST_CAP) {
+        uint32_t P = 1;
+        while (P < K) P <<= 1;
+        for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) key[j] = j < K ? list[j] : 0xffffffffu;
+        __syncthreads();
+        // bitonic sort of P keys (indices are distinct)
+        for (uint32_t size = 2; size <= P; size <<= 1)
+            for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+                for (uint32_t j = threadIdx.x; j < P; j += blockDim.x) {
+                    const uint32_t o = j ^ stride;
+                    if (o > j) {
+                        const uint32_t u = key[j], v = key[o];
+                        if ((u > v) == ((j & size) == 0)) { key[j] = v; key[o] = u; }
+                    }
+                }
+                __syncthreads();
+            }
+        for (uint32_t j = threadIdx.x; j < K; j += blockDim.x) list[j] = key[j];
+    } else if (run && K > RM_LIST_CAP) {
+        scan_single_dev(flag, st->nc, rank, nullptr, nullptr, nullptr);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (run) {
+            st->rm_L = K;
+            st->rm_sparse = K <= RM_LIST_CAP ? 1u : 0u;
+        }
+        st->rm_K = 0;
+    }
 }
 
 struct TopoArrays {
@@ -241,11 +298,16 @@ struct TopoArrays {
 // midpoint markers.  L = number of long connections (st->rm_L).  Capacity is
 // checked first, before anything is written (every block evaluates the same
 // condition; one raises the halt).
-__global__ void k_rm_apply(TopoArrays t, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ rank, int nx,
-                           int ny, uint32_t mcap, uint32_t ccap, DevStatus* st) {
-    if (topo_halted(st)) return;
+__global__ __launch_bounds__(256) void k_rm_apply(TopoArrays t, const uint32_t* __restrict__ flag,
+                                                  const uint32_t* __restrict__ rank, const uint32_t* __restrict__ list,
+                                                  int nx, int ny, uint32_t mcap, uint32_t ccap, DevStatus* st) {
+    __shared__ uint32_t key[RM_LIST_CAP];
+    __shared__ int go;
+    // one reading of the status per workgroup, so that all its threads agree
+    if (threadIdx.x == 0) go = !topo_halted(st) && st->rm_L > 0;
+    __syncthreads();
+    if (!go) return;
     const uint32_t L = st->rm_L;
-    if (L == 0) return;
     const uint32_t nc = st->nc, nm = st->nm, maxid = st->maxid;
     if (nm + L + 1 > mcap || nc + L > ccap) {
         if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&st->bits, LBIB_HALT_GROW);
@@ -255,8 +317,24 @@ __global__ void k_rm_apply(TopoArrays t, const uint32_t* __restrict__ flag, cons
         st->rm_nc0 = nc;
         st->rm_nm0 = nm;
     }
+    const bool sparse = st->rm_sparse != 0u;   // then L <= RM_LIST_CAP
+    if (sparse) {
+        for (uint32_t j = threadIdx.x; j < L; j += blockDim.x) key[j] = list[j];
+        __syncthreads();
+    }
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const uint32_t r = rank[c];
+        uint32_t r;
+        if (sparse) {   // entries of the sorted list below c
+            uint32_t lo = 0, hi = L;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (key[mid] < c) lo = mid + 1;
+                else hi = mid;
+            }
+            r = lo;
+        } else {
+            r = rank[c];
+        }
         if (!flag[c]) {
             const uint32_t p = c - r;
             t.ca2[p] = t.ca[c]; t.cb2[p] = t.cb[c]; t.cdom2[p] = t.cdom[c]; t.cbc2[p] = t.cbc[c];
@@ -829,22 +907,19 @@ int launch_ring_areas(lbib_ctx* c, const AreaMass* mass) {
 // Rounds of remeshing; LBIB_FLAG_REMESH asks for another pass when a
 // connection is still long (a marker moves far less than MAXLENGTH per
 // iteration, so one round is the steady-state case).
-int launch_remesh(lbib_ctx* c, int rounds, bool in_step) {
+int launch_remesh(lbib_ctx* c, int rounds) {
     TopoArrays t = topo_arrays(c);
     for (int r = 0; r < rounds; r++) {
-        k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, 0, nullptr,
-                                                            nullptr, c->st);
-        // the geometry stage's first round (rm_any and rm_L start at zero)
-        // skips the scan when no connection is long
-        const uint32_t* gate = (in_step && r == 0) ? &c->st->rm_any : nullptr;
-        if (scan_exclusive(c, c->flag_c, c->rank_c, &c->st->nc, 0, (uint32_t)c->ccap, &c->st->rm_L, gate)) return -1;
-        k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->nx, c->ny,
+        k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->flag_c, c->rm_list, 0,
+                                                            nullptr, nullptr, c->st);
+        k_rm_rank<<<1, 1024, 0, c->stream>>>(c->flag_c, c->rank_c, c->rm_list, c->st);
+        k_rm_apply<<<grid_for(c->ccap), 256, 0, c->stream>>>(t, c->flag_c, c->rank_c, c->rm_list, c->nx, c->ny,
                                                              (uint32_t)c->mcap, (uint32_t)c->ccap, c->st);
         k_rm_commit<<<grid_for(2 * (long)c->ccap), 256, 0, c->stream>>>(t, c->min_c, c->mout_c, c->st);
     }
     // with the clearing of the ring counters for the rebuild that follows
-    k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, nullptr, c->ndom, c->ring_start,
-                                                        c->ring_cnt, c->st);
+    k_rm_flag<<<grid_for(c->ccap), 256, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, nullptr, nullptr, c->ndom,
+                                                        c->ring_start, c->ring_cnt, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

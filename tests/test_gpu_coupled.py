@@ -75,6 +75,23 @@ def _compare(sim, o, ns, tag):
 SIGNAL = dict(production=1e-3, decay=1e-5, switchoff=5000, init_inside=0.0, prod_domain=1)
 
 
+@pytest.mark.parametrize("ring2_first", [False, True])
+def test_equidistant_cut_candidates_go_to_the_first_connection(orc, ring2_first):
+    """Two connections cut the lattice line y = 8 at exactly x = 5.5, so the W
+    slot of site (6, 8) gets two candidates at the same distance; the
+    connection earlier in the list keeps it (Connection::insertBoundaryNode,
+    Connection.cpp:181-222), and the scanline carries the winner's domain
+    along row 8 (the oracle pin of this geometry derives the row by hand)."""
+    from test_oracle_pins import TIE_SIZE, tie_geometry
+    nodes, conns = tie_geometry(ring2_first)
+    print(f"case equidistant cut candidates, ring2_first={ring2_first}", flush=True)
+    sim, o = _build(orc, *TIE_SIZE, 1.0, nodes, conns, [], [], box=False)
+    dom_o, _ = o.get_domains()
+    winner = 2 if ring2_first else 1
+    np.testing.assert_array_equal(dom_o[8, :14], [0] * 6 + [winner] * 4 + [0] * 4)
+    np.testing.assert_array_equal(sim.context().get_domains(), dom_o)
+
+
 @pytest.mark.parametrize("steps", [1, 12, 150])
 def test_single_cell_tension_matches_oracle(orc, steps):
     nx = ny = 64
