@@ -16,9 +16,13 @@ __global__ void k_step_begin(DevStatus* st, uint32_t it) { step_begin_dev(st, it
 
 // the main stream waits for the topology stage still running on the second
 // stream (its kernels raise stops the next iteration's begin must see)
+// The join event is recorded on the second stream only when the main stream
+// waits, after everything enqueued there so far (one marker packet fewer in
+// the common case, where the forces and binning follow the topology stage).
 int join_topology(lbib_ctx* c) {
     if (!c->topo_pending) return 0;
     c->topo_pending = false;
+    LBIB_HIP(hipEventRecord(c->ev_join, c->stream2));
     LBIB_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     return 0;
 }
@@ -113,16 +117,6 @@ int ensure_markers(lbib_ctx* c, int nm) {
     for (auto a : darr) if (regrow(c, a, old, cap)) return -1;
     for (auto a : uarr) if (regrow(c, a, old, cap)) return -1;
     for (auto a : iarr) if (regrow(c, a, old ? old + 1 : 0, cap + 1)) return -1;
-    // per-step work lists: rebuilt every step
-    dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
-    if (dev_alloc(c, &c->tslot, cap) || dev_alloc(c, &c->tlist, cap) || dev_alloc(c, &c->tpos, cap)) return -1;
-    if (c->prec != LBIB_FP64) {
-        dev_free(c->pslot); dev_free(c->pe_x); dev_free(c->pe_y); dev_free(c->pe_fx); dev_free(c->pe_fy);
-        const size_t ne = 4 * (size_t)cap;
-        if (dev_alloc(c, &c->pslot, ne) || dev_alloc(c, &c->pe_x, ne) || dev_alloc(c, &c->pe_y, ne) ||
-            dev_alloc(c, &c->pe_fx, ne) || dev_alloc(c, &c->pe_fy, ne))
-            return -1;
-    }
     c->mcap = cap;
     if (ensure_scan(c)) return -1;
     return ensure_junctions(c);
@@ -394,8 +388,16 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     c->live_sites = c->n;
     {
         const size_t nt = (size_t)c->stx * c->sty;
-        if (dev_alloc(c, &c->tag, n) || dev_alloc(c, &c->tact, nt) || dev_alloc(c, &c->tcnt, nt) ||
-            dev_alloc(c, &c->toff, nt))
+        if (dev_alloc(c, &c->tag, n) || dev_alloc(c, &c->tact, nt) || dev_alloc(c, &c->tcnt, nt))
+            return -1;
+        // marker bins: BINCAP entries per tile (ordered bins always, push
+        // bins for the order-free spread of mixed-precision lattices)
+        const size_t nb = nt * (size_t)BINCAP;
+        if (dev_alloc(c, &c->tlist, nb) || dev_alloc(c, &c->tpos, nb) || dev_alloc(c, &c->ovf, (size_t)OVF_CAP) ||
+            dev_alloc(c, &c->shard, (size_t)SH_COUNT * NSHARD * SHARD_STRIDE))
+            return -1;
+        if (c->prec != LBIB_FP64 &&
+            (dev_alloc(c, &c->pb_pos, nb) || dev_alloc(c, &c->pb_r, nb) || dev_alloc(c, &c->pb_f, nb)))
             return -1;
     }
     if (dev_alloc(c, &c->best_dist, 4 * n, 0xff) || dev_alloc(c, &c->best_seq, 4 * n, 0xff) ||
@@ -406,6 +408,8 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
         return -1;
     if (dev_alloc(c, &c->st, 1)) return -1;
     LBIB_HIP(hipHostMalloc((void**)&c->st_host, sizeof(DevStatus), hipHostMallocDefault));
+    LBIB_HIP(hipHostMalloc((void**)&c->shard_host, (size_t)SH_COUNT * NSHARD * SHARD_STRIDE * sizeof(uint32_t),
+                           hipHostMallocDefault));
     if (ensure_markers(c, cfg->max_markers > 0 ? cfg->max_markers : 1024)) return -1;
     if (ensure_connections(c, cfg->max_connections > 0 ? cfg->max_connections : 1024)) return -1;
     if (ensure_domains(c, 256)) return -1;
@@ -444,9 +448,9 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->u); dev_free(c->Fx); dev_free(c->Fy); dev_free(c->msite);
     dev_free(c->mass_site); dev_free(c->mass_val); dev_free(c->pert_buf);
     dev_free(c->rtile); dev_free(c->rects); dev_free(c->dconc); dev_free(c->vsegs);
-    dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt); dev_free(c->toff);
-    dev_free(c->tslot); dev_free(c->tlist); dev_free(c->tpos);
-    dev_free(c->pslot); dev_free(c->pe_x); dev_free(c->pe_y); dev_free(c->pe_fx); dev_free(c->pe_fy);
+    dev_free(c->tag); dev_free(c->tact); dev_free(c->tcnt);
+    dev_free(c->tlist); dev_free(c->tpos); dev_free(c->ovf); dev_free(c->shard);
+    dev_free(c->pb_pos); dev_free(c->pb_r); dev_free(c->pb_f);
     double** darr[14] = {&c->mx, &c->my, &c->mvx, &c->mvy, &c->mfx, &c->mfy, &c->mox, &c->moy,
                          &c->sx_m, &c->sy_m, &c->svx_m, &c->svy_m, &c->sox_m, &c->soy_m};
     uint32_t** uarr[19] = {&c->mid, &c->sid_m, &c->mdead, &c->rec_alive, &c->keep_m, &c->pos_m, &c->keep_r,
@@ -466,6 +470,7 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->ring_start); dev_free(c->ring_cnt); dev_free(c->ring_off_u); dev_free(c->area);
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->shard_host) (void)hipHostFree(c->shard_host);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -1103,13 +1108,19 @@ int lbib_set_box_outlet(lbib_ctx* c, int on) {
 }
 int lbib_read_status(lbib_ctx* c, lbib_status* out) {
     if (check_ctx(c) || !out) return -1;
+    const size_t nsh = (size_t)SH_COUNT * NSHARD * SHARD_STRIDE;
     LBIB_HIP(hipMemcpyAsync(c->st_host, c->st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+    LBIB_HIP(hipMemcpyAsync(c->shard_host, c->shard, nsh * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     LBIB_HIP(hipStreamSynchronize(c->stream));
     const DevStatus& s = *c->st_host;
     out->bits = (s.bits & ERR_EDGE_REMESH) ? ((s.bits & ~ERR_EDGE_REMESH) | LBIB_ERR_EDGE) : s.bits;
     out->first_bad_marker = s.first_bad == INT_MAX ? -1 : s.first_bad;
-    out->n_touched_sites = (int32_t)s.ntouched;
-    out->n_boundary_nodes = (int32_t)s.nbn;
+    uint32_t touched = 0;
+    for (int k = 0; k < NSHARD; k++) touched += c->shard_host[(SH_TOUCHED * NSHARD + k) * SHARD_STRIDE];
+    out->n_touched_sites = (int32_t)touched;
+    uint32_t nbn = 0;
+    for (int k = 0; k < NSHARD; k++) nbn += c->shard_host[(SH_NBN * NSHARD + k) * SHARD_STRIDE];
+    out->n_boundary_nodes = (int32_t)nbn;
     out->n_reinit = (int32_t)s.nchanged;
     out->n_candidates = (int32_t)s.ncand;
     out->n_markers = (int32_t)s.nm;

@@ -36,7 +36,8 @@ constexpr uint32_t SEQ_TIE = 0xfffffffeu;
 __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
                                                    const double* __restrict__ mx, const double* __restrict__ my,
                                                    const Cand* __restrict__ cand, uint32_t* tag,
-                                                   unsigned long long* best_dist, uint32_t* best_seq, DevStatus* st) {
+                                                   unsigned long long* best_dist, uint32_t* best_seq,
+                                                   uint32_t* shard, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->nc, nprev = st->ncand_prev;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -54,9 +55,9 @@ __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ c
         best_dist[b] = ~0ull;
         best_seq[b] = ~0u;
     }
+    if (blockIdx.x == 0 && threadIdx.x < NSHARD) shard[(SH_NBN * NSHARD + threadIdx.x) * SHARD_STRIDE] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st->ncand = 0;
-        st->nbn = 0;
         st->nchanged = 0;
         st->rm_total = 0;
         st->co_removed = 0;
@@ -184,57 +185,79 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
 // E,N,W,S; every slot with a candidate has exactly one winner, so that is the
 // winner in the site's highest slot with a candidate (a set best distance),
 // which reports the site when its domain changes.
-__global__ __launch_bounds__(1024) void k_cut_win(const Cand* cand, long n, const unsigned long long* __restrict__ best_dist,
-                          const uint32_t* __restrict__ best_seq, uint32_t* tag,
-                          uint32_t* bn_dom, double* bc_v, int32_t* bc_desc, uint32_t* changed, uint32_t* rmark,
-                          uint8_t* rstate, int local_update, DevStatus* st) {
+// A workgroup takes chunks of WIN_PER_WG candidates (two per lane) and lists
+// its reported sites with one atomic per chunk; the winners are counted in
+// sharded counters.
+constexpr int WIN_WG = 1024, WIN_PER_WG = 2 * WIN_WG;
+struct WinOut {
+    uint32_t won, add, site, dom;
+};
+__device__ __forceinline__ WinOut cut_winner(const Cand* cand, uint32_t j, uint32_t nc, long n,
+                                             const unsigned long long* __restrict__ best_dist,
+                                             const uint32_t* __restrict__ best_seq, uint32_t* tag, uint32_t* bn_dom,
+                                             double* bc_v, int32_t* bc_desc, int local_update) {
+    WinOut o{0u, 0u, 0u, 0u};
+    if (j >= nc) return o;
+    const Cand& e = cand[j];
+    const long slotix = (long)(e.slot - 1) * n + e.site;
+    const long b = sel_index(e.site, e.slot);
+    const uint32_t dom_now = tag[e.site] & DOM_MASK;   // domain bits are not changed here
+    const uint32_t bs = best_seq[b];
+    if (bs == ~0u ? (unsigned long long)__double_as_longlong(e.dist2) == best_dist[b] : bs == e.seq) {
+        o.won = 1;
+        o.site = (uint32_t)e.site;
+        o.dom = e.domain;
+        bn_dom[slotix] = e.domain;
+        uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
+        if (e.desc >= 0 && bc_v != nullptr) {
+            bits |= 1u << (BC_SHIFT + e.slot - 1);
+            bc_v[2 * slotix] = e.vx;
+            bc_v[2 * slotix + 1] = e.vy;
+            bc_desc[slotix] = e.desc;
+        }
+        atomicOr(&tag[e.site], bits);
+        if (local_update && dom_now != e.domain) {
+            bool highest = true;
+            for (int s2 = e.slot + 1; s2 <= 4; s2++)
+                if (best_dist[sel_index(e.site, s2)] != ~0ull) highest = false;
+            o.add = highest ? 1u : 0u;
+        }
+    }
+    return o;
+}
+
+__global__ __launch_bounds__(WIN_WG) void k_cut_win(const Cand* cand, long n,
+                                                    const unsigned long long* __restrict__ best_dist,
+                                                    const uint32_t* __restrict__ best_seq, uint32_t* tag,
+                                                    uint32_t* bn_dom, double* bc_v, int32_t* bc_desc,
+                                                    uint32_t* changed, uint32_t* rmark, uint8_t* rstate,
+                                                    uint32_t* shard, int local_update, DevStatus* st) {
     if (halted_beside_topology(st)) return;
     const uint32_t nc = st->ncand;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t j0 = blockIdx.x * blockDim.x; j0 < nc; j0 += stride) {
-        const uint32_t j = j0 + threadIdx.x;
-        uint32_t won = 0;
-        bool add = false;
-        int site = 0;
-        uint32_t dom = 0;
-        if (j < nc) {
-            const Cand& e = cand[j];
-            const long slotix = (long)(e.slot - 1) * n + e.site;
-            const long b = sel_index(e.site, e.slot);
-            const uint32_t dom_now = tag[e.site] & DOM_MASK;   // domain bits are not changed here
-            const uint32_t bs = best_seq[b];
-            if (bs == ~0u ? (unsigned long long)__double_as_longlong(e.dist2) == best_dist[b] : bs == e.seq) {
-                won = 1;
-                site = e.site;
-                dom = e.domain;
-                bn_dom[slotix] = e.domain;
-                uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
-                if (e.desc >= 0 && bc_v != nullptr) {
-                    bits |= 1u << (BC_SHIFT + e.slot - 1);
-                    bc_v[2 * slotix] = e.vx;
-                    bc_v[2 * slotix + 1] = e.vy;
-                    bc_desc[slotix] = e.desc;
-                }
-                atomicOr(&tag[e.site], bits);
-                if (local_update && dom_now != e.domain) {
-                    bool highest = true;
-                    for (int s2 = e.slot + 1; s2 <= 4; s2++)
-                        if (best_dist[sel_index(e.site, s2)] != ~0ull) highest = false;
-                    add = highest;
-                }
+    for (uint32_t j0 = blockIdx.x * (uint32_t)WIN_PER_WG; j0 < nc; j0 += gridDim.x * (uint32_t)WIN_PER_WG) {
+        // both candidates' loads in flight together
+        const WinOut a = cut_winner(cand, j0 + threadIdx.x, nc, n, best_dist, best_seq, tag, bn_dom, bc_v, bc_desc,
+                                    local_update);
+        const WinOut b = cut_winner(cand, j0 + WIN_WG + threadIdx.x, nc, n, best_dist, best_seq, tag, bn_dom, bc_v,
+                                    bc_desc, local_update);
+        const uint32_t nw = (uint32_t)__popcll(__ballot(a.won != 0u)) + (uint32_t)__popcll(__ballot(b.won != 0u));
+        if ((threadIdx.x & 63) == 0 && nw)
+            atomicAdd(&shard[(SH_NBN * NSHARD + ((blockIdx.x * (WIN_WG / 64) + threadIdx.x / 64) % NSHARD)) *
+                             SHARD_STRIDE],
+                      nw);
+        const uint32_t p0 = block_alloc(&st->nchanged, a.add + b.add);
+        auto report = [&](const WinOut& w, uint32_t p) {
+            if (p < (uint32_t)REINIT_CAP) {
+                changed[2 * p] = w.site;
+                changed[2 * p + 1] = w.dom;
+                rmark[w.site] = p + 1;
+                rstate[p] = 0;
+            } else {
+                atomicOr(&st->bits, LBIB_ERR_CAPACITY);
             }
-        }
-        (void)block_alloc(&st->nbn, won);
-        const uint32_t p = block_alloc(&st->nchanged, add ? 1u : 0u);
-        if (!add) continue;
-        if (p < (uint32_t)REINIT_CAP) {
-            changed[2 * p] = (uint32_t)site;
-            changed[2 * p + 1] = dom;
-            rmark[site] = p + 1;
-            rstate[p] = 0;
-        } else {
-            atomicOr(&st->bits, LBIB_ERR_CAPACITY);
-        }
+        };
+        if (a.add) report(a, p0);
+        if (b.add) report(b, p0 + a.add);
     }
 }
 
@@ -522,7 +545,7 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     // connection counts are device-resident (remeshing changes them): grids
     // cover the capacity and the kernels read st->nc
     k_geo_begin<<<gs, bs, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->cand, c->tag, c->best_dist, c->best_seq,
-                                          c->st);
+                                          c->shard, c->st);
     // 1024-wide workgroups: one candidate-counter atomic per 1024 connections
     if (c->ccap > 0)
         k_cut_gen<<<(c->ccap + 1023) / 1024, 1024, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom, c->cbc,
@@ -541,11 +564,15 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
         const int r = launch_topology(c, coarsen);
         c->stream = main;
         if (r) return -1;
-        LBIB_HIP(hipEventRecord(c->ev_join, c->stream2));
     }
     k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
-    k_cut_win<<<256, 1024, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->tag, c->bn_dom, c->bc_v,
-                                        c->bc_desc, c->changed, c->rmark, c->rstate, cure ? 0 : 1, c->st);
+    // enough workgroups for about twice the connections' candidates (the
+    // count is device-resident; the kernel loops over more)
+    const int wgrid = (int)std::min<long>(std::max<long>((2L * std::max(c->nc, 1) + WIN_PER_WG - 1) / WIN_PER_WG, 64),
+                                          1024);
+    k_cut_win<<<wgrid, WIN_WG, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->tag, c->bn_dom, c->bc_v,
+                                               c->bc_desc, c->changed, c->rmark, c->rstate, c->shard, cure ? 0 : 1,
+                                               c->st);
     if (!cure) {
         ReinitArgs a;
         a.nx = c->nx; a.ny = c->ny; a.n = c->n; a.ns = c->ns;

@@ -24,24 +24,12 @@ namespace lbib {
 
 // ForceSolver::calculateForce (ForceSolver.cpp:42-85) with the force structs of
 // ForceStructs.hpp:49-588; per marker, structs in map/push order.
-// Also the first kernel of the lattice stage: clears the spread tiles and,
-// when it runs on the main stream, opens a pending iteration (lbib_step_begin).
-__global__ void k_marker_force(const double* __restrict__ mx, const double* __restrict__ my,
-                               const int32_t* __restrict__ off, const int32_t* __restrict__ type,
-                               const int32_t* __restrict__ partner, const double* __restrict__ prm,
-                               double* __restrict__ mfx, double* __restrict__ mfy, int ntiles, uint32_t* tcnt,
-                               uint8_t* tact, int begin, uint32_t begin_it, DevStatus* st) {
-    if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
-    if (step_stopped(st)) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) st->ntouched = 0;
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
-        tcnt[t] = 0;
-        tact[t] = 0;
-    }
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (int)st->nm) return;
-    const double x1 = mx[k], y1 = my[k];
-    double fx = 0.0, fy = 0.0;
+__device__ __forceinline__ void marker_force(const int32_t* __restrict__ off, const int32_t* __restrict__ type,
+                                             const int32_t* __restrict__ partner, const double* __restrict__ prm,
+                                             const double* __restrict__ mx, const double* __restrict__ my, int k,
+                                             double x1, double y1, double& fx, double& fy) {
+    fx = 0.0;
+    fy = 0.0;
     for (int j = off[k]; j < off[k + 1]; j++) {
         const double* p = prm + 4 * (long)j;
         const int t = type[j];
@@ -78,71 +66,147 @@ __global__ void k_marker_force(const double* __restrict__ mx, const double* __re
             fy += 0;
         }
     }
-    mfx[k] = fx;
-    mfy[k] = fy;
 }
 
-// ---- ordered spread over 16x16 site tiles --------------------------------
-// Markers are binned by the tile of floor(X), floor(Y).  A tile's sites can be
-// reached only by markers binned in the 3x3 tiles around it (the stencil
-// reaches 2 sites).  One workgroup per tile gathers the markers whose stencil
-// overlaps it, ranks them by index (= GeometryNode id order; indices are
-// distinct, so a marker's rank is the number of smaller ones), evaluates each
-// marker's eight delta factors once, and every lane adds delta*F for its site
-// in rank order, starting from 0.0 like FluidSolver::resetForce.  Tiles with
-// no marker nearby are flagged inactive and the lattice kernel reads no force.
+// ---- marker bins over 16x16 site tiles -----------------------------------
+// A tile's sites can be reached only by markers whose stencil overlaps it.
+// Two kinds of bins: ordered bins list each marker once, in the tile of
+// floor(X), floor(Y) (the FP64 spread gathers the 3x3 tiles around a tile and
+// the junction search the tiles within its radius); push bins list each
+// marker in every tile its stencil reaches, with what the order-free spread
+// needs.  Each tile holds BINCAP entries; a slot is taken with one atomic per
+// run of lanes binning into the same tile (consecutive markers of a membrane
+// mostly share one), and entries past the capacity go to the overflow list.
 constexpr int ST = 16;            // spread tile edge
 constexpr int SP_IDX = 1024;      // markers gathered per tile
 constexpr int SP_CH = 128;        // marker data staged per pass
 
-__global__ void k_tile_count(int ntx, int nty, const double* __restrict__ mx, const double* __restrict__ my,
-                             uint32_t* tcnt, uint32_t* tslot, uint8_t* tact, const DevStatus* st) {
+// Clears the bins before a binning pass: tile counts, active flags, the
+// overflow count and the touched-tile shards (a stopped step keeps them).
+__global__ void k_bin_clear(uint32_t* tcnt, uint8_t* tact, int nt, uint32_t* shard, DevStatus* st) {
     if (step_stopped(st)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = k < (int)st->nm;
-    int fx = 0, fy = 0, home = -1;
-    if (live) {
-        fx = (int)floor(mx[k]);
-        fy = (int)floor(my[k]);
-        home = (fy / ST) * ntx + fx / ST;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+        tcnt[t] = 0u;
+        tact[t] = 0;
     }
-    // consecutive markers (one membrane) mostly share a home tile: the head
-    // lane of each run of equal tiles in the wavefront takes one atomic
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) st->novf = 0u;
+        if (threadIdx.x < NSHARD) shard[(SH_TOUCHED * NSHARD + threadIdx.x) * SHARD_STRIDE] = 0u;
+    }
+}
+
+// The slot of this lane's entry in tile `key`: the head lane of each run of
+// equal keys in the wavefront takes the run's slots with one atomic.  Lanes
+// that bin nothing pass a key of their own (-1 - lane), which ends every run.
+// Called by the whole wavefront.
+__device__ __forceinline__ uint32_t run_slot(int key, bool part, uint32_t* tcnt) {
     const int lane = threadIdx.x & 63;
-    const int hprev = __shfl_up(home, 1, 64);
-    const bool head = live && (lane == 0 || hprev != home);
-    const unsigned long long heads = __ballot(head);
     const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+    const int kprev = __shfl_up(key, 1, 64);
+    const bool head = lane == 0 || kprev != key;
+    const unsigned long long heads = __ballot(head);
     const int hl = 63 - __clzll((long long)(heads & le));
     const unsigned long long gt = heads & ~le;
     const int nextl = gt ? __ffsll((long long)gt) - 1 : 64;
     uint32_t base = 0;
-    if (head) {
-        // lanes past the last live marker are not live: the run ends there
-        int end = nextl;
-        const int nlive = (int)st->nm - (k - lane);
-        if (end > nlive) end = nlive;
-        base = atomicAdd(&tcnt[home], (uint32_t)(end - lane));
-    }
-    base = __shfl(base, hl < 0 ? 0 : hl, 64);
-    if (!live) return;
-    tslot[k] = base + (uint32_t)(lane - hl);
-    for (int ty = (fy - 1) / ST; ty <= (fy + 2) / ST; ty++)
-        for (int tx = (fx - 1) / ST; tx <= (fx + 2) / ST; tx++)
-            if (tx < ntx && ty < nty) tact[ty * ntx + tx] = 1;
+    if (head && part) base = atomicAdd(&tcnt[key], (uint32_t)(nextl - lane));
+    base = __shfl(base, hl, 64);
+    return base + (uint32_t)(lane - hl);
 }
 
-__global__ void k_tile_scatter(int ntx, const double* __restrict__ mx, const double* __restrict__ my,
-                               const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tslot, int32_t* tlist,
-                               uint32_t* tpos,
-                               const DevStatus* st) {
+__device__ __forceinline__ void bin_overflow(BinOvf* ovf, DevStatus* st, const BinOvf& e) {
+    const uint32_t o = atomicAdd(&st->novf, 1u);
+    if (o < OVF_CAP) ovf[o] = e;
+    else atomicOr(&st->bits, LBIB_ERR_CAPACITY);
+}
+
+// The tiles a marker's 4x4 stencil reaches: quadrant q takes the tile of the
+// stencil's corner (q & 1: right column, q & 2: top row) when it differs from
+// the tiles of the lower quadrants.
+struct StencilTiles {
+    int tx0, tx1, ty0, ty1;
+    __device__ StencilTiles(int x0, int y0) : tx0(x0 / ST), tx1((x0 + 3) / ST), ty0(y0 / ST), ty1((y0 + 3) / ST) {}
+    __device__ bool has(int q) const { return ((q & 1) == 0 || tx1 != tx0) && ((q & 2) == 0 || ty1 != ty0); }
+    __device__ int tile(int q, int ntx) const { return ((q & 2) ? ty1 : ty0) * ntx + ((q & 1) ? tx1 : tx0); }
+};
+
+struct BinArgs {
+    const double *mx, *my;
+    const int32_t *off, *type, *partner;   // force CSR; off == nullptr: bin only
+    const double* prm;
+    double *mfx, *mfy;
+    int ntx, nty;
+    uint32_t* tcnt;
+    uint8_t* tact;
+    int32_t* tlist;                        // ordered bins
+    uint32_t* tpos;
+    uint32_t* pb_pos;                      // push bins
+    float2 *pb_r, *pb_f;
+    BinOvf* ovf;
+};
+
+// The marker forces and the binning of the lattice stage in one pass over the
+// markers (PUSH: push bins for the order-free spread, else ordered bins).
+// When it runs on the main stream it also opens a pending iteration
+// (lbib_step_begin).  Bins must be clear (k_bin_clear).
+template <int PUSH>
+__global__ __launch_bounds__(256) void k_force_bin(const BinArgs a, int begin, uint32_t begin_it, DevStatus* st) {
+    if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     if (step_stopped(st)) return;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (int)st->nm) return;
-    const int fx = (int)floor(mx[k]), fy = (int)floor(my[k]);
-    const uint32_t e = toff[(fy / ST) * ntx + fx / ST] + tslot[k];
-    tlist[e] = k;
-    tpos[e] = (uint32_t)fx | ((uint32_t)fy << 16);   // markers lie inside the lattice: 0 <= fx, fy < 65536
+    const bool live = k < (int)st->nm;
+    const int lane = threadIdx.x & 63;
+    double X = 2.0, Y = 2.0, fx = 0.0, fy = 0.0;
+    if (live) {
+        X = a.mx[k];
+        Y = a.my[k];
+        if (a.off) {
+            marker_force(a.off, a.type, a.partner, a.prm, a.mx, a.my, k, X, Y, fx, fy);
+            a.mfx[k] = fx;
+            a.mfy[k] = fy;
+        }
+    }
+    // markers lie inside the lattice (the uploads and the move check the
+    // edges): 0 <= floor(X), floor(Y) < 65536
+    const int ix = (int)floor(X), iy = (int)floor(Y);
+    if (!PUSH) {
+        const int key = live ? (iy / ST) * a.ntx + ix / ST : -1 - lane;
+        const uint32_t slot = run_slot(key, live, a.tcnt);
+        if (!live) return;
+        const uint32_t pos = (uint32_t)ix | ((uint32_t)iy << 16);
+        if (slot < (uint32_t)BINCAP) {
+            const long e = (long)key * BINCAP + slot;
+            a.tlist[e] = k;
+            a.tpos[e] = pos;
+        } else {
+            bin_overflow(a.ovf, st, BinOvf{key, pos, k, 0.0f, 0.0f, 0.0f, 0.0f});
+        }
+        for (int ty = (iy - 1) / ST; ty <= (iy + 2) / ST; ty++)
+            for (int tx = (ix - 1) / ST; tx <= (ix + 2) / ST; tx++)
+                if (tx < a.ntx && ty < a.nty) a.tact[ty * a.ntx + tx] = 1;
+    } else {
+        const int x0 = ix - 1, y0 = iy - 1;
+        const StencilTiles T(x0, y0);
+        const uint32_t pos = (uint32_t)x0 | ((uint32_t)y0 << 16);
+        const float2 r = make_float2((float)(X - (double)x0), (float)(Y - (double)y0));
+        const float2 f = make_float2((float)fx, (float)fy);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const bool part = live && T.has(q);
+            const int key = part ? T.tile(q, a.ntx) : -1 - lane;
+            const uint32_t slot = run_slot(key, part, a.tcnt);
+            if (!part) continue;
+            a.tact[key] = 1;
+            if (slot < (uint32_t)BINCAP) {
+                const long e = (long)key * BINCAP + slot;
+                a.pb_pos[e] = pos;
+                a.pb_r[e] = r;
+                a.pb_f[e] = f;
+            } else {
+                bin_overflow(a.ovf, st, BinOvf{key, pos, k, r.x, r.y, f.x, f.y});
+            }
+        }
+    }
 }
 
 // Shared staging of one tile's spread.
@@ -151,8 +215,9 @@ struct SpreadLds {
     double fxa[SP_CH][4], fya[SP_CH][4], sfx[SP_CH], sfy[SP_CH];
     int32_t s0x[SP_CH], s0y[SP_CH];
     unsigned long long smask[2][ST * ST];
-    uint32_t hcnt[9], hoff[9], hpre[10];
-    uint32_t n;
+    uint32_t hcnt[9], hpre[10];
+    int32_t htile[9];
+    uint32_t n, novf;
 };
 
 // One tile of the ordered spread.  The gathered markers are ranked by index
@@ -163,43 +228,59 @@ struct SpreadLds {
 // order.  Called by the whole workgroup with a workgroup-uniform tile;
 // returns false when the tile's marker count exceeds SP_IDX.
 __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny, int ntx, int nty,
-                                            const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
-                                            const int32_t* __restrict__ tlist, const uint32_t* __restrict__ tpos,
+                                            const uint32_t* __restrict__ tcnt, const int32_t* __restrict__ tlist,
+                                            const uint32_t* __restrict__ tpos, const BinOvf* __restrict__ ovf,
                                             const double* __restrict__ mx, const double* __restrict__ my,
                                             const double* __restrict__ mfx, const double* __restrict__ mfy,
-                                            double* Fx, double* Fy) {
+                                            double* Fx, double* Fy, const DevStatus* st) {
     const int tx = t % ntx, ty = t / ntx;
     const int bx = tx * ST, by = ty * ST;
-    // the 3x3 home tiles' lists, loaded at once
+    // the 3x3 home tiles' bins, counts loaded at once
     if (threadIdx.x < 9) {
         const int hx = tx + (int)(threadIdx.x % 3) - 1, hy = ty + (int)(threadIdx.x / 3) - 1;
         const bool in = hx >= 0 && hy >= 0 && hx < ntx && hy < nty;
         const int h = hy * ntx + hx;
         L.hcnt[threadIdx.x] = in ? tcnt[h] : 0u;
-        L.hoff[threadIdx.x] = in ? toff[h] : 0u;
+        L.htile[threadIdx.x] = in ? h : -1;
     }
-    if (threadIdx.x == 0) L.n = 0;
+    if (threadIdx.x == 0) {
+        L.n = 0;
+        L.novf = min(st->novf, OVF_CAP);
+    }
     L.smask[0][threadIdx.x] = 0ull;
     L.smask[1][threadIdx.x] = 0ull;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = 0;
-        for (int h = 0; h < 9; h++) { L.hpre[h] = a; a += L.hcnt[h]; }
+        bool over = false;
+        for (int h = 0; h < 9; h++) {
+            L.hpre[h] = a;
+            a += min(L.hcnt[h], (uint32_t)BINCAP);
+            over |= L.hcnt[h] > (uint32_t)BINCAP;
+        }
         L.hpre[9] = a;
+        if (!over) L.novf = 0;   // none of these tiles overflowed
     }
     __syncthreads();
+    // keep only markers whose 4x4 stencil overlaps this tile (the binning
+    // stored floor(X), floor(Y) with the entry)
+    auto gather = [&](int k, uint32_t pp) {
+        const int fx = (int)(pp & 0xffffu), fy = (int)(pp >> 16);
+        if (fx + 2 < bx || fx - 1 > bx + ST - 1 || fy + 2 < by || fy - 1 > by + ST - 1) return;
+        const uint32_t p = atomicAdd(&L.n, 1u);
+        if (p < (uint32_t)SP_IDX) L.idx[p] = k;
+    };
     for (uint32_t j = threadIdx.x; j < L.hpre[9]; j += blockDim.x) {
         int h = 0;
         while (j >= L.hpre[h + 1]) h++;
-        const uint32_t e = L.hoff[h] + (j - L.hpre[h]);
-        const int k = tlist[e];
-        // keep only markers whose 4x4 stencil overlaps this tile (the binning
-        // stored floor(X), floor(Y) with the list entry)
-        const uint32_t pp = tpos[e];
-        const int fx = (int)(pp & 0xffffu), fy = (int)(pp >> 16);
-        if (fx + 2 < bx || fx - 1 > bx + ST - 1 || fy + 2 < by || fy - 1 > by + ST - 1) continue;
-        const uint32_t p = atomicAdd(&L.n, 1u);
-        if (p < (uint32_t)SP_IDX) L.idx[p] = k;
+        const long e = (long)L.htile[h] * BINCAP + (j - L.hpre[h]);
+        gather(tlist[e], tpos[e]);
+    }
+    for (uint32_t j = threadIdx.x; j < L.novf; j += blockDim.x) {
+        const BinOvf o = ovf[j];
+        bool mine = false;
+        for (int h = 0; h < 9; h++) mine |= o.tile == L.htile[h];
+        if (mine) gather(o.k, o.pos);
     }
     __syncthreads();
     const uint32_t m = L.n;
@@ -276,18 +357,19 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
 // otherwise load some XCDs far more than others.
 template <int MAP>
 __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
-                                                     const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tcnt,
-                                                     const int32_t* __restrict__ tlist,
-                                                     const uint32_t* __restrict__ tpos, const double* __restrict__ mx,
-                                                     const double* __restrict__ my, const double* __restrict__ mfx,
-                                                     const double* __restrict__ mfy, double* Fx, double* Fy,
-                                                     int begin, uint32_t begin_it, DevStatus* st) {
+                                                     const uint32_t* __restrict__ tcnt, const int32_t* __restrict__ tlist,
+                                                     const uint32_t* __restrict__ tpos, const BinOvf* __restrict__ ovf,
+                                                     const double* __restrict__ mx, const double* __restrict__ my,
+                                                     const double* __restrict__ mfx, const double* __restrict__ mfy,
+                                                     double* Fx, double* Fy, uint32_t* shard, int begin,
+                                                     uint32_t begin_it, DevStatus* st) {
     static_assert(SP_CH == 128, "two 64-bit site masks per staged chunk");
     if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     const int t = MAP == 0 ? (int)xcd_contiguous(blockIdx.x, gridDim.x) : row_interleaved_tile(blockIdx.x, ntx, nty);
     if (t < 0 || !tact[t] || step_stopped(st)) return;
+    if (threadIdx.x == 0) atomicAdd(&shard[(SH_TOUCHED * NSHARD + (blockIdx.x % NSHARD)) * SHARD_STRIDE], 1u);
     __shared__ SpreadLds L;
-    if (!spread_tile(L, t, nx, ny, ntx, nty, toff, tcnt, tlist, tpos, mx, my, mfx, mfy, Fx, Fy) &&
+    if (!spread_tile(L, t, nx, ny, ntx, nty, tcnt, tlist, tpos, ovf, mx, my, mfx, mfy, Fx, Fy, st) &&
         threadIdx.x == 0)
         atomicOr(&st->bits, LBIB_ERR_CAPACITY);
 }
@@ -313,84 +395,18 @@ __device__ __forceinline__ float fast_cos_f32(float x) {
     return c;
 }
 
-// The tiles a marker's 4x4 stencil reaches: quadrant q takes the tile of the
-// stencil's corner (q & 1: right column, q & 2: top row) when it differs from
-// the tiles of the lower quadrants.
-struct StencilTiles {
-    int tx0, tx1, ty0, ty1;
-    __device__ StencilTiles(int x0, int y0) : tx0(x0 / ST), tx1((x0 + 3) / ST), ty0(y0 / ST), ty1((y0 + 3) / ST) {}
-    __device__ bool has(int q) const { return ((q & 1) == 0 || tx1 != tx0) && ((q & 2) == 0 || ty1 != ty0); }
-    __device__ int tile(int q, int ntx) const { return ((q & 2) ? ty1 : ty0) * ntx + ((q & 1) ? tx1 : tx0); }
-};
-
-// Counts each marker into the tiles its stencil reaches (tcnt, zeroed by
-// k_marker_force) and keeps its slot per quadrant; runs of lanes with the
-// same tile in a wavefront take one atomic.
-__global__ void k_push_count(int ntx, const double* __restrict__ mx, const double* __restrict__ my, uint32_t* tcnt,
-                             uint32_t* pslot, int pcap, uint8_t* tact, const DevStatus* st) {
-    if (step_stopped(st)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = k < (int)st->nm;
-    int x0 = 0, y0 = 0;
-    if (live) {
-        x0 = (int)floor(mx[k]) - 1;
-        y0 = (int)floor(my[k]) - 1;
-    }
-    const StencilTiles T(x0, y0);
-    const int lane = threadIdx.x & 63;
-    const unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const bool part = live && T.has(q);
-        const int key = part ? T.tile(q, ntx) : -1 - lane;   // lanes not taking part end every run
-        const int kprev = __shfl_up(key, 1, 64);
-        const bool head = lane == 0 || kprev != key;
-        const unsigned long long heads = __ballot(head);
-        const int hl = 63 - __clzll((long long)(heads & le));
-        const unsigned long long gt = heads & ~le;
-        const int nextl = gt ? __ffsll((long long)gt) - 1 : 64;
-        uint32_t base = 0;
-        if (head && part) base = atomicAdd(&tcnt[key], (uint32_t)(nextl - lane));
-        base = __shfl(base, hl, 64);
-        if (part) {
-            pslot[(size_t)q * pcap + k] = base + (uint32_t)(lane - hl);
-            tact[key] = 1;
-        }
-    }
-}
-
-__global__ void k_push_scatter(int ntx, const double* __restrict__ mx, const double* __restrict__ my,
-                               const double* __restrict__ mfx, const double* __restrict__ mfy,
-                               const uint32_t* __restrict__ toff, const uint32_t* __restrict__ pslot, int pcap,
-                               double* ex, double* ey, double* efx, double* efy, const DevStatus* st) {
-    if (step_stopped(st)) return;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= (int)st->nm) return;
-    const double X = mx[k], Y = my[k], fx = mfx[k], fy = mfy[k];
-    const StencilTiles T((int)floor(X) - 1, (int)floor(Y) - 1);
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        if (!T.has(q)) continue;
-        const uint32_t e = toff[T.tile(q, ntx)] + pslot[(size_t)q * pcap + k];
-        ex[e] = X;
-        ey[e] = Y;
-        efx[e] = fx;
-        efy[e] = fy;
-    }
-}
-
 template <int MAP>
-__global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, int nty, const uint32_t* __restrict__ toff,
-                                                      const uint32_t* __restrict__ tcnt, const double* __restrict__ ex,
-                                                      const double* __restrict__ ey, const double* __restrict__ efx,
-                                                      const double* __restrict__ efy, float2* F, int begin,
-                                                      uint32_t begin_it, DevStatus* st) {
+__global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, int nty, const uint32_t* __restrict__ tcnt,
+                                                      const uint32_t* __restrict__ pb_pos, const float2* __restrict__ pb_r,
+                                                      const float2* __restrict__ pb_f, const BinOvf* __restrict__ ovf,
+                                                      float2* F, uint32_t* shard, int begin, uint32_t begin_it,
+                                                      DevStatus* st) {
     if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     const int t = MAP == 0 ? (int)xcd_contiguous(blockIdx.x, gridDim.x) : row_interleaved_tile(blockIdx.x, ntx, nty);
     if (t < 0 || step_stopped(st)) return;
     const uint32_t n = tcnt[t];
     if (n == 0) return;
-    const uint32_t o = toff[t];
+    if (threadIdx.x == 0) atomicAdd(&shard[(SH_TOUCHED * NSHARD + (blockIdx.x % NSHARD)) * SHARD_STRIDE], 1u);
     __shared__ unsigned long long acc[2][ST * ST];
     __shared__ uint32_t bad[ST * ST];   // a non-finite or out-of-range product reached the site
     acc[0][threadIdx.x] = 0ull;
@@ -399,18 +415,18 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, i
     __syncthreads();
     const int tx = t % ntx, ty = t / ntx;
     const int bx = tx * ST, by = ty * ST;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const double X = ex[o + i], Y = ey[o + i];
-        const int x0 = (int)floor(X) - 1, y0 = (int)floor(Y) - 1;
-        const float rx = (float)(X - (double)x0), ry = (float)(Y - (double)y0);
+    // one entry: stencil corner (x0, y0), offsets (rx, ry) = (X - x0, Y - y0)
+    // and the force, all as the binning rounded them
+    auto add = [&](uint32_t pos, float2 r, float2 f) {
+        const int x0 = (int)(pos & 0xffffu), y0 = (int)(pos >> 16);
         float fa[4], fb[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            fa[q] = 1.0f + fast_cos_f32(1.57079632679f * ((float)q - rx));
-            fb[q] = 1.0f + fast_cos_f32(1.57079632679f * ((float)q - ry));
+            fa[q] = 1.0f + fast_cos_f32(1.57079632679f * ((float)q - r.x));
+            fb[q] = 1.0f + fast_cos_f32(1.57079632679f * ((float)q - r.y));
         }
         // delta = fa * fb / 16: the 1/16 goes into the scale
-        const float gx = (float)efx[o + i] * (kFixScale / 16.0f), gy = (float)efy[o + i] * (kFixScale / 16.0f);
+        const float gx = f.x * (kFixScale / 16.0f), gy = f.y * (kFixScale / 16.0f);
 #pragma unroll
         for (int qy = 0; qy < 4; qy++) {
             const int sy = y0 + qy - by;
@@ -429,6 +445,16 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, i
                 atomicAdd(&acc[0][site], (unsigned long long)__float2ll_rn(vx));
                 atomicAdd(&acc[1][site], (unsigned long long)__float2ll_rn(vy));
             }
+        }
+    };
+    const long o = (long)t * BINCAP;
+    const uint32_t nb = min(n, (uint32_t)BINCAP);
+    for (uint32_t i = threadIdx.x; i < nb; i += blockDim.x) add(pb_pos[o + i], pb_r[o + i], pb_f[o + i]);
+    if (n > (uint32_t)BINCAP) {
+        const uint32_t no = min(st->novf, OVF_CAP);
+        for (uint32_t j = threadIdx.x; j < no; j += blockDim.x) {
+            const BinOvf e = ovf[j];
+            if (e.tile == t) add(e.pos, make_float2(e.rx, e.ry), make_float2(e.fx, e.fy));
         }
     }
     __syncthreads();
@@ -521,6 +547,26 @@ __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mv
 
 static inline int nblocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+static BinArgs bin_args(lbib_ctx* c, bool forces) {
+    BinArgs a;
+    a.mx = c->mx; a.my = c->my;
+    a.off = forces ? c->fs_off : nullptr;
+    a.type = c->fs_type; a.partner = c->fs_partner; a.prm = c->fs_prm;
+    a.mfx = c->mfx; a.mfy = c->mfy;
+    a.ntx = c->stx; a.nty = c->sty;
+    a.tcnt = c->tcnt; a.tact = c->tact;
+    a.tlist = c->tlist; a.tpos = c->tpos;
+    a.pb_pos = c->pb_pos; a.pb_r = c->pb_r; a.pb_f = c->pb_f;
+    a.ovf = c->ovf;
+    return a;
+}
+
+static void launch_bin_clear(lbib_ctx* c) {
+    const int nt = c->stx * c->sty;
+    k_bin_clear<<<std::max(std::min(nblocks(nt, 256), 1024), 1), 256, 0, c->stream>>>(c->tcnt, c->tact, nt, c->shard,
+                                                                                      c->st);
+}
+
 // The marker forces and the tile binning read the topology, the marker
 // positions and the force structs only.  When nothing enqueued since the last
 // geometry stage changed those (forces_beside), they follow the topology stage
@@ -540,11 +586,15 @@ int launch_force_and_spread(lbib_ctx* c) {
     if (!beside && join_topology(c)) return -1;
     const hipStream_t main = c->stream;
     if (beside) c->stream = c->stream2;
+    const bool mixed = c->prec != LBIB_FP64;
     // marker counts are device-resident (remeshing changes them): grids cover
     // the capacity and the kernels read st->nm
-    k_marker_force<<<std::max(nblocks(c->mcap, bs), 1), bs, 0, c->stream>>>(
-        c->mx, c->my, c->fs_off, c->fs_type, c->fs_partner, c->fs_prm, c->mfx, c->mfy, ntiles, c->tcnt, c->tact,
-        beside ? 0 : begin, begin_it, c->st);
+    launch_bin_clear(c);
+    const BinArgs a = bin_args(c, true);
+    const int fgrid = std::max(nblocks(c->mcap, bs), 1);
+    if (mixed) k_force_bin<1><<<fgrid, bs, 0, c->stream>>>(a, beside ? 0 : begin, begin_it, c->st);
+    else k_force_bin<0><<<fgrid, bs, 0, c->stream>>>(a, beside ? 0 : begin, begin_it, c->st);
+    c->stream = main;
     if (c->nm == 0) {
         LBIB_HIP(hipGetLastError());
         return 0;
@@ -553,26 +603,6 @@ int launch_force_and_spread(lbib_ctx* c) {
     // rectangles are unevenly spread); one lattice keeps tiles together
     const int map = c->nrect > 0 ? 1 : 0;
     const int grid = map ? 8 * ((c->sty + 7) / 8) * c->stx : ntiles;
-    const bool mixed = c->prec != LBIB_FP64;
-    int r = 0;
-    if (mixed) {
-        k_push_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->tcnt, c->pslot, c->mcap,
-                                                                 c->tact, c->st);
-        r = scan_tiles(c, c->tact, &c->st->ntouched);
-        if (!r)
-            k_push_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->mfx, c->mfy, c->toff,
-                                                                       c->pslot, c->mcap, c->pe_x, c->pe_y, c->pe_fx,
-                                                                       c->pe_fy, c->st);
-    } else {
-        k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
-                                                                 c->tact, c->st);
-        r = scan_tiles(c, c->tact, &c->st->ntouched);
-        if (!r)
-            k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot,
-                                                                       c->tlist, c->tpos, c->st);
-    }
-    c->stream = main;
-    if (r) return -1;
     const int sbegin = beside ? begin : 0;
     if (beside) {
         LBIB_HIP(hipEventRecord(c->ev_join, c->stream2));
@@ -582,38 +612,29 @@ int launch_force_and_spread(lbib_ctx* c) {
     if (mixed) {
         float2* F = reinterpret_cast<float2*>(c->Fx);
         if (map)
-            k_spread_fixed<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->toff, c->tcnt, c->pe_x,
-                                                          c->pe_y, c->pe_fx, c->pe_fy, F, sbegin, begin_it, c->st);
+            k_spread_fixed<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tcnt, c->pb_pos, c->pb_r,
+                                                          c->pb_f, c->ovf, F, c->shard, sbegin, begin_it, c->st);
         else
-            k_spread_fixed<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->toff, c->tcnt, c->pe_x,
-                                                          c->pe_y, c->pe_fx, c->pe_fy, F, sbegin, begin_it, c->st);
+            k_spread_fixed<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tcnt, c->pb_pos, c->pb_r,
+                                                          c->pb_f, c->ovf, F, c->shard, sbegin, begin_it, c->st);
     } else if (map) {
-        k_tile_spread<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                      c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy,
+        k_tile_spread<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->tcnt, c->tlist, c->tpos,
+                                                      c->ovf, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->shard,
                                                       sbegin, begin_it, c->st);
     } else {
-        k_tile_spread<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->toff, c->tcnt,
-                                                      c->tlist, c->tpos, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy,
+        k_tile_spread<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->tcnt, c->tlist, c->tpos,
+                                                      c->ovf, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->shard,
                                                       sbegin, begin_it, c->st);
     }
     LBIB_HIP(hipGetLastError());
     return 0;
 }
 
-__global__ void k_tile_clear(uint32_t* tcnt, uint32_t n, const DevStatus* st) {
-    if (step_stopped(st)) return;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) tcnt[t] = 0u;
-}
-
+// Ordered bins of the markers without the forces (the junction search).
 int launch_tile_bins(lbib_ctx* c) {
-    const int bs = 256;
-    const uint32_t nt = (uint32_t)c->stx * (uint32_t)c->sty;
-    k_tile_clear<<<std::min(nblocks((long)nt, bs), 1024), bs, 0, c->stream>>>(c->tcnt, nt, c->st);
-    k_tile_count<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->sty, c->mx, c->my, c->tcnt, c->tslot,
-                                                             c->tact, c->st);
-    if (scan_exclusive(c, c->tcnt, c->toff, nullptr, nt, nt, nullptr)) return -1;
-    k_tile_scatter<<<nblocks(c->mcap, bs), bs, 0, c->stream>>>(c->stx, c->mx, c->my, c->toff, c->tslot, c->tlist, c->tpos,
-                                                               c->st);
+    launch_bin_clear(c);
+    const BinArgs a = bin_args(c, false);
+    k_force_bin<0><<<std::max(nblocks(c->mcap, 256), 1), 256, 0, c->stream>>>(a, 0, 0u, c->st);
     LBIB_HIP(hipGetLastError());
     return 0;
 }

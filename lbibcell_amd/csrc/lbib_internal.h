@@ -35,6 +35,17 @@ constexpr int REINIT_LDS = 8192;          // level state of the changed sites ke
 constexpr uint32_t RM_LIST_CAP = 4096;
 constexpr int REINIT_CAP = 1 << 20;       // changed sites per step (global-memory state beyond REINIT_LDS)
 constexpr int SEQ_PER_CONN = 256;   // candidates per connection (segment < 127 lu)
+// Marker bins of the 16x16 spread tiles: BINCAP entries per tile, written in
+// one pass by atomic slot counts; entries beyond a tile's capacity go to one
+// shared overflow list of OVF_CAP entries that every reader also scans.
+constexpr int BINCAP = 256;
+constexpr uint32_t OVF_CAP = 1u << 16;
+// Sharded counters: a statistic that thousands of workgroups add to is spread
+// over NSHARD words 128 bytes apart (workgroup b adds to word b % NSHARD), so
+// the atomics do not queue on one address; readers sum the shards.
+constexpr int NSHARD = 64;
+constexpr int SHARD_STRIDE = 32;    // uint32 words between shards (one 128-byte line)
+enum { SH_TOUCHED = 0, SH_NBN = 1, SH_COUNT = 2 };
 
 struct SpeciesDev {
     int kind;
@@ -49,11 +60,9 @@ struct SpeciesDev {
 struct DevStatus {
     uint32_t bits;
     int32_t first_bad;
-    uint32_t ntouched;      // spread: 16x16 tiles written this step
     uint32_t pool_used;     // unused
     uint32_t ncand;         // cut-link candidates of the current step
     uint32_t ncand_prev;    // candidates of the last completed cure (boundary-node reset list)
-    uint32_t nbn;           // winners
     uint32_t nchanged;      // sites whose local domain changed
     uint32_t nm, nc, maxid; // topology counts and the largest GeometryNode id
     uint32_t rm_L;          // long connections in the current remesh round
@@ -66,6 +75,7 @@ struct DevStatus {
     uint32_t rm_K;             // long connections listed by the current remesh round's flag pass
     uint32_t rm_sparse;        // the round's ranks come from the sorted list (else from the scan)
     uint32_t topo_changed;     // remesh or coarsening changed the topology in this step
+    uint32_t novf;             // entries in the bin overflow list
 };
 
 // Raised by remeshing for a new node within 2 sites of the edge; reported as
@@ -126,10 +136,15 @@ __device__ __forceinline__ bool site_frame(const int32_t* rtile, const RectDev* 
     return x - R.x0 < R.w && y - R.y0 < R.h;
 }
 
-struct SpreadEntry {
-    int32_t marker;
-    int32_t pad;
-    double cx, cy;
+// An entry of the bin overflow list: the tile it belongs to, the floor
+// coordinates of the marker (ordered bins: floor(X) | floor(Y) << 16; push
+// bins: those of the stencil corner), and either the marker index (ordered
+// bins) or its offsets from the stencil corner and force in FP32 (push bins).
+struct BinOvf {
+    int32_t tile;
+    uint32_t pos;
+    int32_t k;
+    float rx, ry, fx, fy;
 };
 
 struct Cand {
@@ -188,14 +203,18 @@ struct lbib_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool topo_pending = false, forces_beside = false;
-    uint32_t *tcnt = nullptr, *toff = nullptr, *tslot = nullptr;
+    uint32_t* tcnt = nullptr;          // [nt] entries binned into each tile (may exceed BINCAP)
     uint8_t* tact = nullptr;
-    int32_t* tlist = nullptr;
-    uint32_t* tpos = nullptr;          // floor(X) | floor(Y) << 16 of each tlist entry
-    // mixed-precision spread: every tile a marker's stencil reaches lists the
-    // marker with its position and force (at most 4 entries per marker)
-    uint32_t* pslot = nullptr;         // [4][mcap] slot of the marker in each reached tile
-    double *pe_x = nullptr, *pe_y = nullptr, *pe_fx = nullptr, *pe_fy = nullptr;   // [4 * mcap]
+    // ordered bins (FP64 spread, junction search): markers by the tile of
+    // floor(X), floor(Y); BINCAP entries per tile
+    int32_t* tlist = nullptr;          // [nt][BINCAP] marker index
+    uint32_t* tpos = nullptr;          // [nt][BINCAP] floor(X) | floor(Y) << 16
+    // push bins (mixed-precision spread): every tile a marker's stencil
+    // reaches lists the stencil corner, the offsets from it and the force
+    uint32_t* pb_pos = nullptr;        // [nt][BINCAP] x0 | y0 << 16
+    float2 *pb_r = nullptr, *pb_f = nullptr;   // [nt][BINCAP] (X - x0, Y - y0), (Fx, Fy) in FP32
+    lbib::BinOvf* ovf = nullptr;       // [OVF_CAP] entries past a tile's capacity (count: st->novf)
+    uint32_t* shard = nullptr;         // [SH_COUNT][NSHARD][SHARD_STRIDE] sharded counters
     // markers
     int mcap = 0, nm = 0;
     double *mx = nullptr, *my = nullptr, *mvx = nullptr, *mvy = nullptr;
@@ -291,6 +310,7 @@ struct lbib_ctx {
     // status
     lbib::DevStatus* st = nullptr;
     lbib::DevStatus* st_host = nullptr;   // pinned
+    uint32_t* shard_host = nullptr;       // pinned copy of the sharded counters
 };
 
 namespace lbib {
@@ -396,7 +416,6 @@ int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_
                    uint32_t* total, const uint32_t* gate = nullptr);
 // exclusive scan of the per-tile marker counts (tcnt -> toff); for short tile
 // arrays also the number of nonzero active flags into *nactive
-int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive);
 int launch_inout(lbib_ctx* c);
 // gated: skip (keeping the current ring order) unless st->topo_changed
 int launch_rings(lbib_ctx* c, bool counters_cleared = false, bool gated = false);

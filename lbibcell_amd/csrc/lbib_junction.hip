@@ -40,8 +40,8 @@ __global__ void k_junction_clear(uint32_t* in, int n, const DevStatus* st) {
 // GeometryNode::getDomainIdOfAdjacentConnections (GeometryNode.cpp:267-283):
 // the incoming connection's domain (both adjacent connections carry it)
 __global__ void k_junction_find(int ntx, int nty, double radius, const double* __restrict__ mx,
-                                const double* __restrict__ my, const uint32_t* __restrict__ toff,
-                                const uint32_t* __restrict__ tcnt, const int32_t* __restrict__ tlist,
+                                const double* __restrict__ my, const uint32_t* __restrict__ tcnt,
+                                const int32_t* __restrict__ tlist, const BinOvf* __restrict__ ovf,
                                 const uint32_t* __restrict__ cdom, const int32_t* __restrict__ min_c,
                                 const int32_t* __restrict__ rtile, int32_t* __restrict__ part, uint32_t* in_cnt,
                                 int32_t* __restrict__ in_src, DevStatus* st) {
@@ -59,23 +59,36 @@ __global__ void k_junction_find(int ntx, int nty, double radius, const double* _
     const int rq = rtile ? rtile[hy * ntx + hx] : -1;
     int best = -1;
     double bd = 0.0;
+    auto consider = [&](int q) {
+        const double dx = mx[q] - x, dy = my[q] - y;
+        const double d = sqrt(dx * dx + dy * dy);
+        if (d > radius) return;
+        if (cdom[min_c[q]] == avoid) return;
+        if (best < 0 || d < bd || (d == bd && q < best)) {
+            best = q;
+            bd = d;
+        }
+    };
+    bool over = false;
     for (int ty = ty0; ty <= ty1; ty++)
         for (int tx = tx0; tx <= tx1; tx++) {
             const int t = ty * ntx + tx;
             if (rtile && rtile[t] != rq) continue;
-            const uint32_t b = toff[t], e = b + tcnt[t];
-            for (uint32_t i = b; i < e; i++) {
-                const int q = tlist[i];
-                const double dx = mx[q] - x, dy = my[q] - y;
-                const double d = sqrt(dx * dx + dy * dy);
-                if (d > radius) continue;
-                if (cdom[min_c[q]] == avoid) continue;
-                if (best < 0 || d < bd || (d == bd && q < best)) {
-                    best = q;
-                    bd = d;
-                }
-            }
+            const uint32_t n = tcnt[t];
+            over |= n > (uint32_t)BINCAP;
+            const long b = (long)t * BINCAP;
+            for (uint32_t i = 0; i < min(n, (uint32_t)BINCAP); i++) consider(tlist[b + i]);
         }
+    // entries past a tile's capacity
+    if (over) {
+        const uint32_t no = min(st->novf, OVF_CAP);
+        for (uint32_t j = 0; j < no; j++) {
+            const int t = ovf[j].tile, tx = t % ntx, ty = t / ntx;
+            if (tx < tx0 || tx > tx1 || ty < ty0 || ty > ty1) continue;
+            if (rtile && rtile[t] != rq) continue;
+            consider(ovf[j].k);
+        }
+    }
     part[k] = best;
     if (best >= 0) {
         const uint32_t slot = atomicAdd(&in_cnt[best], 1u);
@@ -184,7 +197,7 @@ int launch_forces_junctions(lbib_ctx* c, double radius, double k, double l0) {
     if (launch_tile_bins(c)) return -1;
     k_junction_clear<<<blocks_for(c->mcap, bs), bs, 0, c->stream>>>(c->j_in, c->jcap, c->st);
     k_junction_find<<<blocks_for(c->mcap, bs), bs, 0, c->stream>>>(
-        c->stx, c->sty, radius, c->mx, c->my, c->toff, c->tcnt, c->tlist, c->cdom, c->min_c,
+        c->stx, c->sty, radius, c->mx, c->my, c->tcnt, c->tlist, c->ovf, c->cdom, c->min_c,
         c->nrect ? c->rtile : nullptr, c->j_part, c->j_in, c->j_src, c->st);
     k_junction_count<<<blocks_for(c->mcap, bs), bs, 0, c->stream>>>(c->fs_off, c->fs_type, c->j_part, c->j_in,
                                                                      c->j_cnt, c->st);

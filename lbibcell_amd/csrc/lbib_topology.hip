@@ -153,14 +153,6 @@ __global__ __launch_bounds__(1024) void k_scan_single(const uint32_t* __restrict
     scan_single_dev(in, nptr ? *nptr : nfix, out, total, nz, nz_total);
 }
 
-int scan_tiles(lbib_ctx* c, const uint8_t* active, uint32_t* nactive) {
-    const uint32_t nt = (uint32_t)c->stx * (uint32_t)c->sty;
-    if (nt > SCAN_SINGLE_MAX) return scan_exclusive(c, c->tcnt, c->toff, nullptr, nt, nt, nullptr);
-    k_scan_single<<<1, 1024, 0, c->stream>>>(c->tcnt, nullptr, nt, c->toff, nullptr, active, nactive, nullptr);
-    LBIB_HIP(hipGetLastError());
-    return 0;
-}
-
 int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
                    uint32_t* total, const uint32_t* gate) {
     const uint32_t nb = (cap + SCAN_B - 1) / SCAN_B;
