@@ -57,6 +57,7 @@ __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ c
     }
     if (blockIdx.x == 0 && threadIdx.x < NSHARD) shard[(SH_NBN * NSHARD + threadIdx.x) * SHARD_STRIDE] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->cure_epoch = st->cure_epoch + 1u;
         st->ncand = 0;
         st->nchanged = 0;
         st->rm_total = 0;
@@ -178,6 +179,11 @@ __global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned 
     }
 }
 
+// rmark[site] of a site listed for re-initialisation: the cure that listed it
+// (a counter k_geo_begin advances) and its position p in the changed list, so
+// that entries left by earlier cures never match.
+__device__ __forceinline__ uint64_t rmark_val(uint32_t epoch, uint32_t p) { return ((uint64_t)epoch << 32) | p; }
+
 // Winners: mark the slot, record the domain and (for BC-carrying connections)
 // the node velocity of Connection::calculateVelocity (Connection.cpp:146-168).
 // Then PhysicalNode::updateDomainIdentifier (PhysicalNode.cpp:109-124): a site
@@ -230,10 +236,10 @@ __global__ __launch_bounds__(WIN_WG) void k_cut_win(const Cand* cand, long n,
                                                     const unsigned long long* __restrict__ best_dist,
                                                     const uint32_t* __restrict__ best_seq, uint32_t* tag,
                                                     uint32_t* bn_dom, double* bc_v, int32_t* bc_desc,
-                                                    uint32_t* changed, uint32_t* rmark, uint8_t* rstate,
+                                                    uint32_t* changed, uint64_t* rmark, uint8_t* rstate,
                                                     uint32_t* shard, int local_update, DevStatus* st) {
     if (halted_beside_topology(st)) return;
-    const uint32_t nc = st->ncand;
+    const uint32_t nc = st->ncand, epoch = st->cure_epoch;
     for (uint32_t j0 = blockIdx.x * (uint32_t)WIN_PER_WG; j0 < nc; j0 += gridDim.x * (uint32_t)WIN_PER_WG) {
         // both candidates' loads in flight together
         const WinOut a = cut_winner(cand, j0 + threadIdx.x, nc, n, best_dist, best_seq, tag, bn_dom, bc_v, bc_desc,
@@ -250,7 +256,7 @@ __global__ __launch_bounds__(WIN_WG) void k_cut_win(const Cand* cand, long n,
             if (p < (uint32_t)REINIT_CAP) {
                 changed[2 * p] = w.site;
                 changed[2 * p + 1] = w.dom;
-                rmark[w.site] = p + 1;
+                rmark[w.site] = rmark_val(epoch, p);
                 rstate[p] = 0;
             } else {
                 atomicOr(&st->bits, LBIB_ERR_CAPACITY);
@@ -269,7 +275,7 @@ struct ReinitArgs {
     const void* u;       // velocity pairs (ux, uy): FP64, or FP32 for mixed precision
     uint32_t* tag;
     const uint32_t* changed;
-    const uint32_t* rmark;
+    const uint64_t* rmark;
     uint32_t it;
     int stx;
     const DevStatus* st;    // iteration of ITER_NEXT
@@ -278,7 +284,10 @@ struct ReinitArgs {
     SpeciesDev sp[LBIB_MAX_SPECIES];
 };
 
-// P: storage format; the re-initialisation itself is FP64 arithmetic.
+// P: storage format; the re-initialisation itself is FP64 arithmetic.  The
+// neighbour tags and the axis neighbours' concentrations are loaded before
+// any is used (the sums below keep the reference's order), so a site costs a
+// few dependent memory round trips rather than one per neighbour.
 template <int NS, int P>
 __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd) {
     const int x = (int)(i % a.nx), y = (int)(i / a.nx);
@@ -288,17 +297,25 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
     if (!site_frame(a.rtile, a.rects, a.stx, a.nx, a.ny, x, y, fr)) {
         fr.x0 = 0; fr.y0 = 0; fr.w = a.nx; fr.h = a.ny; fr.r = -1;   // no domain owns it
     }
-    auto nb = [&](int d) -> long {
+    long nbi[9];
+#pragma unroll
+    for (int d = 1; d < 9; d++) {
         const int xx = fr.x0 + (x - fr.x0 + EXd[d] + fr.w) % fr.w;
         const int yy = fr.y0 + (y - fr.y0 + EYd[d] + fr.h) % fr.h;
-        return (long)yy * a.nx + xx;
-    };
+        nbi[d] = (long)yy * a.nx + xx;
+    }
     constexpr int GQ = NS > 0 ? 5 * NS : 5;   // species planes in the storage blocks
     using GS = GStore<P>;
     auto cswap = [&](int k, long j) -> double {
         return c_swapped(GS::get(a.g, k, 0, j, a.n, GQ), GS::get(a.g, k, 1, j, a.n, GQ), GS::get(a.g, k, 2, j, a.n, GQ),
                          GS::get(a.g, k, 3, j, a.n, GQ), GS::get(a.g, k, 4, j, a.n, GQ));
     };
+    // the neighbours' domains (no neighbour is re-initialised concurrently:
+    // sites of one level are never adjacent) and the site's own values
+    uint32_t tn[9];
+#pragma unroll
+    for (int d = 1; d < 9; d++) tn[d] = a.tag[nbi[d]] & DOM_MASK;
+    const uint32_t t0 = a.tag[i];
     double ux_i, uy_i;
     if (P == LBIB_FP64) {
         ux_i = static_cast<const double*>(a.u)[2 * i];
@@ -307,27 +324,29 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
         ux_i = (double)static_cast<const float*>(a.u)[2 * i];
         uy_i = (double)static_cast<const float*>(a.u)[2 * i + 1];
     }
-    a.tag[i] = (a.tag[i] & ~DOM_MASK) | nd;
-    const uint32_t tg = a.tag[i];
+    const uint32_t tg = (t0 & ~DOM_MASK) | nd;
+    a.tag[i] = tg;
+    const int dir1[4] = {DE, DN, DW, DS};
+    const int dir2[4] = {DNE, DNW, DSW, DSE};
     for (int k = 0; k < NS; k++) {
+        double c1[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) c1[q] = cswap(k, nbi[dir1[q]]);
         double sumC = 0.0;
         int counter = 0;
-        const int dir1[4] = {DE, DN, DW, DS};
+#pragma unroll
         for (int q = 0; q < 4; q++) {
             const int d = dir1[q];
-            const long j = nb(d);
             const bool hasbn = (tg >> (BN_SHIFT + d - 1)) & 1u;
-            if (!hasbn && nd == (a.tag[j] & DOM_MASK)) { sumC += cswap(k, j); counter++; }
+            if (!hasbn && nd == tn[d]) { sumC += c1[q]; counter++; }
         }
         if (counter == 0) {
-            const int dir2[4] = {DNE, DNW, DSW, DSE};
-            for (int q = 0; q < 4; q++) {
-                const long j = nb(dir2[q]);
-                if (nd == (a.tag[j] & DOM_MASK)) { sumC += cswap(k, j); counter++; }
-            }
+            for (int q = 0; q < 4; q++)
+                if (nd == tn[dir2[q]]) { sumC += cswap(k, nbi[dir2[q]]); counter++; }
         }
         if (counter == 0 && a.sp[k].kind != LBIB_SPECIES_PLAIN) {
-            for (int q = 0; q < 4; q++) { sumC += cswap(k, nb(dir1[q])); counter++; }
+#pragma unroll
+            for (int q = 0; q < 4; q++) { sumC += c1[q]; counter++; }
         }
         sumC /= static_cast<double>(counter);
         double d[5];
@@ -351,8 +370,7 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
 // follow it still hold their old values when it runs, as in the reference.
 // Ready sites are never adjacent, so each level runs in parallel and the
 // result is identical.  A changed neighbour is found through rmark[site]
-// (1 + its position in the list, checked against the list, so entries left by
-// earlier steps never match).
+// (its position in the list, tagged with the cure epoch).
 //
 // The first REINIT_GRID_LEVELS levels run as one grid-wide launch each (most
 // sites are ready at once); state 2 + L marks a site done by level launch L,
@@ -361,37 +379,43 @@ __device__ void reinit_site(const ReinitArgs& a, const long i, const uint32_t nd
 // >= 2 = done).
 constexpr int REINIT_GRID_LEVELS = 2;
 
-__device__ __forceinline__ int reinit_find(const ReinitArgs& a, uint32_t cnt, uint32_t site) {
-    const uint32_t m = a.rmark[site];
-    return (m != 0u && m <= cnt && a.changed[2 * (m - 1)] == site) ? (int)(m - 1) : -1;
-}
-
-// are all changed neighbours preceding site i finished (state in [lo, hi))?
-__device__ __forceinline__ bool reinit_ready(const ReinitArgs& a, uint32_t cnt, uint32_t i, const uint8_t* state,
-                                             int lo, int hi) {
+// Are all changed neighbours preceding site i finished (state in [lo, hi))?
+// The eight neighbours' marks are loaded at once, then the states of the
+// listed ones.  A changed neighbour carries this cure's epoch in its mark.
+__device__ __forceinline__ bool reinit_ready(const ReinitArgs& a, uint32_t cnt, uint32_t epoch, uint32_t i,
+                                             const uint8_t* state, int lo, int hi) {
     const int EXd[8] = {1, 0, -1, 0, 1, -1, -1, 1};
     const int EYd[8] = {0, 1, 0, -1, 1, 1, -1, -1};
     const int x = (int)(i % a.nx), y = (int)(i / a.nx);
+    uint64_t m[8];
+#pragma unroll
     for (int d = 0; d < 8; d++) {
         const int xx = (x + EXd[d] + a.nx) % a.nx, yy = (y + EYd[d] + a.ny) % a.ny;
         const uint32_t j = (uint32_t)yy * a.nx + xx;
-        if (j < i) {
-            const int u = reinit_find(a, cnt, j);
-            if (u >= 0 && (state[u] < lo || state[u] >= hi)) return false;
+        m[d] = j < i ? a.rmark[j] : 0ull;
+    }
+    bool ready = true;
+#pragma unroll
+    for (int d = 0; d < 8; d++) {
+        const uint32_t p = (uint32_t)m[d];
+        if ((uint32_t)(m[d] >> 32) == epoch && p < cnt) {
+            if (hi <= lo) return false;   // a listed predecessor: not ready at this level
+            const int s = state[p];
+            ready = ready && s >= lo && s < hi;
         }
     }
-    return true;
+    return ready;
 }
 
 template <int NS, int P>
 __global__ __launch_bounds__(256) void k_reinit_level(const ReinitArgs a, uint8_t* state, int level, DevStatus* st) {
     if (halted_beside_topology(st)) return;
-    const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
+    const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP), epoch = st->cure_epoch;
     bool any = false;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
         if (state[t] != 0) continue;
         const uint32_t i = a.changed[2 * t];
-        if (!reinit_ready(a, cnt, i, state, 2, 2 + level)) continue;
+        if (!reinit_ready(a, cnt, epoch, i, state, 2, 2 + level)) continue;
         reinit_site<NS, P>(a, (long)i, a.changed[2 * t + 1]);
         state[t] = (uint8_t)(2 + level);
         any = true;
@@ -404,7 +428,7 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gs
     if (halted_beside_topology(st)) return;
     __shared__ uint8_t lstate[REINIT_LDS];
     __shared__ int pending;
-    const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP);
+    const uint32_t cnt = min(st->nchanged, (uint32_t)REINIT_CAP), epoch = st->cure_epoch;
     if (cnt == 0) return;
     const bool in_lds = cnt <= (uint32_t)REINIT_LDS;
     uint8_t* state = in_lds ? lstate : gstate;
@@ -419,7 +443,7 @@ __global__ __launch_bounds__(1024) void k_reinit(const ReinitArgs a, uint8_t* gs
         for (uint32_t t = threadIdx.x; t < cnt; t += blockDim.x) {
             if (state[t] != 0) continue;
             left = true;
-            if (reinit_ready(a, cnt, a.changed[2 * t], state, 2, 256)) state[t] = 1;
+            if (reinit_ready(a, cnt, epoch, a.changed[2 * t], state, 2, 256)) state[t] = 1;
         }
         if (left) pending = 1;
         __syncthreads();

@@ -76,6 +76,7 @@ struct DevStatus {
     uint32_t rm_sparse;        // the round's ranks come from the sorted list (else from the scan)
     uint32_t topo_changed;     // remesh or coarsening changed the topology in this step
     uint32_t novf;             // entries in the bin overflow list
+    uint32_t cure_epoch;       // cures begun (tags the re-initialisation marks of the current one)
 };
 
 // Raised by remeshing for a new node within 2 sites of the edge; reported as
@@ -273,7 +274,7 @@ struct lbib_ctx {
     double* bc_v = nullptr;                    // [4][n][2]
     int32_t* bc_desc = nullptr;                // [4][n]
     uint32_t* changed = nullptr;               // [REINIT_CAP][2]
-    uint32_t* rmark = nullptr;                 // [n] 1 + position in changed (validated on read)
+    uint64_t* rmark = nullptr;                 // [n] (cure epoch << 32) | position in changed
     uint8_t* rstate = nullptr;                 // [REINIT_CAP] level state
     // rings / domains (indexed by domain id)
     int dcap = 0, ndom = 0;
