@@ -79,7 +79,8 @@ __device__ __forceinline__ void marker_force(const int32_t* __restrict__ off, co
 // mostly share one), and entries past the capacity go to the overflow list.
 constexpr int ST = 16;            // spread tile edge
 constexpr int SP_IDX = 1024;      // markers gathered per tile
-constexpr int SP_CH = 128;        // marker data staged per pass
+constexpr int SP_CH = 64;         // marker data staged per pass (one 64-bit site mask)
+constexpr int SP_MW = SP_CH / 64;  // mask words per site
 
 // Clears the bins before a binning pass: tile counts, active flags, the
 // overflow count and the touched-tile shards (a stopped step keeps them).
@@ -214,7 +215,7 @@ struct SpreadLds {
     int32_t idx[SP_IDX];   // gathered, then in rank order
     double fxa[SP_CH][4], fya[SP_CH][4], sfx[SP_CH], sfy[SP_CH];
     int32_t s0x[SP_CH], s0y[SP_CH];
-    unsigned long long smask[2][ST * ST];
+    unsigned long long smask[SP_MW][ST * ST];
     uint32_t hcnt[9], hpre[10];
     int32_t htile[9];
     uint32_t n, novf;
@@ -247,8 +248,8 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
         L.n = 0;
         L.novf = min(st->novf, OVF_CAP);
     }
-    L.smask[0][threadIdx.x] = 0ull;
-    L.smask[1][threadIdx.x] = 0ull;
+#pragma unroll
+    for (int h = 0; h < SP_MW; h++) L.smask[h][threadIdx.x] = 0ull;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = 0;
@@ -329,7 +330,7 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
         }
         __syncthreads();
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < SP_MW; h++) {
             unsigned long long mk = L.smask[h][threadIdx.x];
             while (mk) {
                 const int j = 64 * h + __ffsll((long long)mk) - 1;
@@ -363,7 +364,7 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
                                                      const double* __restrict__ mfx, const double* __restrict__ mfy,
                                                      double* Fx, double* Fy, uint32_t* shard, int begin,
                                                      uint32_t begin_it, DevStatus* st) {
-    static_assert(SP_CH == 128, "two 64-bit site masks per staged chunk");
+    static_assert(SP_CH % 64 == 0, "whole 64-bit site masks per staged chunk");
     if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     const int t = MAP == 0 ? (int)xcd_contiguous(blockIdx.x, gridDim.x) : row_interleaved_tile(blockIdx.x, ntx, nty);
     if (t < 0 || !tact[t] || step_stopped(st)) return;
@@ -378,11 +379,13 @@ __global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, in
 // Lattices stored in FP32 or 16-bit formats do not reproduce the reference's
 // FP64 sums anyway, so their force field is summed without an order: each
 // tile lists every marker whose stencil reaches it (with its position and
-// force), and a workgroup per tile adds the FP32 products delta*F as int64
+// force), and one wavefront per tile adds the FP32 products delta*F as int64
 // multiples of 2^-36 with LDS atomics.  Integer addition is associative, so
 // the result does not depend on the order of the atomics (runs are
 // reproducible); the quantum 2^-36 is far below FP32 resolution of the
-// forces involved.
+// forces involved.  A wavefront per tile (5 KB of LDS) lets 32 tiles reside
+// on a CU at once: the pass over the lattice's tiles is bound by how many
+// are in flight, not by their arithmetic.
 constexpr float kFixScale = 68719476736.0f;    // 2^36
 constexpr double kFixInv = 1.0 / 68719476736.0;
 
@@ -396,7 +399,7 @@ __device__ __forceinline__ float fast_cos_f32(float x) {
 }
 
 template <int MAP>
-__global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, int nty, const uint32_t* __restrict__ tcnt,
+__global__ __launch_bounds__(64) void k_spread_fixed(int nx, int ny, int ntx, int nty, const uint32_t* __restrict__ tcnt,
                                                       const uint32_t* __restrict__ pb_pos, const float2* __restrict__ pb_r,
                                                       const float2* __restrict__ pb_f, const BinOvf* __restrict__ ovf,
                                                       float2* F, uint32_t* shard, int begin, uint32_t begin_it,
@@ -409,9 +412,11 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, i
     if (threadIdx.x == 0) atomicAdd(&shard[(SH_TOUCHED * NSHARD + (blockIdx.x % NSHARD)) * SHARD_STRIDE], 1u);
     __shared__ unsigned long long acc[2][ST * ST];
     __shared__ uint32_t bad[ST * ST];   // a non-finite or out-of-range product reached the site
-    acc[0][threadIdx.x] = 0ull;
-    acc[1][threadIdx.x] = 0ull;
-    bad[threadIdx.x] = 0u;
+    for (int s = threadIdx.x; s < ST * ST; s += blockDim.x) {
+        acc[0][s] = 0ull;
+        acc[1][s] = 0ull;
+        bad[s] = 0u;
+    }
     __syncthreads();
     const int tx = t % ntx, ty = t / ntx;
     const int bx = tx * ST, by = ty * ST;
@@ -458,14 +463,14 @@ __global__ __launch_bounds__(256) void k_spread_fixed(int nx, int ny, int ntx, i
         }
     }
     __syncthreads();
-    const int px = bx + (int)(threadIdx.x % ST), py = by + (int)(threadIdx.x / ST);
-    if (px < nx && py < ny) {
+    for (int s = threadIdx.x; s < ST * ST; s += blockDim.x) {
+        const int px = bx + s % ST, py = by + s / ST;
+        if (px >= nx || py >= ny) continue;
         float2 v;
-        if (bad[threadIdx.x]) {
+        if (bad[s]) {
             v = make_float2(__int_as_float(0x7fc00000), __int_as_float(0x7fc00000));
         } else {
-            v = make_float2((float)((double)(long long)acc[0][threadIdx.x] * kFixInv),
-                            (float)((double)(long long)acc[1][threadIdx.x] * kFixInv));
+            v = make_float2((float)((double)(long long)acc[0][s] * kFixInv), (float)((double)(long long)acc[1][s] * kFixInv));
         }
         F[(long)py * nx + px] = v;
     }
@@ -612,10 +617,10 @@ int launch_force_and_spread(lbib_ctx* c) {
     if (mixed) {
         float2* F = reinterpret_cast<float2*>(c->Fx);
         if (map)
-            k_spread_fixed<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tcnt, c->pb_pos, c->pb_r,
+            k_spread_fixed<1><<<grid, 64, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tcnt, c->pb_pos, c->pb_r,
                                                           c->pb_f, c->ovf, F, c->shard, sbegin, begin_it, c->st);
         else
-            k_spread_fixed<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tcnt, c->pb_pos, c->pb_r,
+            k_spread_fixed<0><<<grid, 64, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tcnt, c->pb_pos, c->pb_r,
                                                           c->pb_f, c->ovf, F, c->shard, sbegin, begin_it, c->st);
     } else if (map) {
         k_tile_spread<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->tcnt, c->tlist, c->tpos,
