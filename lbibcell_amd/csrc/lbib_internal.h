@@ -415,8 +415,6 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode);
 int launch_area_regulator(lbib_ctx* c, double target, double kp, double maxs, double factor);
 int scan_exclusive(lbib_ctx* c, const uint32_t* in, uint32_t* out, const uint32_t* nptr, uint32_t nfix, uint32_t cap,
                    uint32_t* total, const uint32_t* gate = nullptr);
-// exclusive scan of the per-tile marker counts (tcnt -> toff); for short tile
-// arrays also the number of nonzero active flags into *nactive
 int launch_inout(lbib_ctx* c);
 // gated: skip (keeping the current ring order) unless st->topo_changed
 int launch_rings(lbib_ctx* c, bool counters_cleared = false, bool gated = false);

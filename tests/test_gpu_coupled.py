@@ -11,8 +11,8 @@ from lbibcell_amd import capi, runner, workloads
 pytestmark = pytest.mark.gpu
 
 
-def _build(orc, nx, ny, tau, nodes, conns, species, bio, celltypes=None, box=True, bc=""):
-    sim = runner.Simulation(nx, ny, tau)
+def _build(orc, nx, ny, tau, nodes, conns, species, bio, celltypes=None, box=True, bc="", precision=capi.FP64):
+    sim = runner.Simulation(nx, ny, tau, precision=precision)
     o = orc.Oracle(nx, ny, tau)
     for name, kind, t, prm in species:
         sim.add_species(name, capi.species_spec(kind, t, **prm))
