@@ -413,22 +413,43 @@ def measure_adjoint_ms(size, steps, K, ncell=1600, batch=2, snapshots=20):
 LONG_CHECKS = (100, 1000, 5000)
 
 
+# The sensitivity of the coupled model itself: FP64 runs whose initial marker
+# positions differ from the reference run's by at most JITTER_LU lattice units
+# (1e-12: far below any rounding of the path; 1e-8: about what FP32
+# arithmetic moves a marker by in a hundred steps).  Their distance from the
+# FP64 run shows how far the model carries a small difference in the input.
+JITTER_LU = (1e-12, 1e-8)
+
+
+def _jittered(nodes, amp, seed=7):
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-amp, amp, size=(len(nodes), 2))
+    return [(nid, x + d[k, 0], y + d[k, 1]) for k, (nid, x, y) in enumerate(nodes)]
+
+
 def long_window_errors(kind, precisions, steps):
     """L2 errors of every storage precision against the FP64 run over a long
     window, sampled at LONG_CHECKS (<= steps).  kind "tissue": the 2048^2 cells
     of the headline laid out 100 sites from the edges, area regulator KP 1e-6
     and clamp 4e-4 (the headline's own regulator carries the grown tissue into
     the lattice edge in iteration 88, as it does in the reference), box outlet;
-    kind "ensemble": the 256-domain batched ensemble.  Returns {prec: {step:
-    {speed, concentration, cell_area}}} plus the seconds each run took."""
+    kind "ensemble": the 256-domain batched ensemble.  "fp64_jitter<amp>" are
+    the FP64 runs from initial positions moved by at most amp.  Returns
+    {prec: {step: {speed, concentration, cell_area}}} against FP64, the same
+    against FP32 for the 16-bit stores (their storage error alone), plus the
+    seconds each run took."""
     from lbibcell_amd import capi, runner, workloads
     checks = [c for c in LONG_CHECKS if c <= steps]
     fields, secs = {}, {}
-    for p in ["fp64"] + [q for q in precisions if q != "fp64"]:
+    jit = {f"fp64_jitter{a:.0e}": a for a in JITTER_LU}
+    for p in ["fp64"] + list(jit) + [q for q in precisions if q != "fp64"]:
+        prec = "fp64" if p in jit else p
         if kind == "tissue":
             nodes, conns = workloads.hex_tissue(2048, 2048, 20, 20, radius=40.0, rjit=4.0, n=150, pos_jit=2.0,
                                                 seed=1, margin=100.0)
-            sim = runner.Simulation(2048, 2048, TAU_F, precision=PRECISIONS[p])
+            if p in jit:
+                nodes = _jittered(nodes, jit[p])
+            sim = runner.Simulation(2048, 2048, TAU_F, precision=PRECISIONS[prec])
             sim.add_species("tutorial_01_CDESolverD2Q5_SIGNAL",
                             capi.species_spec(capi.SPECIES_SIGNAL, TAU_C, **SIGNAL))
             sim.add_geometry(nodes, conns)
@@ -442,7 +463,9 @@ def long_window_errors(kind, precisions, steps):
             live = None
         else:
             ens = workloads.batched_ensemble(ndom=256, seed=1)
-            sim = build_batched(ens, PRECISIONS[p])
+            if p in jit:
+                ens = dict(ens, nodes=_jittered(ens["nodes"], jit[p]))
+            sim = build_batched(ens, PRECISIONS[prec])
             live = np.zeros((ens["ny"], ens["nx"]), dtype=bool)
             for x0, y0, w, h, *_ in ens["rects"]:
                 live[y0:y0 + h, x0:x0 + w] = True
@@ -461,18 +484,23 @@ def long_window_errors(kind, precisions, steps):
             fields[p][c] = (sp, C, sim.areas())
         secs[p] = time.perf_counter() - t0
         sim.close()
-    out = {}
-    for p in fields:
-        if p == "fp64":
-            continue
-        out[p] = {}
-        for c in checks:
-            sp, C, a = fields[p][c]
-            sr, Cr, ar = fields["fp64"][c]
-            common = sorted(set(a) & set(ar))
-            out[p][str(c)] = {"speed": l2_rel(sp, sr), "concentration": l2_rel(C, Cr),
-                              "cell_area": l2_rel([a[d] for d in common], [ar[d] for d in common])}
-    return {"errors_vs_fp64": out, "seconds": secs, "steps": steps}
+    def errors(ref, skip):
+        out = {}
+        for p in fields:
+            if p in skip:
+                continue
+            out[p] = {}
+            for c in checks:
+                sp, C, a = fields[p][c]
+                sr, Cr, ar = fields[ref][c]
+                common = sorted(set(a) & set(ar))
+                out[p][str(c)] = {"speed": l2_rel(sp, sr), "concentration": l2_rel(C, Cr),
+                                  "cell_area": l2_rel([a[d] for d in common], [ar[d] for d in common])}
+        return out
+    res = {"errors_vs_fp64": errors("fp64", {"fp64"}), "seconds": secs, "steps": steps}
+    if "fp32" in fields:
+        res["errors_vs_fp32"] = errors("fp32", {"fp64", "fp32"} | set(jit))
+    return res
 
 
 def l2_rel(a, b):
