@@ -63,16 +63,17 @@ __global__ __launch_bounds__(256) void k_stream_collide_f64(const LatticeArgs a)
     const double tauI = fr.r < 0 ? a.tauI : a.rects[fr.r].tauI;
     const double tau = fr.r < 0 ? a.tau : a.rects[fr.r].tau;
 
+    // every population is read once per step: non-temporal loads
     double f[9];
-    f[DT] = a.fs[i];
-    f[DE] = a.fs[1 * n + r0 + xm];
-    f[DN] = a.fs[2 * n + rS + x];
-    f[DW] = a.fs[3 * n + r0 + xp];
-    f[DS] = a.fs[4 * n + rN + x];
-    f[DNE] = a.fs[5 * n + rS + xm];
-    f[DNW] = a.fs[6 * n + rS + xp];
-    f[DSW] = a.fs[7 * n + rN + xp];
-    f[DSE] = a.fs[8 * n + rN + xm];
+    f[DT] = __builtin_nontemporal_load(&a.fs[i]);
+    f[DE] = __builtin_nontemporal_load(&a.fs[1 * n + r0 + xm]);
+    f[DN] = __builtin_nontemporal_load(&a.fs[2 * n + rS + x]);
+    f[DW] = __builtin_nontemporal_load(&a.fs[3 * n + r0 + xp]);
+    f[DS] = __builtin_nontemporal_load(&a.fs[4 * n + rN + x]);
+    f[DNE] = __builtin_nontemporal_load(&a.fs[5 * n + rS + xm]);
+    f[DNW] = __builtin_nontemporal_load(&a.fs[6 * n + rS + xp]);
+    f[DSW] = __builtin_nontemporal_load(&a.fs[7 * n + rN + xp]);
+    f[DSE] = __builtin_nontemporal_load(&a.fs[8 * n + rN + xm]);
 
     const uint32_t tg = a.tag[i];
     const uint32_t dom = tg & DOM_MASK;
@@ -97,11 +98,11 @@ __global__ __launch_bounds__(256) void k_stream_collide_f64(const LatticeArgs a)
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const double* gs = a.gs + s * sn;
-        g[s][DT] = gs[i];
-        g[s][DE] = gs[1 * n + r0 + xm];
-        g[s][DN] = gs[2 * n + rS + x];
-        g[s][DW] = gs[3 * n + r0 + xp];
-        g[s][DS] = gs[4 * n + rN + x];
+        g[s][DT] = __builtin_nontemporal_load(&gs[i]);
+        g[s][DE] = __builtin_nontemporal_load(&gs[1 * n + r0 + xm]);
+        g[s][DN] = __builtin_nontemporal_load(&gs[2 * n + rS + x]);
+        g[s][DW] = __builtin_nontemporal_load(&gs[3 * n + r0 + xp]);
+        g[s][DS] = __builtin_nontemporal_load(&gs[4 * n + rN + x]);
     }
     if (NS > 0 && (tg >> BC_SHIFT)) {
         // Cut links carrying BoundarySolverNoFluxD2Q5: slot o holds a boundary

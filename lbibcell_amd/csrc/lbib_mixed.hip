@@ -64,23 +64,23 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
         const long rS = (long)((y == fr[k].y0) ? fr[k].y0 + fr[k].h - 1 : y - 1) * a.nx;
         const long rN = (long)((y == fr[k].y0 + fr[k].h - 1) ? fr[k].y0 : y + 1) * a.nx;
         const long i = r0 + xx;
-        // pulled deviations d_q = f_q - w_q
-        d[k][DT] = C::dec(fs[pidx<P>(DT, i, n, 9)]);
-        d[k][DE] = C::dec(fs[pidx<P>(DE, r0 + xm, n, 9)]);
-        d[k][DN] = C::dec(fs[pidx<P>(DN, rS + xx, n, 9)]);
-        d[k][DW] = C::dec(fs[pidx<P>(DW, r0 + xp, n, 9)]);
-        d[k][DS] = C::dec(fs[pidx<P>(DS, rN + xx, n, 9)]);
-        d[k][DNE] = C::dec(fs[pidx<P>(DNE, rS + xm, n, 9)]);
-        d[k][DNW] = C::dec(fs[pidx<P>(DNW, rS + xp, n, 9)]);
-        d[k][DSW] = C::dec(fs[pidx<P>(DSW, rN + xp, n, 9)]);
-        d[k][DSE] = C::dec(fs[pidx<P>(DSE, rN + xm, n, 9)]);
+        // pulled deviations d_q = f_q - w_q (each read once per step: non-temporal)
+        d[k][DT] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DT, i, n, 9)]));
+        d[k][DE] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DE, r0 + xm, n, 9)]));
+        d[k][DN] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DN, rS + xx, n, 9)]));
+        d[k][DW] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DW, r0 + xp, n, 9)]));
+        d[k][DS] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DS, rN + xx, n, 9)]));
+        d[k][DNE] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DNE, rS + xm, n, 9)]));
+        d[k][DNW] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DNW, rS + xp, n, 9)]));
+        d[k][DSW] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DSW, rN + xp, n, 9)]));
+        d[k][DSE] = C::dec(__builtin_nontemporal_load(&fs[pidx<P>(DSE, rN + xm, n, 9)]));
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            g[k][s][DT] = C::dec(gsb[pidx<P>(5 * s + DT, i, n, GQ)]);
-            g[k][s][DE] = C::dec(gsb[pidx<P>(5 * s + DE, r0 + xm, n, GQ)]);
-            g[k][s][DN] = C::dec(gsb[pidx<P>(5 * s + DN, rS + xx, n, GQ)]);
-            g[k][s][DW] = C::dec(gsb[pidx<P>(5 * s + DW, r0 + xp, n, GQ)]);
-            g[k][s][DS] = C::dec(gsb[pidx<P>(5 * s + DS, rN + xx, n, GQ)]);
+            g[k][s][DT] = C::dec(__builtin_nontemporal_load(&gsb[pidx<P>(5 * s + DT, i, n, GQ)]));
+            g[k][s][DE] = C::dec(__builtin_nontemporal_load(&gsb[pidx<P>(5 * s + DE, r0 + xm, n, GQ)]));
+            g[k][s][DN] = C::dec(__builtin_nontemporal_load(&gsb[pidx<P>(5 * s + DN, rS + xx, n, GQ)]));
+            g[k][s][DW] = C::dec(__builtin_nontemporal_load(&gsb[pidx<P>(5 * s + DW, r0 + xp, n, GQ)]));
+            g[k][s][DS] = C::dec(__builtin_nontemporal_load(&gsb[pidx<P>(5 * s + DS, rN + xx, n, GQ)]));
         }
         tg[k] = a.tag[i];
         Fx[k] = 0.0f;
@@ -215,7 +215,10 @@ struct PairLoad {
     bool lo_edge, hi_edge;
     __device__ long at(int x) const { return pidx<P>(q, r + x, n, nq); }
     // the pair (x0, x0 + 1) is contiguous: x0 and the row start are even
-    __device__ uint32_t aligned() const { return *reinterpret_cast<const uint32_t*>(base + at(x0)); }
+    // (each population is read once per step: non-temporal load)
+    __device__ uint32_t aligned() const {
+        return __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(base + at(x0)));
+    }
     // values at x0-1, x0 (pull from the west)
     __device__ void west(float& a0, float& a1) const {
         const uint32_t own = aligned();
@@ -422,17 +425,18 @@ struct LaneVec {
     static constexpr int NW = V * (int)sizeof(T) / 4;   // dwords per lane
     uint32_t w[NW];
     uint32_t edge;   // storage bits of the site shifted in from outside the lane
+    // every population is read once per step: non-temporal loads
     __device__ void load(const T* p) {
         const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
         if constexpr (NW % 4 == 0) {
 #pragma unroll
             for (int k = 0; k < NW; k += 4) {
-                const uint4 v = *reinterpret_cast<const uint4*>(q + k);
+                const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(q + k));
                 w[k] = v.x; w[k + 1] = v.y; w[k + 2] = v.z; w[k + 3] = v.w;
             }
         } else {
             static_assert(NW == 2, "lane vectors of 8 or a multiple of 16 bytes");
-            const uint2 v = *reinterpret_cast<const uint2*>(q);
+            const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(q));
             w[0] = v.x; w[1] = v.y;
         }
     }

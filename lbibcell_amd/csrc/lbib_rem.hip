@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_rem(const MixedArgs a) {
         mo[q][0] = M[src[q]];
         mo[q][1] = M[n + src[q]];
         mo[q][2] = M[2 * n + src[q]];
-        rf[q] = R[(long)q * n + src[q]];
+        rf[q] = __builtin_nontemporal_load(&R[(long)q * n + src[q]]);   // read once per step
     }
     float cs[NSS][5];
     uint16_t rs[NSS][5];
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_rem(const MixedArgs a) {
 #pragma unroll
         for (int q = 0; q < 5; q++) {
             cs[s][q] = Cg[(long)s * n + src[q]];
-            rs[s][q] = Rg[(long)(5 * s + q) * n + src[q]];
+            rs[s][q] = __builtin_nontemporal_load(&Rg[(long)(5 * s + q) * n + src[q]]);
         }
     const uint32_t tg = a.tag[i];
     float Fx = 0.0f, Fy = 0.0f;
