@@ -137,6 +137,7 @@ int ensure_connections(lbib_ctx* c, int nc) {
     // room for 8 candidates per connection plus slack for long first segments.
     const int candcap = cap * 8 + 65536;
     if (regrow(c, &c->cand, c->cand ? (size_t)c->candcap : 0, candcap)) return -1;
+    if (c->cand_v && regrow(c, &c->cand_v, (size_t)c->candcap, candcap)) return -1;
     c->candcap = candcap;
     c->ccap = cap;
     return ensure_scan(c);
@@ -318,9 +319,12 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
     lbib_ctx* c = new (std::nothrow) lbib_ctx();
     if (!c) return -1;
     c->cfg = *cfg;
-    if (cfg->nx < 4 || cfg->ny < 4 || cfg->nx > 65535 || cfg->ny > 65535 || cfg->nspecies < 0 ||
-        cfg->nspecies > LBIB_MAX_SPECIES || !(cfg->tau_fluid > 0.5)) {
-        c->err = "invalid configuration (4 <= nx, ny <= 65535, 0 <= nspecies <= 4, tau_fluid > 0.5)";
+    // the cut-link selection entries index (site, slot) pairs in 32 bits
+    if (cfg->nx < 4 || cfg->ny < 4 || cfg->nx > 65535 || cfg->ny > 65535 ||
+        (long)cfg->nx * (long)cfg->ny > (1L << 30) || cfg->nspecies < 0 || cfg->nspecies > LBIB_MAX_SPECIES ||
+        !(cfg->tau_fluid > 0.5)) {
+        c->err = "invalid configuration (4 <= nx, ny <= 65535, nx * ny <= 2^30, 0 <= nspecies <= 4, "
+                 "tau_fluid > 0.5)";
         *out = c;
         return -1;
     }
@@ -400,9 +404,7 @@ int lbib_create(const lbib_config* cfg, lbib_ctx** out) {
             (dev_alloc(c, &c->pb_pos, nb) || dev_alloc(c, &c->pb_r, nb) || dev_alloc(c, &c->pb_f, nb)))
             return -1;
     }
-    if (dev_alloc(c, &c->best_dist, 4 * n, 0xff) || dev_alloc(c, &c->best_seq, 4 * n, 0xff) ||
-        dev_alloc(c, &c->bn_dom, 4 * n))
-        return -1;
+    if (dev_alloc(c, &c->best, 4 * n, 0xff) || dev_alloc(c, &c->bn_dom, 4 * n)) return -1;
     if (dev_alloc(c, &c->changed, 2 * (size_t)REINIT_CAP) || dev_alloc(c, &c->rmark, (size_t)c->n) ||
         dev_alloc(c, &c->rstate, (size_t)REINIT_CAP))
         return -1;
@@ -465,7 +467,7 @@ void lbib_destroy(lbib_ctx* c) {
     dev_free(c->fs_prm); dev_free(c->bcd_off); dev_free(c->bcd_sp);
     dev_free(c->j_part); dev_free(c->j_src); dev_free(c->j_off); dev_free(c->j_in); dev_free(c->j_cnt);
     dev_free(c->j_ofs); dev_free(c->j_total); dev_free(c->j_type); dev_free(c->j_partner); dev_free(c->j_prm);
-    dev_free(c->cand); dev_free(c->best_dist); dev_free(c->best_seq); dev_free(c->bn_dom); dev_free(c->bc_v);
+    dev_free(c->cand); dev_free(c->cand_v); dev_free(c->best); dev_free(c->bn_dom); dev_free(c->bc_v);
     dev_free(c->bc_desc); dev_free(c->changed); dev_free(c->rmark); dev_free(c->rstate);
     dev_free(c->ring_start); dev_free(c->ring_cnt); dev_free(c->ring_off_u); dev_free(c->area);
     dev_free(c->celltype); dev_free(c->target); dev_free(c->mtab); dev_free(c->st);
@@ -918,7 +920,9 @@ int lbib_set_bc_descriptors(lbib_ctx* c, int nd, const int32_t* off, const int32
         return -1;
     c->nbcd = nd;
     if (nd > 0 && !c->bc_v) {
-        if (dev_alloc(c, &c->bc_v, 8 * c->n) || dev_alloc(c, &c->bc_desc, 4 * c->n)) return -1;
+        if (dev_alloc(c, &c->bc_v, 8 * c->n) || dev_alloc(c, &c->bc_desc, 4 * c->n) ||
+            dev_alloc(c, &c->cand_v, (size_t)c->candcap))
+            return -1;
     }
     c->any_bc = nd > 0;
     LBIB_HIP(hipStreamSynchronize(c->stream));

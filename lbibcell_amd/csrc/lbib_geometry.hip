@@ -21,9 +21,9 @@ namespace lbib {
 __device__ __forceinline__ bool halted(const DevStatus* st) { return (st->bits & STOP_BITS) != 0u; }
 
 // closest-wins selection entries, site-major: the four slots of a site share
-// one 32-byte (distance) / 16-byte (sequence) span
+// one 64-byte span
 __device__ __forceinline__ long sel_index(int site, int slot) { return (long)site * 4 + (slot - 1); }
-// best_seq of a selection entry where two candidates met at the same distance
+// the seq of a selection entry where two candidates met at the same distance
 // (above every insertion number, below the cleared value)
 constexpr uint32_t SEQ_TIE = 0xfffffffeu;
 
@@ -35,8 +35,7 @@ constexpr uint32_t SEQ_TIE = 0xfffffffeu;
 // these to be rerun, which repeats them unchanged.
 __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ ca, const int32_t* __restrict__ cb,
                                                    const double* __restrict__ mx, const double* __restrict__ my,
-                                                   const Cand* __restrict__ cand, uint32_t* tag,
-                                                   unsigned long long* best_dist, uint32_t* best_seq,
+                                                   const Cand* __restrict__ cand, uint32_t* tag, SelEntry* best,
                                                    uint32_t* shard, DevStatus* st) {
     if (halted(st)) return;
     const uint32_t nc = st->nc, nprev = st->ncand_prev;
@@ -47,13 +46,11 @@ __global__ __launch_bounds__(256) void k_geo_begin(const int32_t* __restrict__ c
             atomicOr(&st->bits, LBIB_HALT_PERTURB_CONN);
     }
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nprev; j += stride) {
-        const int site = cand[j].site, slot = cand[j].slot;
+        const uint32_t ent = cand[j].ent, site = ent >> 2;
         // every thread touching a site stores the same value: no atomic
         const uint32_t t0 = tag[site];
         if (t0 & ~DOM_MASK) tag[site] = t0 & DOM_MASK;
-        const long b = sel_index(site, slot);
-        best_dist[b] = ~0ull;
-        best_seq[b] = ~0u;
+        best[ent] = SelEntry{~0ull, ~0u, ~0u};
     }
     if (blockIdx.x == 0 && threadIdx.x < NSHARD) shard[(SH_NBN * NSHARD + threadIdx.x) * SHARD_STRIDE] = 0u;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -82,8 +79,8 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
                           const int32_t* __restrict__ cb, const uint32_t* __restrict__ cdom,
                           const int32_t* __restrict__ cbc, const double* __restrict__ mx,
                           const double* __restrict__ my, const double* __restrict__ mvx,
-                          const double* __restrict__ mvy, int want_bc, Cand* cand,
-                          int candcap, unsigned long long* best_dist, uint32_t* best_seq, DevStatus* st) {
+                          const double* __restrict__ mvy, int want_bc, Cand* cand, CandV* cand_v,
+                          int candcap, SelEntry* best, DevStatus* st) {
     if (halted(st)) return;
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = c < (int)st->nc;
@@ -125,29 +122,22 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
             atomicOr(&st->bits, LBIB_ERR_EDGE);
             sx = 0; sy = 0;
         }
-        e.site = sy * nx + sx;
-        e.slot = slot;
+        const long site = (long)sy * nx + sx;
+        e.ent = (uint32_t)sel_index((int)site, slot);
         e.seq = (uint32_t)c * SEQ_PER_CONN + k;
         e.domain = d;
         e.dist2 = sqd((double)sx, (double)sy, bx, by);
-        e.bx = bx;
-        e.by = by;
         e.desc = desc;
-        e.vx = 0.0;
-        e.vy = 0.0;
+        cand[base + k] = e;
         if (desc >= 0) {
             const double l = sqrt(sqd(x1, y1, bx, by));
             const double L = sqrt(sqd(x2, y2, x1, y1));
-            e.vx = (vxb - vxa) * l / L + vxa;
-            e.vy = (vyb - vya) * l / L + vya;
+            cand_v[base + k] = CandV{(vxb - vxa) * l / L + vxa, (vyb - vya) * l / L + vya};
         }
-        e.win = 0;
-        cand[base + k] = e;
-        const long b = sel_index(e.site, slot);
         const unsigned long long key = (unsigned long long)__double_as_longlong(e.dist2);
         // a candidate meeting an equal distance already in place marks the
         // entry: only such entries need the insertion-number pass
-        if (atomicMin(&best_dist[b], key) == key) best_seq[b] = SEQ_TIE;
+        if (atomicMin(&best[e.ent].dist, key) == key) best[e.ent].seq = SEQ_TIE;
         k++;
     };
     // vertical lattice lines: node S at (i, floor y) [its slot N], node N at (i, ceil y) [slot S]
@@ -166,16 +156,14 @@ __global__ __launch_bounds__(1024) void k_cut_gen(int nx, int ny, long n, const 
     }
 }
 
-__global__ void k_cut_tie(const Cand* __restrict__ cand, long n, const unsigned long long* __restrict__ best_dist,
-                          uint32_t* best_seq, DevStatus* st) {
+__global__ void k_cut_tie(const Cand* __restrict__ cand, long n, SelEntry* best, DevStatus* st) {
     if (halted_beside_topology(st)) return;
     const uint32_t nc = st->ncand;
     if (blockIdx.x == 0 && threadIdx.x == 0) st->ncand_prev = nc;   // this cure's reset list
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nc; j += gridDim.x * blockDim.x) {
         const Cand e = cand[j];
-        const long b = sel_index(e.site, e.slot);
-        if (best_seq[b] != ~0u && (unsigned long long)__double_as_longlong(e.dist2) == best_dist[b])
-            atomicMin(&best_seq[b], e.seq);
+        SelEntry& b = best[e.ent];
+        if (b.seq != ~0u && (unsigned long long)__double_as_longlong(e.dist2) == b.dist) atomicMin(&b.seq, e.seq);
     }
 }
 
@@ -198,43 +186,43 @@ constexpr int WIN_WG = 1024, WIN_PER_WG = 2 * WIN_WG;
 struct WinOut {
     uint32_t won, add, site, dom;
 };
-__device__ __forceinline__ WinOut cut_winner(const Cand* cand, uint32_t j, uint32_t nc, long n,
-                                             const unsigned long long* __restrict__ best_dist,
-                                             const uint32_t* __restrict__ best_seq, uint32_t* tag, uint32_t* bn_dom,
+__device__ __forceinline__ WinOut cut_winner(const Cand* cand, const CandV* cand_v, uint32_t j, uint32_t nc, long n,
+                                             const SelEntry* __restrict__ best, uint32_t* tag, uint32_t* bn_dom,
                                              double* bc_v, int32_t* bc_desc, int local_update) {
     WinOut o{0u, 0u, 0u, 0u};
     if (j >= nc) return o;
-    const Cand& e = cand[j];
-    const long slotix = (long)(e.slot - 1) * n + e.site;
-    const long b = sel_index(e.site, e.slot);
-    const uint32_t dom_now = tag[e.site] & DOM_MASK;   // domain bits are not changed here
-    const uint32_t bs = best_seq[b];
-    if (bs == ~0u ? (unsigned long long)__double_as_longlong(e.dist2) == best_dist[b] : bs == e.seq) {
+    const Cand e = cand[j];
+    const uint32_t site = e.ent >> 2;
+    const int slot = (int)(e.ent & 3u) + 1;
+    const long slotix = (long)(slot - 1) * n + site;
+    const uint32_t dom_now = tag[site] & DOM_MASK;   // domain bits are not changed here
+    const SelEntry b = best[e.ent];
+    if (b.seq == ~0u ? (unsigned long long)__double_as_longlong(e.dist2) == b.dist : b.seq == e.seq) {
         o.won = 1;
-        o.site = (uint32_t)e.site;
+        o.site = site;
         o.dom = e.domain;
         bn_dom[slotix] = e.domain;
-        uint32_t bits = 1u << (BN_SHIFT + e.slot - 1);
+        uint32_t bits = 1u << (BN_SHIFT + slot - 1);
         if (e.desc >= 0 && bc_v != nullptr) {
-            bits |= 1u << (BC_SHIFT + e.slot - 1);
-            bc_v[2 * slotix] = e.vx;
-            bc_v[2 * slotix + 1] = e.vy;
+            bits |= 1u << (BC_SHIFT + slot - 1);
+            const CandV v = cand_v[j];
+            bc_v[2 * slotix] = v.vx;
+            bc_v[2 * slotix + 1] = v.vy;
             bc_desc[slotix] = e.desc;
         }
-        atomicOr(&tag[e.site], bits);
+        atomicOr(&tag[site], bits);
         if (local_update && dom_now != e.domain) {
             bool highest = true;
-            for (int s2 = e.slot + 1; s2 <= 4; s2++)
-                if (best_dist[sel_index(e.site, s2)] != ~0ull) highest = false;
+            for (int s2 = slot + 1; s2 <= 4; s2++)
+                if (best[sel_index((int)site, s2)].dist != ~0ull) highest = false;
             o.add = highest ? 1u : 0u;
         }
     }
     return o;
 }
 
-__global__ __launch_bounds__(WIN_WG) void k_cut_win(const Cand* cand, long n,
-                                                    const unsigned long long* __restrict__ best_dist,
-                                                    const uint32_t* __restrict__ best_seq, uint32_t* tag,
+__global__ __launch_bounds__(WIN_WG) void k_cut_win(const Cand* cand, const CandV* cand_v, long n,
+                                                    const SelEntry* __restrict__ best, uint32_t* tag,
                                                     uint32_t* bn_dom, double* bc_v, int32_t* bc_desc,
                                                     uint32_t* changed, uint64_t* rmark, uint8_t* rstate,
                                                     uint32_t* shard, int local_update, DevStatus* st) {
@@ -242,10 +230,10 @@ __global__ __launch_bounds__(WIN_WG) void k_cut_win(const Cand* cand, long n,
     const uint32_t nc = st->ncand, epoch = st->cure_epoch;
     for (uint32_t j0 = blockIdx.x * (uint32_t)WIN_PER_WG; j0 < nc; j0 += gridDim.x * (uint32_t)WIN_PER_WG) {
         // both candidates' loads in flight together
-        const WinOut a = cut_winner(cand, j0 + threadIdx.x, nc, n, best_dist, best_seq, tag, bn_dom, bc_v, bc_desc,
+        const WinOut a = cut_winner(cand, cand_v, j0 + threadIdx.x, nc, n, best, tag, bn_dom, bc_v, bc_desc,
                                     local_update);
-        const WinOut b = cut_winner(cand, j0 + WIN_WG + threadIdx.x, nc, n, best_dist, best_seq, tag, bn_dom, bc_v,
-                                    bc_desc, local_update);
+        const WinOut b = cut_winner(cand, cand_v, j0 + WIN_WG + threadIdx.x, nc, n, best, tag, bn_dom, bc_v, bc_desc,
+                                    local_update);
         const uint32_t nw = (uint32_t)__popcll(__ballot(a.won != 0u)) + (uint32_t)__popcll(__ballot(b.won != 0u));
         if ((threadIdx.x & 63) == 0 && nw)
             atomicAdd(&shard[(SH_NBN * NSHARD + ((blockIdx.x * (WIN_WG / 64) + threadIdx.x / 64) % NSHARD)) *
@@ -568,14 +556,12 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
     const int gs = 1024;
     // connection counts are device-resident (remeshing changes them): grids
     // cover the capacity and the kernels read st->nc
-    k_geo_begin<<<gs, bs, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->cand, c->tag, c->best_dist, c->best_seq,
-                                          c->shard, c->st);
+    k_geo_begin<<<gs, bs, 0, c->stream>>>(c->ca, c->cb, c->mx, c->my, c->cand, c->tag, c->best, c->shard, c->st);
     // 1024-wide workgroups: one candidate-counter atomic per 1024 connections
     if (c->ccap > 0)
         k_cut_gen<<<(c->ccap + 1023) / 1024, 1024, 0, c->stream>>>(c->nx, c->ny, c->n, c->ca, c->cb, c->cdom, c->cbc,
                                                                c->mx, c->my, c->mvx, c->mvy, c->bc_v ? 1 : 0,
-                                                               c->cand, c->candcap, c->best_dist, c->best_seq,
-                                                               c->st);
+                                                               c->cand, c->cand_v, c->candcap, c->best, c->st);
     // From here on nothing of this stage reads the connections or markers:
     // the topology stage (remesh, rings, coarsening), which reads and
     // rewrites them, runs beside the rest on the second stream.
@@ -589,12 +575,12 @@ int launch_geometry(lbib_ctx* c, uint32_t it, int mode) {
         c->stream = main;
         if (r) return -1;
     }
-    k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->st);
+    k_cut_tie<<<gs, bs, 0, c->stream>>>(c->cand, c->n, c->best, c->st);
     // enough workgroups for about twice the connections' candidates (the
     // count is device-resident; the kernel loops over more)
     const int wgrid = (int)std::min<long>(std::max<long>((2L * std::max(c->nc, 1) + WIN_PER_WG - 1) / WIN_PER_WG, 64),
                                           1024);
-    k_cut_win<<<wgrid, WIN_WG, 0, c->stream>>>(c->cand, c->n, c->best_dist, c->best_seq, c->tag, c->bn_dom, c->bc_v,
+    k_cut_win<<<wgrid, WIN_WG, 0, c->stream>>>(c->cand, c->cand_v, c->n, c->best, c->tag, c->bn_dom, c->bc_v,
                                                c->bc_desc, c->changed, c->rmark, c->rstate, c->shard, cure ? 0 : 1,
                                                c->st);
     if (!cure) {

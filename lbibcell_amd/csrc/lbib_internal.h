@@ -148,16 +148,28 @@ struct BinOvf {
     float rx, ry, fx, fy;
 };
 
+// A cut-link candidate: its distance, its (site, slot) selection entry
+// site * 4 + slot - 1, its global insertion number, the domain it carries and
+// the connection's boundary descriptor (-1: none; the boundary-node velocity
+// of a candidate with a descriptor is kept beside, in CandV).
 struct Cand {
-    int32_t site;
-    int32_t slot;       // boundary slot of the site: E,N,W,S (1..4)
+    double dist2;
+    uint32_t ent;
     uint32_t seq;       // global insertion order: conn * SEQ_PER_CONN + k
     uint32_t domain;
-    double dist2;
-    double bx, by;      // boundary-node position
-    double vx, vy;      // boundary-node velocity (connections with a boundary condition)
-    int32_t desc;       // boundary-condition descriptor of the connection (-1: none)
-    int32_t win;
+    int32_t desc;
+};
+struct CandV {
+    double vx, vy;
+};
+// Closest-wins selection entry of one (site, slot): the minimum distance (the
+// bits of the double) and the minimum insertion number among candidates at
+// that distance (kept only where two met at it); one 16-byte record, so a
+// site's four slots share a 64-byte span.
+struct SelEntry {
+    unsigned long long dist;
+    uint32_t seq;
+    uint32_t pad;
 };
 
 }  // namespace lbib
@@ -268,8 +280,8 @@ struct lbib_ctx {
     // cut links
     int candcap = 0;
     lbib::Cand* cand = nullptr;
-    unsigned long long* best_dist = nullptr;   // [n][4] (site-major)
-    uint32_t* best_seq = nullptr;              // [n][4]
+    lbib::CandV* cand_v = nullptr;             // [candcap] (connections with a boundary condition)
+    lbib::SelEntry* best = nullptr;            // [n][4] (site-major)
     uint32_t* bn_dom = nullptr;                // [4][n]
     double* bc_v = nullptr;                    // [4][n][2]
     int32_t* bc_desc = nullptr;                // [4][n]
