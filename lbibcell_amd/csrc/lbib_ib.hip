@@ -81,6 +81,8 @@ constexpr int ST = 16;            // spread tile edge
 constexpr int SP_IDX = 1024;      // markers gathered per tile
 constexpr int SP_CH = 64;         // marker data staged per pass (one 64-bit site mask)
 constexpr int SP_MW = SP_CH / 64;  // mask words per site
+constexpr int SPW = 128;           // workgroup width of the ordered spread
+constexpr int SPS = ST * ST / SPW; // sites of the tile per lane
 
 // Clears the bins before a binning pass: tile counts, active flags, the
 // overflow count and the touched-tile shards (a stopped step keeps them).
@@ -224,10 +226,12 @@ struct SpreadLds {
 // One tile of the ordered spread.  The gathered markers are ranked by index
 // and staged SP_CH at a time with their eight delta factors; each staged
 // marker sets its bit in a per-site mask of the sites its stencil covers in
-// this tile, and every lane then walks only the set bits of its site's mask,
-// lowest first, so it adds delta*F of exactly the covering markers in rank
-// order.  Called by the whole workgroup with a workgroup-uniform tile;
-// returns false when the tile's marker count exceeds SP_IDX.
+// this tile, and every lane then walks only the set bits of its sites' masks
+// (SPS sites per lane), lowest first, so it adds delta*F of exactly the
+// covering markers in rank order.  Called by the whole workgroup of SPW lanes
+// with a workgroup-uniform tile; returns false when the tile's marker count
+// exceeds SP_IDX.  A 128-lane workgroup with 10 KB of LDS lets 16 tiles
+// reside per CU.
 __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny, int ntx, int nty,
                                             const uint32_t* __restrict__ tcnt, const int32_t* __restrict__ tlist,
                                             const uint32_t* __restrict__ tpos, const BinOvf* __restrict__ ovf,
@@ -249,7 +253,9 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
         L.novf = min(st->novf, OVF_CAP);
     }
 #pragma unroll
-    for (int h = 0; h < SP_MW; h++) L.smask[h][threadIdx.x] = 0ull;
+    for (int u = 0; u < SPS; u++)
+#pragma unroll
+        for (int h = 0; h < SP_MW; h++) L.smask[h][threadIdx.x + SPW * u] = 0ull;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t a = 0;
@@ -286,14 +292,14 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
     __syncthreads();
     const uint32_t m = L.n;
     if (m > (uint32_t)SP_IDX) return false;
-    // rank sort in place: ranks into registers, then one scatter (the
-    // workgroup is 256 wide, so each lane ranks at most SP_IDX / 256 markers)
-    constexpr int RPL = SP_IDX / 256;
+    // rank sort in place: ranks into registers, then one scatter (each lane
+    // ranks at most SP_IDX / SPW markers)
+    constexpr int RPL = SP_IDX / SPW;
     int32_t rv[RPL];
     uint32_t rr[RPL];
 #pragma unroll
     for (int q = 0; q < RPL; q++) {
-        const uint32_t j = threadIdx.x + 256u * q;
+        const uint32_t j = threadIdx.x + (uint32_t)SPW * q;
         rv[q] = j < m ? L.idx[j] : 0;
         uint32_t r = 0;
         if (j < m)
@@ -303,11 +309,15 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < RPL; q++)
-        if (threadIdx.x + 256u * q < m) L.idx[rr[q]] = rv[q];
+        if (threadIdx.x + (uint32_t)SPW * q < m) L.idx[rr[q]] = rv[q];
     __syncthreads();
-    const int lx = (int)(threadIdx.x % ST), ly = (int)(threadIdx.x / ST);
-    const int px = bx + lx, py = by + ly;
-    double ax = 0.0, ay = 0.0;
+    // each lane sums SPS sites of the tile (independent chains)
+    double ax[SPS], ay[SPS];
+#pragma unroll
+    for (int u = 0; u < SPS; u++) {
+        ax[u] = 0.0;
+        ay[u] = 0.0;
+    }
     for (uint32_t c0 = 0; c0 < m; c0 += SP_CH) {
         const uint32_t mc = min((uint32_t)SP_CH, m - c0);
         for (uint32_t j = threadIdx.x; j < mc; j += blockDim.x) {
@@ -330,23 +340,33 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
         }
         __syncthreads();
 #pragma unroll
-        for (int h = 0; h < SP_MW; h++) {
-            unsigned long long mk = L.smask[h][threadIdx.x];
-            while (mk) {
-                const int j = 64 * h + __ffsll((long long)mk) - 1;
-                mk &= mk - 1ull;
-                const double dd = (L.fxa[j][px - L.s0x[j]] * L.fya[j][py - L.s0y[j]]) / 16.0;
-                ax += dd * L.sfx[j];
-                ay += dd * L.sfy[j];
+        for (int u = 0; u < SPS; u++) {
+            const int sl = (int)threadIdx.x + SPW * u;
+            const int px = bx + sl % ST, py = by + sl / ST;
+#pragma unroll
+            for (int h = 0; h < SP_MW; h++) {
+                unsigned long long mk = L.smask[h][sl];
+                while (mk) {
+                    const int j = 64 * h + __ffsll((long long)mk) - 1;
+                    mk &= mk - 1ull;
+                    const double dd = (L.fxa[j][px - L.s0x[j]] * L.fya[j][py - L.s0y[j]]) / 16.0;
+                    ax[u] += dd * L.sfx[j];
+                    ay[u] += dd * L.sfy[j];
+                }
+                L.smask[h][sl] = 0ull;
             }
-            L.smask[h][threadIdx.x] = 0ull;
         }
         __syncthreads();
     }
-    if (px < nx && py < ny) {
-        const long site = (long)py * nx + px;
-        Fx[site] = ax;
-        Fy[site] = ay;
+#pragma unroll
+    for (int u = 0; u < SPS; u++) {
+        const int sl = (int)threadIdx.x + SPW * u;
+        const int px = bx + sl % ST, py = by + sl / ST;
+        if (px < nx && py < ny) {
+            const long site = (long)py * nx + px;
+            Fx[site] = ax[u];
+            Fy[site] = ay[u];
+        }
     }
     return true;
 }
@@ -357,7 +377,7 @@ __device__ __forceinline__ bool spread_tile(SpreadLds& L, int t, int nx, int ny,
 // (row_interleaved_tile), for a batched atlas whose occupied rectangles would
 // otherwise load some XCDs far more than others.
 template <int MAP>
-__global__ __launch_bounds__(256) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
+__global__ __launch_bounds__(SPW) void k_tile_spread(int nx, int ny, int ntx, int nty, const uint8_t* __restrict__ tact,
                                                      const uint32_t* __restrict__ tcnt, const int32_t* __restrict__ tlist,
                                                      const uint32_t* __restrict__ tpos, const BinOvf* __restrict__ ovf,
                                                      const double* __restrict__ mx, const double* __restrict__ my,
@@ -623,11 +643,11 @@ int launch_force_and_spread(lbib_ctx* c) {
             k_spread_fixed<0><<<grid, 64, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tcnt, c->pb_pos, c->pb_r,
                                                           c->pb_f, c->ovf, F, c->shard, sbegin, begin_it, c->st);
     } else if (map) {
-        k_tile_spread<1><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->tcnt, c->tlist, c->tpos,
+        k_tile_spread<1><<<grid, SPW, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->tcnt, c->tlist, c->tpos,
                                                       c->ovf, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->shard,
                                                       sbegin, begin_it, c->st);
     } else {
-        k_tile_spread<0><<<grid, 256, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->tcnt, c->tlist, c->tpos,
+        k_tile_spread<0><<<grid, SPW, 0, c->stream>>>(c->nx, c->ny, c->stx, c->sty, c->tact, c->tcnt, c->tlist, c->tpos,
                                                       c->ovf, c->mx, c->my, c->mfx, c->mfy, c->Fx, c->Fy, c->shard,
                                                       sbegin, begin_it, c->st);
     }

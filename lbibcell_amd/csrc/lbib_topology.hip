@@ -511,9 +511,12 @@ __global__ __launch_bounds__(256) void k_ring_area(const uint32_t* __restrict__ 
         if (with_mass && threadIdx.x == 0) area_regulator_mass(am, d, false, 0.0);
         return;
     }
-    // the shoelace terms in parallel, their sum in ring order by one lane
+    // the shoelace terms in parallel, their sum in ring order by one lane;
+    // that lane reads the terms eight at a time (four 16-byte LDS reads in
+    // flight) so the chain of additions does not wait on each read
     constexpr int CH = 2048;
-    __shared__ double term[CH];
+    __shared__ double2 term2[CH / 2];
+    double* term = reinterpret_cast<double*>(term2);
     const uint32_t base = roff[d];
     double sum = 0.0;
     for (uint32_t c0 = 0; c0 < n; c0 += CH) {
@@ -525,8 +528,15 @@ __global__ __launch_bounds__(256) void k_ring_area(const uint32_t* __restrict__ 
             term[k] = xk * yn - xn * yk;
         }
         __syncthreads();
-        if (threadIdx.x == 0)
-            for (uint32_t k = 0; k < m; k++) sum += term[k];
+        if (threadIdx.x == 0) {
+            uint32_t k = 0;
+            for (; k + 8 <= m; k += 8) {
+                const double2 t0 = term2[k / 2], t1 = term2[k / 2 + 1], t2 = term2[k / 2 + 2], t3 = term2[k / 2 + 3];
+                sum += t0.x; sum += t0.y; sum += t1.x; sum += t1.y;
+                sum += t2.x; sum += t2.y; sum += t3.x; sum += t3.y;
+            }
+            for (; k < m; k++) sum += term[k];
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
