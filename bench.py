@@ -97,6 +97,9 @@ DTYPES = {"fp64": "f64", "fp32": "f32", "fp16": "f32 arithmetic, f16 storage (f 
           "fp16r": "f32 arithmetic, f32 moments + f16 remainders (f - feq) storage"}
 # the lattice kernel each precision uses on one lattice with the default
 # shape (FP32 and 16-bit storage: the vector kernel, two and four sites per lane)
+# the lattice kernel's HIP-event timing brackets every PROFILE_EVERY-th launch
+# of the timed region (each event pair idles the stream for a few microseconds)
+PROFILE_EVERY = 4
 KERNELS = {"fp64": "k_stream_collide_f64<1,0>", "fp32": "k_stream_collide_vec<1,0,FP32,2>",
            "fp16": "k_stream_collide_vec<1,0,FP16_DEV,4>", "bf16": "k_stream_collide_vec<1,0,BF16_DEV,4>",
            "fp16r": "k_stream_collide_rem<1,0>"}
@@ -112,7 +115,7 @@ def measure(nx, ny, nodes, conns, prec_name, steps, warmup, run_ahead):
     sim.run(warmup)
     sim.sync()
     ctx = sim.context()
-    ctx.profile_enable(True)
+    ctx.profile_enable(True, every=PROFILE_EVERY)
     ctx.profile_read()
     sim.reset_times()
     log(f"[{prec_name}] timing {steps} steps")
@@ -166,7 +169,7 @@ def measure_batched(ens, prec_name, steps, warmup, run_ahead, graph_steps=10):
         sim.run(graph_steps)          # records the block once, outside the timed region
         sim.sync()
     g0 = sim.graph_stats()
-    ctx.profile_enable(True)
+    ctx.profile_enable(True, every=PROFILE_EVERY)
     ctx.profile_read()
     sim.reset_times()
     sim.sync()
@@ -233,7 +236,7 @@ def measure_multispecies(size, nodes, conns, K, prec_name, steps, warmup, run_ah
     ctx = sim.context()
     chosen, rows = ctx.autotune(reps=5)
     log(f"[multispecies {prec_name}] tuned lattice shape {chosen[:3]} ({chosen[3] * 1e3:.1f} us)")
-    ctx.profile_enable(True)
+    ctx.profile_enable(True, every=PROFILE_EVERY)
     ctx.profile_read()
     sim.reset_times()
     sim.sync()

@@ -121,6 +121,9 @@ typedef struct lbib_status {
 } lbib_status;
 
 /* ---- context ---------------------------------------------------------- */
+/* Fails (-1, lbib_last_error) unless 4 <= nx, ny <= 65535, nx * ny <= 2^30
+ * (the cut-link selection indexes (site, slot) pairs in 32 bits),
+ * 0 <= nspecies <= LBIB_MAX_SPECIES and tau_fluid > 0.5. */
 int lbib_create(const lbib_config* cfg, lbib_ctx** out);
 /* Replaces: SimulationRunner ctor (SimulationRunner.cpp:50-64) + the per-site
  * solver objects of GeometryHandler::generatePhysicalGrid (GeometryHandler.cpp:463-474). */
@@ -481,10 +484,14 @@ int lbib_adj_gradient_ms(lbib_adj* adj, double* d_tau_c, double* d_kappa, double
 
 /* ---- measurement ------------------------------------------------------------
  * No counterpart in the reference (it has no profiling): HIP events recorded
- * on the context's stream around every launch of the fused lattice kernel.
- * lbib_profile_read synchronises, returns the summed kernel time, the launch
- * count and the algorithmic bytes of one launch (population read + write at
- * storage precision: (2*9 + 2*5*nspecies) * bytes * nx * ny), and resets. */
+ * on the context's stream around launches of the fused lattice kernel, every
+ * on-th launch after the call (on = 1: every launch; 0 disables).  Each pair
+ * of event packets costs the stream a few microseconds of idle time, so a
+ * period above 1 perturbs the timed run less.
+ * lbib_profile_read synchronises, returns the summed kernel time, the count
+ * of timed launches and the algorithmic bytes of one launch (population read
+ * + write at storage precision: (2*9 + 2*5*nspecies) * bytes * nx * ny), and
+ * resets. */
 int lbib_profile_enable(lbib_ctx* ctx, int on);
 int lbib_profile_read(lbib_ctx* ctx, double* kernel_ms, int64_t* launches, double* bytes_per_launch);
 /* Attainable-bandwidth yardstick: a plain 16-byte-vector copy of nbytes

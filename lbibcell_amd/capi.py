@@ -365,8 +365,9 @@ class Context:
     def rewind(self, it):
         self._chk(self.lib.lbib_rewind(self.h, it))
 
-    def profile_enable(self, on=True):
-        self._chk(self.lib.lbib_profile_enable(self.h, 1 if on else 0))
+    def profile_enable(self, on=True, every=1):
+        """Time every `every`-th fused lattice launch with HIP events."""
+        self._chk(self.lib.lbib_profile_enable(self.h, max(int(every), 1) if on else 0))
 
     def profile_read(self):
         """(summed lattice-kernel ms, launches, algorithmic bytes per launch)"""

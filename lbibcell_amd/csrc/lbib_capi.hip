@@ -295,7 +295,7 @@ struct Staging {
 std::vector<int64_t> graph_key(const lbib_ctx* c) {
     const LatticeShape sh = lattice_shape(c);
     return {(int64_t)c->alloc_gen, c->cur, c->mtab_live, c->msite_live, c->nm == 0, c->nc == 0, c->ndom,
-            sh.variant, sh.wg, sh.spt, c->prof_on, c->nrect};
+            sh.variant, sh.wg, sh.spt, c->prof_every, c->nrect};
 }
 void graph_drop(lbib_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
@@ -969,13 +969,14 @@ static int next_event(lbib_ctx* c, hipEvent_t* ev) {
 static int stream_collide(lbib_ctx* c, uint32_t it, bool ftile) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->capturing) it = ITER_NEXT;
-    if (c->prof_on && !c->capturing) {
+    const bool timed = c->prof_every > 0 && !c->capturing && (c->prof_seq++ % (uint64_t)c->prof_every) == 0;
+    if (timed) {
         if (next_event(c, &e0) || next_event(c, &e1)) return -1;
         LBIB_HIP(hipEventRecord(e0, c->stream));
     }
     const int r = c->prec == LBIB_FP64 ? launch_stream_collide_f64(c, it, ftile) : launch_stream_collide_mixed(c, it, ftile);
     if (r) return r;
-    if (c->prof_on && !c->capturing) LBIB_HIP(hipEventRecord(e1, c->stream));
+    if (timed) LBIB_HIP(hipEventRecord(e1, c->stream));
     return 0;
 }
 
@@ -1038,7 +1039,8 @@ int lbib_cure_lattice(lbib_ctx* c) {
 }
 int lbib_profile_enable(lbib_ctx* c, int on) {
     if (check_ctx(c)) return -1;
-    c->prof_on = on != 0;
+    c->prof_every = on > 0 ? on : 0;
+    c->prof_seq = 0;
     return 0;
 }
 int lbib_profile_read(lbib_ctx* c, double* kernel_ms, int64_t* launches, double* bytes_per_launch) {

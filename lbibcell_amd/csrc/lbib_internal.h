@@ -298,7 +298,8 @@ struct lbib_ctx {
     double* mtab = nullptr;
     bool mtab_live = false;
     // measurement: event pairs around lattice-kernel launches
-    bool prof_on = false;
+    int prof_every = 0;        // every prof_every-th lattice launch is timed (0 = none)
+    uint64_t prof_seq = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     // run-ahead: host launch state when each iteration began
