@@ -96,12 +96,12 @@ DTYPES = {"fp64": "f64", "fp32": "f32", "fp16": "f32 arithmetic, f16 storage (f 
           "bf16": "f32 arithmetic, bf16 storage (f - w)",
           "fp16r": "f32 arithmetic, f32 moments + f16 remainders (f - feq) storage"}
 # the lattice kernel each precision uses on one lattice with the default
-# shape (FP32 and 16-bit storage: the vector kernel, two and four sites per lane)
+# shape (one site per lane in 256-wide groups)
 # the lattice kernel's HIP-event timing brackets every PROFILE_EVERY-th launch
 # of the timed region (each event pair idles the stream for a few microseconds)
 PROFILE_EVERY = 4
-KERNELS = {"fp64": "k_stream_collide_f64<1,0>", "fp32": "k_stream_collide_vec<1,0,FP32,2>",
-           "fp16": "k_stream_collide_vec<1,0,FP16_DEV,4>", "bf16": "k_stream_collide_vec<1,0,BF16_DEV,4>",
+KERNELS = {"fp64": "k_stream_collide_f64<1,0>", "fp32": "k_stream_collide_mx<1,0,FP32,1>",
+           "fp16": "k_stream_collide_mx<1,0,FP16_DEV,1>", "bf16": "k_stream_collide_mx<1,0,BF16_DEV,1>",
            "fp16r": "k_stream_collide_rem<1,0>"}
 
 

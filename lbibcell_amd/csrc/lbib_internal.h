@@ -369,13 +369,13 @@ inline LatticeShape lattice_shape(const lbib_ctx* c) {
                                 (c->lat_variant == 2 && (vec_apply(c, c->lat_spt) || c->prec == LBIB_FP16_REM))))
         return {c->lat_variant, c->lat_wg, c->lat_spt};
     if (c->prec == LBIB_FP64 || c->prec == LBIB_FP16_REM) return {0, 256, 1};
-    // one lattice: the vector kernel (every population plane one 8- or
-    // 16-byte load per lane, all issued before the arithmetic), measured
-    // fastest on 2048^2: FP32 two sites per lane in 256-wide groups, 16-bit
-    // storage four sites per lane in 128-wide groups; a batched atlas keeps
-    // the per-site shapes until the tuner has timed them on its segments
-    if (c->nrect == 0 && c->prec == LBIB_FP32 && vec_apply(c, 2)) return {2, 256, 2};
-    if (c->nrect == 0 && vec_apply(c, 4)) return {2, 128, 4};
+    // one lattice: one site per lane in 256-wide groups.  With the
+    // populations read non-temporally it beats the vector kernel (several
+    // sites per lane, one wide load per plane) inside the coupled 2048^2
+    // step in FP32 (91 against 97 us) and 16-bit storage (63 against 68 us);
+    // a batched atlas keeps the paired or two-site shapes until the tuner
+    // has timed them on its segments
+    if (c->nrect == 0) return {0, 256, 1};
     return pairs_apply(c) ? LatticeShape{1, 256, 1} : LatticeShape{0, 256, 2};
 }
 struct TunedShape {
