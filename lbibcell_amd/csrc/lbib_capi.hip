@@ -295,7 +295,7 @@ struct Staging {
 std::vector<int64_t> graph_key(const lbib_ctx* c) {
     const LatticeShape sh = lattice_shape(c);
     return {(int64_t)c->alloc_gen, c->cur, c->mtab_live, c->msite_live, c->nm == 0, c->nc == 0, c->ndom,
-            sh.variant, sh.wg, sh.spt, c->prof_every, c->nrect};
+            sh.variant, sh.wg, sh.spt, c->prof_every, c->nrect, c->bins_clear};
 }
 void graph_drop(lbib_ctx* c) {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);

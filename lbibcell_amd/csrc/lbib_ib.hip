@@ -84,19 +84,18 @@ constexpr int SP_MW = SP_CH / 64;  // mask words per site
 constexpr int SPW = 128;           // workgroup width of the ordered spread
 constexpr int SPS = ST * ST / SPW; // sites of the tile per lane
 
-// Clears the bins before a binning pass: tile counts, active flags, the
-// overflow count and the touched-tile shards (a stopped step keeps them).
-__global__ void k_bin_clear(uint32_t* tcnt, uint8_t* tact, int nt, uint32_t* shard, DevStatus* st) {
-    if (step_stopped(st)) return;
+// Clears the bins: tile counts, active flags and the overflow count.  Run
+// by itself before a binning pass that does not follow an interpolation,
+// and by every interpolation (k_interp_move) for the next step's binning,
+// also in a stopped step (a rerun then bins into empty tiles).
+__device__ __forceinline__ void clear_bins(uint32_t* tcnt, uint8_t* tact, int nt, DevStatus* st) {
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
         tcnt[t] = 0u;
         tact[t] = 0;
     }
-    if (blockIdx.x == 0) {
-        if (threadIdx.x == 0) st->novf = 0u;
-        if (threadIdx.x < NSHARD) shard[(SH_TOUCHED * NSHARD + threadIdx.x) * SHARD_STRIDE] = 0u;
-    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->novf = 0u;
 }
+__global__ void k_bin_clear(uint32_t* tcnt, uint8_t* tact, int nt, DevStatus* st) { clear_bins(tcnt, tact, nt, st); }
 
 // The slot of this lane's entry in tile `key`: the head lane of each run of
 // equal keys in the wavefront takes the run's slots with one atomic.  Lanes
@@ -146,6 +145,7 @@ struct BinArgs {
     uint32_t* pb_pos;                      // push bins
     float2 *pb_r, *pb_f;
     BinOvf* ovf;
+    uint32_t* shard;
 };
 
 // The marker forces and the binning of the lattice stage in one pass over the
@@ -156,6 +156,8 @@ template <int PUSH>
 __global__ __launch_bounds__(256) void k_force_bin(const BinArgs a, int begin, uint32_t begin_it, DevStatus* st) {
     if (begin && blockIdx.x == 0 && threadIdx.x == 0) step_begin_dev(st, begin_it);
     if (step_stopped(st)) return;
+    // the touched-tile count of the spread that follows
+    if (blockIdx.x == 0 && threadIdx.x < NSHARD) a.shard[(SH_TOUCHED * NSHARD + threadIdx.x) * SHARD_STRIDE] = 0u;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = k < (int)st->nm;
     const int lane = threadIdx.x & 63;
@@ -507,7 +509,10 @@ __global__ __launch_bounds__(64) void k_spread_fixed(int nx, int ny, int ntx, in
 template <typename UT>
 __global__ void k_interp_move(int nx, int ny, double* mx, double* my, double* mvx,
                               double* mvy, double* mox, double* moy, const UT* __restrict__ u,
-                              const int32_t* rtile, const RectDev* rects, int stx, DevStatus* st) {
+                              const int32_t* rtile, const RectDev* rects, int stx, uint32_t* tcnt, uint8_t* tact,
+                              int nt, DevStatus* st) {
+    // the lattice kernel has read the active-tile flags: the bins are free
+    clear_bins(tcnt, tact, nt, st);
     if (step_stopped(st)) return;
     // the grid covers the capacity: renumber over the workgroups with markers
     const uint32_t nm = st->nm, nact = (nm + blockDim.x - 1) / blockDim.x;
@@ -583,13 +588,15 @@ static BinArgs bin_args(lbib_ctx* c, bool forces) {
     a.tlist = c->tlist; a.tpos = c->tpos;
     a.pb_pos = c->pb_pos; a.pb_r = c->pb_r; a.pb_f = c->pb_f;
     a.ovf = c->ovf;
+    a.shard = c->shard;
     return a;
 }
 
+// empties the bins unless the last kernel to touch them cleared them
 static void launch_bin_clear(lbib_ctx* c) {
+    if (c->bins_clear) return;
     const int nt = c->stx * c->sty;
-    k_bin_clear<<<std::max(std::min(nblocks(nt, 256), 1024), 1), 256, 0, c->stream>>>(c->tcnt, c->tact, nt, c->shard,
-                                                                                      c->st);
+    k_bin_clear<<<std::max(std::min(nblocks(nt, 256), 1024), 1), 256, 0, c->stream>>>(c->tcnt, c->tact, nt, c->st);
 }
 
 // The marker forces and the tile binning read the topology, the marker
@@ -615,6 +622,7 @@ int launch_force_and_spread(lbib_ctx* c) {
     // marker counts are device-resident (remeshing changes them): grids cover
     // the capacity and the kernels read st->nm
     launch_bin_clear(c);
+    c->bins_clear = false;
     const BinArgs a = bin_args(c, true);
     const int fgrid = std::max(nblocks(c->mcap, bs), 1);
     if (mixed) k_force_bin<1><<<fgrid, bs, 0, c->stream>>>(a, beside ? 0 : begin, begin_it, c->st);
@@ -658,6 +666,7 @@ int launch_force_and_spread(lbib_ctx* c) {
 // Ordered bins of the markers without the forces (the junction search).
 int launch_tile_bins(lbib_ctx* c) {
     launch_bin_clear(c);
+    c->bins_clear = false;
     const BinArgs a = bin_args(c, false);
     k_force_bin<0><<<std::max(nblocks(c->mcap, 256), 1), 256, 0, c->stream>>>(a, 0, 0u, c->st);
     LBIB_HIP(hipGetLastError());
@@ -669,14 +678,17 @@ int launch_clear_force(lbib_ctx*) { return 0; }
 int launch_interp_move(lbib_ctx* c) {
     if (c->nm == 0) return 0;
     const int32_t* rt = c->nrect ? c->rtile : nullptr;
+    const int nt = c->stx * c->sty;
     if (c->prec == LBIB_FP64)
         k_interp_move<double><<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(
-            c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy, c->mox, c->moy, c->u, rt, c->rects, c->stx, c->st);
+            c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy, c->mox, c->moy, c->u, rt, c->rects, c->stx, c->tcnt,
+            c->tact, nt, c->st);
     else
         k_interp_move<float><<<nblocks(c->mcap, 256), 256, 0, c->stream>>>(
             c->nx, c->ny, c->mx, c->my, c->mvx, c->mvy, c->mox, c->moy, reinterpret_cast<const float*>(c->u), rt,
-            c->rects, c->stx, c->st);
+            c->rects, c->stx, c->tcnt, c->tact, nt, c->st);
     LBIB_HIP(hipGetLastError());
+    c->bins_clear = true;
     return 0;
 }
 

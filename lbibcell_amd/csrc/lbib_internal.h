@@ -10,6 +10,7 @@
 //   tag[n]            bits 0-23 domain id, 24-27 boundary node present in slot
 //                     E,N,W,S, 28-31 that boundary node carries a no-flux BC
 //   tact[sty][stx]    1 if the 16x16 site tile was written by this step's spread
+//                     (cleared by the interpolation after the lattice kernel)
 // Mixed-precision contexts replace f/g with 16-bit deviations from the rest
 // state (see lbib_lattice.hip).
 #pragma once
@@ -216,6 +217,10 @@ struct lbib_ctx {
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool topo_pending = false, forces_beside = false;
+    // the marker bins are empty: the last kernel to touch them was the
+    // interpolation, which clears them once the lattice kernel has read the
+    // active-tile flags (allocation zero-fills them)
+    bool bins_clear = true;
     uint32_t* tcnt = nullptr;          // [nt] entries binned into each tile (may exceed BINCAP)
     uint8_t* tact = nullptr;
     // ordered bins (FP64 spread, junction search): markers by the tile of

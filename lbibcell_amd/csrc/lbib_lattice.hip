@@ -357,7 +357,9 @@ int launch_copy_probe(lbib_ctx* c, const void* src, void* dst, long nbytes) {
 int autotune_f64(lbib_ctx* c, int reps, std::vector<TunedShape>& out) {
     TuneScratch sc;
     if (sc.init(c, 2 * (size_t)c->n * sizeof(double))) return -1;
-    LatticeArgs a = make_args(c, 0, true);
+    // the active-tile flags are cleared after every step: the timed launches
+    // read the force field at every site (the most a step reads)
+    LatticeArgs a = make_args(c, 0, false);
     a.u = (double*)sc.u;
     a.msite = nullptr;
     a.mtab = nullptr;

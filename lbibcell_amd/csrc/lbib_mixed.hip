@@ -956,7 +956,9 @@ int launch_init_mixed(lbib_ctx* c, const double* u0_dev) {
 int autotune_mixed(lbib_ctx* c, int reps, std::vector<TunedShape>& out) {
     TuneScratch sc;
     if (sc.init(c, 2 * (size_t)c->n * sizeof(float))) return -1;
-    MixedArgs a = make_mixed_args(c, 0, true);
+    // the active-tile flags are cleared after every step: the timed launches
+    // read the force field at every site (the most a step reads)
+    MixedArgs a = make_mixed_args(c, 0, false);
     a.u = (float*)sc.u;
     a.msite = nullptr;
     a.mtab = nullptr;
