@@ -358,10 +358,16 @@ int lbib_set_batch_layout(lbib_ctx* ctx, int nrect, const lbib_rect* rects);
 /* Lattice launch shape.  variant 0 = one site per lane in a row tile (the
  * FP64 kernel, and the mixed kernel with sites_per_lane 1, 2 or 4 at
  * workgroup widths 64, 128 or 256); variant 1 = two adjacent sites per lane
- * with 32-bit accesses (16-bit storage, even widths only).  lbib_autotune
- * times every shape that applies on the current state (scratch velocity
- * output, mass and status untouched: the simulation is not advanced), keeps
- * the fastest and reports all timings; lbib_set_lattice_shape fixes one. */
+ * with 32-bit accesses (16-bit storage, even widths only); variant 2 =
+ * sites_per_lane consecutive sites per lane, one wide access per plane (2
+ * or 4 in FP32, 4 or 8 in 16-bit storage, widths a multiple of it; 1 for
+ * the moments-and-remainders store, whose variant 2 shares loaded moments
+ * between lanes).  The default is {0, 256, 1} on one lattice and the paired
+ * or two-site shape on a batched atlas.  lbib_autotune times every shape
+ * that applies on the current state (scratch velocity output, mass and
+ * status untouched: the simulation is not advanced; the force field is read
+ * at every site), keeps the fastest and reports all timings;
+ * lbib_set_lattice_shape fixes one. */
 typedef struct lbib_lattice_shape {
     int32_t variant, workgroup, sites_per_lane;
     double ms;                  /* average launch time measured by the tuner */
