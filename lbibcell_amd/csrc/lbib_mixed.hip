@@ -50,14 +50,23 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
     float g[SPT][NS > 0 ? NS : 1][5];
     uint32_t tg[SPT];
     float Fx[SPT], Fy[SPT], m[SPT], pu[SPT], pv[SPT];
+    bool any = false;
 #pragma unroll
     for (int k = 0; k < SPT; k++) {
         const int x = xb + k * W;
         live[k] = x < a.nx && site_frame(a.rtile, a.rects, a.ntx, a.nx, a.ny, x, y, fr[k]);
         if (!live[k]) { fr[k].x0 = 0; fr[k].y0 = 0; fr[k].w = a.nx; fr[k].h = a.ny; fr[k].r = -1; }
-        // a site outside every rectangle reads the workgroup's first site,
-        // one address for all such lanes, and writes nothing
+        // a site outside every rectangle of a lane that has live sites reads
+        // the workgroup's first site, one address for all such lanes, and
+        // writes nothing
         xs[k] = live[k] ? x : x0w;
+        any |= live[k];
+    }
+    // a lane whose sites all lie outside every rectangle has nothing to do
+    // (no workgroup barrier or cross-lane operation follows)
+    if (!any) return;
+#pragma unroll
+    for (int k = 0; k < SPT; k++) {
         const int xx = xs[k];
         const int xm = (xx == fr[k].x0) ? fr[k].x0 + fr[k].w - 1 : xx - 1;
         const int xp = (xx == fr[k].x0 + fr[k].w - 1) ? fr[k].x0 : xx + 1;
