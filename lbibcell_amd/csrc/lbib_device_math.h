@@ -17,7 +17,7 @@ __device__ constexpr double kDoublePi = 2.0 * M_PI;
 __device__ constexpr double kEPS = 10E-10;            // Constants.hpp:34
 __device__ constexpr double kMAXLENGTH = 0.5;         // Constants.hpp:49
 
-// Population storage formats: FP64 (the parity path), FP32, and FP16 / BF16
+// Population storage formats: FP64 (the parity path), and FP32, FP16 / BF16
 // (fluid populations as deviations f_i - w_i; species populations as they
 // are).  dec/enc convert one stored value to and from the arithmetic type.
 template <int P>
@@ -31,8 +31,10 @@ struct Codec<LBIB_FP64> {
 };
 template <>
 struct Codec<LBIB_FP32> {
+    // fluid populations as deviations f_i - w_i: FP32 spacing at |f - w|
+    // (about 1e-9 here) instead of at w_i (up to 3e-8)
     using T = float;
-    static constexpr bool dev = false;
+    static constexpr bool dev = true;
     __device__ static float dec(T v) { return v; }
     __device__ static T enc(float x) { return x; }
 };

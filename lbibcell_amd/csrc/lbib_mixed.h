@@ -57,16 +57,21 @@ struct MixedArgs {
 };
 
 // Deviation-form BGK with Luo (FM 0) or Guo (FM 1 as written, FM 2 textbook)
-// forcing; d[] in, d[] out, post-collision velocity out.
+// forcing; d[] in, d[] out, post-collision velocity out.  rho - 1 enters the
+// equilibrium as the deviation sum itself, never as (1 + sum) - 1.
 template <int FM>
 __device__ __forceinline__ void collide_dev_f32(float d[9], float m, float Fx, float Fy, float tauI, float tau,
                                                 float gx, float gy, float& uxo, float& uyo) {
     float sd = 0.0f;
 #pragma unroll
     for (int q = 0; q < 9; q++) sd += d[q];
+    // the density deviation rho - 1 (with the mass source) is kept apart from
+    // the 1: a source of 1e-5 added to rho itself would be rounded to the
+    // spacing of FP32 at 1 (1.2e-7), to the spacing at |rho - 1| here
+    float dr = sd;
+    if (isfinite(m)) dr += m;
     const float rho0 = 1.0f + sd;
-    float rho = rho0;
-    if (isfinite(m)) rho += m;
+    const float rho = 1.0f + dr;
     const float rhoI = 1.0f / rho;
     float jx = d[DE] + d[DNE] + d[DSE] - (d[DNW] + d[DW] + d[DSW]);
     float jy = d[DNE] + d[DN] + d[DNW] - (d[DSW] + d[DS] + d[DSE]);
@@ -79,7 +84,7 @@ __device__ __forceinline__ void collide_dev_f32(float d[9], float m, float Fx, f
         const float w = wq(q);
         const float ex = (float)exq(q), ey = (float)eyq(q);
         const float eu = ex * ux + ey * uy;
-        const float deq = w * (rho - 1.0f) + w * rho * (3.0f * eu + 4.5f * eu * eu - u2);
+        const float deq = w * dr + w * rho * (3.0f * eu + 4.5f * eu * eu - u2);
         float fc;
         if (FM == 0) {
             fc = 3.0f * w * rho0 * (ex * Fx + ey * Fy);

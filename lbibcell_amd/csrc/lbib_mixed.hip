@@ -1,6 +1,6 @@
 // lbib_mixed.hip -- mixed-precision lattice kernels: FP32 arithmetic on
-// populations stored in FP32, or as FP16 / BF16 deviations from the rest
-// state (f_i - w_i for the fluid; species populations are stored as they are,
+// populations stored as FP32, FP16 or BF16 deviations from the rest state
+// (f_i - w_i for the fluid; species populations are stored as they are,
 // their rest state being C = 0).  Membrane markers, forces and areas stay
 // FP64; the velocity field is kept as FP32 pairs.
 //
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_mx(const MixedArgs a) {
                 for (int q = 0; q < 9; q++) sd += d[k][q];
                 const float sf = 1.0f / (1.0f + sd);
 #pragma unroll
-                for (int q = 0; q < 9; q++) d[k][q] = (wq(q) + d[k][q]) * sf - wq(q);
+                for (int q = 0; q < 9; q++) d[k][q] = (d[k][q] - wq(q) * sd) * sf;   // (w + d) / (1 + sd) - w
 #pragma unroll
                 for (int s = 0; s < NS; s++)
 #pragma unroll
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_h16(const MixedArgs a) {
                 for (int q = 0; q < 9; q++) sd += d[k][q];
                 const float sf = 1.0f / (1.0f + sd);
 #pragma unroll
-                for (int q = 0; q < 9; q++) d[k][q] = (wq(q) + d[k][q]) * sf - wq(q);
+                for (int q = 0; q < 9; q++) d[k][q] = (d[k][q] - wq(q) * sd) * sf;   // (w + d) / (1 + sd) - w
 #pragma unroll
                 for (int s = 0; s < NS; s++)
 #pragma unroll
@@ -724,7 +724,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_vec(const MixedArgs a, c
                 for (int q = 0; q < 9; q++) sd += d[q];
                 const float sf = 1.0f / (1.0f + sd);
 #pragma unroll
-                for (int q = 0; q < 9; q++) d[q] = (wq(q) + d[q]) * sf - wq(q);
+                for (int q = 0; q < 9; q++) d[q] = (d[q] - wq(q) * sd) * sf;   // (w + d) / (1 + sd) - w
 #pragma unroll
                 for (int s = 0; s < NS; s++)
 #pragma unroll
@@ -770,7 +770,10 @@ __global__ __launch_bounds__(256) void k_stream_collide_vec(const MixedArgs a, c
 }
 
 // FP64 natural-order fields <-> storage (current buffers); field 0 = fluid.
-// src/dst planes q0 .. q0 + count/n - 1 of nq in the storage blocks.
+// src/dst planes q0 .. q0 + count/n - 1 of nq in the storage blocks.  The
+// deviations are taken from the exact weights, so that the rest state is
+// stored as zeros and a stored state reads back with density 1 + sum(d).
+__device__ __forceinline__ double wq_exact(int q) { return q == 0 ? 4.0 / 9.0 : (q < 5 ? 1.0 / 9.0 : 1.0 / 36.0); }
 template <int P>
 __global__ void k_pack(const double* __restrict__ src, void* dst, long count, int fluid, long n, int q0, int nq) {
     using C = Codec<P>;
@@ -778,7 +781,7 @@ __global__ void k_pack(const double* __restrict__ src, void* dst, long count, in
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (long)gridDim.x * blockDim.x) {
         const int q = (int)(k / n);
         float v = (float)src[k];
-        if (fluid && C::dev) v = (float)(src[k] - (double)wq(q));
+        if (fluid && C::dev) v = (float)(src[k] - wq_exact(q));
         out[pidx<P>(q0 + q, k - (long)q * n, n, nq)] = C::enc(v);
     }
 }
@@ -789,7 +792,7 @@ __global__ void k_unpack(const void* src, double* __restrict__ dst, long count, 
     for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (long)gridDim.x * blockDim.x) {
         const int q = (int)(k / n);
         double v = (double)C::dec(in[pidx<P>(q0 + q, k - (long)q * n, n, nq)]);
-        if (fluid && C::dev) v += (double)wq(q);
+        if (fluid && C::dev) v += wq_exact(q);
         dst[k] = v;
     }
 }

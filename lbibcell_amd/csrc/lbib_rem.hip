@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_rem(const MixedArgs a) {
             for (int q = 0; q < 9; q++) sd += d[q];
             const float sf = 1.0f / (1.0f + sd);
 #pragma unroll
-            for (int q = 0; q < 9; q++) d[q] = (wq(q) + d[q]) * sf - wq(q);
+            for (int q = 0; q < 9; q++) d[q] = (d[q] - wq(q) * sd) * sf;   // (w + d) / (1 + sd) - w
 #pragma unroll
             for (int s = 0; s < NS; s++)
 #pragma unroll
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_stream_collide_rem_sh(const MixedArgs a
             for (int q = 0; q < 9; q++) sd += d[q];
             const float sf = 1.0f / (1.0f + sd);
 #pragma unroll
-            for (int q = 0; q < 9; q++) d[q] = (wq(q) + d[q]) * sf - wq(q);
+            for (int q = 0; q < 9; q++) d[q] = (d[q] - wq(q) * sd) * sf;   // (w + d) / (1 + sd) - w
 #pragma unroll
             for (int s = 0; s < NS; s++)
 #pragma unroll

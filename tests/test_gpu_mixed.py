@@ -104,9 +104,9 @@ def test_mixed_storage_round_trip(prec):
         ctx.close()
         return
     uround = {"fp32": 2.0 ** -24, "fp16": 2.0 ** -11, "bf16": 2.0 ** -8}[prec]
-    # fluid: deviation (or value for FP32) rounded once (FP16 subnormals: half
-    # of the 2^-24 spacing); species: value rounded once
-    mag = np.abs(f - W[:, None, None]) if prec != "fp32" else np.abs(f)
+    # fluid: deviation from the exact weights rounded once (FP16 subnormals:
+    # half of the 2^-24 spacing); species: value rounded once
+    mag = np.abs(f - W[:, None, None])
     floor = 2.0 ** -25 if prec == "fp16" else 0.0
     assert np.all(np.abs(ctx.get_populations() - f) <= 1.001 * uround * mag + floor)
     assert np.all(np.abs(ctx.get_species(0) - g) <= 1.001 * uround * np.abs(g))
