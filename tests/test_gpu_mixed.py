@@ -165,3 +165,23 @@ def test_mixed_coupled_run_error_against_fp64(prec):
               "fp16": (7e-4, 9e-4, 2e-6, 1e-6), "bf16": (1e-2, 4e-2, 1.5e-5, 1.5e-5)}[prec]
     assert e_u <= bounds[0] and e_c <= bounds[1] and e_a <= bounds[3]
     assert not same_topology or e_x <= bounds[2]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp16r"])
+def test_mixed_coupled_long_window_within_target(prec):
+    """FP32 arithmetic with FP32 or moments-and-remainders storage keeps speed
+    and concentration within 1e-4 (relative L2) of the FP64 path over 1500
+    coupled steps of the growing cell: the density deviation and the area
+    regulator's mass source are summed apart from the 1, so the source is
+    not rounded to the FP32 spacing at 1."""
+    steps = 1500
+    print(f"case coupled long {prec}", flush=True)
+    ref = _coupled(capi.FP64, steps)
+    mx = _coupled(PRECS[prec], steps)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    e_u, e_c = rel(mx["u"], ref["u"]), rel(mx["c"], ref["c"])
+    e_a = abs(mx["area"][1] - ref["area"][1]) / ref["area"][1]
+    print(f"  L2 rel error after {steps} steps: speed {e_u:.2e} concentration {e_c:.2e} area {e_a:.2e}", flush=True)
+    # measured on MI355X: FP32 2.1e-6 / 1.0e-5 / 9.1e-10, FP16R 2.4e-6 / 1.2e-5 /
+    # 1.1e-9; pinned at about 2.5x, well inside the 1e-4 target
+    assert e_u <= 6e-6 and e_c <= 3e-5 and e_a <= 3e-9
