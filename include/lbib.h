@@ -31,7 +31,7 @@ typedef struct lbib_ctx lbib_ctx;
 /* Storage precision of populations. */
 enum lbib_precision {
     LBIB_FP64 = 0,        /* FP64 storage and arithmetic (parity path) */
-    LBIB_FP32 = 1,        /* FP32 storage and arithmetic */
+    LBIB_FP32 = 1,        /* FP32 storage of f_i - w_i, FP32 arithmetic */
     LBIB_FP16_DEV = 2,    /* FP16 storage of f_i - w_i*rho_ref, FP32 arithmetic */
     LBIB_BF16_DEV = 3,    /* BF16 storage of f_i - w_i*rho_ref, FP32 arithmetic */
     LBIB_FP16_REM = 4     /* FP32 moments (rho - 1, j; species C) plus FP16 remainders
